@@ -1,0 +1,139 @@
+"""ctypes binding of ``libafsi_hip.so`` (the C ABI declared in include/afsi_hip.h).
+
+The library is loaded from this package directory only; there is no CPU
+fallback: if it is missing every call raises. Device pointers come from
+``Tensor.data_ptr()`` and kernels run on ``torch.cuda.current_stream()``.
+The library binds to the HIP runtime torch already loaded (same SONAME), so
+torch streams and allocations are valid handles for it.
+"""
+import ctypes
+from ctypes import POINTER, Structure, c_char_p, c_double, c_int, c_int32, c_int64, c_size_t, c_uint32, c_void_p
+from pathlib import Path
+import threading
+
+import torch
+
+LIBRARY = Path(__file__).resolve().parent / 'libafsi_hip.so'
+_lock = threading.Lock()
+_lib = None
+
+MAX_LEVELS = 16
+
+
+class Grid3(Structure):
+    _fields_ = [('n', c_int64 * 3), ('h', c_double * 3), ('origin', c_double * 3),
+                ('center_origin', c_double * 3), ('periodic', c_int32), ('reserved', c_int32)]
+
+
+class MGPlan3(Structure):
+    _fields_ = [('levels', c_int32), ('periodic', c_int32), ('smoother', c_int32),
+                ('nu', c_int32), ('omega', c_double),
+                ('n', (c_int64 * 3) * MAX_LEVELS), ('h2', (c_double * 3) * MAX_LEVELS),
+                ('inv_h2', (c_double * 3) * MAX_LEVELS),
+                ('p', c_void_p * MAX_LEVELS), ('rhs', c_void_p * MAX_LEVELS),
+                ('tmp', c_void_p * MAX_LEVELS), ('coarse_inverse', c_void_p),
+                ('coarse_cells', c_int64), ('reduce_partials', c_void_p),
+                ('reduce_result', c_void_p)]
+
+
+_P = c_void_p
+_SIGNATURES = {
+    'afsi_last_error': (c_char_p, []),
+    'afsi_abi_version': (c_int, []),
+    'afsi_prepare_compact_f64': (c_int, [_P, c_int64, POINTER(Grid3), _P, _P, _P, _P]),
+    'afsi_compact_gather_f64': (c_int, [_P, _P, _P, c_int64, c_int64, c_int64, c_int64, _P, c_int64, _P, _P]),
+    'afsi_compact_gather_f32': (c_int, [_P, _P, _P, c_int64, c_int64, c_int64, c_int64, _P, c_int64, _P, _P]),
+    'afsi_compact_spread_f64': (c_int, [_P, c_int64, _P, _P, c_int64, c_int64, c_int64, c_int64, c_double, _P, _P, _P]),
+    'afsi_compact_spread_f32': (c_int, [_P, c_int64, _P, _P, c_int64, c_int64, c_int64, c_int64, c_double, _P, _P, _P]),
+    'afsi_indexed_gather_f64': (c_int, [_P, c_int64, c_int64, _P, _P, c_int64, c_int64, _P, _P, _P]),
+    'afsi_indexed_gather_f32': (c_int, [_P, c_int64, c_int64, _P, _P, c_int64, c_int64, _P, _P, _P]),
+    'afsi_indexed_spread_f64': (c_int, [_P, c_int64, c_int64, _P, _P, c_int64, c_int64, c_double, _P, _P, _P]),
+    'afsi_indexed_spread_f32': (c_int, [_P, c_int64, c_int64, _P, _P, c_int64, c_int64, c_double, _P, _P, _P]),
+    'afsi_bin_key_space': (c_int64, [POINTER(Grid3), POINTER(c_int32)]),
+    'afsi_bin_workspace_bytes': (c_size_t, [c_int64]),
+    'afsi_bin_points': (c_int, [_P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, c_size_t, _P, _P]),
+    'afsi_spread_workspace_bytes': (c_size_t, [POINTER(Grid3), POINTER(c_int32)]),
+    'afsi_spread_tiled_f64': (c_int, [_P, _P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, _P, _P, c_size_t, _P, _P]),
+    'afsi_interp_tiled_f64': (c_int, [_P, c_int64, POINTER(Grid3), _P, _P, _P, _P, _P, _P, _P]),
+    'afsi_mg_residual_3d_f64': (c_int, [_P, _P, _P, POINTER(c_int64), POINTER(c_double), c_int, _P]),
+    'afsi_mg_residual_3d_f32': (c_int, [_P, _P, _P, POINTER(c_int64), POINTER(c_double), c_int, _P]),
+    'afsi_mg_jacobi_3d_f64': (c_int, [_P, _P, _P, POINTER(c_int64), POINTER(c_double), POINTER(c_double), c_double, c_int, _P]),
+    'afsi_mg_jacobi_3d_f32': (c_int, [_P, _P, _P, POINTER(c_int64), POINTER(c_double), POINTER(c_double), c_double, c_int, _P]),
+    'afsi_mg_rbgs_3d_f64': (c_int, [_P, _P, POINTER(c_int64), POINTER(c_double), POINTER(c_double), c_int, c_int, _P]),
+    'afsi_mg_rbgs_3d_f32': (c_int, [_P, _P, POINTER(c_int64), POINTER(c_double), POINTER(c_double), c_int, c_int, _P]),
+    'afsi_mg_restrict_3d_f64': (c_int, [_P, _P, POINTER(c_int64), _P]),
+    'afsi_mg_restrict_3d_f32': (c_int, [_P, _P, POINTER(c_int64), _P]),
+    'afsi_mg_residual_restrict_3d_f64': (c_int, [_P, _P, _P, POINTER(c_int64), POINTER(c_double), c_int, _P]),
+    'afsi_mg_residual_restrict_3d_f32': (c_int, [_P, _P, _P, POINTER(c_int64), POINTER(c_double), c_int, _P]),
+    'afsi_mg_prolong_add_3d_f64': (c_int, [_P, _P, POINTER(c_int64), c_int, _P]),
+    'afsi_mg_prolong_add_3d_f32': (c_int, [_P, _P, POINTER(c_int64), c_int, _P]),
+    'afsi_dense_matvec_f64': (c_int, [_P, _P, _P, c_int64, _P]),
+    'afsi_dense_matvec_f32': (c_int, [_P, _P, _P, c_int64, _P]),
+    'afsi_reduce_partials': (c_int64, [c_int64]),
+    'afsi_reduce_f64': (c_int, [_P, c_int64, _P, _P, _P]),
+    'afsi_reduce_f32': (c_int, [_P, c_int64, _P, _P, _P]),
+    'afsi_sub_mean_f64': (c_int, [_P, c_int64, _P, _P]),
+    'afsi_sub_mean_f32': (c_int, [_P, c_int64, _P, _P]),
+    'afsi_fill_f64': (c_int, [_P, c_int64, c_double, _P]),
+    'afsi_fill_f32': (c_int, [_P, c_int64, c_double, _P]),
+    'afsi_divergence_3d_f64': (c_int, [_P, _P, _P, _P, POINTER(c_int64), POINTER(c_double), c_double, c_int, _P]),
+    'afsi_grad_correct_3d_f64': (c_int, [_P, _P, _P, _P, POINTER(c_int64), POINTER(c_double), c_double, c_int, _P]),
+    'afsi_axpy_f64': (c_int, [_P, _P, c_int64, c_double, _P]),
+    'afsi_mg3d_cycles_f64': (c_int, [POINTER(MGPlan3), c_int32, _P]),
+    'afsi_mg3d_cycles_f32': (c_int, [POINTER(MGPlan3), c_int32, _P]),
+    'afsi_mg3d_residual_f64': (c_int, [POINTER(MGPlan3), _P]),
+    'afsi_mg3d_residual_f32': (c_int, [POINTER(MGPlan3), _P]),
+}
+
+
+def exported_names():
+    return sorted(_SIGNATURES)
+
+
+def load():
+    """Load the library (once) and attach signatures; raise if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIBRARY.exists():
+            raise RuntimeError(f'{LIBRARY} is missing: build it with '
+                               '`python -c "import __graft_entry__ as g; g.build()"`')
+        lib = ctypes.CDLL(str(LIBRARY))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def call(name, *args):
+    """Call an entry point; raise RuntimeError with the library's message."""
+    lib = load()
+    status = getattr(lib, name)(*args)
+    if status:
+        message = lib.afsi_last_error().decode(errors='replace')
+        raise RuntimeError(f'{name}: {message}')
+
+
+def stream():
+    return c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    return c_void_p(t.data_ptr()) if t is not None else c_void_p(0)
+
+
+def i64x3(values):
+    return (c_int64 * 3)(*[int(v) for v in values])
+
+
+def f64x3(values):
+    return (c_double * 3)(*[float(v) for v in values])
+
+
+def i32x3(values):
+    return (c_int32 * 3)(*[int(v) for v in values])

@@ -1,0 +1,283 @@
+// Point binning: integer cell keys, stable LSD radix sort, CSR offsets.
+//
+// The cell of a point on axis a is floor((x_a-origin_a)/h_a - 1) + 1: the
+// reference's face-lattice stencil base (mac/compact_transfer.py:161-163)
+// plus one, evaluated with the same IEEE operations (sub, div, sub, floor),
+// so keys are bit-identical to the oracle's. Keys are tile-major: tiles of
+// tx*ty*tz cells, tiles and cells inside a tile both x-major (z fastest), so
+// every tile's points are one contiguous CSR segment.
+//
+// Sort: 8-bit-digit LSD radix passes. Each pass: per-block digit histogram
+// (LDS atomics), per-digit exclusive scan over blocks, and a stable scatter
+// that ranks keys inside a block with wave64 ballots: the lanes holding the
+// same digit are found by AND-ing eight __ballot masks, their order is the
+// lane order, and waves/rounds are combined in index order through LDS
+// counters, so equal keys keep their input order (a stable sort).
+#include "common.hpp"
+
+namespace afsi {
+
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ROUNDS = 8;  // keys per thread per block
+constexpr int SORT_TILE = SORT_THREADS * SORT_ROUNDS;
+constexpr int RADIX = 256;
+constexpr int SORT_WAVES = SORT_THREADS / WAVE;
+
+static int64_t tile_count(const afsi_grid3& g, const int32_t t[3], int a) {
+    return cdiv(g.n[a], t[a]);
+}
+
+static int64_t key_space(const afsi_grid3& g, const int32_t t[3]) {
+    return tile_count(g, t, 0) * tile_count(g, t, 1) * tile_count(g, t, 2) *
+           int64_t(t[0]) * t[1] * t[2];
+}
+
+struct TileGeom {
+    int32_t t[3];
+    int64_t nt[3];
+};
+
+__device__ inline int64_t cell_of(double xa, double o, double h, int64_t n, bool periodic,
+                                  int* flags) {
+    const double fl = floor((xa - o) / h - 1.0);
+    if (!isfinite(fl)) {
+        raise_flag(flags, F_NONFINITE);
+        return 0;
+    }
+    int64_t c = int64_t(fl) + 1;
+    if (periodic) return wrap(c, n);
+    if (c < 0 || c >= n) {
+        raise_flag(flags, F_SUPPORT);
+        c = c < 0 ? 0 : n - 1;  // keep every later access in bounds
+    }
+    return c;
+}
+
+__global__ void keys_kernel(const double* __restrict__ x, int64_t points, afsi_grid3 g,
+                            TileGeom tg, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals, int* flags) {
+    for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < points;
+         p += int64_t(gridDim.x) * blockDim.x) {
+        int64_t c[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double xa = x[3 * p + a];
+            if (!g.periodic) {
+                const double s0 = (xa - g.origin[a]) / g.h[a];
+                if (!(s0 >= 2.0) || !(s0 < double(g.n[a]) - 2.0)) raise_flag(flags, F_SUPPORT);
+            }
+            c[a] = cell_of(xa, g.origin[a], g.h[a], g.n[a], g.periodic != 0, flags);
+        }
+        const int64_t tid = ((c[0] / tg.t[0]) * tg.nt[1] + c[1] / tg.t[1]) * tg.nt[2] + c[2] / tg.t[2];
+        const int64_t local = ((c[0] % tg.t[0]) * tg.t[1] + c[1] % tg.t[1]) * tg.t[2] + c[2] % tg.t[2];
+        keys[p] = uint32_t(tid * (int64_t(tg.t[0]) * tg.t[1] * tg.t[2]) + local);
+        vals[p] = uint32_t(p);
+    }
+}
+
+__global__ void __launch_bounds__(SORT_THREADS) radix_hist_kernel(const uint32_t* __restrict__ keys,
+                                                                 int64_t n, int shift,
+                                                                 uint32_t* __restrict__ hist,
+                                                                 int64_t nblocks) {
+    __shared__ uint32_t h[RADIX];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t start = int64_t(blockIdx.x) * SORT_TILE;
+    for (int r = 0; r < SORT_ROUNDS; ++r) {
+        const int64_t i = start + int64_t(r) * SORT_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RADIX - 1)], 1u);
+    }
+    __syncthreads();
+    hist[int64_t(threadIdx.x) * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of hist[d][0..nblocks) for digit d = blockIdx.x; total[d].
+__global__ void __launch_bounds__(SORT_THREADS) radix_scan_kernel(uint32_t* __restrict__ hist,
+                                                                 int64_t nblocks,
+                                                                 uint32_t* __restrict__ total) {
+    __shared__ uint32_t s[SORT_THREADS];
+    uint32_t* row = hist + int64_t(blockIdx.x) * nblocks;
+    uint32_t running = 0;
+    for (int64_t base = 0; base < nblocks; base += SORT_THREADS) {
+        const int64_t i = base + threadIdx.x;
+        const uint32_t v = i < nblocks ? row[i] : 0u;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        // Hillis-Steele inclusive scan in LDS.
+        for (int d = 1; d < SORT_THREADS; d <<= 1) {
+            const uint32_t add = threadIdx.x >= d ? s[threadIdx.x - d] : 0u;
+            __syncthreads();
+            s[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < nblocks) row[i] = running + s[threadIdx.x] - v;
+        running += s[SORT_THREADS - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) total[blockIdx.x] = running;
+}
+
+__global__ void __launch_bounds__(SORT_THREADS) radix_scatter_kernel(
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n, int shift,
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ total, int64_t nblocks,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
+    __shared__ uint32_t base[RADIX];            // global start of this block's digit d
+    __shared__ uint32_t cnt[SORT_WAVES][RADIX];  // per-wave counts, then prefixes
+    __shared__ uint32_t scan[RADIX];
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1), wave = tid / WAVE;
+    // digit bases: exclusive scan of totals (identical in every block).
+    scan[tid] = total[tid];
+    __syncthreads();
+    for (int d = 1; d < RADIX; d <<= 1) {
+        const uint32_t add = tid >= d ? scan[tid - d] : 0u;
+        __syncthreads();
+        scan[tid] += add;
+        __syncthreads();
+    }
+    base[tid] = scan[tid] - total[tid] + hist[int64_t(tid) * nblocks + blockIdx.x];
+    __syncthreads();
+    const int64_t start = int64_t(blockIdx.x) * SORT_TILE;
+    const uint64_t lt = (uint64_t(1) << lane) - 1;
+    for (int r = 0; r < SORT_ROUNDS; ++r) {
+        const int64_t i = start + int64_t(r) * SORT_THREADS + tid;
+        const bool valid = i < n;
+        const uint32_t key = valid ? keys[i] : 0u;
+        const uint32_t val = valid ? vals[i] : 0u;
+        const uint32_t digit = (key >> shift) & (RADIX - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t m = __ballot((digit >> b) & 1u);
+            peers &= ((digit >> b) & 1u) ? m : ~m;
+        }
+        for (int w = 0; w < SORT_WAVES; ++w) cnt[w][tid] = 0u;
+        __syncthreads();
+        const uint32_t rank = __popcll(peers & lt);
+        // The lowest peer lane publishes the wave's count for this digit.
+        if (valid && (peers & lt) == 0) cnt[wave][digit] = uint32_t(__popcll(peers));
+        __syncthreads();
+        {   // thread tid owns digit tid: prefix over waves, advance the base.
+            uint32_t run = base[tid];
+            for (int w = 0; w < SORT_WAVES; ++w) {
+                const uint32_t c = cnt[w][tid];
+                cnt[w][tid] = run;
+                run += c;
+            }
+            base[tid] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = cnt[wave][digit] + rank;
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+    }
+}
+
+// cell_ptr[k] = lower_bound(sorted keys, k) for k in [0, key_space].
+__global__ void cell_ptr_kernel(const uint32_t* __restrict__ keys, int64_t n, int64_t space,
+                                int32_t* __restrict__ ptr) {
+    for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k <= space;
+         k += int64_t(gridDim.x) * blockDim.x) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (int64_t(keys[mid]) < k) lo = mid + 1;
+            else hi = mid;
+        }
+        ptr[k] = int32_t(lo);
+    }
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+static int64_t sort_blocks(int64_t n) { return cdiv(n, SORT_TILE); }
+
+size_t bin_workspace_bytes(int64_t points) {
+    const int64_t nb = sort_blocks(points) > 0 ? sort_blocks(points) : 1;
+    return 4 * align256(size_t(points) * 4)        // keys/vals ping-pong buffers
+           + align256(size_t(nb) * RADIX * 4)      // hist
+           + align256(RADIX * 4);                  // totals
+}
+
+}  // namespace afsi
+
+using namespace afsi;
+
+extern "C" {
+
+int64_t afsi_bin_key_space(const afsi_grid3* grid, const int32_t tile[3]) {
+    if (!grid || !tile || tile[0] <= 0 || tile[1] <= 0 || tile[2] <= 0) return -1;
+    return key_space(*grid, tile);
+}
+
+size_t afsi_bin_workspace_bytes(int64_t points) { return bin_workspace_bytes(points); }
+
+int afsi_bin_points(const double* x, int64_t points, const afsi_grid3* grid,
+                    const int32_t tile[3], uint32_t* keys_sorted, int32_t* perm,
+                    int32_t* cell_ptr, void* workspace, size_t workspace_bytes, int* flags,
+                    void* stream) {
+    AFSI_REQUIRE(grid && tile && points >= 0, "invalid binning arguments");
+    AFSI_REQUIRE(tile[0] > 0 && tile[1] > 0 && tile[2] > 0, "tile sizes must be positive");
+    AFSI_REQUIRE(grid->n[0] > 0 && grid->n[1] > 0 && grid->n[2] > 0, "invalid grid");
+    AFSI_REQUIRE(points < (int64_t(1) << 31), "at most 2^31-1 points per instance");
+    const int64_t space = key_space(*grid, tile);
+    AFSI_REQUIRE(space < (int64_t(1) << 31), "cell key space exceeds 2^31");
+    AFSI_REQUIRE(cell_ptr, "null cell_ptr");
+    const hipStream_t s = hipStream_t(stream);
+    if (points == 0) {
+        cell_ptr_kernel<<<grid_for(space + 1, 256, 8192), 256, 0, s>>>(keys_sorted, 0, space,
+                                                                         cell_ptr);
+        AFSI_LAUNCH_CHECK();
+        return OK;
+    }
+    AFSI_REQUIRE(x && keys_sorted && perm && workspace, "null pointer");
+    AFSI_REQUIRE(workspace_bytes >= bin_workspace_bytes(points), "binning workspace too small");
+    char* ws = static_cast<char*>(workspace);
+    const size_t kb = align256(size_t(points) * 4);
+    uint32_t* k0 = reinterpret_cast<uint32_t*>(ws);
+    uint32_t* v0 = reinterpret_cast<uint32_t*>(ws + kb);
+    uint32_t* k1 = reinterpret_cast<uint32_t*>(ws + 2 * kb);
+    uint32_t* v1 = reinterpret_cast<uint32_t*>(ws + 3 * kb);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(ws + 4 * kb);
+    const int64_t nb = sort_blocks(points);
+    uint32_t* total = reinterpret_cast<uint32_t*>(ws + 4 * kb + align256(size_t(nb) * RADIX * 4));
+
+    TileGeom tg;
+    for (int a = 0; a < 3; ++a) {
+        tg.t[a] = tile[a];
+        tg.nt[a] = tile_count(*grid, tile, a);
+    }
+    keys_kernel<<<grid_for(points, 256, 8192), 256, 0, s>>>(x, points, *grid, tg, k0, v0, flags);
+    AFSI_LAUNCH_CHECK();
+    int bits = 0;
+    while ((int64_t(1) << bits) < space) ++bits;
+    int passes = (bits + 7) / 8;
+    if (passes < 1) passes = 1;
+    // Even pass count ends in k0/v0; copy target chosen so the last pass
+    // writes the caller's keys_sorted / perm arrays directly.
+    uint32_t* kin = k0;
+    uint32_t* vin = v0;
+    for (int pass = 0; pass < passes; ++pass) {
+        const bool last = pass == passes - 1;
+        uint32_t* kout = last ? keys_sorted : (kin == k0 ? k1 : k0);
+        uint32_t* vout = last ? reinterpret_cast<uint32_t*>(perm) : (vin == v0 ? v1 : v0);
+        radix_hist_kernel<<<unsigned(nb), SORT_THREADS, 0, s>>>(kin, points, 8 * pass, hist, nb);
+        AFSI_LAUNCH_CHECK();
+        radix_scan_kernel<<<RADIX, SORT_THREADS, 0, s>>>(hist, nb, total);
+        AFSI_LAUNCH_CHECK();
+        radix_scatter_kernel<<<unsigned(nb), SORT_THREADS, 0, s>>>(kin, vin, points, 8 * pass,
+                                                                   hist, total, nb, kout, vout);
+        AFSI_LAUNCH_CHECK();
+        kin = kout;
+        vin = vout;
+    }
+    cell_ptr_kernel<<<grid_for(space + 1, 256, 8192), 256, 0, s>>>(keys_sorted, points, space,
+                                                                     cell_ptr);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,6 @@
+"""3D MAC grid: IB transfer backend, multigrid pressure solve, projection."""
+from .grid import MACGrid
+from .multigrid import MGOptions, GeometricMultigrid
+from .flow import MACFlow, MACFlowResult
+
+__all__ = ['MACGrid', 'MGOptions', 'GeometricMultigrid', 'MACFlow', 'MACFlowResult']

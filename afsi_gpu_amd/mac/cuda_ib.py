@@ -1,0 +1,177 @@
+"""Drop-in IB backend: the call signatures of afsi_torch ``mac/cuda_ib.py``.
+
+``build``, ``require_cuda``, ``indexed_gather``, ``indexed_spread``,
+``compact_gather`` and ``compact_spread`` keep the reference's names,
+argument meaning, ownership (fresh outputs; inputs made contiguous) and
+error behaviour (mac/cuda_ib.py:38-90; mac/csrc/transfer.cpp:10-57), and
+run hand-written gfx950 kernels from ``libafsi_hip.so`` on the current
+stream. Out-of-range indices do not trap the device as the reference's
+CUDA_KERNEL_ASSERT does; they set a flag and are skipped, and
+``check=True`` turns a raised flag into ``RuntimeError``.
+"""
+import threading
+import torch
+
+from .. import _abi
+
+_lock = threading.Lock()
+_loaded = False
+
+
+def build(*, verbose=False):
+    """mac/cuda_ib.py:38-60: load the prebuilt library once; refuse to run
+    without a GPU or during graph capture."""
+    global _loaded
+    if _loaded:
+        return
+    if not torch.cuda.is_available():
+        raise RuntimeError('IB HIP backend requires a ROCm-enabled PyTorch and an available GPU')
+    if torch.cuda.is_current_stream_capturing():
+        raise RuntimeError('load the IB HIP library before graph capture')
+    with _lock:
+        if _loaded:
+            return
+        _abi.load()
+        if verbose:
+            print(f'afsi_hip ABI v{_abi.load().afsi_abi_version()} loaded from {_abi.LIBRARY}')
+        _loaded = True
+
+
+def require_cuda(tensor):
+    """mac/cuda_ib.py:63-66."""
+    if not tensor.is_cuda:
+        raise ValueError('ib_backend=cuda requires CUDA tensors; select reference explicitly on CPU')
+    build()
+
+
+def _same(t, f, dtype):
+    """mac/csrc/transfer.cpp:10-14 checks, as RuntimeError."""
+    if not t.is_cuda or t.device != f.device:
+        raise RuntimeError('IB transfer requires one CUDA device')
+    if t.dtype != dtype:
+        raise RuntimeError('IB transfer requires contiguous matching tensors')
+    if t.requires_grad:
+        raise RuntimeError('IB forward transfer does not register autograd')
+
+
+def _field(f):
+    if f.dtype not in (torch.float32, torch.float64):
+        raise RuntimeError('IB supports float32/float64')
+    _same(f, f, f.dtype)
+
+
+def _suffix(dtype):
+    return 'f64' if dtype == torch.float64 else 'f32'
+
+
+def _flags(device):
+    return torch.zeros((), dtype=torch.int32, device=device)
+
+
+def _raise_flags(flags, check):
+    if check and int(flags.item()):
+        raise RuntimeError('IB index out of range')
+
+
+def indexed_gather(field, indices, weights, *, check=False):
+    """U[p,c] = sum_k field[indices[p,k],c]*weights[p,k] (mac/cuda_ib.py:69-71)."""
+    require_cuda(field)
+    field, indices, weights = field.contiguous(), indices.contiguous(), weights.contiguous()
+    _field(field)
+    _same(indices, field, torch.int64)
+    _same(weights, field, field.dtype)
+    if field.dim() != 2 or field.size(1) <= 0:
+        raise RuntimeError('IB indexed field must be N x C')
+    if indices.dim() != 2 or indices.size(1) <= 0 or indices.shape != weights.shape:
+        raise RuntimeError('IB indexed stencil shape mismatch')
+    if field.size(0) <= 0:
+        raise RuntimeError('empty IB grid')
+    out = torch.empty((indices.size(0), field.size(1)), device=field.device, dtype=field.dtype)
+    flags = _flags(field.device)
+    _abi.call(f'afsi_indexed_gather_{_suffix(field.dtype)}', _abi.ptr(field), field.size(0),
+              field.size(1), _abi.ptr(indices), _abi.ptr(weights), indices.size(0),
+              indices.size(1), _abi.ptr(out), _abi.ptr(flags), _abi.stream())
+    _raise_flags(flags, check)
+    return out
+
+
+def indexed_spread(field, indices, weights, size, volume=1., *, check=False):
+    """Fresh zeroed (size,C) with out[ids[p,k],c] += (f[p,c]*w[p,k])/volume
+    (mac/cuda_ib.py:74-76)."""
+    require_cuda(field)
+    field, indices, weights = field.contiguous(), indices.contiguous(), weights.contiguous()
+    _field(field)
+    _same(indices, field, torch.int64)
+    _same(weights, field, field.dtype)
+    if field.dim() != 2 or field.size(1) <= 0:
+        raise RuntimeError('IB indexed field must be N x C')
+    if indices.dim() != 2 or indices.size(1) <= 0 or indices.shape != weights.shape:
+        raise RuntimeError('IB indexed stencil shape mismatch')
+    size, volume = int(size), float(volume)
+    if size <= 0 or not volume > 0 or volume == float('inf'):
+        raise RuntimeError('invalid IB output size/volume')
+    if field.size(0) != indices.size(0):
+        raise RuntimeError('IB spread point count mismatch')
+    out = torch.empty((size, field.size(1)), device=field.device, dtype=field.dtype)
+    flags = _flags(field.device)
+    _abi.call(f'afsi_indexed_spread_{_suffix(field.dtype)}', _abi.ptr(field), field.size(0),
+              field.size(1), _abi.ptr(indices), _abi.ptr(weights), indices.size(1), size, volume,
+              _abi.ptr(out), _abi.ptr(flags), _abi.stream())
+    _raise_flags(flags, check)
+    return out
+
+
+def _compact_args(stencil, c, dtype, device):
+    base, phi = stencil.base[c].contiguous(), stencil.phi[c].contiguous()
+    if base.dim() != 2 or base.size(1) != 3:
+        raise RuntimeError('invalid compact IB field/base')
+    if phi.dim() != 3 or phi.size(0) != base.size(0) or phi.size(1) != 3 or phi.size(2) != 4:
+        raise RuntimeError('invalid compact IB weights')
+    if base.dtype != torch.int64 or phi.dtype != dtype or base.device != device or phi.device != device:
+        raise RuntimeError('IB transfer requires contiguous matching tensors')
+    return base, phi
+
+
+def compact_gather(grid, velocity, stencil, *, check=False):
+    """(P,3) point velocities from three face arrays (mac/cuda_ib.py:79-83)."""
+    require_cuda(velocity[0])
+    dtype, device = velocity[0].dtype, velocity[0].device
+    count = stencil.base.shape[1]
+    out = torch.empty((count, 3), device=device, dtype=dtype)
+    flags = _flags(device)
+    for c, u in enumerate(velocity):
+        u = u.contiguous()
+        _field(u)
+        nx, ny, nz = grid.face_shape(c)
+        if u.numel() != nx * ny * nz:
+            raise RuntimeError('compact IB field length mismatch')
+        base, phi = _compact_args(stencil, c, dtype, device)
+        _abi.call(f'afsi_compact_gather_{_suffix(dtype)}', _abi.ptr(u), _abi.ptr(base),
+                  _abi.ptr(phi), count, nx, ny, nz, _abi.c_void_p(out.data_ptr() + c * out.element_size()),
+                  3, _abi.ptr(flags), _abi.stream())
+    _raise_flags(flags, check)
+    return out
+
+
+def compact_spread(grid, force, stencil, *, check=False):
+    """Three fresh face arrays f_c += (w*F_c)/V (mac/cuda_ib.py:86-90)."""
+    require_cuda(force)
+    force = force.contiguous()
+    _field(force)
+    dtype, device = force.dtype, force.device
+    count = stencil.base.shape[1]
+    if force.shape != (count, 3):
+        raise RuntimeError('compact IB field length mismatch')
+    flags = _flags(device)
+    result = []
+    for c in range(3):
+        shape = grid.face_shape(c)
+        out = torch.empty(shape, device=device, dtype=dtype)
+        base, phi = _compact_args(stencil, c, dtype, device)
+        _abi.call(f'afsi_compact_spread_{_suffix(dtype)}',
+                  _abi.c_void_p(force.data_ptr() + c * force.element_size()), 3, _abi.ptr(base),
+                  _abi.ptr(phi), count, *shape, float(grid.volume), _abi.ptr(out), _abi.ptr(flags),
+                  _abi.stream())
+        result.append(out)
+    _raise_flags(flags, check)
+    return tuple(result)

@@ -1,0 +1,254 @@
+"""Geometric multigrid for the MAC pressure equation on the GPU.
+
+Mirrors ``MGOptions`` and ``GeometricMultigrid`` of mac/multigrid.py:9-172:
+same hierarchy rule (halve until min<=4 or any odd, coarsest <=512 cells),
+same coarse inverse inv(A + 11^T/n), same V-cycle (smooth, mean removal,
+residual restriction, coarse mean removal, recursion, trilinear
+prolongation, smooth), same compatibility/finite checks, tolerance
+max(atol, rtol*||b||), checks every ``check_every`` cycles, messages
+('invalid pressure RHS', 'incompatible', 'did not converge', 'nonfinite')
+and ownership (the result is a clone, never workspace storage).
+
+All cycle launches are issued by the native driver ``afsi_mg3d_cycles_*``;
+with ``backend='graph'`` each block of 1..check_every cycles plus the
+residual reduction is captured once as a HIP graph and replayed. Build-own
+options: ``smoother='rbgs'`` (red-black Gauss-Seidel) and periodic grids.
+"""
+from dataclasses import dataclass
+from math import isfinite, prod
+import numpy as np
+import torch
+
+from .. import _abi
+from .cuda_ib import build as _build
+
+OMEGA = 2 / 3  # mac/multigrid.py:109
+
+
+@dataclass(frozen=True)
+class MGOptions:
+    rtol: float = 1e-10
+    atol: float = 1e-12
+    max_cycles: int = 100
+    smooth: int = 4
+    check_every: int = 2
+    smoother: str = 'jacobi'
+
+    def __post_init__(self):
+        if (any(not isfinite(x) or x < 0 for x in (self.rtol, self.atol)) or
+                self.rtol + self.atol == 0 or any(type(n) is not int or n < 1 for n in
+                (self.max_cycles, self.smooth, self.check_every))):
+            raise ValueError('invalid multigrid options')
+        if self.smoother not in ('jacobi', 'rbgs'):
+            raise ValueError('smoother must be jacobi or rbgs')
+
+
+def hierarchy(shape, spacing):
+    """mac/multigrid.py:33-48."""
+    shapes, spacings = [], []
+    shape, spacing = tuple(shape), tuple(spacing)
+    while True:
+        shapes.append(shape)
+        spacings.append(spacing)
+        if min(shape) <= 4 or any(n % 2 for n in shape):
+            break
+        shape = tuple(n // 2 for n in shape)
+        spacing = tuple(2 * h for h in spacing)
+    if prod(shape) > 512:
+        raise ValueError('MAC dimensions must coarsen to <=512 cells; use powers of two')
+    return shapes, spacings
+
+
+def coarse_operator(shape, spacing, periodic):
+    """Dense A of the coarsest level (setup only; host FP64)."""
+    count = prod(shape)
+    ids = np.arange(count).reshape(shape)
+    A = np.zeros((count, count))
+    for c, h in enumerate(spacing):
+        w = 1 / h ** 2
+        if periodic:
+            a, b = ids.reshape(-1), np.roll(ids, -1, axis=c).reshape(-1)
+        else:
+            sl_a = [slice(None)] * 3
+            sl_b = [slice(None)] * 3
+            sl_a[c], sl_b[c] = slice(None, -1), slice(1, None)
+            a, b = ids[tuple(sl_a)].reshape(-1), ids[tuple(sl_b)].reshape(-1)
+        np.add.at(A, (a, a), w)
+        np.add.at(A, (b, b), w)
+        np.add.at(A, (a, b), -w)
+        np.add.at(A, (b, a), -w)
+    return A
+
+
+class GeometricMultigrid:
+    def __init__(self, grid, *, device='cuda', dtype=torch.float64, options=None, backend='graph'):
+        if backend not in ('torch', 'fused', 'workspace', 'graph'):
+            raise ValueError('pressure backend must be torch, fused, workspace or graph')
+        if dtype not in (torch.float64, torch.float32):
+            raise ValueError('multigrid supports float64/float32')
+        _build()
+        self.grid = grid
+        self.options = MGOptions() if options is None else options
+        self.dtype, self.device = dtype, torch.device(device)
+        if self.device.type != 'cuda':
+            raise ValueError('the HIP multigrid requires a GPU device')
+        self.periodic = bool(getattr(grid, 'periodic', False))
+        self.shapes, self.spacings = hierarchy(grid.shape, grid.spacing)
+        self.use_graphs = backend == 'graph'
+        self.backend = 'hip-graph' if self.use_graphs else 'hip'
+        self.suffix = 'f64' if dtype == torch.float64 else 'f32'
+        kw = dict(device=self.device, dtype=dtype)
+        self.p = [torch.zeros(s, **kw) for s in self.shapes]
+        self.rhs = [torch.zeros(s, **kw) for s in self.shapes]
+        self.tmp = [torch.zeros(s, **kw) for s in self.shapes]
+        A = coarse_operator(self.shapes[-1], self.spacings[-1], self.periodic)
+        count = prod(self.shapes[-1])
+        # mac/multigrid.py:49-59: the rank-one term fixes the constant nullspace.
+        self.coarse_inverse = torch.tensor(np.linalg.inv(A + np.ones_like(A) / count), **kw)
+        cells0 = prod(self.shapes[0])
+        nparts = int(_abi.load().afsi_reduce_partials(cells0))
+        self.partials = torch.zeros(4 * nparts, device=self.device, dtype=torch.float64)
+        self.results = torch.zeros((8, 4), device=self.device, dtype=torch.float64)
+        self.plan = self._make_plan()
+        self.allocated_bytes = sum(t.numel() * t.element_size()
+                                   for group in (self.p, self.rhs, self.tmp) for t in group)
+        self.graphs = {}
+        if self.use_graphs:
+            if self.options.check_every > 16:
+                raise ValueError('pressure graph check_every must be <=16')
+            self._capture()
+
+    def _make_plan(self):
+        pl = _abi.MGPlan3()
+        pl.levels = len(self.shapes)
+        pl.periodic = int(self.periodic)
+        pl.smoother = 1 if self.options.smoother == 'rbgs' else 0
+        pl.nu = self.options.smooth
+        pl.omega = OMEGA
+        for l, (s, h) in enumerate(zip(self.shapes, self.spacings)):
+            for a in range(3):
+                pl.n[l][a] = s[a]
+                pl.h2[l][a] = h[a] * h[a]
+                pl.inv_h2[l][a] = 1 / h[a] ** 2
+            pl.p[l] = self.p[l].data_ptr()
+            pl.rhs[l] = self.rhs[l].data_ptr()
+            pl.tmp[l] = self.tmp[l].data_ptr()
+        pl.coarse_inverse = self.coarse_inverse.data_ptr()
+        pl.coarse_cells = prod(self.shapes[-1])
+        pl.reduce_partials = self.partials.data_ptr()
+        pl.reduce_result = self.results[0].data_ptr()
+        return pl
+
+    # -- native calls on the current stream ---------------------------------
+    def _cycles(self, count):
+        _abi.call(f'afsi_mg3d_cycles_{self.suffix}', self.plan, int(count), _abi.stream())
+
+    def _residual(self):
+        _abi.call(f'afsi_mg3d_residual_{self.suffix}', self.plan, _abi.stream())
+
+    def _reduce(self, x, slot):
+        _abi.call(f'afsi_reduce_{self.suffix}', _abi.ptr(x), x.numel(), _abi.ptr(self.partials),
+                  _abi.ptr(self.results[slot]), _abi.stream())
+
+    def _sub_mean(self, x, slot):
+        _abi.call(f'afsi_sub_mean_{self.suffix}', _abi.ptr(x), x.numel(),
+                  _abi.ptr(self.results[slot]), _abi.stream())
+
+    def _block(self, count):
+        self._cycles(count)
+        self._residual()
+
+    def _capture(self):
+        self.p[0].zero_()
+        self.rhs[0].zero_()
+        stream = torch.cuda.Stream(device=self.device)
+        stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(stream):
+            for _ in range(2):
+                self._block(self.options.check_every)
+        torch.cuda.current_stream().wait_stream(stream)
+        for count in range(1, self.options.check_every + 1):
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph, stream=stream):
+                self._block(count)
+            self.graphs[count] = graph
+        torch.cuda.current_stream().wait_stream(stream)
+        torch.cuda.synchronize(self.device)
+
+    def advance(self, count):
+        """Run ``count`` V-cycles and the level-0 residual reduction."""
+        if self.graphs:
+            self.graphs[count].replay()
+        else:
+            self._block(count)
+
+    # -- reference-compatible API ---------------------------------------------
+    def cycle(self):
+        """One V-cycle on the workspace iterate (MGWorkspace.cycle)."""
+        self._cycles(1)
+        return self.p[0]
+
+    def initialize(self, p, rhs):
+        self.p[0].copy_(p)
+        self.rhs[0].copy_(rhs)
+
+    def residual_norm(self):
+        self._residual()
+        return self.results[0, 1].sqrt()
+
+    @torch.no_grad()
+    def solve(self, rhs, initial=None, *, rtol=None, history=None):
+        """mac/multigrid.py:125-172 (workspace branch :75-98). ``history``, if
+        a list, receives the level-0 residual norm after every check."""
+        rtol = self.options.rtol if rtol is None else rtol
+        if not isfinite(rtol) or rtol < 0:
+            raise ValueError('invalid multigrid solve tolerance')
+        if rhs.shape != self.shapes[0] or rhs.device != self.device or rhs.dtype != self.dtype:
+            raise ValueError('invalid pressure RHS')
+        if initial is not None and (initial.shape != rhs.shape or initial.device != rhs.device
+                                    or initial.dtype != rhs.dtype):
+            raise ValueError('invalid initial pressure')
+        w_p, w_rhs = self.p[0], self.rhs[0]
+        w_rhs.copy_(rhs)
+        if initial is None:
+            w_p.zero_()
+        else:
+            w_p.copy_(initial)
+        # _start_metrics (mac/multigrid.py:68-73) with one host synchronisation.
+        self._reduce(w_rhs, 1)
+        self._reduce(w_p, 2)
+        self._sub_mean(w_rhs, 1)
+        self._sub_mean(w_p, 2)
+        self._reduce(w_rhs, 3)
+        self._residual()
+        count = w_rhs.numel()
+        (s1, _, a1, nf1), (_, _, _, nf2), (_, q3, _, _), (_, q0, _, nf0) = \
+            self.results[[1, 2, 3, 0]].tolist()
+        if nf1:
+            raise ValueError('invalid pressure RHS')
+        mean, absolute_mean = s1 / count, a1 / count
+        if abs(mean) > 1e-12 + 1e-10 * absolute_mean:
+            raise ValueError('incompatible Neumann pressure RHS: net flux must vanish')
+        if nf2:
+            raise ValueError('invalid initial pressure')
+        rhs_norm, residual = q3 ** .5, q0 ** .5
+        if not isfinite(rhs_norm) or not isfinite(residual) or nf0:
+            raise RuntimeError('nonfinite multigrid residual or RHS norm')
+        tolerance = max(self.options.atol, rtol * rhs_norm)
+        cycle = 0
+        if history is not None:
+            history.append(residual)
+        while residual > tolerance and cycle < self.options.max_cycles:
+            count = min(self.options.check_every, self.options.max_cycles - cycle)
+            self.advance(count)
+            cycle += count
+            _, q0, _, nf0 = self.results[0].tolist()
+            residual = q0 ** .5
+            if history is not None:
+                history.append(residual)
+            if nf0 or not isfinite(residual):
+                raise RuntimeError('nonfinite multigrid residual')
+        if residual > tolerance:
+            raise RuntimeError(f'pressure multigrid did not converge: {residual:g} > {tolerance:g}')
+        return w_p.clone(), dict(cycles=cycle, residual_norm=residual, tolerance=tolerance,
+                                 backend=self.backend)

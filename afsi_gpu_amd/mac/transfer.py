@@ -1,0 +1,143 @@
+"""Point-level 3D IB transfer on MAC faces: stencils, binning, tiled kernels.
+
+* ``CompactStencil`` / ``from_points``: the reference's compact separable
+  stencil (mac/compact_transfer.py:151-166,284-290), built on the GPU with
+  the reference's two-branch Peskin formula; consumed by
+  ``cuda_ib.compact_gather`` / ``compact_spread``.
+* ``bin_points``: build-own stable CSR of points by tile-major cell key.
+* ``TiledTransfer``: spread / interpolate all three components straight
+  from positions over the CSR (weights in registers, per-tile LDS
+  accumulation), the fast path the IB step uses.
+"""
+from dataclasses import dataclass
+import torch
+
+from .. import _abi
+from .cuda_ib import require_cuda
+
+DEFAULT_TILE = (8, 8, 8)
+
+
+@dataclass(frozen=True)
+class CompactStencil:
+    base: torch.Tensor  # (3,P,3) int64, lower neighbour per staggered lattice
+    phi: torch.Tensor   # (3,P,3,4), four 1D weights per axis
+
+    @property
+    def storage_bytes(self):
+        return self.base.numel() * self.base.element_size() + self.phi.numel() * self.phi.element_size()
+
+
+def _check_points(points):
+    if points.ndim != 2 or points.shape[1] != 3 or points.dtype != torch.float64:
+        raise ValueError('positions must be (N,3) float64')
+
+
+def from_points(points, grid, *, check=True):
+    """Compact stencil of FP64 points (mac/compact_transfer.py:284-290).
+
+    With ``check`` the support predicate of mac/compact_transfer.py:234-236
+    is enforced (one host synchronisation), raising the reference's
+    'support' ValueError."""
+    require_cuda(points)
+    _check_points(points)
+    points = points.contiguous()
+    count = points.shape[0]
+    base = torch.empty((3, count, 3), dtype=torch.int64, device=points.device)
+    phi = torch.empty((3, count, 3, 4), dtype=torch.float64, device=points.device)
+    flags = torch.zeros((), dtype=torch.int32, device=points.device)
+    g = grid.abi()
+    _abi.call('afsi_prepare_compact_f64', _abi.ptr(points), count, g, _abi.ptr(base), _abi.ptr(phi),
+              _abi.ptr(flags), _abi.stream())
+    if check and int(flags.item()):
+        raise ValueError('MAC IB support reaches a wall or is nonfinite; enlarge/refine the fluid box')
+    return CompactStencil(base, phi)
+
+
+@dataclass
+class PointBins:
+    """Stable CSR of points: keys_sorted (P) uint32 as int32 storage,
+    perm (P) int32 = stable argsort(keys), cell_ptr (key_space+1) int32."""
+    tile: tuple
+    keys: torch.Tensor
+    perm: torch.Tensor
+    cell_ptr: torch.Tensor
+    flags: torch.Tensor
+
+    def check(self):
+        bits = int(self.flags.item())
+        if bits & 4:
+            raise ValueError('nonfinite IB point positions')
+        if bits & 1:
+            raise ValueError('MAC IB support reaches a wall or is nonfinite; enlarge/refine the fluid box')
+        return bits
+
+
+class Binner:
+    """Reusable buffers for ``afsi_bin_points`` at a fixed point count."""
+
+    def __init__(self, grid, count, device, tile=DEFAULT_TILE):
+        self.grid, self.count, self.tile = grid, int(count), tuple(int(t) for t in tile)
+        self._g = grid.abi()
+        self._tile = _abi.i32x3(self.tile)
+        _abi.load()
+        space = _abi.load().afsi_bin_key_space(self._g, self._tile)
+        if space <= 0:
+            raise ValueError('invalid binning tile')
+        self.key_space = int(space)
+        self.keys = torch.empty(self.count, dtype=torch.int32, device=device)
+        self.perm = torch.empty(self.count, dtype=torch.int32, device=device)
+        self.cell_ptr = torch.empty(self.key_space + 1, dtype=torch.int32, device=device)
+        self.flags = torch.zeros((), dtype=torch.int32, device=device)
+        nbytes = int(_abi.load().afsi_bin_workspace_bytes(self.count))
+        self.workspace = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
+
+    def __call__(self, points):
+        if points.shape != (self.count, 3) or points.dtype != torch.float64:
+            raise ValueError('positions must match the binner (N,3) float64')
+        points = points.contiguous()
+        self.flags.zero_()
+        _abi.call('afsi_bin_points', _abi.ptr(points), self.count, self._g, self._tile,
+                  _abi.ptr(self.keys), _abi.ptr(self.perm), _abi.ptr(self.cell_ptr),
+                  _abi.ptr(self.workspace), self.workspace.numel(), _abi.ptr(self.flags),
+                  _abi.stream())
+        return PointBins(self.tile, self.keys, self.perm, self.cell_ptr, self.flags)
+
+
+def bin_points(points, grid, tile=DEFAULT_TILE):
+    """One-shot binning (fresh buffers)."""
+    require_cuda(points)
+    _check_points(points)
+    return Binner(grid, points.shape[0], points.device, tile)(points)
+
+
+class TiledTransfer:
+    """Spread / interpolate over binned points for a fixed grid and tile."""
+
+    def __init__(self, grid, device, tile=DEFAULT_TILE):
+        self.grid, self.tile = grid, tuple(int(t) for t in tile)
+        self._g = grid.abi()
+        self._tile = _abi.i32x3(self.tile)
+        nbytes = int(_abi.load().afsi_spread_workspace_bytes(self._g, self._tile))
+        self.workspace = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
+        self.flags = torch.zeros((), dtype=torch.int32, device=device)
+
+    def spread(self, points, force, bins, out=None):
+        """Three face arrays f_c = sum_p w_c(p) F_c(p) / V."""
+        if out is None:
+            out = self.grid.zeros(device=points.device, dtype=torch.float64)
+        _abi.call('afsi_spread_tiled_f64', _abi.ptr(points), _abi.ptr(force), points.shape[0],
+                  self._g, self._tile, _abi.ptr(bins.perm), _abi.ptr(bins.cell_ptr),
+                  _abi.ptr(out[0]), _abi.ptr(out[1]), _abi.ptr(out[2]), _abi.ptr(self.workspace),
+                  self.workspace.numel(), _abi.ptr(self.flags), _abi.stream())
+        return out
+
+    def interpolate(self, points, velocity, bins=None, out=None):
+        """(P,3) velocities U_c(p) = sum w_c(p) u_c in the caller's point order."""
+        if out is None:
+            out = torch.empty_like(points)
+        perm = bins.perm if bins is not None else None
+        _abi.call('afsi_interp_tiled_f64', _abi.ptr(points), points.shape[0], self._g,
+                  _abi.ptr(perm), _abi.ptr(velocity[0]), _abi.ptr(velocity[1]),
+                  _abi.ptr(velocity[2]), _abi.ptr(out), _abi.ptr(self.flags), _abi.stream())
+        return out

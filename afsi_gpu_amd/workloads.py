@@ -1,0 +1,134 @@
+"""Seeded synthetic point clouds and forces for the hot-path workloads.
+
+All draws use ``numpy.random.Generator(PCG64(seed))``; positions and forces
+are FP64. Shapes:
+
+* ``shell(a, count, thickness, ...)``: points on an ellipsoid with semi-axes
+  ``a`` centred at ``centre``, uniform in area (sphere directions mapped to
+  the ellipsoid, accepted with probability proportional to the area
+  element sqrt((bc x)^2+(ac y)^2+(ab z)^2)), offset along the outward normal
+  by U(-thickness/2, thickness/2).
+* ``patch``: the same construction restricted to the part of the shell
+  within ``radius`` of the surface point at the +x tip.
+* ``curves2d``: 2D sinusoidal curves y = 0.5 +- 0.1 sin(2 pi x / Lx) with
+  jitter of +-0.05 h.
+"""
+import numpy as np
+
+GEOMETRY_SEED = 20261202
+
+
+def rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def _sphere(gen, n):
+    v = gen.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def _area_weight(u, a):
+    b = np.array([a[1] * a[2], a[0] * a[2], a[0] * a[1]])
+    return np.sqrt(((u * b) ** 2).sum(1)) / b.max()
+
+
+def _place(gen, u, a, centre, thickness):
+    surf = u * a
+    normal = surf / a ** 2
+    normal /= np.linalg.norm(normal, axis=1, keepdims=True)
+    offset = gen.uniform(-thickness / 2, thickness / 2, size=(len(u), 1))
+    return centre + surf + offset * normal, surf
+
+
+def shell(gen, a, count, thickness=0.02, centre=(.5, .5, .5), truncate_x=None):
+    """``count`` shell points; ``truncate_x`` keeps surface points with
+    x - centre_x <= truncate_x (a truncated shell)."""
+    a, centre = np.asarray(a, float), np.asarray(centre, float)
+    out = []
+    have = 0
+    while have < count:
+        u = _sphere(gen, max(2 * (count - have), 1024))
+        keep = gen.uniform(size=len(u)) < _area_weight(u, a)
+        if truncate_x is not None:
+            keep &= (u[:, 0] * a[0]) <= truncate_x
+        u = u[keep]
+        pts, _ = _place(gen, u, a, centre, thickness)
+        out.append(pts)
+        have += len(pts)
+    return np.concatenate(out)[:count]
+
+
+def patch(gen, a, count, radius=0.05, thickness=0.02, centre=(.5, .5, .5)):
+    """``count`` points of the shell within ``radius`` of its +x tip."""
+    a, centre = np.asarray(a, float), np.asarray(centre, float)
+    tip = np.array([a[0], 0., 0.])
+    cap = min(np.pi, 2.5 * radius / min(a[1], a[2]))
+    out = []
+    have = 0
+    while have < count:
+        n = max(4 * (count - have), 4096)
+        # uniform directions in the cap of half-angle `cap` around +x
+        cos_t = gen.uniform(np.cos(cap), 1., size=n)
+        phi = gen.uniform(0., 2 * np.pi, size=n)
+        sin_t = np.sqrt(1 - cos_t ** 2)
+        u = np.stack([cos_t, sin_t * np.cos(phi), sin_t * np.sin(phi)], 1)
+        keep = gen.uniform(size=n) < _area_weight(u, a)
+        keep &= np.linalg.norm(u * a - tip, axis=1) <= radius
+        u = u[keep]
+        pts, _ = _place(gen, u, a, centre, thickness)
+        out.append(pts)
+        have += len(pts)
+    return np.concatenate(out)[:count]
+
+
+def ellipsoid_area(a):
+    """Knud Thomsen's approximation (p=1.6075), for splitting counts."""
+    p = 1.6075
+    x, y, z = (float(v) ** p for v in a)
+    return 4 * np.pi * ((x * y + x * z + y * z) / 3) ** (1 / p)
+
+
+def normal_forces(gen, count, dim=3):
+    return gen.normal(size=(count, dim))
+
+
+def lognormal_forces(gen, count):
+    magnitude = gen.lognormal(0., 1., size=(count, 1))
+    return magnitude * _sphere(gen, count)
+
+
+def config2(points=200_000, seed=GEOMETRY_SEED):
+    """64^3 unit cube: truncated shell (0.3,0.2,0.2), thickness 0.02,
+    truncation x - 0.5 <= 0.6*0.3 (build's choice), N(0,1) forces."""
+    gen = rng(seed)
+    x = shell(gen, (.3, .2, .2), points, .02, truncate_x=.6 * .3)
+    return x, normal_forces(gen, points)
+
+
+def config3(points=1_500_000, seed=GEOMETRY_SEED + 1):
+    """128^3 unit cube: 90% on shells (0.35,0.25,0.25) and (0.3,0.2,0.2)
+    split by area, 10% clustered within 0.05 of the outer +x tip;
+    forces LogNormal(0,1) magnitude, uniform direction."""
+    gen = rng(seed)
+    clustered = points // 10
+    rest = points - clustered
+    outer, inner = (.35, .25, .25), (.3, .2, .2)
+    ao, ai = ellipsoid_area(outer), ellipsoid_area(inner)
+    n_outer = int(round(rest * ao / (ao + ai)))
+    x = np.concatenate([shell(gen, outer, n_outer), shell(gen, inner, rest - n_outer),
+                        patch(gen, outer, clustered)])
+    return x, lognormal_forces(gen, points)
+
+
+def curves2d(gen, lengths=(4., 1.), shape=(256, 64), per_curve=2048):
+    """Config-1 curves: x_i = Lx (i+1/2)/N, y = 0.5 + k 0.1 sin(2 pi x / Lx),
+    k = +-1, plus U(-0.05h, 0.05h) jitter on both coordinates."""
+    h = lengths[0] / shape[0]
+    i = np.arange(per_curve)
+    xs = lengths[0] * (i + .5) / per_curve
+    pts = []
+    for k in (1, -1):
+        y = .5 + k * .1 * np.sin(2 * np.pi * xs / lengths[0])
+        pts.append(np.stack([xs, y], 1))
+    pts = np.concatenate(pts)
+    return pts + gen.uniform(-.05 * h, .05 * h, size=pts.shape)

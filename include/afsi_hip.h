@@ -1,0 +1,224 @@
+/*
+ * afsi_hip: MI355X (gfx950) kernels for the MAC/IB per-timestep hot path.
+ *
+ * C ABI: raw device pointers, sizes and a hipStream_t passed as void*.
+ * Every entry point returns 0 on success or a nonzero status; the message
+ * is available from afsi_last_error() on the calling thread. Kernels run on
+ * the given stream and never synchronise it. Device-side validation (point
+ * support, index range, non-finite values) ORs bits into an optional
+ * caller-owned device int (`flags`, may be NULL):
+ *   1 support violated, 2 index out of range, 4 non-finite, 8 capacity.
+ *
+ * Reference citations are relative to loveIroha/AFSI_GPU src/afsi_torch.
+ */
+#ifndef AFSI_HIP_H
+#define AFSI_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Uniform 3D MAC grid. `center_origin` = origin + 0.5*h evaluated by the
+ * caller in FP64 exactly as mac/grid.py:38-40 face_origin does, so integer
+ * stencil bases are bit-identical to the reference's. */
+typedef struct afsi_grid3 {
+    int64_t n[3];             /* cells per axis */
+    double h[3];              /* spacing = length / n (mac/grid.py:27-29) */
+    double origin[3];         /* box origin; face lattice along its own axis */
+    double center_origin[3];  /* origin + 0.5*h; cell-centred lattice */
+    int32_t periodic;         /* 0: closed Neumann box (reference); 1: periodic */
+    int32_t reserved;
+} afsi_grid3;
+
+const char* afsi_last_error(void);
+int afsi_abi_version(void);
+
+/* ---- Stencil preparation -------------------------------------------------
+ * Replaces mac/compact_transfer.py:161-166 (_prepare) as used by
+ * CompactFETransfer.from_points (:284-290): base (3,P,3) int64 and phi
+ * (3,P,3,4) for the three staggered lattices, phi with the reference's two-
+ * branch formula (ib.py:57-67). Sets flag 1 for points failing the support
+ * predicate of mac/compact_transfer.py:234-236 (closed box only). */
+int afsi_prepare_compact_f64(const double* x, int64_t points, const afsi_grid3* grid,
+                             int64_t* base, double* phi, int* flags, void* stream);
+
+/* ---- Compact separable transfer (one component per call) ----------------
+ * Replace torch.ops.afsi_ib_cuda.compact_gather / compact_spread
+ * (mac/csrc/transfer.cpp:49-52,215-228; mac/cuda_ib.py:79-90).
+ * field/out are the flat C-order face array of shape (nx,ny,nz) of this
+ * component; base (P,3) int64, phi (P,3,4). The gather writes out[p*stride].
+ * The spread reads force[p*stride], zero-fills `out` and accumulates
+ * (w*f)/volume with hardware FP atomics (out = fresh zeroed result). */
+int afsi_compact_gather_f64(const double* field, const int64_t* base, const double* phi,
+                            int64_t points, int64_t nx, int64_t ny, int64_t nz,
+                            double* out, int64_t out_stride, int* flags, void* stream);
+int afsi_compact_gather_f32(const float* field, const int64_t* base, const float* phi,
+                            int64_t points, int64_t nx, int64_t ny, int64_t nz,
+                            float* out, int64_t out_stride, int* flags, void* stream);
+int afsi_compact_spread_f64(const double* force, int64_t force_stride, const int64_t* base,
+                            const double* phi, int64_t points, int64_t nx, int64_t ny,
+                            int64_t nz, double volume, double* out, int* flags, void* stream);
+int afsi_compact_spread_f32(const float* force, int64_t force_stride, const int64_t* base,
+                            const float* phi, int64_t points, int64_t nx, int64_t ny,
+                            int64_t nz, double volume, float* out, int* flags, void* stream);
+
+/* ---- Generic indexed transfer --------------------------------------------
+ * Replace torch.ops.afsi_ib_cuda.indexed_gather / indexed_spread
+ * (mac/csrc/transfer.cpp:49-52,200-214; mac/cuda_ib.py:69-76): field (N,C),
+ * ids (P,K) int64, signed weights (P,K). Spread zero-fills out (size,C) and
+ * accumulates (f*w)/volume. Out-of-range ids set flag 2 and are skipped. */
+int afsi_indexed_gather_f64(const double* field, int64_t nodes, int64_t comps,
+                            const int64_t* ids, const double* weights, int64_t points,
+                            int64_t neighbors, double* out, int* flags, void* stream);
+int afsi_indexed_gather_f32(const float* field, int64_t nodes, int64_t comps,
+                            const int64_t* ids, const float* weights, int64_t points,
+                            int64_t neighbors, float* out, int* flags, void* stream);
+int afsi_indexed_spread_f64(const double* field, int64_t points, int64_t comps,
+                            const int64_t* ids, const double* weights, int64_t neighbors,
+                            int64_t size, double volume, double* out, int* flags, void* stream);
+int afsi_indexed_spread_f32(const float* field, int64_t points, int64_t comps,
+                            const int64_t* ids, const float* weights, int64_t neighbors,
+                            int64_t size, double volume, float* out, int* flags, void* stream);
+
+/* ---- Point binning (build-own; no reference counterpart) ------------------
+ * Integer cell of each point = face-lattice base + 1 on every axis
+ * (the base of mac/compact_transfer.py:161-163), wrapped on a periodic grid.
+ * Keys use a tile-major cell numbering (tile tx*ty*tz, both levels x-major).
+ * Output: keys_sorted (P) uint32, perm (P) int32 = stable argsort of the
+ * keys, cell_ptr (key_space+1) int32 CSR offsets. Stable LSD radix sort. */
+int64_t afsi_bin_key_space(const afsi_grid3* grid, const int32_t tile[3]);
+size_t afsi_bin_workspace_bytes(int64_t points);
+int afsi_bin_points(const double* x, int64_t points, const afsi_grid3* grid,
+                    const int32_t tile[3], uint32_t* keys_sorted, int32_t* perm,
+                    int32_t* cell_ptr, void* workspace, size_t workspace_bytes,
+                    int* flags, void* stream);
+
+/* ---- Tiled transfer over the binned CSR (build-own fast path) -------------
+ * Same mathematics as compact_spread/compact_gather for all three
+ * components at once, weights computed from positions in registers.
+ * Spread: one work item per (tile, chunk of points); per-tile LDS
+ * accumulation of all three components with ds_add_f64, then one global
+ * atomic per touched face. u,v,w are overwritten (zero-filled first).
+ * Interp: one thread per binned point, out (P,3) in original point order. */
+size_t afsi_spread_workspace_bytes(const afsi_grid3* grid, const int32_t tile[3]);
+int afsi_spread_tiled_f64(const double* x, const double* force, int64_t points,
+                          const afsi_grid3* grid, const int32_t tile[3],
+                          const int32_t* perm, const int32_t* cell_ptr,
+                          double* u, double* v, double* w, void* workspace,
+                          size_t workspace_bytes, int* flags, void* stream);
+int afsi_interp_tiled_f64(const double* x, int64_t points, const afsi_grid3* grid,
+                          const int32_t* perm, const double* u, const double* v,
+                          const double* w, double* out, int* flags, void* stream);
+
+/* ---- 3D pressure multigrid kernels ---------------------------------------
+ * Replace the MG kernel protocol of mac/mg_workspace.py:13-30 /
+ * mac/_triton_mg.py:121-151 (smooth, residual, restrict, residual_restrict,
+ * prolong_add). n = cells of the level, h2 = h*h per axis (operator
+ * (p_nb-p)/h2 in the order of mac/_triton_mg.py:26-32), inv_h2 = 1/h**2
+ * (diagonal of mac/multigrid.py:38-41). periodic=1 wraps (build-own). */
+int afsi_mg_residual_3d_f64(const double* p, const double* rhs, double* out,
+                            const int64_t n[3], const double h2[3], int periodic, void* stream);
+int afsi_mg_residual_3d_f32(const float* p, const float* rhs, float* out,
+                            const int64_t n[3], const double h2[3], int periodic, void* stream);
+/* Weighted Jacobi out = p + omega*(rhs-A p)/diag (mac/multigrid.py:107-110). */
+int afsi_mg_jacobi_3d_f64(const double* p, const double* rhs, double* out, const int64_t n[3],
+                          const double h2[3], const double inv_h2[3], double omega,
+                          int periodic, void* stream);
+int afsi_mg_jacobi_3d_f32(const float* p, const float* rhs, float* out, const int64_t n[3],
+                          const double h2[3], const double inv_h2[3], double omega,
+                          int periodic, void* stream);
+/* One colour of red-black Gauss-Seidel in place; colour = (i+j+k)%2
+ * (build-own smoother; no reference counterpart). */
+int afsi_mg_rbgs_3d_f64(double* p, const double* rhs, const int64_t n[3], const double h2[3],
+                        const double inv_h2[3], int color, int periodic, void* stream);
+int afsi_mg_rbgs_3d_f32(float* p, const float* rhs, const int64_t n[3], const double h2[3],
+                        const double inv_h2[3], int color, int periodic, void* stream);
+/* 8-child average (mac/_triton_mg.py:52-68); nc = coarse cells. */
+int afsi_mg_restrict_3d_f64(const double* fine, double* coarse, const int64_t nc[3], void* stream);
+int afsi_mg_restrict_3d_f32(const float* fine, float* coarse, const int64_t nc[3], void* stream);
+/* avg-pool of rhs-A p (mac/_triton_mg.py:71-86); n = fine cells. */
+int afsi_mg_residual_restrict_3d_f64(const double* p, const double* rhs, double* coarse,
+                                     const int64_t n[3], const double h2[3], int periodic,
+                                     void* stream);
+int afsi_mg_residual_restrict_3d_f32(const float* p, const float* rhs, float* coarse,
+                                     const int64_t n[3], const double h2[3], int periodic,
+                                     void* stream);
+/* fine += trilinear(coarse) (mac/_triton_mg.py:89-118); n = fine cells. */
+int afsi_mg_prolong_add_3d_f64(const double* coarse, double* fine, const int64_t n[3],
+                               int periodic, void* stream);
+int afsi_mg_prolong_add_3d_f32(const float* coarse, float* fine, const int64_t n[3],
+                               int periodic, void* stream);
+/* y = A x for a dense row-major (n x n) FP64 matrix: the coarse solve
+ * coarse_inverse @ rhs of mac/multigrid.py:114 and mac/mg_workspace.py:259. */
+int afsi_dense_matvec_f64(const double* A, const double* x, double* y, int64_t n, void* stream);
+int afsi_dense_matvec_f32(const float* A, const float* x, float* y, int64_t n, void* stream);
+
+/* ---- Reductions and vector helpers ----------------------------------------
+ * Deterministic two-stage reduction: result[0]=sum, [1]=sum of squares,
+ * [2]=sum |x|, [3]=count of non-finite entries, all FP64. `partials` is
+ * caller scratch of afsi_reduce_partials(count) * 4 doubles. */
+int64_t afsi_reduce_partials(int64_t count);
+int afsi_reduce_f64(const double* x, int64_t count, double* partials, double* result,
+                    void* stream);
+int afsi_reduce_f32(const float* x, int64_t count, double* partials, double* result,
+                    void* stream);
+/* x -= sum[0]/count (mean removal with a device-resident sum). */
+int afsi_sub_mean_f64(double* x, int64_t count, const double* sum, void* stream);
+int afsi_sub_mean_f32(float* x, int64_t count, const double* sum, void* stream);
+int afsi_fill_f64(double* x, int64_t count, double value, void* stream);
+int afsi_fill_f32(float* x, int64_t count, double value, void* stream);
+
+/* ---- MAC projection pieces (mac/flow.py:66-71,101-120; mac/grid.py:70-83)
+ * out = scale * D(u,v,w), D = sum_c diff(u_c)/h_c; scale = -rho/dt. */
+int afsi_divergence_3d_f64(const double* u, const double* v, const double* w, double* out,
+                           const int64_t n[3], const double h[3], double scale, int periodic,
+                           void* stream);
+/* u_c -= coef * G_c p (coef = dt/rho); Neumann boundary-normal faces keep
+ * zero gradient (mac/grid.py:74-83). */
+int afsi_grad_correct_3d_f64(double* u, double* v, double* w, const double* p,
+                             const int64_t n[3], const double h[3], double coef, int periodic,
+                             void* stream);
+/* y += a * x (the predictor u* = u + (dt/rho) f of the IB step). */
+int afsi_axpy_f64(double* y, const double* x, int64_t count, double a, void* stream);
+
+/* ---- Native V-cycle driver -------------------------------------------------
+ * Replaces MGWorkspace.cycle/_block/residual_norm (mac/mg_workspace.py:
+ * 246-305): issues every launch of `cycles` V-cycles on one stream from C++
+ * (capturable as one hipGraph). Per level: nu sweeps (Jacobi ping-pong with
+ * tmp, or in-place red-black GS), mean removal, fused residual-restrict,
+ * coarse mean removal, zero coarse iterate, recurse, prolong-add, nu sweeps,
+ * mean removal; the coarsest level is coarse_inverse @ rhs minus its mean
+ * (mac/multigrid.py:112-123). Buffers belong to the caller; `T` of p/rhs/
+ * tmp/coarse_inverse is the function's precision. reduce_partials holds
+ * afsi_reduce_partials(level-0 cells)*4 doubles, reduce_result 4 doubles. */
+#define AFSI_MG_MAX_LEVELS 16
+typedef struct afsi_mg3d_plan {
+    int32_t levels;
+    int32_t periodic;
+    int32_t smoother;        /* 0: weighted Jacobi (omega); 1: red-black GS */
+    int32_t nu;              /* sweeps before and after the coarse correction */
+    double omega;
+    int64_t n[AFSI_MG_MAX_LEVELS][3];
+    double h2[AFSI_MG_MAX_LEVELS][3];
+    double inv_h2[AFSI_MG_MAX_LEVELS][3];
+    void* p[AFSI_MG_MAX_LEVELS];
+    void* rhs[AFSI_MG_MAX_LEVELS];
+    void* tmp[AFSI_MG_MAX_LEVELS];
+    const void* coarse_inverse;
+    int64_t coarse_cells;
+    double* reduce_partials;
+    double* reduce_result;
+} afsi_mg3d_plan;
+int afsi_mg3d_cycles_f64(const afsi_mg3d_plan* plan, int32_t cycles, void* stream);
+int afsi_mg3d_cycles_f32(const afsi_mg3d_plan* plan, int32_t cycles, void* stream);
+/* tmp[0] = rhs[0] - A p[0]; reduce_result = its reduction (sum, sum of
+ * squares, sum |.|, non-finite count). */
+int afsi_mg3d_residual_f64(const afsi_mg3d_plan* plan, void* stream);
+int afsi_mg3d_residual_f32(const afsi_mg3d_plan* plan, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AFSI_HIP_H */

@@ -1,0 +1,27 @@
+"""Oracle: one spread -> project -> interpolate step (FP64).
+
+Test infrastructure only (see ``oracle/__init__.py``).
+
+Follows the ordering of the reference's coupled step (mac/coupling.py:74-85:
+prepare -> spread -> flow step -> interpolate -> x + dt*U) with the point
+forces given directly (no FE mass solve) and the predictor reduced to the
+force term u* = u + (dt/rho) f (convection/viscosity are not on this path):
+  f = spread(F)                      mac/compact_transfer.py:312-325
+  p = MG(-(rho/dt) D u*, p_prev)     mac/flow.py:101-120
+  u = u* - (dt/rho) G p
+  U = gather(u)                      mac/compact_transfer.py:327-341
+  x_new = x + dt U                   mac/coupling.py:84-85
+"""
+import numpy as np
+
+from . import ib3d, mac3d
+
+
+def ib_step(x, F, u, p_prev, grid, mg, *, dt, rho=1.):
+    base, phi = ib3d.prepare_compact(x, grid)
+    f = ib3d.spread_grid(F, base, phi, grid)
+    star = tuple(uc + dt / rho * fc for uc, fc in zip(u, f))
+    velocity, p, info = mac3d.project(star, grid.spacing, mg, dt=dt, rho=rho, initial=p_prev)
+    U = ib3d.gather_grid(velocity, base, phi, grid)
+    return dict(force=f, velocity=velocity, pressure=p, point_velocity=U,
+                x=x + dt * U, info=info, tentative=star)

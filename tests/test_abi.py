@@ -1,0 +1,46 @@
+"""The C-ABI library loads and exports every entry point include/afsi_hip.h
+declares (no GPU calls: the CPU suite only loads the library)."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / 'include' / 'afsi_hip.h'
+
+
+def declared():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r'^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(afsi_[a-z0-9_]+)\s*\(',
+                                 text, re.M)))
+
+
+def test_header_declarations_are_bound():
+    from afsi_gpu_amd import _abi
+    names = declared()
+    assert len(names) >= 40
+    assert sorted(_abi.exported_names()) == names
+
+
+def test_library_exports_every_declared_symbol():
+    from afsi_gpu_amd import _abi
+    if not _abi.LIBRARY.exists():
+        pytest.fail(f'{_abi.LIBRARY} not built (run __graft_entry__.build())')
+    lib = ctypes.CDLL(str(_abi.LIBRARY))
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing
+    lib = _abi.load()
+    assert lib.afsi_abi_version() >= 1
+    assert lib.afsi_reduce_partials(10**6) >= 1
+
+
+def test_host_validation_without_device_work():
+    # Argument validation runs on the host before any launch.
+    from afsi_gpu_amd import _abi
+    with pytest.raises(RuntimeError, match='invalid multigrid plan'):
+        _abi.call('afsi_mg3d_cycles_f64', _abi.MGPlan3(), 1, None)
+    with pytest.raises(RuntimeError, match='tile sizes must be positive'):
+        g = _abi.Grid3()
+        _abi.call('afsi_bin_points', None, 0, g, _abi.i32x3((0, 1, 1)), None, None, None, None, 0,
+                  None, None)

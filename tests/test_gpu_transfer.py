@@ -1,0 +1,205 @@
+"""GPU parity of stencil preparation, binning and IB transfers vs the oracle.
+
+Bars: integer work (bases, keys, perm, CSR) bit-exact; FP64 transfers
+within 1e-12 relative (l2 over the output, and max-abs relative to the
+output's max) of oracle/ib3d.py on the same seeded inputs.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import binning as obin
+from oracle import ib3d
+from afsi_gpu_amd import _abi, workloads
+from afsi_gpu_amd.mac import cuda_ib
+from afsi_gpu_amd.mac.grid import MACGrid
+from afsi_gpu_amd.mac.transfer import TiledTransfer, bin_points, from_points
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    scale = max(np.linalg.norm(b), 1e-300)
+    return np.linalg.norm(a - b) / scale
+
+
+def max_err(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def grids(periodic):
+    return (MACGrid((16, 20, 24), (4., 6., 8.), (-2., -3., -4.), periodic),
+            ib3d.Grid3((16, 20, 24), (4., 6., 8.), (-2., -3., -4.), periodic))
+
+
+def points_in(grid, n, seed, margin=2.5):
+    gen = np.random.default_rng(seed)
+    lo = np.asarray(grid.origin) + margin * np.asarray(grid.spacing)
+    hi = np.asarray(grid.origin) + np.asarray(grid.lengths) - margin * np.asarray(grid.spacing)
+    return gen.uniform(lo, hi, size=(n, 3)), gen.normal(size=(n, 3))
+
+
+@pytest.mark.parametrize('periodic', [False, True])
+def test_prepare_compact_bit_exact(gpu, periodic):
+    g, og = grids(periodic)
+    x, _ = points_in(og, 5000, 1)
+    x[:50] = np.round(x[:50] / np.asarray(og.spacing)) * np.asarray(og.spacing)  # exact nodes
+    base, phi = ib3d.prepare_compact(x, og)
+    st = from_points(torch.tensor(x, device=gpu), g, check=False)
+    assert np.array_equal(st.base.cpu().numpy(), base)
+    gphi = st.phi.cpu().numpy()
+    diff = np.abs(gphi - phi)
+    print('phi entries differing from the oracle:', int((diff > 0).sum()), 'max', diff.max())
+    assert diff.max() <= 1e-15
+
+
+def test_prepare_rejects_wall_support(gpu):
+    g, og = grids(False)
+    x = torch.tensor([[og.origin[0] + .1 * og.spacing[0], 0., 0.]], device=gpu, dtype=torch.float64)
+    with pytest.raises(ValueError, match='support'):
+        from_points(x, g)
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_compact_gather_spread(gpu, dtype):
+    # The compact ops keep the reference's closed-box indexing (no wrap).
+    g, og = grids(False)
+    x, F = points_in(og, 3000, 2)
+    base, phi = ib3d.prepare_compact(x, og)
+    gen = np.random.default_rng(3)
+    u = tuple(gen.normal(size=og.face_shape(c)) for c in range(3))
+    U_ref = ib3d.gather_grid(u, base, phi, og)
+    f_ref = ib3d.spread_grid(F, base, phi, og)
+    st = from_points(torch.tensor(x, device=gpu), g)
+    st = type(st)(st.base, st.phi.to(dtype))
+    U = cuda_ib.compact_gather(g, tuple(torch.tensor(a, device=gpu, dtype=dtype) for a in u), st,
+                               check=True)
+    f = cuda_ib.compact_spread(g, torch.tensor(F, device=gpu, dtype=dtype), st, check=True)
+    tol = TOL if dtype == torch.float64 else 2e-6
+    assert rel_err(U.cpu(), U_ref) < tol
+    for c in range(3):
+        assert rel_err(f[c].cpu(), f_ref[c]) < tol
+        assert tuple(f[c].shape) == og.face_shape(c)
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+@pytest.mark.parametrize('neighbors,components', [(16, 1), (64, 3), (37, 2)])
+def test_indexed_signed_duplicate_weights(gpu, dtype, neighbors, components):
+    gen = np.random.default_rng(31)
+    ids = gen.integers(0, 53, size=(19, neighbors))
+    w = gen.normal(size=ids.shape)
+    u = gen.normal(size=(53, components))
+    f = gen.normal(size=(19, components))
+    v = .017
+    t = lambda a: torch.tensor(a, device=gpu, dtype=dtype)
+    tid = torch.tensor(ids, device=gpu)
+    got = cuda_ib.indexed_gather(t(u), tid, t(w), check=True)
+    den = cuda_ib.indexed_spread(t(f), tid, t(w), 53, v, check=True)
+    tol = TOL if dtype == torch.float64 else 3e-6
+    assert rel_err(got.cpu(), ib3d.indexed_gather(u, ids, w)) < tol
+    assert rel_err(den.cpu(), ib3d.indexed_spread(f, ids, w, 53, v)) < tol
+
+
+def test_indexed_empty_and_rejects_cpu(gpu):
+    f = torch.ones((0, 1), device=gpu, dtype=torch.float64)
+    ids = torch.zeros((0, 2), dtype=torch.int64, device=gpu)
+    w = torch.ones((0, 2), device=gpu, dtype=torch.float64)
+    out = cuda_ib.indexed_spread(f, ids, w, 3)
+    assert out.shape == (3, 1) and out.count_nonzero().item() == 0
+    with pytest.raises(ValueError, match='requires CUDA'):
+        cuda_ib.indexed_gather(torch.zeros(4, 1), torch.zeros(1, 2, dtype=torch.int64), torch.ones(1, 2))
+    bad = torch.tensor([[0, 99]], device=gpu)
+    with pytest.raises(RuntimeError, match='out of range'):
+        cuda_ib.indexed_gather(torch.ones((4, 1), device=gpu, dtype=torch.float64), bad,
+                               torch.ones((1, 2), device=gpu, dtype=torch.float64), check=True)
+
+
+@pytest.mark.parametrize('periodic', [False, True])
+@pytest.mark.parametrize('tile', [(8, 8, 8), (4, 8, 16), (5, 3, 7)])
+def test_binning_bit_exact(gpu, periodic, tile):
+    g, og = grids(periodic)
+    x, _ = points_in(og, 20000, 4)
+    x[::5] = x[7]  # long runs of equal keys exercise stability
+    if periodic:
+        x[1] = np.asarray(og.origin) - .3 * np.asarray(og.spacing)
+        x[2] = np.asarray(og.origin) + np.asarray(og.lengths) + .1
+    keys, perm, ptr = obin.bin_points(x, og, tile)
+    b = bin_points(torch.tensor(x, device=gpu), g, tile)
+    b.check()
+    assert np.array_equal(b.perm.cpu().numpy().astype(np.int64), perm)
+    assert np.array_equal(b.keys.cpu().numpy().astype(np.uint32).astype(np.int64), keys[perm])
+    assert np.array_equal(b.cell_ptr.cpu().numpy().astype(np.int64), ptr)
+
+
+def test_binning_config2_bit_exact(gpu):
+    x, _ = workloads.config2()
+    og = ib3d.Grid3((64, 64, 64), (1., 1., 1.), periodic=True)
+    g = MACGrid((64, 64, 64), (1., 1., 1.), periodic=True)
+    keys, perm, ptr = obin.bin_points(x, og, (8, 8, 8))
+    b = bin_points(torch.tensor(x, device=gpu), g, (8, 8, 8))
+    assert np.array_equal(b.perm.cpu().numpy().astype(np.int64), perm)
+    assert np.array_equal(b.cell_ptr.cpu().numpy().astype(np.int64), ptr)
+
+
+def test_binning_empty(gpu):
+    g, _ = grids(True)
+    b = bin_points(torch.zeros((0, 3), device=gpu, dtype=torch.float64), g)
+    assert b.cell_ptr.count_nonzero().item() == 0
+
+
+@pytest.mark.parametrize('periodic', [False, True])
+@pytest.mark.parametrize('tile', [(8, 8, 8), (4, 4, 4), (5, 3, 7)])
+def test_tiled_spread_and_interp(gpu, periodic, tile):
+    g, og = grids(periodic)
+    x, F = points_in(og, 20000, 5)
+    x[:3000] = x[0] + 1e-3 * np.random.default_rng(6).normal(size=(3000, 3))  # dense cluster
+    if periodic:
+        x[1] = np.asarray(og.origin) - .3 * np.asarray(og.spacing)
+    base, phi = ib3d.prepare_compact(x, og)
+    f_ref = ib3d.spread_grid(F, base, phi, og)
+    gen = np.random.default_rng(7)
+    u = tuple(gen.normal(size=og.face_shape(c)) for c in range(3))
+    U_ref = ib3d.gather_grid(u, base, phi, og)
+    xt, Ft = torch.tensor(x, device=gpu), torch.tensor(F, device=gpu)
+    bins = bin_points(xt, g, tile)
+    tr = TiledTransfer(g, gpu, tile)
+    f = tr.spread(xt, Ft, bins)
+    U = tr.interpolate(xt, tuple(torch.tensor(a, device=gpu) for a in u), bins)
+    assert int(tr.flags.item()) == 0
+    for c in range(3):
+        assert rel_err(f[c].cpu(), f_ref[c]) < TOL
+        assert max_err(f[c].cpu(), f_ref[c]) < 10 * TOL
+    assert rel_err(U.cpu(), U_ref) < TOL
+    assert max_err(U.cpu(), U_ref) < 10 * TOL
+
+
+def test_tiled_transfer_config2(gpu):
+    x, F = workloads.config2()
+    og = ib3d.Grid3((64, 64, 64), (1., 1., 1.), periodic=True)
+    g = MACGrid((64, 64, 64), (1., 1., 1.), periodic=True)
+    base, phi = ib3d.prepare_compact(x, og)
+    f_ref = ib3d.spread_grid(F, base, phi, og)
+    U_ref = ib3d.gather_grid(f_ref, base, phi, og)
+    xt, Ft = torch.tensor(x, device=gpu), torch.tensor(F, device=gpu)
+    bins = bin_points(xt, g)
+    tr = TiledTransfer(g, gpu)
+    f = tr.spread(xt, Ft, bins)
+    U = tr.interpolate(xt, f, bins)
+    errs = [rel_err(f[c].cpu(), f_ref[c]) for c in range(3)] + [rel_err(U.cpu(), U_ref)]
+    print('config2 spread/interp l2-relative errors:', errs)
+    assert max(errs) < TOL
+
+
+def test_sqrt_and_division_round_like_numpy(gpu):
+    # In-bounds check of the compiler's FP64 sqrt/div expansion: the
+    # two-branch Peskin phi exercises both on 200k random distances.
+    gen = np.random.default_rng(12)
+    g, og = grids(True)
+    x = gen.uniform(0, 1, size=(200000, 3)) * np.asarray(og.lengths) + np.asarray(og.origin)
+    _, phi = ib3d.prepare_compact(x, og)
+    st = from_points(torch.tensor(x, device=gpu), g, check=False)
+    gphi = st.phi.cpu().numpy()
+    assert int((gphi != phi).sum()) == 0
