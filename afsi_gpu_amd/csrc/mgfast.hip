@@ -1,0 +1,1100 @@
+// Fused V-cycle for gfx950: few launches per cycle, one HBM pass per sweep.
+//
+// * rb_sweep_kernel: one full red-black Gauss-Seidel sweep per launch. A
+//   workgroup stages its 8x8x32 tile plus a 2-cell halo of p in LDS,
+//   updates the red cells of the tile and its 1-cell ring in LDS, then the
+//   black cells of the tile, and writes the tile (out of place: p_in is
+//   never written, so every tile reads start-of-sweep values). Update
+//   p = (rhs + sum_nb w_a p_nb) / sum w_a, w_a = 1/h_a^2, over the existing
+//   neighbours, which equals p + (rhs - A p)/diag.
+// * Mean removal. The reference subtracts the mean of p after each
+//   smoothing stage and of every restricted residual (mac/multigrid.py:
+//   107-123). The V-cycle commutes with adding a constant (A 1 = 0, both
+//   smoothers and the prolongation reproduce constants, and the coarse
+//   inverse of a zero-mean residual has zero mean), so on the fine levels
+//   those subtractions change nothing but rounding; this driver performs
+//   them inside the LDS coarse kernel only; the caller removes the mean of
+//   the level-0 iterate once, after the last cycle. No kernel needs a
+//   grid-wide reduction, fence or atomic.
+// * coarse_cycle_kernel: one 1024-thread workgroup runs the complete
+//   V-cycle of every level with <= COARSE_MAX cells (smoothing, restriction,
+//   means, the dense coarse solve, prolongation) out of LDS.
+// * Tiles are mapped XCD-aware: consecutive workgroup ids are dealt
+//   round-robin to the 8 XCDs, so each XCD is given one contiguous run of
+//   tiles and the halos it re-reads stay in its own L2.
+#include "mgcommon.hpp"
+
+namespace afsi {
+
+template <class T>
+int launch_fill(T* x, int64_t count, T value, hipStream_t s);
+int launch_reduce_final(const double* partials, int64_t nparts, double* result, hipStream_t s);
+
+constexpr int COARSE_THREADS = 1024;
+constexpr int64_t COARSE_MAX = 4096;  // cells of the finest level solved in LDS
+
+struct Dim32 {
+    int n0, n1, n2;
+};
+
+template <class T>
+struct W3 {
+    T w[3];  // 1/h^2
+};
+
+__device__ inline int wrap32(int g, int n) {
+    while (g < 0) g += n;
+    while (g >= n) g -= n;
+    return g;
+}
+
+__device__ inline int64_t xcd_tile(int64_t b, int64_t ntiles) {
+    if (ntiles % 8) return b;
+    return (b % 8) * (ntiles / 8) + b / 8;
+}
+
+// Gauss-Seidel / Jacobi pieces on an array with strides (sx, sy, 1).
+// s = sum_nb w_a p_nb and d = sum w_a over the neighbours that exist.
+template <class T, bool P>
+__device__ inline void neighbour_sums(const T* __restrict__ a, int li, int sx, int sy, int gx,
+                                      int gy, int gz, const Dim32& d, const W3<T>& k, T& s, T& dg) {
+    s = T(0);
+    dg = T(0);
+    if (P || gx + 1 < d.n0) { s += k.w[0] * a[li + sx]; dg += k.w[0]; }
+    if (P || gx > 0) { s += k.w[0] * a[li - sx]; dg += k.w[0]; }
+    if (P || gy + 1 < d.n1) { s += k.w[1] * a[li + sy]; dg += k.w[1]; }
+    if (P || gy > 0) { s += k.w[1] * a[li - sy]; dg += k.w[1]; }
+    if (P || gz + 1 < d.n2) { s += k.w[2] * a[li + 1]; dg += k.w[2]; }
+    if (P || gz > 0) { s += k.w[2] * a[li - 1]; dg += k.w[2]; }
+}
+
+// A p = sum_nb w_a (p - p_nb) over existing neighbours (difference form:
+// neighbour differences of a smooth field are nearly exact, so the residual
+// keeps its accuracy when p carries a constant offset).
+template <class T, bool P>
+__device__ inline T apply_diff(const T* __restrict__ a, int li, int sx, int sy,
+                               int gx, int gy, int gz, const Dim32& d, const W3<T>& k) {
+    const T c = a[li];
+    T r = T(0);
+    if (P) {
+        r += k.w[0] * (c - a[li + (gx + 1 < d.n0 ? sx : -(d.n0 - 1) * sx)]);
+        r += k.w[0] * (c - a[li + (gx > 0 ? -sx : (d.n0 - 1) * sx)]);
+        r += k.w[1] * (c - a[li + (gy + 1 < d.n1 ? sy : -(d.n1 - 1) * sy)]);
+        r += k.w[1] * (c - a[li + (gy > 0 ? -sy : (d.n1 - 1) * sy)]);
+        r += k.w[2] * (c - a[li + (gz + 1 < d.n2 ? 1 : -(d.n2 - 1))]);
+        r += k.w[2] * (c - a[li + (gz > 0 ? -1 : (d.n2 - 1))]);
+    } else {
+        if (gx + 1 < d.n0) r += k.w[0] * (c - a[li + sx]);
+        if (gx > 0) r += k.w[0] * (c - a[li - sx]);
+        if (gy + 1 < d.n1) r += k.w[1] * (c - a[li + sy]);
+        if (gy > 0) r += k.w[1] * (c - a[li - sy]);
+        if (gz + 1 < d.n2) r += k.w[2] * (c - a[li + 1]);
+        if (gz > 0) r += k.w[2] * (c - a[li - 1]);
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// fine levels
+
+// Tile TX x TY x TZ cells, NT threads. All global loads of a thread (its
+// share of the p box and the rhs of its red-ring and black-tile cells) are
+// issued into registers before the first LDS write, so their latencies
+// overlap instead of serialising one loop iteration at a time.
+template <class T, bool P, int TX, int TY, int TZ, int NT>
+__global__ void __launch_bounds__(NT) rb_sweep_kernel(const T* __restrict__ pin,
+                                                      const T* __restrict__ rhs,
+                                                      T* __restrict__ pout, Dim32 d, W3<T> k,
+                                                      int zero_in, int ntx, int nty, int ntz) {
+    constexpr int BX = TX + 4, BY = TY + 4, BZ = TZ + 4, BOX = BX * BY * BZ;
+    constexpr int RX = TX + 2, RY = TY + 2, RZ = TZ + 2, RING = RX * RY * RZ;
+    constexpr int TILE = TX * TY * TZ;
+    constexpr int NB = (BOX + NT - 1) / NT, NR = (RING + NT - 1) / NT, NL = (TILE + NT - 1) / NT;
+    __shared__ T box[BOX];
+    const int tile = int(xcd_tile(blockIdx.x, int64_t(ntx) * nty * ntz));
+    const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
+    const int x0 = tx * TX - 2, y0 = ty * TY - 2, z0 = tz * TZ - 2;
+    const int sxg = d.n1 * d.n2;
+    const int t = threadIdx.x;
+    T pv[NB], rr[NR], rt[NL];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int e = t + j * NT;
+        pv[j] = T(0);
+        if (e < BOX && !zero_in) {
+            const int lz = e % BZ, ly = (e / BZ) % BY, lx = e / (BZ * BY);
+            int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+            if (P) {
+                gx = wrap32(gx, d.n0); gy = wrap32(gy, d.n1); gz = wrap32(gz, d.n2);
+                pv[j] = pin[gx * sxg + gy * d.n2 + gz];
+            } else if (gx >= 0 && gx < d.n0 && gy >= 0 && gy < d.n1 && gz >= 0 && gz < d.n2) {
+                pv[j] = pin[gx * sxg + gy * d.n2 + gz];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        const int e = t + j * NT;
+        rr[j] = T(0);
+        if (e < RING) {
+            const int lz = e % RZ + 1, ly = (e / RZ) % RY + 1, lx = e / (RZ * RY) + 1;
+            int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+            bool in = true;
+            if (P) { gx = wrap32(gx, d.n0); gy = wrap32(gy, d.n1); gz = wrap32(gz, d.n2); }
+            else in = gx >= 0 && gx < d.n0 && gy >= 0 && gy < d.n1 && gz >= 0 && gz < d.n2;
+            if (in && !((gx + gy + gz) & 1)) rr[j] = rhs[gx * sxg + gy * d.n2 + gz];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+        const int e = t + j * NT;
+        rt[j] = T(0);
+        if (e < TILE) {
+            const int lz = e % TZ + 2, ly = (e / TZ) % TY + 2, lx = e / (TZ * TY) + 2;
+            const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+            if (gx < d.n0 && gy < d.n1 && gz < d.n2 && ((gx + gy + gz) & 1))
+                rt[j] = rhs[gx * sxg + gy * d.n2 + gz];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int e = t + j * NT;
+        if (e < BOX) box[e] = pv[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        const int e = t + j * NT;
+        if (e >= RING) continue;
+        const int lz = e % RZ + 1, ly = (e / RZ) % RY + 1, lx = e / (RZ * RY) + 1;
+        int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+        if (P) {
+            gx = wrap32(gx, d.n0); gy = wrap32(gy, d.n1); gz = wrap32(gz, d.n2);
+        } else if (gx < 0 || gx >= d.n0 || gy < 0 || gy >= d.n1 || gz < 0 || gz >= d.n2) {
+            continue;
+        }
+        if ((gx + gy + gz) & 1) continue;  // red = even parity
+        const int li = (lx * BY + ly) * BZ + lz;
+        T sum, dg;
+        neighbour_sums<T, P>(box, li, BY * BZ, BZ, gx, gy, gz, d, k, sum, dg);
+        box[li] = (rr[j] + sum) / dg;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+        const int e = t + j * NT;
+        if (e >= TILE) continue;
+        const int lz = e % TZ + 2, ly = (e / TZ) % TY + 2, lx = e / (TZ * TY) + 2;
+        const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+        if (gx >= d.n0 || gy >= d.n1 || gz >= d.n2) continue;
+        const int li = (lx * BY + ly) * BZ + lz;
+        T v = box[li];
+        if ((gx + gy + gz) & 1) {
+            T sum, dg;
+            neighbour_sums<T, P>(box, li, BY * BZ, BZ, gx, gy, gz, d, k, sum, dg);
+            v = (rt[j] + sum) / dg;
+        }
+        pout[gx * sxg + gy * d.n2 + gz] = v;
+    }
+}
+
+// weighted Jacobi: out = p + omega*(rhs - A p)/diag
+template <class T, bool P>
+__global__ void __launch_bounds__(256) jacobi_fast_kernel(const T* __restrict__ pin,
+                                                          const T* __restrict__ rhs,
+                                                          T* __restrict__ pout, Dim32 d, W3<T> k,
+                                                          T omega, int zero_in) {
+    const int sxg = d.n1 * d.n2;
+    const int total = sxg * d.n0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int x = i / sxg;
+        const int r = i - x * sxg;
+        const int y = r / d.n2, z = r - y * d.n2;
+        const T dg = (P || x + 1 < d.n0 ? k.w[0] : T(0)) + (P || x > 0 ? k.w[0] : T(0)) +
+                     (P || y + 1 < d.n1 ? k.w[1] : T(0)) + (P || y > 0 ? k.w[1] : T(0)) +
+                     (P || z + 1 < d.n2 ? k.w[2] : T(0)) + (P || z > 0 ? k.w[2] : T(0));
+        const T c = zero_in ? T(0) : pin[i];
+        const T a = zero_in ? T(0) : apply_diff<T, P>(pin, i, sxg, d.n2, x, y, z, d, k);
+        pout[i] = c + omega * (rhs[i] - a) / dg;
+    }
+}
+
+// coarse = avg8(rhs - A p)
+template <class T, bool P>
+__global__ void __launch_bounds__(256) restrict_fast_kernel(const T* __restrict__ p,
+                                                            const T* __restrict__ rhs,
+                                                            T* __restrict__ coarse, Dim32 d,
+                                                            W3<T> k) {
+    const int c1 = d.n1 / 2, c2 = d.n2 / 2;
+    const int cs = c1 * c2;
+    const int total = cs * (d.n0 / 2);
+    const int sxg = d.n1 * d.n2;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int x = i / cs;
+        const int r = i - x * cs;
+        const int y = r / c2, z = r - y * c2;
+        T v = T(0);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) {
+                    const int fx = 2 * x + a, fy = 2 * y + b, fz = 2 * z + cc;
+                    const int j = fx * sxg + fy * d.n2 + fz;
+                    v += rhs[j] - apply_diff<T, P>(p, j, sxg, d.n2, fx, fy, fz, d, k);
+                }
+        coarse[i] = v * T(0.125);
+    }
+}
+
+template <class T, bool P>
+__device__ inline T prolong32(const T* __restrict__ coarse, int x, int y, int z, int C0, int C1,
+                              int C2) {
+    const int ax = x / 2 - 1 + x % 2, ay = y / 2 - 1 + y % 2, az = z / 2 - 1 + z % 2;
+    int x0, x1, y0, y1, z0, z1;
+    if (P) {
+        x0 = wrap32(ax, C0); x1 = wrap32(ax + 1, C0);
+        y0 = wrap32(ay, C1); y1 = wrap32(ay + 1, C1);
+        z0 = wrap32(az, C2); z1 = wrap32(az + 1, C2);
+    } else {
+        x0 = ax > 0 ? ax : 0; x1 = ax + 1 < C0 - 1 ? ax + 1 : C0 - 1;
+        y0 = ay > 0 ? ay : 0; y1 = ay + 1 < C1 - 1 ? ay + 1 : C1 - 1;
+        z0 = az > 0 ? az : 0; z1 = az + 1 < C2 - 1 ? az + 1 : C2 - 1;
+    }
+    const T wx = (x % 2 == 0) ? T(0.75) : T(0.25);
+    const T wy = (y % 2 == 0) ? T(0.75) : T(0.25);
+    const T wz = (z % 2 == 0) ? T(0.75) : T(0.25);
+    const int s1 = C2, s0 = C1 * C2;
+    const T* b00 = coarse + x0 * s0 + y0 * s1;
+    const T* b01 = coarse + x0 * s0 + y1 * s1;
+    const T* b10 = coarse + x1 * s0 + y0 * s1;
+    const T* b11 = coarse + x1 * s0 + y1 * s1;
+    const T c00 = b00[z0] * (T(1) - wz) + b00[z1] * wz;
+    const T c01 = b01[z0] * (T(1) - wz) + b01[z1] * wz;
+    const T c10 = b10[z0] * (T(1) - wz) + b10[z1] * wz;
+    const T c11 = b11[z0] * (T(1) - wz) + b11[z1] * wz;
+    return (c00 * (T(1) - wy) + c01 * wy) * (T(1) - wx) + (c10 * (T(1) - wy) + c11 * wy) * wx;
+}
+
+template <class T, bool P>
+__global__ void __launch_bounds__(256) prolong_fast_kernel(const T* __restrict__ coarse,
+                                                           T* __restrict__ fine, Dim32 d) {
+    const int sxg = d.n1 * d.n2;
+    const int total = sxg * d.n0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int x = i / sxg;
+        const int r = i - x * sxg;
+        const int y = r / d.n2, z = r - y * d.n2;
+        fine[i] = fine[i] + prolong32<T, P>(coarse, x, y, z, d.n0 / 2, d.n1 / 2, d.n2 / 2);
+    }
+}
+
+// per-workgroup [sum r, sum r^2, sum |r|, non-finite] of r = rhs - A p
+template <class T, bool P>
+__global__ void __launch_bounds__(256) residual_partials_kernel(const T* __restrict__ p,
+                                                                const T* __restrict__ rhs, Dim32 d,
+                                                                W3<T> k, double* __restrict__ partials) {
+    __shared__ double sh[4][256 / WAVE];
+    const int sxg = d.n1 * d.n2;
+    const int total = sxg * d.n0;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int x = i / sxg;
+        const int rr = i - x * sxg;
+        const int y = rr / d.n2, z = rr - y * d.n2;
+        const double r = double(rhs[i] - apply_diff<T, P>(p, i, sxg, d.n2, x, y, z, d, k));
+        if (isfinite(r)) {
+            v[0] += r;
+            v[1] += r * r;
+            v[2] += fabs(r);
+        } else {
+            v[3] += 1.0;
+        }
+    }
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    for (int q = 0; q < 4; ++q) {
+        double t = v[q];
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1) t += __shfl_xor(t, dd, WAVE);
+        if (lane == 0) sh[q][wave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (int w = 0; w < 256 / WAVE; ++w) t += sh[threadIdx.x][w];
+        partials[4 * blockIdx.x + threadIdx.x] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row kernels (levels with an even z extent <= ZROW): one wavefront per
+// (x, y) row, lanes along z. Row decode, wrap and boundary tests are
+// wave-uniform, lanes map to z (or to the red / black cells of a row by
+// parity), so no per-element integer division remains.
+
+constexpr int ZROW = 128;
+
+__device__ inline int face_class(int g, int n) { return g == 0 ? 0 : (g == n - 1 ? 2 : 1); }
+
+// 1/diag of the Neumann operator by boundary class (0 low, 1 interior,
+// 2 high) per axis; entry t of a 27-table.
+template <class T>
+__device__ inline T inv_diag_entry(int t, const W3<T>& k) {
+    const int cls[3] = {t / 9, (t / 3) % 3, t % 3};
+    T dg = T(0);
+    for (int a = 0; a < 3; ++a) dg += k.w[a] * T(cls[a] == 1 ? 2 : 1);
+    return T(1) / dg;
+}
+
+template <class T, bool P>
+__device__ inline int wrap_or(int g, int n, bool& in) {
+    if (P) return wrap32(g, n);
+    in = in && g >= 0 && g < n;
+    return g;
+}
+
+// One red-black Gauss-Seidel sweep of a TX x TY column of rows spanning all
+// of z. LDS holds the (TX+4) x (TY+4) rows of p; red is updated on the
+// tile and its 1-row ring, then black on the tile. Lane r < ROWS first
+// writes the record of box row r (wrapped global offset, coordinates,
+// in-domain flag) to LDS; the row loops then read their record, so the
+// per-row bookkeeping is done once per row for the whole workgroup.
+struct RowRec {
+    int off, gx, gy, in;
+};
+
+template <class T, bool P, int TX, int TY>
+__global__ void __launch_bounds__(256) rb_rows_kernel(const T* __restrict__ pin,
+                                                      const T* __restrict__ rhs,
+                                                      T* __restrict__ pout, Dim32 d, W3<T> k,
+                                                      int zero_in, int ntx, int nty) {
+    constexpr int NW = 4;
+    constexpr int BYr = TY + 4, ROWS = (TX + 4) * BYr;
+    constexpr int RYr = TY + 2, RING = (TX + 2) * RYr;
+    constexpr int TILE = TX * TY;
+    constexpr int LR = (ROWS + NW - 1) / NW, RR = (RING + NW - 1) / NW, TR = (TILE + NW - 1) / NW;
+    static_assert(ROWS <= 256, "one thread per box row");
+    extern __shared__ unsigned char raw[];
+    T* box = reinterpret_cast<T*>(raw);
+    __shared__ T tab[27];
+    __shared__ RowRec rec[ROWS];
+    const int n2 = d.n2, h2 = n2 >> 1;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tile = int(xcd_tile(blockIdx.x, int64_t(ntx) * nty));
+    const int ty = tile % nty, tx = tile / nty;
+    const int x0 = tx * TX - 2, y0 = ty * TY - 2;
+    const int sxg = d.n1 * n2;
+    if (threadIdx.x < ROWS) {
+        const int r = threadIdx.x;
+        int gx = x0 + r / BYr, gy = y0 + r % BYr;
+        int in = 1;
+        if (P) {
+            gx = gx < 0 ? gx + d.n0 : (gx >= d.n0 ? gx - d.n0 : gx);
+            gy = gy < 0 ? gy + d.n1 : (gy >= d.n1 ? gy - d.n1 : gy);
+        } else {
+            in = gx >= 0 && gx < d.n0 && gy >= 0 && gy < d.n1;
+        }
+        rec[r] = RowRec{in ? gx * sxg + gy * n2 : 0, gx, gy, in};
+    }
+    if (!P && threadIdx.x < 27) tab[threadIdx.x] = inv_diag_entry<T>(threadIdx.x, k);
+    __syncthreads();
+    auto load_rec = [&](int r) {
+        RowRec q = rec[r];
+        q.off = __builtin_amdgcn_readfirstlane(q.off);
+        q.gx = __builtin_amdgcn_readfirstlane(q.gx);
+        q.gy = __builtin_amdgcn_readfirstlane(q.gy);
+        q.in = __builtin_amdgcn_readfirstlane(q.in);
+        return q;
+    };
+    T pv[LR][2];
+#pragma unroll
+    for (int q = 0; q < LR; ++q) {
+        const int r = wave + q * NW;
+        pv[q][0] = pv[q][1] = T(0);
+        if (r < ROWS && !zero_in) {
+            const RowRec rc = load_rec(r);
+            if (rc.in) {
+                const T* src = pin + rc.off;
+                if (lane < n2) pv[q][0] = src[lane];
+                if (lane + 64 < n2) pv[q][1] = src[lane + 64];
+            }
+        }
+    }
+    T rr[RR], rt[TR];
+#pragma unroll
+    for (int q = 0; q < RR; ++q) {
+        const int rw = wave + q * NW;
+        rr[q] = T(0);
+        if (rw < RING && lane < h2) {
+            const RowRec rc = load_rec((1 + rw / RYr) * BYr + 1 + rw % RYr);
+            if (rc.in) rr[q] = rhs[rc.off + 2 * lane + ((rc.gx + rc.gy) & 1)];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < TR; ++q) {
+        const int tw = wave + q * NW;
+        rt[q] = T(0);
+        if (tw < TILE && lane < h2) {
+            const RowRec rc = load_rec((2 + tw / TY) * BYr + 2 + tw % TY);
+            if (x0 + 2 + tw / TY < d.n0 && y0 + 2 + tw % TY < d.n1)
+                rt[q] = rhs[rc.off + 2 * lane + ((rc.gx + rc.gy + 1) & 1)];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < LR; ++q) {
+        const int r = wave + q * NW;
+        if (r < ROWS) {
+            if (lane < n2) box[r * n2 + lane] = pv[q][0];
+            if (lane + 64 < n2) box[r * n2 + lane + 64] = pv[q][1];
+        }
+    }
+    const T inv_p = T(1) / (T(2) * (k.w[0] + k.w[1] + k.w[2]));
+    __syncthreads();
+    auto update = [&](const T* row, int r, int gx, int gy, int z, T b) -> T {
+        T s = T(0);
+        if (P) {
+            s += k.w[2] * row[z + 1 < n2 ? z + 1 : 0];
+            s += k.w[2] * row[z > 0 ? z - 1 : n2 - 1];
+        } else {
+            if (z + 1 < n2) s += k.w[2] * row[z + 1];
+            if (z > 0) s += k.w[2] * row[z - 1];
+        }
+        if (P || gy + 1 < d.n1) s += k.w[1] * box[(r + 1) * n2 + z];
+        if (P || gy > 0) s += k.w[1] * box[(r - 1) * n2 + z];
+        if (P || gx + 1 < d.n0) s += k.w[0] * box[(r + BYr) * n2 + z];
+        if (P || gx > 0) s += k.w[0] * box[(r - BYr) * n2 + z];
+        const T inv = P ? inv_p : tab[face_class(gx, d.n0) * 9 + face_class(gy, d.n1) * 3 + face_class(z, n2)];
+        return (b + s) * inv;
+    };
+#pragma unroll
+    for (int q = 0; q < RR; ++q) {
+        const int rw = wave + q * NW;
+        if (rw >= RING || lane >= h2) continue;
+        const int r = (1 + rw / RYr) * BYr + 1 + rw % RYr;
+        const RowRec rc = load_rec(r);
+        if (!rc.in) continue;
+        T* row = box + r * n2;
+        const int z = 2 * lane + ((rc.gx + rc.gy) & 1);
+        row[z] = update(row, r, rc.gx, rc.gy, z, rr[q]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TR; ++q) {
+        const int tw = wave + q * NW;
+        if (tw >= TILE || lane >= h2) continue;
+        if (x0 + 2 + tw / TY >= d.n0 || y0 + 2 + tw % TY >= d.n1) continue;
+        const int r = (2 + tw / TY) * BYr + 2 + tw % TY;
+        const RowRec rc = load_rec(r);
+        const T* row = box + r * n2;
+        const int zb = 2 * lane + ((rc.gx + rc.gy + 1) & 1), zr = 2 * lane + ((rc.gx + rc.gy) & 1);
+        T* dst = pout + rc.off;
+        dst[zb] = update(row, r, rc.gx, rc.gy, zb, rt[q]);
+        dst[zr] = row[zr];
+    }
+}
+
+// Pointers to a row and its four x/y neighbour rows (null where a Neumann
+// neighbour does not exist).
+template <class T, bool P>
+struct Rows5 {
+    const T *c, *xm, *xp, *ym, *yp;
+};
+
+template <class T, bool P>
+__device__ inline Rows5<T, P> rows5(const T* base, int x, int y, const Dim32& d) {
+    const int sx = d.n1 * d.n2, sy = d.n2;
+    Rows5<T, P> r;
+    r.c = base + x * sx + y * sy;
+    if (P) {
+        r.xm = base + (x > 0 ? x - 1 : d.n0 - 1) * sx + y * sy;
+        r.xp = base + (x + 1 < d.n0 ? x + 1 : 0) * sx + y * sy;
+        r.ym = base + x * sx + (y > 0 ? y - 1 : d.n1 - 1) * sy;
+        r.yp = base + x * sx + (y + 1 < d.n1 ? y + 1 : 0) * sy;
+    } else {
+        r.xm = x > 0 ? r.c - sx : nullptr;
+        r.xp = x + 1 < d.n0 ? r.c + sx : nullptr;
+        r.ym = y > 0 ? r.c - sy : nullptr;
+        r.yp = y + 1 < d.n1 ? r.c + sy : nullptr;
+    }
+    return r;
+}
+
+// (A p)_z on a row, difference form.
+template <class T, bool P>
+__device__ inline T apply_row(const Rows5<T, P>& r, int z, int n2, const W3<T>& k) {
+    const T c = r.c[z];
+    T a = T(0);
+    if (P) {
+        a += k.w[0] * (c - r.xp[z]);
+        a += k.w[0] * (c - r.xm[z]);
+        a += k.w[1] * (c - r.yp[z]);
+        a += k.w[1] * (c - r.ym[z]);
+        a += k.w[2] * (c - r.c[z + 1 < n2 ? z + 1 : 0]);
+        a += k.w[2] * (c - r.c[z > 0 ? z - 1 : n2 - 1]);
+    } else {
+        if (r.xp) a += k.w[0] * (c - r.xp[z]);
+        if (r.xm) a += k.w[0] * (c - r.xm[z]);
+        if (r.yp) a += k.w[1] * (c - r.yp[z]);
+        if (r.ym) a += k.w[1] * (c - r.ym[z]);
+        if (z + 1 < n2) a += k.w[2] * (c - r.c[z + 1]);
+        if (z > 0) a += k.w[2] * (c - r.c[z - 1]);
+    }
+    return a;
+}
+
+// coarse = avg8(rhs - A p); one wavefront per coarse row.
+template <class T, bool P>
+__global__ void __launch_bounds__(256) restrict_rows_kernel(const T* __restrict__ p,
+                                                            const T* __restrict__ rhs,
+                                                            T* __restrict__ coarse, Dim32 d,
+                                                            W3<T> k) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    const int C0 = d.n0 / 2, C1 = d.n1 / 2, C2 = d.n2 / 2;
+    const int row = blockIdx.x * 4 + wave;
+    if (row >= C0 * C1) return;
+    const int cx = row / C1, cy = row - cx * C1;
+    Rows5<T, P> pr[2][2];
+    const T* rr[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            pr[a][b] = rows5<T, P>(p, 2 * cx + a, 2 * cy + b, d);
+            rr[a][b] = rhs + (2 * cx + a) * d.n1 * d.n2 + (2 * cy + b) * d.n2;
+        }
+    T* out = coarse + (cx * C1 + cy) * C2;
+    for (int cz = lane; cz < C2; cz += 64) {
+        T v = T(0);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const int z = 2 * cz + c;
+                    v += rr[a][b][z] - apply_row<T, P>(pr[a][b], z, d.n2, k);
+                }
+        out[cz] = v * T(0.125);
+    }
+}
+
+// fine += trilinear(coarse); one wavefront per fine row.
+template <class T, bool P>
+__global__ void __launch_bounds__(256) prolong_rows_kernel(const T* __restrict__ coarse,
+                                                           T* __restrict__ fine, Dim32 d) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    const int row = blockIdx.x * 4 + wave;
+    if (row >= d.n0 * d.n1) return;
+    const int x = row / d.n1, y = row - x * d.n1;
+    const int C0 = d.n0 / 2, C1 = d.n1 / 2, C2 = d.n2 / 2;
+    const int ax = x / 2 - 1 + (x & 1), ay = y / 2 - 1 + (y & 1);
+    int x0, x1, y0, y1;
+    if (P) {
+        x0 = wrap32(ax, C0); x1 = wrap32(ax + 1, C0);
+        y0 = wrap32(ay, C1); y1 = wrap32(ay + 1, C1);
+    } else {
+        x0 = ax > 0 ? ax : 0; x1 = ax + 1 < C0 - 1 ? ax + 1 : C0 - 1;
+        y0 = ay > 0 ? ay : 0; y1 = ay + 1 < C1 - 1 ? ay + 1 : C1 - 1;
+    }
+    const T wx = (x & 1) ? T(0.25) : T(0.75);
+    const T wy = (y & 1) ? T(0.25) : T(0.75);
+    const T* c00 = coarse + (x0 * C1 + y0) * C2;
+    const T* c01 = coarse + (x0 * C1 + y1) * C2;
+    const T* c10 = coarse + (x1 * C1 + y0) * C2;
+    const T* c11 = coarse + (x1 * C1 + y1) * C2;
+    T* f = fine + (x * d.n1 + y) * d.n2;
+    for (int z = lane; z < d.n2; z += 64) {
+        const int az = z / 2 - 1 + (z & 1);
+        int z0, z1;
+        if (P) { z0 = az < 0 ? az + C2 : az; z1 = az + 1 >= C2 ? az + 1 - C2 : az + 1; }
+        else { z0 = az > 0 ? az : 0; z1 = az + 1 < C2 - 1 ? az + 1 : C2 - 1; }
+        const T wz = (z & 1) ? T(0.25) : T(0.75);
+        const T v00 = c00[z0] * (T(1) - wz) + c00[z1] * wz;
+        const T v01 = c01[z0] * (T(1) - wz) + c01[z1] * wz;
+        const T v10 = c10[z0] * (T(1) - wz) + c10[z1] * wz;
+        const T v11 = c11[z0] * (T(1) - wz) + c11[z1] * wz;
+        f[z] = f[z] + ((v00 * (T(1) - wy) + v01 * wy) * (T(1) - wx) + (v10 * (T(1) - wy) + v11 * wy) * wx);
+    }
+}
+
+// per-workgroup [sum r, sum r^2, sum |r|, non-finite] of r = rhs - A p,
+// one wavefront per row, grid-stride over rows.
+template <class T, bool P>
+__global__ void __launch_bounds__(256) residual_rows_kernel(const T* __restrict__ p,
+                                                            const T* __restrict__ rhs, Dim32 d,
+                                                            W3<T> k, double* __restrict__ partials) {
+    __shared__ double sh[4][4];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int row = blockIdx.x * 4 + wave; row < d.n0 * d.n1; row += gridDim.x * 4) {
+        const int x = row / d.n1, y = row - x * d.n1;
+        const Rows5<T, P> pr = rows5<T, P>(p, x, y, d);
+        const T* rr = rhs + x * d.n1 * d.n2 + y * d.n2;
+        for (int z = lane; z < d.n2; z += 64) {
+            const double r = double(rr[z] - apply_row<T, P>(pr, z, d.n2, k));
+            if (isfinite(r)) {
+                v[0] += r;
+                v[1] += r * r;
+                v[2] += fabs(r);
+            } else {
+                v[3] += 1.0;
+            }
+        }
+    }
+    for (int q = 0; q < 4; ++q) {
+        double t = v[q];
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1) t += __shfl_xor(t, dd, WAVE);
+        if (lane == 0) sh[q][wave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (int w = 0; w < 4; ++w) t += sh[threadIdx.x][w];
+        partials[4 * blockIdx.x + threadIdx.x] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// single-workgroup V-cycle over the coarse levels, all arrays in LDS
+
+template <class T>
+struct CoarseArgs {
+    int32_t levels;  // number of levels handled (>= 1)
+    int32_t smoother, nu;
+    T omega;
+    Dim32 d[AFSI_MG_MAX_LEVELS];
+    W3<T> k[AFSI_MG_MAX_LEVELS];
+    int32_t off_p[AFSI_MG_MAX_LEVELS], off_r[AFSI_MG_MAX_LEVELS], off_w[AFSI_MG_MAX_LEVELS];
+    const T* rhs_in;   // first handled level's rhs (global)
+    const T* p_in;     // its initial iterate, or null (zero)
+    T* p_out;          // mean-free result of the first handled level
+    const T* coarse_inverse;
+    int32_t coarse_cells;
+};
+
+__device__ inline double block_sum_all(double v, double* scratch) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < COARSE_THREADS / WAVE; ++w) t += scratch[w];
+    return t;
+}
+
+template <class T>
+__device__ inline void lds_remove_mean(T* a, int n, double* scratch) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += COARSE_THREADS) s += double(a[i]);
+    const T m = T(block_sum_all(s, scratch) / double(n));
+    for (int i = threadIdx.x; i < n; i += COARSE_THREADS) a[i] = a[i] - m;
+    __syncthreads();
+}
+
+template <class T, bool P>
+__device__ inline void lds_sums(const T* a, int i, int x, int y, int z, const Dim32& d,
+                                const W3<T>& k, T& s, T& dg) {
+    const int sx = d.n1 * d.n2, sy = d.n2;
+    if (P) {
+        s = k.w[0] * a[i + (x + 1 < d.n0 ? sx : -(d.n0 - 1) * sx)] +
+            k.w[0] * a[i + (x > 0 ? -sx : (d.n0 - 1) * sx)] +
+            k.w[1] * a[i + (y + 1 < d.n1 ? sy : -(d.n1 - 1) * sy)] +
+            k.w[1] * a[i + (y > 0 ? -sy : (d.n1 - 1) * sy)] +
+            k.w[2] * a[i + (z + 1 < d.n2 ? 1 : -(d.n2 - 1))] +
+            k.w[2] * a[i + (z > 0 ? -1 : (d.n2 - 1))];
+        dg = k.w[0] + k.w[0] + k.w[1] + k.w[1] + k.w[2] + k.w[2];
+    } else {
+        neighbour_sums<T, false>(a, i, sx, sy, x, y, z, d, k, s, dg);
+    }
+}
+
+__device__ inline void coords(int i, const Dim32& d, int& x, int& y, int& z) {
+    const int s = d.n1 * d.n2;
+    x = i / s;
+    const int r = i - x * s;
+    y = r / d.n2;
+    z = r - y * d.n2;
+}
+
+template <class T, bool P>
+__device__ inline void lds_smooth(T* p, T* w, const T* r, const Dim32& d, const W3<T>& k,
+                                  int smoother, int nu, T omega) {
+    const int n = d.n0 * d.n1 * d.n2;
+    for (int it = 0; it < nu; ++it) {
+        if (smoother == 1) {
+            for (int color = 0; color < 2; ++color) {
+                for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
+                    int x, y, z;
+                    coords(i, d, x, y, z);
+                    if (((x + y + z) & 1) != color) continue;
+                    T s, dg;
+                    lds_sums<T, P>(p, i, x, y, z, d, k, s, dg);
+                    p[i] = (r[i] + s) / dg;
+                }
+                __syncthreads();
+            }
+        } else {
+            for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
+                int x, y, z;
+                coords(i, d, x, y, z);
+                T s, dg;
+                lds_sums<T, P>(p, i, x, y, z, d, k, s, dg);
+                const T c = p[i];
+                const T a = apply_diff<T, P>(p, i, d.n1 * d.n2, d.n2, x, y, z, d, k);
+                w[i] = c + omega * (r[i] - a) / dg;
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < n; i += COARSE_THREADS) p[i] = w[i];
+            __syncthreads();
+        }
+    }
+}
+
+template <class T, bool P>
+__global__ void __launch_bounds__(COARSE_THREADS) coarse_cycle_kernel(CoarseArgs<T> a) {
+    extern __shared__ unsigned char smem_raw[];
+    T* smem = reinterpret_cast<T*>(smem_raw);
+    __shared__ double scratch[COARSE_THREADS / WAVE];
+    const int L = a.levels;
+    {
+        const Dim32& d = a.d[0];
+        const int n = d.n0 * d.n1 * d.n2;
+        T* p = smem + a.off_p[0];
+        T* r = smem + a.off_r[0];
+        for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
+            r[i] = a.rhs_in[i];
+            p[i] = a.p_in ? a.p_in[i] : T(0);
+        }
+        __syncthreads();
+    }
+    for (int l = 0; l < L - 1; ++l) {
+        const Dim32& d = a.d[l];
+        T* p = smem + a.off_p[l];
+        T* r = smem + a.off_r[l];
+        T* w = smem + a.off_w[l];
+        lds_smooth<T, P>(p, w, r, d, a.k[l], a.smoother, a.nu, a.omega);
+        lds_remove_mean(p, d.n0 * d.n1 * d.n2, scratch);
+        const Dim32& c = a.d[l + 1];
+        T* rc = smem + a.off_r[l + 1];
+        T* pc = smem + a.off_p[l + 1];
+        const int nc = c.n0 * c.n1 * c.n2;
+        for (int i = threadIdx.x; i < nc; i += COARSE_THREADS) {
+            int x, y, z;
+            coords(i, c, x, y, z);
+            T v = T(0);
+            for (int aa = 0; aa < 2; ++aa)
+                for (int b = 0; b < 2; ++b)
+                    for (int cc = 0; cc < 2; ++cc) {
+                        const int fx = 2 * x + aa, fy = 2 * y + b, fz = 2 * z + cc;
+                        const int j = (fx * d.n1 + fy) * d.n2 + fz;
+                        v += r[j] - apply_diff<T, P>(p, j, d.n1 * d.n2, d.n2, fx, fy, fz, d, a.k[l]);
+                    }
+            rc[i] = v * T(0.125);
+            pc[i] = T(0);
+        }
+        __syncthreads();
+        lds_remove_mean(rc, nc, scratch);
+    }
+    {
+        const Dim32& d = a.d[L - 1];
+        T* p = smem + a.off_p[L - 1];
+        const T* r = smem + a.off_r[L - 1];
+        const int m = a.coarse_cells;
+        // tpr threads per row (a power of two <= 64, inside one wave)
+        int tpr = 1;
+        while (tpr < 64 && tpr * 2 * m <= COARSE_THREADS) tpr *= 2;
+        const int lane = threadIdx.x % tpr;
+        for (int row0 = 0; row0 < m; row0 += COARSE_THREADS / tpr) {
+            const int row = row0 + threadIdx.x / tpr;
+            T s = T(0);
+            if (row < m)
+                for (int col = lane; col < m; col += tpr) s += a.coarse_inverse[row * m + col] * r[col];
+            for (int dd = tpr / 2; dd >= 1; dd >>= 1) s += __shfl_xor(s, dd, WAVE);
+            if (row < m && lane == 0) p[row] = s;
+        }
+        __syncthreads();
+        lds_remove_mean(p, d.n0 * d.n1 * d.n2, scratch);
+    }
+    for (int l = L - 2; l >= 0; --l) {
+        const Dim32& d = a.d[l];
+        T* p = smem + a.off_p[l];
+        T* r = smem + a.off_r[l];
+        T* w = smem + a.off_w[l];
+        const T* pc = smem + a.off_p[l + 1];
+        const int n = d.n0 * d.n1 * d.n2;
+        for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
+            int x, y, z;
+            coords(i, d, x, y, z);
+            p[i] = p[i] + prolong32<T, P>(pc, x, y, z, d.n0 / 2, d.n1 / 2, d.n2 / 2);
+        }
+        __syncthreads();
+        lds_smooth<T, P>(p, w, r, d, a.k[l], a.smoother, a.nu, a.omega);
+        lds_remove_mean(p, n, scratch);
+    }
+    {
+        const Dim32& d = a.d[0];
+        const int n = d.n0 * d.n1 * d.n2;
+        const T* p = smem + a.off_p[0];
+        for (int i = threadIdx.x; i < n; i += COARSE_THREADS) a.p_out[i] = p[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+
+template <class T>
+struct FastRunner {
+    const afsi_mg3d_plan& pl;
+    hipStream_t s;
+    int coarse_first;  // first level solved in LDS
+
+    T* P(int l) const { return static_cast<T*>(pl.p[l]); }
+    T* R(int l) const { return static_cast<T*>(pl.rhs[l]); }
+    T* W(int l) const { return static_cast<T*>(pl.tmp[l]); }
+    Dim32 D(int l) const { return Dim32{int(pl.n[l][0]), int(pl.n[l][1]), int(pl.n[l][2])}; }
+    int64_t cells(int l) const { return pl.n[l][0] * pl.n[l][1] * pl.n[l][2]; }
+    W3<T> K(int l) const {
+        W3<T> k;
+        for (int a = 0; a < 3; ++a) k.w[a] = T(pl.inv_h2[l][a]);
+        return k;
+    }
+
+    bool rows_ok(int l) const { return pl.n[l][2] <= ZROW && pl.n[l][2] % 2 == 0; }
+
+    template <int TXr, int TYr>
+    int rb_rows(int l, const T* in, T* out, bool zero_in) const {
+        const Dim32 d = D(l);
+        const int ntx = int(cdiv(d.n0, TXr)), nty = int(cdiv(d.n1, TYr));
+        const size_t bytes = size_t((TXr + 4) * (TYr + 4)) * d.n2 * sizeof(T);
+        static bool configured = false;
+        if (!configured) {
+            AFSI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(rb_rows_kernel<T, true, TXr, TYr>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+            AFSI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(rb_rows_kernel<T, false, TXr, TYr>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+            configured = true;
+        }
+        const unsigned g = unsigned(ntx * nty);
+        if (pl.periodic) rb_rows_kernel<T, true, TXr, TYr><<<g, 256, bytes, s>>>(in, R(l), out, d, K(l), zero_in, ntx, nty);
+        else rb_rows_kernel<T, false, TXr, TYr><<<g, 256, bytes, s>>>(in, R(l), out, d, K(l), zero_in, ntx, nty);
+        AFSI_LAUNCH_CHECK();
+        return OK;
+    }
+
+    int sweep(int l, const T* in, T* out, bool zero_in) const {
+        const Dim32 d = D(l);
+        const W3<T> k = K(l);
+        if (pl.smoother == 1 && rows_ok(l)) {
+            if (cdiv(d.n0, 4) * cdiv(d.n1, 4) >= 256) return rb_rows<4, 4>(l, in, out, zero_in);
+            return rb_rows<2, 2>(l, in, out, zero_in);
+        }
+        if (pl.smoother == 1) {
+            // 8x8x32 tiles on large levels, 4x4x32 where that leaves < 512 tiles
+            const bool big = cdiv(d.n0, 8) * cdiv(d.n1, 8) * cdiv(d.n2, 32) >= 512;
+            const int tx = big ? 8 : 4, ty = big ? 8 : 4, tz = 32;
+            const int ntx = int(cdiv(d.n0, tx)), nty = int(cdiv(d.n1, ty)), ntz = int(cdiv(d.n2, tz));
+            const unsigned g = unsigned(ntx * nty * ntz);
+            if (big) {
+                if (pl.periodic)
+                    rb_sweep_kernel<T, true, 8, 8, 32, 512><<<g, 512, 0, s>>>(in, R(l), out, d, k, zero_in, ntx, nty, ntz);
+                else
+                    rb_sweep_kernel<T, false, 8, 8, 32, 512><<<g, 512, 0, s>>>(in, R(l), out, d, k, zero_in, ntx, nty, ntz);
+            } else {
+                if (pl.periodic)
+                    rb_sweep_kernel<T, true, 4, 4, 32, 256><<<g, 256, 0, s>>>(in, R(l), out, d, k, zero_in, ntx, nty, ntz);
+                else
+                    rb_sweep_kernel<T, false, 4, 4, 32, 256><<<g, 256, 0, s>>>(in, R(l), out, d, k, zero_in, ntx, nty, ntz);
+            }
+        } else {
+            const unsigned g = grid_for(cells(l), 256, 16384);
+            if (pl.periodic)
+                jacobi_fast_kernel<T, true><<<g, 256, 0, s>>>(in, R(l), out, d, k, T(pl.omega), zero_in);
+            else
+                jacobi_fast_kernel<T, false><<<g, 256, 0, s>>>(in, R(l), out, d, k, T(pl.omega), zero_in);
+        }
+        AFSI_LAUNCH_CHECK();
+        return OK;
+    }
+
+    // nu sweeps ping-ponging between p and tmp; `cur` tracks the result.
+    int smooth(int l, T*& cur, bool zero_first) const {
+        for (int it = 0; it < pl.nu; ++it) {
+            T* out = (cur == P(l)) ? W(l) : P(l);
+            if (int r = sweep(l, cur, out, it == 0 && zero_first)) return r;
+            cur = out;
+        }
+        return OK;
+    }
+
+    int coarse(int first, const T* p_in, T* p_out) const {
+        CoarseArgs<T> a{};
+        a.levels = pl.levels - first;
+        a.smoother = pl.smoother;
+        a.nu = pl.nu;
+        a.omega = T(pl.omega);
+        int32_t off = 0;
+        for (int j = 0; j < a.levels; ++j) {
+            const int l = first + j;
+            a.d[j] = D(l);
+            a.k[j] = K(l);
+            const int32_t n = int32_t(cells(l));
+            a.off_p[j] = off; off += n;
+            a.off_r[j] = off; off += n;
+            a.off_w[j] = off; off += (pl.smoother == 1 ? 0 : n);
+        }
+        const size_t bytes = size_t(off) * sizeof(T);
+        AFSI_REQUIRE(bytes <= 150 * 1024, "coarse levels exceed LDS");
+        a.rhs_in = R(first);
+        a.p_in = p_in;
+        a.p_out = p_out;
+        a.coarse_inverse = static_cast<const T*>(pl.coarse_inverse);
+        a.coarse_cells = int32_t(pl.coarse_cells);
+        static bool configured = false;
+        if (!configured) {
+            AFSI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(coarse_cycle_kernel<T, false>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+            AFSI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(coarse_cycle_kernel<T, true>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+            configured = true;
+        }
+        if (pl.periodic) coarse_cycle_kernel<T, true><<<1, COARSE_THREADS, bytes, s>>>(a);
+        else coarse_cycle_kernel<T, false><<<1, COARSE_THREADS, bytes, s>>>(a);
+        AFSI_LAUNCH_CHECK();
+        return OK;
+    }
+
+    int cycle(int l, T*& cur) const {
+        if (l == coarse_first) return coarse(l, l == 0 ? cur : nullptr, cur);
+        if (int r = smooth(l, cur, l > 0)) return r;  // coarse iterates start at zero
+        const Dim32 d = D(l);
+        const W3<T> k = K(l);
+        if (rows_ok(l)) {
+            const unsigned g = unsigned(cdiv(int64_t(d.n0 / 2) * (d.n1 / 2), 4));
+            if (pl.periodic) restrict_rows_kernel<T, true><<<g, 256, 0, s>>>(cur, R(l), R(l + 1), d, k);
+            else restrict_rows_kernel<T, false><<<g, 256, 0, s>>>(cur, R(l), R(l + 1), d, k);
+        } else {
+            const unsigned g = grid_for(cells(l + 1), 256, 16384);
+            if (pl.periodic) restrict_fast_kernel<T, true><<<g, 256, 0, s>>>(cur, R(l), R(l + 1), d, k);
+            else restrict_fast_kernel<T, false><<<g, 256, 0, s>>>(cur, R(l), R(l + 1), d, k);
+        }
+        AFSI_LAUNCH_CHECK();
+        T* cc = P(l + 1);
+        if (int r = cycle(l + 1, cc)) return r;
+        if (rows_ok(l)) {
+            const unsigned gf = unsigned(cdiv(int64_t(d.n0) * d.n1, 4));
+            if (pl.periodic) prolong_rows_kernel<T, true><<<gf, 256, 0, s>>>(cc, cur, d);
+            else prolong_rows_kernel<T, false><<<gf, 256, 0, s>>>(cc, cur, d);
+        } else {
+            const unsigned gf = grid_for(cells(l), 256, 16384);
+            if (pl.periodic) prolong_fast_kernel<T, true><<<gf, 256, 0, s>>>(cc, cur, d);
+            else prolong_fast_kernel<T, false><<<gf, 256, 0, s>>>(cc, cur, d);
+        }
+        AFSI_LAUNCH_CHECK();
+        return smooth(l, cur, false);
+    }
+};
+
+static int choose_coarse(const afsi_mg3d_plan& pl) {
+    for (int l = 0; l < pl.levels; ++l) {
+        const int64_t c = pl.n[l][0] * pl.n[l][1] * pl.n[l][2];
+        if (c <= COARSE_MAX) return l;
+    }
+    return pl.levels - 1;
+}
+
+template <class T>
+static int fast_check(const afsi_mg3d_plan* pl) {
+    AFSI_REQUIRE(pl && pl->levels >= 1 && pl->levels <= AFSI_MG_MAX_LEVELS, "invalid multigrid plan");
+    AFSI_REQUIRE(pl->nu >= 1 && (pl->smoother == 0 || pl->smoother == 1), "invalid smoother");
+    AFSI_REQUIRE(pl->reduce_partials && pl->reduce_result, "missing reduction scratch");
+    for (int l = 0; l < pl->levels; ++l) {
+        AFSI_REQUIRE(pl->n[l][0] > 0 && pl->n[l][1] > 0 && pl->n[l][2] > 0 && pl->p[l] && pl->rhs[l],
+                     "invalid level %d", l);
+        AFSI_REQUIRE(pl->n[l][0] * pl->n[l][1] * pl->n[l][2] < (int64_t(1) << 31),
+                     "level %d exceeds 2^31 cells", l);
+        AFSI_REQUIRE(pl->tmp[l], "missing smoothing buffer at level %d", l);
+        if (l + 1 < pl->levels)
+            for (int a = 0; a < 3; ++a)
+                AFSI_REQUIRE(pl->n[l][a] == 2 * pl->n[l + 1][a], "levels %d/%d are not 2:1", l, l + 1);
+    }
+    const int last = pl->levels - 1;
+    AFSI_REQUIRE(pl->coarse_inverse &&
+                     pl->coarse_cells == pl->n[last][0] * pl->n[last][1] * pl->n[last][2],
+                 "coarse inverse does not match the coarsest level");
+    AFSI_REQUIRE(pl->coarse_cells <= 512, "coarsest level exceeds 512 cells");
+    return OK;
+}
+
+template <class T>
+static int fast_cycles(const afsi_mg3d_plan* pl, int32_t cycles, void* stream) {
+    if (int r = fast_check<T>(pl)) return r;
+    AFSI_REQUIRE(cycles >= 0, "negative cycle count");
+    const FastRunner<T> run{*pl, hipStream_t(stream), choose_coarse(*pl)};
+    for (int c = 0; c < cycles; ++c) {
+        T* cur = run.P(0);
+        if (int r = run.cycle(0, cur)) return r;
+        AFSI_REQUIRE(cur == run.P(0), "level-0 iterate left the canonical buffer");
+    }
+    return OK;
+}
+
+template <class T>
+static int fast_residual(const afsi_mg3d_plan* pl, void* stream) {
+    if (int r = fast_check<T>(pl)) return r;
+    const FastRunner<T> run{*pl, hipStream_t(stream), choose_coarse(*pl)};
+    const hipStream_t s = hipStream_t(stream);
+    const int64_t nb = grid_for(run.cells(0), 256, pl->partials_capacity / 4);
+    AFSI_REQUIRE(4 * nb <= pl->partials_capacity, "reduction scratch too small");
+    if (run.rows_ok(0)) {
+        const int64_t rows = pl->n[0][0] * pl->n[0][1];
+        int64_t nr = cdiv(rows, 4);
+        if (nr > pl->partials_capacity / 4) nr = pl->partials_capacity / 4;
+        if (pl->periodic)
+            residual_rows_kernel<T, true><<<unsigned(nr), 256, 0, s>>>(run.P(0), run.R(0), run.D(0), run.K(0),
+                                                                      pl->reduce_partials);
+        else
+            residual_rows_kernel<T, false><<<unsigned(nr), 256, 0, s>>>(run.P(0), run.R(0), run.D(0), run.K(0),
+                                                                       pl->reduce_partials);
+        AFSI_LAUNCH_CHECK();
+        return launch_reduce_final(pl->reduce_partials, nr, pl->reduce_result, s);
+    }
+    if (pl->periodic)
+        residual_partials_kernel<T, true><<<unsigned(nb), 256, 0, s>>>(run.P(0), run.R(0), run.D(0), run.K(0),
+                                                                      pl->reduce_partials);
+    else
+        residual_partials_kernel<T, false><<<unsigned(nb), 256, 0, s>>>(run.P(0), run.R(0), run.D(0), run.K(0),
+                                                                       pl->reduce_partials);
+    AFSI_LAUNCH_CHECK();
+    return launch_reduce_final(pl->reduce_partials, nb, pl->reduce_result, s);
+}
+
+}  // namespace afsi
+
+extern "C" {
+int64_t afsi_mg3d_partials_doubles(const int64_t n[3]) {
+    if (!n || n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return -1;
+    int64_t blocks = afsi::cdiv(n[0] * n[1] * n[2], 256);
+    if (blocks > 4096) blocks = 4096;
+    return 4 * blocks;
+}
+int afsi_mg3d_cycles_f64(const afsi_mg3d_plan* plan, int32_t cycles, void* stream) {
+    return afsi::fast_cycles<double>(plan, cycles, stream);
+}
+int afsi_mg3d_cycles_f32(const afsi_mg3d_plan* plan, int32_t cycles, void* stream) {
+    return afsi::fast_cycles<float>(plan, cycles, stream);
+}
+int afsi_mg3d_residual_f64(const afsi_mg3d_plan* plan, void* stream) {
+    return afsi::fast_residual<double>(plan, stream);
+}
+int afsi_mg3d_residual_f32(const afsi_mg3d_plan* plan, void* stream) {
+    return afsi::fast_residual<float>(plan, stream);
+}
+}  // extern "C"

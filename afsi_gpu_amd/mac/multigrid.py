@@ -82,8 +82,8 @@ def coarse_operator(shape, spacing, periodic):
 
 class GeometricMultigrid:
     def __init__(self, grid, *, device='cuda', dtype=torch.float64, options=None, backend='graph'):
-        if backend not in ('torch', 'fused', 'workspace', 'graph'):
-            raise ValueError('pressure backend must be torch, fused, workspace or graph')
+        if backend not in ('torch', 'fused', 'workspace', 'graph', 'unfused'):
+            raise ValueError('pressure backend must be torch, fused, workspace, graph or unfused')
         if dtype not in (torch.float64, torch.float32):
             raise ValueError('multigrid supports float64/float32')
         _build()
@@ -95,7 +95,8 @@ class GeometricMultigrid:
         self.periodic = bool(getattr(grid, 'periodic', False))
         self.shapes, self.spacings = hierarchy(grid.shape, grid.spacing)
         self.use_graphs = backend == 'graph'
-        self.backend = 'hip-graph' if self.use_graphs else 'hip'
+        self.fused = backend != 'unfused'
+        self.backend = ('hip-graph' if self.use_graphs else 'hip') if self.fused else 'hip-unfused'
         self.suffix = 'f64' if dtype == torch.float64 else 'f32'
         kw = dict(device=self.device, dtype=dtype)
         self.p = [torch.zeros(s, **kw) for s in self.shapes]
@@ -106,8 +107,10 @@ class GeometricMultigrid:
         # mac/multigrid.py:49-59: the rank-one term fixes the constant nullspace.
         self.coarse_inverse = torch.tensor(np.linalg.inv(A + np.ones_like(A) / count), **kw)
         cells0 = prod(self.shapes[0])
-        nparts = int(_abi.load().afsi_reduce_partials(cells0))
-        self.partials = torch.zeros(4 * nparts, device=self.device, dtype=torch.float64)
+        lib = _abi.load()
+        capacity = max(4 * int(lib.afsi_reduce_partials(cells0)),
+                       int(lib.afsi_mg3d_partials_doubles(_abi.i64x3(self.shapes[0]))))
+        self.partials = torch.zeros(capacity, device=self.device, dtype=torch.float64)
         self.results = torch.zeros((8, 4), device=self.device, dtype=torch.float64)
         self.plan = self._make_plan()
         self.allocated_bytes = sum(t.numel() * t.element_size()
@@ -137,14 +140,23 @@ class GeometricMultigrid:
         pl.coarse_cells = prod(self.shapes[-1])
         pl.reduce_partials = self.partials.data_ptr()
         pl.reduce_result = self.results[0].data_ptr()
+        pl.partials_capacity = self.partials.numel()
         return pl
 
     # -- native calls on the current stream ---------------------------------
     def _cycles(self, count):
-        _abi.call(f'afsi_mg3d_cycles_{self.suffix}', self.plan, int(count), _abi.stream())
+        kind = '' if self.fused else 'unfused_'
+        _abi.call(f'afsi_mg3d_cycles_{kind}{self.suffix}', self.plan, int(count), _abi.stream())
 
     def _residual(self):
-        _abi.call(f'afsi_mg3d_residual_{self.suffix}', self.plan, _abi.stream())
+        kind = '' if self.fused else 'unfused_'
+        _abi.call(f'afsi_mg3d_residual_{kind}{self.suffix}', self.plan, _abi.stream())
+
+    def _materialize(self):
+        """Remove the constant offset the fused driver leaves in p[0]."""
+        if self.fused:
+            self._reduce(self.p[0], 4)
+            self._sub_mean(self.p[0], 4)
 
     def _reduce(self, x, slot):
         _abi.call(f'afsi_reduce_{self.suffix}', _abi.ptr(x), x.numel(), _abi.ptr(self.partials),
@@ -186,6 +198,7 @@ class GeometricMultigrid:
     def cycle(self):
         """One V-cycle on the workspace iterate (MGWorkspace.cycle)."""
         self._cycles(1)
+        self._materialize()
         return self.p[0]
 
     def initialize(self, p, rhs):
@@ -250,5 +263,6 @@ class GeometricMultigrid:
                 raise RuntimeError('nonfinite multigrid residual')
         if residual > tolerance:
             raise RuntimeError(f'pressure multigrid did not converge: {residual:g} > {tolerance:g}')
+        self._materialize()
         return w_p.clone(), dict(cycles=cycle, residual_norm=residual, tolerance=tolerance,
                                  backend=self.backend)

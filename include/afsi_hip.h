@@ -210,13 +210,33 @@ typedef struct afsi_mg3d_plan {
     int64_t coarse_cells;
     double* reduce_partials;
     double* reduce_result;
+    int64_t partials_capacity; /* doubles in reduce_partials */
 } afsi_mg3d_plan;
+/* Doubles of reduce_partials the fused driver needs for level-0 extent n. */
+int64_t afsi_mg3d_partials_doubles(const int64_t n[3]);
+/* Fused driver (the production path). Per fine level: nu single-launch
+ * sweeps (red-black: LDS tile + 2-cell halo, red on tile+1 ring then
+ * black; or weighted Jacobi), one residual-restriction, one prolongation.
+ * All levels with <= 4096 cells run as one 1024-thread workgroup out of
+ * LDS (including the reference's mean removals and the dense coarse solve).
+ * The fine levels skip the reference's intermediate mean subtractions
+ * (mac/multigrid.py:107-123): the V-cycle commutes with adding a constant,
+ * so they change only rounding; p[0] is returned with an arbitrary
+ * constant offset that the caller removes once after the last cycle. */
 int afsi_mg3d_cycles_f64(const afsi_mg3d_plan* plan, int32_t cycles, void* stream);
 int afsi_mg3d_cycles_f32(const afsi_mg3d_plan* plan, int32_t cycles, void* stream);
-/* tmp[0] = rhs[0] - A p[0]; reduce_result = its reduction (sum, sum of
- * squares, sum |.|, non-finite count). */
+/* reduce_result = [sum, sum of squares, sum |.|, non-finite count] of
+ * rhs[0] - A p[0] (two launches, no grid-wide synchronisation). */
 int afsi_mg3d_residual_f64(const afsi_mg3d_plan* plan, void* stream);
 int afsi_mg3d_residual_f32(const afsi_mg3d_plan* plan, void* stream);
+/* Unfused driver with the reference workspace's launch structure (one
+ * launch per colour sweep, separate reductions and mean subtractions,
+ * mac/mg_workspace.py:246-305); kept as a cross-check of the fused one.
+ * tmp[0] receives the residual in the _residual_ variant. */
+int afsi_mg3d_cycles_unfused_f64(const afsi_mg3d_plan* plan, int32_t cycles, void* stream);
+int afsi_mg3d_cycles_unfused_f32(const afsi_mg3d_plan* plan, int32_t cycles, void* stream);
+int afsi_mg3d_residual_unfused_f64(const afsi_mg3d_plan* plan, void* stream);
+int afsi_mg3d_residual_unfused_f32(const afsi_mg3d_plan* plan, void* stream);
 
 #ifdef __cplusplus
 }

@@ -102,10 +102,13 @@ def manufactured(shape, lengths):
     return h, exact
 
 
+@pytest.mark.parametrize('backend', ['workspace', 'unfused'])
 @pytest.mark.parametrize('smoother', ['jacobi', 'rbgs'])
 @pytest.mark.parametrize('periodic', [False, True])
-@pytest.mark.parametrize('shape', [(4, 4, 4), (8, 12, 16), (16, 16, 16)])
-def test_v_cycle_matches_oracle(gpu, smoother, periodic, shape):
+@pytest.mark.parametrize('shape', [(4, 4, 4), (8, 12, 16), (16, 16, 16), (32, 32, 32), (64, 32, 48)])
+def test_v_cycle_matches_oracle(gpu, backend, smoother, periodic, shape):
+    # (16,16,16) runs entirely in the single-workgroup LDS kernel; (32,32,32)
+    # and (64,32,48) add fine levels (partial z tiles for 48 and 24).
     lengths = (2., 3., 4.)
     h, exact = manufactured(shape, lengths)
     rhs = mac3d.negative_laplacian(exact, h, periodic)
@@ -115,17 +118,17 @@ def test_v_cycle_matches_oracle(gpu, smoother, periodic, shape):
     ref = mac3d.Multigrid(shape, h, mac3d.MGOptions(smooth=3, smoother=smoother), periodic)
     expected = ref.cycle(0, start - start.mean(), rhs)
     mg = GeometricMultigrid(MACGrid(shape, lengths, periodic=periodic), device=gpu, options=opts,
-                            backend='workspace')
+                            backend=backend)
     mg.initialize(torch.tensor(start - start.mean(), device=gpu), torch.tensor(rhs, device=gpu))
     got = mg.cycle().cpu().numpy()
     np.testing.assert_allclose(got, expected, rtol=2e-11, atol=2e-12 * np.abs(expected).max())
 
 
-@pytest.mark.parametrize('backend', ['workspace', 'graph'])
+@pytest.mark.parametrize('backend', ['workspace', 'graph', 'unfused'])
 @pytest.mark.parametrize('smoother', ['jacobi', 'rbgs'])
 @pytest.mark.parametrize('periodic', [False, True])
 def test_solve_residual_history_matches_oracle(gpu, backend, smoother, periodic):
-    shape, lengths = (32, 32, 32), (2., 3., 4.)
+    shape, lengths = (64, 32, 48), (2., 3., 4.)
     h, exact = manufactured(shape, lengths)
     rhs = mac3d.negative_laplacian(exact, h, periodic)
     options = MGOptions(smooth=2, smoother=smoother, check_every=1)
