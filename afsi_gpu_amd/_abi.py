@@ -33,7 +33,8 @@ class MGPlan3(Structure):
                 ('p', c_void_p * MAX_LEVELS), ('rhs', c_void_p * MAX_LEVELS),
                 ('tmp', c_void_p * MAX_LEVELS), ('coarse_inverse', c_void_p),
                 ('coarse_cells', c_int64), ('reduce_partials', c_void_p),
-                ('reduce_result', c_void_p), ('partials_capacity', c_int64)]
+                ('reduce_result', c_void_p), ('partials_capacity', c_int64),
+                ('zero_initial', c_int32), ('reserved', c_int32)]
 
 
 _P = c_void_p
@@ -80,6 +81,11 @@ _SIGNATURES = {
     'afsi_grad_correct_3d_f64': (c_int, [_P, _P, _P, _P, POINTER(c_int64), POINTER(c_double), c_double, c_int, _P]),
     'afsi_axpy_f64': (c_int, [_P, _P, c_int64, c_double, _P]),
     'afsi_mg3d_partials_doubles': (c_int64, [POINTER(c_int64)]),
+    'afsi_mg3d_supported': (c_int, [POINTER(MGPlan3)]),
+    'afsi_mg3d_sweep_f64': (c_int, [POINTER(MGPlan3), c_int32, _P]),
+    'afsi_mg3d_sweep_f32': (c_int, [POINTER(MGPlan3), c_int32, _P]),
+    'afsi_refine_residual_3d': (c_int, [_P, _P, _P, POINTER(c_int64), POINTER(c_double), c_int, _P, c_int64, _P, _P]),
+    'afsi_refine_update': (c_int, [_P, _P, c_int64, _P]),
     'afsi_mg3d_cycles_f64': (c_int, [POINTER(MGPlan3), c_int32, _P]),
     'afsi_mg3d_cycles_f32': (c_int, [POINTER(MGPlan3), c_int32, _P]),
     'afsi_mg3d_residual_f64': (c_int, [POINTER(MGPlan3), _P]),

@@ -23,6 +23,7 @@
 //   round-robin to the 8 XCDs, so each XCD is given one contiguous run of
 //   tiles and the halos it re-reads stay in its own L2.
 #include "mgcommon.hpp"
+#include <mutex>
 
 namespace afsi {
 
@@ -32,6 +33,7 @@ int launch_reduce_final(const double* partials, int64_t nparts, double* result, 
 
 constexpr int COARSE_THREADS = 1024;
 constexpr int64_t COARSE_MAX = 4096;  // cells of the finest level solved in LDS
+constexpr int COARSE_LIST = 4800;     // packed cell descriptors of all LDS levels
 
 struct Dim32 {
     int n0, n1, n2;
@@ -661,7 +663,62 @@ __global__ void __launch_bounds__(256) residual_rows_kernel(const T* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// single-workgroup V-cycle over the coarse levels, all arrays in LDS
+// FP64 iterative refinement around FP32 V-cycles: r = b - A x in FP64 with
+// the FP32 copy of r written for the inner solve and [sum, sum r^2,
+// sum |r|, non-finite] partials per workgroup; and x += e.
+
+template <bool P>
+__global__ void __launch_bounds__(256) refine_residual_kernel(const double* __restrict__ x,
+                                                              const double* __restrict__ b,
+                                                              float* __restrict__ r32, Dim32 d,
+                                                              W3<double> k,
+                                                              double* __restrict__ partials) {
+    __shared__ double sh[4][4];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int row = blockIdx.x * 4 + wave; row < d.n0 * d.n1; row += gridDim.x * 4) {
+        const int xx = row / d.n1, yy = row - xx * d.n1;
+        const Rows5<double, P> pr = rows5<double, P>(x, xx, yy, d);
+        const int off = row * d.n2;
+        for (int z = lane; z < d.n2; z += 64) {
+            const double r = b[off + z] - apply_row<double, P>(pr, z, d.n2, k);
+            r32[off + z] = float(r);
+            if (isfinite(r)) {
+                v[0] += r;
+                v[1] += r * r;
+                v[2] += fabs(r);
+            } else {
+                v[3] += 1.0;
+            }
+        }
+    }
+    for (int q = 0; q < 4; ++q) {
+        double t = v[q];
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1) t += __shfl_xor(t, dd, WAVE);
+        if (lane == 0) sh[q][wave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (int w = 0; w < 4; ++w) t += sh[threadIdx.x][w];
+        partials[4 * blockIdx.x + threadIdx.x] = t;
+    }
+}
+
+__global__ void refine_update_kernel(double* __restrict__ x, const float* __restrict__ e,
+                                     int64_t count) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+         i += int64_t(gridDim.x) * blockDim.x)
+        x[i] = x[i] + double(e[i]);
+}
+
+// ---------------------------------------------------------------------------
+// single-workgroup V-cycle over the coarse levels, all arrays in LDS.
+// Work items are enumerated per colour (lanes map to the red or the black
+// cells of rows directly) and decoded with float-reciprocal division plus
+// an exact integer correction; 1/diag comes from a per-level table.
 
 template <class T>
 struct CoarseArgs {
@@ -678,6 +735,14 @@ struct CoarseArgs {
     int32_t coarse_cells;
 };
 
+// floor(e / d) for 0 <= e < 2^22, d >= 1
+__device__ inline int qdiv(int e, int d) {
+    int q = int((float(e) + 0.5f) / float(d));
+    if (q * d > e) --q;
+    else if ((q + 1) * d <= e) ++q;
+    return q;
+}
+
 __device__ inline double block_sum_all(double v, double* scratch) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
@@ -686,6 +751,7 @@ __device__ inline double block_sum_all(double v, double* scratch) {
     if (lane == 0) scratch[wave] = v;
     __syncthreads();
     double t = 0.0;
+#pragma unroll
     for (int w = 0; w < COARSE_THREADS / WAVE; ++w) t += scratch[w];
     return t;
 }
@@ -699,57 +765,103 @@ __device__ inline void lds_remove_mean(T* a, int n, double* scratch) {
     __syncthreads();
 }
 
+// sum_nb w_a p_nb (and, for Neumann, the boundary class index) at (x,y,z).
 template <class T, bool P>
-__device__ inline void lds_sums(const T* a, int i, int x, int y, int z, const Dim32& d,
-                                const W3<T>& k, T& s, T& dg) {
+__device__ inline T lds_nsum(const T* a, int i, int x, int y, int z, const Dim32& d,
+                             const W3<T>& k) {
     const int sx = d.n1 * d.n2, sy = d.n2;
+    T s = T(0);
     if (P) {
-        s = k.w[0] * a[i + (x + 1 < d.n0 ? sx : -(d.n0 - 1) * sx)] +
-            k.w[0] * a[i + (x > 0 ? -sx : (d.n0 - 1) * sx)] +
-            k.w[1] * a[i + (y + 1 < d.n1 ? sy : -(d.n1 - 1) * sy)] +
-            k.w[1] * a[i + (y > 0 ? -sy : (d.n1 - 1) * sy)] +
-            k.w[2] * a[i + (z + 1 < d.n2 ? 1 : -(d.n2 - 1))] +
-            k.w[2] * a[i + (z > 0 ? -1 : (d.n2 - 1))];
-        dg = k.w[0] + k.w[0] + k.w[1] + k.w[1] + k.w[2] + k.w[2];
+        s += k.w[0] * a[i + (x + 1 < d.n0 ? sx : -(d.n0 - 1) * sx)];
+        s += k.w[0] * a[i + (x > 0 ? -sx : (d.n0 - 1) * sx)];
+        s += k.w[1] * a[i + (y + 1 < d.n1 ? sy : -(d.n1 - 1) * sy)];
+        s += k.w[1] * a[i + (y > 0 ? -sy : (d.n1 - 1) * sy)];
+        s += k.w[2] * a[i + (z + 1 < d.n2 ? 1 : -(d.n2 - 1))];
+        s += k.w[2] * a[i + (z > 0 ? -1 : (d.n2 - 1))];
     } else {
-        neighbour_sums<T, false>(a, i, sx, sy, x, y, z, d, k, s, dg);
+        if (x + 1 < d.n0) s += k.w[0] * a[i + sx];
+        if (x > 0) s += k.w[0] * a[i - sx];
+        if (y + 1 < d.n1) s += k.w[1] * a[i + sy];
+        if (y > 0) s += k.w[1] * a[i - sy];
+        if (z + 1 < d.n2) s += k.w[2] * a[i + 1];
+        if (z > 0) s += k.w[2] * a[i - 1];
+    }
+    return s;
+}
+
+template <class T, bool P>
+__device__ inline T lds_inv(const T* tab, T inv_p, int x, int y, int z, const Dim32& d) {
+    if (P) return inv_p;
+    return tab[face_class(x, d.n0) * 9 + face_class(y, d.n1) * 3 + face_class(z, d.n2)];
+}
+
+// Packed cell descriptor: x (bits 24-29), y (18-23), z (12-17), index (0-11).
+__device__ inline uint32_t pack_cell(int x, int y, int z, int i) {
+    return (uint32_t(x) << 24) | (uint32_t(y) << 18) | (uint32_t(z) << 12) | uint32_t(i);
+}
+__device__ inline void unpack_cell(uint32_t c, int& x, int& y, int& z, int& i) {
+    x = int(c >> 24);
+    y = int((c >> 18) & 63);
+    z = int((c >> 12) & 63);
+    i = int(c & 4095);
+}
+
+// Lists: for a level with even n2, cells of colour c (x+y+z even: red) at
+// list[c*n/2 + row*n2/2 + z/2]; otherwise all cells in index order.
+__device__ inline void build_list(uint32_t* list, const Dim32& d) {
+    const int n = d.n0 * d.n1 * d.n2;
+    const bool split = (d.n2 & 1) == 0;
+    for (int e = threadIdx.x; e < n; e += COARSE_THREADS) {
+        const int row = qdiv(e, d.n2), z = e - row * d.n2;
+        const int x = qdiv(row, d.n1), y = row - x * d.n1;
+        const uint32_t c = pack_cell(x, y, z, e);
+        if (split) list[((x + y + z) & 1) * (n >> 1) + row * (d.n2 >> 1) + (z >> 1)] = c;
+        else list[e] = c;
     }
 }
 
-__device__ inline void coords(int i, const Dim32& d, int& x, int& y, int& z) {
-    const int s = d.n1 * d.n2;
-    x = i / s;
-    const int r = i - x * s;
-    y = r / d.n2;
-    z = r - y * d.n2;
-}
-
 template <class T, bool P>
-__device__ inline void lds_smooth(T* p, T* w, const T* r, const Dim32& d, const W3<T>& k,
-                                  int smoother, int nu, T omega) {
+__device__ inline void lds_smooth(T* p, T* w, const T* r, const uint32_t* list, const Dim32& d,
+                                  const W3<T>& k, const T* tab, T inv_p, int smoother, int nu,
+                                  T omega) {
     const int n = d.n0 * d.n1 * d.n2;
-    for (int it = 0; it < nu; ++it) {
-        if (smoother == 1) {
+    const int sx = d.n1 * d.n2, sy = d.n2;
+    if (smoother == 1) {
+        const int half = n >> 1;
+        for (int it = 0; it < nu; ++it) {
             for (int color = 0; color < 2; ++color) {
-                for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
-                    int x, y, z;
-                    coords(i, d, x, y, z);
-                    if (((x + y + z) & 1) != color) continue;
-                    T s, dg;
-                    lds_sums<T, P>(p, i, x, y, z, d, k, s, dg);
-                    p[i] = (r[i] + s) / dg;
+                for (int e = threadIdx.x; e < half; e += COARSE_THREADS) {
+                    int x, y, z, i;
+                    unpack_cell(list[color * half + e], x, y, z, i);
+                    T s;
+                    if (P) {
+                        s = k.w[0] * p[i + (x + 1 < d.n0 ? sx : -(d.n0 - 1) * sx)];
+                        s += k.w[0] * p[i + (x > 0 ? -sx : (d.n0 - 1) * sx)];
+                        s += k.w[1] * p[i + (y + 1 < d.n1 ? sy : -(d.n1 - 1) * sy)];
+                        s += k.w[1] * p[i + (y > 0 ? -sy : (d.n1 - 1) * sy)];
+                        s += k.w[2] * p[i + (z + 1 < d.n2 ? 1 : -(d.n2 - 1))];
+                        s += k.w[2] * p[i + (z > 0 ? -1 : (d.n2 - 1))];
+                    } else {
+                        s = T(0);
+                        if (x + 1 < d.n0) s += k.w[0] * p[i + sx];
+                        if (x > 0) s += k.w[0] * p[i - sx];
+                        if (y + 1 < d.n1) s += k.w[1] * p[i + sy];
+                        if (y > 0) s += k.w[1] * p[i - sy];
+                        if (z + 1 < d.n2) s += k.w[2] * p[i + 1];
+                        if (z > 0) s += k.w[2] * p[i - 1];
+                    }
+                    p[i] = (r[i] + s) * lds_inv<T, P>(tab, inv_p, x, y, z, d);
                 }
                 __syncthreads();
             }
-        } else {
-            for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
-                int x, y, z;
-                coords(i, d, x, y, z);
-                T s, dg;
-                lds_sums<T, P>(p, i, x, y, z, d, k, s, dg);
-                const T c = p[i];
-                const T a = apply_diff<T, P>(p, i, d.n1 * d.n2, d.n2, x, y, z, d, k);
-                w[i] = c + omega * (r[i] - a) / dg;
+        }
+    } else {
+        for (int it = 0; it < nu; ++it) {
+            for (int e = threadIdx.x; e < n; e += COARSE_THREADS) {
+                int x, y, z, i;
+                unpack_cell(list[e], x, y, z, i);
+                const T a = apply_diff<T, P>(p, i, sx, sy, x, y, z, d, k);
+                w[i] = p[i] + omega * (r[i] - a) * lds_inv<T, P>(tab, inv_p, x, y, z, d);
             }
             __syncthreads();
             for (int i = threadIdx.x; i < n; i += COARSE_THREADS) p[i] = w[i];
@@ -763,7 +875,23 @@ __global__ void __launch_bounds__(COARSE_THREADS) coarse_cycle_kernel(CoarseArgs
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
     __shared__ double scratch[COARSE_THREADS / WAVE];
+    __shared__ T tabs[AFSI_MG_MAX_LEVELS][27];
+    __shared__ uint32_t lists[COARSE_LIST];
+    __shared__ int list_off[AFSI_MG_MAX_LEVELS];
     const int L = a.levels;
+    if (threadIdx.x == 0) {
+        int off = 0;
+        for (int l = 0; l < L; ++l) {
+            list_off[l] = off;
+            off += a.d[l].n0 * a.d[l].n1 * a.d[l].n2;
+        }
+    }
+    if (!P && threadIdx.x < 27 * L) {
+        const int l = threadIdx.x / 27;
+        tabs[l][threadIdx.x - 27 * l] = inv_diag_entry<T>(threadIdx.x - 27 * l, a.k[l]);
+    }
+    __syncthreads();
+    for (int l = 0; l < L; ++l) build_list(lists + list_off[l], a.d[l]);
     {
         const Dim32& d = a.d[0];
         const int n = d.n0 * d.n1 * d.n2;
@@ -777,25 +905,29 @@ __global__ void __launch_bounds__(COARSE_THREADS) coarse_cycle_kernel(CoarseArgs
     }
     for (int l = 0; l < L - 1; ++l) {
         const Dim32& d = a.d[l];
+        const W3<T>& k = a.k[l];
         T* p = smem + a.off_p[l];
         T* r = smem + a.off_r[l];
         T* w = smem + a.off_w[l];
-        lds_smooth<T, P>(p, w, r, d, a.k[l], a.smoother, a.nu, a.omega);
+        const T inv_p = T(1) / (T(2) * (k.w[0] + k.w[1] + k.w[2]));
+        lds_smooth<T, P>(p, w, r, lists + list_off[l], d, k, tabs[l], inv_p, a.smoother, a.nu, a.omega);
         lds_remove_mean(p, d.n0 * d.n1 * d.n2, scratch);
         const Dim32& c = a.d[l + 1];
         T* rc = smem + a.off_r[l + 1];
         T* pc = smem + a.off_p[l + 1];
+        const uint32_t* clist = lists + list_off[l + 1];
         const int nc = c.n0 * c.n1 * c.n2;
-        for (int i = threadIdx.x; i < nc; i += COARSE_THREADS) {
-            int x, y, z;
-            coords(i, c, x, y, z);
+        const int sx = d.n1 * d.n2, sy = d.n2;
+        for (int e = threadIdx.x; e < nc; e += COARSE_THREADS) {
+            int x, y, z, i;
+            unpack_cell(clist[e], x, y, z, i);
             T v = T(0);
             for (int aa = 0; aa < 2; ++aa)
                 for (int b = 0; b < 2; ++b)
                     for (int cc = 0; cc < 2; ++cc) {
                         const int fx = 2 * x + aa, fy = 2 * y + b, fz = 2 * z + cc;
-                        const int j = (fx * d.n1 + fy) * d.n2 + fz;
-                        v += r[j] - apply_diff<T, P>(p, j, d.n1 * d.n2, d.n2, fx, fy, fz, d, a.k[l]);
+                        const int j = fx * sx + fy * sy + fz;
+                        v += r[j] - apply_diff<T, P>(p, j, sx, sy, fx, fy, fz, d, k);
                     }
             rc[i] = v * T(0.125);
             pc[i] = T(0);
@@ -825,18 +957,21 @@ __global__ void __launch_bounds__(COARSE_THREADS) coarse_cycle_kernel(CoarseArgs
     }
     for (int l = L - 2; l >= 0; --l) {
         const Dim32& d = a.d[l];
+        const W3<T>& k = a.k[l];
         T* p = smem + a.off_p[l];
         T* r = smem + a.off_r[l];
         T* w = smem + a.off_w[l];
         const T* pc = smem + a.off_p[l + 1];
+        const uint32_t* list = lists + list_off[l];
         const int n = d.n0 * d.n1 * d.n2;
-        for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
-            int x, y, z;
-            coords(i, d, x, y, z);
+        for (int e = threadIdx.x; e < n; e += COARSE_THREADS) {
+            int x, y, z, i;
+            unpack_cell(list[e], x, y, z, i);
             p[i] = p[i] + prolong32<T, P>(pc, x, y, z, d.n0 / 2, d.n1 / 2, d.n2 / 2);
         }
         __syncthreads();
-        lds_smooth<T, P>(p, w, r, d, a.k[l], a.smoother, a.nu, a.omega);
+        const T inv_p = T(1) / (T(2) * (k.w[0] + k.w[1] + k.w[2]));
+        lds_smooth<T, P>(p, w, r, list, d, k, tabs[l], inv_p, a.smoother, a.nu, a.omega);
         lds_remove_mean(p, n, scratch);
     }
     {
@@ -849,6 +984,42 @@ __global__ void __launch_bounds__(COARSE_THREADS) coarse_cycle_kernel(CoarseArgs
 
 // ---------------------------------------------------------------------------
 // host driver
+
+// Dynamic LDS a kernel may request: the device's per-workgroup LDS minus the
+// kernel's static LDS. The first call per (kernel, device) raises the
+// kernel's dynamic-LDS attribute to that amount; `need` is checked against it.
+static int dynamic_lds(const void* fn, size_t need) {
+    struct Entry { const void* fn; int dev; size_t avail; };
+    static std::mutex lock;
+    static Entry cache[64];
+    static int used = 0;
+    int dev = 0;
+    AFSI_HIP(hipGetDevice(&dev));
+    size_t avail = 0;
+    bool found = false;
+    {
+        std::lock_guard<std::mutex> g(lock);
+        for (int i = 0; i < used && !found; ++i)
+            if (cache[i].fn == fn && cache[i].dev == dev) { avail = cache[i].avail; found = true; }
+    }
+    if (!found) {
+        int per_block = 0, optin = 0;
+        AFSI_HIP(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        AFSI_HIP(hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev));
+        const size_t limit = size_t(per_block > optin ? per_block : optin);
+        hipFuncAttributes attr{};
+        AFSI_HIP(hipFuncGetAttributes(&attr, fn));
+        avail = limit > attr.sharedSizeBytes ? limit - attr.sharedSizeBytes : 0;
+        AFSI_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(avail)));
+        std::lock_guard<std::mutex> g(lock);
+        if (used < 64) cache[used++] = Entry{fn, dev, avail};
+    }
+    if (need > avail) {
+        set_error("kernel needs %zu bytes of dynamic LDS; the device allows %zu", need, avail);
+        return E_LIMIT;
+    }
+    return OK;
+}
 
 template <class T>
 struct FastRunner {
@@ -874,14 +1045,10 @@ struct FastRunner {
         const Dim32 d = D(l);
         const int ntx = int(cdiv(d.n0, TXr)), nty = int(cdiv(d.n1, TYr));
         const size_t bytes = size_t((TXr + 4) * (TYr + 4)) * d.n2 * sizeof(T);
-        static bool configured = false;
-        if (!configured) {
-            AFSI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(rb_rows_kernel<T, true, TXr, TYr>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-            AFSI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(rb_rows_kernel<T, false, TXr, TYr>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-            configured = true;
-        }
+        if (int r = dynamic_lds(pl.periodic ? reinterpret_cast<const void*>(rb_rows_kernel<T, true, TXr, TYr>)
+                                            : reinterpret_cast<const void*>(rb_rows_kernel<T, false, TXr, TYr>),
+                                bytes))
+            return r;
         const unsigned g = unsigned(ntx * nty);
         if (pl.periodic) rb_rows_kernel<T, true, TXr, TYr><<<g, 256, bytes, s>>>(in, R(l), out, d, K(l), zero_in, ntx, nty);
         else rb_rows_kernel<T, false, TXr, TYr><<<g, 256, bytes, s>>>(in, R(l), out, d, K(l), zero_in, ntx, nty);
@@ -951,29 +1118,25 @@ struct FastRunner {
             a.off_w[j] = off; off += (pl.smoother == 1 ? 0 : n);
         }
         const size_t bytes = size_t(off) * sizeof(T);
-        AFSI_REQUIRE(bytes <= 150 * 1024, "coarse levels exceed LDS");
         a.rhs_in = R(first);
         a.p_in = p_in;
         a.p_out = p_out;
         a.coarse_inverse = static_cast<const T*>(pl.coarse_inverse);
         a.coarse_cells = int32_t(pl.coarse_cells);
-        static bool configured = false;
-        if (!configured) {
-            AFSI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(coarse_cycle_kernel<T, false>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-            AFSI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(coarse_cycle_kernel<T, true>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-            configured = true;
-        }
+        if (int r = dynamic_lds(pl.periodic ? reinterpret_cast<const void*>(coarse_cycle_kernel<T, true>)
+                                            : reinterpret_cast<const void*>(coarse_cycle_kernel<T, false>),
+                                bytes))
+            return r;
         if (pl.periodic) coarse_cycle_kernel<T, true><<<1, COARSE_THREADS, bytes, s>>>(a);
         else coarse_cycle_kernel<T, false><<<1, COARSE_THREADS, bytes, s>>>(a);
         AFSI_LAUNCH_CHECK();
         return OK;
     }
 
-    int cycle(int l, T*& cur) const {
-        if (l == coarse_first) return coarse(l, l == 0 ? cur : nullptr, cur);
-        if (int r = smooth(l, cur, l > 0)) return r;  // coarse iterates start at zero
+    int cycle(int l, T*& cur, bool zero_initial = false) const {
+        if (l == coarse_first) return coarse(l, (l == 0 && !zero_initial) ? cur : nullptr, cur);
+        // coarse iterates (and a zero initial level-0 iterate) start at zero
+        if (int r = smooth(l, cur, l > 0 || zero_initial)) return r;
         const Dim32 d = D(l);
         const W3<T> k = K(l);
         if (rows_ok(l)) {
@@ -1002,12 +1165,19 @@ struct FastRunner {
     }
 };
 
+// First level solved by the LDS kernel: at most COARSE_MAX cells, every
+// extent <= 64, and all of its levels' cell lists within COARSE_LIST.
 static int choose_coarse(const afsi_mg3d_plan& pl) {
     for (int l = 0; l < pl.levels; ++l) {
-        const int64_t c = pl.n[l][0] * pl.n[l][1] * pl.n[l][2];
-        if (c <= COARSE_MAX) return l;
+        int64_t total = 0;
+        bool ok = pl.n[l][0] * pl.n[l][1] * pl.n[l][2] <= COARSE_MAX;
+        for (int j = l; j < pl.levels && ok; ++j) {
+            ok = pl.n[j][0] <= 64 && pl.n[j][1] <= 64 && pl.n[j][2] <= 64;
+            total += pl.n[j][0] * pl.n[j][1] * pl.n[j][2];
+        }
+        if (ok && total <= COARSE_LIST) return l;
     }
-    return pl.levels - 1;
+    return -1;
 }
 
 template <class T>
@@ -1030,6 +1200,8 @@ static int fast_check(const afsi_mg3d_plan* pl) {
                      pl->coarse_cells == pl->n[last][0] * pl->n[last][1] * pl->n[last][2],
                  "coarse inverse does not match the coarsest level");
     AFSI_REQUIRE(pl->coarse_cells <= 512, "coarsest level exceeds 512 cells");
+    AFSI_REQUIRE(choose_coarse(*pl) >= 0,
+                 "fused driver needs coarse levels with extents <= 64 (use the unfused driver)");
     return OK;
 }
 
@@ -1040,7 +1212,7 @@ static int fast_cycles(const afsi_mg3d_plan* pl, int32_t cycles, void* stream) {
     const FastRunner<T> run{*pl, hipStream_t(stream), choose_coarse(*pl)};
     for (int c = 0; c < cycles; ++c) {
         T* cur = run.P(0);
-        if (int r = run.cycle(0, cur)) return r;
+        if (int r = run.cycle(0, cur, c == 0 && pl->zero_initial)) return r;
         AFSI_REQUIRE(cur == run.P(0), "level-0 iterate left the canonical buffer");
     }
     return OK;
@@ -1084,6 +1256,50 @@ int64_t afsi_mg3d_partials_doubles(const int64_t n[3]) {
     int64_t blocks = afsi::cdiv(n[0] * n[1] * n[2], 256);
     if (blocks > 4096) blocks = 4096;
     return 4 * blocks;
+}
+int afsi_refine_residual_3d(const double* x, const double* b, float* r32, const int64_t n[3],
+                            const double inv_h2[3], int periodic, double* partials,
+                            int64_t partials_capacity, double* result, void* stream) {
+    using namespace afsi;
+    AFSI_REQUIRE(n && inv_h2 && x && b && r32 && partials && result, "invalid refinement arguments");
+    AFSI_REQUIRE(n[0] > 0 && n[1] > 0 && n[2] > 0 && n[0] * n[1] * n[2] < (int64_t(1) << 31),
+                 "invalid grid");
+    const Dim32 d{int(n[0]), int(n[1]), int(n[2])};
+    W3<double> k;
+    for (int a = 0; a < 3; ++a) k.w[a] = inv_h2[a];
+    int64_t nb = cdiv(n[0] * n[1], 4);
+    if (nb > partials_capacity / 4) nb = partials_capacity / 4;
+    AFSI_REQUIRE(nb >= 1, "reduction scratch too small");
+    const hipStream_t s = hipStream_t(stream);
+    if (periodic) refine_residual_kernel<true><<<unsigned(nb), 256, 0, s>>>(x, b, r32, d, k, partials);
+    else refine_residual_kernel<false><<<unsigned(nb), 256, 0, s>>>(x, b, r32, d, k, partials);
+    AFSI_LAUNCH_CHECK();
+    return launch_reduce_final(partials, nb, result, s);
+}
+int afsi_refine_update(double* x, const float* e, int64_t count, void* stream) {
+    using namespace afsi;
+    AFSI_REQUIRE(count >= 0 && (count == 0 || (x && e)), "invalid refinement update");
+    if (count == 0) return OK;
+    refine_update_kernel<<<grid_for(count, 256, 16384), 256, 0, hipStream_t(stream)>>>(x, e, count);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+int afsi_mg3d_sweep_f64(const afsi_mg3d_plan* plan, int32_t level, void* stream) {
+    using namespace afsi;
+    if (int r = fast_check<double>(plan)) return r;
+    AFSI_REQUIRE(level >= 0 && level < plan->levels, "invalid level");
+    const FastRunner<double> run{*plan, hipStream_t(stream), choose_coarse(*plan)};
+    return run.sweep(level, run.P(level), run.W(level), false);
+}
+int afsi_mg3d_sweep_f32(const afsi_mg3d_plan* plan, int32_t level, void* stream) {
+    using namespace afsi;
+    if (int r = fast_check<float>(plan)) return r;
+    AFSI_REQUIRE(level >= 0 && level < plan->levels, "invalid level");
+    const FastRunner<float> run{*plan, hipStream_t(stream), choose_coarse(*plan)};
+    return run.sweep(level, run.P(level), run.W(level), false);
+}
+int afsi_mg3d_supported(const afsi_mg3d_plan* plan) {
+    return afsi::fast_check<double>(plan) == afsi::OK ? 1 : 0;
 }
 int afsi_mg3d_cycles_f64(const afsi_mg3d_plan* plan, int32_t cycles, void* stream) {
     return afsi::fast_cycles<double>(plan, cycles, stream);

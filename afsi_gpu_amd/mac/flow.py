@@ -4,13 +4,15 @@ rhs = -(rho/dt) D u* (HIP divergence kernel), p = multigrid solve with the
 previous pressure as the initial guess, u = u* - (dt/rho) G p (HIP gradient
 correction; Neumann boundary-normal faces untouched, mac/grid.py:74-83).
 The predictor ``step`` (convection, viscosity) is outside this hot path.
+``precision='mixed'`` solves the pressure with FP32 V-cycles inside FP64
+iterative refinement (divergence, gradient and velocities stay FP64).
 """
 from dataclasses import dataclass
 from math import isfinite
 import torch
 
 from .. import _abi
-from .multigrid import GeometricMultigrid
+from .multigrid import GeometricMultigrid, MixedMultigrid
 
 
 @dataclass(frozen=True)
@@ -40,14 +42,21 @@ def correct(velocity, pressure, grid, coef):
 
 class MACFlow:
     def __init__(self, grid, *, dt, rho=1., mu=1., device='cuda', dtype=torch.float64,
-                 options=None, pressure_backend='graph'):
+                 options=None, pressure_backend='graph', precision='fp64'):
         if any(not isfinite(v) or v <= 0 for v in (dt, rho, mu)):
             raise ValueError('this explicit-viscous MAC scheme requires positive dt/rho/mu')
         if dtype != torch.float64:
             raise ValueError('the plain projection is FP64')
         self.grid, self.dt, self.rho, self.mu = grid, float(dt), float(rho), float(mu)
-        self.pressure_solver = GeometricMultigrid(grid, device=device, dtype=dtype,
-                                                  options=options, backend=pressure_backend)
+        if precision not in ('fp64', 'mixed'):
+            raise ValueError('precision must be fp64 or mixed')
+        self.precision = precision
+        if precision == 'mixed':
+            self.pressure_solver = MixedMultigrid(grid, device=device, options=options,
+                                                  backend=pressure_backend)
+        else:
+            self.pressure_solver = GeometricMultigrid(grid, device=device, dtype=dtype,
+                                                      options=options, backend=pressure_backend)
         self._rhs = torch.empty(grid.shape, device=device, dtype=dtype)
 
     @torch.no_grad()

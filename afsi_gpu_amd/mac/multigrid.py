@@ -113,6 +113,9 @@ class GeometricMultigrid:
         self.partials = torch.zeros(capacity, device=self.device, dtype=torch.float64)
         self.results = torch.zeros((8, 4), device=self.device, dtype=torch.float64)
         self.plan = self._make_plan()
+        if self.fused and not lib.afsi_mg3d_supported(self.plan):
+            self.fused = False
+            self.backend = 'hip-unfused'
         self.allocated_bytes = sum(t.numel() * t.element_size()
                                    for group in (self.p, self.rhs, self.tmp) for t in group)
         self.graphs = {}
@@ -266,3 +269,148 @@ class GeometricMultigrid:
         self._materialize()
         return w_p.clone(), dict(cycles=cycle, residual_norm=residual, tolerance=tolerance,
                                  backend=self.backend)
+
+
+class MixedMultigrid:
+    """Mixed-precision pressure solve (build-own): FP32 V-cycles inside FP64
+    iterative refinement.
+
+    Each outer iteration runs one fused FP32 V-cycle on the FP32 copy of the
+    FP64 residual (zero initial guess), adds the FP32 correction to the FP64
+    iterate, and recomputes r = b - A x in FP64. The residual, the
+    accumulation of x and the removal of the constant null space (mean of
+    b and of the result) stay in FP64, so the iteration converges to the
+    FP64 tolerance max(atol, rtol*||b||) of ``GeometricMultigrid.solve``
+    (the FP32 cycle only has to reduce the error by the V-cycle factor).
+    ``solve`` keeps the signature, checks and messages of the FP64 solver.
+    """
+
+    def __init__(self, grid, *, device='cuda', options=None, backend='graph'):
+        self.options = options or MGOptions(smooth=2, smoother='rbgs')
+        self.device = torch.device(device)
+        self.grid = grid
+        self.inner = GeometricMultigrid(grid, device=device, dtype=torch.float32,
+                                        options=self.options, backend='workspace')
+        if not self.inner.fused:
+            raise ValueError('mixed precision needs the fused multigrid driver')
+        self.inner.plan.zero_initial = 1
+        self.shapes = self.inner.shapes
+        self.periodic = self.inner.periodic
+        self.dtype = torch.float64
+        shape = self.shapes[0]
+        self.x = torch.zeros(shape, device=self.device, dtype=torch.float64)
+        self.b = torch.zeros(shape, device=self.device, dtype=torch.float64)
+        self.partials = torch.zeros_like(self.inner.partials)
+        self.results = torch.zeros((8, 4), device=self.device, dtype=torch.float64)
+        h = self.inner.spacings[0]
+        self._n = _abi.i64x3(shape)
+        self._w = _abi.f64x3([1 / a ** 2 for a in h])
+        self.backend = 'hip-mixed-graph' if backend == 'graph' else 'hip-mixed'
+        self.graphs = {}
+        if backend == 'graph':
+            self._capture()
+
+    def _residual(self):
+        _abi.call('afsi_refine_residual_3d', _abi.ptr(self.x), _abi.ptr(self.b),
+                  _abi.ptr(self.inner.rhs[0]), self._n, self._w, int(self.periodic),
+                  _abi.ptr(self.partials), self.partials.numel(), _abi.ptr(self.results[0]),
+                  _abi.stream())
+
+    def _iteration(self):
+        self.inner._cycles(1)
+        _abi.call('afsi_refine_update', _abi.ptr(self.x), _abi.ptr(self.inner.p[0]),
+                  self.x.numel(), _abi.stream())
+        self._residual()
+
+    def _block(self, count):
+        for _ in range(count):
+            self._iteration()
+
+    def _capture(self):
+        self.x.zero_()
+        self.b.zero_()
+        stream = torch.cuda.Stream(device=self.device)
+        stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(stream):
+            self._residual()
+            for _ in range(2):
+                self._block(self.options.check_every)
+        torch.cuda.current_stream().wait_stream(stream)
+        for count in range(1, self.options.check_every + 1):
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph, stream=stream):
+                self._block(count)
+            self.graphs[count] = graph
+        torch.cuda.current_stream().wait_stream(stream)
+        torch.cuda.synchronize(self.device)
+
+    def advance(self, count):
+        if self.graphs:
+            self.graphs[count].replay()
+        else:
+            self._block(count)
+
+    def _reduce(self, t, slot):
+        _abi.call('afsi_reduce_f64', _abi.ptr(t), t.numel(), _abi.ptr(self.partials),
+                  _abi.ptr(self.results[slot]), _abi.stream())
+
+    def _sub_mean(self, t, slot):
+        _abi.call('afsi_sub_mean_f64', _abi.ptr(t), t.numel(), _abi.ptr(self.results[slot]),
+                  _abi.stream())
+
+    @torch.no_grad()
+    def solve(self, rhs, initial=None, *, rtol=None, history=None):
+        """Same contract as ``GeometricMultigrid.solve``; ``info['cycles']``
+        counts outer iterations (one FP32 V-cycle each)."""
+        rtol = self.options.rtol if rtol is None else rtol
+        if not isfinite(rtol) or rtol < 0:
+            raise ValueError('invalid multigrid solve tolerance')
+        if rhs.shape != self.shapes[0] or rhs.device != self.device or rhs.dtype != torch.float64:
+            raise ValueError('invalid pressure RHS')
+        if initial is not None and (initial.shape != rhs.shape or initial.device != rhs.device
+                                    or initial.dtype != rhs.dtype):
+            raise ValueError('invalid initial pressure')
+        self.b.copy_(rhs)
+        if initial is None:
+            self.x.zero_()
+        else:
+            self.x.copy_(initial)
+        self._reduce(self.b, 1)
+        self._reduce(self.x, 2)
+        self._sub_mean(self.b, 1)
+        self._sub_mean(self.x, 2)
+        self._reduce(self.b, 3)
+        self._residual()
+        count = self.b.numel()
+        (s1, _, a1, nf1), (_, _, _, nf2), (_, q3, _, _), (_, q0, _, nf0) = \
+            self.results[[1, 2, 3, 0]].tolist()
+        if nf1:
+            raise ValueError('invalid pressure RHS')
+        mean, absolute_mean = s1 / count, a1 / count
+        if abs(mean) > 1e-12 + 1e-10 * absolute_mean:
+            raise ValueError('incompatible Neumann pressure RHS: net flux must vanish')
+        if nf2:
+            raise ValueError('invalid initial pressure')
+        rhs_norm, residual = q3 ** .5, q0 ** .5
+        if not isfinite(rhs_norm) or not isfinite(residual) or nf0:
+            raise RuntimeError('nonfinite multigrid residual or RHS norm')
+        tolerance = max(self.options.atol, rtol * rhs_norm)
+        cycle = 0
+        if history is not None:
+            history.append(residual)
+        while residual > tolerance and cycle < self.options.max_cycles:
+            n = min(self.options.check_every, self.options.max_cycles - cycle)
+            self.advance(n)
+            cycle += n
+            _, q0, _, nf0 = self.results[0].tolist()
+            residual = q0 ** .5
+            if history is not None:
+                history.append(residual)
+            if nf0 or not isfinite(residual):
+                raise RuntimeError('nonfinite multigrid residual')
+        if residual > tolerance:
+            raise RuntimeError(f'pressure multigrid did not converge: {residual:g} > {tolerance:g}')
+        self._reduce(self.x, 4)
+        self._sub_mean(self.x, 4)
+        return self.x.clone(), dict(cycles=cycle, residual_norm=residual, tolerance=tolerance,
+                                    backend=self.backend)

@@ -19,13 +19,14 @@ from .mac.transfer import DEFAULT_TILE, Binner, TiledTransfer
 
 class IBStep:
     def __init__(self, grid, points, *, dt, rho=1., device='cuda', options=None,
-                 tile=DEFAULT_TILE, pressure_backend='graph'):
+                 tile=DEFAULT_TILE, pressure_backend='graph', precision='fp64'):
         self.grid, self.dt, self.rho = grid, float(dt), float(rho)
         self.device = torch.device(device)
         self.count = int(points)
         self.options = options or MGOptions(smooth=2, smoother='rbgs')
+        self.precision = precision
         self.flow = MACFlow(grid, dt=dt, rho=rho, device=device, options=self.options,
-                            pressure_backend=pressure_backend)
+                            pressure_backend=pressure_backend, precision=precision)
         self.binner = Binner(grid, self.count, self.device, tile)
         self.transfer = TiledTransfer(grid, self.device, tile)
         self.force_density = grid.zeros(device=self.device)

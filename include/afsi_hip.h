@@ -211,7 +211,12 @@ typedef struct afsi_mg3d_plan {
     double* reduce_partials;
     double* reduce_result;
     int64_t partials_capacity; /* doubles in reduce_partials */
+    int32_t zero_initial;      /* fused driver: treat p[0] as zero on the first cycle */
+    int32_t reserved;
 } afsi_mg3d_plan;
+/* 1 if the fused driver accepts the plan (every level with <= 4096 cells
+ * that it solves in LDS must have extents <= 64), else 0. */
+int afsi_mg3d_supported(const afsi_mg3d_plan* plan);
 /* Doubles of reduce_partials the fused driver needs for level-0 extent n. */
 int64_t afsi_mg3d_partials_doubles(const int64_t n[3]);
 /* Fused driver (the production path). Per fine level: nu single-launch
@@ -229,6 +234,19 @@ int afsi_mg3d_cycles_f32(const afsi_mg3d_plan* plan, int32_t cycles, void* strea
  * rhs[0] - A p[0] (two launches, no grid-wide synchronisation). */
 int afsi_mg3d_residual_f64(const afsi_mg3d_plan* plan, void* stream);
 int afsi_mg3d_residual_f32(const afsi_mg3d_plan* plan, void* stream);
+/* One smoothing sweep of `level` from p[level] into tmp[level] (exactly
+ * the kernel the fused driver launches there); for timing. */
+int afsi_mg3d_sweep_f64(const afsi_mg3d_plan* plan, int32_t level, void* stream);
+int afsi_mg3d_sweep_f32(const afsi_mg3d_plan* plan, int32_t level, void* stream);
+/* FP64 iterative refinement around FP32 V-cycles (build-own mixed
+ * precision): r = b - A x in FP64 (difference form), r32 = float(r), and
+ * result = [sum r, sum r^2, sum |r|, non-finite] (two launches);
+ * partials holds partials_capacity doubles. */
+int afsi_refine_residual_3d(const double* x, const double* b, float* r32, const int64_t n[3],
+                            const double inv_h2[3], int periodic, double* partials,
+                            int64_t partials_capacity, double* result, void* stream);
+/* x += e (FP64 accumulation of an FP32 correction). */
+int afsi_refine_update(double* x, const float* e, int64_t count, void* stream);
 /* Unfused driver with the reference workspace's launch structure (one
  * launch per colour sweep, separate reductions and mean subtractions,
  * mac/mg_workspace.py:246-305); kept as a cross-check of the fused one.
