@@ -13,7 +13,8 @@ timed between device synchronisations. `value` is the mixed path's steps/s.
 
 `roofline` is for the dominant kernel of the mixed step, the level-0
 red-black sweep (algorithmic bytes: read p and rhs, write p = 3 x 4 B per
-cell in FP32), timed with HIP events on the stream it is launched on.
+cell in FP32), timed with HIP events on the stream it is launched on;
+`traffic` is its measured HBM bytes per launch (rocprofv3 counters).
 `cpu_baseline` times the FP64 NumPy oracle on a bounded sample.
 """
 import argparse
@@ -29,6 +30,20 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = 'IB spread+interp Mpts/s & MAC steps/s at 128³ vs plain; p50/p99 served latency'
 HBM_PEAK = 8.0e12
+# HBM bytes per launch of the roofline kernel, measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes and scaled by a copy kernel of known size.
+TRAFFIC_FILE = ROOT / 'profiles' / 'rb_sweep_f32_128cubed_hbm_traffic.md'
+
+
+def measured_traffic(n):
+    """traffic_bytes_per_launch from the counter summary (measured at
+    128^3), or None for another grid or when the summary is absent."""
+    if n != 128 or not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if line.startswith('- traffic_bytes_per_launch:'):
+            return float(line.split(':')[1])
+    return None
 PHASES = ('bin', 'spread', 'project', 'interp')
 
 
@@ -40,6 +55,8 @@ def parse():
     ap.add_argument('--points', type=int, default=1_500_000)
     ap.add_argument('--n', type=int, default=128)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--kernels-only', type=int, default=0, metavar='R',
+                    help='only launch the roofline sweep and a copy kernel R times each')
     return ap.parse_args()
 
 
@@ -109,21 +126,48 @@ def one_step(torch, stepper, x0, F):
     return U.clone(), tuple(c.clone() for c in u), tentative, after
 
 
-def time_sweep(torch, solver, repeats=20):
-    """Average duration of the level-0 sweep kernel (HIP events on its stream)."""
+def time_sweep(torch, solver, repeats=50, batches=5):
+    """Average duration of the level-0 sweep kernel: HIP events on the
+    stream the sweeps are launched on, around `repeats` back-to-back
+    launches; the mean over `batches` such brackets, divided by `repeats`."""
     from afsi_gpu_amd import _abi
     name = f'afsi_mg3d_sweep_{solver.suffix}'
     _abi.call(name, solver.plan, 0, _abi.stream())
     stream = torch.cuda.current_stream()
-    start = [torch.cuda.Event(enable_timing=True) for _ in range(repeats)]
-    end = [torch.cuda.Event(enable_timing=True) for _ in range(repeats)]
+    start = [torch.cuda.Event(enable_timing=True) for _ in range(batches)]
+    end = [torch.cuda.Event(enable_timing=True) for _ in range(batches)]
     torch.cuda.synchronize()
-    for i in range(repeats):
+    for i in range(batches):
         start[i].record(stream)
-        _abi.call(name, solver.plan, 0, _abi.stream())
+        for _ in range(repeats):
+            _abi.call(name, solver.plan, 0, _abi.stream())
         end[i].record(stream)
     torch.cuda.synchronize()
-    return float(np.mean([a.elapsed_time(b) for a, b in zip(start, end)])) / 1e3
+    return float(np.mean([a.elapsed_time(b) for a, b in zip(start, end)])) / 1e3 / repeats
+
+
+def kernels_only(torch, n, repeats):
+    """For hardware-counter passes: `repeats` launches of the level-0 FP32
+    red-black sweep at n^3 (the roofline kernel), then `repeats` launches of
+    a plain copy kernel of known size (out = 1.0 * src over n^3 float64:
+    8 n^3 bytes read, 8 n^3 bytes written), which scales the counters."""
+    from afsi_gpu_amd.mac import MACGrid, MGOptions, MixedMultigrid
+    grid = MACGrid((n, n, n), (1., 1., 1.), periodic=True)
+    solver = MixedMultigrid(grid, device='cuda:0', options=MGOptions(smooth=2, smoother='rbgs'),
+                            backend='workspace').inner
+    gen = torch.Generator(device='cuda:0').manual_seed(0)
+    solver.p[0].copy_(torch.randn(solver.shapes[0], device='cuda:0', generator=gen))
+    solver.rhs[0].copy_(torch.randn(solver.shapes[0], device='cuda:0', generator=gen))
+    sweep_s = time_sweep(torch, solver, repeats=repeats, batches=1)
+    src = torch.randn(n ** 3, device='cuda:0', dtype=torch.float64, generator=gen)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(repeats):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps(dict(kernels_only=True, n=n, repeats=repeats, sweep_us=sweep_s * 1e6,
+                          sweep_algorithmic_bytes=12 * n ** 3, copy_bytes_read=8 * n ** 3,
+                          copy_bytes_written=8 * n ** 3)))
 
 
 def main():
@@ -132,6 +176,9 @@ def main():
         print(json.dumps(dict(error='this build runs on one GPU; --gpus must be 1')))
         sys.exit(2)
     import torch
+    if args.kernels_only:
+        kernels_only(torch, args.n, args.kernels_only)
+        return
     from afsi_gpu_amd import workloads
     from afsi_gpu_amd.mac import MACGrid, MGOptions
     from afsi_gpu_amd.step import IBStep
@@ -170,7 +217,7 @@ def main():
                              'red-black GS V(2,2) to rtol 1e-10; mixed = FP32 V-cycles in FP64 '
                              'refinement', grid=n, points=P, precision='mixed'),
         roofline=dict(bound='hbm', achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
-                      frac=achieved / HBM_PEAK, traffic=None,
+                      frac=achieved / HBM_PEAK, traffic=measured_traffic(n),
                       kernel='rb_rows_kernel<float,true,4,4> (level-0 sweep)',
                       algorithmic_bytes=sweep_bytes, avg_us=sweep_s * 1e6),
         plain_fp64=dict(steps_per_s=plain['steps_per_s'], ms_per_step=plain['ms_per_step'],
