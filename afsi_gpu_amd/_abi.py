@@ -37,7 +37,24 @@ class MGPlan3(Structure):
                 ('zero_initial', c_int32), ('reserved', c_int32)]
 
 
+class Grid2(Structure):
+    _fields_ = [('n', c_int64 * 2), ('h', c_double * 2), ('lengths', c_double * 2),
+                ('periodic', c_int32), ('reserved', c_int32)]
+
+
+class MGPlan2(Structure):
+    _fields_ = [('levels', c_int32), ('periodic', c_int32), ('smoother', c_int32),
+                ('nu', c_int32), ('omega', c_double),
+                ('n', (c_int64 * 2) * MAX_LEVELS), ('h2', (c_double * 2) * MAX_LEVELS),
+                ('inv_h2', (c_double * 2) * MAX_LEVELS),
+                ('p', c_void_p * MAX_LEVELS), ('rhs', c_void_p * MAX_LEVELS),
+                ('tmp', c_void_p * MAX_LEVELS), ('coarse_inverse', c_void_p),
+                ('coarse_cells', c_int64), ('reduce_partials', c_void_p),
+                ('reduce_result', c_void_p), ('partials_capacity', c_int64)]
+
+
 _P = c_void_p
+_I64, _F64 = POINTER(c_int64), POINTER(c_double)
 _SIGNATURES = {
     'afsi_last_error': (c_char_p, []),
     'afsi_abi_version': (c_int, []),
@@ -94,6 +111,23 @@ _SIGNATURES = {
     'afsi_mg3d_cycles_unfused_f32': (c_int, [POINTER(MGPlan3), c_int32, _P]),
     'afsi_mg3d_residual_unfused_f64': (c_int, [POINTER(MGPlan3), _P]),
     'afsi_mg3d_residual_unfused_f32': (c_int, [POINTER(MGPlan3), _P]),
+    'afsi_channel_prepare_f64': (c_int, [_P, c_int64, POINTER(Grid2), _P, _P, _P, _P, _P, _P]),
+    'afsi_mg_residual_2d_f64': (c_int, [_P, _P, _P, _I64, _F64, c_int, _P]),
+    'afsi_mg_residual_2d_f32': (c_int, [_P, _P, _P, _I64, _F64, c_int, _P]),
+    'afsi_mg_jacobi_2d_f64': (c_int, [_P, _P, _P, _I64, _F64, _F64, c_double, c_int, _P]),
+    'afsi_mg_jacobi_2d_f32': (c_int, [_P, _P, _P, _I64, _F64, _F64, c_double, c_int, _P]),
+    'afsi_mg_rbgs_2d_f64': (c_int, [_P, _P, _I64, _F64, _F64, c_int, _P]),
+    'afsi_mg_rbgs_2d_f32': (c_int, [_P, _P, _I64, _F64, _F64, c_int, _P]),
+    'afsi_mg_residual_restrict_2d_f64': (c_int, [_P, _P, _P, _I64, _F64, c_int, _P]),
+    'afsi_mg_residual_restrict_2d_f32': (c_int, [_P, _P, _P, _I64, _F64, c_int, _P]),
+    'afsi_mg_prolong_add_2d_f64': (c_int, [_P, _P, _I64, c_int, _P]),
+    'afsi_mg_prolong_add_2d_f32': (c_int, [_P, _P, _I64, c_int, _P]),
+    'afsi_divergence_2d_f64': (c_int, [_P, _P, _P, _I64, _F64, c_double, c_int, _P]),
+    'afsi_grad_correct_2d_f64': (c_int, [_P, _P, _P, _I64, _F64, c_double, c_int, _P]),
+    'afsi_mg2d_cycles_f64': (c_int, [POINTER(MGPlan2), c_int32, _P]),
+    'afsi_mg2d_cycles_f32': (c_int, [POINTER(MGPlan2), c_int32, _P]),
+    'afsi_mg2d_residual_f64': (c_int, [POINTER(MGPlan2), _P]),
+    'afsi_mg2d_residual_f32': (c_int, [POINTER(MGPlan2), _P]),
 }
 
 

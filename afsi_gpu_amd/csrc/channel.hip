@@ -1,0 +1,582 @@
+// 2D staggered channel on gfx950: wall-reflected 16-link stencil, 5-point
+// pressure operator, multigrid kernels and a native V-cycle driver.
+//
+// Two boundary variants (template parameter P):
+//   P = false  the reference channel (mac2d/grid.py:61-71): zero normal
+//              derivative at the inlet (x = 0) and the walls (y = 0, Ly),
+//              p = 0 half a cell beyond the outlet (+2p/hx^2 on the last
+//              row); non-singular, no mean removal (mac2d/multigrid.py).
+//   P = true   periodic in x, walls in y (build-own); singular, so the
+//              driver removes means as the 3D closed box does
+//              (mac/multigrid.py:107-123).
+// Cell (x, y) of an nx x ny level is element x*ny + y. Every elementwise
+// kernel evaluates the reference's expression in the reference's order
+// (mac2d/_triton_mg.py:7-75), so results are bit-identical to the
+// restatement in FP64 (compiled with -ffp-contract=off).
+#include "common.hpp"
+
+namespace afsi {
+
+namespace {
+
+constexpr int CH_THREADS = 256;
+
+struct Dim2 {
+    int nx, ny;
+};
+
+template <class T>
+struct Coef2 {
+    T hx2, hy2;    // h^2 (operator divisions)
+    T ihx2, ihy2;  // 1/h^2 (diagonal)
+};
+
+// -A p at cell (x, y), mac2d/_triton_mg.py:7-21 order: -(right diff)
+// + (left diff) per axis, then the outlet term.
+template <class T, bool P>
+__device__ inline T ch_apply(const T* __restrict__ p, int x, int y, Dim2 d, const Coef2<T>& k,
+                             T& centre) {
+    const int i = x * d.ny + y;
+    const T c = p[i];
+    centre = c;
+    T a = T(0);
+    if (P || x + 1 < d.nx) a -= (p[x + 1 < d.nx ? i + d.ny : y] - c) / k.hx2;
+    if (P || x > 0) a += (c - p[x > 0 ? i - d.ny : (d.nx - 1) * d.ny + y]) / k.hx2;
+    if (y + 1 < d.ny) a -= (p[i + 1] - c) / k.hy2;
+    if (y > 0) a += (c - p[i - 1]) / k.hy2;
+    if (!P && x == d.nx - 1) a += T(2) * c / k.hx2;
+    return a;
+}
+
+// Diagonal of A, mac2d/multigrid.py:27-33 order (2/hx^2 == 2*(1/hx^2)).
+template <class T, bool P>
+__device__ inline T ch_diag(int x, int y, Dim2 d, const Coef2<T>& k) {
+    T g = T(0);
+    if (P || x + 1 < d.nx) g += k.ihx2;
+    if (P || x > 0) g += k.ihx2;
+    if (y + 1 < d.ny) g += k.ihy2;
+    if (y > 0) g += k.ihy2;
+    if (!P && x == d.nx - 1) g += T(2) * k.ihx2;
+    return g;
+}
+
+// Mode 0: out = rhs - A p. Mode 1: out = p + (omega*(rhs - A p))/diag.
+template <class T, bool P, int Mode>
+__global__ void __launch_bounds__(CH_THREADS) ch_stencil_kernel(const T* __restrict__ p,
+                                                                const T* __restrict__ rhs,
+                                                                T* __restrict__ out, Dim2 d,
+                                                                Coef2<T> k, T omega) {
+    const int n = d.nx * d.ny;
+    for (int i = blockIdx.x * CH_THREADS + threadIdx.x; i < n; i += gridDim.x * CH_THREADS) {
+        const int x = i / d.ny, y = i - x * d.ny;
+        T c;
+        const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
+        out[i] = Mode == 0 ? r : c + (omega * r) / ch_diag<T, P>(x, y, d, k);
+    }
+}
+
+// One colour of a red-black Gauss-Seidel sweep, in place: cells with
+// (x + y) % 2 == color get p += (rhs - A p)/diag (their neighbours have the
+// other colour, so no cell read here is written here).
+template <class T, bool P>
+__global__ void __launch_bounds__(CH_THREADS) ch_rbgs_kernel(T* __restrict__ p,
+                                                             const T* __restrict__ rhs, Dim2 d,
+                                                             Coef2<T> k, int color) {
+    // ny even: row x holds ny/2 cells of this colour at y = 2j + ((x + color) & 1)
+    const int hy = d.ny / 2, half = d.nx * hy;
+    for (int e = blockIdx.x * CH_THREADS + threadIdx.x; e < half; e += gridDim.x * CH_THREADS) {
+        const int x = e / hy, y = 2 * (e - x * hy) + ((x + color) & 1);
+        const int i = x * d.ny + y;
+        T c;
+        const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
+        p[i] = c + r / ch_diag<T, P>(x, y, d, k);
+    }
+}
+
+// Fallback colour pass for odd ny: one thread per cell.
+template <class T, bool P>
+__global__ void __launch_bounds__(CH_THREADS) ch_rbgs_any_kernel(T* __restrict__ p,
+                                                                 const T* __restrict__ rhs,
+                                                                 Dim2 d, Coef2<T> k, int color) {
+    const int n = d.nx * d.ny;
+    for (int i = blockIdx.x * CH_THREADS + threadIdx.x; i < n; i += gridDim.x * CH_THREADS) {
+        const int x = i / d.ny, y = i - x * d.ny;
+        if (((x + y) & 1) != color) continue;
+        T c;
+        const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
+        p[i] = c + r / ch_diag<T, P>(x, y, d, k);
+    }
+}
+
+// coarse = (((r00 + r01) + r10) + r11) * 0.25 with r = rhs - A p on the fine
+// level (mac2d/_triton_mg.py:40-51); d is the fine level.
+template <class T, bool P>
+__global__ void __launch_bounds__(CH_THREADS) ch_residual_restrict_kernel(
+    const T* __restrict__ p, const T* __restrict__ rhs, T* __restrict__ coarse, Dim2 d,
+    Coef2<T> k) {
+    const int cy = d.ny / 2, n = (d.nx / 2) * cy;
+    for (int i = blockIdx.x * CH_THREADS + threadIdx.x; i < n; i += gridDim.x * CH_THREADS) {
+        const int X = i / cy, Y = i - X * cy;
+        const int x = 2 * X, y = 2 * Y, j = x * d.ny + y;
+        T c;
+        const T r00 = rhs[j] - ch_apply<T, P>(p, x, y, d, k, c);
+        const T r01 = rhs[j + 1] - ch_apply<T, P>(p, x, y + 1, d, k, c);
+        const T r10 = rhs[j + d.ny] - ch_apply<T, P>(p, x + 1, y, d, k, c);
+        const T r11 = rhs[j + d.ny + 1] - ch_apply<T, P>(p, x + 1, y + 1, d, k, c);
+        coarse[i] = (((r00 + r01) + r10) + r11) * T(0.25);
+    }
+}
+
+// Coarse value with ghosts: clamped (even) at the inlet and walls, negated
+// (odd) beyond the outlet; periodic: wrapped in x.
+template <class T, bool P>
+__device__ inline T ch_coarse_value(const T* __restrict__ c, int x, int y, int NX, int NY) {
+    const int yy = y < 0 ? 0 : (y >= NY ? NY - 1 : y);
+    if (P) {
+        const int xx = x < 0 ? x + NX : (x >= NX ? x - NX : x);
+        return c[xx * NY + yy];
+    }
+    const T sign = x >= NX ? T(-1) : T(1);
+    const int xx = x < 0 ? 0 : (x >= NX ? NX - 1 : x);
+    return sign * c[xx * NY + yy];
+}
+
+// fine += bilinear prolongation of coarse (mac2d/_triton_mg.py:54-75);
+// d is the fine level.
+template <class T, bool P>
+__global__ void __launch_bounds__(CH_THREADS) ch_prolong_add_kernel(const T* __restrict__ coarse,
+                                                                    T* __restrict__ fine, Dim2 d) {
+    const int n = d.nx * d.ny, NX = d.nx / 2, NY = d.ny / 2;
+    for (int i = blockIdx.x * CH_THREADS + threadIdx.x; i < n; i += gridDim.x * CH_THREADS) {
+        const int x = i / d.ny, y = i - x * d.ny;
+        const int ax = x / 2 - 1 + (x & 1), ay = y / 2 - 1 + (y & 1);
+        const T wx = (x & 1) ? T(0.25) : T(0.75), wy = (y & 1) ? T(0.25) : T(0.75);
+        const T c00 = ch_coarse_value<T, P>(coarse, ax, ay, NX, NY);
+        const T c01 = ch_coarse_value<T, P>(coarse, ax, ay + 1, NX, NY);
+        const T c10 = ch_coarse_value<T, P>(coarse, ax + 1, ay, NX, NY);
+        const T c11 = ch_coarse_value<T, P>(coarse, ax + 1, ay + 1, NX, NY);
+        const T v = (c00 * (T(1) - wy) + c01 * wy) * (T(1) - wx) + (c10 * (T(1) - wy) + c11 * wy) * wx;
+        fine[i] = fine[i] + v;
+    }
+}
+
+// out = scale * ((u[x+1] - u[x])/hx + (v[y+1] - v[y])/hy)
+// (mac2d/grid.py:46-48, then the caller's -rho/dt).
+template <bool P>
+__global__ void __launch_bounds__(CH_THREADS) ch_divergence_kernel(const double* __restrict__ u,
+                                                                   const double* __restrict__ v,
+                                                                   double* __restrict__ out,
+                                                                   Dim2 d, double hx, double hy,
+                                                                   double scale) {
+    const int n = d.nx * d.ny;
+    for (int i = blockIdx.x * CH_THREADS + threadIdx.x; i < n; i += gridDim.x * CH_THREADS) {
+        const int x = i / d.ny, y = i - x * d.ny;
+        const int xr = P ? (x + 1 < d.nx ? x + 1 : 0) : x + 1;
+        const double du = (u[xr * d.ny + y] - u[i]) / hx;
+        const double dv = (v[x * (d.ny + 1) + y + 1] - v[x * (d.ny + 1) + y]) / hy;
+        out[i] = scale * (du + dv);
+    }
+}
+
+// u -= coef*gx, v -= coef*gy with the channel gradient (mac2d/grid.py:51-58:
+// gx[0] = 0, gx[nx] = -2 p[nx-1]/hx, gy = 0 on the walls) or the periodic
+// one (gx[x] = (p[x] - p[x-1 mod nx])/hx). Faces whose gradient is zero
+// keep their value exactly.
+template <bool P>
+__global__ void __launch_bounds__(CH_THREADS) ch_grad_correct_kernel(double* __restrict__ u,
+                                                                     double* __restrict__ v,
+                                                                     const double* __restrict__ p,
+                                                                     Dim2 d, double hx, double hy,
+                                                                     double coef) {
+    const int nu = (P ? d.nx : d.nx + 1) * d.ny, nv = d.nx * (d.ny + 1);
+    for (int i = blockIdx.x * CH_THREADS + threadIdx.x; i < nu + nv; i += gridDim.x * CH_THREADS) {
+        if (i < nu) {
+            const int x = i / d.ny, y = i - x * d.ny;
+            double g;
+            if (P) g = (p[x * d.ny + y] - p[(x > 0 ? x - 1 : d.nx - 1) * d.ny + y]) / hx;
+            else if (x == 0) continue;
+            else if (x == d.nx) g = -2 * p[(d.nx - 1) * d.ny + y] / hx;
+            else g = (p[x * d.ny + y] - p[(x - 1) * d.ny + y]) / hx;
+            u[i] = u[i] - coef * g;
+        } else {
+            const int j = i - nu, x = j / (d.ny + 1), y = j - x * (d.ny + 1);
+            if (y == 0 || y == d.ny) continue;
+            const double g = (p[x * d.ny + y] - p[x * d.ny + y - 1]) / hy;
+            v[j] = v[j] - coef * g;
+        }
+    }
+}
+
+// Wall-reflected 16-link stencil of mac2d/transfer.py:61-81, one thread per
+// (component, point): links k = 4 i + j at node (bx + i, by + j), weight
+// phi(sx - node_x) * phi(sy - node_y); rows outside the walls reflect
+// (u: j<0 -> -j-1, j>=ny -> 2ny-j-1; v: j<0 -> -j, j>ny -> 2ny-j) with the
+// weight negated; periodic grids wrap the x node. Points outside the
+// support region of mac2d/transfer.py:88-95 set flag bit 1.
+__global__ void __launch_bounds__(CH_THREADS) ch_prepare_kernel(
+    const double* __restrict__ xy, int64_t points, Dim2 d, double hx, double hy, double Lx,
+    double Ly, int periodic, int64_t* __restrict__ ids_u, double* __restrict__ w_u,
+    int64_t* __restrict__ ids_v, double* __restrict__ w_v, int* __restrict__ flags) {
+    const int64_t total = 2 * points;
+    for (int64_t t = int64_t(blockIdx.x) * CH_THREADS + threadIdx.x; t < total;
+         t += int64_t(gridDim.x) * CH_THREADS) {
+        const int c = int(t / points);
+        const int64_t q = t - c * points;
+        const double px = xy[2 * q], py = xy[2 * q + 1];
+        bool ok = isfinite(px) && isfinite(py) && py >= -hy && py <= Ly + hy;
+        if (!periodic) ok = ok && px >= 2 * hx && px <= Lx - 2 * hx;
+        int64_t* ids = c == 0 ? ids_u : ids_v;
+        double* w = c == 0 ? w_u : w_v;
+        if (!ok) {
+            raise_flag(flags, F_SUPPORT);
+            for (int k = 0; k < 16; ++k) {
+                ids[q * 16 + k] = 0;
+                w[q * 16 + k] = 0.0;
+            }
+            continue;
+        }
+        const double ox = hx * (c == 0 ? 0.0 : 0.5), oy = hy * (c == 1 ? 0.0 : 0.5);
+        const double sx = (px - ox) / hx, sy = (py - oy) / hy;
+        const int64_t bx = int64_t(floor(sx - 1)), by = int64_t(floor(sy - 1));
+        const int64_t fny = d.ny + (c == 1);
+        double fx[4], fy[4];
+        for (int a = 0; a < 4; ++a) {
+            fx[a] = peskin_phi(sx - double(bx + a));
+            fy[a] = peskin_phi(sy - double(by + a));
+        }
+        for (int i = 0; i < 4; ++i) {
+            int64_t nxi = bx + i;
+            if (periodic) nxi = ((nxi % d.nx) + d.nx) % d.nx;
+            for (int j = 0; j < 4; ++j) {
+                const int64_t row = by + j;
+                bool outside;
+                int64_t refl;
+                if (c == 0) {
+                    outside = row < 0 || row >= d.ny;
+                    refl = row < 0 ? -row - 1 : (row >= d.ny ? 2 * d.ny - row - 1 : row);
+                } else {
+                    outside = row < 0 || row > d.ny;
+                    refl = row < 0 ? -row : (row > d.ny ? 2 * d.ny - row : row);
+                }
+                const double wt = fx[i] * fy[j];
+                ids[q * 16 + 4 * i + j] = nxi * fny + refl;
+                w[q * 16 + 4 * i + j] = outside ? -wt : wt;
+            }
+        }
+    }
+}
+
+bool dims2_ok(const int64_t n[2]) {
+    return n && n[0] > 0 && n[1] > 0 && n[0] * n[1] < (int64_t(1) << 31);
+}
+
+template <class T>
+Coef2<T> coef2(const double h2[2], const double inv_h2[2]) {
+    return Coef2<T>{T(h2[0]), T(h2[1]), T(inv_h2 ? inv_h2[0] : 0.0), T(inv_h2 ? inv_h2[1] : 0.0)};
+}
+
+unsigned blocks_for(int64_t n) { return grid_for(n, CH_THREADS, 8192); }
+
+template <class T, int Mode>
+int ch_stencil(const T* p, const T* rhs, T* out, const int64_t n[2], const double h2[2],
+               const double inv_h2[2], double omega, int periodic, hipStream_t s) {
+    AFSI_REQUIRE(dims2_ok(n) && h2 && p && rhs && out && (Mode == 0 || inv_h2),
+                 "invalid channel stencil arguments");
+    const Dim2 d{int(n[0]), int(n[1])};
+    const Coef2<T> k = coef2<T>(h2, inv_h2);
+    const unsigned g = blocks_for(n[0] * n[1]);
+    if (periodic) ch_stencil_kernel<T, true, Mode><<<g, CH_THREADS, 0, s>>>(p, rhs, out, d, k, T(omega));
+    else ch_stencil_kernel<T, false, Mode><<<g, CH_THREADS, 0, s>>>(p, rhs, out, d, k, T(omega));
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+template <class T>
+int ch_rbgs(T* p, const T* rhs, const int64_t n[2], const double h2[2], const double inv_h2[2],
+            int color, int periodic, hipStream_t s) {
+    AFSI_REQUIRE(dims2_ok(n) && h2 && inv_h2 && p && rhs, "invalid channel smoothing arguments");
+    const Dim2 d{int(n[0]), int(n[1])};
+    const Coef2<T> k = coef2<T>(h2, inv_h2);
+    if (d.ny % 2 == 0) {
+        const unsigned g = blocks_for((n[0] * n[1] + 1) / 2);
+        if (periodic) ch_rbgs_kernel<T, true><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
+        else ch_rbgs_kernel<T, false><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
+    } else {
+        const unsigned g = blocks_for(n[0] * n[1]);
+        if (periodic) ch_rbgs_any_kernel<T, true><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
+        else ch_rbgs_any_kernel<T, false><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
+    }
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+template <class T>
+int ch_residual_restrict(const T* p, const T* rhs, T* coarse, const int64_t n[2],
+                         const double h2[2], int periodic, hipStream_t s) {
+    AFSI_REQUIRE(dims2_ok(n) && h2 && p && rhs && coarse && n[0] % 2 == 0 && n[1] % 2 == 0,
+                 "invalid channel restriction arguments");
+    const Dim2 d{int(n[0]), int(n[1])};
+    const Coef2<T> k = coef2<T>(h2, nullptr);
+    const unsigned g = blocks_for(n[0] * n[1] / 4);
+    if (periodic) ch_residual_restrict_kernel<T, true><<<g, CH_THREADS, 0, s>>>(p, rhs, coarse, d, k);
+    else ch_residual_restrict_kernel<T, false><<<g, CH_THREADS, 0, s>>>(p, rhs, coarse, d, k);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+template <class T>
+int ch_prolong(const T* coarse, T* fine, const int64_t n[2], int periodic, hipStream_t s) {
+    AFSI_REQUIRE(dims2_ok(n) && coarse && fine && n[0] % 2 == 0 && n[1] % 2 == 0,
+                 "invalid channel prolongation arguments");
+    const Dim2 d{int(n[0]), int(n[1])};
+    const unsigned g = blocks_for(n[0] * n[1]);
+    if (periodic) ch_prolong_add_kernel<T, true><<<g, CH_THREADS, 0, s>>>(coarse, fine, d);
+    else ch_prolong_add_kernel<T, false><<<g, CH_THREADS, 0, s>>>(coarse, fine, d);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+}  // namespace
+
+// ---- native V-cycle driver ---------------------------------------------------
+
+template <class T>
+struct ChannelRunner {
+    const afsi_mg2d_plan& pl;
+    hipStream_t s;
+
+    T* P(int l) const { return static_cast<T*>(pl.p[l]); }
+    T* R(int l) const { return static_cast<T*>(pl.rhs[l]); }
+    T* W(int l) const { return static_cast<T*>(pl.tmp[l]); }
+    int64_t cells(int l) const { return pl.n[l][0] * pl.n[l][1]; }
+
+    int remove_mean(T* x, int64_t count) const;
+
+    int smooth(int l) const {
+        if (pl.smoother == 1) {
+            for (int k = 0; k < pl.nu; ++k)
+                for (int color = 0; color < 2; ++color)
+                    if (int r = ch_rbgs<T>(P(l), R(l), pl.n[l], pl.h2[l], pl.inv_h2[l], color,
+                                           pl.periodic, s))
+                        return r;
+        } else {
+            T* a = P(l);
+            T* b = W(l);
+            for (int k = 0; k < pl.nu; ++k) {
+                if (int r = ch_stencil<T, 1>(a, R(l), b, pl.n[l], pl.h2[l], pl.inv_h2[l], pl.omega,
+                                             pl.periodic, s))
+                    return r;
+                T* t = a; a = b; b = t;
+            }
+            if (a != P(l))
+                AFSI_HIP(hipMemcpyAsync(P(l), a, size_t(cells(l)) * sizeof(T),
+                                        hipMemcpyDeviceToDevice, s));
+        }
+        return pl.periodic ? remove_mean(P(l), cells(l)) : OK;
+    }
+
+    int cycle(int l) const {
+        const int last = pl.levels - 1;
+        if (l == last) {
+            if (int r = dense_matvec(R(l), P(l))) return r;
+            return pl.periodic ? remove_mean(P(l), cells(l)) : OK;
+        }
+        if (int r = smooth(l)) return r;
+        if (int r = ch_residual_restrict<T>(P(l), R(l), R(l + 1), pl.n[l], pl.h2[l], pl.periodic, s))
+            return r;
+        if (pl.periodic)
+            if (int r = remove_mean(R(l + 1), cells(l + 1))) return r;
+        if (int r = fill(P(l + 1), cells(l + 1))) return r;
+        if (int r = cycle(l + 1)) return r;
+        if (int r = ch_prolong<T>(P(l + 1), P(l), pl.n[l], pl.periodic, s)) return r;
+        return smooth(l);
+    }
+
+    int dense_matvec(const T* x, T* y) const;
+    int fill(T* x, int64_t count) const;
+};
+
+template <>
+int ChannelRunner<double>::remove_mean(double* x, int64_t count) const {
+    if (int r = afsi_reduce_f64(x, count, pl.reduce_partials, pl.reduce_result, s)) return r;
+    return afsi_sub_mean_f64(x, count, pl.reduce_result, s);
+}
+template <>
+int ChannelRunner<float>::remove_mean(float* x, int64_t count) const {
+    if (int r = afsi_reduce_f32(x, count, pl.reduce_partials, pl.reduce_result, s)) return r;
+    return afsi_sub_mean_f32(x, count, pl.reduce_result, s);
+}
+template <>
+int ChannelRunner<double>::dense_matvec(const double* x, double* y) const {
+    return afsi_dense_matvec_f64(static_cast<const double*>(pl.coarse_inverse), x, y,
+                                 pl.coarse_cells, s);
+}
+template <>
+int ChannelRunner<float>::dense_matvec(const float* x, float* y) const {
+    return afsi_dense_matvec_f32(static_cast<const float*>(pl.coarse_inverse), x, y,
+                                 pl.coarse_cells, s);
+}
+template <>
+int ChannelRunner<double>::fill(double* x, int64_t count) const {
+    return afsi_fill_f64(x, count, 0.0, s);
+}
+template <>
+int ChannelRunner<float>::fill(float* x, int64_t count) const {
+    return afsi_fill_f32(x, count, 0.0, s);
+}
+
+template <class T>
+static int channel_check(const afsi_mg2d_plan* pl) {
+    AFSI_REQUIRE(pl && pl->levels >= 1 && pl->levels <= AFSI_MG_MAX_LEVELS, "invalid multigrid plan");
+    AFSI_REQUIRE(pl->nu >= 1 && (pl->smoother == 0 || pl->smoother == 1), "invalid smoother");
+    AFSI_REQUIRE(pl->reduce_partials && pl->reduce_result, "missing reduction scratch");
+    for (int l = 0; l < pl->levels; ++l) {
+        AFSI_REQUIRE(dims2_ok(pl->n[l]) && pl->p[l] && pl->rhs[l] && pl->tmp[l], "invalid level %d", l);
+        if (l + 1 < pl->levels)
+            for (int a = 0; a < 2; ++a)
+                AFSI_REQUIRE(pl->n[l][a] == 2 * pl->n[l + 1][a], "levels %d/%d are not 2:1", l, l + 1);
+    }
+    const int last = pl->levels - 1;
+    AFSI_REQUIRE(pl->coarse_inverse && pl->coarse_cells == pl->n[last][0] * pl->n[last][1] &&
+                     pl->coarse_cells <= 512,
+                 "coarse inverse does not match the coarsest level");
+    AFSI_REQUIRE(4 * afsi_reduce_partials(pl->n[0][0] * pl->n[0][1]) <= pl->partials_capacity,
+                 "reduction scratch too small");
+    return OK;
+}
+
+template <class T>
+static int channel_cycles(const afsi_mg2d_plan* pl, int32_t cycles, void* stream) {
+    if (int r = channel_check<T>(pl)) return r;
+    AFSI_REQUIRE(cycles >= 0, "negative cycle count");
+    const ChannelRunner<T> run{*pl, hipStream_t(stream)};
+    for (int c = 0; c < cycles; ++c)
+        if (int r = run.cycle(0)) return r;
+    return OK;
+}
+
+template <class T>
+static int channel_residual(const afsi_mg2d_plan* pl, void* stream) {
+    if (int r = channel_check<T>(pl)) return r;
+    const hipStream_t s = hipStream_t(stream);
+    T* tmp = static_cast<T*>(pl->tmp[0]);
+    if (int r = ch_stencil<T, 0>(static_cast<const T*>(pl->p[0]), static_cast<const T*>(pl->rhs[0]),
+                                 tmp, pl->n[0], pl->h2[0], nullptr, 0.0, pl->periodic, s))
+        return r;
+    const int64_t count = pl->n[0][0] * pl->n[0][1];
+    if constexpr (sizeof(T) == 8)
+        return afsi_reduce_f64(tmp, count, pl->reduce_partials, pl->reduce_result, s);
+    else
+        return afsi_reduce_f32(tmp, count, pl->reduce_partials, pl->reduce_result, s);
+}
+
+}  // namespace afsi
+
+using namespace afsi;
+
+extern "C" {
+
+int afsi_mg_residual_2d_f64(const double* p, const double* rhs, double* out, const int64_t n[2],
+                            const double h2[2], int periodic, void* stream) {
+    return ch_stencil<double, 0>(p, rhs, out, n, h2, nullptr, 0.0, periodic, hipStream_t(stream));
+}
+int afsi_mg_residual_2d_f32(const float* p, const float* rhs, float* out, const int64_t n[2],
+                            const double h2[2], int periodic, void* stream) {
+    return ch_stencil<float, 0>(p, rhs, out, n, h2, nullptr, 0.0, periodic, hipStream_t(stream));
+}
+int afsi_mg_jacobi_2d_f64(const double* p, const double* rhs, double* out, const int64_t n[2],
+                          const double h2[2], const double inv_h2[2], double omega, int periodic,
+                          void* stream) {
+    return ch_stencil<double, 1>(p, rhs, out, n, h2, inv_h2, omega, periodic, hipStream_t(stream));
+}
+int afsi_mg_jacobi_2d_f32(const float* p, const float* rhs, float* out, const int64_t n[2],
+                          const double h2[2], const double inv_h2[2], double omega, int periodic,
+                          void* stream) {
+    return ch_stencil<float, 1>(p, rhs, out, n, h2, inv_h2, omega, periodic, hipStream_t(stream));
+}
+int afsi_mg_rbgs_2d_f64(double* p, const double* rhs, const int64_t n[2], const double h2[2],
+                        const double inv_h2[2], int periodic, void* stream) {
+    for (int color = 0; color < 2; ++color)
+        if (int r = ch_rbgs<double>(p, rhs, n, h2, inv_h2, color, periodic, hipStream_t(stream)))
+            return r;
+    return OK;
+}
+int afsi_mg_rbgs_2d_f32(float* p, const float* rhs, const int64_t n[2], const double h2[2],
+                        const double inv_h2[2], int periodic, void* stream) {
+    for (int color = 0; color < 2; ++color)
+        if (int r = ch_rbgs<float>(p, rhs, n, h2, inv_h2, color, periodic, hipStream_t(stream)))
+            return r;
+    return OK;
+}
+int afsi_mg_residual_restrict_2d_f64(const double* p, const double* rhs, double* coarse,
+                                     const int64_t n[2], const double h2[2], int periodic,
+                                     void* stream) {
+    return ch_residual_restrict<double>(p, rhs, coarse, n, h2, periodic, hipStream_t(stream));
+}
+int afsi_mg_residual_restrict_2d_f32(const float* p, const float* rhs, float* coarse,
+                                     const int64_t n[2], const double h2[2], int periodic,
+                                     void* stream) {
+    return ch_residual_restrict<float>(p, rhs, coarse, n, h2, periodic, hipStream_t(stream));
+}
+int afsi_mg_prolong_add_2d_f64(const double* coarse, double* fine, const int64_t n[2], int periodic,
+                               void* stream) {
+    return ch_prolong<double>(coarse, fine, n, periodic, hipStream_t(stream));
+}
+int afsi_mg_prolong_add_2d_f32(const float* coarse, float* fine, const int64_t n[2], int periodic,
+                               void* stream) {
+    return ch_prolong<float>(coarse, fine, n, periodic, hipStream_t(stream));
+}
+
+int afsi_divergence_2d_f64(const double* u, const double* v, double* out, const int64_t n[2],
+                           const double h[2], double scale, int periodic, void* stream) {
+    AFSI_REQUIRE(dims2_ok(n) && h && u && v && out, "invalid divergence arguments");
+    const Dim2 d{int(n[0]), int(n[1])};
+    const hipStream_t s = hipStream_t(stream);
+    const unsigned g = blocks_for(n[0] * n[1]);
+    if (periodic) ch_divergence_kernel<true><<<g, CH_THREADS, 0, s>>>(u, v, out, d, h[0], h[1], scale);
+    else ch_divergence_kernel<false><<<g, CH_THREADS, 0, s>>>(u, v, out, d, h[0], h[1], scale);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_grad_correct_2d_f64(double* u, double* v, const double* p, const int64_t n[2],
+                             const double h[2], double coef, int periodic, void* stream) {
+    AFSI_REQUIRE(dims2_ok(n) && h && u && v && p, "invalid gradient arguments");
+    const Dim2 d{int(n[0]), int(n[1])};
+    const hipStream_t s = hipStream_t(stream);
+    const unsigned g = blocks_for((n[0] + 1) * n[1] + n[0] * (n[1] + 1));
+    if (periodic) ch_grad_correct_kernel<true><<<g, CH_THREADS, 0, s>>>(u, v, p, d, h[0], h[1], coef);
+    else ch_grad_correct_kernel<false><<<g, CH_THREADS, 0, s>>>(u, v, p, d, h[0], h[1], coef);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_channel_prepare_f64(const double* x, int64_t points, const afsi_grid2* grid,
+                             int64_t* ids_u, double* w_u, int64_t* ids_v, double* w_v, int* flags,
+                             void* stream) {
+    AFSI_REQUIRE(grid && dims2_ok(grid->n) && points >= 0 && flags, "invalid channel prepare arguments");
+    AFSI_REQUIRE(grid->n[0] >= 4 && grid->n[1] >= 4, "channel grid needs >= 4 cells per axis");
+    if (points == 0) return OK;
+    AFSI_REQUIRE(x && ids_u && w_u && ids_v && w_v, "invalid channel prepare arguments");
+    const Dim2 d{int(grid->n[0]), int(grid->n[1])};
+    ch_prepare_kernel<<<blocks_for(2 * points), CH_THREADS, 0, hipStream_t(stream)>>>(
+        x, points, d, grid->h[0], grid->h[1], grid->lengths[0], grid->lengths[1], grid->periodic,
+        ids_u, w_u, ids_v, w_v, flags);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_mg2d_cycles_f64(const afsi_mg2d_plan* plan, int32_t cycles, void* stream) {
+    return channel_cycles<double>(plan, cycles, stream);
+}
+int afsi_mg2d_cycles_f32(const afsi_mg2d_plan* plan, int32_t cycles, void* stream) {
+    return channel_cycles<float>(plan, cycles, stream);
+}
+int afsi_mg2d_residual_f64(const afsi_mg2d_plan* plan, void* stream) {
+    return channel_residual<double>(plan, stream);
+}
+int afsi_mg2d_residual_f32(const afsi_mg2d_plan* plan, void* stream) {
+    return channel_residual<float>(plan, stream);
+}
+
+}  // extern "C"

@@ -256,6 +256,100 @@ int afsi_mg3d_cycles_unfused_f32(const afsi_mg3d_plan* plan, int32_t cycles, voi
 int afsi_mg3d_residual_unfused_f64(const afsi_mg3d_plan* plan, void* stream);
 int afsi_mg3d_residual_unfused_f32(const afsi_mg3d_plan* plan, void* stream);
 
+/* ==== 2D staggered channel (mac2d) ========================================
+ * Cells nx x ny, cell (x, y) at x*ny + y; faces u (nx+1, ny), v (nx, ny+1).
+ * periodic = 0: the reference channel (mac2d/grid.py:61-71: Neumann inlet
+ * and walls, p = 0 half a cell beyond the outlet, non-singular);
+ * periodic = 1 (build-own): periodic in x (u has nx faces), walls in y. */
+typedef struct afsi_grid2 {
+    int64_t n[2];        /* cells per axis */
+    double h[2];         /* spacing = length / n (mac2d/grid.py:19-21) */
+    double lengths[2];   /* channel lengths (support test of mac2d/transfer.py:88-95) */
+    int32_t periodic;
+    int32_t reserved;
+} afsi_grid2;
+
+/* Replaces TriangleTransfer._prepare (mac2d/transfer.py:61-81) for points
+ * (P,2): per component (P,16) int64 ids and FP64 weights, links k = 4i+j,
+ * rows outside the walls reflected with negated weights; periodic grids
+ * wrap x. Points outside the support region of mac2d/transfer.py:88-95
+ * (periodic: y only) set flag 1 and get zero weights. Spread and gather
+ * then use afsi_indexed_spread_* / afsi_indexed_gather_* per component
+ * (mac2d/transfer.py:108-127). */
+int afsi_channel_prepare_f64(const double* x, int64_t points, const afsi_grid2* grid,
+                             int64_t* ids_u, double* w_u, int64_t* ids_v, double* w_v, int* flags,
+                             void* stream);
+
+/* Multigrid kernels of the channel (mac2d/_triton_mg.py:24-75, bit-exact
+ * in FP64): residual out = rhs - A p; Jacobi out = p + (omega*(rhs-Ap))/diag
+ * (mac2d/multigrid.py:10-13, omega = 2/3); red-black GS (build-own, (x+y)
+ * even first, in place); residual-restriction to the 2:1 coarse level;
+ * ghost-based bilinear prolong-add. n = cells of the (fine) level. */
+int afsi_mg_residual_2d_f64(const double* p, const double* rhs, double* out, const int64_t n[2],
+                            const double h2[2], int periodic, void* stream);
+int afsi_mg_residual_2d_f32(const float* p, const float* rhs, float* out, const int64_t n[2],
+                            const double h2[2], int periodic, void* stream);
+int afsi_mg_jacobi_2d_f64(const double* p, const double* rhs, double* out, const int64_t n[2],
+                          const double h2[2], const double inv_h2[2], double omega, int periodic,
+                          void* stream);
+int afsi_mg_jacobi_2d_f32(const float* p, const float* rhs, float* out, const int64_t n[2],
+                          const double h2[2], const double inv_h2[2], double omega, int periodic,
+                          void* stream);
+int afsi_mg_rbgs_2d_f64(double* p, const double* rhs, const int64_t n[2], const double h2[2],
+                        const double inv_h2[2], int periodic, void* stream);
+int afsi_mg_rbgs_2d_f32(float* p, const float* rhs, const int64_t n[2], const double h2[2],
+                        const double inv_h2[2], int periodic, void* stream);
+int afsi_mg_residual_restrict_2d_f64(const double* p, const double* rhs, double* coarse,
+                                     const int64_t n[2], const double h2[2], int periodic,
+                                     void* stream);
+int afsi_mg_residual_restrict_2d_f32(const float* p, const float* rhs, float* coarse,
+                                     const int64_t n[2], const double h2[2], int periodic,
+                                     void* stream);
+int afsi_mg_prolong_add_2d_f64(const double* coarse, double* fine, const int64_t n[2], int periodic,
+                               void* stream);
+int afsi_mg_prolong_add_2d_f32(const float* coarse, float* fine, const int64_t n[2], int periodic,
+                               void* stream);
+
+/* Channel divergence out = scale*(du/hx + dv/hy) (mac2d/grid.py:46-48) and
+ * in-place correction u -= coef*G p with the outlet gradient -2p/hx
+ * (mac2d/grid.py:51-58; mac2d/flow.py:88-90). */
+int afsi_divergence_2d_f64(const double* u, const double* v, double* out, const int64_t n[2],
+                           const double h[2], double scale, int periodic, void* stream);
+int afsi_grad_correct_2d_f64(double* u, double* v, const double* p, const int64_t n[2],
+                             const double h[2], double coef, int periodic, void* stream);
+
+/* Replaces ChannelWorkspace.advance/residual_norm (mac2d/mg_workspace.py:
+ * 55-106) and ChannelMultigrid._cycle (mac2d/multigrid.py:65-81): cycles
+ * V(nu,nu) issued on one stream (capturable). Reference channel: no mean
+ * removal, coarsest = inverse(A) @ rhs. Periodic: means removed after each
+ * smoothing stage, from restricted residuals and from the coarsest solution
+ * (inverse(A + 11^T/n)), as mac/multigrid.py:107-123 does in 3D.
+ * tmp[l] is the Jacobi ping-pong buffer and, at level 0, the residual. */
+typedef struct afsi_mg2d_plan {
+    int32_t levels;
+    int32_t periodic;
+    int32_t smoother;        /* 0: weighted Jacobi (omega); 1: red-black GS */
+    int32_t nu;
+    double omega;
+    int64_t n[AFSI_MG_MAX_LEVELS][2];
+    double h2[AFSI_MG_MAX_LEVELS][2];
+    double inv_h2[AFSI_MG_MAX_LEVELS][2];
+    void* p[AFSI_MG_MAX_LEVELS];
+    void* rhs[AFSI_MG_MAX_LEVELS];
+    void* tmp[AFSI_MG_MAX_LEVELS];
+    const void* coarse_inverse;
+    int64_t coarse_cells;
+    double* reduce_partials;
+    double* reduce_result;
+    int64_t partials_capacity; /* doubles in reduce_partials */
+} afsi_mg2d_plan;
+int afsi_mg2d_cycles_f64(const afsi_mg2d_plan* plan, int32_t cycles, void* stream);
+int afsi_mg2d_cycles_f32(const afsi_mg2d_plan* plan, int32_t cycles, void* stream);
+/* reduce_result = [sum, sum of squares, sum |.|, non-finite count] of
+ * rhs[0] - A p[0]; the residual is left in tmp[0]. */
+int afsi_mg2d_residual_f64(const afsi_mg2d_plan* plan, void* stream);
+int afsi_mg2d_residual_f32(const afsi_mg2d_plan* plan, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,3 +25,25 @@ def ib_step(x, F, u, p_prev, grid, mg, *, dt, rho=1.):
     U = ib3d.gather_grid(velocity, base, phi, grid)
     return dict(force=f, velocity=velocity, pressure=p, point_velocity=U,
                 x=x + dt * U, info=info, tentative=star)
+
+
+def ib_step_2d(x, F, velocity, p_prev, channel, mg, *, dt, rho=1.):
+    """The channel step (mac2d/coupling.py:66-109 ordering, point forces
+    given): reflected stencil, spread, u* = u + (dt/rho) f with the
+    boundary() of mac2d/grid.py:74-79 (inlet face kept, wall normal faces
+    zero), projection, gather, x + dt U."""
+    from . import mac2d
+    if not mac2d.support_ok(x, channel):
+        raise ValueError('valve IB points leave the channel support region')
+    ids, w = mac2d.prepare(x, channel)
+    f = mac2d.spread(F, ids, w, channel)
+    u = velocity[0] + dt / rho * f[0]
+    v = velocity[1] + dt / rho * f[1]
+    if not channel.periodic:
+        u[0] = velocity[0][0]
+    v[:, 0] = 0
+    v[:, -1] = 0
+    (pu, pv), p, info = mac2d.project((u, v), channel.spacing, mg, dt=dt, rho=rho, initial=p_prev)
+    U = mac2d.gather((pu, pv), ids, w)
+    return dict(force=f, velocity=(pu, pv), pressure=p, point_velocity=U, x=x + dt * U,
+                info=info, tentative=(u, v), ids=ids, weights=w)
