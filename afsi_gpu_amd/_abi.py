@@ -34,7 +34,22 @@ class MGPlan3(Structure):
                 ('tmp', c_void_p * MAX_LEVELS), ('coarse_inverse', c_void_p),
                 ('coarse_cells', c_int64), ('reduce_partials', c_void_p),
                 ('reduce_result', c_void_p), ('partials_capacity', c_int64),
-                ('zero_initial', c_int32), ('reserved', c_int32)]
+                ('zero_initial', c_int32), ('reserved', c_int32),
+                ('sweep_tile', c_int32 * MAX_LEVELS), ('sweep_blocks', c_int32 * MAX_LEVELS)]
+
+
+class MGBItem(Structure):
+    _fields_ = [('plan', MGPlan3), ('x', c_void_p * 2), ('b', c_void_p)]
+
+
+class MGBBatch(Structure):
+    _fields_ = [('count', c_int32), ('reserved', c_int32), ('items', POINTER(MGBItem)),
+                ('workspace', c_void_p), ('workspace_bytes', c_size_t), ('active', c_void_p),
+                ('tol', c_void_p), ('results', c_void_p), ('partials', c_void_p)]
+
+
+def cast_items(array):
+    return ctypes.cast(array, POINTER(MGBItem))
 
 
 class Grid2(Structure):
@@ -72,7 +87,7 @@ _SIGNATURES = {
     'afsi_bin_points': (c_int, [_P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, c_size_t, _P, _P]),
     'afsi_spread_workspace_bytes': (c_size_t, [POINTER(Grid3), POINTER(c_int32)]),
     'afsi_spread_tiled_f64': (c_int, [_P, _P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, _P, _P, c_size_t, _P, _P]),
-    'afsi_interp_tiled_f64': (c_int, [_P, c_int64, POINTER(Grid3), _P, _P, _P, _P, _P, _P, _P]),
+    'afsi_interp_tiled_f64': (c_int, [_P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, _P, _P, _P, c_size_t, _P, _P]),
     'afsi_mg_residual_3d_f64': (c_int, [_P, _P, _P, POINTER(c_int64), POINTER(c_double), c_int, _P]),
     'afsi_mg_residual_3d_f32': (c_int, [_P, _P, _P, POINTER(c_int64), POINTER(c_double), c_int, _P]),
     'afsi_mg_jacobi_3d_f64': (c_int, [_P, _P, _P, POINTER(c_int64), POINTER(c_double), POINTER(c_double), c_double, c_int, _P]),
@@ -103,6 +118,7 @@ _SIGNATURES = {
     'afsi_mg3d_sweep_f32': (c_int, [POINTER(MGPlan3), c_int32, _P]),
     'afsi_refine_residual_3d': (c_int, [_P, _P, _P, POINTER(c_int64), POINTER(c_double), c_int, _P, c_int64, _P, _P]),
     'afsi_refine_update': (c_int, [_P, _P, c_int64, _P]),
+    'afsi_refine_step_3d': (c_int, [_P, _P, _P, _P, _P, POINTER(c_int64), POINTER(c_double), c_int, _P, c_int64, _P, _P]),
     'afsi_mg3d_cycles_f64': (c_int, [POINTER(MGPlan3), c_int32, _P]),
     'afsi_mg3d_cycles_f32': (c_int, [POINTER(MGPlan3), c_int32, _P]),
     'afsi_mg3d_residual_f64': (c_int, [POINTER(MGPlan3), _P]),
@@ -111,6 +127,12 @@ _SIGNATURES = {
     'afsi_mg3d_cycles_unfused_f32': (c_int, [POINTER(MGPlan3), c_int32, _P]),
     'afsi_mg3d_residual_unfused_f64': (c_int, [POINTER(MGPlan3), _P]),
     'afsi_mg3d_residual_unfused_f32': (c_int, [POINTER(MGPlan3), _P]),
+    'afsi_mgb_workspace_bytes': (c_size_t, [POINTER(MGBBatch)]),
+    'afsi_mgb_partials_doubles': (c_int64, [POINTER(MGBBatch)]),
+    'afsi_mgb_prepare': (c_int, [POINTER(MGBBatch), _P]),
+    'afsi_mgb_begin': (c_int, [POINTER(MGBBatch), c_double, c_double, _P]),
+    'afsi_mgb_iterate': (c_int, [POINTER(MGBBatch), c_int32, c_int32, _P]),
+    'afsi_mgb_finish': (c_int, [POINTER(MGBBatch), c_int32, _P]),
     'afsi_channel_prepare_f64': (c_int, [_P, c_int64, POINTER(Grid2), _P, _P, _P, _P, _P, _P]),
     'afsi_mg_residual_2d_f64': (c_int, [_P, _P, _P, _I64, _F64, c_int, _P]),
     'afsi_mg_residual_2d_f32': (c_int, [_P, _P, _P, _I64, _F64, c_int, _P]),

@@ -1,0 +1,225 @@
+"""Many independent IB instances stepped together (build-own).
+
+``BatchedMixedMultigrid`` solves the pressure equations of uneven 3D
+instances at once through ``afsi_mgb_*``: every multigrid phase (the
+red-black sweeps of a level, restriction, prolongation, the LDS-resident
+coarse levels, the FP64 refinement step) is one launch over a device work
+list of (instance, tile or row) entries, and convergence is tracked per
+instance on the device (converged instances skip their work; the host
+reads the flags once per ``check_every`` iterations). The solve keeps the
+contract of ``MixedMultigrid.solve`` per instance: FP32 V-cycles inside
+FP64 iterative refinement, tolerance max(atol, rtol ||b||), the same
+compatibility/finite checks and messages, cloned results.
+
+``BatchedIBStep`` advances a list of 3D instances by one IB step each:
+per-instance binning, spread and interpolation, then the batched pressure
+solve.
+"""
+from ctypes import c_double, c_void_p
+from math import isfinite
+import torch
+
+from . import _abi
+from .mac.flow import correct, divergence
+from .mac.multigrid import GeometricMultigrid, MGOptions
+from .mac.transfer import DEFAULT_TILE, Binner, TiledTransfer
+
+
+class BatchedMixedMultigrid:
+    def __init__(self, grids, *, device='cuda', options=None, graphs=True):
+        self.options = options or MGOptions(smooth=2, smoother='rbgs')
+        if self.options.smoother != 'rbgs':
+            raise ValueError('the batched solve uses the red-black smoother')
+        self.device = torch.device(device)
+        self.grids = list(grids)
+        if not self.grids:
+            raise ValueError('empty batch')
+        self.inner = []
+        self.x, self.b = [], []
+        for g in self.grids:
+            mg = GeometricMultigrid(g, device=self.device, dtype=torch.float32,
+                                    options=self.options, backend='workspace')
+            if not mg.fused:
+                raise ValueError(f'grid {g.shape} is not supported by the fused driver')
+            self.inner.append(mg)
+            shape = mg.shapes[0]
+            self.x.append([torch.zeros(shape, device=self.device, dtype=torch.float64)
+                           for _ in range(2)])
+            self.b.append(torch.zeros(shape, device=self.device, dtype=torch.float64))
+        n = len(self.grids)
+        self.items = (_abi.MGBItem * n)()
+        for i, mg in enumerate(self.inner):
+            self.items[i].plan = mg.plan
+            self.items[i].x[0] = self.x[i][0].data_ptr()
+            self.items[i].x[1] = self.x[i][1].data_ptr()
+            self.items[i].b = self.b[i].data_ptr()
+        self.active = torch.zeros(n, device=self.device, dtype=torch.int32)
+        self.tol = torch.zeros(n, device=self.device, dtype=torch.float64)
+        self.results = torch.zeros((n, 16), device=self.device, dtype=torch.float64)
+        bt = self._batch(None, 0, None)
+        lib = _abi.load()
+        nbytes = int(lib.afsi_mgb_workspace_bytes(bt))
+        parts = int(lib.afsi_mgb_partials_doubles(bt))
+        if nbytes <= 0 or parts <= 0:
+            bt.workspace_bytes = 1
+            _abi.call('afsi_mgb_prepare', bt, _abi.stream())  # raises with the reason
+        self.workspace = torch.empty(nbytes, device=self.device, dtype=torch.uint8)
+        self.partials = torch.zeros(parts, device=self.device, dtype=torch.float64)
+        self.batch = self._batch(self.workspace, nbytes, self.partials)
+        _abi.call('afsi_mgb_prepare', self.batch, _abi.stream())
+        self.cur = 0
+        self.graphs = {}
+        self.backend = 'hip-batched-mixed-graph' if graphs else 'hip-batched-mixed'
+        if graphs:
+            self._capture()
+
+    def _batch(self, workspace, nbytes, partials):
+        bt = _abi.MGBBatch()
+        bt.count = len(self.grids)
+        bt.items = _abi.cast_items(self.items)
+        bt.workspace = c_void_p(workspace.data_ptr()) if workspace is not None else None
+        bt.workspace_bytes = nbytes
+        bt.active = self.active.data_ptr()
+        bt.tol = self.tol.data_ptr()
+        bt.results = self.results.data_ptr()
+        bt.partials = partials.data_ptr() if partials is not None else None
+        return bt
+
+    @property
+    def allocated_bytes(self):
+        fp32 = sum(mg.allocated_bytes for mg in self.inner)
+        fp64 = sum(3 * b.numel() * 8 for b in self.b)
+        return fp32 + fp64 + self.workspace.numel()
+
+    def _iterate(self, start, count):
+        _abi.call('afsi_mgb_iterate', self.batch, int(start), int(count), _abi.stream())
+
+    def _capture(self):
+        stream = torch.cuda.Stream(device=self.device)
+        stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(stream):
+            _abi.call('afsi_mgb_begin', self.batch, c_double(self.options.rtol),
+                      c_double(self.options.atol), _abi.stream())
+            for start in (0, 1):
+                self._iterate(start, self.options.check_every)
+        torch.cuda.current_stream().wait_stream(stream)
+        for start in (0, 1):
+            for count in range(1, self.options.check_every + 1):
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph, stream=stream):
+                    self._iterate(start, count)
+                self.graphs[start, count] = graph
+        torch.cuda.current_stream().wait_stream(stream)
+        torch.cuda.synchronize(self.device)
+
+    def advance(self, count):
+        if self.graphs:
+            self.graphs[self.cur, count].replay()
+        else:
+            self._iterate(self.cur, count)
+        self.cur = (self.cur + count) % 2
+
+    @torch.no_grad()
+    def solve(self, rhs, initial=None):
+        """Per instance i: the FP64 solution of A p_i = rhs_i (compatible,
+        mean-free) and an info dict like ``MixedMultigrid.solve``'s."""
+        n = len(self.grids)
+        if len(rhs) != n or (initial is not None and len(initial) != n):
+            raise ValueError('one right-hand side (and initial guess) per instance')
+        for i in range(n):
+            r = rhs[i]
+            if r.shape != self.b[i].shape or r.device != self.device or r.dtype != torch.float64:
+                raise ValueError('invalid pressure RHS')
+            self.b[i].copy_(r)
+            if initial is None or initial[i] is None:
+                self.x[i][0].zero_()
+            else:
+                if initial[i].shape != r.shape or initial[i].dtype != r.dtype:
+                    raise ValueError('invalid initial pressure')
+                self.x[i][0].copy_(initial[i])
+        self.cur = 0
+        _abi.call('afsi_mgb_begin', self.batch, c_double(self.options.rtol),
+                  c_double(self.options.atol), _abi.stream())
+        res = self.results.tolist()
+        tol = self.tol.tolist()
+        infos = [None] * n
+        for i in range(n):
+            count = self.b[i].numel()
+            s1, a1, nf1 = res[i][4], res[i][6], res[i][7]
+            if nf1:
+                raise ValueError(f'invalid pressure RHS (instance {i})')
+            if abs(s1 / count) > 1e-12 + 1e-10 * a1 / count:
+                raise ValueError(f'incompatible Neumann pressure RHS: net flux must vanish (instance {i})')
+            if res[i][15]:
+                raise ValueError(f'invalid initial pressure (instance {i})')
+            if not isfinite(res[i][9]) or not isfinite(res[i][1]) or res[i][3]:
+                raise RuntimeError(f'nonfinite multigrid residual or RHS norm (instance {i})')
+        residual = [r[1] ** .5 for r in res]
+        cycles = [0] * n
+        active = self.active.tolist()
+        cycle = 0
+        while any(active) and cycle < self.options.max_cycles:
+            step = min(self.options.check_every, self.options.max_cycles - cycle)
+            self.advance(step)
+            cycle += step
+            now = self.active.tolist()
+            res = self.results.tolist()
+            for i in range(n):
+                if active[i]:
+                    cycles[i] = cycle
+                    residual[i] = res[i][1] ** .5
+                    if res[i][3] or not isfinite(residual[i]):
+                        raise RuntimeError(f'nonfinite multigrid residual (instance {i})')
+            active = now
+        for i in range(n):
+            if residual[i] > tol[i]:
+                raise RuntimeError(f'pressure multigrid did not converge (instance {i}): '
+                                   f'{residual[i]:g} > {tol[i]:g}')
+        _abi.call('afsi_mgb_finish', self.batch, int(self.cur), _abi.stream())
+        out = [self.x[i][self.cur].clone() for i in range(n)]
+        for i in range(n):
+            infos[i] = dict(cycles=cycles[i], residual_norm=residual[i], tolerance=tol[i],
+                            backend=self.backend)
+        return out, infos
+
+
+class BatchedIBStep:
+    """One IB step for each of many 3D instances (FP64 transfers, batched
+    mixed-precision pressure solve). ``grids[i]`` is a ``MACGrid`` and
+    ``counts[i]`` its number of points."""
+
+    def __init__(self, grids, counts, *, dt, rho=1., device='cuda', options=None,
+                 tile=DEFAULT_TILE, graphs=True):
+        self.grids = list(grids)
+        self.dt, self.rho = float(dt), float(rho)
+        self.device = torch.device(device)
+        self.solver = BatchedMixedMultigrid(self.grids, device=self.device, options=options,
+                                            graphs=graphs)
+        self.binners = [Binner(g, c, self.device, tile) for g, c in zip(self.grids, counts)]
+        self.transfers = [TiledTransfer(g, self.device, tile) for g in self.grids]
+        self.density = [g.zeros(device=self.device) for g in self.grids]
+        self.rhs = [torch.empty(g.shape, device=self.device, dtype=torch.float64) for g in self.grids]
+        self.point_velocity = [torch.empty((c, 3), device=self.device, dtype=torch.float64)
+                               for c in counts]
+
+    @torch.no_grad()
+    def step(self, xs, forces, velocities, pressures=None, *, advance=True):
+        a = self.dt / self.rho
+        bins = []
+        for i, g in enumerate(self.grids):
+            b = self.binners[i](xs[i])
+            bins.append(b)
+            f = self.transfers[i].spread(xs[i], forces[i], b, out=self.density[i])
+            for uc, fc in zip(velocities[i], f):
+                _abi.call('afsi_axpy_f64', _abi.ptr(uc), _abi.ptr(fc), uc.numel(), a, _abi.stream())
+            divergence(velocities[i], g, -self.rho / self.dt, out=self.rhs[i])
+        ps, infos = self.solver.solve(self.rhs, pressures)
+        Us = []
+        for i, g in enumerate(self.grids):
+            correct(velocities[i], ps[i], g, a)
+            U = self.transfers[i].interpolate(xs[i], velocities[i], bins[i], out=self.point_velocity[i])
+            if advance:
+                _abi.call('afsi_axpy_f64', _abi.ptr(xs[i]), _abi.ptr(U), xs[i].numel(), self.dt,
+                          _abi.stream())
+            Us.append(U)
+        return velocities, ps, Us, infos

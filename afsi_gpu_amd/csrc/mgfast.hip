@@ -23,7 +23,9 @@
 //   round-robin to the 8 XCDs, so each XCD is given one contiguous run of
 //   tiles and the halos it re-reads stay in its own L2.
 #include "mgcommon.hpp"
+#include <algorithm>
 #include <mutex>
+#include <vector>
 
 namespace afsi {
 
@@ -356,21 +358,26 @@ __device__ inline int wrap_or(int g, int n, bool& in) {
     return g;
 }
 
-// One red-black Gauss-Seidel sweep of a TX x TY column of rows spanning all
-// of z. LDS holds the (TX+4) x (TY+4) rows of p; red is updated on the
-// tile and its 1-row ring, then black on the tile. Lane r < ROWS first
+// One red-black Gauss-Seidel sweep of TX x TY columns of rows spanning all
+// of z. LDS holds the (TX+4) x (TY+4) rows of p of one tile; red is updated
+// on the tile and its 1-row ring, then black on the tile. Lane r < ROWS
 // writes the record of box row r (wrapped global offset, coordinates,
-// in-domain flag) to LDS; the row loops then read their record, so the
-// per-row bookkeeping is done once per row for the whole workgroup.
+// in-domain flag) to LDS, so per-row bookkeeping is done once per row.
+// The kernel is persistent: workgroup b sweeps tiles b, b + G, ... (XCD-
+// aware: each XCD owns a contiguous run of tiles, so halo re-reads stay in
+// its L2) and software-pipelined: once tile j is staged in LDS, the global
+// loads of tile j+1 are issued before the red and black phases of tile j,
+// so HBM traffic overlaps compute. G = ntiles gives one tile per workgroup.
 struct RowRec {
     int off, gx, gy, in;
 };
 
-template <class T, bool P, int TX, int TY>
-__global__ void __launch_bounds__(256) rb_rows_kernel(const T* __restrict__ pin,
-                                                      const T* __restrict__ rhs,
-                                                      T* __restrict__ pout, Dim32 d, W3<T> k,
-                                                      int zero_in, int ntx, int nty) {
+// Tiles first, first + slots, ... (count of them) of one level.
+template <class T, bool P, int TX, int TY, int N2>
+__device__ __forceinline__ void rb_rows_tiles(const T* __restrict__ pin, const T* __restrict__ rhs,
+                                              T* __restrict__ pout, const Dim32 d, const W3<T> k,
+                                              int zero_in, int ntx, int nty, int first, int count,
+                                              int slots) {
     constexpr int NW = 4;
     constexpr int BYr = TY + 4, ROWS = (TX + 4) * BYr;
     constexpr int RYr = TY + 2, RING = (TX + 2) * RYr;
@@ -380,80 +387,82 @@ __global__ void __launch_bounds__(256) rb_rows_kernel(const T* __restrict__ pin,
     extern __shared__ unsigned char raw[];
     T* box = reinterpret_cast<T*>(raw);
     __shared__ T tab[27];
-    __shared__ RowRec rec[ROWS];
-    const int n2 = d.n2, h2 = n2 >> 1;
+    __shared__ RowRec rec[2][ROWS];
+    // N2 > 0: the z extent is N2 and the x/y extents are multiples of TX/TY
+    // (compile-time lane tests and tile bounds); N2 = 0: any extents.
+    constexpr bool EXACT = N2 > 0;
+    const int n2 = N2 > 0 ? N2 : d.n2, h2 = n2 >> 1;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int tile = int(xcd_tile(blockIdx.x, int64_t(ntx) * nty));
-    const int ty = tile % nty, tx = tile / nty;
-    const int x0 = tx * TX - 2, y0 = ty * TY - 2;
+    if (wave >= NW) __builtin_unreachable();  // 256 threads = NW wavefronts
     const int sxg = d.n1 * n2;
-    if (threadIdx.x < ROWS) {
-        const int r = threadIdx.x;
-        int gx = x0 + r / BYr, gy = y0 + r % BYr;
-        int in = 1;
-        if (P) {
-            gx = gx < 0 ? gx + d.n0 : (gx >= d.n0 ? gx - d.n0 : gx);
-            gy = gy < 0 ? gy + d.n1 : (gy >= d.n1 ? gy - d.n1 : gy);
-        } else {
-            in = gx >= 0 && gx < d.n0 && gy >= 0 && gy < d.n1;
+    auto build = [&](int buf, int tile) {
+        if (threadIdx.x < ROWS) {
+            const int r = threadIdx.x;
+            int gx = (tile / nty) * TX - 2 + r / BYr, gy = (tile % nty) * TY - 2 + r % BYr;
+            int in = 1;
+            if (P) {
+                gx = gx < 0 ? gx + d.n0 : (gx >= d.n0 ? gx - d.n0 : gx);
+                gy = gy < 0 ? gy + d.n1 : (gy >= d.n1 ? gy - d.n1 : gy);
+            } else {
+                in = gx >= 0 && gx < d.n0 && gy >= 0 && gy < d.n1;
+            }
+            rec[buf][r] = RowRec{in ? gx * sxg + gy * n2 : 0, gx, gy, in};
         }
-        rec[r] = RowRec{in ? gx * sxg + gy * n2 : 0, gx, gy, in};
-    }
-    if (!P && threadIdx.x < 27) tab[threadIdx.x] = inv_diag_entry<T>(threadIdx.x, k);
-    __syncthreads();
-    auto load_rec = [&](int r) {
-        RowRec q = rec[r];
+    };
+    auto load_rec = [&](int buf, int r) {
+        RowRec q = rec[buf][r];
         q.off = __builtin_amdgcn_readfirstlane(q.off);
         q.gx = __builtin_amdgcn_readfirstlane(q.gx);
         q.gy = __builtin_amdgcn_readfirstlane(q.gy);
         q.in = __builtin_amdgcn_readfirstlane(q.in);
         return q;
     };
-    T pv[LR][2];
+    T pv[LR][2], rr[RR], rt[TR];
+    // global loads of one tile into registers (records in rec[buf])
+    auto fetch = [&](int buf, int tile) {
+        const int x0 = (tile / nty) * TX - 2, y0 = (tile % nty) * TY - 2;
 #pragma unroll
-    for (int q = 0; q < LR; ++q) {
-        const int r = wave + q * NW;
-        pv[q][0] = pv[q][1] = T(0);
-        if (r < ROWS && !zero_in) {
-            const RowRec rc = load_rec(r);
-            if (rc.in) {
-                const T* src = pin + rc.off;
-                if (lane < n2) pv[q][0] = src[lane];
-                if (lane + 64 < n2) pv[q][1] = src[lane + 64];
+        for (int q = 0; q < LR; ++q) pv[q][0] = pv[q][1] = T(0);
+        if (!zero_in) {
+#pragma unroll
+            for (int q = 0; q < LR; ++q) {
+                const int r = wave + q * NW;
+                if (r < ROWS) {
+                    const RowRec rc = load_rec(buf, r);
+                    if (P || rc.in) {
+                        const T* src = pin + rc.off;
+                        if (lane < n2) pv[q][0] = src[lane];
+                        if (lane + 64 < n2) pv[q][1] = src[lane + 64];
+                    }
+                }
             }
         }
-    }
-    T rr[RR], rt[TR];
 #pragma unroll
-    for (int q = 0; q < RR; ++q) {
-        const int rw = wave + q * NW;
-        rr[q] = T(0);
-        if (rw < RING && lane < h2) {
-            const RowRec rc = load_rec((1 + rw / RYr) * BYr + 1 + rw % RYr);
-            if (rc.in) rr[q] = rhs[rc.off + 2 * lane + ((rc.gx + rc.gy) & 1)];
+        for (int q = 0; q < RR; ++q) {
+            const int rw = wave + q * NW;
+            rr[q] = T(0);
+            if (rw < RING && lane < h2) {
+                const RowRec rc = load_rec(buf, (1 + rw / RYr) * BYr + 1 + rw % RYr);
+                if (P || rc.in) rr[q] = rhs[rc.off + 2 * lane + ((rc.gx + rc.gy) & 1)];
+            }
         }
-    }
 #pragma unroll
-    for (int q = 0; q < TR; ++q) {
-        const int tw = wave + q * NW;
-        rt[q] = T(0);
-        if (tw < TILE && lane < h2) {
-            const RowRec rc = load_rec((2 + tw / TY) * BYr + 2 + tw % TY);
-            if (x0 + 2 + tw / TY < d.n0 && y0 + 2 + tw % TY < d.n1)
-                rt[q] = rhs[rc.off + 2 * lane + ((rc.gx + rc.gy + 1) & 1)];
+        for (int q = 0; q < TR; ++q) {
+            const int tw = wave + q * NW;
+            rt[q] = T(0);
+            if (tw < TILE && lane < h2) {
+                const RowRec rc = load_rec(buf, (2 + tw / TY) * BYr + 2 + tw % TY);
+                if (EXACT || (x0 + 2 + tw / TY < d.n0 && y0 + 2 + tw % TY < d.n1))
+                    rt[q] = rhs[rc.off + 2 * lane + ((rc.gx + rc.gy + 1) & 1)];
+            }
         }
-    }
-#pragma unroll
-    for (int q = 0; q < LR; ++q) {
-        const int r = wave + q * NW;
-        if (r < ROWS) {
-            if (lane < n2) box[r * n2 + lane] = pv[q][0];
-            if (lane + 64 < n2) box[r * n2 + lane + 64] = pv[q][1];
-        }
-    }
-    const T inv_p = T(1) / (T(2) * (k.w[0] + k.w[1] + k.w[2]));
+    };
+    build(0, first);
+    if (!P && threadIdx.x < 27) tab[threadIdx.x] = inv_diag_entry<T>(threadIdx.x, k);
     __syncthreads();
+    fetch(0, first);
+    const T inv_p = T(1) / (T(2) * (k.w[0] + k.w[1] + k.w[2]));
     auto update = [&](const T* row, int r, int gx, int gy, int z, T b) -> T {
         T s = T(0);
         if (P) {
@@ -470,31 +479,71 @@ __global__ void __launch_bounds__(256) rb_rows_kernel(const T* __restrict__ pin,
         const T inv = P ? inv_p : tab[face_class(gx, d.n0) * 9 + face_class(gy, d.n1) * 3 + face_class(z, n2)];
         return (b + s) * inv;
     };
+    for (int j = 0; j < count; ++j) {
+        const int cur = j & 1;
+        const int tile = first + j * slots;
+        const int x0 = (tile / nty) * TX - 2, y0 = (tile % nty) * TY - 2;
 #pragma unroll
-    for (int q = 0; q < RR; ++q) {
-        const int rw = wave + q * NW;
-        if (rw >= RING || lane >= h2) continue;
-        const int r = (1 + rw / RYr) * BYr + 1 + rw % RYr;
-        const RowRec rc = load_rec(r);
-        if (!rc.in) continue;
-        T* row = box + r * n2;
-        const int z = 2 * lane + ((rc.gx + rc.gy) & 1);
-        row[z] = update(row, r, rc.gx, rc.gy, z, rr[q]);
-    }
-    __syncthreads();
+        for (int q = 0; q < LR; ++q) {
+            const int r = wave + q * NW;
+            if (r < ROWS) {
+                if (lane < n2) box[r * n2 + lane] = pv[q][0];
+                if (lane + 64 < n2) box[r * n2 + lane + 64] = pv[q][1];
+            }
+        }
+        T rrc[RR], rtc[TR];
 #pragma unroll
-    for (int q = 0; q < TR; ++q) {
-        const int tw = wave + q * NW;
-        if (tw >= TILE || lane >= h2) continue;
-        if (x0 + 2 + tw / TY >= d.n0 || y0 + 2 + tw % TY >= d.n1) continue;
-        const int r = (2 + tw / TY) * BYr + 2 + tw % TY;
-        const RowRec rc = load_rec(r);
-        const T* row = box + r * n2;
-        const int zb = 2 * lane + ((rc.gx + rc.gy + 1) & 1), zr = 2 * lane + ((rc.gx + rc.gy) & 1);
-        T* dst = pout + rc.off;
-        dst[zb] = update(row, r, rc.gx, rc.gy, zb, rt[q]);
-        dst[zr] = row[zr];
+        for (int q = 0; q < RR; ++q) rrc[q] = rr[q];
+#pragma unroll
+        for (int q = 0; q < TR; ++q) rtc[q] = rt[q];
+        const bool more = j + 1 < count;
+        if (more) build(cur ^ 1, tile + slots);
+        __syncthreads();
+        if (more) fetch(cur ^ 1, tile + slots);
+#pragma unroll
+        for (int q = 0; q < RR; ++q) {
+            const int rw = wave + q * NW;
+            if (rw >= RING || lane >= h2) continue;
+            const int r = (1 + rw / RYr) * BYr + 1 + rw % RYr;
+            const RowRec rc = load_rec(cur, r);
+            if (!P && !rc.in) continue;
+            T* row = box + r * n2;
+            const int z = 2 * lane + ((rc.gx + rc.gy) & 1);
+            row[z] = update(row, r, rc.gx, rc.gy, z, rrc[q]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < TR; ++q) {
+            const int tw = wave + q * NW;
+            if (tw >= TILE || lane >= h2) continue;
+            if (!EXACT && (x0 + 2 + tw / TY >= d.n0 || y0 + 2 + tw % TY >= d.n1)) continue;
+            const int r = (2 + tw / TY) * BYr + 2 + tw % TY;
+            const RowRec rc = load_rec(cur, r);
+            const T* row = box + r * n2;
+            const int zb = 2 * lane + ((rc.gx + rc.gy + 1) & 1), zr = 2 * lane + ((rc.gx + rc.gy) & 1);
+            T* dst = pout + rc.off;
+            dst[zb] = update(row, r, rc.gx, rc.gy, zb, rtc[q]);
+            dst[zr] = row[zr];
+        }
+        __syncthreads();
     }
+}
+
+template <class T, bool P, int TX, int TY, int N2>
+__global__ void __launch_bounds__(256) rb_rows_kernel(const T* __restrict__ pin,
+                                                      const T* __restrict__ rhs,
+                                                      T* __restrict__ pout, Dim32 d, W3<T> k,
+                                                      int zero_in, int ntx, int nty) {
+    // tile sequence of this workgroup
+    const int ntiles = ntx * nty;
+    const bool by_xcd = (gridDim.x % 8 == 0) && (ntiles % 8 == 0);
+    const int per = by_xcd ? ntiles / 8 : ntiles;
+    const int slots = by_xcd ? int(gridDim.x) / 8 : int(gridDim.x);
+    const int slot = by_xcd ? int(blockIdx.x) / 8 : int(blockIdx.x);
+    const int first = (by_xcd ? int(blockIdx.x % 8) * per : 0) + slot;
+    const int count = slot < per ? (per - slot + slots - 1) / slots : 0;
+    if (count == 0) return;  // uniform over the workgroup
+    rb_rows_tiles<T, P, TX, TY, N2>(pin, rhs, pout, d, k, zero_in, ntx, nty, first, count, slots);
 }
 
 // Pointers to a row and its four x/y neighbour rows (null where a Neumann
@@ -548,14 +597,11 @@ __device__ inline T apply_row(const Rows5<T, P>& r, int z, int n2, const W3<T>& 
 
 // coarse = avg8(rhs - A p); one wavefront per coarse row.
 template <class T, bool P>
-__global__ void __launch_bounds__(256) restrict_rows_kernel(const T* __restrict__ p,
-                                                            const T* __restrict__ rhs,
-                                                            T* __restrict__ coarse, Dim32 d,
-                                                            W3<T> k) {
+__device__ __forceinline__ void restrict_row(const T* __restrict__ p, const T* __restrict__ rhs,
+                                             T* __restrict__ coarse, const Dim32 d, const W3<T> k,
+                                             int row) {
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int C0 = d.n0 / 2, C1 = d.n1 / 2, C2 = d.n2 / 2;
-    const int row = blockIdx.x * 4 + wave;
     if (row >= C0 * C1) return;
     const int cx = row / C1, cy = row - cx * C1;
     Rows5<T, P> pr[2][2];
@@ -583,13 +629,20 @@ __global__ void __launch_bounds__(256) restrict_rows_kernel(const T* __restrict_
     }
 }
 
+template <class T, bool P>
+__global__ void __launch_bounds__(256) restrict_rows_kernel(const T* __restrict__ p,
+                                                            const T* __restrict__ rhs,
+                                                            T* __restrict__ coarse, Dim32 d,
+                                                            W3<T> k) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    restrict_row<T, P>(p, rhs, coarse, d, k, int(blockIdx.x) * 4 + wave);
+}
+
 // fine += trilinear(coarse); one wavefront per fine row.
 template <class T, bool P>
-__global__ void __launch_bounds__(256) prolong_rows_kernel(const T* __restrict__ coarse,
-                                                           T* __restrict__ fine, Dim32 d) {
+__device__ __forceinline__ void prolong_row(const T* __restrict__ coarse, T* __restrict__ fine,
+                                            const Dim32 d, int row) {
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-    const int row = blockIdx.x * 4 + wave;
     if (row >= d.n0 * d.n1) return;
     const int x = row / d.n1, y = row - x * d.n1;
     const int C0 = d.n0 / 2, C1 = d.n1 / 2, C2 = d.n2 / 2;
@@ -621,6 +674,13 @@ __global__ void __launch_bounds__(256) prolong_rows_kernel(const T* __restrict__
         const T v11 = c11[z0] * (T(1) - wz) + c11[z1] * wz;
         f[z] = f[z] + ((v00 * (T(1) - wy) + v01 * wy) * (T(1) - wx) + (v10 * (T(1) - wy) + v11 * wy) * wx);
     }
+}
+
+template <class T, bool P>
+__global__ void __launch_bounds__(256) prolong_rows_kernel(const T* __restrict__ coarse,
+                                                           T* __restrict__ fine, Dim32 d) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    prolong_row<T, P>(coarse, fine, d, int(blockIdx.x) * 4 + wave);
 }
 
 // per-workgroup [sum r, sum r^2, sum |r|, non-finite] of r = rhs - A p,
@@ -707,6 +767,76 @@ __global__ void __launch_bounds__(256) refine_residual_kernel(const double* __re
     }
 }
 
+// One refinement step in one pass: x_out = x + e (FP64 + FP32 correction)
+// and r = b - A x_out in FP64, with the neighbours' x + e formed on the fly
+// from the same expression that writes them, so the result is identical to
+// an update pass followed by a residual pass. Reduce: also the residual's
+// block partials [sum, sum of squares, sum |.|, non-finite].
+template <bool P, bool Reduce>
+__global__ void __launch_bounds__(256) refine_step_kernel(const double* __restrict__ x,
+                                                          const float* __restrict__ e,
+                                                          const double* __restrict__ b,
+                                                          double* __restrict__ xo,
+                                                          float* __restrict__ r32, Dim32 d,
+                                                          W3<double> k,
+                                                          double* __restrict__ partials) {
+    __shared__ double sh[4][4];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int row = blockIdx.x * 4 + wave; row < d.n0 * d.n1; row += gridDim.x * 4) {
+        const int xx = row / d.n1, yy = row - xx * d.n1;
+        const Rows5<double, P> px = rows5<double, P>(x, xx, yy, d);
+        const Rows5<float, P> pe = rows5<float, P>(e, xx, yy, d);
+        const int off = row * d.n2;
+        for (int z = lane; z < d.n2; z += 64) {
+            const double c = px.c[z] + double(pe.c[z]);
+            double a = 0.0;
+            if (P) {
+                a += k.w[0] * (c - (px.xp[z] + double(pe.xp[z])));
+                a += k.w[0] * (c - (px.xm[z] + double(pe.xm[z])));
+                a += k.w[1] * (c - (px.yp[z] + double(pe.yp[z])));
+                a += k.w[1] * (c - (px.ym[z] + double(pe.ym[z])));
+                const int zp = z + 1 < d.n2 ? z + 1 : 0, zm = z > 0 ? z - 1 : d.n2 - 1;
+                a += k.w[2] * (c - (px.c[zp] + double(pe.c[zp])));
+                a += k.w[2] * (c - (px.c[zm] + double(pe.c[zm])));
+            } else {
+                if (px.xp) a += k.w[0] * (c - (px.xp[z] + double(pe.xp[z])));
+                if (px.xm) a += k.w[0] * (c - (px.xm[z] + double(pe.xm[z])));
+                if (px.yp) a += k.w[1] * (c - (px.yp[z] + double(pe.yp[z])));
+                if (px.ym) a += k.w[1] * (c - (px.ym[z] + double(pe.ym[z])));
+                if (z + 1 < d.n2) a += k.w[2] * (c - (px.c[z + 1] + double(pe.c[z + 1])));
+                if (z > 0) a += k.w[2] * (c - (px.c[z - 1] + double(pe.c[z - 1])));
+            }
+            const double r = b[off + z] - a;
+            xo[off + z] = c;
+            r32[off + z] = float(r);
+            if (Reduce) {
+                if (isfinite(r)) {
+                    v[0] += r;
+                    v[1] += r * r;
+                    v[2] += fabs(r);
+                } else {
+                    v[3] += 1.0;
+                }
+            }
+        }
+    }
+    if (!Reduce) return;
+    for (int q = 0; q < 4; ++q) {
+        double t = v[q];
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1) t += __shfl_xor(t, dd, WAVE);
+        if (lane == 0) sh[q][wave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (int w = 0; w < 4; ++w) t += sh[threadIdx.x][w];
+        partials[4 * blockIdx.x + threadIdx.x] = t;
+    }
+}
+
 __global__ void refine_update_kernel(double* __restrict__ x, const float* __restrict__ e,
                                      int64_t count) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
@@ -715,7 +845,13 @@ __global__ void refine_update_kernel(double* __restrict__ x, const float* __rest
 }
 
 // ---------------------------------------------------------------------------
-// single-workgroup V-cycle over the coarse levels, all arrays in LDS.
+// single-workgroup V-cycle over the coarse levels, all arrays in LDS. Like
+// the fine levels it removes no means: the operator annihilates constants,
+// smoothing and prolongation carry a constant unchanged, restricted
+// residuals of a zero-mean right-hand side have zero mean, and the coarsest
+// inv(A + 11^T/n) solve returns the mean of its right-hand side, so the
+// reference's mean subtractions change only rounding; the caller removes
+// the level-0 mean once after the last cycle.
 // Work items are enumerated per colour (lanes map to the red or the black
 // cells of rows directly) and decoded with float-reciprocal division plus
 // an exact integer correction; 1/diag comes from a per-level table.
@@ -733,6 +869,7 @@ struct CoarseArgs {
     T* p_out;          // mean-free result of the first handled level
     const T* coarse_inverse;
     int32_t coarse_cells;
+    const int32_t* active;  // batched launches: skip when *active == 0 (null: always run)
 };
 
 // floor(e / d) for 0 <= e < 2^22, d >= 1
@@ -741,28 +878,6 @@ __device__ inline int qdiv(int e, int d) {
     if (q * d > e) --q;
     else if ((q + 1) * d <= e) ++q;
     return q;
-}
-
-__device__ inline double block_sum_all(double v, double* scratch) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    __syncthreads();
-    if (lane == 0) scratch[wave] = v;
-    __syncthreads();
-    double t = 0.0;
-#pragma unroll
-    for (int w = 0; w < COARSE_THREADS / WAVE; ++w) t += scratch[w];
-    return t;
-}
-
-template <class T>
-__device__ inline void lds_remove_mean(T* a, int n, double* scratch) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += COARSE_THREADS) s += double(a[i]);
-    const T m = T(block_sum_all(s, scratch) / double(n));
-    for (int i = threadIdx.x; i < n; i += COARSE_THREADS) a[i] = a[i] - m;
-    __syncthreads();
 }
 
 // sum_nb w_a p_nb (and, for Neumann, the boundary class index) at (x,y,z).
@@ -871,10 +986,9 @@ __device__ inline void lds_smooth(T* p, T* w, const T* r, const uint32_t* list, 
 }
 
 template <class T, bool P>
-__global__ void __launch_bounds__(COARSE_THREADS) coarse_cycle_kernel(CoarseArgs<T> a) {
+__device__ __forceinline__ void coarse_cycle_body(const CoarseArgs<T>& a) {
     extern __shared__ unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
-    __shared__ double scratch[COARSE_THREADS / WAVE];
     __shared__ T tabs[AFSI_MG_MAX_LEVELS][27];
     __shared__ uint32_t lists[COARSE_LIST];
     __shared__ int list_off[AFSI_MG_MAX_LEVELS];
@@ -911,7 +1025,6 @@ __global__ void __launch_bounds__(COARSE_THREADS) coarse_cycle_kernel(CoarseArgs
         T* w = smem + a.off_w[l];
         const T inv_p = T(1) / (T(2) * (k.w[0] + k.w[1] + k.w[2]));
         lds_smooth<T, P>(p, w, r, lists + list_off[l], d, k, tabs[l], inv_p, a.smoother, a.nu, a.omega);
-        lds_remove_mean(p, d.n0 * d.n1 * d.n2, scratch);
         const Dim32& c = a.d[l + 1];
         T* rc = smem + a.off_r[l + 1];
         T* pc = smem + a.off_p[l + 1];
@@ -933,7 +1046,6 @@ __global__ void __launch_bounds__(COARSE_THREADS) coarse_cycle_kernel(CoarseArgs
             pc[i] = T(0);
         }
         __syncthreads();
-        lds_remove_mean(rc, nc, scratch);
     }
     {
         const Dim32& d = a.d[L - 1];
@@ -953,7 +1065,6 @@ __global__ void __launch_bounds__(COARSE_THREADS) coarse_cycle_kernel(CoarseArgs
             if (row < m && lane == 0) p[row] = s;
         }
         __syncthreads();
-        lds_remove_mean(p, d.n0 * d.n1 * d.n2, scratch);
     }
     for (int l = L - 2; l >= 0; --l) {
         const Dim32& d = a.d[l];
@@ -972,7 +1083,6 @@ __global__ void __launch_bounds__(COARSE_THREADS) coarse_cycle_kernel(CoarseArgs
         __syncthreads();
         const T inv_p = T(1) / (T(2) * (k.w[0] + k.w[1] + k.w[2]));
         lds_smooth<T, P>(p, w, r, list, d, k, tabs[l], inv_p, a.smoother, a.nu, a.omega);
-        lds_remove_mean(p, n, scratch);
     }
     {
         const Dim32& d = a.d[0];
@@ -981,6 +1091,248 @@ __global__ void __launch_bounds__(COARSE_THREADS) coarse_cycle_kernel(CoarseArgs
         for (int i = threadIdx.x; i < n; i += COARSE_THREADS) a.p_out[i] = p[i];
     }
 }
+
+template <class T, bool P>
+__global__ void __launch_bounds__(COARSE_THREADS) coarse_cycle_kernel(CoarseArgs<T> a) {
+    coarse_cycle_body<T, P>(a);
+}
+
+// ---------------------------------------------------------------------------
+// batched kernels: entry i of a launch's descriptor array covers workgroups
+// [begin_i, begin_{i+1}); workgroup b finds its entry by binary search over
+// the (few) entries with wave-uniform scalar loads.
+
+template <class D>
+__device__ inline int find_entry(const D* __restrict__ desc, int n, int b) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (desc[mid].begin <= b) lo = mid;
+        else hi = mid;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+template <class T>
+struct SweepDesc {
+    const T* pin;
+    const T* rhs;
+    T* pout;
+    const int32_t* active;
+    Dim32 d;
+    W3<T> k;
+    int ntx, nty;
+    int begin;
+    int zero_in;
+};
+
+template <class T>
+struct RowDesc {
+    const T* a;  // restriction: p; prolongation: coarse p
+    const T* b;  // restriction: rhs
+    T* c;        // restriction: coarse rhs; prolongation: fine p
+    const int32_t* active;
+    Dim32 d;     // fine level
+    W3<T> k;
+    int begin;
+};
+
+struct RefDesc {
+    const double* x;
+    const float* e;
+    const double* b;
+    double* xo;
+    float* r32;
+    const int32_t* active;
+    Dim32 d;
+    W3<double> k;
+    int begin;
+};
+
+struct VecDesc {
+    double* x;
+    int n;
+    int begin;
+};
+
+constexpr int VEC_CHUNK = 4096;  // elements per workgroup in batched vector passes
+
+template <class T, bool P>
+__global__ void __launch_bounds__(256) rb_rows_batch_kernel(const SweepDesc<T>* __restrict__ desc,
+                                                            int n) {
+    const int b = blockIdx.x;
+    const SweepDesc<T>& s = desc[find_entry(desc, n, b)];
+    if (s.active && s.active[0] == 0) return;  // uniform over the workgroup
+    rb_rows_tiles<T, P, 2, 2, 0>(s.pin, s.rhs, s.pout, s.d, s.k, s.zero_in, s.ntx, s.nty,
+                                 b - s.begin, 1, 1);
+}
+
+template <class T, bool P>
+__global__ void __launch_bounds__(256) restrict_batch_kernel(const RowDesc<T>* __restrict__ desc,
+                                                             int n) {
+    const int b = blockIdx.x;
+    const RowDesc<T>& s = desc[find_entry(desc, n, b)];
+    if (s.active && s.active[0] == 0) return;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    restrict_row<T, P>(s.a, s.b, s.c, s.d, s.k, (b - s.begin) * 4 + wave);
+}
+
+template <class T, bool P>
+__global__ void __launch_bounds__(256) prolong_batch_kernel(const RowDesc<T>* __restrict__ desc,
+                                                            int n) {
+    const int b = blockIdx.x;
+    const RowDesc<T>& s = desc[find_entry(desc, n, b)];
+    if (s.active && s.active[0] == 0) return;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    prolong_row<T, P>(s.a, s.c, s.d, (b - s.begin) * 4 + wave);
+}
+
+template <class T, bool P>
+__global__ void __launch_bounds__(COARSE_THREADS) coarse_batch_kernel(const CoarseArgs<T>* __restrict__ args) {
+    const CoarseArgs<T>& a = args[blockIdx.x];
+    if (a.active && a.active[0] == 0) return;
+    coarse_cycle_body<T, P>(a);
+}
+
+__device__ inline void block_stats(double v[4], double* __restrict__ out) {
+    __shared__ double sh[4][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int q = 0; q < 4; ++q) {
+        double t = v[q];
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1) t += __shfl_xor(t, dd, WAVE);
+        if (lane == 0) sh[q][wave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (int w = 0; w < 4; ++w) t += sh[threadIdx.x][w];
+        out[threadIdx.x] = t;
+    }
+}
+
+__device__ inline void add_stats(double v[4], double r) {
+    if (isfinite(r)) {
+        v[0] += r;
+        v[1] += r * r;
+        v[2] += fabs(r);
+    } else {
+        v[3] += 1.0;
+    }
+}
+
+// Step: x_out = x + e and r = b - A x_out (one wavefront per row);
+// otherwise r = b - A x. Inactive instances copy x to x_out (Step).
+template <bool P, bool Step, bool Reduce>
+__global__ void __launch_bounds__(256) refine_batch_kernel(const RefDesc* __restrict__ desc, int n,
+                                                           double* __restrict__ partials) {
+    const int b = blockIdx.x;
+    const RefDesc& s = desc[find_entry(desc, n, b)];
+    const bool live = !(s.active && s.active[0] == 0);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const Dim32 d = s.d;
+    const W3<double> k = s.k;
+    const int row = (b - s.begin) * 4 + wave;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (row < d.n0 * d.n1) {
+        const int off = row * d.n2;
+        if (!live) {
+            if (Step)
+                for (int z = lane; z < d.n2; z += 64) s.xo[off + z] = s.x[off + z];
+        } else {
+            const int xx = row / d.n1, yy = row - xx * d.n1;
+            const Rows5<double, P> px = rows5<double, P>(s.x, xx, yy, d);
+            const Rows5<float, P> pe = rows5<float, P>(s.e, xx, yy, d);
+            auto val = [&](const double* xr, const float* er, int z) -> double {
+                return Step ? xr[z] + double(er[z]) : xr[z];
+            };
+            for (int z = lane; z < d.n2; z += 64) {
+                const double c = val(px.c, pe.c, z);
+                double a = 0.0;
+                if (P) {
+                    a += k.w[0] * (c - val(px.xp, pe.xp, z));
+                    a += k.w[0] * (c - val(px.xm, pe.xm, z));
+                    a += k.w[1] * (c - val(px.yp, pe.yp, z));
+                    a += k.w[1] * (c - val(px.ym, pe.ym, z));
+                    const int zp = z + 1 < d.n2 ? z + 1 : 0, zm = z > 0 ? z - 1 : d.n2 - 1;
+                    a += k.w[2] * (c - val(px.c, pe.c, zp));
+                    a += k.w[2] * (c - val(px.c, pe.c, zm));
+                } else {
+                    if (px.xp) a += k.w[0] * (c - val(px.xp, pe.xp, z));
+                    if (px.xm) a += k.w[0] * (c - val(px.xm, pe.xm, z));
+                    if (px.yp) a += k.w[1] * (c - val(px.yp, pe.yp, z));
+                    if (px.ym) a += k.w[1] * (c - val(px.ym, pe.ym, z));
+                    if (z + 1 < d.n2) a += k.w[2] * (c - val(px.c, pe.c, z + 1));
+                    if (z > 0) a += k.w[2] * (c - val(px.c, pe.c, z - 1));
+                }
+                const double r = s.b[off + z] - a;
+                if (Step) s.xo[off + z] = c;
+                s.r32[off + z] = float(r);
+                if (Reduce) add_stats(v, r);
+            }
+        }
+    }
+    if (Reduce) block_stats(v, partials + 4 * b);
+}
+
+// Per instance: sum its workgroups' partials in a fixed order, store the
+// stats at results[16 i + slot]; slot 0 (the residual) also updates
+// active[i] (0 once the residual meets tol[i] or is non-finite). Frozen
+// (inactive) instances are left untouched.
+template <class D>
+__global__ void __launch_bounds__(64) batch_reduce_kernel(const D* __restrict__ desc, int n,
+                                                          int total_blocks,
+                                                          const double* __restrict__ partials,
+                                                          double* __restrict__ results, int slot,
+                                                          const double* __restrict__ tol,
+                                                          int32_t* __restrict__ active) {
+    const int i = blockIdx.x;
+    if (slot == 0 && active[i] == 0) return;
+    const int beg = desc[i].begin, end = i + 1 < n ? desc[i + 1].begin : total_blocks;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = beg + int(threadIdx.x); b < end; b += 64)
+        for (int q = 0; q < 4; ++q) v[q] += partials[4 * b + q];
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1) v[q] += __shfl_xor(v[q], dd, WAVE);
+    }
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 4; ++q) results[16 * i + slot + q] = v[q];
+        if (slot == 0 && (v[3] > 0.0 || !(sqrt(v[1]) > tol[i]))) active[i] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(256) vec_partials_kernel(const VecDesc* __restrict__ desc, int n,
+                                                           double* __restrict__ partials) {
+    const int b = blockIdx.x;
+    const VecDesc& s = desc[find_entry(desc, n, b)];
+    const int lo = (b - s.begin) * VEC_CHUNK, hi = min(s.n, lo + VEC_CHUNK);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = lo + int(threadIdx.x); j < hi; j += 256) add_stats(v, s.x[j]);
+    block_stats(v, partials + 4 * b);
+}
+
+__global__ void __launch_bounds__(256) vec_sub_mean_kernel(const VecDesc* __restrict__ desc, int n,
+                                                           const double* __restrict__ results,
+                                                           int slot) {
+    const int b = blockIdx.x;
+    const int i = find_entry(desc, n, b);
+    const VecDesc& s = desc[i];
+    const double m = results[16 * i + slot] / double(s.n);
+    const int lo = (b - s.begin) * VEC_CHUNK, hi = min(s.n, lo + VEC_CHUNK);
+    for (int j = lo + int(threadIdx.x); j < hi; j += 256) s.x[j] = s.x[j] - m;
+}
+
+__global__ void batch_begin_kernel(int n, const double* __restrict__ results, double* __restrict__ tol,
+                                   int32_t* __restrict__ active, double rtol, double atol) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // b statistics after the mean removal sit in slot 8
+    tol[i] = fmax(atol, rtol * sqrt(results[16 * i + 8 + 1]));
+    active[i] = 1;
+}
+
 
 // ---------------------------------------------------------------------------
 // host driver
@@ -1040,27 +1392,44 @@ struct FastRunner {
 
     bool rows_ok(int l) const { return pl.n[l][2] <= ZROW && pl.n[l][2] % 2 == 0; }
 
+    template <int TXr, int TYr, int N2>
+    int rb_rows_n(int l, const T* in, T* out, bool zero_in) const {
+        const Dim32 d = D(l);
+        const int ntx = int(cdiv(d.n0, TXr)), nty = int(cdiv(d.n1, TYr));
+        const int ntiles = ntx * nty;
+        const size_t bytes = size_t((TXr + 4) * (TYr + 4)) * d.n2 * sizeof(T);
+        const void* fn = pl.periodic ? reinterpret_cast<const void*>(rb_rows_kernel<T, true, TXr, TYr, N2>)
+                                     : reinterpret_cast<const void*>(rb_rows_kernel<T, false, TXr, TYr, N2>);
+        if (int r = dynamic_lds(fn, bytes)) return r;
+        int g = pl.sweep_blocks[l] > 0 ? pl.sweep_blocks[l] : ntiles;
+        if (g > ntiles) g = ntiles;
+        if (pl.periodic)
+            rb_rows_kernel<T, true, TXr, TYr, N2><<<unsigned(g), 256, bytes, s>>>(in, R(l), out, d, K(l), zero_in, ntx, nty);
+        else
+            rb_rows_kernel<T, false, TXr, TYr, N2><<<unsigned(g), 256, bytes, s>>>(in, R(l), out, d, K(l), zero_in, ntx, nty);
+        AFSI_LAUNCH_CHECK();
+        return OK;
+    }
+
+    // z extent as a compile-time constant where the level tiles exactly
     template <int TXr, int TYr>
     int rb_rows(int l, const T* in, T* out, bool zero_in) const {
         const Dim32 d = D(l);
-        const int ntx = int(cdiv(d.n0, TXr)), nty = int(cdiv(d.n1, TYr));
-        const size_t bytes = size_t((TXr + 4) * (TYr + 4)) * d.n2 * sizeof(T);
-        if (int r = dynamic_lds(pl.periodic ? reinterpret_cast<const void*>(rb_rows_kernel<T, true, TXr, TYr>)
-                                            : reinterpret_cast<const void*>(rb_rows_kernel<T, false, TXr, TYr>),
-                                bytes))
-            return r;
-        const unsigned g = unsigned(ntx * nty);
-        if (pl.periodic) rb_rows_kernel<T, true, TXr, TYr><<<g, 256, bytes, s>>>(in, R(l), out, d, K(l), zero_in, ntx, nty);
-        else rb_rows_kernel<T, false, TXr, TYr><<<g, 256, bytes, s>>>(in, R(l), out, d, K(l), zero_in, ntx, nty);
-        AFSI_LAUNCH_CHECK();
-        return OK;
+        if (d.n0 % TXr == 0 && d.n1 % TYr == 0) {
+            if (d.n2 == 128) return rb_rows_n<TXr, TYr, 128>(l, in, out, zero_in);
+            if (d.n2 == 64) return rb_rows_n<TXr, TYr, 64>(l, in, out, zero_in);
+            if (d.n2 == 32) return rb_rows_n<TXr, TYr, 32>(l, in, out, zero_in);
+        }
+        return rb_rows_n<TXr, TYr, 0>(l, in, out, zero_in);
     }
 
     int sweep(int l, const T* in, T* out, bool zero_in) const {
         const Dim32 d = D(l);
         const W3<T> k = K(l);
         if (pl.smoother == 1 && rows_ok(l)) {
-            if (cdiv(d.n0, 4) * cdiv(d.n1, 4) >= 256) return rb_rows<4, 4>(l, in, out, zero_in);
+            int tile = pl.sweep_tile[l];
+            if (tile != 2 && tile != 4) tile = cdiv(d.n0, 4) * cdiv(d.n1, 4) >= 1024 ? 4 : 2;
+            if (tile == 4) return rb_rows<4, 4>(l, in, out, zero_in);
             return rb_rows<2, 2>(l, in, out, zero_in);
         }
         if (pl.smoother == 1) {
@@ -1248,6 +1617,267 @@ static int fast_residual(const afsi_mg3d_plan* pl, void* stream) {
     return launch_reduce_final(pl->reduce_partials, nb, pl->reduce_result, s);
 }
 
+// ---------------------------------------------------------------------------
+// batched mixed-precision solve (host side)
+
+struct BatchLayout {
+    int n = 0, levels = 0, nu = 0, periodic = 0;
+    int cf[1024];                      // first LDS-solved level per instance
+    std::vector<int> members[AFSI_MG_MAX_LEVELS];
+    size_t sweep_off[AFSI_MG_MAX_LEVELS][3];  // k = 0 (zero input), even k, odd k
+    int sweep_blocks[AFSI_MG_MAX_LEVELS];
+    size_t sweep_lds[AFSI_MG_MAX_LEVELS];
+    size_t restrict_off[AFSI_MG_MAX_LEVELS], prolong_off[AFSI_MG_MAX_LEVELS];
+    int restrict_blocks[AFSI_MG_MAX_LEVELS], prolong_blocks[AFSI_MG_MAX_LEVELS];
+    size_t coarse_off = 0, coarse_lds = 0;
+    size_t ref_off[3] = {0, 0, 0};     // residual of x[0]; step from x[0]; step from x[1]
+    int ref_blocks = 0;
+    size_t vec_off[3] = {0, 0, 0};     // b, x[0], x[1]
+    int vec_blocks[3] = {0, 0, 0};
+    size_t total = 0;
+    int64_t partials = 0;              // doubles
+};
+
+static size_t align64(size_t b) { return (b + 63) & ~size_t(63); }
+
+static int batch_layout(const afsi_mgb_batch* bt, BatchLayout& L) {
+    AFSI_REQUIRE(bt && bt->items && bt->count >= 1 && bt->count <= 1024, "invalid batch (1..1024 instances)");
+    L.n = bt->count;
+    L.periodic = bt->items[0].plan.periodic;
+    L.nu = bt->items[0].plan.nu;
+    for (int i = 0; i < L.n; ++i) {
+        const afsi_mgb_item& it = bt->items[i];
+        const afsi_mg3d_plan& pl = it.plan;
+        if (int r = fast_check<float>(&pl)) return r;
+        AFSI_REQUIRE(pl.periodic == L.periodic, "instances of one batch share the boundary type");
+        AFSI_REQUIRE(pl.smoother == 1 && pl.nu == L.nu, "batched instances use red-black GS with one nu");
+        AFSI_REQUIRE(it.x[0] && it.x[1] && it.b && it.x[0] != it.x[1], "invalid FP64 iterate buffers");
+        L.cf[i] = choose_coarse(pl);
+        for (int l = 0; l < L.cf[i]; ++l)
+            AFSI_REQUIRE(pl.n[l][2] <= ZROW && pl.n[l][2] % 2 == 0 && pl.n[l][0] % 2 == 0 &&
+                             pl.n[l][1] % 2 == 0,
+                         "instance %d level %d: batched sweeps need even extents and z <= %d", i, l, ZROW);
+        AFSI_REQUIRE(pl.n[0][0] * pl.n[0][1] * pl.n[0][2] < (int64_t(1) << 31), "instance too large");
+        if (L.cf[i] > L.levels) L.levels = L.cf[i];
+    }
+    size_t off = 0;
+    int64_t partials = 0;
+    for (int l = 0; l < L.levels; ++l) {
+        L.members[l].clear();
+        int sb = 0, rb = 0, pb = 0;
+        size_t lds = 0;
+        for (int i = 0; i < L.n; ++i) {
+            if (l >= L.cf[i]) continue;
+            L.members[l].push_back(i);
+            const int64_t* nn = bt->items[i].plan.n[l];
+            sb += int((nn[0] / 2) * (nn[1] / 2));
+            rb += int(cdiv((nn[0] / 2) * (nn[1] / 2), 4));
+            pb += int(cdiv(nn[0] * nn[1], 4));
+            const size_t box = size_t(6 * 6) * size_t(nn[2]) * sizeof(float);
+            if (box > lds) lds = box;
+        }
+        L.sweep_blocks[l] = sb;
+        L.restrict_blocks[l] = rb;
+        L.prolong_blocks[l] = pb;
+        L.sweep_lds[l] = lds;
+        const size_t m = L.members[l].size();
+        for (int v = 0; v < 3; ++v) {
+            L.sweep_off[l][v] = off;
+            off = align64(off + m * sizeof(SweepDesc<float>));
+        }
+        L.restrict_off[l] = off;
+        off = align64(off + m * sizeof(RowDesc<float>));
+        L.prolong_off[l] = off;
+        off = align64(off + m * sizeof(RowDesc<float>));
+    }
+    L.coarse_off = off;
+    off = align64(off + size_t(L.n) * sizeof(CoarseArgs<float>));
+    L.coarse_lds = 0;
+    L.ref_blocks = 0;
+    for (int i = 0; i < L.n; ++i) {
+        const afsi_mg3d_plan& pl = bt->items[i].plan;
+        size_t cells = 0;
+        for (int l = L.cf[i]; l < pl.levels; ++l) cells += 2 * size_t(pl.n[l][0] * pl.n[l][1] * pl.n[l][2]);
+        if (cells * sizeof(float) > L.coarse_lds) L.coarse_lds = cells * sizeof(float);
+        L.ref_blocks += int(cdiv(pl.n[0][0] * pl.n[0][1], 4));
+    }
+    for (int v = 0; v < 3; ++v) {
+        L.ref_off[v] = off;
+        off = align64(off + size_t(L.n) * sizeof(RefDesc));
+    }
+    for (int v = 0; v < 3; ++v) {
+        L.vec_off[v] = off;
+        off = align64(off + size_t(L.n) * sizeof(VecDesc));
+        int nb = 0;
+        for (int i = 0; i < L.n; ++i) {
+            const int64_t* nn = bt->items[i].plan.n[0];
+            nb += int(cdiv(nn[0] * nn[1] * nn[2], VEC_CHUNK));
+        }
+        L.vec_blocks[v] = nb;
+    }
+    L.total = off;
+    partials = std::max<int64_t>(L.ref_blocks, L.vec_blocks[0]);
+    L.partials = 4 * partials;
+    return OK;
+}
+
+template <class D>
+static D* at(void* ws, size_t off) {
+    return reinterpret_cast<D*>(static_cast<char*>(ws) + off);
+}
+
+static int batch_prepare(const afsi_mgb_batch* bt, hipStream_t s) {
+    BatchLayout L;
+    if (int r = batch_layout(bt, L)) return r;
+    AFSI_REQUIRE(bt->workspace && bt->workspace_bytes >= L.total, "batch workspace too small");
+    AFSI_REQUIRE(bt->active && bt->tol && bt->results && bt->partials, "missing batch state buffers");
+    std::vector<char> host(L.total, 0);
+    void* hw = host.data();
+    for (int l = 0; l < L.levels; ++l) {
+        int sb = 0, rb = 0, pb = 0;
+        for (size_t j = 0; j < L.members[l].size(); ++j) {
+            const int i = L.members[l][j];
+            const afsi_mg3d_plan& pl = bt->items[i].plan;
+            const Dim32 d{int(pl.n[l][0]), int(pl.n[l][1]), int(pl.n[l][2])};
+            W3<float> k;
+            for (int a = 0; a < 3; ++a) k.w[a] = float(pl.inv_h2[l][a]);
+            float* P = static_cast<float*>(pl.p[l]);
+            float* W = static_cast<float*>(pl.tmp[l]);
+            const float* R = static_cast<const float*>(pl.rhs[l]);
+            const int ntx = d.n0 / 2, nty = d.n1 / 2;
+            const int32_t* act = bt->active + i;
+            at<SweepDesc<float>>(hw, L.sweep_off[l][0])[j] = SweepDesc<float>{P, R, W, act, d, k, ntx, nty, sb, 1};
+            at<SweepDesc<float>>(hw, L.sweep_off[l][1])[j] = SweepDesc<float>{P, R, W, act, d, k, ntx, nty, sb, 0};
+            at<SweepDesc<float>>(hw, L.sweep_off[l][2])[j] = SweepDesc<float>{W, R, P, act, d, k, ntx, nty, sb, 0};
+            float* cur = (L.nu % 2) ? W : P;  // iterate after the pre-smoothing sweeps
+            at<RowDesc<float>>(hw, L.restrict_off[l])[j] =
+                RowDesc<float>{cur, R, static_cast<float*>(pl.rhs[l + 1]), act, d, k, rb};
+            at<RowDesc<float>>(hw, L.prolong_off[l])[j] =
+                RowDesc<float>{static_cast<const float*>(pl.p[l + 1]), nullptr, cur, act, d, k, pb};
+            sb += ntx * nty;
+            rb += int(cdiv(int64_t(d.n0 / 2) * (d.n1 / 2), 4));
+            pb += int(cdiv(int64_t(d.n0) * d.n1, 4));
+        }
+    }
+    int rbeg = 0, vbeg = 0;
+    for (int i = 0; i < L.n; ++i) {
+        const afsi_mgb_item& it = bt->items[i];
+        const afsi_mg3d_plan& pl = it.plan;
+        CoarseArgs<float> a{};
+        const int first = L.cf[i];
+        a.levels = pl.levels - first;
+        a.smoother = pl.smoother;
+        a.nu = pl.nu;
+        a.omega = float(pl.omega);
+        int32_t o = 0;
+        for (int j = 0; j < a.levels; ++j) {
+            const int l = first + j;
+            a.d[j] = Dim32{int(pl.n[l][0]), int(pl.n[l][1]), int(pl.n[l][2])};
+            for (int q = 0; q < 3; ++q) a.k[j].w[q] = float(pl.inv_h2[l][q]);
+            const int32_t c = int32_t(pl.n[l][0] * pl.n[l][1] * pl.n[l][2]);
+            a.off_p[j] = o; o += c;
+            a.off_r[j] = o; o += c;
+            a.off_w[j] = o;
+        }
+        a.rhs_in = static_cast<const float*>(pl.rhs[first]);
+        a.p_in = nullptr;  // zero initial guess on every level
+        a.p_out = static_cast<float*>(pl.p[first]);
+        a.coarse_inverse = static_cast<const float*>(pl.coarse_inverse);
+        a.coarse_cells = int32_t(pl.coarse_cells);
+        a.active = bt->active + i;
+        at<CoarseArgs<float>>(hw, L.coarse_off)[i] = a;
+        const Dim32 d0{int(pl.n[0][0]), int(pl.n[0][1]), int(pl.n[0][2])};
+        W3<double> k0;
+        for (int q = 0; q < 3; ++q) k0.w[q] = pl.inv_h2[0][q];
+        const float* e = static_cast<const float*>(pl.p[0]);
+        float* r32 = static_cast<float*>(pl.rhs[0]);
+        const int32_t* act = bt->active + i;
+        at<RefDesc>(hw, L.ref_off[0])[i] = RefDesc{it.x[0], e, it.b, it.x[0], r32, nullptr, d0, k0, rbeg};
+        at<RefDesc>(hw, L.ref_off[1])[i] = RefDesc{it.x[0], e, it.b, it.x[1], r32, act, d0, k0, rbeg};
+        at<RefDesc>(hw, L.ref_off[2])[i] = RefDesc{it.x[1], e, it.b, it.x[0], r32, act, d0, k0, rbeg};
+        rbeg += int(cdiv(int64_t(d0.n0) * d0.n1, 4));
+        const int cells = d0.n0 * d0.n1 * d0.n2;
+        at<VecDesc>(hw, L.vec_off[0])[i] = VecDesc{it.b, cells, vbeg};
+        at<VecDesc>(hw, L.vec_off[1])[i] = VecDesc{it.x[0], cells, vbeg};
+        at<VecDesc>(hw, L.vec_off[2])[i] = VecDesc{it.x[1], cells, vbeg};
+        vbeg += int(cdiv(cells, VEC_CHUNK));
+    }
+    AFSI_HIP(hipMemcpyAsync(bt->workspace, host.data(), L.total, hipMemcpyHostToDevice, s));
+    AFSI_HIP(hipStreamSynchronize(s));
+    return OK;
+}
+
+template <bool P>
+static int batch_vcycle(const afsi_mgb_batch* bt, const BatchLayout& L, hipStream_t s) {
+    void* ws = bt->workspace;
+    const void* sweep_fn = reinterpret_cast<const void*>(rb_rows_batch_kernel<float, P>);
+    for (int l = 0; l < L.levels; ++l) {
+        const int m = int(L.members[l].size());
+        if (int r = dynamic_lds(sweep_fn, L.sweep_lds[l])) return r;
+        for (int it = 0; it < L.nu; ++it) {
+            const int v = it == 0 ? 0 : (it % 2 ? 2 : 1);
+            rb_rows_batch_kernel<float, P><<<unsigned(L.sweep_blocks[l]), 256, L.sweep_lds[l], s>>>(
+                at<SweepDesc<float>>(ws, L.sweep_off[l][v]), m);
+        }
+        restrict_batch_kernel<float, P><<<unsigned(L.restrict_blocks[l]), 256, 0, s>>>(
+            at<RowDesc<float>>(ws, L.restrict_off[l]), m);
+        AFSI_LAUNCH_CHECK();
+    }
+    const void* coarse_fn = reinterpret_cast<const void*>(coarse_batch_kernel<float, P>);
+    if (int r = dynamic_lds(coarse_fn, L.coarse_lds)) return r;
+    coarse_batch_kernel<float, P><<<unsigned(L.n), COARSE_THREADS, L.coarse_lds, s>>>(
+        at<CoarseArgs<float>>(ws, L.coarse_off));
+    AFSI_LAUNCH_CHECK();
+    for (int l = L.levels - 1; l >= 0; --l) {
+        const int m = int(L.members[l].size());
+        prolong_batch_kernel<float, P><<<unsigned(L.prolong_blocks[l]), 256, 0, s>>>(
+            at<RowDesc<float>>(ws, L.prolong_off[l]), m);
+        for (int it = L.nu; it < 2 * L.nu; ++it) {
+            const int v = it % 2 ? 2 : 1;
+            rb_rows_batch_kernel<float, P><<<unsigned(L.sweep_blocks[l]), 256, L.sweep_lds[l], s>>>(
+                at<SweepDesc<float>>(ws, L.sweep_off[l][v]), m);
+        }
+        AFSI_LAUNCH_CHECK();
+    }
+    return OK;
+}
+
+template <bool P>
+static int batch_refine(const afsi_mgb_batch* bt, const BatchLayout& L, int which, bool reduce,
+                        hipStream_t s) {
+    const RefDesc* desc = at<RefDesc>(bt->workspace, L.ref_off[which]);
+    const unsigned g = unsigned(L.ref_blocks);
+    if (which == 0) refine_batch_kernel<P, false, true><<<g, 256, 0, s>>>(desc, L.n, bt->partials);
+    else if (reduce) refine_batch_kernel<P, true, true><<<g, 256, 0, s>>>(desc, L.n, bt->partials);
+    else refine_batch_kernel<P, true, false><<<g, 256, 0, s>>>(desc, L.n, bt->partials);
+    AFSI_LAUNCH_CHECK();
+    if (which == 0 || reduce) {
+        batch_reduce_kernel<RefDesc><<<unsigned(L.n), 64, 0, s>>>(desc, L.n, L.ref_blocks, bt->partials,
+                                                                  bt->results, 0, bt->tol, bt->active);
+        AFSI_LAUNCH_CHECK();
+    }
+    return OK;
+}
+
+// stats of vector v into results slot, optionally followed by mean removal
+static int batch_vec(const afsi_mgb_batch* bt, const BatchLayout& L, int v, int slot, bool sub,
+                     hipStream_t s) {
+    const VecDesc* desc = at<VecDesc>(bt->workspace, L.vec_off[v]);
+    vec_partials_kernel<<<unsigned(L.vec_blocks[v]), 256, 0, s>>>(desc, L.n, bt->partials);
+    batch_reduce_kernel<VecDesc><<<unsigned(L.n), 64, 0, s>>>(desc, L.n, L.vec_blocks[v], bt->partials,
+                                                              bt->results, slot, bt->tol, bt->active);
+    if (sub) vec_sub_mean_kernel<<<unsigned(L.vec_blocks[v]), 256, 0, s>>>(desc, L.n, bt->results, slot);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+static int batch_check_ws(const afsi_mgb_batch* bt, BatchLayout& L) {
+    if (int r = batch_layout(bt, L)) return r;
+    AFSI_REQUIRE(bt->workspace && bt->workspace_bytes >= L.total, "batch workspace too small");
+    AFSI_REQUIRE(bt->active && bt->tol && bt->results && bt->partials, "missing batch state buffers");
+    return OK;
+}
+
 }  // namespace afsi
 
 extern "C" {
@@ -1276,6 +1906,34 @@ int afsi_refine_residual_3d(const double* x, const double* b, float* r32, const 
     AFSI_LAUNCH_CHECK();
     return launch_reduce_final(partials, nb, result, s);
 }
+int afsi_refine_step_3d(const double* x, const float* e, const double* b, double* x_out,
+                        float* r32, const int64_t n[3], const double inv_h2[3], int periodic,
+                        double* partials, int64_t partials_capacity, double* result, void* stream) {
+    using namespace afsi;
+    AFSI_REQUIRE(n && inv_h2 && x && e && b && x_out && r32, "invalid refinement arguments");
+    AFSI_REQUIRE(x != x_out, "the refinement step needs distinct input and output iterates");
+    AFSI_REQUIRE(n[0] > 0 && n[1] > 0 && n[2] > 0 && n[0] * n[1] * n[2] < (int64_t(1) << 31),
+                 "invalid grid");
+    const Dim32 d{int(n[0]), int(n[1]), int(n[2])};
+    W3<double> k;
+    for (int a = 0; a < 3; ++a) k.w[a] = inv_h2[a];
+    const hipStream_t s = hipStream_t(stream);
+    int64_t nb = cdiv(n[0] * n[1], 4);
+    if (result) {
+        AFSI_REQUIRE(partials, "missing reduction scratch");
+        if (nb > partials_capacity / 4) nb = partials_capacity / 4;
+        AFSI_REQUIRE(nb >= 1, "reduction scratch too small");
+        if (periodic) refine_step_kernel<true, true><<<unsigned(nb), 256, 0, s>>>(x, e, b, x_out, r32, d, k, partials);
+        else refine_step_kernel<false, true><<<unsigned(nb), 256, 0, s>>>(x, e, b, x_out, r32, d, k, partials);
+        AFSI_LAUNCH_CHECK();
+        return launch_reduce_final(partials, nb, result, s);
+    }
+    if (nb > 16384) nb = 16384;
+    if (periodic) refine_step_kernel<true, false><<<unsigned(nb), 256, 0, s>>>(x, e, b, x_out, r32, d, k, nullptr);
+    else refine_step_kernel<false, false><<<unsigned(nb), 256, 0, s>>>(x, e, b, x_out, r32, d, k, nullptr);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
 int afsi_refine_update(double* x, const float* e, int64_t count, void* stream) {
     using namespace afsi;
     AFSI_REQUIRE(count >= 0 && (count == 0 || (x && e)), "invalid refinement update");
@@ -1283,6 +1941,56 @@ int afsi_refine_update(double* x, const float* e, int64_t count, void* stream) {
     refine_update_kernel<<<grid_for(count, 256, 16384), 256, 0, hipStream_t(stream)>>>(x, e, count);
     AFSI_LAUNCH_CHECK();
     return OK;
+}
+size_t afsi_mgb_workspace_bytes(const afsi_mgb_batch* batch) {
+    afsi::BatchLayout L;
+    return afsi::batch_layout(batch, L) == afsi::OK ? L.total : 0;
+}
+int64_t afsi_mgb_partials_doubles(const afsi_mgb_batch* batch) {
+    afsi::BatchLayout L;
+    return afsi::batch_layout(batch, L) == afsi::OK ? L.partials : -1;
+}
+int afsi_mgb_prepare(const afsi_mgb_batch* batch, void* stream) {
+    return afsi::batch_prepare(batch, hipStream_t(stream));
+}
+int afsi_mgb_begin(const afsi_mgb_batch* batch, double rtol, double atol, void* stream) {
+    using namespace afsi;
+    BatchLayout L;
+    if (int r = batch_check_ws(batch, L)) return r;
+    const hipStream_t s = hipStream_t(stream);
+    if (int r = batch_vec(batch, L, 0, 4, true, s)) return r;    // b: stats, mean removal
+    if (int r = batch_vec(batch, L, 1, 12, true, s)) return r;   // x[0]: stats, mean removal
+    if (int r = batch_vec(batch, L, 0, 8, false, s)) return r;   // ||b|| after the removal
+    batch_begin_kernel<<<unsigned(cdiv(L.n, 256)), 256, 0, s>>>(L.n, batch->results, batch->tol,
+                                                               batch->active, rtol, atol);
+    AFSI_LAUNCH_CHECK();
+    return L.periodic ? batch_refine<true>(batch, L, 0, true, s) : batch_refine<false>(batch, L, 0, true, s);
+}
+int afsi_mgb_iterate(const afsi_mgb_batch* batch, int32_t start, int32_t count, void* stream) {
+    using namespace afsi;
+    BatchLayout L;
+    if (int r = batch_check_ws(batch, L)) return r;
+    AFSI_REQUIRE((start == 0 || start == 1) && count >= 1, "invalid batch iteration range");
+    const hipStream_t s = hipStream_t(stream);
+    for (int it = 0; it < count; ++it) {
+        const int from = (start + it) % 2;
+        const bool last = it == count - 1;
+        if (L.periodic) {
+            if (int r = batch_vcycle<true>(batch, L, s)) return r;
+            if (int r = batch_refine<true>(batch, L, 1 + from, last, s)) return r;
+        } else {
+            if (int r = batch_vcycle<false>(batch, L, s)) return r;
+            if (int r = batch_refine<false>(batch, L, 1 + from, last, s)) return r;
+        }
+    }
+    return OK;
+}
+int afsi_mgb_finish(const afsi_mgb_batch* batch, int32_t cur, void* stream) {
+    using namespace afsi;
+    BatchLayout L;
+    if (int r = batch_check_ws(batch, L)) return r;
+    AFSI_REQUIRE(cur == 0 || cur == 1, "invalid iterate index");
+    return batch_vec(batch, L, 1 + cur, 12, true, hipStream_t(stream));
 }
 int afsi_mg3d_sweep_f64(const afsi_mg3d_plan* plan, int32_t level, void* stream) {
     using namespace afsi;

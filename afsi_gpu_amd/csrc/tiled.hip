@@ -61,23 +61,41 @@ static TileSpec make_spec(const afsi_grid3& g, const int32_t tile[3]) {
     return s;
 }
 
+// Box extents as compile-time constants for cubic tiles of edge T (T = 0:
+// read them from the TileSpec at run time).
+template <int T>
+__device__ inline int box_dim(const TileSpec& ts, int c, int a) {
+    return T > 0 ? T + 4 - (a == c ? 1 : 0) : ts.dim[c][a];
+}
+template <int T>
+__device__ inline int box_off(const TileSpec& ts, int c) {
+    return T > 0 ? c * (T + 3) * (T + 4) * (T + 4) : ts.off[c];
+}
+template <int T>
+__device__ inline int tile_edge(const TileSpec& ts, int a) {
+    return T > 0 ? T : ts.t[a];
+}
+
+__device__ inline int wrap32(int i, int n) {
+    const int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
 struct PointWeights {
-    int64_t bf[3], bc[3];  // face-lattice and centre-lattice bases
+    int bf[3], bc[3];  // face-lattice and centre-lattice bases
     double wf[3][4], wc[3][4];
 };
 
-// Bases and weights of one point; `shift` moves unwrapped bases into the
-// frame of the (wrapped) cell used for binning.
-__device__ inline void point_weights(const double xp[3], const afsi_grid3& g,
-                                     PointWeights& pw) {
+// Bases and weights of one point (mac/compact_transfer.py:161-166 bases).
+__device__ inline void point_weights(const double xp[3], const afsi_grid3& g, PointWeights& pw) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const double sf = (xp[a] - g.origin[a]) / g.h[a];
         const double ff = floor(sf - 1.0);
         const double sc = (xp[a] - g.center_origin[a]) / g.h[a];
         const double fc = floor(sc - 1.0);
-        pw.bf[a] = isfinite(ff) ? int64_t(ff) : 0;
-        pw.bc[a] = isfinite(fc) ? int64_t(fc) : 0;
+        pw.bf[a] = (ff > -1e9 && ff < 1e9) ? int(ff) : 0;
+        pw.bc[a] = (fc > -1e9 && fc < 1e9) ? int(fc) : 0;
         peskin_weights<double>(sf - (ff + 1.0), pw.wf[a]);
         peskin_weights<double>(sc - (fc + 1.0), pw.wc[a]);
     }
@@ -110,69 +128,116 @@ __global__ void chunk_scan_kernel(const int32_t* __restrict__ cell_ptr, int64_t 
     if (threadIdx.x == 0) chunk_start[ntiles] = running;
 }
 
+// Work item w -> tile (last t with chunk_start[t] <= w), its first corner
+// cell t0 and the point range [pbeg, pend) of its chunk.
+struct WorkItem {
+    int t0[3];
+    int pbeg, pend;
+};
+
+__device__ inline WorkItem work_item(int w, const int32_t* __restrict__ chunk_start,
+                                     const int32_t* __restrict__ cell_ptr, const TileSpec& ts) {
+    int lo = 0, hi = int(ts.ntiles);
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (chunk_start[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    const int tile = lo;
+    const int chunk = w - chunk_start[tile];
+    const int nt1 = int(ts.nt[1]), nt2 = int(ts.nt[2]);
+    WorkItem it;
+    it.t0[0] = (tile / (nt2 * nt1)) * ts.t[0];
+    it.t0[1] = ((tile / nt2) % nt1) * ts.t[1];
+    it.t0[2] = (tile % nt2) * ts.t[2];
+    const int tv = int(ts.tvol);
+    it.pbeg = cell_ptr[tile * tv] + chunk * SPREAD_CHUNK;
+    it.pend = min(cell_ptr[(tile + 1) * tv], it.pbeg + SPREAD_CHUNK);
+    return it;
+}
+
+// Box entry e of component c -> global face offset, or -1 outside a closed
+// box. Box axis a starts at t0[a] - 1 (own axis) or t0[a] - 2.
+template <int T>
+__device__ inline int box_face(int e, int c, const int t0[3], const TileSpec& ts,
+                               const afsi_grid3& g) {
+    const int d1 = box_dim<T>(ts, c, 1), d2 = box_dim<T>(ts, c, 2);
+    const int l[3] = {e / (d1 * d2), (e / d2) % d1, e % d2};
+    int idx = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int nf = int(g.n[a]) + ((!g.periodic && a == c) ? 1 : 0);
+        int q = t0[a] - (a == c ? 1 : 2) + l[a];
+        if (g.periodic) q = q < 0 ? q + nf : (q >= nf ? q - nf : q);
+        else if (q < 0 || q >= nf) return -1;
+        idx = idx * nf + q;
+    }
+    return idx;
+}
+
+// Local box base of a point for component c (-1 in lb[0]: outside the box).
+template <int T>
+__device__ inline bool box_base(const PointWeights& pw, const int shift[3], int c, const int t0[3],
+                                const TileSpec& ts, int lb[3]) {
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int b = (a == c ? pw.bf[a] : pw.bc[a]) + shift[a];
+        lb[a] = b - (t0[a] - (a == c ? 1 : 2));
+        ok = ok && lb[a] >= 0 && lb[a] + 3 < box_dim<T>(ts, c, a);
+    }
+    return ok;
+}
+
+// Periodic frame shift of a point's unwrapped bases onto its binning cell.
+__device__ inline void frame_shift(const PointWeights& pw, const afsi_grid3& g, int shift[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int cell = pw.bf[a] + 1;
+        shift[a] = g.periodic ? wrap32(cell, int(g.n[a])) - cell : 0;
+    }
+}
+
+template <int T>
 __global__ void __launch_bounds__(SPREAD_THREADS) spread_tiled_kernel(
     const double* __restrict__ x, const double* __restrict__ force,
     const int32_t* __restrict__ perm, const int32_t* __restrict__ cell_ptr,
     const int32_t* __restrict__ chunk_start, afsi_grid3 g, TileSpec ts, double volume,
     double* __restrict__ out0, double* __restrict__ out1, double* __restrict__ out2, int* flags) {
     extern __shared__ double lds[];
-    const int32_t total = chunk_start[ts.ntiles];
+    const int total = chunk_start[ts.ntiles];
     double* outs[3] = {out0, out1, out2};
-    for (int32_t w = blockIdx.x; w < total; w += gridDim.x) {
-        // tile = last t with chunk_start[t] <= w
-        int64_t lo = 0, hi = ts.ntiles;
-        while (hi - lo > 1) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (chunk_start[mid] <= w) lo = mid;
-            else hi = mid;
-        }
-        const int64_t tile = lo;
-        const int32_t chunk = w - chunk_start[tile];
-        const int64_t tz = tile % ts.nt[2], ty = (tile / ts.nt[2]) % ts.nt[1],
-                      tx = tile / (ts.nt[2] * ts.nt[1]);
-        const int64_t t0[3] = {tx * ts.t[0], ty * ts.t[1], tz * ts.t[2]};
-        const int32_t pbeg = cell_ptr[tile * ts.tvol] + chunk * SPREAD_CHUNK;
-        const int32_t pend = min(cell_ptr[(tile + 1) * ts.tvol], pbeg + SPREAD_CHUNK);
-        for (int32_t e = threadIdx.x; e < ts.lds; e += SPREAD_THREADS) lds[e] = 0.0;
+    const int nlds = T > 0 ? 3 * (T + 3) * (T + 4) * (T + 4) : ts.lds;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        const WorkItem it = work_item(w, chunk_start, cell_ptr, ts);
+        for (int e = threadIdx.x; e < nlds; e += SPREAD_THREADS) lds[e] = 0.0;
         __syncthreads();
         // Each thread takes a contiguous slice so lanes of one wave sit in
         // different cells and LDS atomics rarely collide.
-        const int32_t n = pend - pbeg;
-        const int32_t per = (n + SPREAD_THREADS - 1) / SPREAD_THREADS;
-        const int32_t j0 = pbeg + threadIdx.x * per;
-        const int32_t j1 = min(pend, j0 + per);
-        for (int32_t j = j0; j < j1; ++j) {
-            const int64_t idx = perm[j];
+        const int n = it.pend - it.pbeg;
+        const int per = (n + SPREAD_THREADS - 1) / SPREAD_THREADS;
+        const int j0 = it.pbeg + int(threadIdx.x) * per;
+        const int j1 = min(it.pend, j0 + per);
+        for (int j = j0; j < j1; ++j) {
+            const int idx = perm[j];
             const double xp[3] = {x[3 * idx], x[3 * idx + 1], x[3 * idx + 2]};
             PointWeights pw;
             point_weights(xp, g, pw);
-            // Shift unwrapped bases into this tile's frame (periodic wrap).
-            int64_t shift[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const int64_t cell = pw.bf[a] + 1;
-                shift[a] = g.periodic ? wrap(cell, g.n[a]) - cell : 0;
-            }
+            int shift[3];
+            frame_shift(pw, g, shift);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const double fv = force[3 * idx + c] / volume;
-                int64_t lb[3];
-                bool ok = true;
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const int64_t b = (a == c ? pw.bf[a] : pw.bc[a]) + shift[a];
-                    lb[a] = b - (t0[a] - (a == c ? 1 : 2));
-                    ok = ok && lb[a] >= 0 && lb[a] + 3 < ts.dim[c][a];
-                }
-                if (!ok) {
+                int lb[3];
+                if (!box_base<T>(pw, shift, c, it.t0, ts, lb)) {
                     raise_flag(flags, F_INDEX);
                     continue;
                 }
                 const double* wx = (c == 0) ? pw.wf[0] : pw.wc[0];
                 const double* wy = (c == 1) ? pw.wf[1] : pw.wc[1];
                 const double* wz = (c == 2) ? pw.wf[2] : pw.wc[2];
-                const int32_t d1 = ts.dim[c][1], d2 = ts.dim[c][2];
-                double* box = lds + ts.off[c];
+                const int d1 = box_dim<T>(ts, c, 1), d2 = box_dim<T>(ts, c, 2);
+                double* box = lds + box_off<T>(ts, c);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -187,51 +252,101 @@ __global__ void __launch_bounds__(SPREAD_THREADS) spread_tiled_kernel(
         }
         __syncthreads();
         // Flush: one global atomic per nonzero LDS entry.
-        for (int32_t e = threadIdx.x; e < ts.lds; e += SPREAD_THREADS) {
+        for (int e = threadIdx.x; e < nlds; e += SPREAD_THREADS) {
             const double v = lds[e];
             if (v == 0.0) continue;
-            const int c = e < ts.off[1] ? 0 : (e < ts.off[2] ? 1 : 2);
-            const int32_t r = e - ts.off[c];
-            const int32_t d1 = ts.dim[c][1], d2 = ts.dim[c][2];
-            const int64_t l[3] = {r / (d1 * d2), (r / d2) % d1, r % d2};
-            int64_t gidx[3], nf[3];
-            bool ok = true;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                nf[a] = g.n[a] + ((!g.periodic && a == c) ? 1 : 0);
-                int64_t q = t0[a] - (a == c ? 1 : 2) + l[a];
-                if (g.periodic) q = wrap(q, g.n[a]);
-                ok = ok && q >= 0 && q < nf[a];
-                gidx[a] = q;
-            }
-            if (!ok) {
+            const int c = e < box_off<T>(ts, 1) ? 0 : (e < box_off<T>(ts, 2) ? 1 : 2);
+            const int f = box_face<T>(e - box_off<T>(ts, c), c, it.t0, ts, g);
+            if (f < 0) {
                 raise_flag(flags, F_INDEX);
                 continue;
             }
-            unsafeAtomicAdd(outs[c] + (gidx[0] * nf[1] + gidx[1]) * nf[2] + gidx[2], v);
+            unsafeAtomicAdd(outs[c] + f, v);
         }
         __syncthreads();
     }
 }
 
-__global__ void __launch_bounds__(256) interp_tiled_kernel(
-    const double* __restrict__ x, int64_t points, const int32_t* __restrict__ perm,
-    afsi_grid3 g, const double* __restrict__ u0, const double* __restrict__ u1,
+// Interpolation over the same work list: the tile's face box of all three
+// components is staged in LDS once per work item, then each point gathers
+// its 3 x 64 values from LDS (consecutive threads take consecutive binned
+// points; results go to the caller's point order).
+template <int T>
+__global__ void __launch_bounds__(SPREAD_THREADS) interp_lds_kernel(
+    const double* __restrict__ x, const int32_t* __restrict__ perm,
+    const int32_t* __restrict__ cell_ptr, const int32_t* __restrict__ chunk_start, afsi_grid3 g,
+    TileSpec ts, const double* __restrict__ u0, const double* __restrict__ u1,
     const double* __restrict__ u2, double* __restrict__ out, int* flags) {
+    extern __shared__ double lds[];
+    const int total = chunk_start[ts.ntiles];
+    const double* us[3] = {u0, u1, u2};
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        const WorkItem it = work_item(w, chunk_start, cell_ptr, ts);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int vol = box_dim<T>(ts, c, 0) * box_dim<T>(ts, c, 1) * box_dim<T>(ts, c, 2);
+            double* box = lds + box_off<T>(ts, c);
+            for (int e = threadIdx.x; e < vol; e += SPREAD_THREADS) {
+                const int f = box_face<T>(e, c, it.t0, ts, g);
+                box[e] = f >= 0 ? us[c][f] : 0.0;
+            }
+        }
+        __syncthreads();
+        for (int j = it.pbeg + int(threadIdx.x); j < it.pend; j += SPREAD_THREADS) {
+            const int idx = perm[j];
+            const double xp[3] = {x[3 * idx], x[3 * idx + 1], x[3 * idx + 2]};
+            PointWeights pw;
+            point_weights(xp, g, pw);
+            int shift[3];
+            frame_shift(pw, g, shift);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                int lb[3];
+                double sum = 0.0;
+                if (!box_base<T>(pw, shift, c, it.t0, ts, lb)) {
+                    raise_flag(flags, F_INDEX);
+                } else {
+                    const double* wx = (c == 0) ? pw.wf[0] : pw.wc[0];
+                    const double* wy = (c == 1) ? pw.wf[1] : pw.wc[1];
+                    const double* wz = (c == 2) ? pw.wf[2] : pw.wc[2];
+                    const int d1 = box_dim<T>(ts, c, 1), d2 = box_dim<T>(ts, c, 2);
+                    const double* box = lds + box_off<T>(ts, c);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) {
+                            const double wxy = wx[i] * wy[jj];
+                            const double* row = box + ((lb[0] + i) * d1 + (lb[1] + jj)) * d2 + lb[2];
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) sum += row[k] * (wxy * wz[k]);
+                        }
+                    }
+                }
+                out[3 * idx + c] = sum;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Unbinned interpolation: one thread per point, gathers from HBM/L2.
+__global__ void __launch_bounds__(256) interp_points_kernel(
+    const double* __restrict__ x, int64_t points, afsi_grid3 g, const double* __restrict__ u0,
+    const double* __restrict__ u1, const double* __restrict__ u2, double* __restrict__ out,
+    int* flags) {
     const double* us[3] = {u0, u1, u2};
     for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < points;
          j += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t idx = perm ? perm[j] : j;
-        const double xp[3] = {x[3 * idx], x[3 * idx + 1], x[3 * idx + 2]};
+        const double xp[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
         PointWeights pw;
         point_weights(xp, g, pw);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            int64_t nf[3], b[3];
+            int nf[3], b[3];
             bool ok = true;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                nf[a] = g.n[a] + ((!g.periodic && a == c) ? 1 : 0);
+                nf[a] = int(g.n[a]) + ((!g.periodic && a == c) ? 1 : 0);
                 b[a] = (a == c) ? pw.bf[a] : pw.bc[a];
                 if (!g.periodic) ok = ok && b[a] >= 0 && b[a] + 3 < nf[a];
             }
@@ -242,16 +357,16 @@ __global__ void __launch_bounds__(256) interp_tiled_kernel(
                 const double* wx = (c == 0) ? pw.wf[0] : pw.wc[0];
                 const double* wy = (c == 1) ? pw.wf[1] : pw.wc[1];
                 const double* wz = (c == 2) ? pw.wf[2] : pw.wc[2];
-                int64_t zi[4];
+                int zi[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) zi[k] = g.periodic ? wrap(b[2] + k, nf[2]) : b[2] + k;
+                for (int k = 0; k < 4; ++k) zi[k] = g.periodic ? wrap32(b[2] + k, nf[2]) : b[2] + k;
                 const double* u = us[c];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const int64_t xi = g.periodic ? wrap(b[0] + i, nf[0]) : b[0] + i;
+                    const int xi = g.periodic ? wrap32(b[0] + i, nf[0]) : b[0] + i;
 #pragma unroll
                     for (int jj = 0; jj < 4; ++jj) {
-                        const int64_t yj = g.periodic ? wrap(b[1] + jj, nf[1]) : b[1] + jj;
+                        const int yj = g.periodic ? wrap32(b[1] + jj, nf[1]) : b[1] + jj;
                         const double wxy = wx[i] * wy[jj];
                         const double* row = u + (xi * nf[1] + yj) * nf[2];
 #pragma unroll
@@ -259,7 +374,7 @@ __global__ void __launch_bounds__(256) interp_tiled_kernel(
                     }
                 }
             }
-            out[3 * idx + c] = sum;
+            out[3 * j + c] = sum;
         }
     }
 }
@@ -278,16 +393,28 @@ size_t afsi_spread_workspace_bytes(const afsi_grid3* grid, const int32_t tile[3]
     return align256(size_t(ts.ntiles + 1) * sizeof(int32_t));
 }
 
+static int cubic_edge(const int32_t tile[3]) {
+    if (tile[0] == tile[1] && tile[1] == tile[2] && (tile[0] == 8 || tile[0] == 4)) return tile[0];
+    return 0;
+}
+
+static int check_tiled(const afsi_grid3& g, const int32_t tile[3]) {
+    AFSI_REQUIRE(tile[0] > 0 && tile[1] > 0 && tile[2] > 0, "tile sizes must be positive");
+    AFSI_REQUIRE((g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1) < (int64_t(1) << 31),
+                 "grid exceeds 2^31 faces per component");
+    return OK;
+}
+
 int afsi_spread_tiled_f64(const double* x, const double* force, int64_t points,
                           const afsi_grid3* grid, const int32_t tile[3], const int32_t* perm,
                           const int32_t* cell_ptr, double* u, double* v, double* w,
                           void* workspace, size_t workspace_bytes, int* flags, void* stream) {
-    AFSI_REQUIRE(grid && tile && points >= 0, "invalid spread arguments");
-    AFSI_REQUIRE(tile[0] > 0 && tile[1] > 0 && tile[2] > 0, "tile sizes must be positive");
+    AFSI_REQUIRE(grid && tile && points >= 0 && points < (int64_t(1) << 31), "invalid spread arguments");
     AFSI_REQUIRE(u && v && w, "null face array");
     const afsi_grid3& g = *grid;
+    if (int r = check_tiled(g, tile)) return r;
     const TileSpec ts = make_spec(g, tile);
-    AFSI_REQUIRE(size_t(ts.lds) * sizeof(double) <= 160 * 1024, "tile too large for LDS");
+    AFSI_REQUIRE(size_t(ts.lds) * sizeof(double) <= 64 * 1024, "tile too large for LDS");
     const hipStream_t s = hipStream_t(stream);
     const int64_t nf[3][3] = {
         {g.n[0] + !g.periodic, g.n[1], g.n[2]},
@@ -307,20 +434,64 @@ int afsi_spread_tiled_f64(const double* x, const double* force, int64_t points,
     const int64_t upper = ts.ntiles + points / SPREAD_CHUNK + 1;
     const unsigned blocks = unsigned(upper < 4096 ? upper : 4096);
     const size_t shmem = size_t(ts.lds) * sizeof(double);
-    spread_tiled_kernel<<<blocks, SPREAD_THREADS, shmem, s>>>(x, force, perm, cell_ptr, chunk_start,
-                                                             g, ts, volume, u, v, w, flags);
+    switch (cubic_edge(tile)) {
+    case 8:
+        spread_tiled_kernel<8><<<blocks, SPREAD_THREADS, shmem, s>>>(x, force, perm, cell_ptr, chunk_start,
+                                                                    g, ts, volume, u, v, w, flags);
+        break;
+    case 4:
+        spread_tiled_kernel<4><<<blocks, SPREAD_THREADS, shmem, s>>>(x, force, perm, cell_ptr, chunk_start,
+                                                                    g, ts, volume, u, v, w, flags);
+        break;
+    default:
+        spread_tiled_kernel<0><<<blocks, SPREAD_THREADS, shmem, s>>>(x, force, perm, cell_ptr, chunk_start,
+                                                                    g, ts, volume, u, v, w, flags);
+    }
     AFSI_LAUNCH_CHECK();
     return OK;
 }
 
 int afsi_interp_tiled_f64(const double* x, int64_t points, const afsi_grid3* grid,
-                          const int32_t* perm, const double* u, const double* v, const double* w,
-                          double* out, int* flags, void* stream) {
-    AFSI_REQUIRE(grid && points >= 0, "invalid interpolation arguments");
+                          const int32_t tile[3], const int32_t* perm, const int32_t* cell_ptr,
+                          const double* u, const double* v, const double* w, double* out,
+                          void* workspace, size_t workspace_bytes, int* flags, void* stream) {
+    AFSI_REQUIRE(grid && points >= 0 && points < (int64_t(1) << 31), "invalid interpolation arguments");
     if (points == 0) return OK;
     AFSI_REQUIRE(x && u && v && w && out, "null pointer");
-    interp_tiled_kernel<<<grid_for(points, 256, 16384), 256, 0, hipStream_t(stream)>>>(
-        x, points, perm, *grid, u, v, w, out, flags);
+    const afsi_grid3& g = *grid;
+    const hipStream_t s = hipStream_t(stream);
+    if (!perm) {
+        AFSI_REQUIRE((g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1) < (int64_t(1) << 31),
+                     "grid exceeds 2^31 faces per component");
+        interp_points_kernel<<<grid_for(points, 256, 16384), 256, 0, s>>>(x, points, g, u, v, w, out, flags);
+        AFSI_LAUNCH_CHECK();
+        return OK;
+    }
+    AFSI_REQUIRE(tile && cell_ptr && workspace, "binned interpolation needs tile, cell_ptr, workspace");
+    if (int r = check_tiled(g, tile)) return r;
+    AFSI_REQUIRE(workspace_bytes >= afsi_spread_workspace_bytes(grid, tile),
+                 "interpolation workspace too small");
+    const TileSpec ts = make_spec(g, tile);
+    AFSI_REQUIRE(size_t(ts.lds) * sizeof(double) <= 64 * 1024, "tile too large for LDS");
+    int32_t* chunk_start = static_cast<int32_t*>(workspace);
+    chunk_scan_kernel<<<1, 1024, 0, s>>>(cell_ptr, ts.ntiles, ts.tvol, chunk_start);
+    AFSI_LAUNCH_CHECK();
+    const int64_t upper = ts.ntiles + points / SPREAD_CHUNK + 1;
+    const unsigned blocks = unsigned(upper < 4096 ? upper : 4096);
+    const size_t shmem = size_t(ts.lds) * sizeof(double);
+    switch (cubic_edge(tile)) {
+    case 8:
+        interp_lds_kernel<8><<<blocks, SPREAD_THREADS, shmem, s>>>(x, perm, cell_ptr, chunk_start, g, ts,
+                                                                  u, v, w, out, flags);
+        break;
+    case 4:
+        interp_lds_kernel<4><<<blocks, SPREAD_THREADS, shmem, s>>>(x, perm, cell_ptr, chunk_start, g, ts,
+                                                                  u, v, w, out, flags);
+        break;
+    default:
+        interp_lds_kernel<0><<<blocks, SPREAD_THREADS, shmem, s>>>(x, perm, cell_ptr, chunk_start, g, ts,
+                                                                  u, v, w, out, flags);
+    }
     AFSI_LAUNCH_CHECK();
     return OK;
 }

@@ -276,13 +276,15 @@ class MixedMultigrid:
     iterative refinement.
 
     Each outer iteration runs one fused FP32 V-cycle on the FP32 copy of the
-    FP64 residual (zero initial guess), adds the FP32 correction to the FP64
-    iterate, and recomputes r = b - A x in FP64. The residual, the
-    accumulation of x and the removal of the constant null space (mean of
-    b and of the result) stay in FP64, so the iteration converges to the
-    FP64 tolerance max(atol, rtol*||b||) of ``GeometricMultigrid.solve``
-    (the FP32 cycle only has to reduce the error by the V-cycle factor).
-    ``solve`` keeps the signature, checks and messages of the FP64 solver.
+    FP64 residual (zero initial guess), then one fused pass adds the FP32
+    correction to the FP64 iterate (ping-pong buffers) and recomputes
+    r = b - A x in FP64; the residual norm is reduced only where the solve
+    checks convergence. The residual, the accumulation of x and the removal
+    of the constant null space (mean of b and of the result) stay in FP64,
+    so the iteration converges to the FP64 tolerance max(atol, rtol*||b||)
+    of ``GeometricMultigrid.solve`` (the FP32 cycle only has to reduce the
+    error by the V-cycle factor). ``solve`` keeps the signature, checks and
+    messages of the FP64 solver.
     """
 
     def __init__(self, grid, *, device='cuda', options=None, backend='graph'):
@@ -298,7 +300,8 @@ class MixedMultigrid:
         self.periodic = self.inner.periodic
         self.dtype = torch.float64
         shape = self.shapes[0]
-        self.x = torch.zeros(shape, device=self.device, dtype=torch.float64)
+        self.xs = [torch.zeros(shape, device=self.device, dtype=torch.float64) for _ in range(2)]
+        self.cur = 0
         self.b = torch.zeros(shape, device=self.device, dtype=torch.float64)
         self.partials = torch.zeros_like(self.inner.partials)
         self.results = torch.zeros((8, 4), device=self.device, dtype=torch.float64)
@@ -310,45 +313,58 @@ class MixedMultigrid:
         if backend == 'graph':
             self._capture()
 
+    @property
+    def x(self):
+        """The current FP64 iterate."""
+        return self.xs[self.cur]
+
     def _residual(self):
         _abi.call('afsi_refine_residual_3d', _abi.ptr(self.x), _abi.ptr(self.b),
                   _abi.ptr(self.inner.rhs[0]), self._n, self._w, int(self.periodic),
                   _abi.ptr(self.partials), self.partials.numel(), _abi.ptr(self.results[0]),
                   _abi.stream())
 
-    def _iteration(self):
+    def _iteration(self, start, reduce):
+        """FP32 V-cycle, then x_{1-start} = x_start + e and the new residual."""
         self.inner._cycles(1)
-        _abi.call('afsi_refine_update', _abi.ptr(self.x), _abi.ptr(self.inner.p[0]),
-                  self.x.numel(), _abi.stream())
-        self._residual()
+        _abi.call('afsi_refine_step_3d', _abi.ptr(self.xs[start]), _abi.ptr(self.inner.p[0]),
+                  _abi.ptr(self.b), _abi.ptr(self.xs[1 - start]), _abi.ptr(self.inner.rhs[0]),
+                  self._n, self._w, int(self.periodic), _abi.ptr(self.partials),
+                  self.partials.numel(), _abi.ptr(self.results[0]) if reduce else None,
+                  _abi.stream())
 
-    def _block(self, count):
-        for _ in range(count):
-            self._iteration()
+    def _block(self, start, count):
+        for i in range(count):
+            self._iteration((start + i) % 2, i == count - 1)
 
     def _capture(self):
-        self.x.zero_()
+        for x in self.xs:
+            x.zero_()
         self.b.zero_()
         stream = torch.cuda.Stream(device=self.device)
         stream.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(stream):
             self._residual()
-            for _ in range(2):
-                self._block(self.options.check_every)
+            for start in (0, 1):
+                self._block(start, self.options.check_every)
         torch.cuda.current_stream().wait_stream(stream)
-        for count in range(1, self.options.check_every + 1):
-            graph = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(graph, stream=stream):
-                self._block(count)
-            self.graphs[count] = graph
+        for start in (0, 1):
+            for count in range(1, self.options.check_every + 1):
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph, stream=stream):
+                    self._block(start, count)
+                self.graphs[start, count] = graph
         torch.cuda.current_stream().wait_stream(stream)
         torch.cuda.synchronize(self.device)
 
     def advance(self, count):
+        """``count`` outer iterations from the current iterate, then the
+        residual norm reduction."""
         if self.graphs:
-            self.graphs[count].replay()
+            self.graphs[self.cur, count].replay()
         else:
-            self._block(count)
+            self._block(self.cur, count)
+        self.cur = (self.cur + count) % 2
 
     def _reduce(self, t, slot):
         _abi.call('afsi_reduce_f64', _abi.ptr(t), t.numel(), _abi.ptr(self.partials),
@@ -371,6 +387,7 @@ class MixedMultigrid:
                                     or initial.dtype != rhs.dtype):
             raise ValueError('invalid initial pressure')
         self.b.copy_(rhs)
+        self.cur = 0
         if initial is None:
             self.x.zero_()
         else:

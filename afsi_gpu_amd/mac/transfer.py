@@ -136,8 +136,12 @@ class TiledTransfer:
         """(P,3) velocities U_c(p) = sum w_c(p) u_c in the caller's point order."""
         if out is None:
             out = torch.empty_like(points)
+        if bins is not None and tuple(bins.tile) != self.tile:
+            raise ValueError('bins were built with another tile')
         perm = bins.perm if bins is not None else None
-        _abi.call('afsi_interp_tiled_f64', _abi.ptr(points), points.shape[0], self._g,
-                  _abi.ptr(perm), _abi.ptr(velocity[0]), _abi.ptr(velocity[1]),
-                  _abi.ptr(velocity[2]), _abi.ptr(out), _abi.ptr(self.flags), _abi.stream())
+        cell_ptr = bins.cell_ptr if bins is not None else None
+        _abi.call('afsi_interp_tiled_f64', _abi.ptr(points), points.shape[0], self._g, self._tile,
+                  _abi.ptr(perm), _abi.ptr(cell_ptr), _abi.ptr(velocity[0]), _abi.ptr(velocity[1]),
+                  _abi.ptr(velocity[2]), _abi.ptr(out), _abi.ptr(self.workspace),
+                  self.workspace.numel(), _abi.ptr(self.flags), _abi.stream())
         return out

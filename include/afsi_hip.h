@@ -101,7 +101,11 @@ int afsi_bin_points(const double* x, int64_t points, const afsi_grid3* grid,
  * Spread: one work item per (tile, chunk of points); per-tile LDS
  * accumulation of all three components with ds_add_f64, then one global
  * atomic per touched face. u,v,w are overwritten (zero-filled first).
- * Interp: one thread per binned point, out (P,3) in original point order. */
+ * Interp: the same work items; each stages its tile's face box of all three
+ * components in LDS and gathers its points' 192 values from there; out (P,3)
+ * in the caller's point order. With perm = NULL (unbinned) one thread per
+ * point gathers from HBM and tile/cell_ptr/workspace are unused. Cubic
+ * tiles of edge 8 or 4 use compile-time box extents. */
 size_t afsi_spread_workspace_bytes(const afsi_grid3* grid, const int32_t tile[3]);
 int afsi_spread_tiled_f64(const double* x, const double* force, int64_t points,
                           const afsi_grid3* grid, const int32_t tile[3],
@@ -109,8 +113,9 @@ int afsi_spread_tiled_f64(const double* x, const double* force, int64_t points,
                           double* u, double* v, double* w, void* workspace,
                           size_t workspace_bytes, int* flags, void* stream);
 int afsi_interp_tiled_f64(const double* x, int64_t points, const afsi_grid3* grid,
-                          const int32_t* perm, const double* u, const double* v,
-                          const double* w, double* out, int* flags, void* stream);
+                          const int32_t tile[3], const int32_t* perm, const int32_t* cell_ptr,
+                          const double* u, const double* v, const double* w, double* out,
+                          void* workspace, size_t workspace_bytes, int* flags, void* stream);
 
 /* ---- 3D pressure multigrid kernels ---------------------------------------
  * Replace the MG kernel protocol of mac/mg_workspace.py:13-30 /
@@ -213,6 +218,11 @@ typedef struct afsi_mg3d_plan {
     int64_t partials_capacity; /* doubles in reduce_partials */
     int32_t zero_initial;      /* fused driver: treat p[0] as zero on the first cycle */
     int32_t reserved;
+    /* fused red-black sweeps, per level (0 = automatic): tile edge TX = TY
+     * (2 or 4) and workgroups per launch (persistent: each workgroup sweeps
+     * every G-th tile, overlapping the next tile's loads with compute). */
+    int32_t sweep_tile[AFSI_MG_MAX_LEVELS];
+    int32_t sweep_blocks[AFSI_MG_MAX_LEVELS];
 } afsi_mg3d_plan;
 /* 1 if the fused driver accepts the plan (every level with <= 4096 cells
  * that it solves in LDS must have extents <= 64), else 0. */
@@ -223,11 +233,11 @@ int64_t afsi_mg3d_partials_doubles(const int64_t n[3]);
  * sweeps (red-black: LDS tile + 2-cell halo, red on tile+1 ring then
  * black; or weighted Jacobi), one residual-restriction, one prolongation.
  * All levels with <= 4096 cells run as one 1024-thread workgroup out of
- * LDS (including the reference's mean removals and the dense coarse solve).
- * The fine levels skip the reference's intermediate mean subtractions
- * (mac/multigrid.py:107-123): the V-cycle commutes with adding a constant,
- * so they change only rounding; p[0] is returned with an arbitrary
- * constant offset that the caller removes once after the last cycle. */
+ * LDS (including the dense coarse solve). No level performs the
+ * reference's intermediate mean subtractions (mac/multigrid.py:107-123):
+ * the V-cycle commutes with adding a constant, so they change only
+ * rounding; p[0] is returned with an arbitrary constant offset that the
+ * caller removes once after the last cycle. */
 int afsi_mg3d_cycles_f64(const afsi_mg3d_plan* plan, int32_t cycles, void* stream);
 int afsi_mg3d_cycles_f32(const afsi_mg3d_plan* plan, int32_t cycles, void* stream);
 /* reduce_result = [sum, sum of squares, sum |.|, non-finite count] of
@@ -245,6 +255,14 @@ int afsi_mg3d_sweep_f32(const afsi_mg3d_plan* plan, int32_t level, void* stream)
 int afsi_refine_residual_3d(const double* x, const double* b, float* r32, const int64_t n[3],
                             const double inv_h2[3], int periodic, double* partials,
                             int64_t partials_capacity, double* result, void* stream);
+/* One fused refinement step: x_out = x + e and r32 = float(b - A x_out) in
+ * FP64 (neighbour values formed from the same x + e expression; identical
+ * to afsi_refine_update followed by afsi_refine_residual_3d). x_out must not
+ * alias x. result = NULL skips the reduction; otherwise result receives
+ * [sum r, sum r^2, sum |r|, non-finite] (partials as above). */
+int afsi_refine_step_3d(const double* x, const float* e, const double* b, double* x_out,
+                        float* r32, const int64_t n[3], const double inv_h2[3], int periodic,
+                        double* partials, int64_t partials_capacity, double* result, void* stream);
 /* x += e (FP64 accumulation of an FP32 correction). */
 int afsi_refine_update(double* x, const float* e, int64_t count, void* stream);
 /* Unfused driver with the reference workspace's launch structure (one
@@ -255,6 +273,54 @@ int afsi_mg3d_cycles_unfused_f64(const afsi_mg3d_plan* plan, int32_t cycles, voi
 int afsi_mg3d_cycles_unfused_f32(const afsi_mg3d_plan* plan, int32_t cycles, void* stream);
 int afsi_mg3d_residual_unfused_f64(const afsi_mg3d_plan* plan, void* stream);
 int afsi_mg3d_residual_unfused_f32(const afsi_mg3d_plan* plan, void* stream);
+
+/* ==== Batched mixed-precision pressure solve (build-own) ====================
+ * Many independent 3D instances of uneven size solved together: one launch
+ * per multigrid phase covers every instance (a device work list of
+ * (instance, tile/row) entries; each workgroup finds its entry by binary
+ * search), the coarse levels run one 1024-thread workgroup per instance,
+ * and convergence is tracked on the device (active[i] = 0 once instance i
+ * meets max(atol, rtol ||b_i||); inactive instances skip their work).
+ * Each item: the FP32 plan of its inner V-cycle (as for
+ * afsi_mg3d_cycles_f32, zero initial guess, red-black smoother, every
+ * level above the LDS-solved ones with even z extent <= 128), its FP64
+ * iterates x[2] (ping-pong) and FP64 right-hand side b. One outer
+ * iteration = one batched FP32 V-cycle on r32 = plan.rhs[0], then
+ * x_new = x + plan.p[0] and the FP64 residual (afsi_refine_step_3d). */
+typedef struct afsi_mgb_item {
+    afsi_mg3d_plan plan;
+    double* x[2];
+    double* b;
+} afsi_mgb_item;
+typedef struct afsi_mgb_batch {
+    int32_t count;
+    int32_t reserved;
+    const afsi_mgb_item* items;   /* host array of count items */
+    void* workspace;              /* device, afsi_mgb_workspace_bytes */
+    size_t workspace_bytes;
+    int32_t* active;              /* device, count */
+    double* tol;                  /* device, count */
+    double* results;              /* device, 16 per instance: [sum, sum sq, sum |.|, non-finite]
+                                     of the residual, of b before and after its mean removal,
+                                     and of the iterate whose mean was last removed */
+    double* partials;             /* device, afsi_mgb_partials_doubles */
+} afsi_mgb_batch;
+size_t afsi_mgb_workspace_bytes(const afsi_mgb_batch* batch);
+int64_t afsi_mgb_partials_doubles(const afsi_mgb_batch* batch);
+/* Builds the per-launch descriptor arrays and copies them to the workspace
+ * (synchronises the stream; call once after the buffers are allocated). */
+int afsi_mgb_prepare(const afsi_mgb_batch* batch, void* stream);
+/* b_i and x_i[0] lose their means (FP64), tol_i = max(atol, rtol ||b_i||),
+ * results[16i..16i+3] = residual stats of x_i[0], results[16i+4..16i+7] =
+ * stats of b_i before the mean removal, results[16i+12..] = stats of x_i[0]
+ * before its mean removal, active_i = residual > tol_i. */
+int afsi_mgb_begin(const afsi_mgb_batch* batch, double rtol, double atol, void* stream);
+/* `count` outer iterations from x[start] (the iterate alternates between
+ * x[0] and x[1]); the last one reduces the residual and updates active.
+ * Inactive instances copy their iterate so the alternation stays aligned. */
+int afsi_mgb_iterate(const afsi_mgb_batch* batch, int32_t start, int32_t count, void* stream);
+/* x_i[cur] loses its mean (FP64). */
+int afsi_mgb_finish(const afsi_mgb_batch* batch, int32_t cur, void* stream);
 
 /* ==== 2D staggered channel (mac2d) ========================================
  * Cells nx x ny, cell (x, y) at x*ny + y; faces u (nx+1, ny), v (nx, ny+1).

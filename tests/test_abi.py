@@ -44,3 +44,33 @@ def test_host_validation_without_device_work():
         g = _abi.Grid3()
         _abi.call('afsi_bin_points', None, 0, g, _abi.i32x3((0, 1, 1)), None, None, None, None, 0,
                   None, None)
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Size and field offsets of every ctypes mirror equal the C structs'
+    (compiled from include/afsi_hip.h with the host compiler)."""
+    import shutil
+    import subprocess
+    from afsi_gpu_amd import _abi
+    cxx = shutil.which('g++') or shutil.which('c++')
+    if cxx is None:
+        pytest.skip('no host C++ compiler')
+    mirrors = {'afsi_grid3': _abi.Grid3, 'afsi_grid2': _abi.Grid2, 'afsi_mg3d_plan': _abi.MGPlan3,
+               'afsi_mg2d_plan': _abi.MGPlan2, 'afsi_mgb_item': _abi.MGBItem,
+               'afsi_mgb_batch': _abi.MGBBatch}
+    lines = ['#include <cstdio>', '#include <cstddef>', '#include "afsi_hip.h"', 'int main() {']
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'  printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines.append('  return 0; }')
+    src = tmp_path / 'layout.cpp'
+    src.write_text('\n'.join(lines))
+    exe = tmp_path / 'layout'
+    subprocess.run([cxx, '-I', str(ROOT / 'include'), str(src), '-o', str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split('\n')
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in mirrors.items():
+        assert got[cname, 'size'] == ctypes.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert got[cname, field] == getattr(py, field).offset, (cname, field)

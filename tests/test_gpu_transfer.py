@@ -167,8 +167,12 @@ def test_tiled_spread_and_interp(gpu, periodic, tile):
     bins = bin_points(xt, g, tile)
     tr = TiledTransfer(g, gpu, tile)
     f = tr.spread(xt, Ft, bins)
-    U = tr.interpolate(xt, tuple(torch.tensor(a, device=gpu) for a in u), bins)
+    ut = tuple(torch.tensor(a, device=gpu) for a in u)
+    U = tr.interpolate(xt, ut, bins)
     assert int(tr.flags.item()) == 0
+    # unbinned path (one thread per point, HBM gathers): same weights and
+    # summation order as the LDS-staged binned path, so identical bits
+    assert torch.equal(tr.interpolate(xt, ut, None), U)
     for c in range(3):
         assert rel_err(f[c].cpu(), f_ref[c]) < TOL
         assert max_err(f[c].cpu(), f_ref[c]) < 10 * TOL
