@@ -15,7 +15,7 @@ compatibility/finite checks and messages, cloned results.
 per-instance binning, spread and interpolation, then the batched pressure
 solve.
 """
-from ctypes import c_double, c_void_p
+from ctypes import c_double
 from math import isfinite
 import torch
 
@@ -25,34 +25,50 @@ from .mac.multigrid import GeometricMultigrid, MGOptions
 from .mac.transfer import DEFAULT_TILE, Binner, TiledTransfer
 
 
+class MixedState:
+    """Resident per-instance storage of the batched mixed solve: the FP32
+    multigrid hierarchy of its inner V-cycle and its FP64 iterates and
+    right-hand side."""
+
+    def __init__(self, grid, *, device='cuda', options=None):
+        self.grid = grid
+        self.options = options or MGOptions(smooth=2, smoother='rbgs')
+        self.device = torch.device(device)
+        self.inner = GeometricMultigrid(grid, device=self.device, dtype=torch.float32,
+                                        options=self.options, backend='workspace')
+        if not self.inner.fused:
+            raise ValueError(f'grid {grid.shape} is not supported by the fused driver')
+        shape = self.inner.shapes[0]
+        self.x = [torch.zeros(shape, device=self.device, dtype=torch.float64) for _ in range(2)]
+        self.b = torch.zeros(shape, device=self.device, dtype=torch.float64)
+
+    @property
+    def allocated_bytes(self):
+        return self.inner.allocated_bytes + 3 * self.b.numel() * 8
+
+
 class BatchedMixedMultigrid:
-    def __init__(self, grids, *, device='cuda', options=None, graphs=True):
+    def __init__(self, grids=None, *, device='cuda', options=None, graphs=True, states=None):
         self.options = options or MGOptions(smooth=2, smoother='rbgs')
         if self.options.smoother != 'rbgs':
             raise ValueError('the batched solve uses the red-black smoother')
         self.device = torch.device(device)
-        self.grids = list(grids)
-        if not self.grids:
+        if states is None:
+            states = [MixedState(g, device=self.device, options=self.options) for g in (grids or [])]
+        self.states = list(states)
+        if not self.states:
             raise ValueError('empty batch')
-        self.inner = []
-        self.x, self.b = [], []
-        for g in self.grids:
-            mg = GeometricMultigrid(g, device=self.device, dtype=torch.float32,
-                                    options=self.options, backend='workspace')
-            if not mg.fused:
-                raise ValueError(f'grid {g.shape} is not supported by the fused driver')
-            self.inner.append(mg)
-            shape = mg.shapes[0]
-            self.x.append([torch.zeros(shape, device=self.device, dtype=torch.float64)
-                           for _ in range(2)])
-            self.b.append(torch.zeros(shape, device=self.device, dtype=torch.float64))
-        n = len(self.grids)
+        self.grids = [st.grid for st in self.states]
+        self.inner = [st.inner for st in self.states]
+        self.x = [st.x for st in self.states]
+        self.b = [st.b for st in self.states]
+        n = len(self.states)
         self.items = (_abi.MGBItem * n)()
-        for i, mg in enumerate(self.inner):
-            self.items[i].plan = mg.plan
-            self.items[i].x[0] = self.x[i][0].data_ptr()
-            self.items[i].x[1] = self.x[i][1].data_ptr()
-            self.items[i].b = self.b[i].data_ptr()
+        for i, st in enumerate(self.states):
+            self.items[i].plan = st.inner.plan
+            self.items[i].x[0] = st.x[0].data_ptr()
+            self.items[i].x[1] = st.x[1].data_ptr()
+            self.items[i].b = st.b.data_ptr()
         self.active = torch.zeros(n, device=self.device, dtype=torch.int32)
         self.tol = torch.zeros(n, device=self.device, dtype=torch.float64)
         self.results = torch.zeros((n, 16), device=self.device, dtype=torch.float64)
@@ -75,9 +91,9 @@ class BatchedMixedMultigrid:
 
     def _batch(self, workspace, nbytes, partials):
         bt = _abi.MGBBatch()
-        bt.count = len(self.grids)
+        bt.count = len(self.states)
         bt.items = _abi.cast_items(self.items)
-        bt.workspace = c_void_p(workspace.data_ptr()) if workspace is not None else None
+        bt.workspace = workspace.data_ptr() if workspace is not None else None
         bt.workspace_bytes = nbytes
         bt.active = self.active.data_ptr()
         bt.tol = self.tol.data_ptr()
@@ -87,9 +103,7 @@ class BatchedMixedMultigrid:
 
     @property
     def allocated_bytes(self):
-        fp32 = sum(mg.allocated_bytes for mg in self.inner)
-        fp64 = sum(3 * b.numel() * 8 for b in self.b)
-        return fp32 + fp64 + self.workspace.numel()
+        return sum(st.allocated_bytes for st in self.states) + self.workspace.numel()
 
     def _iterate(self, start, count):
         _abi.call('afsi_mgb_iterate', self.batch, int(start), int(count), _abi.stream())

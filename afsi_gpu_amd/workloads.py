@@ -132,3 +132,82 @@ def curves2d(gen, lengths=(4., 1.), shape=(256, 64), per_curve=2048):
         pts.append(np.stack([xs, y], 1))
     pts = np.concatenate(pts)
     return pts + gen.uniform(-.05 * h, .05 * h, size=pts.shape)
+
+
+# ---------------------------------------------------------------------------
+# many-instance workloads (batched runs and the request stream)
+
+GRIDS_3D = ((32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96))
+GRIDS_2D = ((128, 32), (256, 64), (512, 128))
+
+
+def random_shell(gen, count):
+    """A random centred ellipsoidal shell (semi-axes U(0.15, 0.35) each,
+    thickness 0.02) in the unit cube."""
+    return shell(gen, gen.uniform(.15, .35, size=3), count)
+
+
+def random_curves(gen, count, shape, lengths=(4., 1.)):
+    """Two sinusoidal curves y = 0.5 +- A sin(2 pi x / Lx + phi),
+    A ~ U(0.05, 0.15), phi ~ U(0, 2 pi), count points in total, with
+    U(-0.05 h, 0.05 h) jitter (h = Lx / nx)."""
+    h = lengths[0] / shape[0]
+    per = count // 2
+    amp, phase = gen.uniform(.05, .15), gen.uniform(0, 2 * np.pi)
+    pts = []
+    for k, n in ((1, per), (-1, count - per)):
+        xs = lengths[0] * (np.arange(n) + .5) / n
+        pts.append(np.stack([xs, .5 + k * amp * np.sin(2 * np.pi * xs / lengths[0] + phase)], 1))
+    pts = np.concatenate(pts)
+    return pts + gen.uniform(-.05 * h, .05 * h, size=pts.shape)
+
+
+def instance_spec(gen, kind):
+    """(grid shape, point count) of one instance: 'batch' draws the grid
+    uniformly from the 3D and 2D sets and points log-uniform in [1e4, 3e5];
+    'serve-2d' / 'serve-3d' / 'serve-128' draw the request-stream classes
+    (2D 128x32..512x128 with 2k-20k points; 3D 32^3-64^3 with 20k-200k;
+    128^3 with 0.5M-1.5M)."""
+    def loguniform(lo, hi):
+        return int(round(np.exp(gen.uniform(np.log(lo), np.log(hi)))))
+    if kind == 'batch':
+        shapes = GRIDS_3D + GRIDS_2D
+        shape = shapes[gen.integers(len(shapes))]
+        return shape, loguniform(1e4, 3e5)
+    if kind == 'serve-2d':
+        return GRIDS_2D[gen.integers(3)], loguniform(2e3, 2e4)
+    if kind == 'serve-3d':
+        return GRIDS_3D[gen.integers(3)], loguniform(2e4, 2e5)
+    if kind == 'serve-128':
+        return (128, 128, 128), loguniform(5e5, 1.5e6)
+    raise ValueError(kind)
+
+
+def instance_points(gen, shape, count):
+    """Positions and N(0,1) forces of an instance (shell in 3D, curves in 2D)."""
+    if len(shape) == 3:
+        x = random_shell(gen, count)
+    else:
+        x = random_curves(gen, count, shape)
+    return x, gen.normal(size=x.shape)
+
+
+def request_stream(seed, rate=200., duration=60., per_instance=10, live=20,
+                   mix=((.6, 'serve-2d'), (.3, 'serve-3d'), (.1, 'serve-128'))):
+    """Poisson arrivals at ``rate``/s over ``duration`` s. Arrival k belongs
+    to instance (k mod live) + live * (k // (live * per_instance)): ``live``
+    instances receive requests in turn, each ``per_instance`` requests.
+    Returns (arrival times, instance ids, {instance id: (shape, count, seed)})."""
+    gen = rng(seed)
+    times = np.cumsum(gen.exponential(1 / rate, size=int(rate * duration * 1.5) + 16))
+    times = times[times < duration]
+    k = np.arange(len(times))
+    inst = (k % live) + live * (k // (live * per_instance))
+    specs = {}
+    probs = np.array([p for p, _ in mix])
+    kinds = [kd for _, kd in mix]
+    for i in np.unique(inst):
+        kind = kinds[gen.choice(len(kinds), p=probs / probs.sum())]
+        shape, count = instance_spec(gen, kind)
+        specs[int(i)] = (shape, count, int(gen.integers(2 ** 31)))
+    return times, inst, specs

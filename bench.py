@@ -11,6 +11,12 @@ interpolation -> x += dt U. Two precisions on identical inputs:
 Inputs are resident in HBM before the timed region; K steps per path are
 timed between device synchronisations. `value` is the mixed path's steps/s.
 
+`batched` (config 4): 64 uneven instances stepped together (3D: one
+batched mixed pressure solve, one launch per multigrid phase) against the
+same instances stepped one at a time on the plain path. `serving`
+(config 5, bounded sample): a Poisson request stream served with
+continuous batching, p50/p99 latency.
+
 `roofline` is for the dominant kernel of the mixed step, the level-0
 red-black sweep (algorithmic bytes: read p and rhs, write p = 3 x 4 B per
 cell in FP32), timed with HIP events on the stream it is launched on;
@@ -57,6 +63,10 @@ def parse():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--kernels-only', type=int, default=0, metavar='R',
                     help='only launch the roofline sweep and a copy kernel R times each')
+    ap.add_argument('--serve-seconds', type=float, default=5.,
+                    help='arrival window of the served-latency sample (0: skip)')
+    ap.add_argument('--batch-instances', type=int, default=64,
+                    help='instances of the batched sample (0: skip)')
     return ap.parse_args()
 
 
@@ -146,6 +156,101 @@ def time_sweep(torch, solver, repeats=50, batches=5):
     return float(np.mean([a.elapsed_time(b) for a, b in zip(start, end)])) / 1e3 / repeats
 
 
+def serving_sample(torch, seconds, rate=200.):
+    """Config 5 request stream (Poisson arrivals at `rate`/s; 60% 2D
+    128x32..512x128 with 2k-20k points, 30% 3D 32^3-64^3 with 20k-200k,
+    10% 128^3 with 0.5M-1.5M; 10 requests per instance) over a window of
+    `seconds`, served with continuous batching; latency percentiles."""
+    from afsi_gpu_amd import workloads
+    from afsi_gpu_amd.serve import Server
+    times, inst, specs = workloads.request_stream(workloads.GEOMETRY_SEED + 5, rate=rate,
+                                                  duration=seconds)
+    warm_t, warm_i, warm_s = workloads.request_stream(workloads.GEOMETRY_SEED + 6, rate=rate,
+                                                      duration=.2)
+    Server(device='cuda:0').run(warm_t, warm_i, warm_s)  # compile/cache warm-up
+    stats = Server(device='cuda:0').run(times, inst, specs)
+    out = stats.summary()
+    out.update(rate_per_s=rate, window_s=seconds,
+               sample=f'{len(times)} requests over {seconds:g} s of the 60 s stream, '
+                      f'{len(specs)} instances')
+    return out
+
+
+def batched_sample(torch, instances, steps=3, warmup=1):
+    """Config 4: `instances` independent instances, grids uniform over
+    {32^3, 48^3, 64^3, 96^3, 128x32, 256x64, 512x128}, points log-uniform in
+    [1e4, 3e5] (random shells / sinusoidal curves), N(0,1) forces. Batched:
+    3D steps with one batched mixed pressure solve (one launch per phase),
+    2D channel steps per instance; against the same instances stepped one
+    at a time on the plain FP64 path."""
+    from afsi_gpu_amd import workloads
+    from afsi_gpu_amd.batch import BatchedIBStep
+    from afsi_gpu_amd.mac import MACGrid, MGOptions
+    from afsi_gpu_amd.mac2d import ChannelGrid, ChannelIBStep
+    from afsi_gpu_amd.step import IBStep
+    dev = torch.device('cuda:0')
+    opts = MGOptions(smooth=2, smoother='rbgs')
+    gen = workloads.rng(workloads.GEOMETRY_SEED + 4)
+    specs = [workloads.instance_spec(gen, 'batch') for _ in range(instances)]
+    data = [workloads.instance_points(gen, shape, count) for shape, count in specs]
+    i3 = [i for i, (s, _) in enumerate(specs) if len(s) == 3]
+    i2 = [i for i, (s, _) in enumerate(specs) if len(s) == 2]
+    points = sum(c for _, c in specs)
+    g3 = [MACGrid(specs[i][0], (1., 1., 1.), periodic=True) for i in i3]
+    g2 = [ChannelGrid(specs[i][0], (4., 1.), periodic=True) for i in i2]
+    s2 = [ChannelIBStep(g, dt=1e-3, device=dev, options=opts) for g in g2]
+
+    def fresh():
+        xs = [torch.tensor(data[i][0], device=dev) for i in range(instances)]
+        fs = [torch.tensor(data[i][1], device=dev) for i in range(instances)]
+        return xs, fs
+
+    def run(step_all):
+        xs, fs = fresh()
+        state = {}
+        for k in range(warmup + steps):
+            if k == warmup:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+            step_all(xs, fs, state)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    batch3 = BatchedIBStep(g3, [specs[i][1] for i in i3], dt=1e-3, device=dev, options=opts)
+
+    def batched(xs, fs, state):
+        v3 = state.setdefault('v3', [g.zeros(device=dev) for g in g3])
+        p3 = state.get('p3')
+        v3, p3, _, _ = batch3.step([xs[i] for i in i3], [fs[i] for i in i3], v3, p3)
+        state['p3'] = p3
+        for j, i in enumerate(i2):
+            v = state.setdefault(('v2', j), g2[j].zeros(device=dev))
+            state[('v2', j)], state[('p2', j)], _, _ = s2[j].step(xs[i], fs[i], v, state.get(('p2', j)))
+
+    plain3 = [IBStep(g, specs[i][1], dt=1e-3, device=dev, options=opts, precision='fp64')
+              for g, i in zip(g3, i3)]
+
+    def sequential(xs, fs, state):
+        for j, i in enumerate(i3):
+            v = state.setdefault(('v3', j), g3[j].zeros(device=dev))
+            v, p, _, _ = plain3[j].step(xs[i], fs[i], v, state.get(('p3', j)))
+            state[('v3', j)], state[('p3', j)] = v, p
+        for j, i in enumerate(i2):
+            v = state.setdefault(('v2', j), g2[j].zeros(device=dev))
+            state[('v2', j)], state[('p2', j)], _, _ = s2[j].step(xs[i], fs[i], v, state.get(('p2', j)))
+
+    t_batch = run(batched)
+    t_seq = run(sequential)
+    return dict(instances=instances, instances_3d=len(i3), instances_2d=len(i2), points=points,
+                batched=dict(steps_per_s=1 / t_batch, instance_steps_per_s=instances / t_batch,
+                             mpts_per_s=points / t_batch / 1e6, ms_per_step=t_batch * 1e3),
+                sequential_plain=dict(steps_per_s=1 / t_seq, instance_steps_per_s=instances / t_seq,
+                                      mpts_per_s=points / t_seq / 1e6, ms_per_step=t_seq * 1e3),
+                speedup=t_seq / t_batch,
+                note='one step = every instance advanced once; 3D batched = mixed precision with '
+                     'one launch per multigrid phase, 2D = FP64 channel steps in both runs')
+
+
 def kernels_only(torch, n, repeats):
     """For hardware-counter passes: `repeats` launches of the level-0 FP32
     red-black sweep at n^3 (the roofline kernel), then `repeats` launches of
@@ -232,6 +337,10 @@ def main():
         accuracy=dict(interp_velocity_rel_l2_mixed_vs_fp64=u_err,
                       face_velocity_rel_l2_mixed_vs_fp64=face_err,
                       divergence_rel_fp64=div64 / div0, divergence_rel_mixed=divmx / div0))
+    if args.batch_instances:
+        result['batched'] = batched_sample(torch, args.batch_instances)
+    if args.serve_seconds > 0:
+        result['serving'] = serving_sample(torch, args.serve_seconds)
     if not args.no_cpu_baseline:
         result['cpu_baseline'] = cpu_baseline(x_np, F_np, n, int(round(plain['mg_cycles_mean'])))
     print(json.dumps(result))
