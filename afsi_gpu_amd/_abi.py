@@ -38,6 +38,17 @@ class MGPlan3(Structure):
                 ('sweep_tile', c_int32 * MAX_LEVELS), ('sweep_blocks', c_int32 * MAX_LEVELS)]
 
 
+class TBItem(Structure):
+    _fields_ = [('x', c_void_p), ('force', c_void_p), ('face', c_void_p * 3), ('out', c_void_p),
+                ('points', c_int64), ('grid', Grid3)]
+
+
+class TBBatch(Structure):
+    _fields_ = [('count', c_int32), ('tile', c_int32 * 3), ('items', POINTER(TBItem)),
+                ('keys_sorted', c_void_p), ('perm', c_void_p), ('cell_ptr', c_void_p),
+                ('workspace', c_void_p), ('workspace_bytes', c_size_t), ('flags', c_void_p)]
+
+
 class MGBItem(Structure):
     _fields_ = [('plan', MGPlan3), ('x', c_void_p * 2), ('b', c_void_p)]
 
@@ -127,6 +138,11 @@ _SIGNATURES = {
     'afsi_mg3d_cycles_unfused_f32': (c_int, [POINTER(MGPlan3), c_int32, _P]),
     'afsi_mg3d_residual_unfused_f64': (c_int, [POINTER(MGPlan3), _P]),
     'afsi_mg3d_residual_unfused_f32': (c_int, [POINTER(MGPlan3), _P]),
+    'afsi_tb_sizes': (c_int, [POINTER(TBBatch), POINTER(c_int64), POINTER(c_int64), POINTER(c_size_t)]),
+    'afsi_tb_prepare': (c_int, [POINTER(TBBatch), _P]),
+    'afsi_tb_bin': (c_int, [POINTER(TBBatch), _P]),
+    'afsi_tb_spread': (c_int, [POINTER(TBBatch), _P]),
+    'afsi_tb_interp': (c_int, [POINTER(TBBatch), _P]),
     'afsi_mgb_workspace_bytes': (c_size_t, [POINTER(MGBBatch)]),
     'afsi_mgb_partials_doubles': (c_int64, [POINTER(MGBBatch)]),
     'afsi_mgb_prepare': (c_int, [POINTER(MGBBatch), _P]),

@@ -11,18 +11,19 @@ contract of ``MixedMultigrid.solve`` per instance: FP32 V-cycles inside
 FP64 iterative refinement, tolerance max(atol, rtol ||b||), the same
 compatibility/finite checks and messages, cloned results.
 
-``BatchedIBStep`` advances a list of 3D instances by one IB step each:
-per-instance binning, spread and interpolation, then the batched pressure
-solve.
+``BatchedTransfer`` bins, spreads and interpolates all instances with
+one launch per phase (one key space, one work list of (instance, tile,
+chunk)). ``BatchedIBStep`` advances a list of 3D instances by one IB step
+each with both.
 """
-from ctypes import c_double
+from ctypes import byref, c_double, c_int64, c_size_t
 from math import isfinite
 import torch
 
 from . import _abi
 from .mac.flow import correct, divergence
 from .mac.multigrid import GeometricMultigrid, MGOptions
-from .mac.transfer import DEFAULT_TILE, Binner, TiledTransfer
+from .mac.transfer import DEFAULT_TILE
 
 
 class MixedState:
@@ -197,43 +198,129 @@ class BatchedMixedMultigrid:
         return out, infos
 
 
+class BatchedTransfer:
+    """Binning, spread and interpolation of many 3D instances with one
+    launch per phase (``afsi_tb_*``). ``bind`` records the per-instance
+    tensors (positions, forces, face arrays, point-velocity outputs) and
+    uploads the descriptors; call it again when any tensor changes."""
+
+    def __init__(self, grids, counts, *, device='cuda', tile=DEFAULT_TILE):
+        self.grids = list(grids)
+        self.counts = [int(c) for c in counts]
+        self.device = torch.device(device)
+        self.tile = tuple(int(t) for t in tile)
+        self.total = sum(self.counts)
+        self.items = None
+        self.bound = None
+        self.flags = torch.zeros((), device=self.device, dtype=torch.int32)
+
+    def bind(self, xs, forces, faces, outs):
+        n = len(self.grids)
+        items = (_abi.TBItem * n)()
+        for i in range(n):
+            if xs[i].shape != (self.counts[i], 3):
+                raise ValueError('positions must match the bound point counts')
+            items[i].x = xs[i].data_ptr()
+            items[i].force = forces[i].data_ptr()
+            for c in range(3):
+                items[i].face[c] = faces[i][c].data_ptr()
+            items[i].out = outs[i].data_ptr()
+            items[i].points = self.counts[i]
+            items[i].grid = self.grids[i].abi()
+        bt = _abi.TBBatch()
+        bt.count = n
+        for a in range(3):
+            bt.tile[a] = self.tile[a]
+        bt.items = ctypes_items(items)
+        total, keys, nbytes = c_int64(), c_int64(), c_size_t()
+        _abi.call('afsi_tb_sizes', bt, byref(total), byref(keys), byref(nbytes))
+        if self.bound is None or self.bound[0] != (keys.value, nbytes.value):
+            self.keys = torch.empty(max(total.value, 1), device=self.device, dtype=torch.int32)
+            self.perm = torch.empty(max(total.value, 1), device=self.device, dtype=torch.int32)
+            self.cell_ptr = torch.empty(keys.value + 1, device=self.device, dtype=torch.int32)
+            self.workspace = torch.empty(nbytes.value, device=self.device, dtype=torch.uint8)
+        bt.keys_sorted = self.keys.data_ptr()
+        bt.perm = self.perm.data_ptr()
+        bt.cell_ptr = self.cell_ptr.data_ptr()
+        bt.workspace = self.workspace.data_ptr()
+        bt.workspace_bytes = self.workspace.numel()
+        bt.flags = self.flags.data_ptr()
+        _abi.call('afsi_tb_prepare', bt, _abi.stream())
+        self.items, self.batch = items, bt
+        self.bound = ((keys.value, nbytes.value), tuple(t.data_ptr() for t in xs))
+        return self
+
+    def share_bins(self, other):
+        """Use ``other``'s sorted keys, perm and cell_ptr (same instances)."""
+        self.batch.keys_sorted = other.keys.data_ptr()
+        self.batch.perm = other.perm.data_ptr()
+        self.batch.cell_ptr = other.cell_ptr.data_ptr()
+        self.batch.flags = other.flags.data_ptr()
+        self.keys, self.perm, self.cell_ptr = other.keys, other.perm, other.cell_ptr
+        return self
+
+    def bin(self):
+        _abi.call('afsi_tb_bin', self.batch, _abi.stream())
+
+    def spread(self):
+        _abi.call('afsi_tb_spread', self.batch, _abi.stream())
+
+    def interpolate(self):
+        _abi.call('afsi_tb_interp', self.batch, _abi.stream())
+
+
+def ctypes_items(array):
+    from ctypes import POINTER, cast
+    return cast(array, POINTER(_abi.TBItem))
+
+
 class BatchedIBStep:
-    """One IB step for each of many 3D instances (FP64 transfers, batched
-    mixed-precision pressure solve). ``grids[i]`` is a ``MACGrid`` and
-    ``counts[i]`` its number of points."""
+    """One IB step for each of many 3D instances: one launch per phase for
+    binning, spread, the multigrid phases of the mixed-precision pressure
+    solve and interpolation (per-instance launches only for the predictor
+    axpy, divergence and gradient correction). ``grids[i]`` is a
+    ``MACGrid`` and ``counts[i]`` its number of points."""
 
     def __init__(self, grids, counts, *, dt, rho=1., device='cuda', options=None,
                  tile=DEFAULT_TILE, graphs=True):
         self.grids = list(grids)
+        self.counts = [int(c) for c in counts]
         self.dt, self.rho = float(dt), float(rho)
         self.device = torch.device(device)
         self.solver = BatchedMixedMultigrid(self.grids, device=self.device, options=options,
                                             graphs=graphs)
-        self.binners = [Binner(g, c, self.device, tile) for g, c in zip(self.grids, counts)]
-        self.transfers = [TiledTransfer(g, self.device, tile) for g in self.grids]
         self.density = [g.zeros(device=self.device) for g in self.grids]
         self.rhs = [torch.empty(g.shape, device=self.device, dtype=torch.float64) for g in self.grids]
         self.point_velocity = [torch.empty((c, 3), device=self.device, dtype=torch.float64)
-                               for c in counts]
+                               for c in self.counts]
+        self.spreader = BatchedTransfer(self.grids, self.counts, device=self.device, tile=tile)
+        self.gatherer = BatchedTransfer(self.grids, self.counts, device=self.device, tile=tile)
+        self._key = None
+
+    def _bind(self, xs, forces, velocities):
+        key = tuple(t.data_ptr() for t in (*xs, *forces, *(u for v in velocities for u in v)))
+        if key != self._key:
+            self.spreader.bind(xs, forces, self.density, self.point_velocity)
+            self.gatherer.bind(xs, forces, velocities, self.point_velocity)
+            self.gatherer.share_bins(self.spreader)
+            self._key = key
 
     @torch.no_grad()
     def step(self, xs, forces, velocities, pressures=None, *, advance=True):
         a = self.dt / self.rho
-        bins = []
+        self._bind(xs, forces, velocities)
+        self.spreader.bin()
+        self.spreader.spread()
         for i, g in enumerate(self.grids):
-            b = self.binners[i](xs[i])
-            bins.append(b)
-            f = self.transfers[i].spread(xs[i], forces[i], b, out=self.density[i])
-            for uc, fc in zip(velocities[i], f):
+            for uc, fc in zip(velocities[i], self.density[i]):
                 _abi.call('afsi_axpy_f64', _abi.ptr(uc), _abi.ptr(fc), uc.numel(), a, _abi.stream())
             divergence(velocities[i], g, -self.rho / self.dt, out=self.rhs[i])
         ps, infos = self.solver.solve(self.rhs, pressures)
-        Us = []
         for i, g in enumerate(self.grids):
             correct(velocities[i], ps[i], g, a)
-            U = self.transfers[i].interpolate(xs[i], velocities[i], bins[i], out=self.point_velocity[i])
-            if advance:
-                _abi.call('afsi_axpy_f64', _abi.ptr(xs[i]), _abi.ptr(U), xs[i].numel(), self.dt,
-                          _abi.stream())
-            Us.append(U)
-        return velocities, ps, Us, infos
+        self.gatherer.interpolate()
+        if advance:
+            for i in range(len(self.grids)):
+                _abi.call('afsi_axpy_f64', _abi.ptr(xs[i]), _abi.ptr(self.point_velocity[i]),
+                          xs[i].numel(), self.dt, _abi.stream())
+        return velocities, ps, self.point_velocity, infos

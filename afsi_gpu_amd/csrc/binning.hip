@@ -13,7 +13,7 @@
 // same digit are found by AND-ing eight __ballot masks, their order is the
 // lane order, and waves/rounds are combined in index order through LDS
 // counters, so equal keys keep their input order (a stable sort).
-#include "common.hpp"
+#include "binshare.hpp"
 
 namespace afsi {
 
@@ -32,10 +32,6 @@ static int64_t key_space(const afsi_grid3& g, const int32_t t[3]) {
            int64_t(t[0]) * t[1] * t[2];
 }
 
-struct TileGeom {
-    int32_t t[3];
-    int64_t nt[3];
-};
 
 __device__ inline int64_t cell_of(double xa, double o, double h, int64_t n, bool periodic,
                                   int* flags) {
@@ -53,25 +49,48 @@ __device__ inline int64_t cell_of(double xa, double o, double h, int64_t n, bool
     return c;
 }
 
+// Tile-major key of point p of one instance.
+__device__ inline uint32_t point_key(const double* __restrict__ x, int64_t p, const afsi_grid3& g,
+                                     const TileGeom& tg, int* flags) {
+    int64_t c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double xa = x[3 * p + a];
+        if (!g.periodic) {
+            const double s0 = (xa - g.origin[a]) / g.h[a];
+            if (!(s0 >= 2.0) || !(s0 < double(g.n[a]) - 2.0)) raise_flag(flags, F_SUPPORT);
+        }
+        c[a] = cell_of(xa, g.origin[a], g.h[a], g.n[a], g.periodic != 0, flags);
+    }
+    const int64_t tid = ((c[0] / tg.t[0]) * tg.nt[1] + c[1] / tg.t[1]) * tg.nt[2] + c[2] / tg.t[2];
+    const int64_t local = ((c[0] % tg.t[0]) * tg.t[1] + c[1] % tg.t[1]) * tg.t[2] + c[2] % tg.t[2];
+    return uint32_t(tid * (int64_t(tg.t[0]) * tg.t[1] * tg.t[2]) + local);
+}
+
 __global__ void keys_kernel(const double* __restrict__ x, int64_t points, afsi_grid3 g,
                             TileGeom tg, uint32_t* __restrict__ keys,
                             uint32_t* __restrict__ vals, int* flags) {
     for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < points;
          p += int64_t(gridDim.x) * blockDim.x) {
-        int64_t c[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const double xa = x[3 * p + a];
-            if (!g.periodic) {
-                const double s0 = (xa - g.origin[a]) / g.h[a];
-                if (!(s0 >= 2.0) || !(s0 < double(g.n[a]) - 2.0)) raise_flag(flags, F_SUPPORT);
-            }
-            c[a] = cell_of(xa, g.origin[a], g.h[a], g.n[a], g.periodic != 0, flags);
-        }
-        const int64_t tid = ((c[0] / tg.t[0]) * tg.nt[1] + c[1] / tg.t[1]) * tg.nt[2] + c[2] / tg.t[2];
-        const int64_t local = ((c[0] % tg.t[0]) * tg.t[1] + c[1] % tg.t[1]) * tg.t[2] + c[2] % tg.t[2];
-        keys[p] = uint32_t(tid * (int64_t(tg.t[0]) * tg.t[1] * tg.t[2]) + local);
+        keys[p] = point_key(x, p, g, tg, flags);
         vals[p] = uint32_t(p);
+    }
+}
+
+
+__global__ void keys_batch_kernel(const BinDesc* __restrict__ desc, int n, int64_t total,
+                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < total;
+         j += int64_t(gridDim.x) * blockDim.x) {
+        int lo = 0, hi = n;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (desc[mid].begin <= j) lo = mid;
+            else hi = mid;
+        }
+        const BinDesc& d = desc[lo];
+        keys[j] = uint32_t(d.key_offset) + point_key(d.x, j - d.begin, d.g, d.tg, d.flags);
+        vals[j] = uint32_t(j);
     }
 }
 
@@ -202,6 +221,85 @@ size_t bin_workspace_bytes(int64_t points) {
            + align256(RADIX * 4);                  // totals
 }
 
+// LSD radix sort of (keys, vals) in k0/v0 (ping-pong with k1/v1; the last
+// pass writes keys_sorted/perm), then cell_ptr over [0, space].
+static int sort_and_index(int64_t points, int64_t space, uint32_t* k0, uint32_t* v0, uint32_t* k1,
+                          uint32_t* v1, uint32_t* hist, uint32_t* total, uint32_t* keys_sorted,
+                          int32_t* perm, int32_t* cell_ptr, hipStream_t s) {
+    const int64_t nb = sort_blocks(points);
+    int bits = 0;
+    while ((int64_t(1) << bits) < space) ++bits;
+    int passes = (bits + 7) / 8;
+    if (passes < 1) passes = 1;
+    uint32_t* kin = k0;
+    uint32_t* vin = v0;
+    for (int pass = 0; pass < passes; ++pass) {
+        const bool last = pass == passes - 1;
+        uint32_t* kout = last ? keys_sorted : (kin == k0 ? k1 : k0);
+        uint32_t* vout = last ? reinterpret_cast<uint32_t*>(perm) : (vin == v0 ? v1 : v0);
+        radix_hist_kernel<<<unsigned(nb), SORT_THREADS, 0, s>>>(kin, points, 8 * pass, hist, nb);
+        AFSI_LAUNCH_CHECK();
+        radix_scan_kernel<<<RADIX, SORT_THREADS, 0, s>>>(hist, nb, total);
+        AFSI_LAUNCH_CHECK();
+        radix_scatter_kernel<<<unsigned(nb), SORT_THREADS, 0, s>>>(kin, vin, points, 8 * pass,
+                                                                   hist, total, nb, kout, vout);
+        AFSI_LAUNCH_CHECK();
+        kin = kout;
+        vin = vout;
+    }
+    cell_ptr_kernel<<<grid_for(space + 1, 256, 8192), 256, 0, s>>>(keys_sorted, points, space,
+                                                                     cell_ptr);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+// Workspace slices: keys/vals ping-pong, per-block histograms, totals.
+struct SortSpace {
+    uint32_t *k0, *v0, *k1, *v1, *hist, *total;
+};
+
+static SortSpace sort_space(void* workspace, int64_t points) {
+    char* ws = static_cast<char*>(workspace);
+    const size_t kb = align256(size_t(points) * 4);
+    SortSpace w;
+    w.k0 = reinterpret_cast<uint32_t*>(ws);
+    w.v0 = reinterpret_cast<uint32_t*>(ws + kb);
+    w.k1 = reinterpret_cast<uint32_t*>(ws + 2 * kb);
+    w.v1 = reinterpret_cast<uint32_t*>(ws + 3 * kb);
+    w.hist = reinterpret_cast<uint32_t*>(ws + 4 * kb);
+    w.total = reinterpret_cast<uint32_t*>(ws + 4 * kb + align256(size_t(sort_blocks(points)) * RADIX * 4));
+    return w;
+}
+
+int64_t bin_key_space(const afsi_grid3& g, const int32_t tile[3]) { return key_space(g, tile); }
+
+TileGeom bin_tile_geom(const afsi_grid3& g, const int32_t tile[3]) {
+    TileGeom tg;
+    for (int a = 0; a < 3; ++a) {
+        tg.t[a] = tile[a];
+        tg.nt[a] = tile_count(g, tile, a);
+    }
+    return tg;
+}
+
+int bin_batch(const BinDesc* desc_device, int count, int64_t total_points, int64_t space,
+              uint32_t* keys_sorted, int32_t* perm, int32_t* cell_ptr, void* sort_workspace,
+              hipStream_t s) {
+    AFSI_REQUIRE(total_points < (int64_t(1) << 31) && space < (int64_t(1) << 31),
+                 "batch exceeds 2^31 points or keys");
+    if (total_points == 0) {
+        cell_ptr_kernel<<<grid_for(space + 1, 256, 8192), 256, 0, s>>>(keys_sorted, 0, space, cell_ptr);
+        AFSI_LAUNCH_CHECK();
+        return OK;
+    }
+    const SortSpace w = sort_space(sort_workspace, total_points);
+    keys_batch_kernel<<<grid_for(total_points, 256, 8192), 256, 0, s>>>(desc_device, count, total_points,
+                                                                         w.k0, w.v0);
+    AFSI_LAUNCH_CHECK();
+    return sort_and_index(total_points, space, w.k0, w.v0, w.k1, w.v1, w.hist, w.total, keys_sorted,
+                          perm, cell_ptr, s);
+}
+
 }  // namespace afsi
 
 using namespace afsi;
@@ -252,31 +350,7 @@ int afsi_bin_points(const double* x, int64_t points, const afsi_grid3* grid,
     }
     keys_kernel<<<grid_for(points, 256, 8192), 256, 0, s>>>(x, points, *grid, tg, k0, v0, flags);
     AFSI_LAUNCH_CHECK();
-    int bits = 0;
-    while ((int64_t(1) << bits) < space) ++bits;
-    int passes = (bits + 7) / 8;
-    if (passes < 1) passes = 1;
-    // Even pass count ends in k0/v0; copy target chosen so the last pass
-    // writes the caller's keys_sorted / perm arrays directly.
-    uint32_t* kin = k0;
-    uint32_t* vin = v0;
-    for (int pass = 0; pass < passes; ++pass) {
-        const bool last = pass == passes - 1;
-        uint32_t* kout = last ? keys_sorted : (kin == k0 ? k1 : k0);
-        uint32_t* vout = last ? reinterpret_cast<uint32_t*>(perm) : (vin == v0 ? v1 : v0);
-        radix_hist_kernel<<<unsigned(nb), SORT_THREADS, 0, s>>>(kin, points, 8 * pass, hist, nb);
-        AFSI_LAUNCH_CHECK();
-        radix_scan_kernel<<<RADIX, SORT_THREADS, 0, s>>>(hist, nb, total);
-        AFSI_LAUNCH_CHECK();
-        radix_scatter_kernel<<<unsigned(nb), SORT_THREADS, 0, s>>>(kin, vin, points, 8 * pass,
-                                                                   hist, total, nb, kout, vout);
-        AFSI_LAUNCH_CHECK();
-        kin = kout;
-        vin = vout;
-    }
-    cell_ptr_kernel<<<grid_for(space + 1, 256, 8192), 256, 0, s>>>(keys_sorted, points, space,
-                                                                     cell_ptr);
-    AFSI_LAUNCH_CHECK();
+    return sort_and_index(points, space, k0, v0, k1, v1, hist, total, keys_sorted, perm, cell_ptr, s);
     return OK;
 }
 

@@ -2,22 +2,24 @@
 //
 // Spread: the work list is (tile, chunk of <= SPREAD_CHUNK points); a
 // persistent grid walks it. A work item accumulates the contributions of its
-// points for all three components into an LDS box covering every face the
-// tile's points can touch (tile + 2 cells on each side; one less on the
-// component's own axis), with ds_add_f64, then adds each nonzero LDS entry to
-// HBM with one global atomic. Dense tiles therefore collapse up to thousands
-// of point contributions into one global update per face, and a point is
-// read and its weights evaluated exactly once.
+// points for all three components into an FP64 LDS box covering every face
+// the tile's points can touch (tile + 2 cells on each side; one less on the
+// component's own axis), then adds each nonzero LDS entry to HBM with one
+// FP64 global atomic. Dense tiles therefore collapse many point
+// contributions into one global update per face, and a point is read and
+// its weights evaluated exactly once. Time is set by the LDS atomics
+// (192 per point at about 2.4 lanes/clock/CU for scattered addresses).
 //
-// Interp: one thread per binned point (sorted order, so neighbouring lanes
-// read neighbouring faces through L1/L2), 64 links per component in the
-// link order of the reference's compact kernel, result scattered back to the
-// caller's point order.
+// Interp: work items of <= INTERP_CHUNK points; the tile's face box is
+// staged in LDS and each binned point gathers its 64 links per component
+// from there (link order of the reference's compact kernel); results go to
+// the caller's point order.
 //
 // Weights: per axis f = s - base - 1 and the four Peskin weights from the
 // shared radical sqrt(1+4f-4f^2); base = floor(s-1) exactly as the
 // reference (mac/compact_transfer.py:161-166) with s=(x-o)/h.
-#include "common.hpp"
+#include "binshare.hpp"
+#include <vector>
 
 namespace afsi {
 
@@ -25,7 +27,13 @@ template <class T>
 int launch_fill(T* x, int64_t count, T value, hipStream_t s);
 
 constexpr int SPREAD_THREADS = 256;
-constexpr int SPREAD_CHUNK = 2048;
+
+// Points per work item: the spread's items are short so that items of
+// dense tiles spread over more workgroups (its time is set by LDS atomic
+// throughput, and small items balance it); interpolation stages the whole
+// face box per item, so its items are longer.
+constexpr int SPREAD_CHUNK = 512;
+constexpr int INTERP_CHUNK = 2048;
 
 struct TileSpec {
     int32_t t[3];
@@ -102,7 +110,7 @@ __device__ inline void point_weights(const double xp[3], const afsi_grid3& g, Po
 }
 
 __global__ void chunk_scan_kernel(const int32_t* __restrict__ cell_ptr, int64_t ntiles,
-                                  int64_t tvol, int32_t* __restrict__ chunk_start) {
+                                  int64_t tvol, int chunk, int32_t* __restrict__ chunk_start) {
     // Single workgroup: exclusive scan of ceil(points_in_tile / CHUNK).
     __shared__ int32_t s[1024];
     int32_t running = 0;
@@ -111,7 +119,7 @@ __global__ void chunk_scan_kernel(const int32_t* __restrict__ cell_ptr, int64_t 
         int32_t v = 0;
         if (t < ntiles) {
             const int32_t n = cell_ptr[(t + 1) * tvol] - cell_ptr[t * tvol];
-            v = (n + SPREAD_CHUNK - 1) / SPREAD_CHUNK;
+            v = (n + chunk - 1) / chunk;
         }
         s[threadIdx.x] = v;
         __syncthreads();
@@ -136,7 +144,8 @@ struct WorkItem {
 };
 
 __device__ inline WorkItem work_item(int w, const int32_t* __restrict__ chunk_start,
-                                     const int32_t* __restrict__ cell_ptr, const TileSpec& ts) {
+                                     const int32_t* __restrict__ cell_ptr, const TileSpec& ts,
+                                     int chunk_points) {
     int lo = 0, hi = int(ts.ntiles);
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -151,8 +160,8 @@ __device__ inline WorkItem work_item(int w, const int32_t* __restrict__ chunk_st
     it.t0[1] = ((tile / nt2) % nt1) * ts.t[1];
     it.t0[2] = (tile % nt2) * ts.t[2];
     const int tv = int(ts.tvol);
-    it.pbeg = cell_ptr[tile * tv] + chunk * SPREAD_CHUNK;
-    it.pend = min(cell_ptr[(tile + 1) * tv], it.pbeg + SPREAD_CHUNK);
+    it.pbeg = cell_ptr[tile * tv] + chunk * chunk_points;
+    it.pend = min(cell_ptr[(tile + 1) * tv], it.pbeg + chunk_points);
     return it;
 }
 
@@ -198,6 +207,88 @@ __device__ inline void frame_shift(const PointWeights& pw, const afsi_grid3& g, 
     }
 }
 
+// One spread work item: accumulate its points into the FP64 LDS box with
+// FP64 LDS atomics, then one global atomic per nonzero entry. perm holds
+// point ids offset by `base`. (FP32 LDS atomics are no alternative on
+// gfx950: ds_add_f32 sustains about 1/20 of ds_add_f64's lane rate.)
+template <int T>
+__device__ __forceinline__ void spread_item(const WorkItem& it, const double* __restrict__ x,
+                                            const double* __restrict__ force,
+                                            const int32_t* __restrict__ perm, int base,
+                                            const afsi_grid3& g, const TileSpec& ts, double volume,
+                                            double* const outs[3], int* flags, double* lds) {
+    const int nlds = T > 0 ? 3 * (T + 3) * (T + 4) * (T + 4) : ts.lds;
+    for (int e = threadIdx.x; e < nlds; e += SPREAD_THREADS) lds[e] = 0.0;
+    __syncthreads();
+    // Each thread takes a contiguous slice of the binned points. Lanes in a
+    // dense cell share a stencil base, so each lane walks its (j, k) links
+    // from a lane-dependent rotation: at any instant the lanes of one cell
+    // hit 16 different LDS addresses instead of one.
+    const int rj = (threadIdx.x >> 2) & 3, rk = threadIdx.x & 3;
+    const int n = it.pend - it.pbeg;
+    const int per = (n + SPREAD_THREADS - 1) / SPREAD_THREADS;
+    const int j0 = it.pbeg + int(threadIdx.x) * per;
+    const int j1 = min(it.pend, j0 + per);
+    for (int j = j0; j < j1; ++j) {
+        const int idx = perm[j] - base;
+        const double xp[3] = {x[3 * idx], x[3 * idx + 1], x[3 * idx + 2]};
+        PointWeights pw;
+        point_weights(xp, g, pw);
+        int shift[3];
+        frame_shift(pw, g, shift);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double fv = force[3 * idx + c] / volume;
+            int lb[3];
+            if (!box_base<T>(pw, shift, c, it.t0, ts, lb)) {
+                raise_flag(flags, F_INDEX);
+                continue;
+            }
+            const double* wx = (c == 0) ? pw.wf[0] : pw.wc[0];
+            const double* wy = (c == 1) ? pw.wf[1] : pw.wc[1];
+            const double* wz = (c == 2) ? pw.wf[2] : pw.wc[2];
+            const int d1 = box_dim<T>(ts, c, 1), d2 = box_dim<T>(ts, c, 2);
+            const int b0 = box_off<T>(ts, c) + (lb[0] * d1 + lb[1]) * d2 + lb[2];
+            double wyr[4], wzr[4];
+            int oy[4], oz[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int jq = (q + rj) & 3, kq = (q + rk) & 3;
+                wyr[q] = jq == 0 ? wy[0] : (jq == 1 ? wy[1] : (jq == 2 ? wy[2] : wy[3]));
+                wzr[q] = kq == 0 ? wz[0] : (kq == 1 ? wz[1] : (kq == 2 ? wz[2] : wz[3]));
+                oy[q] = jq * d2;
+                oz[q] = kq;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    const double wxy = wx[i] * wyr[jj];
+                    const int row = b0 + i * d1 * d2 + oy[jj];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        atomicAdd(lds + row + oz[k], (wxy * wzr[k]) * fv);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // Flush: one global atomic per nonzero LDS entry.
+    for (int e = threadIdx.x; e < nlds; e += SPREAD_THREADS) {
+        const double v = lds[e];
+        if (v == 0.0) continue;
+        const int c = e < box_off<T>(ts, 1) ? 0 : (e < box_off<T>(ts, 2) ? 1 : 2);
+        const int f = box_face<T>(e - box_off<T>(ts, c), c, it.t0, ts, g);
+        if (f < 0) {
+            raise_flag(flags, F_INDEX);
+            continue;
+        }
+        unsafeAtomicAdd(outs[c] + f, v);
+    }
+    __syncthreads();
+}
+
 template <int T>
 __global__ void __launch_bounds__(SPREAD_THREADS) spread_tiled_kernel(
     const double* __restrict__ x, const double* __restrict__ force,
@@ -206,65 +297,67 @@ __global__ void __launch_bounds__(SPREAD_THREADS) spread_tiled_kernel(
     double* __restrict__ out0, double* __restrict__ out1, double* __restrict__ out2, int* flags) {
     extern __shared__ double lds[];
     const int total = chunk_start[ts.ntiles];
-    double* outs[3] = {out0, out1, out2};
-    const int nlds = T > 0 ? 3 * (T + 3) * (T + 4) * (T + 4) : ts.lds;
-    for (int w = blockIdx.x; w < total; w += gridDim.x) {
-        const WorkItem it = work_item(w, chunk_start, cell_ptr, ts);
-        for (int e = threadIdx.x; e < nlds; e += SPREAD_THREADS) lds[e] = 0.0;
-        __syncthreads();
-        // Each thread takes a contiguous slice so lanes of one wave sit in
-        // different cells and LDS atomics rarely collide.
-        const int n = it.pend - it.pbeg;
-        const int per = (n + SPREAD_THREADS - 1) / SPREAD_THREADS;
-        const int j0 = it.pbeg + int(threadIdx.x) * per;
-        const int j1 = min(it.pend, j0 + per);
-        for (int j = j0; j < j1; ++j) {
-            const int idx = perm[j];
-            const double xp[3] = {x[3 * idx], x[3 * idx + 1], x[3 * idx + 2]};
-            PointWeights pw;
-            point_weights(xp, g, pw);
-            int shift[3];
-            frame_shift(pw, g, shift);
+    double* const outs[3] = {out0, out1, out2};
+    for (int w = blockIdx.x; w < total; w += gridDim.x)
+        spread_item<T>(work_item(w, chunk_start, cell_ptr, ts, SPREAD_CHUNK), x, force, perm, 0, g, ts, volume,
+                       outs, flags, lds);
+}
+
+// Interpolation over the same work list: the tile's face box of all three
+// components is staged in LDS once per work item, then each point gathers
+// its 3 x 64 values from LDS (consecutive threads take consecutive binned
+// points; results go to the caller's point order).
+// One interpolation work item: stage the box, gather the points' values.
+template <int T>
+__device__ __forceinline__ void interp_item(const WorkItem& it, const double* __restrict__ x,
+                                            const int32_t* __restrict__ perm, int base,
+                                            const afsi_grid3& g, const TileSpec& ts,
+                                            const double* const us[3], double* __restrict__ out,
+                                            int* flags, double* lds) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double fv = force[3 * idx + c] / volume;
-                int lb[3];
-                if (!box_base<T>(pw, shift, c, it.t0, ts, lb)) {
-                    raise_flag(flags, F_INDEX);
-                    continue;
-                }
+    for (int c = 0; c < 3; ++c) {
+        const int vol = box_dim<T>(ts, c, 0) * box_dim<T>(ts, c, 1) * box_dim<T>(ts, c, 2);
+        double* box = lds + box_off<T>(ts, c);
+        for (int e = threadIdx.x; e < vol; e += SPREAD_THREADS) {
+            const int f = box_face<T>(e, c, it.t0, ts, g);
+            box[e] = f >= 0 ? us[c][f] : 0.0;
+        }
+    }
+    __syncthreads();
+    for (int j = it.pbeg + int(threadIdx.x); j < it.pend; j += SPREAD_THREADS) {
+        const int idx = perm[j] - base;
+        const double xp[3] = {x[3 * idx], x[3 * idx + 1], x[3 * idx + 2]};
+        PointWeights pw;
+        point_weights(xp, g, pw);
+        int shift[3];
+        frame_shift(pw, g, shift);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            int lb[3];
+            double sum = 0.0;
+            if (!box_base<T>(pw, shift, c, it.t0, ts, lb)) {
+                raise_flag(flags, F_INDEX);
+            } else {
                 const double* wx = (c == 0) ? pw.wf[0] : pw.wc[0];
                 const double* wy = (c == 1) ? pw.wf[1] : pw.wc[1];
                 const double* wz = (c == 2) ? pw.wf[2] : pw.wc[2];
                 const int d1 = box_dim<T>(ts, c, 1), d2 = box_dim<T>(ts, c, 2);
-                double* box = lds + box_off<T>(ts, c);
+                const double* box = lds + box_off<T>(ts, c);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
 #pragma unroll
                     for (int jj = 0; jj < 4; ++jj) {
                         const double wxy = wx[i] * wy[jj];
-                        double* row = box + ((lb[0] + i) * d1 + (lb[1] + jj)) * d2 + lb[2];
+                        const double* row = box + ((lb[0] + i) * d1 + (lb[1] + jj)) * d2 + lb[2];
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) atomicAdd(row + k, (wxy * wz[k]) * fv);
+                        for (int k = 0; k < 4; ++k) sum += row[k] * (wxy * wz[k]);
                     }
                 }
             }
+            out[3 * idx + c] = sum;
         }
-        __syncthreads();
-        // Flush: one global atomic per nonzero LDS entry.
-        for (int e = threadIdx.x; e < nlds; e += SPREAD_THREADS) {
-            const double v = lds[e];
-            if (v == 0.0) continue;
-            const int c = e < box_off<T>(ts, 1) ? 0 : (e < box_off<T>(ts, 2) ? 1 : 2);
-            const int f = box_face<T>(e - box_off<T>(ts, c), c, it.t0, ts, g);
-            if (f < 0) {
-                raise_flag(flags, F_INDEX);
-                continue;
-            }
-            unsafeAtomicAdd(outs[c] + f, v);
-        }
-        __syncthreads();
     }
+    __syncthreads();
 }
 
 // Interpolation over the same work list: the tile's face box of all three
@@ -279,54 +372,9 @@ __global__ void __launch_bounds__(SPREAD_THREADS) interp_lds_kernel(
     const double* __restrict__ u2, double* __restrict__ out, int* flags) {
     extern __shared__ double lds[];
     const int total = chunk_start[ts.ntiles];
-    const double* us[3] = {u0, u1, u2};
-    for (int w = blockIdx.x; w < total; w += gridDim.x) {
-        const WorkItem it = work_item(w, chunk_start, cell_ptr, ts);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int vol = box_dim<T>(ts, c, 0) * box_dim<T>(ts, c, 1) * box_dim<T>(ts, c, 2);
-            double* box = lds + box_off<T>(ts, c);
-            for (int e = threadIdx.x; e < vol; e += SPREAD_THREADS) {
-                const int f = box_face<T>(e, c, it.t0, ts, g);
-                box[e] = f >= 0 ? us[c][f] : 0.0;
-            }
-        }
-        __syncthreads();
-        for (int j = it.pbeg + int(threadIdx.x); j < it.pend; j += SPREAD_THREADS) {
-            const int idx = perm[j];
-            const double xp[3] = {x[3 * idx], x[3 * idx + 1], x[3 * idx + 2]};
-            PointWeights pw;
-            point_weights(xp, g, pw);
-            int shift[3];
-            frame_shift(pw, g, shift);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                int lb[3];
-                double sum = 0.0;
-                if (!box_base<T>(pw, shift, c, it.t0, ts, lb)) {
-                    raise_flag(flags, F_INDEX);
-                } else {
-                    const double* wx = (c == 0) ? pw.wf[0] : pw.wc[0];
-                    const double* wy = (c == 1) ? pw.wf[1] : pw.wc[1];
-                    const double* wz = (c == 2) ? pw.wf[2] : pw.wc[2];
-                    const int d1 = box_dim<T>(ts, c, 1), d2 = box_dim<T>(ts, c, 2);
-                    const double* box = lds + box_off<T>(ts, c);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-#pragma unroll
-                        for (int jj = 0; jj < 4; ++jj) {
-                            const double wxy = wx[i] * wy[jj];
-                            const double* row = box + ((lb[0] + i) * d1 + (lb[1] + jj)) * d2 + lb[2];
-#pragma unroll
-                            for (int k = 0; k < 4; ++k) sum += row[k] * (wxy * wz[k]);
-                        }
-                    }
-                }
-                out[3 * idx + c] = sum;
-            }
-        }
-        __syncthreads();
-    }
+    const double* const us[3] = {u0, u1, u2};
+    for (int w = blockIdx.x; w < total; w += gridDim.x)
+        interp_item<T>(work_item(w, chunk_start, cell_ptr, ts, INTERP_CHUNK), x, perm, 0, g, ts, us, out, flags, lds);
 }
 
 // Unbinned interpolation: one thread per point, gathers from HBM/L2.
@@ -379,19 +427,148 @@ __global__ void __launch_bounds__(256) interp_points_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// batched transfers: many instances binned into one key space (instance i's
+// keys offset by key_offset_i) and one work list of (instance, tile, chunk)
+
+struct XferDesc {
+    const double* x;
+    const double* force;
+    double* face[3];
+    double* out;
+    afsi_grid3 g;
+    TileSpec ts;
+    double volume;
+    int point_begin;  // first global point id
+    int key_offset;   // first global key
+    int begin;        // first global tile
+};
+
+struct FillDesc {
+    double* p;
+    int n;
+    int begin;  // first workgroup
+};
+
+constexpr int FILL_CHUNK = 4096;
+
+template <class D>
+__device__ inline int find_begin(const D* __restrict__ desc, int n, int b) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (desc[mid].begin <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) fill_batch_kernel(const FillDesc* __restrict__ desc, int n) {
+    const int b = blockIdx.x;
+    const FillDesc& d = desc[find_begin(desc, n, b)];
+    const int lo = (b - d.begin) * FILL_CHUNK, hi = min(d.n, lo + FILL_CHUNK);
+    for (int j = lo + int(threadIdx.x); j < hi; j += 256) d.p[j] = 0.0;
+}
+
+__device__ inline int tile_points(const XferDesc& d, const int32_t* __restrict__ cell_ptr, int t,
+                                  int& beg) {
+    const int tv = int(d.ts.tvol);
+    beg = cell_ptr[d.key_offset + t * tv];
+    return cell_ptr[d.key_offset + (t + 1) * tv] - beg;
+}
+
+// Single workgroup: exclusive scan of the chunk counts of every tile of
+// every instance (global tile order).
+__global__ void chunk_scan_batch_kernel(const XferDesc* __restrict__ desc, int n, int tiles,
+                                        const int32_t* __restrict__ cell_ptr, int chunk,
+                                        int32_t* __restrict__ chunk_start) {
+    __shared__ int32_t s[1024];
+    int32_t running = 0;
+    for (int base = 0; base < tiles; base += 1024) {
+        const int t = base + int(threadIdx.x);
+        int32_t v = 0;
+        if (t < tiles) {
+            const XferDesc& d = desc[find_begin(desc, n, t)];
+            int beg;
+            v = (tile_points(d, cell_ptr, t - d.begin, beg) + chunk - 1) / chunk;
+        }
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int dd = 1; dd < 1024; dd <<= 1) {
+            const int32_t add = threadIdx.x >= dd ? s[threadIdx.x - dd] : 0;
+            __syncthreads();
+            s[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (t < tiles) chunk_start[t] = running + s[threadIdx.x] - v;
+        running += s[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) chunk_start[tiles] = running;
+}
+
+__device__ inline WorkItem batch_item(int w, const XferDesc* __restrict__ desc, int n, int tiles,
+                                      const int32_t* __restrict__ cell_ptr,
+                                      const int32_t* __restrict__ chunk_start, int chunk_points,
+                                      int& inst) {
+    int lo = 0, hi = tiles;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (chunk_start[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    inst = find_begin(desc, n, lo);
+    const XferDesc& d = desc[inst];
+    const int t = lo - d.begin, chunk = w - chunk_start[lo];
+    const int nt1 = int(d.ts.nt[1]), nt2 = int(d.ts.nt[2]);
+    WorkItem it;
+    it.t0[0] = (t / (nt2 * nt1)) * d.ts.t[0];
+    it.t0[1] = ((t / nt2) % nt1) * d.ts.t[1];
+    it.t0[2] = (t % nt2) * d.ts.t[2];
+    int beg;
+    const int cnt = tile_points(d, cell_ptr, t, beg);
+    it.pbeg = beg + chunk * chunk_points;
+    it.pend = min(beg + cnt, it.pbeg + chunk_points);
+    return it;
+}
+
+template <int T>
+__global__ void __launch_bounds__(SPREAD_THREADS) spread_batch_kernel(
+    const XferDesc* __restrict__ desc, int n, int tiles, const int32_t* __restrict__ perm,
+    const int32_t* __restrict__ cell_ptr, const int32_t* __restrict__ chunk_start, int* flags) {
+    extern __shared__ double lds[];
+    const int total = chunk_start[tiles];
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        int i;
+        const WorkItem it = batch_item(w, desc, n, tiles, cell_ptr, chunk_start, SPREAD_CHUNK, i);
+        const XferDesc& d = desc[i];
+        spread_item<T>(it, d.x, d.force, perm, d.point_begin, d.g, d.ts, d.volume, d.face, flags, lds);
+    }
+}
+
+template <int T>
+__global__ void __launch_bounds__(SPREAD_THREADS) interp_batch_kernel(
+    const XferDesc* __restrict__ desc, int n, int tiles, const int32_t* __restrict__ perm,
+    const int32_t* __restrict__ cell_ptr, const int32_t* __restrict__ chunk_start, int* flags) {
+    extern __shared__ double lds[];
+    const int total = chunk_start[tiles];
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        int i;
+        const WorkItem it = batch_item(w, desc, n, tiles, cell_ptr, chunk_start, INTERP_CHUNK, i);
+        const XferDesc& d = desc[i];
+        const double* const us[3] = {d.face[0], d.face[1], d.face[2]};
+        interp_item<T>(it, d.x, perm, d.point_begin, d.g, d.ts, us, d.out, flags, lds);
+    }
+}
+
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 }  // namespace afsi
 
-using namespace afsi;
 
-extern "C" {
+// ---- batched transfer host side ----------------------------------------------
 
-size_t afsi_spread_workspace_bytes(const afsi_grid3* grid, const int32_t tile[3]) {
-    if (!grid || !tile || tile[0] <= 0 || tile[1] <= 0 || tile[2] <= 0) return 0;
-    const TileSpec ts = make_spec(*grid, tile);
-    return align256(size_t(ts.ntiles + 1) * sizeof(int32_t));
-}
+namespace afsi {
 
 static int cubic_edge(const int32_t tile[3]) {
     if (tile[0] == tile[1] && tile[1] == tile[2] && (tile[0] == 8 || tile[0] == 4)) return tile[0];
@@ -403,6 +580,75 @@ static int check_tiled(const afsi_grid3& g, const int32_t tile[3]) {
     AFSI_REQUIRE((g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1) < (int64_t(1) << 31),
                  "grid exceeds 2^31 faces per component");
     return OK;
+}
+
+struct XferLayout {
+    int n = 0;
+    int64_t points = 0, keys = 0;
+    int tiles = 0, fill_blocks = 0;
+    size_t bin_off = 0, xfer_off = 0, fill_off = 0, chunk_off = 0, sort_off = 0, total = 0;
+    size_t lds = 0;
+};
+
+static size_t alignb(size_t b) { return (b + 255) & ~size_t(255); }
+
+static int xfer_layout(const afsi_tb_batch* bt, XferLayout& L) {
+    AFSI_REQUIRE(bt && bt->items && bt->count >= 1, "invalid transfer batch");
+    AFSI_REQUIRE(bt->tile[0] > 0 && bt->tile[1] > 0 && bt->tile[2] > 0, "tile sizes must be positive");
+    L.n = bt->count;
+    TileSpec ts0{};
+    for (int i = 0; i < L.n; ++i) {
+        const afsi_tb_item& it = bt->items[i];
+        AFSI_REQUIRE(it.points >= 0 && it.x && it.force && it.out && it.face[0] && it.face[1] && it.face[2],
+                     "transfer batch item %d: null pointer", i);
+        if (int r = check_tiled(it.grid, bt->tile)) return r;
+        const TileSpec ts = make_spec(it.grid, bt->tile);
+        if (i == 0) ts0 = ts;
+        L.points += it.points;
+        L.keys += bin_key_space(it.grid, bt->tile);
+        L.tiles += int(ts.ntiles);
+        for (int c = 0; c < 3; ++c) {
+            const int64_t nf = (it.grid.n[0] + (!it.grid.periodic && c == 0)) *
+                               (it.grid.n[1] + (!it.grid.periodic && c == 1)) *
+                               (it.grid.n[2] + (!it.grid.periodic && c == 2));
+            L.fill_blocks += int(cdiv(nf, FILL_CHUNK));
+        }
+    }
+    AFSI_REQUIRE(L.points < (int64_t(1) << 31) && L.keys < (int64_t(1) << 31), "batch too large");
+    L.lds = size_t(ts0.lds) * sizeof(double);
+    AFSI_REQUIRE(L.lds <= 64 * 1024, "tile too large for LDS");
+    size_t off = 0;
+    L.bin_off = off; off = alignb(off + size_t(L.n) * sizeof(BinDesc));
+    L.xfer_off = off; off = alignb(off + size_t(L.n) * sizeof(XferDesc));
+    L.fill_off = off; off = alignb(off + size_t(3 * L.n) * sizeof(FillDesc));
+    L.chunk_off = off; off = alignb(off + size_t(L.tiles + 1) * sizeof(int32_t));
+    L.sort_off = off; off = alignb(off + bin_workspace_bytes(L.points > 0 ? L.points : 1));
+    L.total = off;
+    return OK;
+}
+
+template <class D>
+static D* wsat(void* ws, size_t off) {
+    return reinterpret_cast<D*>(static_cast<char*>(ws) + off);
+}
+
+static int xfer_ready(const afsi_tb_batch* bt, XferLayout& L) {
+    if (int r = xfer_layout(bt, L)) return r;
+    AFSI_REQUIRE(bt->workspace && bt->workspace_bytes >= L.total, "transfer batch workspace too small");
+    AFSI_REQUIRE(bt->keys_sorted && bt->perm && bt->cell_ptr && bt->flags, "missing batch buffers");
+    return OK;
+}
+
+}  // namespace afsi
+
+using namespace afsi;
+
+extern "C" {
+
+size_t afsi_spread_workspace_bytes(const afsi_grid3* grid, const int32_t tile[3]) {
+    if (!grid || !tile || tile[0] <= 0 || tile[1] <= 0 || tile[2] <= 0) return 0;
+    const TileSpec ts = make_spec(*grid, tile);
+    return align256(size_t(ts.ntiles + 1) * sizeof(int32_t));
 }
 
 int afsi_spread_tiled_f64(const double* x, const double* force, int64_t points,
@@ -428,25 +674,21 @@ int afsi_spread_tiled_f64(const double* x, const double* force, int64_t points,
     AFSI_REQUIRE(workspace_bytes >= afsi_spread_workspace_bytes(grid, tile),
                  "spread workspace too small");
     int32_t* chunk_start = static_cast<int32_t*>(workspace);
-    chunk_scan_kernel<<<1, 1024, 0, s>>>(cell_ptr, ts.ntiles, ts.tvol, chunk_start);
+    chunk_scan_kernel<<<1, 1024, 0, s>>>(cell_ptr, ts.ntiles, ts.tvol, SPREAD_CHUNK, chunk_start);
     AFSI_LAUNCH_CHECK();
     const double volume = (g.h[0] * g.h[1]) * g.h[2];
     const int64_t upper = ts.ntiles + points / SPREAD_CHUNK + 1;
     const unsigned blocks = unsigned(upper < 4096 ? upper : 4096);
     const size_t shmem = size_t(ts.lds) * sizeof(double);
+#define AFSI_SPREAD(E)                                                                          \
+    spread_tiled_kernel<E><<<blocks, SPREAD_THREADS, shmem, s>>>(x, force, perm, cell_ptr, chunk_start, \
+                                                                g, ts, volume, u, v, w, flags)
     switch (cubic_edge(tile)) {
-    case 8:
-        spread_tiled_kernel<8><<<blocks, SPREAD_THREADS, shmem, s>>>(x, force, perm, cell_ptr, chunk_start,
-                                                                    g, ts, volume, u, v, w, flags);
-        break;
-    case 4:
-        spread_tiled_kernel<4><<<blocks, SPREAD_THREADS, shmem, s>>>(x, force, perm, cell_ptr, chunk_start,
-                                                                    g, ts, volume, u, v, w, flags);
-        break;
-    default:
-        spread_tiled_kernel<0><<<blocks, SPREAD_THREADS, shmem, s>>>(x, force, perm, cell_ptr, chunk_start,
-                                                                    g, ts, volume, u, v, w, flags);
+    case 8: AFSI_SPREAD(8); break;
+    case 4: AFSI_SPREAD(4); break;
+    default: AFSI_SPREAD(0);
     }
+#undef AFSI_SPREAD
     AFSI_LAUNCH_CHECK();
     return OK;
 }
@@ -474,9 +716,9 @@ int afsi_interp_tiled_f64(const double* x, int64_t points, const afsi_grid3* gri
     const TileSpec ts = make_spec(g, tile);
     AFSI_REQUIRE(size_t(ts.lds) * sizeof(double) <= 64 * 1024, "tile too large for LDS");
     int32_t* chunk_start = static_cast<int32_t*>(workspace);
-    chunk_scan_kernel<<<1, 1024, 0, s>>>(cell_ptr, ts.ntiles, ts.tvol, chunk_start);
+    chunk_scan_kernel<<<1, 1024, 0, s>>>(cell_ptr, ts.ntiles, ts.tvol, INTERP_CHUNK, chunk_start);
     AFSI_LAUNCH_CHECK();
-    const int64_t upper = ts.ntiles + points / SPREAD_CHUNK + 1;
+    const int64_t upper = ts.ntiles + points / INTERP_CHUNK + 1;
     const unsigned blocks = unsigned(upper < 4096 ? upper : 4096);
     const size_t shmem = size_t(ts.lds) * sizeof(double);
     switch (cubic_edge(tile)) {
@@ -495,5 +737,112 @@ int afsi_interp_tiled_f64(const double* x, int64_t points, const afsi_grid3* gri
     AFSI_LAUNCH_CHECK();
     return OK;
 }
+
+
+int afsi_tb_sizes(const afsi_tb_batch* batch, int64_t* total_points, int64_t* key_space,
+                  size_t* workspace_bytes) {
+    XferLayout L;
+    if (int r = xfer_layout(batch, L)) return r;
+    if (total_points) *total_points = L.points;
+    if (key_space) *key_space = L.keys;
+    if (workspace_bytes) *workspace_bytes = L.total;
+    return OK;
+}
+
+int afsi_tb_prepare(const afsi_tb_batch* batch, void* stream) {
+    XferLayout L;
+    if (int r = xfer_ready(batch, L)) return r;
+    std::vector<char> host(L.sort_off, 0);
+    void* hw = host.data();
+    int64_t pbeg = 0, kbeg = 0;
+    int tbeg = 0, fbeg = 0;
+    for (int i = 0; i < L.n; ++i) {
+        const afsi_tb_item& it = batch->items[i];
+        const TileSpec ts = make_spec(it.grid, batch->tile);
+        BinDesc b{};
+        b.x = it.x;
+        b.flags = batch->flags;
+        b.g = it.grid;
+        b.tg = bin_tile_geom(it.grid, batch->tile);
+        b.points = it.points;
+        b.begin = pbeg;
+        b.key_offset = kbeg;
+        wsat<BinDesc>(hw, L.bin_off)[i] = b;
+        XferDesc x{};
+        x.x = it.x;
+        x.force = it.force;
+        for (int c = 0; c < 3; ++c) x.face[c] = it.face[c];
+        x.out = it.out;
+        x.g = it.grid;
+        x.ts = ts;
+        x.volume = (it.grid.h[0] * it.grid.h[1]) * it.grid.h[2];
+        x.point_begin = int(pbeg);
+        x.key_offset = int(kbeg);
+        x.begin = tbeg;
+        wsat<XferDesc>(hw, L.xfer_off)[i] = x;
+        for (int c = 0; c < 3; ++c) {
+            const int64_t nf = (it.grid.n[0] + (!it.grid.periodic && c == 0)) *
+                               (it.grid.n[1] + (!it.grid.periodic && c == 1)) *
+                               (it.grid.n[2] + (!it.grid.periodic && c == 2));
+            wsat<FillDesc>(hw, L.fill_off)[3 * i + c] = FillDesc{it.face[c], int(nf), fbeg};
+            fbeg += int(cdiv(nf, FILL_CHUNK));
+        }
+        pbeg += it.points;
+        kbeg += bin_key_space(it.grid, batch->tile);
+        tbeg += int(ts.ntiles);
+    }
+    const hipStream_t s = hipStream_t(stream);
+    AFSI_HIP(hipMemcpyAsync(batch->workspace, host.data(), host.size(), hipMemcpyHostToDevice, s));
+    AFSI_HIP(hipStreamSynchronize(s));
+    return OK;
+}
+
+int afsi_tb_bin(const afsi_tb_batch* batch, void* stream) {
+    XferLayout L;
+    if (int r = xfer_ready(batch, L)) return r;
+    return bin_batch(wsat<BinDesc>(batch->workspace, L.bin_off), L.n, L.points, L.keys, batch->keys_sorted,
+                     batch->perm, batch->cell_ptr, wsat<char>(batch->workspace, L.sort_off),
+                     hipStream_t(stream));
+}
+
+static int tb_transfer(const afsi_tb_batch* batch, bool spread, void* stream) {
+    XferLayout L;
+    if (int r = xfer_ready(batch, L)) return r;
+    const hipStream_t s = hipStream_t(stream);
+    const XferDesc* desc = wsat<XferDesc>(batch->workspace, L.xfer_off);
+    int32_t* chunk_start = wsat<int32_t>(batch->workspace, L.chunk_off);
+    if (spread) {
+        fill_batch_kernel<<<unsigned(L.fill_blocks), 256, 0, s>>>(wsat<FillDesc>(batch->workspace, L.fill_off),
+                                                                 3 * L.n);
+        AFSI_LAUNCH_CHECK();
+    }
+    if (L.points == 0) return OK;
+    const int chunk = spread ? SPREAD_CHUNK : INTERP_CHUNK;
+    chunk_scan_batch_kernel<<<1, 1024, 0, s>>>(desc, L.n, L.tiles, batch->cell_ptr, chunk, chunk_start);
+    AFSI_LAUNCH_CHECK();
+    const int64_t upper = L.tiles + L.points / chunk + 1;
+    const unsigned blocks = unsigned(upper < 4096 ? upper : 4096);
+    const int edge = cubic_edge(batch->tile);
+#define AFSI_TB_ARGS(...) __VA_ARGS__
+#define AFSI_TB_LAUNCH(KERNEL, E)                                                                      \
+    KERNEL<AFSI_TB_ARGS E><<<blocks, SPREAD_THREADS, L.lds, s>>>(desc, L.n, L.tiles, batch->perm,       \
+                                                                 batch->cell_ptr, chunk_start, batch->flags)
+    if (spread) {
+        if (edge == 8) AFSI_TB_LAUNCH(spread_batch_kernel, (8));
+        else if (edge == 4) AFSI_TB_LAUNCH(spread_batch_kernel, (4));
+        else AFSI_TB_LAUNCH(spread_batch_kernel, (0));
+    } else {
+        if (edge == 8) AFSI_TB_LAUNCH(interp_batch_kernel, (8));
+        else if (edge == 4) AFSI_TB_LAUNCH(interp_batch_kernel, (4));
+        else AFSI_TB_LAUNCH(interp_batch_kernel, (0));
+    }
+#undef AFSI_TB_LAUNCH
+#undef AFSI_TB_ARGS
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_tb_spread(const afsi_tb_batch* batch, void* stream) { return tb_transfer(batch, true, stream); }
+int afsi_tb_interp(const afsi_tb_batch* batch, void* stream) { return tb_transfer(batch, false, stream); }
 
 }  // extern "C"

@@ -9,7 +9,8 @@ interpolation -> x += dt U. Two precisions on identical inputs:
   mixed : FP32 V-cycles inside FP64 iterative refinement (FP64 residual,
           mean removal, velocities, point positions and transfers).
 Inputs are resident in HBM before the timed region; K steps per path are
-timed between device synchronisations. `value` is the mixed path's steps/s.
+timed between device synchronisations, with Python's cyclic garbage
+collector paused inside timed regions. `value` is the mixed path's steps/s.
 
 `batched` (config 4): 64 uneven instances stepped together (3D: one
 batched mixed pressure solve, one launch per multigrid phase) against the
@@ -24,6 +25,7 @@ cell in FP32), timed with HIP events on the stream it is launched on;
 `cpu_baseline` times the FP64 NumPy oracle on a bounded sample.
 """
 import argparse
+import gc
 import json
 import sys
 import time
@@ -100,6 +102,19 @@ def cpu_baseline(x, F, n, cycles):
                 vcycle_s=cycle_s)
 
 
+class paused_gc:
+    """Collect, then keep Python's cyclic collector paused inside the timed
+    region (a full collection in a torch-sized heap stalls the host for
+    tens of milliseconds and idles the GPU; no GPU work is skipped)."""
+
+    def __enter__(self):
+        gc.collect()
+        gc.disable()
+
+    def __exit__(self, *exc):
+        gc.enable()
+
+
 def run_path(torch, stepper, x0, F, steps, warmup):
     """Warm up, then time `steps` steps; returns timings and the final state."""
     x = x0.clone()
@@ -112,13 +127,14 @@ def run_path(torch, stepper, x0, F, steps, warmup):
         cycles.append(info['pressure']['cycles'])
     timers = [{ph: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                for ph in PHASES} for _ in range(steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(steps):
-        u, p, U, info = stepper.step(x, F, u, p, timers=timers[k])
-        cycles.append(info['pressure']['cycles'])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    with paused_gc():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            u, p, U, info = stepper.step(x, F, u, p, timers=timers[k])
+            cycles.append(info['pressure']['cycles'])
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
     phase_ms = {ph: float(np.mean([t[ph][0].elapsed_time(t[ph][1]) for t in timers]))
                 for ph in PHASES}
     return dict(steps_per_s=steps / elapsed, ms_per_step=1e3 * elapsed / steps,
@@ -168,7 +184,8 @@ def serving_sample(torch, seconds, rate=200.):
     warm_t, warm_i, warm_s = workloads.request_stream(workloads.GEOMETRY_SEED + 6, rate=rate,
                                                       duration=.2)
     Server(device='cuda:0').run(warm_t, warm_i, warm_s)  # compile/cache warm-up
-    stats = Server(device='cuda:0').run(times, inst, specs)
+    with paused_gc():
+        stats = Server(device='cuda:0').run(times, inst, specs)
     out = stats.summary()
     out.update(rate_per_s=rate, window_s=seconds,
                sample=f'{len(times)} requests over {seconds:g} s of the 60 s stream, '
@@ -208,13 +225,15 @@ def batched_sample(torch, instances, steps=3, warmup=1):
     def run(step_all):
         xs, fs = fresh()
         state = {}
-        for k in range(warmup + steps):
-            if k == warmup:
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
+        for k in range(warmup):
             step_all(xs, fs, state)
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / steps
+        with paused_gc():
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for k in range(steps):
+                step_all(xs, fs, state)
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / steps
 
     batch3 = BatchedIBStep(g3, [specs[i][1] for i in i3], dt=1e-3, device=dev, options=opts)
 

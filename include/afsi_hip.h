@@ -117,6 +117,42 @@ int afsi_interp_tiled_f64(const double* x, int64_t points, const afsi_grid3* gri
                           const double* u, const double* v, const double* w, double* out,
                           void* workspace, size_t workspace_bytes, int* flags, void* stream);
 
+/* ---- Batched transfers (build-own) ------------------------------------------
+ * Many 3D instances binned, spread and interpolated with one launch per
+ * phase: the instances' tile-major key spaces are concatenated (instance
+ * i's keys offset by the key spaces before it) and sorted together, and
+ * spread/interpolation walk one work list of (instance, tile, chunk of
+ * points) with the per-instance LDS boxes of afsi_spread_tiled_f64 /
+ * afsi_interp_tiled_f64. perm holds global point ids (instance i's points
+ * are numbered after those of instances 0..i-1). Faces are overwritten by
+ * the spread; out receives (P_i,3) velocities in each instance's order. */
+typedef struct afsi_tb_item {
+    const double* x;      /* (points,3) positions */
+    const double* force;  /* (points,3) forces */
+    double* face[3];      /* face arrays: spread output, interpolation input */
+    double* out;          /* (points,3) interpolated velocities */
+    int64_t points;
+    afsi_grid3 grid;
+} afsi_tb_item;
+typedef struct afsi_tb_batch {
+    int32_t count;
+    int32_t tile[3];
+    const afsi_tb_item* items;  /* host array */
+    uint32_t* keys_sorted;      /* device, total points */
+    int32_t* perm;              /* device, total points */
+    int32_t* cell_ptr;          /* device, total key space + 1 */
+    void* workspace;            /* device */
+    size_t workspace_bytes;
+    int* flags;                 /* device: 1 support, 2 index, 4 non-finite */
+} afsi_tb_batch;
+int afsi_tb_sizes(const afsi_tb_batch* batch, int64_t* total_points, int64_t* key_space,
+                  size_t* workspace_bytes);
+/* Builds the descriptors in the workspace (synchronises the stream). */
+int afsi_tb_prepare(const afsi_tb_batch* batch, void* stream);
+int afsi_tb_bin(const afsi_tb_batch* batch, void* stream);
+int afsi_tb_spread(const afsi_tb_batch* batch, void* stream);
+int afsi_tb_interp(const afsi_tb_batch* batch, void* stream);
+
 /* ---- 3D pressure multigrid kernels ---------------------------------------
  * Replace the MG kernel protocol of mac/mg_workspace.py:13-30 /
  * mac/_triton_mg.py:121-151 (smooth, residual, restrict, residual_restrict,

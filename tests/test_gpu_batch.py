@@ -110,3 +110,42 @@ def test_batched_ib_step_matches_per_instance_steps(gpu):
         assert infos[i]['cycles'] == info1['pressure']['cycles']
         assert rel(Us[i], U1) < 1e-9
         assert rel(xs[i], x1) < 1e-14
+
+
+@pytest.mark.parametrize('periodic', [True, False])
+def test_batched_transfer_matches_per_instance(gpu, periodic):
+    from afsi_gpu_amd.batch import BatchedTransfer
+    from afsi_gpu_amd.mac.transfer import TiledTransfer, bin_points
+    gen = np.random.default_rng(31)
+    specs = [((32, 32, 32), 15000), ((48, 40, 56), 40000), ((16, 16, 16), 3000), ((64, 64, 64), 80000)]
+    grids = [MACGrid(s, (1., 1., 1.), periodic=periodic) for s, _ in specs]
+    xs = [torch.tensor(workloads.shell(gen, (.3, .2, .2), c), device=gpu) for _, c in specs]
+    fs = [torch.randn((c, 3), device=gpu, dtype=torch.float64) for _, c in specs]
+    faces = [g.zeros(device=gpu) for g in grids]
+    outs = [torch.empty((c, 3), device=gpu, dtype=torch.float64) for _, c in specs]
+    bt = BatchedTransfer(grids, [c for _, c in specs], device=gpu).bind(xs, fs, faces, outs)
+    bt.bin()
+    bt.spread()
+    vel = [tuple(torch.randn(g.face_shape(c), device=gpu, dtype=torch.float64) for c in range(3))
+           for g in grids]
+    gt = BatchedTransfer(grids, [c for _, c in specs], device=gpu).bind(xs, fs, vel, outs)
+    gt.share_bins(bt)
+    gt.interpolate()
+    assert int(bt.flags.item()) == 0
+    cell = bt.cell_ptr.cpu().numpy()
+    perm = bt.perm.cpu().numpy()
+    pbeg, kbeg = 0, 0
+    for i, (g, x) in enumerate(zip(grids, xs)):
+        bins = bin_points(x, g)
+        ks = bins.cell_ptr.numel() - 1
+        seg = cell[kbeg:kbeg + ks + 1]
+        assert np.array_equal(seg - seg[0], bins.cell_ptr.cpu().numpy())
+        assert np.array_equal(perm[seg[0]:seg[-1]] - pbeg, bins.perm.cpu().numpy())
+        tr = TiledTransfer(g, gpu)
+        f = tr.spread(x, fs[i], bins)
+        for a, b in zip(faces[i], f):
+            assert rel(a, b) < 1e-12
+        U = tr.interpolate(x, vel[i], bins)
+        assert rel(outs[i], U) < 1e-12
+        pbeg += x.shape[0]
+        kbeg += ks

@@ -199,3 +199,26 @@ def test_projection_matches_oracle(gpu, periodic):
             bad = [torch.tensor(a, device=gpu) for a in u]
             bad[0][0] = 1.
             flow.project(tuple(bad))
+
+
+def test_autotuned_sweeps_keep_results(gpu):
+    """Tuned tile/grid choices change only speed: a tuned solve equals the
+    default one bit for bit."""
+    from afsi_gpu_amd import autotune
+    grid = MACGrid((64, 64, 64), (1., 1., 1.), periodic=True)
+    opts = MGOptions(smooth=2, smoother='rbgs')
+    gen = np.random.default_rng(3)
+    rhs = gen.normal(size=grid.shape)
+    rhs -= rhs.mean()
+    b = torch.tensor(rhs, device=gpu)
+    ref = GeometricMultigrid(grid, device=gpu, options=opts, backend='workspace')
+    p0, i0 = ref.solve(b)
+    tuned = GeometricMultigrid(grid, device=gpu, options=opts, backend='workspace')
+    chosen = autotune.tune_sweeps(tuned)
+    assert 0 in chosen and chosen[0][0] in (2, 4)
+    p1, i1 = tuned.solve(b)
+    assert i0['cycles'] == i1['cycles']
+    assert torch.equal(p0, p1)
+    cache = autotune.export_cache()
+    autotune.import_cache(cache)
+    assert autotune.export_cache() == cache

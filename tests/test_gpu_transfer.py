@@ -207,3 +207,13 @@ def test_sqrt_and_division_round_like_numpy(gpu):
     st = from_points(torch.tensor(x, device=gpu), g, check=False)
     gphi = st.phi.cpu().numpy()
     assert int((gphi != phi).sum()) == 0
+
+
+def test_transfer_tile_autotuner(gpu):
+    from afsi_gpu_amd import autotune
+    x, F = workloads.config2(50000)
+    g = MACGrid((64, 64, 64), (1., 1., 1.), periodic=True)
+    edge, times = autotune.tune_transfer_tile(g, torch.tensor(x, device=gpu), torch.tensor(F, device=gpu))
+    assert edge in (4, 8) and set(times) == {4, 8}
+    print('transfer tile times (ms):', times)
+
