@@ -196,18 +196,63 @@ __global__ void __launch_bounds__(SORT_THREADS) radix_scatter_kernel(
 }
 
 // cell_ptr[k] = lower_bound(sorted keys, k) for k in [0, key_space].
-__global__ void cell_ptr_kernel(const uint32_t* __restrict__ keys, int64_t n, int64_t space,
-                                int32_t* __restrict__ ptr) {
-    for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k <= space;
-         k += int64_t(gridDim.x) * blockDim.x) {
-        int64_t lo = 0, hi = n;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (int64_t(keys[mid]) < k) lo = mid + 1;
-            else hi = mid;
-        }
-        ptr[k] = int32_t(lo);
+// cell_ptr[c] = lower_bound(keys, c) for c in [0, space], in two passes.
+// Mark: one thread per block of 256 cells binary-searches the block's first
+// point, cell_ptr[256 b] = lower_bound(keys, 256 b) (one short round of
+// independent searches, bounded work per thread).
+// Fill: a workgroup takes one block [c0, c0 + 256), reads its point range
+// [cell_ptr[c0], cell_ptr[c0 + 256]) (n past the end), stages that range of
+// keys in LDS when it fits, and searches each cell there. A block reads its
+// neighbour's start entry while that block rewrites it with the same value.
+constexpr int CELL_PTR_STAGE = 2048;
+
+__device__ inline int64_t lower_bound_keys(const uint32_t* __restrict__ keys, int64_t lo, int64_t hi,
+                                           int64_t k) {
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (int64_t(keys[mid]) < k) lo = mid + 1;
+        else hi = mid;
     }
+    return lo;
+}
+
+__global__ void cell_mark_kernel(const uint32_t* __restrict__ keys, int64_t n, int64_t space,
+                                 int32_t* __restrict__ ptr) {
+    const int64_t nblk = (space >> 8) + 1;  // blocks whose first cell is <= space
+    for (int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; b < nblk;
+         b += int64_t(gridDim.x) * blockDim.x)
+        ptr[b << 8] = int32_t(lower_bound_keys(keys, 0, n, b << 8));
+}
+
+__global__ void __launch_bounds__(256) cell_ptr_kernel(const uint32_t* __restrict__ keys, int64_t n,
+                                                      int64_t space, int32_t* __restrict__ ptr) {
+    __shared__ uint32_t sk[CELL_PTR_STAGE];
+    for (int64_t c0 = int64_t(blockIdx.x) * 256; c0 <= space; c0 += int64_t(gridDim.x) * 256) {
+        const int64_t lo = ptr[c0], hi = c0 + 256 <= space ? int64_t(ptr[c0 + 256]) : n;
+        const int64_t m = hi - lo;
+        const int64_t c = c0 + threadIdx.x;
+        __syncthreads();
+        if (m <= CELL_PTR_STAGE) {
+            for (int j = threadIdx.x; j < m; j += 256) sk[j] = keys[lo + j];
+            __syncthreads();
+            if (c <= space) {
+                int a = 0, e = int(m);
+                while (a < e) {
+                    const int mid = (a + e) >> 1;
+                    if (int64_t(sk[mid]) < c) a = mid + 1;
+                    else e = mid;
+                }
+                ptr[c] = int32_t(lo + a);
+            }
+        } else if (c <= space) {
+            ptr[c] = int32_t(lower_bound_keys(keys, lo, hi, c));
+        }
+    }
+}
+
+static void launch_cell_ptr(const uint32_t* keys, int64_t n, int64_t space, int32_t* ptr, hipStream_t s) {
+    cell_mark_kernel<<<grid_for((space >> 8) + 1, 256, 8192), 256, 0, s>>>(keys, n, space, ptr);
+    cell_ptr_kernel<<<grid_for(space + 1, 256, 8192), 256, 0, s>>>(keys, n, space, ptr);
 }
 
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -247,8 +292,7 @@ static int sort_and_index(int64_t points, int64_t space, uint32_t* k0, uint32_t*
         kin = kout;
         vin = vout;
     }
-    cell_ptr_kernel<<<grid_for(space + 1, 256, 8192), 256, 0, s>>>(keys_sorted, points, space,
-                                                                     cell_ptr);
+    launch_cell_ptr(keys_sorted, points, space, cell_ptr, s);
     AFSI_LAUNCH_CHECK();
     return OK;
 }
@@ -288,7 +332,7 @@ int bin_batch(const BinDesc* desc_device, int count, int64_t total_points, int64
     AFSI_REQUIRE(total_points < (int64_t(1) << 31) && space < (int64_t(1) << 31),
                  "batch exceeds 2^31 points or keys");
     if (total_points == 0) {
-        cell_ptr_kernel<<<grid_for(space + 1, 256, 8192), 256, 0, s>>>(keys_sorted, 0, space, cell_ptr);
+        launch_cell_ptr(keys_sorted, 0, space, cell_ptr, s);
         AFSI_LAUNCH_CHECK();
         return OK;
     }
@@ -326,8 +370,7 @@ int afsi_bin_points(const double* x, int64_t points, const afsi_grid3* grid,
     AFSI_REQUIRE(cell_ptr, "null cell_ptr");
     const hipStream_t s = hipStream_t(stream);
     if (points == 0) {
-        cell_ptr_kernel<<<grid_for(space + 1, 256, 8192), 256, 0, s>>>(keys_sorted, 0, space,
-                                                                         cell_ptr);
+        launch_cell_ptr(keys_sorted, 0, space, cell_ptr, s);
         AFSI_LAUNCH_CHECK();
         return OK;
     }

@@ -109,31 +109,62 @@ __device__ inline void point_weights(const double xp[3], const afsi_grid3& g, Po
     }
 }
 
-__global__ void chunk_scan_kernel(const int32_t* __restrict__ cell_ptr, int64_t ntiles,
-                                  int64_t tvol, int chunk, int32_t* __restrict__ chunk_start) {
-    // Single workgroup: exclusive scan of ceil(points_in_tile / CHUNK).
-    __shared__ int32_t s[1024];
-    int32_t running = 0;
-    for (int64_t base = 0; base < ntiles; base += 1024) {
-        const int64_t t = base + threadIdx.x;
-        int32_t v = 0;
-        if (t < ntiles) {
-            const int32_t n = cell_ptr[(t + 1) * tvol] - cell_ptr[t * tvol];
-            v = (n + chunk - 1) / chunk;
-        }
-        s[threadIdx.x] = v;
-        __syncthreads();
-        for (int d = 1; d < 1024; d <<= 1) {
-            const int32_t add = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
-            __syncthreads();
-            s[threadIdx.x] += add;
-            __syncthreads();
-        }
-        if (t < ntiles) chunk_start[t] = running + s[threadIdx.x] - v;
-        running += s[1023];
-        __syncthreads();
+// Inclusive scan of one value per thread over a 1024-thread workgroup
+// (wave scans with DPP-free shuffles, then the 16 wave totals).
+__device__ inline int32_t block_scan_1024(int32_t v, int32_t* wave_tot) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int32_t up = __shfl_up(v, d, 64);
+        if (lane >= d) v += up;
     }
-    if (threadIdx.x == 0) chunk_start[ntiles] = running;
+    if (lane == 63) wave_tot[wave] = v;
+    __syncthreads();
+    int32_t before = 0;
+    for (int w = 0; w < wave; ++w) before += wave_tot[w];
+    __syncthreads();
+    return v + before;
+}
+
+// Single workgroup: exclusive scan of ceil(points_in_tile / chunk) over the
+// tiles; each thread takes a contiguous run of tiles (its boundary entries
+// of cell_ptr are loaded together), one block scan joins the runs.
+constexpr int SCAN_RUN = 16;
+
+__device__ inline int32_t chunks_of(int32_t points, int chunk) { return (points + chunk - 1) / chunk; }
+
+__global__ void __launch_bounds__(1024) chunk_scan_kernel(const int32_t* __restrict__ cell_ptr,
+                                                          int64_t ntiles, int64_t tvol, int chunk,
+                                                          int32_t* __restrict__ chunk_start) {
+    __shared__ int32_t wave_tot[16];
+    const int64_t per = (ntiles + 1023) / 1024;
+    const int64_t t0 = int64_t(threadIdx.x) * per;
+    const int64_t t1 = t0 + per < ntiles ? t0 + per : ntiles;
+    int32_t sum = 0;
+    if (per <= SCAN_RUN) {
+        int32_t b[SCAN_RUN + 1];
+#pragma unroll
+        for (int q = 0; q <= SCAN_RUN; ++q) b[q] = (t0 + q <= t1) ? cell_ptr[(t0 + q) * tvol] : 0;
+#pragma unroll
+        for (int q = 0; q < SCAN_RUN; ++q)
+            if (t0 + q < t1) sum += chunks_of(b[q + 1] - b[q], chunk);
+        int32_t run = block_scan_1024(sum, wave_tot) - sum;
+#pragma unroll
+        for (int q = 0; q < SCAN_RUN; ++q)
+            if (t0 + q < t1) {
+                chunk_start[t0 + q] = run;
+                run += chunks_of(b[q + 1] - b[q], chunk);
+            }
+        if (threadIdx.x == 1023) chunk_start[ntiles] = run;
+        return;
+    }
+    for (int64_t t = t0; t < t1; ++t) sum += chunks_of(cell_ptr[(t + 1) * tvol] - cell_ptr[t * tvol], chunk);
+    int32_t run = block_scan_1024(sum, wave_tot) - sum;
+    for (int64_t t = t0; t < t1; ++t) {
+        chunk_start[t] = run;
+        run += chunks_of(cell_ptr[(t + 1) * tvol] - cell_ptr[t * tvol], chunk);
+    }
+    if (threadIdx.x == 1023) chunk_start[ntiles] = run;
 }
 
 // Work item w -> tile (last t with chunk_start[t] <= w), its first corner
