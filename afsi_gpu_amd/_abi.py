@@ -79,6 +79,27 @@ class MGPlan2(Structure):
                 ('reduce_result', c_void_p), ('partials_capacity', c_int64)]
 
 
+
+class FEGeometry(Structure):
+    _fields_ = [('cells', c_void_p), ('values', c_void_p), ('weights', c_void_p), ('elements', c_int64),
+                ('nodes_per_cell', c_int32), ('quad', c_int32), ('node_count', c_int64),
+                ('node_ptr', c_void_p), ('node_items', c_void_p)]
+
+
+class CSR(Structure):
+    _fields_ = [('row_ptr', c_void_p), ('cols', c_void_p), ('vals', c_void_p), ('rows', c_int64),
+                ('nnz', c_int64)]
+
+
+class PCGOptions(Structure):
+    _fields_ = [('rtol', c_double), ('atol', c_double), ('max_iterations', c_int32),
+                ('recompute_every', c_int32), ('check_every', c_int32), ('reserved', c_int32)]
+
+
+class PCGInfo(Structure):
+    _fields_ = [('status', c_int32), ('iterations', c_int32), ('residual_norm', c_double),
+                ('rhs_norm', c_double), ('tolerance', c_double)]
+
 _P = c_void_p
 _I64, _F64 = POINTER(c_int64), POINTER(c_double)
 _SIGNATURES = {
@@ -166,6 +187,13 @@ _SIGNATURES = {
     'afsi_mg2d_cycles_f32': (c_int, [POINTER(MGPlan2), c_int32, _P]),
     'afsi_mg2d_residual_f64': (c_int, [POINTER(MGPlan2), _P]),
     'afsi_mg2d_residual_f32': (c_int, [POINTER(MGPlan2), _P]),
+    'afsi_fe_evaluate_f64': (c_int, [POINTER(FEGeometry), _P, _P, _P]),
+    'afsi_fe_weighted_f64': (c_int, [POINTER(FEGeometry), _P, _P, _P]),
+    'afsi_fe_assemble_f64': (c_int, [POINTER(FEGeometry), _P, _P, _P]),
+    'afsi_csr_spmv3_f64': (c_int, [POINTER(CSR), _P, _P, _P]),
+    'afsi_mass_pcg_workspace_bytes': (c_size_t, [c_int64]),
+    'afsi_mass_pcg_f64': (c_int, [POINTER(CSR), _P, _P, _P, _P, POINTER(PCGOptions), _P, c_size_t,
+                                  POINTER(PCGInfo), _P]),
 }
 
 

@@ -1,0 +1,408 @@
+// FE interaction-point transfer and consistent-mass Jacobi PCG (gfx950).
+//
+// The reference's quadrature IB equations (mac/transfer.py:1-5):
+//   M F = b,  f = V^-1 K^T W B F,  M U = B^T W K u,
+// with B the FE evaluation at interaction points (N_qa), W the physical
+// quadrature weights and K the Peskin-4 MAC stencil (the tiled transfer
+// kernels). This file holds B, W B, B^T W, the CSR mass action and the PCG of
+// fluid/solvers.py:39-122 (recurrence, check/restart schedule and
+// true-residual acceptance), driven by a host loop that reads the device
+// state back only at the schedule's check iterations.
+//
+// B^T W is a gather per node over a node-incidence CSR (deterministic order),
+// not an atomic scatter. PCG reductions are two-stage with a fixed number of
+// partial blocks, so every solve is bit-reproducible.
+#include "common.hpp"
+
+namespace afsi {
+namespace {
+
+constexpr int FE_MAX_NODES = 32;  // nodes per cell
+constexpr int FE_MAX_QUAD = 64;   // quadrature points per cell
+constexpr int RED_BLOCKS = 256;   // partial sums of every PCG reduction
+constexpr int RED_THREADS = 256;
+
+// B (scale = false) or W B (scale = true): one thread per interaction point.
+template <bool SCALE>
+__global__ void fe_evaluate_kernel(afsi_fe_geometry g, const double* __restrict__ nodal,
+                                   double* __restrict__ out) {
+    const int64_t total = g.elements * g.quad;
+    for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < total;
+         p += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t e = p / g.quad;
+        const int q = int(p - e * g.quad);
+        const int64_t* cell = g.cells + e * g.nodes_per_cell;
+        const double* N = g.values + int64_t(q) * g.nodes_per_cell;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int a = 0; a < g.nodes_per_cell; ++a) {
+            const double* v = nodal + 3 * cell[a];
+            s0 += N[a] * v[0];
+            s1 += N[a] * v[1];
+            s2 += N[a] * v[2];
+        }
+        if (SCALE) {
+            const double w = g.weights[p];
+            s0 *= w; s1 *= w; s2 *= w;
+        }
+        out[3 * p] = s0;
+        out[3 * p + 1] = s1;
+        out[3 * p + 2] = s2;
+    }
+}
+
+// B^T W: node n sums, over its incidences (e, a) in node_items order,
+// sum_q N_qa W_eq U_(e,q).
+__global__ void fe_assemble_kernel(afsi_fe_geometry g, const double* __restrict__ point_values,
+                                   double* __restrict__ out) {
+    for (int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; n < g.node_count;
+         n += int64_t(gridDim.x) * blockDim.x) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int32_t k = g.node_ptr[n]; k < g.node_ptr[n + 1]; ++k) {
+            const int64_t item = g.node_items[k];
+            const int64_t e = item / g.nodes_per_cell;
+            const int a = int(item - e * g.nodes_per_cell);
+            const double* W = g.weights + e * g.quad;
+            const double* U = point_values + 3 * e * g.quad;
+            double l0 = 0.0, l1 = 0.0, l2 = 0.0;
+            for (int q = 0; q < g.quad; ++q) {
+                const double c = g.values[int64_t(q) * g.nodes_per_cell + a] * W[q];
+                l0 += c * U[3 * q];
+                l1 += c * U[3 * q + 1];
+                l2 += c * U[3 * q + 2];
+            }
+            s0 += l0; s1 += l1; s2 += l2;
+        }
+        out[3 * n] = s0;
+        out[3 * n + 1] = s1;
+        out[3 * n + 2] = s2;
+    }
+}
+
+// y = A x, three columns, one thread per row.
+__global__ void csr_spmv3_kernel(afsi_csr a, const double* __restrict__ x, double* __restrict__ y) {
+    for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < a.rows;
+         r += int64_t(gridDim.x) * blockDim.x) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int32_t k = a.row_ptr[r]; k < a.row_ptr[r + 1]; ++k) {
+            const double v = a.vals[k];
+            const double* xc = x + 3 * int64_t(a.cols[k]);
+            s0 += v * xc[0];
+            s1 += v * xc[1];
+            s2 += v * xc[2];
+        }
+        y[3 * r] = s0;
+        y[3 * r + 1] = s1;
+        y[3 * r + 2] = s2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PCG. Device state (doubles): the scalars of the recurrence and its flags.
+enum : int { S_RZ = 0, S_CURV, S_ALPHA, S_RN, S_TOL, S_ACTIVE, S_BROKEN, S_WORKING, S_TRUE, S_COUNT };
+
+__device__ inline double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// Partial sums over blocks of sum_i a_i b_i (mode 0), sum_i a_i a_i / d_i
+// (mode 1: r . z with z = r / diag), sum_i a_i^2 (mode 2).
+__global__ void __launch_bounds__(RED_THREADS) pcg_dot_kernel(int mode, const double* __restrict__ a,
+                                                              const double* __restrict__ b,
+                                                              const double* __restrict__ diag, int64_t n,
+                                                              double* __restrict__ partials) {
+    __shared__ double sh[RED_THREADS];
+    double s = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * RED_THREADS + threadIdx.x; i < n; i += int64_t(RED_BLOCKS) * RED_THREADS)
+        s += mode == 0 ? a[i] * b[i] : (mode == 1 ? a[i] * (a[i] / diag[i]) : a[i] * a[i]);
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// x += alpha d; r -= alpha Ad; partial sums of r^2.
+__global__ void __launch_bounds__(RED_THREADS) pcg_update_kernel(double* __restrict__ x, double* __restrict__ r,
+                                                                 const double* __restrict__ d,
+                                                                 const double* __restrict__ Ad, int64_t n,
+                                                                 const double* __restrict__ st,
+                                                                 double* __restrict__ partials) {
+    __shared__ double sh[RED_THREADS];
+    const double alpha = st[S_ALPHA];
+    double s = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * RED_THREADS + threadIdx.x; i < n; i += int64_t(RED_BLOCKS) * RED_THREADS) {
+        x[i] = x[i] + alpha * d[i];
+        const double ri = r[i] - alpha * Ad[i];
+        r[i] = ri;
+        s += ri * ri;
+    }
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// r = rhs - Ax; partial sums of r^2.
+__global__ void __launch_bounds__(RED_THREADS) pcg_residual_kernel(double* __restrict__ r,
+                                                                   const double* __restrict__ rhs,
+                                                                   const double* __restrict__ Ax, int64_t n,
+                                                                   double* __restrict__ partials) {
+    __shared__ double sh[RED_THREADS];
+    double s = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * RED_THREADS + threadIdx.x; i < n; i += int64_t(RED_BLOCKS) * RED_THREADS) {
+        const double ri = rhs[i] - Ax[i];
+        r[i] = ri;
+        s += ri * ri;
+    }
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// The scalar steps of the recurrence, one block summing the partials in
+// order. what: 0 curvature -> valid/broken/working/alpha; 1 recurrence norm
+// -> broken/active; 2 r.z -> beta (kept in S_CURV) and rz (restart: active =
+// 1, the direction restarts); 3 true residual norm; 4 initial r.z
+// (restart).
+__global__ void __launch_bounds__(RED_THREADS) pcg_scalar_kernel(int what, int restart,
+                                                                 const double* __restrict__ partials,
+                                                                 double* __restrict__ st) {
+    __shared__ double sh[RED_THREADS];
+    const double t = block_sum(partials[threadIdx.x], sh);
+    if (threadIdx.x != 0) return;
+    if (what == 0) {
+        const double curvature = t, rz = st[S_RZ];
+        const bool active = st[S_ACTIVE] != 0.0;
+        const bool valid = isfinite(curvature) && curvature > 0.0 && isfinite(rz) && rz > 0.0;
+        const bool broken = st[S_BROKEN] != 0.0 || (active && !valid);
+        const bool working = active && valid && !broken;
+        st[S_CURV] = curvature;
+        st[S_ALPHA] = (working ? rz : 0.0) / (working ? curvature : 1.0);
+        st[S_BROKEN] = broken ? 1.0 : 0.0;
+        st[S_WORKING] = working ? 1.0 : 0.0;
+    } else if (what == 1) {
+        const double rn = sqrt(t);
+        const bool broken = st[S_BROKEN] != 0.0 || !isfinite(rn);
+        st[S_RN] = rn;
+        st[S_BROKEN] = broken ? 1.0 : 0.0;
+        st[S_ACTIVE] = (st[S_WORKING] != 0.0 && rn > st[S_TOL]) ? 1.0 : 0.0;
+    } else if (what == 2) {
+        const double next_rz = t;
+        if (restart) {
+            st[S_ACTIVE] = 1.0;
+            st[S_CURV] = 0.0;
+        } else {
+            st[S_CURV] = next_rz / (st[S_ACTIVE] != 0.0 ? st[S_RZ] : 1.0);  // beta
+        }
+        st[S_RZ] = next_rz;
+    } else if (what == 3) {
+        st[S_TRUE] = sqrt(t);
+    } else {
+        st[S_RZ] = t;
+        st[S_ACTIVE] = 1.0;
+        st[S_BROKEN] = 0.0;
+    }
+}
+
+// d = z (restart) or active ? z + beta d : 0, with z = r / diag.
+__global__ void pcg_direction_kernel(double* __restrict__ d, const double* __restrict__ r,
+                                     const double* __restrict__ diag, int64_t n, int restart,
+                                     const double* __restrict__ st) {
+    const double beta = st[S_CURV];
+    const bool active = st[S_ACTIVE] != 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const double z = r[i] / diag[i];
+        d[i] = restart ? z : (active ? z + beta * d[i] : 0.0);
+    }
+}
+
+struct PcgSpace {
+    double *r, *d, *Ad, *partials, *state;
+};
+
+size_t pcg_bytes(int64_t rows) {
+    const size_t v = (size_t(rows) * 3 * sizeof(double) + 255) & ~size_t(255);
+    return 3 * v + RED_BLOCKS * sizeof(double) + 256;
+}
+
+PcgSpace pcg_space(void* ws, int64_t rows) {
+    char* p = static_cast<char*>(ws);
+    const size_t v = (size_t(rows) * 3 * sizeof(double) + 255) & ~size_t(255);
+    PcgSpace s;
+    s.r = reinterpret_cast<double*>(p);
+    s.d = reinterpret_cast<double*>(p + v);
+    s.Ad = reinterpret_cast<double*>(p + 2 * v);
+    s.partials = reinterpret_cast<double*>(p + 3 * v);
+    s.state = reinterpret_cast<double*>(p + 3 * v + RED_BLOCKS * sizeof(double));
+    return s;
+}
+
+int check_geometry(const afsi_fe_geometry* g) {
+    AFSI_REQUIRE(g && g->cells && g->values && g->weights, "null FE geometry");
+    AFSI_REQUIRE(g->elements >= 0 && g->node_count > 0, "invalid FE geometry sizes");
+    AFSI_REQUIRE(g->nodes_per_cell >= 1 && g->nodes_per_cell <= FE_MAX_NODES, "nodes per cell must be in [1, %d]",
+                 FE_MAX_NODES);
+    AFSI_REQUIRE(g->quad >= 1 && g->quad <= FE_MAX_QUAD, "quadrature points per cell must be in [1, %d]", FE_MAX_QUAD);
+    AFSI_REQUIRE(g->elements * g->quad < (int64_t(1) << 40), "too many interaction points");
+    return OK;
+}
+
+}  // namespace
+}  // namespace afsi
+
+using namespace afsi;
+
+extern "C" {
+
+int afsi_fe_evaluate_f64(const afsi_fe_geometry* g, const double* nodal, double* out, void* stream) {
+    if (int r = check_geometry(g)) return r;
+    AFSI_REQUIRE(nodal && out, "null pointer");
+    if (g->elements == 0) return OK;
+    fe_evaluate_kernel<false><<<grid_for(g->elements * g->quad, 256, 8192), 256, 0, hipStream_t(stream)>>>(*g, nodal, out);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_fe_weighted_f64(const afsi_fe_geometry* g, const double* nodal, double* out, void* stream) {
+    if (int r = check_geometry(g)) return r;
+    AFSI_REQUIRE(nodal && out, "null pointer");
+    if (g->elements == 0) return OK;
+    fe_evaluate_kernel<true><<<grid_for(g->elements * g->quad, 256, 8192), 256, 0, hipStream_t(stream)>>>(*g, nodal, out);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_fe_assemble_f64(const afsi_fe_geometry* g, const double* point_values, double* out, void* stream) {
+    if (int r = check_geometry(g)) return r;
+    AFSI_REQUIRE(g->node_ptr && g->node_items && point_values && out, "null pointer");
+    fe_assemble_kernel<<<grid_for(g->node_count, 256, 8192), 256, 0, hipStream_t(stream)>>>(*g, point_values, out);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_csr_spmv3_f64(const afsi_csr* a, const double* x, double* y, void* stream) {
+    AFSI_REQUIRE(a && a->row_ptr && a->cols && a->vals && x && y && a->rows > 0, "invalid CSR product arguments");
+    csr_spmv3_kernel<<<grid_for(a->rows, 256, 8192), 256, 0, hipStream_t(stream)>>>(*a, x, y);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+size_t afsi_mass_pcg_workspace_bytes(int64_t rows) { return rows > 0 ? pcg_bytes(rows) : 0; }
+
+int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* rhs, const double* initial,
+                      double* x, const afsi_pcg_options* opt, void* workspace, size_t workspace_bytes,
+                      afsi_pcg_info* info, void* stream) {
+    AFSI_REQUIRE(a && a->row_ptr && a->cols && a->vals && a->rows > 0, "invalid mass matrix");
+    AFSI_REQUIRE(diagonal && rhs && x && opt && info && workspace, "null pointer");
+    AFSI_REQUIRE(workspace_bytes >= pcg_bytes(a->rows), "PCG workspace too small");
+    AFSI_REQUIRE(opt->max_iterations >= 1 && opt->recompute_every >= 1 && opt->check_every >= 1,
+                 "PCG iteration counts must be positive");
+    AFSI_REQUIRE(std::isfinite(opt->rtol) && std::isfinite(opt->atol) && opt->rtol >= 0 && opt->atol >= 0 &&
+                 opt->rtol + opt->atol > 0, "invalid PCG tolerances");
+    const hipStream_t s = hipStream_t(stream);
+    const int64_t n = a->rows * 3;
+    const PcgSpace w = pcg_space(workspace, a->rows);
+    const unsigned gv = grid_for(a->rows, 256, 8192), ge = grid_for(n, 256, 8192);
+    double host[S_COUNT];
+    auto read_state = [&]() -> int {
+        AFSI_HIP(hipMemcpyAsync(host, w.state, sizeof(host), hipMemcpyDeviceToHost, s));
+        AFSI_HIP(hipStreamSynchronize(s));
+        return OK;
+    };
+    auto scalar = [&](int what, int restart) -> int {
+        pcg_scalar_kernel<<<1, RED_THREADS, 0, s>>>(what, restart, w.partials, w.state);
+        AFSI_LAUNCH_CHECK();
+        return OK;
+    };
+    // true residual r = rhs - A x and its norm (state S_TRUE)
+    auto true_residual = [&]() -> int {
+        csr_spmv3_kernel<<<gv, 256, 0, s>>>(*a, x, w.Ad);
+        AFSI_LAUNCH_CHECK();
+        pcg_residual_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(w.r, rhs, w.Ad, n, w.partials);
+        AFSI_LAUNCH_CHECK();
+        return scalar(3, 0);
+    };
+    // rhs norm and tolerance
+    pcg_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(2, rhs, rhs, diagonal, n, w.partials);
+    AFSI_LAUNCH_CHECK();
+    if (int r = scalar(3, 0)) return r;
+    if (int r = read_state()) return r;
+    const double rhs_norm = host[S_TRUE];
+    const double tol = std::fmax(opt->atol, opt->rtol * rhs_norm);
+    info->rhs_norm = rhs_norm;
+    info->tolerance = tol;
+    info->iterations = 0;
+    if (initial) {
+        if (initial != x) AFSI_HIP(hipMemcpyAsync(x, initial, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+        AFSI_HIP(hipMemsetAsync(x, 0, size_t(n) * sizeof(double), s));
+    }
+    if (int r = true_residual()) return r;
+    if (int r = read_state()) return r;
+    double true_norm = host[S_TRUE];
+    if (true_norm <= tol) {
+        info->status = 0;
+        info->residual_norm = true_norm;
+        return OK;
+    }
+    {   // state: tolerance, then d = z = r / diag, rz = r . z, active, not broken
+        const double init[S_COUNT] = {0, 0, 0, 0, tol, 1, 0, 0, 0};
+        AFSI_HIP(hipMemcpyAsync(w.state, init, sizeof(init), hipMemcpyHostToDevice, s));
+        AFSI_HIP(hipStreamSynchronize(s));
+    }
+    pcg_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(1, w.r, w.r, diagonal, n, w.partials);
+    AFSI_LAUNCH_CHECK();
+    if (int r = scalar(4, 1)) return r;
+    pcg_direction_kernel<<<ge, 256, 0, s>>>(w.d, w.r, diagonal, n, 1, w.state);
+    AFSI_LAUNCH_CHECK();
+    for (int it = 1; it <= opt->max_iterations; ++it) {
+        csr_spmv3_kernel<<<gv, 256, 0, s>>>(*a, w.d, w.Ad);
+        AFSI_LAUNCH_CHECK();
+        pcg_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(0, w.d, w.Ad, diagonal, n, w.partials);
+        AFSI_LAUNCH_CHECK();
+        if (int r = scalar(0, 0)) return r;
+        pcg_update_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(x, w.r, w.d, w.Ad, n, w.state, w.partials);
+        AFSI_LAUNCH_CHECK();
+        if (int r = scalar(1, 0)) return r;
+        bool restart = it % opt->recompute_every == 0;
+        const bool check = restart || it % opt->check_every == 0 || it == opt->max_iterations;
+        bool converged = false;
+        if (check) {
+            if (int r = read_state()) return r;
+            if (host[S_BROKEN] != 0.0) {
+                info->status = 1;
+                info->iterations = it;
+                info->residual_norm = host[S_RN];
+                return OK;
+            }
+            converged = host[S_RN] <= tol;
+        }
+        if (restart || converged || it == opt->max_iterations) {
+            if (int r = true_residual()) return r;
+            if (int r = read_state()) return r;
+            true_norm = host[S_TRUE];
+            info->iterations = it;
+            info->residual_norm = true_norm;
+            if (!std::isfinite(true_norm)) {
+                info->status = 3;
+                return OK;
+            }
+            if (true_norm <= tol) {
+                info->status = 0;
+                return OK;
+            }
+            restart = true;
+        }
+        pcg_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(1, w.r, w.r, diagonal, n, w.partials);
+        AFSI_LAUNCH_CHECK();
+        if (int r = scalar(2, restart ? 1 : 0)) return r;
+        pcg_direction_kernel<<<ge, 256, 0, s>>>(w.d, w.r, diagonal, n, restart ? 1 : 0, w.state);
+        AFSI_LAUNCH_CHECK();
+    }
+    info->status = 2;
+    info->iterations = opt->max_iterations;
+    info->residual_norm = true_norm;
+    return OK;
+}
+
+}  // extern "C"

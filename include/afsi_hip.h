@@ -452,6 +452,68 @@ int afsi_mg2d_cycles_f32(const afsi_mg2d_plan* plan, int32_t cycles, void* strea
 int afsi_mg2d_residual_f64(const afsi_mg2d_plan* plan, void* stream);
 int afsi_mg2d_residual_f32(const afsi_mg2d_plan* plan, void* stream);
 
+/* ---- FE interaction points and the consistent mass ------------------------
+ * The quadrature IB equations of mac/transfer.py:1-5: M F = b,
+ * f = V^-1 K^T W B F, M U = B^T W K u. K is the tiled spread/interpolation
+ * above, applied at the interaction points B x. All arrays FP64 row-major,
+ * nodal and point fields (count, 3). */
+typedef struct afsi_fe_geometry {
+    const int64_t* cells;      /* (elements, nodes_per_cell) node ids */
+    const double* values;      /* (quad, nodes_per_cell): N_qa */
+    const double* weights;     /* (elements, quad): physical weights W_eq */
+    int64_t elements;
+    int32_t nodes_per_cell;    /* <= 32 */
+    int32_t quad;              /* <= 64 */
+    int64_t node_count;
+    const int32_t* node_ptr;   /* (node_count + 1): incidences of each node */
+    const int32_t* node_items; /* e * nodes_per_cell + a, ascending per node */
+} afsi_fe_geometry;
+/* B: out[e*quad + q] = sum_a N_qa nodal[cells[e,a]]; replaces
+ * FETransfer.evaluate / interaction_points (mac/transfer.py:57-58, 69-71). */
+int afsi_fe_evaluate_f64(const afsi_fe_geometry* g, const double* nodal, double* out, void* stream);
+/* W B: the same times W_eq; replaces FETransfer.weighted_force
+ * (mac/transfer.py:66-67). */
+int afsi_fe_weighted_f64(const afsi_fe_geometry* g, const double* nodal, double* out, void* stream);
+/* B^T W: out[n] = sum over the incidences (e, a) of n of
+ * sum_q N_qa W_eq U[e*quad + q] (deterministic order); replaces
+ * FETransfer.assemble_velocity (mac/transfer.py:126-129). */
+int afsi_fe_assemble_f64(const afsi_fe_geometry* g, const double* point_values, double* out, void* stream);
+
+typedef struct afsi_csr {
+    const int32_t* row_ptr;    /* rows + 1 */
+    const int32_t* cols;
+    const double* vals;
+    int64_t rows;
+    int64_t nnz;
+} afsi_csr;
+/* y = A x for three columns; replaces FETransfer.mass_action
+ * (torch.sparse.mm, mac/transfer.py:60-61) and MassSolver.action
+ * (mac/mass_solver.py:53-54). */
+int afsi_csr_spmv3_f64(const afsi_csr* a, const double* x, double* y, void* stream);
+
+typedef struct afsi_pcg_options {
+    double rtol, atol;
+    int32_t max_iterations, recompute_every, check_every, reserved;
+} afsi_pcg_options;
+typedef struct afsi_pcg_info {
+    int32_t status;            /* 0 converged, 1 breakdown, 2 not converged,
+                                  3 nonfinite true residual */
+    int32_t iterations;
+    double residual_norm;      /* true residual norm at return */
+    double rhs_norm;
+    double tolerance;          /* max(atol, rtol * ||rhs||) */
+} afsi_pcg_info;
+size_t afsi_mass_pcg_workspace_bytes(int64_t rows);
+/* Jacobi PCG on A x = rhs (three columns) with the recurrence, check and
+ * restart schedule and true-residual acceptance of pcg()
+ * (fluid/solvers.py:39-122) and MassSolver.solve (mac/mass_solver.py:56-103);
+ * replaces FETransfer.solve_mass (mac/transfer.py:63-64). initial may be
+ * null (zero start) or equal x. The device state is read back only at the
+ * schedule's check iterations. */
+int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* rhs, const double* initial,
+                      double* x, const afsi_pcg_options* opt, void* workspace, size_t workspace_bytes,
+                      afsi_pcg_info* info, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

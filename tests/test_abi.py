@@ -58,7 +58,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     mirrors = {'afsi_grid3': _abi.Grid3, 'afsi_grid2': _abi.Grid2, 'afsi_mg3d_plan': _abi.MGPlan3,
                'afsi_mg2d_plan': _abi.MGPlan2, 'afsi_mgb_item': _abi.MGBItem,
                'afsi_mgb_batch': _abi.MGBBatch, 'afsi_tb_item': _abi.TBItem,
-               'afsi_tb_batch': _abi.TBBatch}
+               'afsi_tb_batch': _abi.TBBatch, 'afsi_fe_geometry': _abi.FEGeometry, 'afsi_csr': _abi.CSR,
+               'afsi_pcg_options': _abi.PCGOptions, 'afsi_pcg_info': _abi.PCGInfo}
     lines = ['#include <cstdio>', '#include <cstddef>', '#include "afsi_hip.h"', 'int main() {']
     for cname, py in mirrors.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')

@@ -1,0 +1,86 @@
+"""The FE interaction-point oracle (oracle/fe.py) pinned by closed forms:
+exact P1 mass matrices, exact evaluation of affine fields, adjointness of
+the assembly, M = B^T W B, force conservation, and the PCG against dense
+solves and its reference error paths (fluid/solvers.py:39-122)."""
+import numpy as np
+import pytest
+
+from oracle import fe
+
+
+def sphere():
+    nodes, tris = fe.icosphere(2, .3, (.1, -.2, .05))
+    return fe.p1_surface(nodes, tris), nodes
+
+
+def solid():
+    nodes, tets = fe.box_tets(3, (-.4, -.3, -.5), (.5, .4, .3))
+    return fe.p1_solid(nodes, tets), nodes
+
+
+def closed_form_mass(g, nodes, dim):
+    """P1 mass: |T|/12 (1 + delta_ab) for triangles, |T|/20 (1 + delta_ab) for tetrahedra."""
+    M = np.zeros((g.node_count, g.node_count))
+    A = dim + 1
+    for cell in g.cells:
+        v = nodes[cell]
+        if dim == 2:
+            size = .5 * np.linalg.norm(np.cross(v[1] - v[0], v[2] - v[0]))
+            local = size / 12 * (np.ones((A, A)) + np.eye(A))
+        else:
+            size = abs(np.dot(v[1] - v[0], np.cross(v[2] - v[0], v[3] - v[0]))) / 6
+            local = size / 20 * (np.ones((A, A)) + np.eye(A))
+        for a in range(A):
+            for b in range(A):
+                M[cell[a], cell[b]] += local[a, b]
+    return M
+
+
+@pytest.mark.parametrize('which', ['sphere', 'solid'])
+def test_mass_matrix_closed_form(which):
+    g, nodes = sphere() if which == 'sphere' else solid()
+    M = fe.mass_matrix(g)
+    expected = closed_form_mass(g, nodes, 2 if which == 'sphere' else 3)
+    np.testing.assert_allclose(M, expected, rtol=1e-13, atol=1e-16)
+    np.testing.assert_allclose(fe.mass_diagonal(g)[:, 0], np.diag(expected), rtol=1e-13)
+    assert np.allclose(M, M.T) and np.linalg.eigvalsh(M).min() > 0
+
+
+@pytest.mark.parametrize('which', ['sphere', 'solid'])
+def test_evaluate_assemble_adjoint_and_mass(which):
+    g, nodes = sphere() if which == 'sphere' else solid()
+    rng = np.random.default_rng(3)
+    # affine fields are reproduced exactly at the physical quadrature points
+    Bm = rng.normal(size=(3, 3))
+    a = rng.normal(size=3)
+    pts = fe.evaluate(g, nodes)
+    np.testing.assert_allclose(fe.evaluate(g, nodes @ Bm.T + a), pts @ Bm.T + a, rtol=1e-13, atol=1e-13)
+    v = rng.normal(size=(g.node_count, 3))
+    U = rng.normal(size=(g.cells.shape[0] * g.values.shape[0], 3))
+    lhs = (fe.assemble(g, U) * v).sum()
+    rhs = (U * fe.weighted(g, v)).sum()
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    np.testing.assert_allclose(fe.assemble(g, fe.evaluate(g, v)), fe.mass_matrix(g) @ v, rtol=1e-11, atol=1e-14)
+    # total weighted force equals the total nodal force after the mass solve
+    b = rng.normal(size=(g.node_count, 3))
+    C = np.linalg.solve(fe.mass_matrix(g), b)
+    np.testing.assert_allclose(fe.weighted(g, C).sum(0), b.sum(0), rtol=1e-11, atol=1e-12)
+
+
+def test_pcg_matches_dense_solve_and_error_paths():
+    g, nodes = solid()
+    M = fe.mass_matrix(g)
+    diag = fe.mass_diagonal(g)
+    rhs = np.sin(5 * nodes)
+    opts = fe.SolverOptions(rtol=1e-11, atol=1e-13, recompute_every=12, check_every=4, max_iterations=500)
+    x, info = fe.pcg(lambda v: M @ v, rhs, diag, .1 * np.cos(nodes), opts)
+    exact = np.linalg.solve(M, rhs)
+    assert info.residual_norm <= info.tolerance and 0 < info.iterations < 500
+    assert np.linalg.norm(rhs - M @ x) <= info.tolerance
+    np.testing.assert_allclose(x, exact, rtol=1e-8, atol=1e-9 * np.abs(exact).max())
+    zero, zinfo = fe.pcg(lambda v: M @ v, np.zeros_like(rhs), diag, None, opts)
+    assert zinfo.iterations == 0 and not zero.any()
+    with pytest.raises(RuntimeError, match='breakdown'):
+        fe.pcg(lambda v: -(M @ v), rhs, diag, None, opts)
+    with pytest.raises(RuntimeError, match='converge'):
+        fe.pcg(lambda v: M @ v, rhs, diag, None, fe.SolverOptions(max_iterations=1, rtol=1e-14))
