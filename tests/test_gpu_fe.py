@@ -152,3 +152,32 @@ def test_fe_transfer_matches_oracle_and_conserves(gpu, which):
         tr.interpolate(tuple(invalid), st)
     with pytest.raises(ValueError, match='invalid FE nodal field'):
         tr.spread(ft[:-1], st)
+
+
+def test_fe_ib_step_matches_oracle(gpu):
+    """One coupled step (mac/coupling.py:74-123 ordering) on a closed box."""
+    from afsi_gpu_amd.mac import MGOptions
+    from afsi_gpu_amd.step import FEIBStep
+    from oracle import mac3d
+    from oracle.step import fe_ib_step
+    g, X = mesh('sphere')
+    grid, ogrid = grids()
+    dt = 1e-3
+    stepper = FEIBStep(grid, on_device(g, gpu), dt=dt, device=gpu, options=MGOptions(smooth=2, smoother='rbgs'))
+    x = X + .013 * np.sin(3 * X)
+    F = np.sin(3 * X) * 10
+    u = grid.zeros(device=gpu)
+    x_new, vel, p, U, info = stepper.step(torch.tensor(x, device=gpu), torch.tensor(F, device=gpu), u,
+                                          diagnostics=True)
+    mg = mac3d.Multigrid(ogrid.shape, ogrid.spacing, mac3d.MGOptions(smooth=2, smoother='rbgs'), periodic=False)
+    ref = fe_ib_step(g, x, F, tuple(np.zeros(ogrid.face_shape(c)) for c in range(3)), None, ogrid, mg, dt=dt)
+    err_U = rel(U.cpu(), ref['nodal_velocity'])
+    err_u = max(rel(vel[c].cpu(), ref['velocity'][c]) for c in range(3))
+    print('FE step: nodal velocity', err_U, 'face velocity', err_u, 'mass PCG iterations',
+          info['force_mass'].iterations, info['velocity_mass'].iterations, 'power error', info['power_error'])
+    assert err_U < 1e-9 and err_u < 1e-9
+    assert np.abs(x_new.cpu().numpy() - ref['x']).max() < 1e-14 + 1e-9 * np.abs(ref['x']).max()
+    assert info['power_error'] <= 1e-9 * max(abs(info['solid_power']), 1e-30)
+    with pytest.raises(ValueError, match='displacement'):
+        FEIBStep(grid, on_device(g, gpu), dt=50., device=gpu).step(
+            torch.tensor(x, device=gpu), torch.tensor(F, device=gpu), grid.zeros(device=gpu))
