@@ -98,7 +98,7 @@ __global__ void csr_spmv3_kernel(afsi_csr a, const double* __restrict__ x, doubl
 
 // ---------------------------------------------------------------------------
 // PCG. Device state (doubles): the scalars of the recurrence and its flags.
-enum : int { S_RZ = 0, S_CURV, S_ALPHA, S_RN, S_TOL, S_ACTIVE, S_BROKEN, S_WORKING, S_TRUE, S_COUNT };
+enum : int { S_RZ = 0, S_CURV, S_ALPHA, S_RN, S_TOL, S_ACTIVE, S_BROKEN, S_WORKING, S_TRUE, S_RHS, S_COUNT };
 
 __device__ inline double block_sum(double v, double* sh) {
     sh[threadIdx.x] = v;
@@ -124,23 +124,57 @@ __global__ void __launch_bounds__(RED_THREADS) pcg_dot_kernel(int mode, const do
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// x += alpha d; r -= alpha Ad; partial sums of r^2.
-__global__ void __launch_bounds__(RED_THREADS) pcg_update_kernel(double* __restrict__ x, double* __restrict__ r,
-                                                                 const double* __restrict__ d,
-                                                                 const double* __restrict__ Ad, int64_t n,
-                                                                 const double* __restrict__ st,
-                                                                 double* __restrict__ partials) {
+// Ad = A d (three columns) and partial sums of d . Ad, rows grid-strided
+// over the fixed reduction grid.
+__global__ void __launch_bounds__(RED_THREADS) pcg_spmv_dot_kernel(afsi_csr a, const double* __restrict__ d,
+                                                                   double* __restrict__ Ad,
+                                                                   double* __restrict__ partials) {
+    __shared__ double sh[RED_THREADS];
+    double s = 0.0;
+    for (int64_t r = int64_t(blockIdx.x) * RED_THREADS + threadIdx.x; r < a.rows;
+         r += int64_t(RED_BLOCKS) * RED_THREADS) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int32_t k = a.row_ptr[r]; k < a.row_ptr[r + 1]; ++k) {
+            const double v = a.vals[k];
+            const double* dc = d + 3 * int64_t(a.cols[k]);
+            s0 += v * dc[0];
+            s1 += v * dc[1];
+            s2 += v * dc[2];
+        }
+        Ad[3 * r] = s0;
+        Ad[3 * r + 1] = s1;
+        Ad[3 * r + 2] = s2;
+        s += (d[3 * r] * s0 + d[3 * r + 1] * s1) + d[3 * r + 2] * s2;
+    }
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// x += alpha d; r -= alpha Ad; partial sums of r^2 (partials[0, B)) and of
+// r . (r / diag) (partials[B, 2B)).
+__global__ void __launch_bounds__(RED_THREADS) pcg_update2_kernel(double* __restrict__ x, double* __restrict__ r,
+                                                                  const double* __restrict__ d,
+                                                                  const double* __restrict__ Ad,
+                                                                  const double* __restrict__ diag, int64_t n,
+                                                                  const double* __restrict__ st,
+                                                                  double* __restrict__ partials) {
     __shared__ double sh[RED_THREADS];
     const double alpha = st[S_ALPHA];
-    double s = 0.0;
+    double s = 0.0, q = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * RED_THREADS + threadIdx.x; i < n; i += int64_t(RED_BLOCKS) * RED_THREADS) {
         x[i] = x[i] + alpha * d[i];
         const double ri = r[i] - alpha * Ad[i];
         r[i] = ri;
         s += ri * ri;
+        q += ri * (ri / diag[i]);
     }
     const double t = block_sum(s, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+    __syncthreads();
+    const double u = block_sum(q, sh);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = t;
+        partials[RED_BLOCKS + blockIdx.x] = u;
+    }
 }
 
 // r = rhs - Ax; partial sums of r^2.
@@ -163,13 +197,38 @@ __global__ void __launch_bounds__(RED_THREADS) pcg_residual_kernel(double* __res
 // order. what: 0 curvature -> valid/broken/working/alpha; 1 recurrence norm
 // -> broken/active; 2 r.z -> beta (kept in S_CURV) and rz (restart: active =
 // 1, the direction restarts); 3 true residual norm; 4 initial r.z
-// (restart).
+// (restart); 5 = 1 then 2 without restart, from the two sums of
+// pcg_update2_kernel (a restart that follows recomputes r.z with 2); 6 rhs
+// norm -> tolerance max(atol, rtol ||rhs||).
 __global__ void __launch_bounds__(RED_THREADS) pcg_scalar_kernel(int what, int restart,
                                                                  const double* __restrict__ partials,
-                                                                 double* __restrict__ st) {
+                                                                 double* __restrict__ st, double atol = 0.0,
+                                                                 double rtol = 0.0) {
     __shared__ double sh[RED_THREADS];
     const double t = block_sum(partials[threadIdx.x], sh);
+    double t2 = 0.0;
+    if (what == 5) {
+        __syncthreads();
+        t2 = block_sum(partials[RED_BLOCKS + threadIdx.x], sh);
+    }
     if (threadIdx.x != 0) return;
+    if (what == 6) {
+        const double rhs_norm = sqrt(t);
+        st[S_RHS] = rhs_norm;
+        st[S_TOL] = fmax(atol, rtol * rhs_norm);
+        return;
+    }
+    if (what == 5) {
+        const double rn = sqrt(t);
+        const bool broken = st[S_BROKEN] != 0.0 || !isfinite(rn);
+        const bool active = st[S_WORKING] != 0.0 && rn > st[S_TOL];
+        st[S_RN] = rn;
+        st[S_BROKEN] = broken ? 1.0 : 0.0;
+        st[S_ACTIVE] = active ? 1.0 : 0.0;
+        st[S_CURV] = t2 / (active ? st[S_RZ] : 1.0);  // beta
+        st[S_RZ] = t2;
+        return;
+    }
     if (what == 0) {
         const double curvature = t, rz = st[S_RZ];
         const bool active = st[S_ACTIVE] != 0.0;
@@ -222,7 +281,7 @@ struct PcgSpace {
 
 size_t pcg_bytes(int64_t rows) {
     const size_t v = (size_t(rows) * 3 * sizeof(double) + 255) & ~size_t(255);
-    return 3 * v + RED_BLOCKS * sizeof(double) + 256;
+    return 3 * v + 2 * RED_BLOCKS * sizeof(double) + 256;
 }
 
 PcgSpace pcg_space(void* ws, int64_t rows) {
@@ -233,7 +292,7 @@ PcgSpace pcg_space(void* ws, int64_t rows) {
     s.d = reinterpret_cast<double*>(p + v);
     s.Ad = reinterpret_cast<double*>(p + 2 * v);
     s.partials = reinterpret_cast<double*>(p + 3 * v);
-    s.state = reinterpret_cast<double*>(p + 3 * v + RED_BLOCKS * sizeof(double));
+    s.state = reinterpret_cast<double*>(p + 3 * v + 2 * RED_BLOCKS * sizeof(double));
     return s;
 }
 
@@ -322,16 +381,12 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
         AFSI_LAUNCH_CHECK();
         return scalar(3, 0);
     };
-    // rhs norm and tolerance
+    // rhs norm and tolerance (device), initial iterate, true residual: one
+    // readback for both norms.
     pcg_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(2, rhs, rhs, diagonal, n, w.partials);
     AFSI_LAUNCH_CHECK();
-    if (int r = scalar(3, 0)) return r;
-    if (int r = read_state()) return r;
-    const double rhs_norm = host[S_TRUE];
-    const double tol = std::fmax(opt->atol, opt->rtol * rhs_norm);
-    info->rhs_norm = rhs_norm;
-    info->tolerance = tol;
-    info->iterations = 0;
+    pcg_scalar_kernel<<<1, RED_THREADS, 0, s>>>(6, 0, w.partials, w.state, opt->atol, opt->rtol);
+    AFSI_LAUNCH_CHECK();
     if (initial) {
         if (initial != x) AFSI_HIP(hipMemcpyAsync(x, initial, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
     } else {
@@ -339,31 +394,32 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
     }
     if (int r = true_residual()) return r;
     if (int r = read_state()) return r;
+    const double rhs_norm = host[S_RHS];
+    const double tol = host[S_TOL];
+    info->rhs_norm = rhs_norm;
+    info->tolerance = tol;
+    info->iterations = 0;
     double true_norm = host[S_TRUE];
     if (true_norm <= tol) {
         info->status = 0;
         info->residual_norm = true_norm;
         return OK;
     }
-    {   // state: tolerance, then d = z = r / diag, rz = r . z, active, not broken
-        const double init[S_COUNT] = {0, 0, 0, 0, tol, 1, 0, 0, 0};
-        AFSI_HIP(hipMemcpyAsync(w.state, init, sizeof(init), hipMemcpyHostToDevice, s));
-        AFSI_HIP(hipStreamSynchronize(s));
-    }
+    // d = z = r / diag, rz = r . z, active, not broken
     pcg_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(1, w.r, w.r, diagonal, n, w.partials);
     AFSI_LAUNCH_CHECK();
     if (int r = scalar(4, 1)) return r;
     pcg_direction_kernel<<<ge, 256, 0, s>>>(w.d, w.r, diagonal, n, 1, w.state);
     AFSI_LAUNCH_CHECK();
+    // Per iteration: A d with d . Ad, alpha, the update with |r|^2 and r . z,
+    // the norm/active/beta scalars, the direction (five launches).
     for (int it = 1; it <= opt->max_iterations; ++it) {
-        csr_spmv3_kernel<<<gv, 256, 0, s>>>(*a, w.d, w.Ad);
-        AFSI_LAUNCH_CHECK();
-        pcg_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(0, w.d, w.Ad, diagonal, n, w.partials);
+        pcg_spmv_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(*a, w.d, w.Ad, w.partials);
         AFSI_LAUNCH_CHECK();
         if (int r = scalar(0, 0)) return r;
-        pcg_update_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(x, w.r, w.d, w.Ad, n, w.state, w.partials);
+        pcg_update2_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(x, w.r, w.d, w.Ad, diagonal, n, w.state, w.partials);
         AFSI_LAUNCH_CHECK();
-        if (int r = scalar(1, 0)) return r;
+        if (int r = scalar(5, 0)) return r;
         bool restart = it % opt->recompute_every == 0;
         const bool check = restart || it % opt->check_every == 0 || it == opt->max_iterations;
         bool converged = false;
@@ -393,9 +449,11 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
             }
             restart = true;
         }
-        pcg_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(1, w.r, w.r, diagonal, n, w.partials);
-        AFSI_LAUNCH_CHECK();
-        if (int r = scalar(2, restart ? 1 : 0)) return r;
+        if (restart) {  // r.z of the (true) residual, active = 1
+            pcg_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(1, w.r, w.r, diagonal, n, w.partials);
+            AFSI_LAUNCH_CHECK();
+            if (int r = scalar(2, 1)) return r;
+        }
         pcg_direction_kernel<<<ge, 256, 0, s>>>(w.d, w.r, diagonal, n, restart ? 1 : 0, w.state);
         AFSI_LAUNCH_CHECK();
     }

@@ -211,3 +211,33 @@ def request_stream(seed, rate=200., duration=60., per_instance=10, live=20,
         shape, count = instance_spec(gen, kind)
         specs[int(i)] = (shape, count, int(gen.integers(2 ** 31)))
     return times, inst, specs
+
+
+def fe_sphere(level, radius=.3, centre=(.5, .5, .5)):
+    """P1 triangulated sphere (subdivided icosahedron, vectorised midpoint
+    subdivision) with the degree-2 three-point triangle rule: returns
+    (nodes (V, 3), cells (E, 3) int64, values N (3, 3), weights W (E, 3))."""
+    t = (1 + 5 ** .5) / 2
+    v = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+                  (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], dtype=np.float64)
+    f = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+                  (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9),
+                  (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)], dtype=np.int64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    for _ in range(level):
+        edges = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        key = np.sort(edges, axis=1)
+        uniq, inverse = np.unique(key, axis=0, return_inverse=True)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        ids = len(v) + inverse.reshape(3, -1)
+        ab, bc, ca = ids[0], ids[1], ids[2]
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        f = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1), np.stack([c, ca, bc], 1),
+                            np.stack([ab, bc, ca], 1)])
+        v = np.concatenate([v, mid])
+    nodes = v * radius + np.asarray(centre)
+    values = np.array([[2 / 3, 1 / 6, 1 / 6], [1 / 6, 2 / 3, 1 / 6], [1 / 6, 1 / 6, 2 / 3]])
+    p = nodes[f]
+    area = .5 * np.linalg.norm(np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), axis=1)
+    return nodes, f, values, np.repeat(area[:, None] / 3, 3, axis=1)

@@ -69,6 +69,8 @@ def parse():
                     help='arrival window of the served-latency sample (0: skip)')
     ap.add_argument('--batch-instances', type=int, default=64,
                     help='instances of the batched sample (0: skip)')
+    ap.add_argument('--fe-level', type=int, default=7,
+                    help='sphere subdivision level of the FE interaction-point sample (0: skip)')
     return ap.parse_args()
 
 
@@ -191,6 +193,53 @@ def serving_sample(torch, seconds, rate=200.):
                sample=f'{len(times)} requests over {seconds:g} s of the 60 s stream, '
                       f'{len(specs)} instances')
     return out
+
+
+def fe_sample(torch, n=128, level=7, steps=10, warmup=2):
+    """FE interaction-point steps (FEIBStep, mixed-precision pressure): a P1
+    sphere of radius 0.3 (icosahedron subdivided `level` times, three
+    quadrature points per triangle) in an n^3 periodic unit cube, random
+    nodal forces of unit surface density; consistent-mass Jacobi PCG on both transfers (rtol 1e-12,
+    warm started)."""
+    from afsi_gpu_amd import workloads
+    from afsi_gpu_amd.mac import MACGrid, MGOptions
+    from afsi_gpu_amd.mac.fe import FEGeometry
+    from afsi_gpu_amd.step import FEIBStep
+    device = torch.device('cuda:0')
+    nodes, cells, values, weights = workloads.fe_sphere(level)
+    t = lambda a, dt=torch.float64: torch.tensor(a, device=device, dtype=dt)
+    geometry = FEGeometry(t(cells, torch.int64), t(values), t(weights), len(nodes))
+    grid = MACGrid((n,) * 3, (1., 1., 1.), periodic=True)
+    stepper = FEIBStep(grid, geometry, dt=1e-3, device=device, options=MGOptions(smooth=2, smoother='rbgs'),
+                       precision='mixed', warm_start=True)
+    x = t(nodes)
+    # O(1) surface force density: nodal forces scale with the nodal area
+    F = t(np.random.default_rng(7).normal(size=nodes.shape) * (weights.sum() / len(nodes)))
+    u = grid.zeros(device=device)
+    p = None
+    for _ in range(warmup):
+        x, u, p, U, info = stepper.step(x, F, u, p)
+    timers = [{ph: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for ph in PHASES} for _ in range(steps)]
+    iters = []
+    with paused_gc():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            x, u, p, U, info = stepper.step(x, F, u, p, timers=timers[k])
+            iters.append((info['force_mass'].iterations, info['velocity_mass'].iterations))
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+    phases = {ph: float(np.mean([tm[ph][0].elapsed_time(tm[ph][1]) for tm in timers])) for ph in PHASES}
+    points = cells.shape[0] * values.shape[0]
+    return dict(steps_per_s=steps / elapsed, ms_per_step=1e3 * elapsed / steps, phases_ms=phases,
+                nodes=len(nodes), elements=int(cells.shape[0]), interaction_points=int(points),
+                mass_pcg_iterations_mean=[float(np.mean([a for a, _ in iters])), float(np.mean([b for _, b in iters]))],
+                spread_with_mass_solve_mpts=points / phases['spread'] / 1e3,
+                interp_with_mass_solve_mpts=points / phases['interp'] / 1e3,
+                workload=f'P1 sphere r=0.3, level {level} ({len(nodes)} nodes, {points} interaction points) '
+                         f'in {n}^3 periodic; FEIBStep (prepare, mass-PCG spread, mixed projection, '
+                         f'interpolate + mass PCG), bin phase = B x + binning')
 
 
 def batched_sample(torch, instances, steps=3, warmup=1):
@@ -360,6 +409,8 @@ def main():
         result['batched'] = batched_sample(torch, args.batch_instances)
     if args.serve_seconds > 0:
         result['serving'] = serving_sample(torch, args.serve_seconds)
+    if args.fe_level > 0:
+        result['fe'] = fe_sample(torch, n, args.fe_level)
     if not args.no_cpu_baseline:
         result['cpu_baseline'] = cpu_baseline(x_np, F_np, n, int(round(plain['mg_cycles_mean'])))
     print(json.dumps(result))
