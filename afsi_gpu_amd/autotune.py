@@ -1,6 +1,7 @@
 """Autotuning of launch shapes (build-own).
 
-* Red-black sweeps: per multigrid level, the tile edge (2 or 4) and the
+* Red-black sweeps: per multigrid level, the tile (2x2, 4x4, 8x4 or 8x8
+  rows) and the
   persistent grid size (one tile per workgroup, or fewer workgroups that
   pipeline several tiles each). Every variant computes the same values
   (the per-cell arithmetic does not depend on the tiling), so tuning only
@@ -19,6 +20,8 @@ from . import _abi
 
 _SWEEPS = {}
 _TILES = {}
+# sweep_tile code -> (x, y) rows of a tile (afsi_mg3d_plan.sweep_tile)
+_SWEEP_TILES = {8: (8, 8), 84: (8, 4), 4: (4, 4), 2: (2, 2)}
 
 
 def _time(fn, repeats):
@@ -51,10 +54,10 @@ def tune_sweeps(solver, repeats=20):
             tile, blocks, us = _SWEEPS[key]
         else:
             best = None
-            for tile in (4, 2):
-                if n0 % tile or n1 % tile:
+            for tile, (tx, ty) in _SWEEP_TILES.items():
+                if n0 % tx or n1 % ty:
                     continue
-                ntiles = (n0 // tile) * (n1 // tile)
+                ntiles = (n0 // tx) * (n1 // ty)
                 for blocks in sorted({ntiles, max(8, ntiles // 2 // 8 * 8), max(8, ntiles // 4 // 8 * 8)},
                                      reverse=True):
                     plan.sweep_tile[level], plan.sweep_blocks[level] = tile, blocks

@@ -1428,7 +1428,10 @@ struct FastRunner {
         const W3<T> k = K(l);
         if (pl.smoother == 1 && rows_ok(l)) {
             int tile = pl.sweep_tile[l];
-            if (tile != 2 && tile != 4) tile = cdiv(d.n0, 4) * cdiv(d.n1, 4) >= 1024 ? 4 : 2;
+            if (tile != 2 && tile != 4 && tile != 8 && tile != 84)
+                tile = cdiv(d.n0, 4) * cdiv(d.n1, 4) >= 1024 ? 4 : 2;
+            if (tile == 8) return rb_rows<8, 8>(l, in, out, zero_in);
+            if (tile == 84) return rb_rows<8, 4>(l, in, out, zero_in);
             if (tile == 4) return rb_rows<4, 4>(l, in, out, zero_in);
             return rb_rows<2, 2>(l, in, out, zero_in);
         }
