@@ -224,3 +224,46 @@ def box_tets(n, lo, hi):
                 for p in ((1, 3), (1, 5), (2, 3), (2, 6), (4, 5), (4, 6)):
                     tets.append((c[0], c[p[0]], c[p[0] | p[1]], c[7]))
     return nodes, np.asarray(tets, dtype=np.int64)
+
+
+# P2 tetrahedra (tetrahedron.py:9-56 node order: 4 vertices, then the midpoints
+# of EDGES = (0,1),(0,2),(0,3),(1,2),(1,3),(2,3))
+EDGES = ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))
+
+
+def tet_rule(n):
+    """Collapsed (Duffy) Gauss-Legendre rule on the unit tetrahedron with n^3
+    positive weights; exact for total degree <= 2n - 3."""
+    u, wu = np.polynomial.legendre.leggauss(n)
+    u, wu = (u + 1) / 2, wu / 2
+    pts, wts = [], []
+    for a, wa in zip(u, wu):
+        for b, wb in zip(u, wu):
+            for c, wc in zip(u, wu):
+                pts.append((a, b * (1 - a), c * (1 - a) * (1 - b)))
+                wts.append(wa * wb * wc * (1 - a) ** 2 * (1 - b))
+    return np.asarray(pts), np.asarray(wts)
+
+
+def p2_basis(points):
+    """N[q, a] of the ten P2 functions at reference points (tetrahedron.py:19-30)."""
+    L = np.concatenate([1 - points.sum(1, keepdims=True), points], 1)
+    i, j = np.array(EDGES).T
+    return np.concatenate([L * (2 * L - 1), 4 * L[:, i] * L[:, j]], 1)
+
+
+def p2_promote(nodes, tets):
+    """One shared midpoint per unique edge (tetrahedron.py:45-56)."""
+    pairs = np.sort(tets[:, np.array(EDGES)].reshape(-1, 2), axis=1)
+    unique, inverse = np.unique(pairs, axis=0, return_inverse=True)
+    mid = nodes[unique].mean(1)
+    return np.concatenate([nodes, mid]), np.concatenate([tets, len(nodes) + inverse.reshape(-1, 6)], 1)
+
+
+def p2_solid(nodes, tets, n=4):
+    """P2 geometry of a P1 tet mesh: N (n^3, 10) from tet_rule(n), W = 6 |T| w."""
+    nodes2, cells = p2_promote(nodes, tets)
+    pts, w = tet_rule(n)
+    v = nodes[tets]
+    vol = np.abs(np.einsum('ei,ei->e', v[:, 1] - v[:, 0], np.cross(v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]))) / 6
+    return Geometry(cells.astype(np.int64), p2_basis(pts), 6 * vol[:, None] * w[None], len(nodes2)), nodes2

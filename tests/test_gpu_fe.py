@@ -22,6 +22,9 @@ def mesh(which):
     if which == 'sphere':
         nodes, tris = ofe.icosphere(2, .3, (.1, -.2, .05))
         return ofe.p1_surface(nodes, tris), nodes
+    if which == 'p2':   # the reference's production element: 10 nodes, 64 points per cell
+        nodes, tets = ofe.box_tets(2, (-.4, -.3, -.5), (.5, .4, .3))
+        return ofe.p2_solid(nodes, tets)
     nodes, tets = ofe.box_tets(3, (-.4, -.3, -.5), (.5, .4, .3))
     return ofe.p1_solid(nodes, tets), nodes
 
@@ -41,7 +44,7 @@ def rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-@pytest.mark.parametrize('which', ['sphere', 'solid'])
+@pytest.mark.parametrize('which', ['sphere', 'solid', 'p2'])
 def test_fe_operators_match_oracle(gpu, which):
     g, X = mesh(which)
     grid, _ = grids()
@@ -110,7 +113,7 @@ def oracle_interpolate(g, ogrid, x, velocity):
     return U
 
 
-@pytest.mark.parametrize('which', ['sphere', 'solid'])
+@pytest.mark.parametrize('which', ['sphere', 'solid', 'p2'])
 def test_fe_transfer_matches_oracle_and_conserves(gpu, which):
     g, X = mesh(which)
     grid, ogrid = grids()

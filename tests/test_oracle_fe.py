@@ -84,3 +84,46 @@ def test_pcg_matches_dense_solve_and_error_paths():
         fe.pcg(lambda v: -(M @ v), rhs, diag, None, opts)
     with pytest.raises(RuntimeError, match='converge'):
         fe.pcg(lambda v: M @ v, rhs, diag, None, fe.SolverOptions(max_iterations=1, rtol=1e-14))
+
+
+def test_p2_rule_and_mass():
+    """The 64-point rule integrates monomials of degree <= 5 exactly
+    (x^a y^b z^c over the unit tetrahedron = a! b! c! / (a+b+c+3)!), and the
+    P2 element mass from it equals an independent degree-11 integration and
+    the classical V/420 table; entries sum to the volume."""
+    from math import factorial
+    pts, w = fe.tet_rule(4)
+    for a in range(6):
+        for b in range(6 - a):
+            for c in range(6 - a - b):
+                exact = factorial(a) * factorial(b) * factorial(c) / factorial(a + b + c + 3)
+                got = (w * pts[:, 0] ** a * pts[:, 1] ** b * pts[:, 2] ** c).sum()
+                assert abs(got - exact) <= 1e-15, (a, b, c)
+    N4 = fe.p2_basis(pts)
+    p7, w7 = fe.tet_rule(7)
+    N7 = fe.p2_basis(p7)
+    M4 = np.einsum('q,qa,qb->ab', w, N4, N4)
+    M7 = np.einsum('q,qa,qb->ab', w7, N7, N7)
+    np.testing.assert_allclose(M4, M7, rtol=1e-13, atol=1e-16)
+    V = 1 / 6
+    table = np.zeros((10, 10))
+    edges = fe.EDGES
+    for i in range(4):
+        for j in range(4):
+            table[i, j] = 6 if i == j else 1
+        for e, (p, q) in enumerate(edges):
+            table[i, 4 + e] = table[4 + e, i] = -4 if i in (p, q) else -6
+    for e, (p, q) in enumerate(edges):
+        for f, (r, s) in enumerate(edges):
+            shared = len({p, q} & {r, s})
+            table[4 + e, 4 + f] = 32 if e == f else (16 if shared == 1 else 8)
+    np.testing.assert_allclose(M4, V / 420 * table, rtol=1e-13, atol=1e-17)
+    nodes, tets = fe.box_tets(2, (-.4, -.3, -.5), (.5, .4, .3))
+    g, nodes2 = fe.p2_solid(nodes, tets)
+    assert g.cells.shape[1] == 10 and g.values.shape == (64, 10)
+    assert abs(fe.mass_matrix(g).sum() - .9 * .7 * .8) < 1e-13
+    assert np.linalg.eigvalsh(fe.mass_matrix(g)).min() > 0
+    # P2 reproduces quadratic fields exactly at the physical quadrature points
+    q = lambda X: np.stack([X[:, 0] ** 2, X[:, 1] * X[:, 2], X[:, 0] - X[:, 2] ** 2], 1)
+    pts_phys = fe.evaluate(g, nodes2)
+    np.testing.assert_allclose(fe.evaluate(g, q(nodes2)), q(pts_phys), rtol=1e-12, atol=1e-14)
