@@ -187,6 +187,9 @@ _SIGNATURES = {
     'afsi_mg2d_cycles_f32': (c_int, [POINTER(MGPlan2), c_int32, _P]),
     'afsi_mg2d_residual_f64': (c_int, [POINTER(MGPlan2), _P]),
     'afsi_mg2d_residual_f32': (c_int, [POINTER(MGPlan2), _P]),
+    'afsi_mac_predict_f64': (c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, c_double, c_double, c_double, _P]),
+    'afsi_mac_speeds_workspace_bytes': (c_size_t, []),
+    'afsi_mac_speeds_f64': (c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, c_size_t, _P]),
     'afsi_fe_evaluate_f64': (c_int, [POINTER(FEGeometry), _P, _P, _P]),
     'afsi_fe_weighted_f64': (c_int, [POINTER(FEGeometry), _P, _P, _P]),
     'afsi_fe_assemble_f64': (c_int, [POINTER(FEGeometry), _P, _P, _P]),

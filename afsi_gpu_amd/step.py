@@ -26,7 +26,7 @@ class IBStep:
         self.options = options or MGOptions(smooth=2, smoother='rbgs')
         self.precision = precision
         self.flow = MACFlow(grid, dt=dt, rho=rho, device=device, options=self.options,
-                            pressure_backend=pressure_backend, precision=precision)
+                            pressure_backend=pressure_backend, precision=precision, predictor=False)
         self.binner = Binner(grid, self.count, self.device, tile)
         self.transfer = TiledTransfer(grid, self.device, tile)
         self.force_density = grid.zeros(device=self.device)
@@ -91,7 +91,7 @@ class FEIBStep:
         self.max_displacement = float(max_displacement)
         self.options = options or MGOptions(smooth=2, smoother='rbgs')
         self.flow = MACFlow(grid, dt=dt, rho=rho, device=device, options=self.options,
-                            pressure_backend=pressure_backend, precision=precision)
+                            pressure_backend=pressure_backend, precision=precision, predictor=False)
         self.transfer = FETransfer(grid, geometry, options=transfer_options, warm_start=warm_start, tile=tile)
         self.rhs = torch.empty(grid.shape, device=self.device, dtype=torch.float64)
         self._h = torch.tensor(grid.spacing, device=self.device, dtype=torch.float64)

@@ -452,6 +452,23 @@ int afsi_mg2d_cycles_f32(const afsi_mg2d_plan* plan, int32_t cycles, void* strea
 int afsi_mg2d_residual_f64(const afsi_mg2d_plan* plan, void* stream);
 int afsi_mg2d_residual_f32(const afsi_mg2d_plan* plan, void* stream);
 
+/* ---- explicit MAC predictor of the closed box ------------------------------
+ * u* = zero_normal(u + dt (-N(u) + (mu/rho) L u + f/rho)) with u =
+ * zero_normal(u0); replaces MACFlow._predict (mac/flow.py:59-64) with the
+ * convection of mac/grid.py:108-134 and velocity_laplacian of
+ * mac/grid.py:96-105. Face arrays of a closed n0 x n1 x n2 box (extent
+ * n[c] + 1 along c); out of place; bit-identical to the FP64 restatement. */
+int afsi_mac_predict_f64(const double* u0, const double* u1, const double* u2, const double* f0,
+                         const double* f1, const double* f2, double* s0, double* s1, double* s2,
+                         const int64_t n[3], const double h[3], double dt, double rho, double mu, void* stream);
+/* out[0..2] = max |u_c|, out[3] = count of non-finite entries of u and f:
+ * the inputs of MACFlow.step's finiteness check and transport screen
+ * (mac/flow.py:137-146). */
+size_t afsi_mac_speeds_workspace_bytes(void);
+int afsi_mac_speeds_f64(const double* u0, const double* u1, const double* u2, const double* f0,
+                        const double* f1, const double* f2, const int64_t n[3], double* out,
+                        void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---- FE interaction points and the consistent mass ------------------------
  * The quadrature IB equations of mac/transfer.py:1-5: M F = b,
  * f = V^-1 K^T W B F, M U = B^T W K u. K is the tiled spread/interpolation

@@ -296,3 +296,88 @@ def fft_poisson_periodic(rhs, spacing):
     P = R / lam
     P.flat[0] = 0.
     return np.real(np.fft.ifftn(P))
+
+
+# ---------------------------------------------------------------------------
+# explicit predictor of the closed box (MACFlow.step)
+
+COURANT_LIMIT = .25            # transport.py:9-11
+VISCOUS_LIMIT = .25
+ADVECTION_DIFFUSION_LIMIT = .25
+
+
+def zero_normal(velocity):
+    """mac/grid.py:62-67: copies with the wall-normal faces set to zero."""
+    out = tuple(np.array(u, dtype=np.float64, copy=True) for u in velocity)
+    for c, u in enumerate(out):
+        u[slab(c, 0, 1)] = 0.
+        u[slab(c, -1, None)] = 0.
+    return out
+
+
+def velocity_laplacian(u, component, spacing):
+    """mac/grid.py:96-105: odd tangential ghosts (no slip), wall-normal rows 0;
+    axes accumulated in order, ((left - 2u) + right) / h^2."""
+    out = np.zeros_like(u)
+    for axis, h in enumerate(spacing):
+        left = np.concatenate((-u[slab(axis, 0, 1)], u[slab(axis, None, -1)]), axis)
+        right = np.concatenate((u[slab(axis, 1, None)], -u[slab(axis, -1, None)]), axis)
+        out += (left - 2 * u + right) / (h * h)
+    out[slab(component, 0, 1)] = 0.
+    out[slab(component, -1, None)] = 0.
+    return out
+
+
+def convection(velocity, spacing):
+    """mac/grid.py:108-134: centered conservative MAC momentum flux with odd
+    ghosts across the walls; wall-normal rows 0."""
+    result = []
+    for c, u in enumerate(velocity):
+        out = np.zeros_like(u)
+        for axis, h in enumerate(spacing):
+            if axis == c:
+                flux = np.square(.5 * (u[slab(c, 1, None)] + u[slab(c, None, -1)]))
+                out[slab(c, 1, -1)] += np.diff(flux, axis=c) / h
+            else:
+                other = velocity[axis]
+                padded = np.concatenate((-other[slab(c, 0, 1)], other, -other[slab(c, -1, None)]), c)
+                advector = .5 * (padded[slab(c, 1, None)] + padded[slab(c, None, -1)])
+                padded_u = np.concatenate((-u[slab(axis, 0, 1)], u, -u[slab(axis, -1, None)]), axis)
+                carried = .5 * (padded_u[slab(axis, 1, None)] + padded_u[slab(axis, None, -1)])
+                out += np.diff(advector * carried, axis=axis) / h
+        out[slab(c, 0, 1)] = 0.
+        out[slab(c, -1, None)] = 0.
+        result.append(out)
+    return tuple(result)
+
+
+def predict(velocity, density, spacing, *, dt, rho=1., mu=1.):
+    """mac/flow.py:59-64: u* = zero_normal(u + dt (-N(u) + (mu/rho) L u + f/rho))
+    with u = zero_normal(velocity)."""
+    vel = zero_normal(velocity)
+    adv = convection(vel, spacing)
+    return zero_normal(tuple(u + dt * (-a + mu / rho * velocity_laplacian(u, c, spacing) + f / rho)
+                             for c, (u, a, f) in enumerate(zip(vel, adv, density))))
+
+
+def transport_numbers(velocity, spacing, *, dt, rho=1., mu=1.):
+    """mac/flow.py:142-146: Courant sum_c dt max|u_c|/h_c, cell Reynolds
+    max_c max|u_c| h_c/(mu/rho), advection-diffusion dt sum_c max|u_c|^2 /
+    (2 mu/rho)."""
+    speeds = np.array([np.abs(u).max() for u in velocity])
+    h = np.asarray(spacing)
+    nu = mu / rho
+    return (float((dt * speeds / h).sum()), float((speeds * h / nu).max()),
+            float(dt * np.square(speeds).sum() / (2 * nu)))
+
+
+def flow_step(velocity, density, spacing, mg, *, dt, rho=1., mu=1., initial=None):
+    """MACFlow.step (mac/flow.py:122-157) on the closed box: predictor, then
+    the projection; raises ValueError on the transport guards."""
+    courant, reynolds, ad = transport_numbers(velocity, spacing, dt=dt, rho=rho, mu=mu)
+    viscous = dt * mu / rho * sum(1 / hh ** 2 for hh in spacing)
+    if courant > COURANT_LIMIT or ad > ADVECTION_DIFFUSION_LIMIT or viscous > VISCOUS_LIMIT:
+        raise ValueError('MAC centered-advection stability guard exceeded')
+    star = predict(velocity, density, spacing, dt=dt, rho=rho, mu=mu)
+    u, p, info = project(star, spacing, mg, dt=dt, rho=rho, initial=initial)
+    return u, p, dict(info, courant=courant, cell_reynolds=reynolds, advection_diffusion_number=ad)

@@ -64,7 +64,7 @@ def test_mixed_projection_divergence_and_velocity(gpu, periodic):
     ref = mac3d.Multigrid(shape, h, mac3d.MGOptions(smooth=2, smoother='rbgs'), periodic)
     v_ref, _, _ = mac3d.project(tuple(u), h, ref, dt=1e-3)
     flow = MACFlow(grid, dt=1e-3, device=gpu, options=MGOptions(smooth=2, smoother='rbgs'),
-                   precision='mixed')
+                   precision='mixed', predictor=False)
     res = flow.project(tuple(torch.tensor(a, device=gpu) for a in u))
     vel = tuple(a.cpu().numpy() for a in res.velocity)
     div = mac3d.divergence(vel, h, periodic)

@@ -198,3 +198,46 @@ def test_projection_divergence_idempotence_and_gradient_force(periodic):
     grad = mac3d.gradient(phi, h, periodic)
     vel3, _, _ = mac3d.project(grad, h, mg, dt=1.)
     assert max(np.abs(v).max() for v in vel3) < 1e-8
+
+
+def face_coordinates(shape, lengths, c):
+    """Closed-box face centres of component c (n+1 along c, cell centres across)."""
+    h = [L / n for L, n in zip(lengths, shape)]
+    axes = [np.arange(shape[a] + 1) * h[a] if a == c else (np.arange(shape[a]) + .5) * h[a] for a in range(3)]
+    return np.stack(np.meshgrid(*axes, indexing='ij'), -1)
+
+
+def test_predictor_known_answers():
+    """tests/test_mac.py:87-100: the sine eigenmode of the velocity Laplacian
+    (eigenvalue sum -4 sin^2(pi/2n)/h^2) and the centered momentum flux of the
+    linear field u_c = x_c (4 u in the interior)."""
+    shape, lengths = (8, 8, 8), (1., 1., 1.)
+    h = [L / n for L, n in zip(lengths, shape)]
+    eig = sum(-4 * np.sin(pi / (2 * n)) ** 2 / hh ** 2 for n, hh in zip(shape, h))
+    for c in range(3):
+        xyz = face_coordinates(shape, lengths, c)
+        u = np.sin(pi * xyz).prod(-1)
+        fields = [np.zeros(face_coordinates(shape, lengths, a).shape[:3]) for a in range(3)]
+        fields[c] = u
+        u = mac3d.zero_normal(fields)[c]
+        np.testing.assert_allclose(mac3d.velocity_laplacian(u, c, h), eig * u, rtol=1e-11, atol=2e-12)
+    velocity = tuple(face_coordinates(shape, lengths, c)[..., c] for c in range(3))
+    for u, a in zip(velocity, mac3d.convection(velocity, h)):
+        np.testing.assert_allclose(a[2:-2, 2:-2, 2:-2], 4 * u[2:-2, 2:-2, 2:-2], atol=1e-12, rtol=1e-12)
+
+
+def test_gradient_body_force_projects_to_rest():
+    """tests/test_mac.py:103-113: a gradient body force leaves zero velocity
+    and its potential as the pressure; too large a viscous number raises."""
+    shape, lengths = (16, 16, 16), (2., 2., 2.)
+    h = tuple(L / n for L, n in zip(lengths, shape))
+    x = [(np.arange(n) + .5) * hh for n, hh in zip(shape, h)]
+    X, Y, Z = np.meshgrid(*x, indexing='ij')
+    p = np.cos(pi * X / 2) * np.cos(pi * Z / 2)
+    mg = mac3d.Multigrid(shape, h, mac3d.MGOptions())
+    zeros = tuple(np.zeros(face_coordinates(shape, lengths, c).shape[:3]) for c in range(3))
+    u, q, _ = mac3d.flow_step(zeros, mac3d.gradient(p, h), h, mg, dt=1e-4)
+    np.testing.assert_allclose(q - q.mean(), p - p.mean(), rtol=1e-8, atol=1e-9)
+    assert max(np.abs(v).max() for v in u) < 1e-11
+    with pytest.raises(ValueError, match='guard'):
+        mac3d.flow_step(zeros, mac3d.gradient(p, h), h, mg, dt=1.)
