@@ -78,20 +78,27 @@ class FEIBStep:
     (M F = b, f = V^-1 K^T W B F), u* = u + (dt/rho) f, project, interpolate
     through the mass solve (M U = B^T W K u), x += dt U with the grid
     displacement guard. Nodal forces are given (the solid model is off the
-    path); the predictor is the force term, as in ``IBStep``."""
+    path). ``predictor='force'`` uses u* = u + (dt/rho) f, as ``IBStep``;
+    ``predictor='full'`` runs ``MACFlow.step`` (transport screen, explicit
+    convection and viscosity, projection; closed box), the reference's
+    flow step."""
 
-    def __init__(self, grid, geometry, *, dt, rho=1., device='cuda', options=None, transfer_options=None,
+    def __init__(self, grid, geometry, *, dt, rho=1., mu=1., device='cuda', options=None, transfer_options=None,
                  warm_start=False, max_displacement=.25, tile=DEFAULT_TILE, pressure_backend='graph',
-                 precision='fp64'):
+                 precision='fp64', predictor='force'):
         from .mac.fe import FETransfer
         if not max_displacement > 0 or max_displacement != max_displacement:
             raise ValueError('max_displacement must be finite and positive')
+        if predictor not in ('force', 'full'):
+            raise ValueError("predictor must be 'force' or 'full'")
         self.grid, self.dt, self.rho = grid, float(dt), float(rho)
         self.device = torch.device(device)
         self.max_displacement = float(max_displacement)
         self.options = options or MGOptions(smooth=2, smoother='rbgs')
-        self.flow = MACFlow(grid, dt=dt, rho=rho, device=device, options=self.options,
-                            pressure_backend=pressure_backend, precision=precision, predictor=False)
+        self.predictor = predictor
+        self.flow = MACFlow(grid, dt=dt, rho=rho, mu=mu, device=device, options=self.options,
+                            pressure_backend=pressure_backend, precision=precision,
+                            predictor=predictor == 'full')
         self.transfer = FETransfer(grid, geometry, options=transfer_options, warm_start=warm_start, tile=tile)
         self.rhs = torch.empty(grid.shape, device=self.device, dtype=torch.float64)
         self._h = torch.tensor(grid.spacing, device=self.device, dtype=torch.float64)
@@ -107,12 +114,18 @@ class FEIBStep:
         f, spread_info = self.transfer.spread(force, stencil)
         mark('spread', 1)
         mark('project', 0)
-        a = self.dt / self.rho
-        for uc, fc in zip(velocity, f):
-            _abi.call('afsi_axpy_f64', _abi.ptr(uc), _abi.ptr(fc), uc.numel(), a, _abi.stream())
-        rhs = divergence(velocity, self.grid, -self.rho / self.dt, out=self.rhs)
-        p, pinfo = self.flow.pressure_solver.solve(rhs, pressure)
-        correct(velocity, p, self.grid, a)
+        if self.predictor == 'full':   # MACFlow.step: transport screen, predictor, projection
+            result = self.flow.step(velocity, f, pressure_initial=pressure)
+            for uc, vc in zip(velocity, result.velocity):
+                uc.copy_(vc)
+            p, pinfo = result.pressure, result.diagnostics
+        else:
+            a = self.dt / self.rho
+            for uc, fc in zip(velocity, f):
+                _abi.call('afsi_axpy_f64', _abi.ptr(uc), _abi.ptr(fc), uc.numel(), a, _abi.stream())
+            rhs = divergence(velocity, self.grid, -self.rho / self.dt, out=self.rhs)
+            p, pinfo = self.flow.pressure_solver.solve(rhs, pressure)
+            correct(velocity, p, self.grid, a)
         mark('project', 1)
         mark('interp', 0)
         U, interp_info = self.transfer.interpolate(velocity, stencil)

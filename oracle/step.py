@@ -49,7 +49,7 @@ def ib_step_2d(x, F, velocity, p_prev, channel, mg, *, dt, rho=1.):
                 info=info, tentative=(u, v), ids=ids, weights=w)
 
 
-def fe_ib_step(g, x, F, u, p_prev, grid, mg, *, dt, rho=1., options=None):
+def fe_ib_step(g, x, F, u, p_prev, grid, mg, *, dt, rho=1., options=None, full=False, mu=1.):
     """The coupled step with the FE quadrature transfer (mac/coupling.py:74-85,
     mac/transfer.py:104-140): M C = F, f = V^-1 K^T W B C at the interaction
     points B x, projection of u + (dt/rho) f, M U = B^T W K u, x + dt U."""
@@ -60,8 +60,11 @@ def fe_ib_step(g, x, F, u, p_prev, grid, mg, *, dt, rho=1., options=None):
     base, phi = ib3d.prepare_compact(fe.interaction_points(g, x), grid)
     C, spread_info = fe.pcg(lambda v: M @ v, F, diag, None, options)
     f = ib3d.spread_grid(fe.weighted(g, C), base, phi, grid)
-    star = tuple(uc + dt / rho * fc for uc, fc in zip(u, f))
-    velocity, p, info = mac3d.project(star, grid.spacing, mg, dt=dt, rho=rho, initial=p_prev)
+    if full:   # MACFlow.step (mac/flow.py:122-157)
+        velocity, p, info = mac3d.flow_step(u, f, grid.spacing, mg, dt=dt, rho=rho, mu=mu, initial=p_prev)
+    else:
+        star = tuple(uc + dt / rho * fc for uc, fc in zip(u, f))
+        velocity, p, info = mac3d.project(star, grid.spacing, mg, dt=dt, rho=rho, initial=p_prev)
     rhs = fe.assemble(g, ib3d.gather_grid(velocity, base, phi, grid))
     U, interp_info = fe.pcg(lambda v: M @ v, rhs, diag, None, options)
     return dict(force=f, velocity=velocity, pressure=p, nodal_velocity=U, x=x + dt * U, info=info,

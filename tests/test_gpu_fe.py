@@ -184,3 +184,32 @@ def test_fe_ib_step_matches_oracle(gpu):
     with pytest.raises(ValueError, match='displacement'):
         FEIBStep(grid, on_device(g, gpu), dt=50., device=gpu).step(
             torch.tensor(x, device=gpu), torch.tensor(F, device=gpu), grid.zeros(device=gpu))
+
+
+def test_fe_ib_step_with_full_flow_step_matches_oracle(gpu):
+    """The coupled step with MACFlow.step (transport screen, explicit
+    convection and viscosity, projection) two steps in a row, as
+    MACIBStepper.step runs it (mac/coupling.py:74-123)."""
+    from afsi_gpu_amd.mac import MGOptions
+    from afsi_gpu_amd.step import FEIBStep
+    from oracle import mac3d
+    from oracle.step import fe_ib_step
+    g, X = mesh('p2')
+    grid, ogrid = grids()
+    dt = 2e-3   # viscous number 2e-3 * 3 * 16 = 0.096
+    stepper = FEIBStep(grid, on_device(g, gpu), dt=dt, device=gpu, options=MGOptions(smooth=2, smoother='rbgs'),
+                       predictor='full')
+    mg = mac3d.Multigrid(ogrid.shape, ogrid.spacing, mac3d.MGOptions(smooth=2, smoother='rbgs'), periodic=False)
+    x = X + .013 * np.sin(3 * X)
+    F = np.sin(3 * X) * 20
+    u = grid.zeros(device=gpu)
+    u_ref = tuple(np.zeros(ogrid.face_shape(c)) for c in range(3))
+    xt, p, p_ref = torch.tensor(x, device=gpu), None, None
+    for k in range(2):
+        xt, u, p, U, info = stepper.step(xt, torch.tensor(F, device=gpu), u, p)
+        ref = fe_ib_step(g, x, F, u_ref, p_ref, ogrid, mg, dt=dt, full=True)
+        x, u_ref, p_ref = ref['x'], ref['velocity'], ref['pressure']
+        err_U = rel(U.cpu(), ref['nodal_velocity'])
+        err_u = max(rel(u[c].cpu(), u_ref[c]) for c in range(3))
+        print('full FE step', k, err_U, err_u, info['pressure']['courant'])
+        assert err_U < 1e-9 and err_u < 1e-9
