@@ -100,6 +100,15 @@ class PCGInfo(Structure):
     _fields_ = [('status', c_int32), ('iterations', c_int32), ('residual_norm', c_double),
                 ('rhs_norm', c_double), ('tolerance', c_double)]
 
+
+class P2Group(Structure):
+    _fields_ = [('cells', c_void_p), ('values', c_void_p), ('weights', c_void_p), ('elements', c_int64),
+                ('quad', c_int64), ('node_count', c_int64)]
+
+
+class P2Tables(Structure):
+    _fields_ = [('base', c_void_p), ('phi', c_void_p), ('total', c_int64), ('offset', c_int64)]
+
 _P = c_void_p
 _I64, _F64 = POINTER(c_int64), POINTER(c_double)
 _SIGNATURES = {
@@ -197,6 +206,11 @@ _SIGNATURES = {
     'afsi_mass_pcg_workspace_bytes': (c_size_t, [c_int64]),
     'afsi_mass_pcg_f64': (c_int, [POINTER(CSR), _P, _P, _P, _P, POINTER(PCGOptions), _P, c_size_t,
                                   POINTER(PCGInfo), _P]),
+    'afsi_p2_prepare_f64': (c_int, [_P, POINTER(P2Group), POINTER(Grid3), POINTER(P2Tables), _P, _P]),
+    'afsi_p2_spread_f64': (c_int, [_P, POINTER(P2Group), POINTER(P2Tables), POINTER(Grid3), c_double, _P, _P, _P,
+                                   _P, _P]),
+    'afsi_p2_gather_assemble_f64': (c_int, [_P, _P, _P, POINTER(P2Group), POINTER(P2Tables), POINTER(Grid3), _P,
+                                            _P, _P]),
 }
 
 

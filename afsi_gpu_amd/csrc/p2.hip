@@ -1,0 +1,467 @@
+// P2 fused cell-block transfer (gfx950): the three operators of
+// mac/csrc/p2_transfer.cpp:67-69 (kernels p2_transfer.cu:12-188).
+//
+// prepare: interaction points x_q = sum_a N_qa x[cells[e,a]] evaluated in
+//   registers and turned directly into the shared face/centre Peskin tables
+//   base (2,total,3) int64 and phi (2,total,3,4) of mac/shared_stencil.py:8-12
+//   (lattice 0 = faces, lattice 1 = centres; component c reads lattice 0 along
+//   c and lattice 1 elsewhere), with the support flag of
+//   mac/compact_transfer.py:94-96 (2 <= (x-origin)/h < n-2, finite).
+// spread: f_c += (phi_x phi_y phi_z) (W_eq sum_a N_qa F[cells[e,a]]_c) / V
+//   without materialising interaction points or point forces.
+// gather-assemble: rhs[cells[e,a]]_c += sum_q (N_qa W_eq) U_c(x_q).
+//
+// One workgroup (4 waves) per element. The element's stencil bounding box,
+// per component, is an LDS patch: the spread accumulates into it with LDS
+// FP64 atomics and flushes each non-zero entry once with a hardware global
+// FP64 atomic; the gather stages it from HBM once and reads it from LDS.
+// Table chunks of 64 points are staged in LDS with coalesced loads.
+//   spread: one wave per point, lane = stencil tap (i,j,k) = (l>>4, l>>2&3, l&3):
+//     the 64 lanes of one LDS atomic hit 64 distinct patch entries, so dense
+//     points never collide inside an instruction.
+//   gather: four lanes per point, lane part p takes the taps with i = p, and
+//     the four partial sums are added in a fixed order.
+// Elements whose patch exceeds P2_PATCH values fall back to direct global
+// atomics (spread) or direct HBM reads (gather); both stay exact.
+#include "common.hpp"
+
+namespace afsi {
+namespace {
+
+constexpr int P2_NODES = 10;
+constexpr int P2_THREADS = 256;
+constexpr int P2_WAVES = P2_THREADS / WAVE;
+constexpr int P2_CHUNK = 64;     // points staged per pass
+constexpr int P2_PATCH = 3072;   // LDS patch capacity (values, three components)
+
+struct Lattice {
+    double origin[2][3];  // [0] faces: origin; [1] centres: origin + h/2
+    double h[3];
+    double limit[3];      // n - 2
+    int64_t n[3];
+};
+
+// ---------------------------------------------------------------------------
+// prepare: one thread per interaction point, all three axes.
+__global__ void __launch_bounds__(256) p2_prepare_kernel(const double* __restrict__ x,
+                                                         const int64_t* __restrict__ cells,
+                                                         const double* __restrict__ N, int64_t ne, int nq,
+                                                         int64_t nodes, Lattice L, int64_t* __restrict__ base,
+                                                         double* __restrict__ phi, int64_t total, int64_t offset,
+                                                         int* __restrict__ invalid) {
+    const int64_t count = ne * nq;
+    for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < count;
+         p += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t e = p / nq;
+        const int q = int(p - e * nq);
+        int64_t node[P2_NODES];
+        bool bad_node = false;
+#pragma unroll
+        for (int a = 0; a < P2_NODES; ++a) {
+            node[a] = cells[e * P2_NODES + a];
+            bad_node = bad_node || node[a] < 0 || node[a] >= nodes;
+        }
+        if (bad_node) {
+            raise_flag(invalid, F_INDEX);
+            continue;
+        }
+        double pos[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int a = 0; a < P2_NODES; ++a) {
+            const double w = N[q * P2_NODES + a];
+            pos[0] += w * x[node[a] * 3];
+            pos[1] += w * x[node[a] * 3 + 1];
+            pos[2] += w * x[node[a] * 3 + 2];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double physical = (pos[a] - L.origin[0][a]) / L.h[a];
+            const bool valid = isfinite(physical) && physical >= 2.0 && physical < L.limit[a];
+            if (!valid) raise_flag(invalid, F_SUPPORT);
+#pragma unroll
+            for (int l = 0; l < 2; ++l) {
+                const double scaled = valid ? (pos[a] - L.origin[l][a]) / L.h[a] : 0.0;
+                const double lower = floor(scaled - 1.0);
+                const double f = scaled - lower - 1.0;
+                double w[4];
+                peskin_weights(f, w);
+                const int64_t index = (l * total + offset + p) * 3 + a;
+                base[index] = int64_t(lower);
+                phi[index * 4] = w[0];
+                phi[index * 4 + 1] = w[1];
+                phi[index * 4 + 2] = w[2];
+                phi[index * 4 + 3] = w[3];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Element patch: per component c the box start[c][a] .. start + ext[c][a].
+struct Patch {
+    int start[3][3];
+    int ext[3][3];
+    int off[4];
+    int status;  // 0, F_SUPPORT (box outside the face arrays) or F_INDEX (node id)
+};
+
+struct Stage {
+    double phi[2][P2_CHUNK * 12];  // [lattice][point * 12 + axis * 4 + k]
+    int base[2][P2_CHUNK * 3];     // [lattice][point * 3 + axis]
+};
+
+__device__ inline int wave_min(int v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v = min(v, __shfl_xor(v, m));
+    return v;
+}
+__device__ inline int wave_max(int v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v = max(v, __shfl_xor(v, m));
+    return v;
+}
+
+constexpr int BASE_LIMIT = 1 << 28;
+
+// Bounding box of the element's bases per lattice and axis, turned into the
+// three component boxes. status = 0 when every box lies inside its face
+// array, every base is representable and every node id of the element is in
+// range. Uniform across the workgroup on return.
+__device__ void plan_patch(Patch& s, int (*red)[12], const int64_t* __restrict__ base, int64_t first, int nq,
+                           int64_t total, const int64_t n[3], const int64_t* __restrict__ cell, int64_t nodes) {
+    const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    int lo[6], hi[6];
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { lo[k] = BASE_LIMIT; hi[k] = -BASE_LIMIT; }
+    for (int q = tid; q < nq; q += P2_THREADS) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int64_t v = base[((k / 3) * total + first + q) * 3 + k % 3];
+            ok = ok && v > -BASE_LIMIT && v < BASE_LIMIT;
+            lo[k] = min(lo[k], int(v));
+            hi[k] = max(hi[k], int(v));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        lo[k] = wave_min(lo[k]);
+        hi[k] = wave_max(hi[k]);
+    }
+    int bad = ok ? 0 : F_SUPPORT;
+    if (tid < P2_NODES && (cell[tid] < 0 || cell[tid] >= nodes)) bad |= F_INDEX;
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) bad |= __shfl_xor(bad, m);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            red[wave][k] = lo[k];
+            red[wave][6 + k] = hi[k];
+        }
+        red[P2_WAVES][wave] = bad;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int l6[6], h6[6];
+        int any_bad = 0;
+        for (int k = 0; k < 6; ++k) {
+            l6[k] = red[0][k];
+            h6[k] = red[0][6 + k];
+            for (int w = 1; w < P2_WAVES; ++w) {
+                l6[k] = min(l6[k], red[w][k]);
+                h6[k] = max(h6[k], red[w][6 + k]);
+            }
+        }
+        for (int w = 0; w < P2_WAVES; ++w) any_bad |= red[P2_WAVES][w];
+        bool valid = any_bad == 0;
+        s.status = any_bad;
+        s.off[0] = 0;
+        for (int c = 0; c < 3; ++c) {
+            int size = 1;
+            for (int a = 0; a < 3; ++a) {
+                const int k = 3 * int(a != c) + a;  // lattice (a != c), axis a
+                s.start[c][a] = l6[k];
+                s.ext[c][a] = h6[k] - l6[k] + 4;
+                const int64_t extent = n[a] + int(a == c);
+                valid = valid && l6[k] >= 0 && int64_t(l6[k]) + s.ext[c][a] <= extent;
+                size = valid ? size * s.ext[c][a] : 0;
+            }
+            s.off[c + 1] = s.off[c] + size;
+        }
+        if (!valid && !s.status) s.status = F_SUPPORT;
+    }
+    __syncthreads();
+}
+
+__device__ inline int64_t face_index(int c, int x, int y, int z, const int64_t n[3]) {
+    return (int64_t(x) * (n[1] + int(c == 1)) + y) * (n[2] + int(c == 2)) + z;
+}
+
+__device__ inline int patch_index(const Patch& s, int c, int x, int y, int z) {
+    return s.off[c] + ((x - s.start[c][0]) * s.ext[c][1] + (y - s.start[c][1])) * s.ext[c][2] + (z - s.start[c][2]);
+}
+
+// Patch entry i -> component and grid coordinates.
+__device__ inline int patch_decode(const Patch& s, int i, int& x, int& y, int& z) {
+    const int c = i >= s.off[2] ? 2 : (i >= s.off[1] ? 1 : 0);
+    int p = i - s.off[c];
+    z = p % s.ext[c][2] + s.start[c][2];
+    p /= s.ext[c][2];
+    y = p % s.ext[c][1] + s.start[c][1];
+    x = p / s.ext[c][1] + s.start[c][0];
+    return c;
+}
+
+// Coalesced copy of one chunk's tables into LDS.
+__device__ inline void stage_chunk(Stage& st, const int64_t* __restrict__ base, const double* __restrict__ phi,
+                                   int64_t first, int q0, int cnt, int64_t total) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        const int64_t row = l * total + first + q0;
+        const double* src = phi + row * 12;
+        for (int i = tid; i < cnt * 12; i += P2_THREADS) st.phi[l][i] = src[i];
+        const int64_t* bsrc = base + row * 3;
+        for (int i = tid; i < cnt * 3; i += P2_THREADS) st.base[l][i] = int(bsrc[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(P2_THREADS) p2_spread_kernel(
+    const double* __restrict__ value, const int64_t* __restrict__ cells, const double* __restrict__ N,
+    const double* __restrict__ W, int nq, const int64_t* __restrict__ base, const double* __restrict__ phi,
+    int64_t total, int64_t offset, double* __restrict__ u0, double* __restrict__ u1, double* __restrict__ u2,
+    int64_t nodes, Lattice L, double volume, int* __restrict__ flags) {
+    __shared__ Patch s;
+    __shared__ int red[P2_WAVES + 1][12];
+    __shared__ double nodal[P2_NODES * 3];
+    __shared__ double force[P2_CHUNK * 3];
+    __shared__ Stage st;
+    __shared__ double patch[P2_PATCH];
+
+    const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    const int64_t e = blockIdx.x, first = offset + e * nq;
+    plan_patch(s, red, base, first, nq, total, L.n, cells + e * P2_NODES, nodes);
+    if (s.status) {
+        if (tid == 0) raise_flag(flags, s.status);
+        return;
+    }
+    const bool cached = s.off[3] <= P2_PATCH;
+    if (tid < P2_NODES * 3) nodal[tid] = value[cells[e * P2_NODES + tid / 3] * 3 + tid % 3];
+    if (cached)
+        for (int i = tid; i < s.off[3]; i += P2_THREADS) patch[i] = 0.0;
+    double* const fields[3] = {u0, u1, u2};
+    const int ti = lane >> 4, tj = (lane >> 2) & 3, tk = lane & 3;
+
+    for (int q0 = 0; q0 < nq; q0 += P2_CHUNK) {
+        const int cnt = min(P2_CHUNK, nq - q0);
+        __syncthreads();  // previous chunk consumed; nodal and patch zero visible
+        stage_chunk(st, base, phi, first, q0, cnt, total);
+        if (tid < cnt * 3) {
+            const int q = tid / 3, c = tid % 3;
+            const double* Nq = N + int64_t(q0 + q) * P2_NODES;
+            double f = 0.0;
+#pragma unroll
+            for (int a = 0; a < P2_NODES; ++a) f += Nq[a] * nodal[a * 3 + c];
+            force[tid] = (f * W[e * nq + q0 + q]) / volume;
+        }
+        __syncthreads();
+        for (int q = wave; q < cnt; q += P2_WAVES) {
+            const double* P0 = st.phi[0] + q * 12;
+            const double* P1 = st.phi[1] + q * 12;
+            const double fx0 = P0[ti], fx1 = P1[ti];
+            const double fy0 = P0[4 + tj], fy1 = P1[4 + tj];
+            const double fz0 = P0[8 + tk], fz1 = P1[8 + tk];
+            const int* B0 = st.base[0] + q * 3;
+            const int* B1 = st.base[1] + q * 3;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double w = c == 0 ? (fx0 * fy1) * fz1 : (c == 1 ? (fx1 * fy0) * fz1 : (fx1 * fy1) * fz0);
+                const double v = w * force[q * 3 + c];
+                const int x = (c == 0 ? B0[0] : B1[0]) + ti;
+                const int y = (c == 1 ? B0[1] : B1[1]) + tj;
+                const int z = (c == 2 ? B0[2] : B1[2]) + tk;
+                if (cached) atomicAdd(patch + patch_index(s, c, x, y, z), v);
+                else unsafeAtomicAdd(fields[c] + face_index(c, x, y, z, L.n), v);
+            }
+        }
+    }
+    if (!cached) return;
+    __syncthreads();
+    for (int i = tid; i < s.off[3]; i += P2_THREADS) {
+        const double v = patch[i];
+        if (v == 0.0) continue;
+        int x, y, z;
+        const int c = patch_decode(s, i, x, y, z);
+        unsafeAtomicAdd(fields[c] + face_index(c, x, y, z, L.n), v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(P2_THREADS) p2_gather_assemble_kernel(
+    const double* __restrict__ u0, const double* __restrict__ u1, const double* __restrict__ u2,
+    const int64_t* __restrict__ cells, const double* __restrict__ N, const double* __restrict__ W, int nq,
+    const int64_t* __restrict__ base, const double* __restrict__ phi, int64_t total, int64_t offset,
+    double* __restrict__ rhs, int64_t nodes, Lattice L, int* __restrict__ flags) {
+    __shared__ Patch s;
+    __shared__ int red[P2_WAVES + 1][12];
+    __shared__ double part[4][P2_CHUNK * 3];  // [i tap][point * 3 + c]
+    __shared__ double sums[30][8];
+    __shared__ Stage st;
+    __shared__ double patch[P2_PATCH];
+
+    const int tid = threadIdx.x;
+    const int64_t e = blockIdx.x, first = offset + e * nq;
+    plan_patch(s, red, base, first, nq, total, L.n, cells + e * P2_NODES, nodes);
+    if (s.status) {
+        if (tid == 0) raise_flag(flags, s.status);
+        return;
+    }
+    const bool cached = s.off[3] <= P2_PATCH;
+    const double* const fields[3] = {u0, u1, u2};
+    if (cached)
+        for (int i = tid; i < s.off[3]; i += P2_THREADS) {
+            int x, y, z;
+            const int c = patch_decode(s, i, x, y, z);
+            patch[i] = fields[c][face_index(c, x, y, z, L.n)];
+        }
+    // thread -> (point in chunk, tap slice i)
+    const int pq = tid % P2_CHUNK, ti = tid / P2_CHUNK;
+    // thread -> (node a, component c, residue r) of the node assembly
+    const int aa = tid / 24, ac = (tid / 8) % 3, ar = tid % 8;
+    double acc = 0.0;
+
+    for (int q0 = 0; q0 < nq; q0 += P2_CHUNK) {
+        const int cnt = min(P2_CHUNK, nq - q0);
+        __syncthreads();
+        stage_chunk(st, base, phi, first, q0, cnt, total);
+        __syncthreads();
+        if (pq < cnt) {
+            const double* P0 = st.phi[0] + pq * 12;
+            const double* P1 = st.phi[1] + pq * 12;
+            const int* B0 = st.base[0] + pq * 3;
+            const int* B1 = st.base[1] + pq * 3;
+            const double fx0 = P0[ti], fx1 = P1[ti];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double wx = c == 0 ? fx0 : fx1;
+                const double* Py = (c == 1 ? P0 : P1) + 4;
+                const double* Pz = (c == 2 ? P0 : P1) + 8;
+                const int x = (c == 0 ? B0[0] : B1[0]) + ti;
+                const int y0 = c == 1 ? B0[1] : B1[1];
+                const int z0 = c == 2 ? B0[2] : B1[2];
+                double got = 0.0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const double wxy = wx * Py[j];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const double v = cached ? patch[patch_index(s, c, x, y0 + j, z0 + k)]
+                                                : fields[c][face_index(c, x, y0 + j, z0 + k, L.n)];
+                        got += v * (wxy * Pz[k]);
+                    }
+                }
+                part[ti][pq * 3 + c] = got;
+            }
+        }
+        __syncthreads();
+        if (tid < 240) {
+            for (int q = ar; q < cnt; q += 8) {
+                const double got = ((part[0][q * 3 + ac] + part[1][q * 3 + ac]) + part[2][q * 3 + ac]) +
+                                   part[3][q * 3 + ac];
+                acc += (N[int64_t(q0 + q) * P2_NODES + aa] * W[e * nq + q0 + q]) * got;
+            }
+        }
+    }
+    if (tid < 240) sums[aa * 3 + ac][ar] = acc;
+    __syncthreads();
+    if (tid < 30) {
+        double v = 0.0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v += sums[tid][r];
+        unsafeAtomicAdd(rhs + cells[e * P2_NODES + tid / 3] * 3 + tid % 3, v);
+    }
+}
+
+int make_lattice(const afsi_grid3* g, Lattice& L) {
+    AFSI_REQUIRE(g, "null grid");
+    AFSI_REQUIRE(!g->periodic, "the P2 fused transfer requires a closed MAC box");
+    for (int a = 0; a < 3; ++a) {
+        AFSI_REQUIRE(g->n[a] > 0 && g->n[a] < BASE_LIMIT, "invalid P2 MAC grid");
+        AFSI_REQUIRE(std::isfinite(g->h[a]) && g->h[a] > 0, "invalid P2 MAC grid");
+        L.n[a] = g->n[a];
+        L.h[a] = g->h[a];
+        L.origin[0][a] = g->origin[a];
+        L.origin[1][a] = g->center_origin[a];
+        L.limit[a] = double(g->n[a] - 2);
+    }
+    return OK;
+}
+
+int check_group(const afsi_p2_group* g, const afsi_p2_tables* t) {
+    AFSI_REQUIRE(g && t, "null P2 group or tables");
+    AFSI_REQUIRE(g->elements >= 0 && g->quad > 0 && g->quad < (1 << 20) && g->node_count > 0,
+                 "P2 transfer requires ten basis functions");
+    AFSI_REQUIRE(g->elements == 0 || (g->cells && g->values), "null P2 group arrays");
+    AFSI_REQUIRE(t->base && t->phi, "P2 IB requires shared face/center tables");
+    AFSI_REQUIRE(t->offset >= 0 && t->total >= 0 && t->offset <= t->total &&
+                     g->elements * g->quad <= t->total - t->offset,
+                 "P2 IB group exceeds table bounds");
+    AFSI_REQUIRE(g->elements < (int64_t(1) << 31), "too many P2 elements in one group");
+    return OK;
+}
+
+}  // namespace
+}  // namespace afsi
+
+using namespace afsi;
+
+extern "C" {
+
+int afsi_p2_prepare_f64(const double* x, const afsi_p2_group* g, const afsi_grid3* grid, afsi_p2_tables* t,
+                        int* invalid, void* stream) {
+    if (int r = check_group(g, t)) return r;
+    Lattice L;
+    if (int r = make_lattice(grid, L)) return r;
+    AFSI_REQUIRE(x, "null positions");
+    const int64_t count = g->elements * g->quad;
+    if (!count) return OK;
+    p2_prepare_kernel<<<grid_for(count, 256), 256, 0, hipStream_t(stream)>>>(
+        x, g->cells, g->values, g->elements, int(g->quad), g->node_count, L, t->base, t->phi, t->total, t->offset,
+        invalid);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_p2_spread_f64(const double* value, const afsi_p2_group* g, const afsi_p2_tables* t,
+                       const afsi_grid3* grid, double volume, double* u0, double* u1, double* u2, int* flags,
+                       void* stream) {
+    if (int r = check_group(g, t)) return r;
+    Lattice L;
+    if (int r = make_lattice(grid, L)) return r;
+    AFSI_REQUIRE(value && u0 && u1 && u2 && g->weights, "null P2 spread arrays");
+    AFSI_REQUIRE(std::isfinite(volume) && volume > 0, "invalid P2 MAC grid");
+    if (!g->elements) return OK;
+    p2_spread_kernel<<<unsigned(g->elements), P2_THREADS, 0, hipStream_t(stream)>>>(
+        value, g->cells, g->values, g->weights, int(g->quad), t->base, t->phi, t->total, t->offset, u0, u1, u2,
+        g->node_count, L, volume, flags);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_p2_gather_assemble_f64(const double* u0, const double* u1, const double* u2, const afsi_p2_group* g,
+                                const afsi_p2_tables* t, const afsi_grid3* grid, double* rhs, int* flags,
+                                void* stream) {
+    if (int r = check_group(g, t)) return r;
+    Lattice L;
+    if (int r = make_lattice(grid, L)) return r;
+    AFSI_REQUIRE(u0 && u1 && u2 && rhs && g->weights, "null P2 gather arrays");
+    if (!g->elements) return OK;
+    p2_gather_assemble_kernel<<<unsigned(g->elements), P2_THREADS, 0, hipStream_t(stream)>>>(
+        u0, u1, u2, g->cells, g->values, g->weights, int(g->quad), t->base, t->phi, t->total, t->offset, rhs,
+        g->node_count, L, flags);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+}  // extern "C"

@@ -8,9 +8,13 @@ the Jacobi PCG of fluid/solvers.py:39-122 / mac/mass_solver.py:39-103.
 B (FE evaluation at the interaction points), W B, B^T W, the CSR mass action
 and the PCG loop are HIP kernels (``csrc/fe.hip``); K is the tiled
 Peskin-4 spread and interpolation of ``mac.transfer`` at the interaction
-points. The mass matrix and the node-incidence lists are assembled once at
-construction (static geometry). There is no CPU fallback: without the HIP
-library every call raises.
+points. With ``fused=True`` (10-node tetrahedra in a closed box) the
+transfer instead runs the P2 fused cell-block kernels of ``mac.p2_transfer``
+(the reference's production P2 path, mac/adaptive_transfer.py:334-405 with
+a CUDA-prepared shared stencil): no interaction points, point forces or bins
+are materialised. The mass matrix and the node-incidence lists are
+assembled once at construction (static geometry). There is no CPU fallback:
+without the HIP library every call raises.
 """
 from dataclasses import dataclass
 from math import isfinite
@@ -18,6 +22,7 @@ from math import isfinite
 import torch
 
 from .. import _abi
+from . import cuda_ib, p2_transfer
 from .transfer import DEFAULT_TILE, Binner, TiledTransfer
 
 
@@ -146,8 +151,13 @@ class FETransfer:
     """GPU mirror of ``FETransfer`` (mac/transfer.py:19-140): same
     constructor arguments, methods, warm start and error messages."""
 
-    def __init__(self, grid, geometry, *, options=None, warm_start=False, tile=DEFAULT_TILE):
+    def __init__(self, grid, geometry, *, options=None, warm_start=False, tile=DEFAULT_TILE, fused=False):
         self.grid, self.geometry = grid, geometry
+        if type(fused) is not bool:
+            raise ValueError('fused must be a bool')
+        if fused and (geometry.cells.shape[-1] != 10 or grid.periodic):
+            raise ValueError('the fused P2 transfer requires 10-node cells in a closed MAC box')
+        self.fused = fused
         if type(warm_start) is not bool:
             raise ValueError('warm_start must be a bool')
         self.warm_start = warm_start
@@ -185,8 +195,12 @@ class FETransfer:
         self.points_count = E * Q
         self.solver = MassSolver(self.mass, self.diagonal, self.options)
         self.tile = tile
-        self.binner = Binner(grid, self.points_count, device, tile)
-        self.transfer = TiledTransfer(grid, device, tile)
+        if fused:
+            self.rule = p2_transfer.rule_from_geometry(geometry)
+            p2_transfer.check_rule(self.rule, geometry.node_count, W)
+        else:
+            self.binner = Binner(grid, self.points_count, device, tile)
+            self.transfer = TiledTransfer(grid, device, tile)
 
     def reset_warm_start(self):
         """Discard cached PCG initial guesses when starting a new trajectory."""
@@ -233,6 +247,10 @@ class FETransfer:
 
     @torch.no_grad()
     def prepare(self, x):
+        if self.fused:
+            self._nodal(x)
+            base, phi = p2_transfer.prepare(self, x, self.rule)
+            return p2_transfer.SharedStencil(base, phi, self.rule)
         points = self.interaction_points(x)
         self.check_support(points)
         bins = self.binner(points)
@@ -245,12 +263,18 @@ class FETransfer:
         coefficient, info = self.solve_mass(nodal_force, self._force_coefficient if self.warm_start else None)
         if self.warm_start:
             self._force_coefficient = coefficient
+        if isinstance(stencil, p2_transfer.SharedStencil):
+            return p2_transfer.spread(self, coefficient, stencil), info
         return self.spread_grid(self.weighted_force(coefficient), stencil), info
 
     def spread_grid(self, force_q, stencil):
+        if isinstance(stencil, p2_transfer.SharedStencil):
+            return cuda_ib.compact_spread(self.grid, force_q.contiguous(), stencil.expanded())
         return self.transfer.spread(stencil.points, force_q.contiguous(), stencil.bins)
 
     def gather_grid(self, velocity, stencil):
+        if isinstance(stencil, p2_transfer.SharedStencil):
+            return cuda_ib.compact_gather(self.grid, tuple(u.contiguous() for u in velocity), stencil.expanded())
         return self.transfer.interpolate(stencil.points, [u.contiguous() for u in velocity], stencil.bins)
 
     def assemble_velocity(self, point_velocity):
@@ -265,7 +289,10 @@ class FETransfer:
         if any(u.device != self.geometry.weights.device or u.dtype != self.geometry.weights.dtype
                or not torch.isfinite(u).all() for u in velocity):
             raise ValueError('invalid MAC interpolation field')
-        rhs = self.assemble_velocity(self.gather_grid(velocity, stencil))
+        if isinstance(stencil, p2_transfer.SharedStencil):
+            rhs = p2_transfer.assemble_velocity(self, velocity, stencil)
+        else:
+            rhs = self.assemble_velocity(self.gather_grid(velocity, stencil))
         result, info = self.solve_mass(rhs, self._velocity_coefficient if self.warm_start else None)
         if self.warm_start:
             self._velocity_coefficient = result

@@ -241,3 +241,65 @@ def fe_sphere(level, radius=.3, centre=(.5, .5, .5)):
     p = nodes[f]
     area = .5 * np.linalg.norm(np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), axis=1)
     return nodes, f, values, np.repeat(area[:, None] / 3, 3, axis=1)
+
+
+# P2 tetrahedra: node order = 4 vertices, then the midpoints of these edges
+# (tetrahedron.py:9-56)
+P2_EDGES = ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))
+
+
+def collapsed_gauss_tet(n):
+    """n^3-point collapsed (Duffy) Gauss-Legendre rule on the unit
+    tetrahedron, positive weights summing to 1/6; exact for total degree
+    <= 2n - 3 (n = 4: the 64-point rule, exact for the P2 mass)."""
+    u, wu = np.polynomial.legendre.leggauss(n)
+    u, wu = (u + 1) / 2, wu / 2
+    a, b, c = np.meshgrid(u, u, u, indexing='ij')
+    wa, wb, wc = np.meshgrid(wu, wu, wu, indexing='ij')
+    pts = np.stack([a, b * (1 - a), c * (1 - a) * (1 - b)], -1).reshape(-1, 3)
+    w = (wa * wb * wc * (1 - a) ** 2 * (1 - b)).reshape(-1)
+    return pts, w
+
+
+def p2_values(points):
+    """The ten P2 basis functions at reference points (tetrahedron.py:19-30)."""
+    L = np.concatenate([1 - points.sum(1, keepdims=True), points], 1)
+    i, j = np.array(P2_EDGES).T
+    return np.concatenate([L * (2 * L - 1), 4 * L[:, i] * L[:, j]], 1)
+
+
+def lv_p2(nt=3, ntheta=48, nphi=70, rule=4, inner=(.36, .24), outer=(.42, .30), centre=(.5, .5, .5),
+          theta=(.15, .6 * np.pi)):
+    """Idealised left-ventricle wall of straight P2 tetrahedra: the truncated
+    prolate shell between the ellipsoids of semi-axes ``inner`` and
+    ``outer`` (long axis along x, apex towards -x), polar angle in
+    ``theta`` (an apical hole of 0.15 rad, truncated beyond the equator),
+    ``nt`` x ``ntheta`` x ``nphi`` hexahedra split into six tetrahedra along
+    their main diagonal, each promoted to ten nodes with shared edge
+    midpoints, and the ``rule``^3-point collapsed Gauss rule. The default
+    has 60,480 elements and 3,870,720 interaction points.
+    Returns (nodes (V,3), cells (E,10) int64, values N (Q,10), weights W (E,Q))."""
+    i, j, k = np.meshgrid(np.arange(nt + 1), np.arange(ntheta + 1), np.arange(nphi), indexing='ij')
+    s = i / nt
+    th = theta[0] + (theta[1] - theta[0]) * j / ntheta
+    ph = 2 * np.pi * k / nphi
+    a = inner[0] + s * (outer[0] - inner[0])
+    b = inner[1] + s * (outer[1] - inner[1])
+    vertices = np.stack([centre[0] - a * np.cos(th), centre[1] + b * np.sin(th) * np.cos(ph),
+                         centre[2] + b * np.sin(th) * np.sin(ph)], -1).reshape(-1, 3)
+    hi, hj, hk = (g.reshape(-1) for g in np.meshgrid(np.arange(nt), np.arange(ntheta), np.arange(nphi),
+                                                        indexing='ij'))
+    node = lambda di, dj, dk: ((hi + di) * (ntheta + 1) + hj + dj) * nphi + (hk + dk) % nphi
+    corner = [node(c & 1, (c >> 1) & 1, (c >> 2) & 1) for c in range(8)]
+    tets = []
+    for p in ((1, 2, 4), (1, 4, 2), (2, 1, 4), (2, 4, 1), (4, 1, 2), (4, 2, 1)):
+        tets.append(np.stack([corner[0], corner[p[0]], corner[p[0] | p[1]], corner[7]], 1))
+    tets = np.concatenate(tets).astype(np.int64)
+    pairs = np.sort(tets[:, np.array(P2_EDGES)].reshape(-1, 2), axis=1)
+    unique, inverse = np.unique(pairs, axis=0, return_inverse=True)
+    nodes = np.concatenate([vertices, vertices[unique].mean(1)])
+    cells = np.concatenate([tets, len(vertices) + inverse.reshape(-1, 6)], 1)
+    pts, w = collapsed_gauss_tet(rule)
+    v = vertices[tets]
+    volume = np.abs(np.einsum('ei,ei->e', v[:, 1] - v[:, 0], np.cross(v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]))) / 6
+    return nodes, cells, p2_values(pts), 6 * volume[:, None] * w[None]

@@ -71,6 +71,7 @@ def parse():
                     help='instances of the batched sample (0: skip)')
     ap.add_argument('--fe-level', type=int, default=7,
                     help='sphere subdivision level of the FE interaction-point sample (0: skip)')
+    ap.add_argument('--no-p2', action='store_true', help='skip the P2 fused-transfer sample')
     return ap.parse_args()
 
 
@@ -242,6 +243,84 @@ def fe_sample(torch, n=128, level=7, steps=10, warmup=2):
                          f'interpolate + mass PCG), bin phase = B x + binning')
 
 
+def p2_sample(torch, n=64, reps=10, warmup=2):
+    """P2 fused cell-block transfer (FETransfer(fused=True)) on an idealised
+    LV wall of straight P2 tetrahedra with the 64-point rule (60,480
+    elements, 3,870,720 interaction points) in an n^3 closed unit box:
+    prepare, spread with the force mass solve, interpolate with the velocity
+    mass solve (cold start: every solve from zero), the mass solve alone, and
+    the two fused kernels, against the
+    unfused transfer (points, binning, tiled kernels) on the same inputs.
+    Algorithmic bytes of the fused kernels: tables 240 B and weight 8 B per
+    point, 8 B per touched face value (written by the spread, read by the
+    gather) and 24 B per node."""
+    from afsi_gpu_amd import workloads
+    from afsi_gpu_amd.mac import MACGrid
+    from afsi_gpu_amd.mac import p2_transfer as p2
+    from afsi_gpu_amd.mac.fe import FEGeometry, FETransfer
+    device = torch.device('cuda:0')
+    nodes, cells, values, weights = workloads.lv_p2()
+    t = lambda a, dt=torch.float64: torch.tensor(a, device=device, dtype=dt)
+    geometry = FEGeometry(t(cells, torch.int64), t(values), t(weights), len(nodes))
+    grid = MACGrid((n,) * 3, (1., 1., 1.))
+    gen = workloads.rng(workloads.GEOMETRY_SEED + 7)
+    x = t(nodes)
+    F = t(gen.normal(size=nodes.shape) * (weights.sum() / len(nodes)))
+    coords = [grid.coordinates(c, device=device) for c in range(3)]
+    velocity = tuple(torch.sin(2 * np.pi * coords[c][..., (c + 1) % 3]) * torch.cos(np.pi * coords[c][..., c])
+                     for c in range(3))
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        start = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+        stop = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+        torch.cuda.synchronize()
+        for k in range(reps):
+            start[k].record()
+            out = fn()
+            stop[k].record()
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in zip(start, stop)])), out
+
+    out = {}
+    for name, fused in (('fused', True), ('unfused', False)):
+        tr = FETransfer(grid, geometry, fused=fused)
+        ms = {}
+        ms['prepare'], st = timed(lambda: tr.prepare(x))
+        ms['spread_with_mass'], (faces, finfo) = timed(lambda: tr.spread(F, st))
+        ms['interp_with_mass'], (U, uinfo) = timed(lambda: tr.interpolate(velocity, st))
+        ms['mass_solve'], _ = timed(lambda: tr.solve_mass(F, None))
+        coefficient, _ = tr.solve_mass(F, None)
+        if fused:
+            ms['spread_kernel'], faces = timed(lambda: p2.spread(tr, coefficient, st))
+            ms['gather_kernel'], _ = timed(lambda: p2.assemble_velocity(tr, velocity, st))
+        else:
+            force_q = tr.weighted_force(coefficient)
+            ms['spread_kernel'], _ = timed(lambda: tr.spread_grid(force_q, st))
+            ms['gather_kernel'], _ = timed(lambda: tr.gather_grid(velocity, st))
+        out[name] = dict(ms=ms, pcg_iterations=[finfo.iterations, uinfo.iterations])
+        if fused:
+            touched = int(sum(int((f != 0).sum()) for f in faces))
+            del st
+        else:
+            del st, force_q
+        del tr
+        torch.cuda.empty_cache()
+    P = cells.shape[0] * values.shape[0]
+    fused_ms = out['fused']['ms']
+    spread_bytes = 248 * P + 8 * touched + 24 * len(nodes)
+    roof = lambda b, ms: dict(achieved=b / (ms * 1e-3) / 1e9, frac=b / (ms * 1e-3) / HBM_PEAK,
+                              algorithmic_bytes=b)
+    return dict(workload=f'idealised LV, P2 tetrahedra, 64-point rule: {cells.shape[0]} elements, {len(nodes)} '
+                         f'nodes, {P} interaction points in {n}^3 closed unit box',
+                points=P, elements=int(cells.shape[0]), nodes=len(nodes), touched_faces=touched,
+                fused=out['fused'], unfused=out['unfused'],
+                spread_mpts=P / fused_ms['spread_kernel'] / 1e3, gather_mpts=P / fused_ms['gather_kernel'] / 1e3,
+                spread_roofline=roof(spread_bytes, fused_ms['spread_kernel']),
+                gather_roofline=roof(spread_bytes, fused_ms['gather_kernel']))
+
+
 def batched_sample(torch, instances, steps=3, warmup=1):
     """Config 4: `instances` independent instances, grids uniform over
     {32^3, 48^3, 64^3, 96^3, 128x32, 256x64, 512x128}, points log-uniform in
@@ -411,6 +490,8 @@ def main():
         result['serving'] = serving_sample(torch, args.serve_seconds)
     if args.fe_level > 0:
         result['fe'] = fe_sample(torch, n, args.fe_level)
+    if not args.no_p2:
+        result['p2'] = p2_sample(torch)
     if not args.no_cpu_baseline:
         result['cpu_baseline'] = cpu_baseline(x_np, F_np, n, int(round(plain['mg_cycles_mean'])))
     print(json.dumps(result))

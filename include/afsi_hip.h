@@ -531,6 +531,53 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
                       double* x, const afsi_pcg_options* opt, void* workspace, size_t workspace_bytes,
                       afsi_pcg_info* info, void* stream);
 
+/* ---- P2 fused cell-block transfer ----------------------------------------
+ * One quadrature group of 10-node (P2) tetrahedra: every element carries the
+ * same reference rule N (quad, 10) and its own physical weights W. */
+typedef struct afsi_p2_group {
+    const int64_t* cells;      /* (elements, 10) node ids */
+    const double* values;      /* (quad, 10): N_qa */
+    const double* weights;     /* (elements, quad): W_eq (unused by prepare) */
+    int64_t elements;
+    int64_t quad;
+    int64_t node_count;
+} afsi_p2_group;
+/* Shared face/centre Peskin tables of mac/shared_stencil.py:8-12:
+ * base (2, total, 3) int64, phi (2, total, 3, 4); lattice 0 is faces, 1 is
+ * centres; component c reads lattice 0 along axis c and lattice 1 elsewhere.
+ * A group occupies rows offset .. offset + elements*quad. */
+typedef struct afsi_p2_tables {
+    int64_t* base;
+    double* phi;
+    int64_t total;
+    int64_t offset;
+} afsi_p2_tables;
+/* Replaces torch.ops.afsi_ib_cuda.p2_prepare_ (mac/csrc/p2_transfer.cpp:50-59,
+ * 67; kernel p2_transfer.cu:12-44): interaction points x_q = sum_a N_qa
+ * x[cells[e,a]] and their shared tables. ORs 1 into *invalid (device int)
+ * when a point is non-finite or 2 <= (x - origin)/h < n - 2 fails on some
+ * axis (mac/compact_transfer.py:94-96), 2 for a node id out of range.
+ * Closed boxes only (grid->periodic = 0). */
+int afsi_p2_prepare_f64(const double* x, const afsi_p2_group* g, const afsi_grid3* grid, afsi_p2_tables* t,
+                        int* invalid, void* stream);
+/* Replaces torch.ops.afsi_ib_cuda.p2_spread_ (p2_transfer.cpp:66-72,
+ * 69; kernel p2_transfer.cu:97-188 with Spread=true): accumulates
+ * f_c += (phi_x phi_y phi_z) (W_eq sum_a N_qa value[cells[e,a]]_c) / volume
+ * into the face arrays u0, u1, u2 (C-order (nx+1,ny,nz), (nx,ny+1,nz),
+ * (nx,ny,nz+1)), which the caller zeroes first. ORs 1 (2) into *flags and
+ * skips an element whose stencil box leaves the face arrays (whose node ids
+ * are out of range). */
+int afsi_p2_spread_f64(const double* value, const afsi_p2_group* g, const afsi_p2_tables* t,
+                       const afsi_grid3* grid, double volume, double* u0, double* u1, double* u2, int* flags,
+                       void* stream);
+/* Replaces torch.ops.afsi_ib_cuda.p2_gather_assemble_ (p2_transfer.cpp:61-65,
+ * 68; kernel p2_transfer.cu:97-188 with Spread=false): accumulates
+ * rhs[cells[e,a]]_c += sum_q (N_qa W_eq) sum_taps u_c w into rhs
+ * (node_count, 3). Same flags as the spread. */
+int afsi_p2_gather_assemble_f64(const double* u0, const double* u1, const double* u2, const afsi_p2_group* g,
+                                const afsi_p2_tables* t, const afsi_grid3* grid, double* rhs, int* flags,
+                                void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -267,3 +267,48 @@ def p2_solid(nodes, tets, n=4):
     v = nodes[tets]
     vol = np.abs(np.einsum('ei,ei->e', v[:, 1] - v[:, 0], np.cross(v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]))) / 6
     return Geometry(cells.astype(np.int64), p2_basis(pts), 6 * vol[:, None] * w[None], len(nodes2)), nodes2
+
+
+# --- P2 fused cell-block transfer (mac/p2_cuda_transfer.py) ------------------
+def p2_points(g, x):
+    """x_q = sum_a N_qa x[cells[e,a]], accumulated a = 0..9 in order from zero
+    with separate multiply and add (p2_transfer.cu:21-28)."""
+    pos = np.zeros((g.cells.shape[0], g.values.shape[0], 3))
+    for a in range(g.cells.shape[1]):
+        pos = pos + g.values[None, :, a, None] * x[g.cells[:, a]][:, None, :]
+    return pos.reshape(-1, 3)
+
+
+def p2_shared_tables(points, grid):
+    """Shared face/centre tables of the P2 prepare (p2_transfer.cu:12-44,
+    layout mac/shared_stencil.py:8-12): base (2,P,3), phi (2,P,3,4) with
+    lattice 0 starting at the origin (faces) and lattice 1 at origin + h/2
+    (centres); f = s - lower - 1 and the four weights (3-2f-R)/8,
+    (3-2f+R)/8, (1+2f+R)/8, (1+2f-R)/8 with R = sqrt(1+4f-4f^2). Returns
+    also the support predicate 2 <= (x-origin)/h < n-2 per point
+    (mac/compact_transfer.py:87,94-96); an invalid axis is tabulated at s = 0
+    as the reference does."""
+    h = np.asarray(grid.spacing)
+    origin = np.asarray(grid.origin, dtype=np.float64)
+    starts = np.stack([origin, origin + .5 * h])
+    physical = (points - origin) / h
+    valid = np.isfinite(physical) & (physical >= 2) & (physical < np.asarray(grid.shape) - 2)
+    base = np.empty((2, len(points), 3), dtype=np.int64)
+    phi = np.empty((2, len(points), 3, 4))
+    for lattice in range(2):
+        scaled = np.where(valid, (points - starts[lattice]) / h, 0.)
+        lower = np.floor(scaled - 1.)
+        f = scaled - lower - 1.
+        R = np.sqrt(1. + 4. * f - 4. * (f * f))
+        base[lattice] = lower.astype(np.int64)
+        phi[lattice] = np.stack([(3 - 2 * f - R) / 8, (3 - 2 * f + R) / 8,
+                                 (1 + 2 * f + R) / 8, (1 + 2 * f - R) / 8], -1)
+    return base, phi, valid.all(1)
+
+
+def shared_expanded(base, phi):
+    """Per-component compact tables of the shared ones (mac/shared_stencil.py:13-20):
+    component c takes lattice 0 along axis c and lattice 1 elsewhere."""
+    cb = np.stack([np.stack([base[int(a != c), :, a] for a in range(3)], -1) for c in range(3)])
+    cp = np.stack([np.stack([phi[int(a != c), :, a, :] for a in range(3)], 1) for c in range(3)])
+    return cb, cp

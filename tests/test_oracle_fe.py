@@ -127,3 +127,51 @@ def test_p2_rule_and_mass():
     q = lambda X: np.stack([X[:, 0] ** 2, X[:, 1] * X[:, 2], X[:, 0] - X[:, 2] ** 2], 1)
     pts_phys = fe.evaluate(g, nodes2)
     np.testing.assert_allclose(fe.evaluate(g, q(nodes2)), q(pts_phys), rtol=1e-12, atol=1e-14)
+
+
+def test_p2_shared_tables_pinned():
+    """The P2 prepare's shared tables (p2_transfer.cu:12-44): interaction
+    points equal the affine image of the reference points (straight P2
+    tetrahedra), the expanded tables equal the per-component compact stencil
+    of oracle/ib3d.py (base bit-exact, phi to rounding: the shared radical
+    against the two-branch Peskin formula), the weights are the Peskin
+    known answers at f = 0 and f = 1/2, and the support predicate agrees."""
+    from oracle import ib3d
+    nodes, tets = fe.box_tets(2, (-.4, -.3, -.5), (.5, .4, .3))
+    g, nodes2 = fe.p2_solid(nodes, tets)
+    x = nodes2 + .01 * np.sin(3 * nodes2)
+    pts = fe.p2_points(g, x)
+    np.testing.assert_allclose(pts, fe.evaluate(g, x), rtol=0, atol=2e-16)
+    # straight element: x_q = v0 + J xi_q
+    ref, _ = fe.tet_rule(4)
+    v = nodes[tets]
+    affine = v[:, None, 0] + np.einsum('eij,qj->eqi', np.stack([v[:, 1] - v[:, 0], v[:, 2] - v[:, 0],
+                                                                 v[:, 3] - v[:, 0]], -1), ref)
+    np.testing.assert_allclose(fe.p2_points(g, nodes2), affine.reshape(-1, 3), rtol=0, atol=1e-15)
+    grid = ib3d.Grid3((16, 20, 24), (4., 6., 8.), (-2., -3., -4.))
+    base, phi, valid = fe.p2_shared_tables(pts, grid)
+    assert valid.all() and ib3d.support_ok(pts, grid)
+    cb, cp = fe.shared_expanded(base, phi)
+    rb, rp = ib3d.prepare_compact(pts, grid)
+    np.testing.assert_array_equal(cb, rb)
+    np.testing.assert_allclose(cp, rp, rtol=0, atol=2e-15)
+    # planted mistake: swapping the lattices breaks the expansion
+    sb, sp = fe.shared_expanded(base[::-1], phi[::-1])
+    assert not np.array_equal(sb, rb)
+    # known answers (ib.py:57-67): f = 0 -> (1/4, 1/2, 1/4, 0);
+    # f = 1/2 -> ((2-r)/8, (2+r)/8, (2+r)/8, (2-r)/8) with r = sqrt(2)
+    dyadic = ib3d.Grid3((16, 16, 32), (4., 2., 4.), (-2., -1., -2.))
+    h = np.asarray(dyadic.spacing)
+    knots = np.asarray(dyadic.origin) + h * np.array([[5., 6., 7.], [5.5, 6.5, 7.5]])
+    kb, kp, kv = fe.p2_shared_tables(knots, dyadic)
+    np.testing.assert_array_equal(kb[0, 0], [4, 5, 6])
+    np.testing.assert_allclose(kp[0, 0], np.tile([.25, .5, .25, 0.], (3, 1)), atol=1e-16)
+    np.testing.assert_allclose(kp[0, 1], np.tile(np.array([2 - 2 ** .5, 2 + 2 ** .5, 2 + 2 ** .5, 2 - 2 ** .5]) / 8, (3, 1)),
+                               atol=1e-16)
+    np.testing.assert_allclose(kp[1, 1], np.tile([.25, .5, .25, 0.], (3, 1)), atol=1e-16)
+    # support: outside, on the lower limit (valid), on the upper limit (invalid)
+    h = np.asarray(grid.spacing)
+    probe = np.asarray(grid.origin) + h * np.array([[1.99, 5, 5], [2., 5., 5.], [5., 18., 5.], [5., 5., 21.999]])
+    assert fe.p2_shared_tables(probe, grid)[2].tolist() == [False, True, False, True]
+    for p in probe:
+        assert fe.p2_shared_tables(p[None], grid)[2][0] == ib3d.support_ok(p[None], grid)
