@@ -19,8 +19,10 @@ namespace {
 
 constexpr int FE_MAX_NODES = 32;  // nodes per cell
 constexpr int FE_MAX_QUAD = 64;   // quadrature points per cell
-constexpr int RED_BLOCKS = 256;   // partial sums of every PCG reduction
+constexpr int RED_BLOCKS = 256;   // partial sums of the vector reductions
 constexpr int RED_THREADS = 256;
+constexpr int SPMV_LANES = 8;     // lanes per CSR row
+constexpr int SPMV_BLOCKS = 2048; // partial sums of the product's d . Ad
 
 // B (scale = false) or W B (scale = true): one thread per interaction point.
 template <bool SCALE>
@@ -78,21 +80,42 @@ __global__ void fe_assemble_kernel(afsi_fe_geometry g, const double* __restrict_
     }
 }
 
-// y = A x, three columns, one thread per row.
-__global__ void csr_spmv3_kernel(afsi_csr a, const double* __restrict__ x, double* __restrict__ y) {
-    for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < a.rows;
-         r += int64_t(gridDim.x) * blockDim.x) {
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (int32_t k = a.row_ptr[r]; k < a.row_ptr[r + 1]; ++k) {
-            const double v = a.vals[k];
-            const double* xc = x + 3 * int64_t(a.cols[k]);
-            s0 += v * xc[0];
-            s1 += v * xc[1];
-            s2 += v * xc[2];
+// Row r of A x (three columns) on SPMV_LANES consecutive lanes: lane j
+// takes the entries row_ptr[r] + j, + j + SPMV_LANES, ...; the lane sums
+// are combined by a fixed xor butterfly, so every lane of the group holds
+// the same, reproducible total.
+__device__ inline void csr_row3(const afsi_csr& a, const double* __restrict__ x, int64_t r, int sub,
+                                double& s0, double& s1, double& s2) {
+    s0 = s1 = s2 = 0.0;
+    const int32_t end = a.row_ptr[r + 1];
+    for (int32_t k = a.row_ptr[r] + sub; k < end; k += SPMV_LANES) {
+        const double v = a.vals[k];
+        const double* xc = x + 3 * int64_t(a.cols[k]);
+        s0 += v * xc[0];
+        s1 += v * xc[1];
+        s2 += v * xc[2];
+    }
+#pragma unroll
+    for (int m = SPMV_LANES / 2; m > 0; m >>= 1) {
+        s0 += __shfl_xor(s0, m);
+        s1 += __shfl_xor(s1, m);
+        s2 += __shfl_xor(s2, m);
+    }
+}
+
+// y = A x, three columns.
+__global__ void __launch_bounds__(256) csr_spmv3_kernel(afsi_csr a, const double* __restrict__ x,
+                                                        double* __restrict__ y) {
+    const int sub = threadIdx.x % SPMV_LANES;
+    const int64_t groups = int64_t(gridDim.x) * blockDim.x / SPMV_LANES;
+    for (int64_t r = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / SPMV_LANES; r < a.rows; r += groups) {
+        double s0, s1, s2;
+        csr_row3(a, x, r, sub, s0, s1, s2);
+        if (sub == 0) {
+            y[3 * r] = s0;
+            y[3 * r + 1] = s1;
+            y[3 * r + 2] = s2;
         }
-        y[3 * r] = s0;
-        y[3 * r + 1] = s1;
-        y[3 * r + 2] = s2;
     }
 }
 
@@ -124,27 +147,24 @@ __global__ void __launch_bounds__(RED_THREADS) pcg_dot_kernel(int mode, const do
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// Ad = A d (three columns) and partial sums of d . Ad, rows grid-strided
-// over the fixed reduction grid.
+// Ad = A d (three columns) and SPMV_BLOCKS partial sums of d . Ad, row
+// groups strided over the fixed grid.
 __global__ void __launch_bounds__(RED_THREADS) pcg_spmv_dot_kernel(afsi_csr a, const double* __restrict__ d,
                                                                    double* __restrict__ Ad,
                                                                    double* __restrict__ partials) {
     __shared__ double sh[RED_THREADS];
+    const int sub = threadIdx.x % SPMV_LANES;
+    const int64_t groups = int64_t(SPMV_BLOCKS) * RED_THREADS / SPMV_LANES;
     double s = 0.0;
-    for (int64_t r = int64_t(blockIdx.x) * RED_THREADS + threadIdx.x; r < a.rows;
-         r += int64_t(RED_BLOCKS) * RED_THREADS) {
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (int32_t k = a.row_ptr[r]; k < a.row_ptr[r + 1]; ++k) {
-            const double v = a.vals[k];
-            const double* dc = d + 3 * int64_t(a.cols[k]);
-            s0 += v * dc[0];
-            s1 += v * dc[1];
-            s2 += v * dc[2];
+    for (int64_t r = (int64_t(blockIdx.x) * RED_THREADS + threadIdx.x) / SPMV_LANES; r < a.rows; r += groups) {
+        double s0, s1, s2;
+        csr_row3(a, d, r, sub, s0, s1, s2);
+        if (sub == 0) {
+            Ad[3 * r] = s0;
+            Ad[3 * r + 1] = s1;
+            Ad[3 * r + 2] = s2;
+            s += (d[3 * r] * s0 + d[3 * r + 1] * s1) + d[3 * r + 2] * s2;
         }
-        Ad[3 * r] = s0;
-        Ad[3 * r + 1] = s1;
-        Ad[3 * r + 2] = s2;
-        s += (d[3 * r] * s0 + d[3 * r + 1] * s1) + d[3 * r + 2] * s2;
     }
     const double t = block_sum(s, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
@@ -205,7 +225,11 @@ __global__ void __launch_bounds__(RED_THREADS) pcg_scalar_kernel(int what, int r
                                                                  double* __restrict__ st, double atol = 0.0,
                                                                  double rtol = 0.0) {
     __shared__ double sh[RED_THREADS];
-    const double t = block_sum(partials[threadIdx.x], sh);
+    // curvature (what 0) comes from the SPMV_BLOCKS partials of the product
+    double own = 0.0;
+    const int count = what == 0 ? SPMV_BLOCKS : RED_BLOCKS;
+    for (int k = threadIdx.x; k < count; k += RED_THREADS) own += partials[k];
+    const double t = block_sum(own, sh);
     double t2 = 0.0;
     if (what == 5) {
         __syncthreads();
@@ -279,9 +303,11 @@ struct PcgSpace {
     double *r, *d, *Ad, *partials, *state;
 };
 
+constexpr int PARTIALS = SPMV_BLOCKS > 2 * RED_BLOCKS ? SPMV_BLOCKS : 2 * RED_BLOCKS;
+
 size_t pcg_bytes(int64_t rows) {
     const size_t v = (size_t(rows) * 3 * sizeof(double) + 255) & ~size_t(255);
-    return 3 * v + 2 * RED_BLOCKS * sizeof(double) + 256;
+    return 3 * v + PARTIALS * sizeof(double) + 256;
 }
 
 PcgSpace pcg_space(void* ws, int64_t rows) {
@@ -292,7 +318,7 @@ PcgSpace pcg_space(void* ws, int64_t rows) {
     s.d = reinterpret_cast<double*>(p + v);
     s.Ad = reinterpret_cast<double*>(p + 2 * v);
     s.partials = reinterpret_cast<double*>(p + 3 * v);
-    s.state = reinterpret_cast<double*>(p + 3 * v + 2 * RED_BLOCKS * sizeof(double));
+    s.state = reinterpret_cast<double*>(p + 3 * v + PARTIALS * sizeof(double));
     return s;
 }
 
@@ -341,7 +367,7 @@ int afsi_fe_assemble_f64(const afsi_fe_geometry* g, const double* point_values, 
 
 int afsi_csr_spmv3_f64(const afsi_csr* a, const double* x, double* y, void* stream) {
     AFSI_REQUIRE(a && a->row_ptr && a->cols && a->vals && x && y && a->rows > 0, "invalid CSR product arguments");
-    csr_spmv3_kernel<<<grid_for(a->rows, 256, 8192), 256, 0, hipStream_t(stream)>>>(*a, x, y);
+    csr_spmv3_kernel<<<grid_for(a->rows * SPMV_LANES, 256, 8192), 256, 0, hipStream_t(stream)>>>(*a, x, y);
     AFSI_LAUNCH_CHECK();
     return OK;
 }
@@ -361,7 +387,7 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
     const hipStream_t s = hipStream_t(stream);
     const int64_t n = a->rows * 3;
     const PcgSpace w = pcg_space(workspace, a->rows);
-    const unsigned gv = grid_for(a->rows, 256, 8192), ge = grid_for(n, 256, 8192);
+    const unsigned gv = grid_for(a->rows * SPMV_LANES, 256, 8192), ge = grid_for(n, 256, 8192);
     double host[S_COUNT];
     auto read_state = [&]() -> int {
         AFSI_HIP(hipMemcpyAsync(host, w.state, sizeof(host), hipMemcpyDeviceToHost, s));
@@ -414,7 +440,7 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
     // Per iteration: A d with d . Ad, alpha, the update with |r|^2 and r . z,
     // the norm/active/beta scalars, the direction (five launches).
     for (int it = 1; it <= opt->max_iterations; ++it) {
-        pcg_spmv_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(*a, w.d, w.Ad, w.partials);
+        pcg_spmv_dot_kernel<<<SPMV_BLOCKS, RED_THREADS, 0, s>>>(*a, w.d, w.Ad, w.partials);
         AFSI_LAUNCH_CHECK();
         if (int r = scalar(0, 0)) return r;
         pcg_update2_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(x, w.r, w.d, w.Ad, diagonal, n, w.state, w.partials);

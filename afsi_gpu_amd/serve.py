@@ -134,6 +134,34 @@ class Server:
             return Instance3D(shape, x, F, device=self.device, dt=self.dt, options=self.options)
         return Instance2D(shape, x, F, device=self.device, dt=self.dt, options=self.options)
 
+    def snapshot(self, path):
+        """Write every resident instance's state (points, forces, face
+        velocities, pressure) to one checkpoint."""
+        from . import checkpoint
+        torch.cuda.synchronize(self.device)
+        instances = {key: dict(kind='3d' if isinstance(inst, Instance3D) else '2d', shape=inst.grid.shape,
+                               x=inst.x, force=inst.force, velocity=tuple(inst.velocity), pressure=inst.pressure)
+                     for key, inst in self.resident.items()}
+        checkpoint.save_resident(path, instances, dict(dt=self.dt))
+
+    def restore(self, path):
+        """Make the instances of a snapshot resident again, with their state;
+        returns their ids."""
+        from . import checkpoint
+        instances, settings = checkpoint.load_resident(path)
+        if abs(settings['dt'] - self.dt) > 1e-15:
+            raise ValueError('snapshot time step differs from the server\'s')
+        for key, saved in instances.items():
+            inst = self._create((saved['shape'],), (saved['x'].numpy(), saved['force'].numpy()))
+            for target, value in zip(inst.velocity, saved['velocity']):
+                if target.shape != value.shape:
+                    raise ValueError('invalid serving checkpoint index')
+                target.copy_(value)
+            inst.pressure = None if saved['pressure'] is None else saved['pressure'].to(self.device)
+            self.resident[key] = inst
+            self.resident_bytes[key] = footprint(saved['shape'], len(saved['x']))
+        return sorted(instances)
+
     def _execute(self, batch):
         three = [inst for inst in batch if isinstance(inst, Instance3D)]
         rhs = [inst.before_solve() for inst in three]

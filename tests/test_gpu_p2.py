@@ -212,7 +212,8 @@ def test_p2_error_paths_and_buffer_ownership(gpu):
     with pytest.raises(ValueError, match='support'):
         p2.prepare(tr, xt * float('nan'), tr.rule)
     after = p2.assemble_velocity(tr, ones, old)
-    torch.testing.assert_close(after, before, atol=0, rtol=0)
+    # element order of the node accumulation varies between launches
+    torch.testing.assert_close(after, before, atol=2e-14, rtol=3e-13)
     # caller buffers: reused in place, validated
     base, phi = torch.empty_like(old.base), torch.empty_like(old.phi)
     b2, p2_ = p2.prepare(tr, xt, tr.rule, buffers=(base, phi))
