@@ -109,6 +109,10 @@ class P2Group(Structure):
 class P2Tables(Structure):
     _fields_ = [('base', c_void_p), ('phi', c_void_p), ('total', c_int64), ('offset', c_int64)]
 
+
+class HashTable(Structure):
+    _fields_ = [('keys', c_void_p), ('values', c_void_p), ('capacity', c_int64), ('flag', c_void_p)]
+
 _P = c_void_p
 _I64, _F64 = POINTER(c_int64), POINTER(c_double)
 _SIGNATURES = {
@@ -211,6 +215,9 @@ _SIGNATURES = {
                                    _P, _P]),
     'afsi_p2_gather_assemble_f64': (c_int, [_P, _P, _P, POINTER(P2Group), POINTER(P2Tables), POINTER(Grid3), _P,
                                             _P, _P]),
+    'afsi_hash_accumulate_f64': (c_int, [_P, _P, c_int64, POINTER(HashTable), _P]),
+    'afsi_p1_contract_f64': (c_int, [POINTER(FEGeometry), _P, _P, c_int, POINTER(Grid3), POINTER(HashTable), _P]),
+    'afsi_csr_spmv_f64': (c_int, [POINTER(CSR), _P, c_int64, _P, c_int64, c_double, _P]),
 }
 
 

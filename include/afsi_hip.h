@@ -578,6 +578,37 @@ int afsi_p2_gather_assemble_f64(const double* u0, const double* u1, const double
                                 const afsi_p2_tables* t, const afsi_grid3* grid, double* rhs, int* flags,
                                 void* stream);
 
+/* ---- CSR-assembled quadrature operator (P1 meshes) -----------------------
+ * Bounded open-addressing table of (key, value) pairs, keys int64 with -1 =
+ * empty, capacity a power of two; the caller fills keys with -1 and values
+ * and *flag with 0 (mac/hash_transfer_assembly.py:26-50). Buckets are the
+ * SplitMix64 finaliser of the key (hash_transfer_assembly.py:17-23), probed
+ * linearly up to 128 slots; a full probe or a negative key ORs 1 into
+ * *flag, an invalid cell or stencil box ORs 2. */
+typedef struct afsi_hash_table {
+    int64_t* keys;
+    double* values;
+    int64_t capacity;
+    int* flag;
+} afsi_hash_table;
+/* Replaces torch.ops.afsi_ib_cuda.ib_accumulate (mac/csrc/ib.cu:110-115,
+ * 154-166): values[slot(keys[i])] += values[i], zeros skipped. */
+int afsi_hash_accumulate_f64(const int64_t* keys, const double* values, int64_t count, const afsi_hash_table* t,
+                             void* stream);
+/* Replaces torch.ops.afsi_ib_cuda.ib_contract (mac/csrc/ib.cu:56-108,
+ * 118-152) for one component: adds B_c[node, face] = sum over the cells'
+ * interaction points of W_eq N_qa (phi_x phi_y phi_z)(face - x_eq) under the
+ * key node * faces_c + face. base (3,P,3) int64 and phi (3,P,3,4) are the
+ * compact tables of the cell-major interaction points (P = elements * quad,
+ * afsi_prepare_compact_f64); cells of at most 4 nodes; closed boxes only. */
+int afsi_p1_contract_f64(const afsi_fe_geometry* g, const int64_t* base, const double* phi, int component,
+                         const afsi_grid3* grid, const afsi_hash_table* t, void* stream);
+/* y[r * y_stride] = scale * sum_k A[r, cols[k]] x[cols[k] * x_stride]: the
+ * CSR products of the assembled spread (B^T c / V) and gather (B u) of
+ * mac/assembled_transfer.py:199-211 (torch.sparse.mm). */
+int afsi_csr_spmv_f64(const afsi_csr* a, const double* x, int64_t x_stride, double* y, int64_t y_stride,
+                      double scale, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

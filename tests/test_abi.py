@@ -60,7 +60,7 @@ def test_struct_layouts_match_the_header(tmp_path):
                'afsi_mgb_batch': _abi.MGBBatch, 'afsi_tb_item': _abi.TBItem,
                'afsi_tb_batch': _abi.TBBatch, 'afsi_fe_geometry': _abi.FEGeometry, 'afsi_csr': _abi.CSR,
                'afsi_pcg_options': _abi.PCGOptions, 'afsi_pcg_info': _abi.PCGInfo,
-               'afsi_p2_group': _abi.P2Group, 'afsi_p2_tables': _abi.P2Tables}
+               'afsi_p2_group': _abi.P2Group, 'afsi_p2_tables': _abi.P2Tables, 'afsi_hash_table': _abi.HashTable}
     lines = ['#include <cstdio>', '#include <cstddef>', '#include "afsi_hip.h"', 'int main() {']
     for cname, py in mirrors.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')

@@ -175,3 +175,38 @@ def test_p2_shared_tables_pinned():
     assert fe.p2_shared_tables(probe, grid)[2].tolist() == [False, True, False, True]
     for p in probe:
         assert fe.p2_shared_tables(p[None], grid)[2][0] == ib3d.support_ok(p[None], grid)
+
+
+def test_assembled_operator_oracle_pinned():
+    """The table restatement: the SplitMix64 finaliser equals the published
+    generator's first output for seed 0 (0xe220a8397b1dcdaf, the mix of the
+    golden-ratio increment), forty colliders of one bucket repeated seven
+    times sum to 0.875 exactly and a full table flags overflow
+    (tests/test_csr_hash_assembly.py:22-40). The dense B_c reproduces the
+    pinned spread and gather of the oracle: B^T (W-weighted coefficient) / V
+    and B u."""
+    from oracle import assembled, ib3d
+    assert assembled.splitmix_finalise(0x9e3779b97f4a7c15) == 0xe220a8397b1dcdaf
+    colliders = [k for k in range(10000) if assembled.bucket(k, 128) == 3][:40]
+    assert len(colliders) == 40
+    keys = np.array(colliders * 7 + [987654])
+    values = np.full(len(keys), .125)
+    values[-1] = 0
+    tk, tv, flag = assembled.table_accumulate(keys, values, 128)
+    k, v = assembled.sorted_entries(tk, tv)
+    assert flag == 0 and k.tolist() == sorted(colliders) and (v == .875).all()
+    assert assembled.table_accumulate(np.arange(20), np.ones(20), 8)[2] == 1
+    g, X = solid()
+    grid = ib3d.Grid3((16, 20, 24), (4., 6., 8.), (-2., -3., -4.))
+    x = X + .01 * np.sin(3 * X)
+    pts = fe.evaluate(g, x)
+    C = np.cos(2 * X)
+    base, phi = ib3d.prepare_compact(pts, grid)
+    spread = ib3d.spread_grid(fe.weighted(g, C), base, phi, grid)
+    u = [np.sin(grid.coordinates(c).sum(-1) + c) for c in range(3)]
+    rhs = fe.assemble(g, ib3d.gather_grid(u, base, phi, grid))
+    for c in range(3):
+        B = assembled.dense_operator(g, pts, grid, c)
+        np.testing.assert_allclose((B.T @ C[:, c]).reshape(spread[c].shape) / grid.volume, spread[c],
+                                   rtol=0, atol=1e-13 * np.abs(spread[c]).max())
+        np.testing.assert_allclose(B @ u[c].reshape(-1), rhs[:, c], rtol=0, atol=1e-13 * np.abs(rhs).max())
