@@ -32,7 +32,9 @@ constexpr int P2_NODES = 10;
 constexpr int P2_THREADS = 256;
 constexpr int P2_WAVES = P2_THREADS / WAVE;
 constexpr int P2_CHUNK = 64;     // points staged per pass
-constexpr int P2_PATCH = 3072;   // LDS patch capacity (values, three components)
+constexpr int P2_PATCH = 1536;   // LDS patch capacity (values, three components)
+constexpr int P2_BLOCK = 4;      // consecutive elements per spread workgroup
+constexpr int P2_SPREAD_PATCH = 2304;
 
 struct Lattice {
     double origin[2][3];  // [0] faces: origin; [1] centres: origin + h/2
@@ -127,17 +129,21 @@ constexpr int BASE_LIMIT = 1 << 28;
 // three component boxes. status = 0 when every box lies inside its face
 // array, every base is representable and every node id of the element is in
 // range. Uniform across the workgroup on return.
+// With ``staged`` (every point of the element already in ``st``, bases
+// range-checked by stage_chunk into ``bad``) the box is taken from LDS.
 __device__ void plan_patch(Patch& s, int (*red)[12], const int64_t* __restrict__ base, int64_t first, int nq,
-                           int64_t total, const int64_t n[3], const int64_t* __restrict__ cell, int64_t nodes) {
+                           int64_t total, const int64_t n[3], const int64_t* __restrict__ cell, int ids,
+                           int64_t nodes, const Stage* staged, bool ok) {
     const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     int lo[6], hi[6];
-    bool ok = true;
 #pragma unroll
     for (int k = 0; k < 6; ++k) { lo[k] = BASE_LIMIT; hi[k] = -BASE_LIMIT; }
     for (int q = tid; q < nq; q += P2_THREADS) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            const int64_t v = base[((k / 3) * total + first + q) * 3 + k % 3];
+            int64_t v;
+            if (staged) v = staged->base[k / 3][q * 3 + k % 3];
+            else v = base[((k / 3) * total + first + q) * 3 + k % 3];
             ok = ok && v > -BASE_LIMIT && v < BASE_LIMIT;
             lo[k] = min(lo[k], int(v));
             hi[k] = max(hi[k], int(v));
@@ -149,7 +155,7 @@ __device__ void plan_patch(Patch& s, int (*red)[12], const int64_t* __restrict__
         hi[k] = wave_max(hi[k]);
     }
     int bad = ok ? 0 : F_SUPPORT;
-    if (tid < P2_NODES && (cell[tid] < 0 || cell[tid] >= nodes)) bad |= F_INDEX;
+    if (tid < ids && (cell[tid] < 0 || cell[tid] >= nodes)) bad |= F_INDEX;
 #pragma unroll
     for (int m = 32; m > 0; m >>= 1) bad |= __shfl_xor(bad, m);
     if (lane == 0) {
@@ -197,102 +203,174 @@ __device__ inline int64_t face_index(int c, int x, int y, int z, const int64_t n
     return (int64_t(x) * (n[1] + int(c == 1)) + y) * (n[2] + int(c == 2)) + z;
 }
 
-__device__ inline int patch_index(const Patch& s, int c, int x, int y, int z) {
-    return s.off[c] + ((x - s.start[c][0]) * s.ext[c][1] + (y - s.start[c][1])) * s.ext[c][2] + (z - s.start[c][2]);
+// The patch geometry in scalar registers (every field is workgroup-uniform),
+// with float reciprocals of the extents for the entry decode.
+struct PatchRegs {
+    int off[4];
+    int start[3][3];
+    int ext[3][3];
+    float inv[3][3];
+};
+
+__device__ inline PatchRegs patch_regs(const Patch& s) {
+    PatchRegs r;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) r.off[c] = __builtin_amdgcn_readfirstlane(s.off[c]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            r.start[c][a] = __builtin_amdgcn_readfirstlane(s.start[c][a]);
+            r.ext[c][a] = __builtin_amdgcn_readfirstlane(s.ext[c][a]);
+            r.inv[c][a] = 1.0f / float(r.ext[c][a]);
+        }
+    return r;
 }
 
-// Patch entry i -> component and grid coordinates.
-__device__ inline int patch_decode(const Patch& s, int i, int& x, int& y, int& z) {
+// i / d for 0 <= i < 2^20 and 1 <= d: the float quotient is within one of
+// the exact one, then corrected.
+__device__ inline int small_div(int i, int d, float inv) {
+    int q = int(float(i) * inv);
+    if (q * d > i) --q;
+    else if ((q + 1) * d <= i) ++q;
+    return q;
+}
+
+template <class T>
+__device__ inline T pick(int c, T a, T b, T d) {
+    return c == 0 ? a : (c == 1 ? b : d);
+}
+
+// Patch entry i (< P2_PATCH) -> component and grid coordinates (selects, no
+// dynamic register indexing).
+__device__ inline int patch_decode(const PatchRegs& s, int i, int& x, int& y, int& z) {
     const int c = i >= s.off[2] ? 2 : (i >= s.off[1] ? 1 : 0);
-    int p = i - s.off[c];
-    z = p % s.ext[c][2] + s.start[c][2];
-    p /= s.ext[c][2];
-    y = p % s.ext[c][1] + s.start[c][1];
-    x = p / s.ext[c][1] + s.start[c][0];
+    const int p = i - pick(c, s.off[0], s.off[1], s.off[2]);
+    const int e2 = pick(c, s.ext[0][2], s.ext[1][2], s.ext[2][2]);
+    const int e1 = pick(c, s.ext[0][1], s.ext[1][1], s.ext[2][1]);
+    const int r = small_div(p, e2, pick(c, s.inv[0][2], s.inv[1][2], s.inv[2][2]));
+    z = p - r * e2 + pick(c, s.start[0][2], s.start[1][2], s.start[2][2]);
+    const int xr = small_div(r, e1, pick(c, s.inv[0][1], s.inv[1][1], s.inv[2][1]));
+    y = r - xr * e1 + pick(c, s.start[0][1], s.start[1][1], s.start[2][1]);
+    x = xr + pick(c, s.start[0][0], s.start[1][0], s.start[2][0]);
     return c;
 }
 
-// Coalesced copy of one chunk's tables into LDS.
-__device__ inline void stage_chunk(Stage& st, const int64_t* __restrict__ base, const double* __restrict__ phi,
+// Coalesced copy of one chunk's tables into LDS; returns false when a base
+// of this thread's share is out of the representable range.
+__device__ inline bool stage_chunk(Stage& st, const int64_t* __restrict__ base, const double* __restrict__ phi,
                                    int64_t first, int q0, int cnt, int64_t total) {
     const int tid = threadIdx.x;
+    bool ok = true;
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         const int64_t row = l * total + first + q0;
         const double* src = phi + row * 12;
         for (int i = tid; i < cnt * 12; i += P2_THREADS) st.phi[l][i] = src[i];
         const int64_t* bsrc = base + row * 3;
-        for (int i = tid; i < cnt * 3; i += P2_THREADS) st.base[l][i] = int(bsrc[i]);
+        for (int i = tid; i < cnt * 3; i += P2_THREADS) {
+            const int64_t v = bsrc[i];
+            ok = ok && v > -BASE_LIMIT && v < BASE_LIMIT;
+            st.base[l][i] = int(v);
+        }
     }
+    return ok;
 }
 
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(P2_THREADS) p2_spread_kernel(
     const double* __restrict__ value, const int64_t* __restrict__ cells, const double* __restrict__ N,
-    const double* __restrict__ W, int nq, const int64_t* __restrict__ base, const double* __restrict__ phi,
-    int64_t total, int64_t offset, double* __restrict__ u0, double* __restrict__ u1, double* __restrict__ u2,
-    int64_t nodes, Lattice L, double volume, int* __restrict__ flags) {
+    const double* __restrict__ W, int64_t ne, int nq, const int64_t* __restrict__ base,
+    const double* __restrict__ phi, int64_t total, int64_t offset, double* __restrict__ u0,
+    double* __restrict__ u1, double* __restrict__ u2, int64_t nodes, Lattice L, double volume,
+    int* __restrict__ flags) {
     __shared__ Patch s;
     __shared__ int red[P2_WAVES + 1][12];
-    __shared__ double nodal[P2_NODES * 3];
+    __shared__ double nodal[P2_BLOCK][P2_NODES * 3];
     __shared__ double force[P2_CHUNK * 3];
     __shared__ Stage st;
-    __shared__ double patch[P2_PATCH];
+    __shared__ double patch[P2_SPREAD_PATCH];
 
     const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
-    const int64_t e = blockIdx.x, first = offset + e * nq;
-    plan_patch(s, red, base, first, nq, total, L.n, cells + e * P2_NODES, nodes);
+    // this workgroup's elements e0 .. e0 + count: consecutive table rows
+    const int64_t e0 = int64_t(blockIdx.x) * P2_BLOCK;
+    const int count = int(min(int64_t(P2_BLOCK), ne - e0));
+    const int points = count * nq;
+    const int64_t first = offset + e0 * nq;
+    const bool single = points <= P2_CHUNK;  // one chunk: stage first, plan from LDS
+    bool ok = true;
+    if (single) {
+        ok = stage_chunk(st, base, phi, first, 0, points, total);
+        __syncthreads();
+    }
+    plan_patch(s, red, base, first, points, total, L.n, cells + e0 * P2_NODES, count * P2_NODES, nodes,
+               single ? &st : nullptr, ok);
     if (s.status) {
         if (tid == 0) raise_flag(flags, s.status);
         return;
     }
-    const bool cached = s.off[3] <= P2_PATCH;
-    if (tid < P2_NODES * 3) nodal[tid] = value[cells[e * P2_NODES + tid / 3] * 3 + tid % 3];
+    const bool cached = s.off[3] <= P2_SPREAD_PATCH;
+    if (tid < count * P2_NODES * 3) {
+        const int k = tid / (P2_NODES * 3), r = tid % (P2_NODES * 3);
+        nodal[k][r] = value[cells[(e0 + k) * P2_NODES + r / 3] * 3 + r % 3];
+    }
     if (cached)
         for (int i = tid; i < s.off[3]; i += P2_THREADS) patch[i] = 0.0;
     double* const fields[3] = {u0, u1, u2};
     const int ti = lane >> 4, tj = (lane >> 2) & 3, tk = lane & 3;
+    const PatchRegs R = patch_regs(s);
+    const int uwave = __builtin_amdgcn_readfirstlane(wave);
+    int tapoff[3];  // this lane's tap inside each component box
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tapoff[c] = (ti * R.ext[c][1] + tj) * R.ext[c][2] + tk;
 
-    for (int q0 = 0; q0 < nq; q0 += P2_CHUNK) {
-        const int cnt = min(P2_CHUNK, nq - q0);
+    for (int g0 = 0; g0 < points; g0 += P2_CHUNK) {
+        const int cnt = min(P2_CHUNK, points - g0);
         __syncthreads();  // previous chunk consumed; nodal and patch zero visible
-        stage_chunk(st, base, phi, first, q0, cnt, total);
+        if (!single) stage_chunk(st, base, phi, first, g0, cnt, total);
         if (tid < cnt * 3) {
-            const int q = tid / 3, c = tid % 3;
-            const double* Nq = N + int64_t(q0 + q) * P2_NODES;
+            const int g = g0 + tid / 3, c = tid % 3;
+            const int k = g / nq, q = g - k * nq;
+            const double* Nq = N + int64_t(q) * P2_NODES;
             double f = 0.0;
 #pragma unroll
-            for (int a = 0; a < P2_NODES; ++a) f += Nq[a] * nodal[a * 3 + c];
-            force[tid] = (f * W[e * nq + q0 + q]) / volume;
+            for (int a = 0; a < P2_NODES; ++a) f += Nq[a] * nodal[k][a * 3 + c];
+            force[tid] = (f * W[(e0 + k) * nq + q]) / volume;
         }
         __syncthreads();
-        for (int q = wave; q < cnt; q += P2_WAVES) {
+        for (int q = uwave; q < cnt; q += P2_WAVES) {
             const double* P0 = st.phi[0] + q * 12;
             const double* P1 = st.phi[1] + q * 12;
             const double fx0 = P0[ti], fx1 = P1[ti];
             const double fy0 = P0[4 + tj], fy1 = P1[4 + tj];
             const double fz0 = P0[8 + tk], fz1 = P1[8 + tk];
-            const int* B0 = st.base[0] + q * 3;
-            const int* B1 = st.base[1] + q * 3;
+            int b[2][3];  // the point's bases, wave-uniform
+#pragma unroll
+            for (int l = 0; l < 2; ++l)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) b[l][a] = __builtin_amdgcn_readfirstlane(st.base[l][q * 3 + a]);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const double w = c == 0 ? (fx0 * fy1) * fz1 : (c == 1 ? (fx1 * fy0) * fz1 : (fx1 * fy1) * fz0);
                 const double v = w * force[q * 3 + c];
-                const int x = (c == 0 ? B0[0] : B1[0]) + ti;
-                const int y = (c == 1 ? B0[1] : B1[1]) + tj;
-                const int z = (c == 2 ? B0[2] : B1[2]) + tk;
-                if (cached) atomicAdd(patch + patch_index(s, c, x, y, z), v);
-                else unsafeAtomicAdd(fields[c] + face_index(c, x, y, z, L.n), v);
+                const int bx = b[int(c != 0)][0], by = b[int(c != 1)][1], bz = b[int(c != 2)][2];
+                if (cached) {
+                    const int point = R.off[c] + ((bx - R.start[c][0]) * R.ext[c][1] + (by - R.start[c][1])) *
+                                      R.ext[c][2] + (bz - R.start[c][2]);
+                    atomicAdd(patch + point + tapoff[c], v);
+                } else {
+                    unsafeAtomicAdd(fields[c] + face_index(c, bx + ti, by + tj, bz + tk, L.n), v);
+                }
             }
         }
     }
     if (!cached) return;
     __syncthreads();
-    for (int i = tid; i < s.off[3]; i += P2_THREADS) {
+    for (int i = tid; i < R.off[3]; i += P2_THREADS) {
         const double v = patch[i];
         if (v == 0.0) continue;
         int x, y, z;
-        const int c = patch_decode(s, i, x, y, z);
+        const int c = patch_decode(R, i, x, y, z);
         unsafeAtomicAdd(fields[c] + face_index(c, x, y, z, L.n), v);
     }
 }
@@ -307,22 +385,31 @@ __global__ void __launch_bounds__(P2_THREADS) p2_gather_assemble_kernel(
     __shared__ int red[P2_WAVES + 1][12];
     __shared__ double part[4][P2_CHUNK * 3];  // [i tap][point * 3 + c]
     __shared__ double sums[30][8];
+    __shared__ double nw[P2_CHUNK * P2_NODES];  // N_qa W_eq of the chunk's points
     __shared__ Stage st;
     __shared__ double patch[P2_PATCH];
 
     const int tid = threadIdx.x;
     const int64_t e = blockIdx.x, first = offset + e * nq;
-    plan_patch(s, red, base, first, nq, total, L.n, cells + e * P2_NODES, nodes);
+    const bool single = nq <= P2_CHUNK;  // one chunk: stage first, plan from LDS
+    bool ok = true;
+    if (single) {
+        ok = stage_chunk(st, base, phi, first, 0, nq, total);
+        __syncthreads();
+    }
+    plan_patch(s, red, base, first, nq, total, L.n, cells + e * P2_NODES, P2_NODES, nodes, single ? &st : nullptr,
+               ok);
     if (s.status) {
         if (tid == 0) raise_flag(flags, s.status);
         return;
     }
     const bool cached = s.off[3] <= P2_PATCH;
     const double* const fields[3] = {u0, u1, u2};
+    const PatchRegs R = patch_regs(s);
     if (cached)
-        for (int i = tid; i < s.off[3]; i += P2_THREADS) {
+        for (int i = tid; i < R.off[3]; i += P2_THREADS) {
             int x, y, z;
-            const int c = patch_decode(s, i, x, y, z);
+            const int c = patch_decode(R, i, x, y, z);
             patch[i] = fields[c][face_index(c, x, y, z, L.n)];
         }
     // thread -> (point in chunk, tap slice i)
@@ -334,7 +421,9 @@ __global__ void __launch_bounds__(P2_THREADS) p2_gather_assemble_kernel(
     for (int q0 = 0; q0 < nq; q0 += P2_CHUNK) {
         const int cnt = min(P2_CHUNK, nq - q0);
         __syncthreads();
-        stage_chunk(st, base, phi, first, q0, cnt, total);
+        if (!single) stage_chunk(st, base, phi, first, q0, cnt, total);
+        for (int i = tid; i < cnt * P2_NODES; i += P2_THREADS)
+            nw[i] = N[int64_t(q0) * P2_NODES + i] * W[e * nq + q0 + i / P2_NODES];
         __syncthreads();
         if (pq < cnt) {
             const double* P0 = st.phi[0] + pq * 12;
@@ -351,14 +440,25 @@ __global__ void __launch_bounds__(P2_THREADS) p2_gather_assemble_kernel(
                 const int y0 = c == 1 ? B0[1] : B1[1];
                 const int z0 = c == 2 ? B0[2] : B1[2];
                 double got = 0.0;
+                if (cached) {
+                    const int ez = R.ext[c][2];
+                    const double* row = patch + R.off[c] +
+                                        ((x - R.start[c][0]) * R.ext[c][1] + (y0 - R.start[c][1])) * ez +
+                                        (z0 - R.start[c][2]);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const double wxy = wx * Py[j];
+                    for (int j = 0; j < 4; ++j) {
+                        const double wxy = wx * Py[j];
+                        const double* r = row + j * ez;
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        const double v = cached ? patch[patch_index(s, c, x, y0 + j, z0 + k)]
-                                                : fields[c][face_index(c, x, y0 + j, z0 + k, L.n)];
-                        got += v * (wxy * Pz[k]);
+                        for (int k = 0; k < 4; ++k) got += r[k] * (wxy * Pz[k]);
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const double wxy = wx * Py[j];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            got += fields[c][face_index(c, x, y0 + j, z0 + k, L.n)] * (wxy * Pz[k]);
                     }
                 }
                 part[ti][pq * 3 + c] = got;
@@ -369,7 +469,7 @@ __global__ void __launch_bounds__(P2_THREADS) p2_gather_assemble_kernel(
             for (int q = ar; q < cnt; q += 8) {
                 const double got = ((part[0][q * 3 + ac] + part[1][q * 3 + ac]) + part[2][q * 3 + ac]) +
                                    part[3][q * 3 + ac];
-                acc += (N[int64_t(q0 + q) * P2_NODES + aa] * W[e * nq + q0 + q]) * got;
+                acc += nw[q * P2_NODES + aa] * got;
             }
         }
     }
@@ -442,8 +542,8 @@ int afsi_p2_spread_f64(const double* value, const afsi_p2_group* g, const afsi_p
     AFSI_REQUIRE(value && u0 && u1 && u2 && g->weights, "null P2 spread arrays");
     AFSI_REQUIRE(std::isfinite(volume) && volume > 0, "invalid P2 MAC grid");
     if (!g->elements) return OK;
-    p2_spread_kernel<<<unsigned(g->elements), P2_THREADS, 0, hipStream_t(stream)>>>(
-        value, g->cells, g->values, g->weights, int(g->quad), t->base, t->phi, t->total, t->offset, u0, u1, u2,
+    p2_spread_kernel<<<unsigned(cdiv(g->elements, P2_BLOCK)), P2_THREADS, 0, hipStream_t(stream)>>>(
+        value, g->cells, g->values, g->weights, g->elements, int(g->quad), t->base, t->phi, t->total, t->offset, u0, u1, u2,
         g->node_count, L, volume, flags);
     AFSI_LAUNCH_CHECK();
     return OK;
