@@ -72,6 +72,8 @@ def parse():
     ap.add_argument('--fe-level', type=int, default=7,
                     help='sphere subdivision level of the FE interaction-point sample (0: skip)')
     ap.add_argument('--no-p2', action='store_true', help='skip the P2 fused-transfer sample')
+    ap.add_argument('--only', choices=('p2', 'fe', 'batched', 'serving'),
+                    help='run one secondary sample alone and print its JSON (for profiling)')
     return ap.parse_args()
 
 
@@ -430,6 +432,12 @@ def main():
     import torch
     if args.kernels_only:
         kernels_only(torch, args.n, args.kernels_only)
+        return
+    if args.only:
+        run = dict(p2=lambda: p2_sample(torch), fe=lambda: fe_sample(torch, args.n, args.fe_level),
+                   batched=lambda: batched_sample(torch, args.batch_instances),
+                   serving=lambda: serving_sample(torch, args.serve_seconds))[args.only]
+        print(json.dumps({args.only: run()}))
         return
     from afsi_gpu_amd import workloads
     from afsi_gpu_amd.mac import MACGrid, MGOptions
