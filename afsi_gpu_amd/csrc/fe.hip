@@ -53,30 +53,42 @@ __global__ void fe_evaluate_kernel(afsi_fe_geometry g, const double* __restrict_
 }
 
 // B^T W: node n sums, over its incidences (e, a) in node_items order,
-// sum_q N_qa W_eq U_(e,q).
-__global__ void fe_assemble_kernel(afsi_fe_geometry g, const double* __restrict__ point_values,
-                                   double* __restrict__ out) {
-    for (int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; n < g.node_count;
-         n += int64_t(gridDim.x) * blockDim.x) {
+// sum_q N_qa W_eq U_(e,q). One wave per node: lane l takes quadrature point
+// q = l % quad of the incidences slot, slot + 64/quad, ... (slot = l / quad),
+// accumulating in that fixed order; the 64 lane sums are then combined by a
+// fixed xor butterfly, so the result is reproducible.
+__global__ void __launch_bounds__(256) fe_assemble_kernel(afsi_fe_geometry g, const double* __restrict__ point_values,
+                                                          double* __restrict__ out) {
+    const int lane = threadIdx.x % WAVE;
+    const int per = WAVE / g.quad;  // incidences per pass (quad <= 64)
+    const int q = lane % g.quad, slot = lane / g.quad;
+    const bool active = slot < per;
+    const int64_t waves = int64_t(gridDim.x) * (blockDim.x / WAVE);
+    for (int64_t n = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / WAVE; n < g.node_count; n += waves) {
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (int32_t k = g.node_ptr[n]; k < g.node_ptr[n + 1]; ++k) {
-            const int64_t item = g.node_items[k];
-            const int64_t e = item / g.nodes_per_cell;
-            const int a = int(item - e * g.nodes_per_cell);
-            const double* W = g.weights + e * g.quad;
-            const double* U = point_values + 3 * e * g.quad;
-            double l0 = 0.0, l1 = 0.0, l2 = 0.0;
-            for (int q = 0; q < g.quad; ++q) {
-                const double c = g.values[int64_t(q) * g.nodes_per_cell + a] * W[q];
-                l0 += c * U[3 * q];
-                l1 += c * U[3 * q + 1];
-                l2 += c * U[3 * q + 2];
+        if (active) {
+            for (int32_t k = g.node_ptr[n] + slot; k < g.node_ptr[n + 1]; k += per) {
+                const int64_t item = g.node_items[k];
+                const int64_t e = item / g.nodes_per_cell;
+                const int a = int(item - e * g.nodes_per_cell);
+                const double c = g.values[int64_t(q) * g.nodes_per_cell + a] * g.weights[e * g.quad + q];
+                const double* U = point_values + 3 * (e * g.quad + q);
+                s0 += c * U[0];
+                s1 += c * U[1];
+                s2 += c * U[2];
             }
-            s0 += l0; s1 += l1; s2 += l2;
         }
-        out[3 * n] = s0;
-        out[3 * n + 1] = s1;
-        out[3 * n + 2] = s2;
+#pragma unroll
+        for (int m = WAVE / 2; m > 0; m >>= 1) {
+            s0 += __shfl_xor(s0, m);
+            s1 += __shfl_xor(s1, m);
+            s2 += __shfl_xor(s2, m);
+        }
+        if (lane == 0) {
+            out[3 * n] = s0;
+            out[3 * n + 1] = s1;
+            out[3 * n + 2] = s2;
+        }
     }
 }
 
@@ -360,7 +372,8 @@ int afsi_fe_weighted_f64(const afsi_fe_geometry* g, const double* nodal, double*
 int afsi_fe_assemble_f64(const afsi_fe_geometry* g, const double* point_values, double* out, void* stream) {
     if (int r = check_geometry(g)) return r;
     AFSI_REQUIRE(g->node_ptr && g->node_items && point_values && out, "null pointer");
-    fe_assemble_kernel<<<grid_for(g->node_count, 256, 8192), 256, 0, hipStream_t(stream)>>>(*g, point_values, out);
+    fe_assemble_kernel<<<grid_for(g->node_count * WAVE, 256, 16384), 256, 0, hipStream_t(stream)>>>(*g, point_values,
+                                                                                                  out);
     AFSI_LAUNCH_CHECK();
     return OK;
 }
