@@ -334,6 +334,29 @@ PcgSpace pcg_space(void* ws, int64_t rows) {
     return s;
 }
 
+// Pinned host slots for look-ahead state readbacks, one set per host thread
+// and device (created on first use, kept for the life of the process).
+struct Readback {
+    int device = -1;
+    double* host = nullptr;  // 2 slots of S_COUNT doubles
+    hipEvent_t done[2] = {nullptr, nullptr};
+};
+thread_local Readback readback;
+
+int ensure_readback() {
+    int device = 0;
+    AFSI_HIP(hipGetDevice(&device));
+    if (readback.host && readback.device == device) return OK;
+    if (!readback.host) AFSI_HIP(hipHostMalloc(reinterpret_cast<void**>(&readback.host), 2 * S_COUNT * sizeof(double),
+                                               hipHostMallocDefault));
+    for (int k = 0; k < 2; ++k) {
+        if (readback.done[k]) AFSI_HIP(hipEventDestroy(readback.done[k]));
+        AFSI_HIP(hipEventCreateWithFlags(&readback.done[k], hipEventDisableTiming));
+    }
+    readback.device = device;
+    return OK;
+}
+
 int check_geometry(const afsi_fe_geometry* g) {
     AFSI_REQUIRE(g && g->cells && g->values && g->weights, "null FE geometry");
     AFSI_REQUIRE(g->elements >= 0 && g->node_count > 0, "invalid FE geometry sizes");
@@ -452,6 +475,22 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
     AFSI_LAUNCH_CHECK();
     // Per iteration: A d with d . Ad, alpha, the update with |r|^2 and r . z,
     // the norm/active/beta scalars, the direction (five launches).
+    //
+    // Check iterations read the device state one check period late: the copy
+    // goes to a pinned slot and the host looks at it when it reaches the next
+    // check, with the iterations in between already queued. A recurrence that
+    // has converged (rn <= tol) or broken down is frozen on the device
+    // (active = 0 gives alpha = 0 and d = 0), so those queued iterations leave
+    // x and r unchanged and the host resumes at the check that decided.
+    // Restart iterations and the last iteration are read synchronously.
+    if (int r = ensure_readback()) return r;
+    int pending = 0, pslot = 0, slot = 0;
+    auto broken_at = [&](int at) {
+        info->status = 1;
+        info->iterations = at;
+        info->residual_norm = host[S_RN];
+        return OK;
+    };
     for (int it = 1; it <= opt->max_iterations; ++it) {
         pcg_spmv_dot_kernel<<<SPMV_BLOCKS, RED_THREADS, 0, s>>>(*a, w.d, w.Ad, w.partials);
         AFSI_LAUNCH_CHECK();
@@ -461,32 +500,52 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
         if (int r = scalar(5, 0)) return r;
         bool restart = it % opt->recompute_every == 0;
         const bool check = restart || it % opt->check_every == 0 || it == opt->max_iterations;
-        bool converged = false;
         if (check) {
-            if (int r = read_state()) return r;
-            if (host[S_BROKEN] != 0.0) {
-                info->status = 1;
-                info->iterations = it;
-                info->residual_norm = host[S_RN];
-                return OK;
+            bool converged = false;
+            int at = it;
+            if (pending) {  // the previous check decides first
+                AFSI_HIP(hipEventSynchronize(readback.done[pslot]));
+                for (int k = 0; k < S_COUNT; ++k) host[k] = readback.host[pslot * S_COUNT + k];
+                if (host[S_BROKEN] != 0.0) return broken_at(pending);
+                if (host[S_RN] <= tol) {
+                    converged = true;
+                    at = pending;
+                }
+                pending = 0;
             }
-            converged = host[S_RN] <= tol;
-        }
-        if (restart || converged || it == opt->max_iterations) {
-            if (int r = true_residual()) return r;
-            if (int r = read_state()) return r;
-            true_norm = host[S_TRUE];
-            info->iterations = it;
-            info->residual_norm = true_norm;
-            if (!std::isfinite(true_norm)) {
-                info->status = 3;
-                return OK;
+            if (!converged) {
+                if (restart || it == opt->max_iterations) {
+                    if (int r = read_state()) return r;
+                    if (host[S_BROKEN] != 0.0) return broken_at(it);
+                    converged = host[S_RN] <= tol;
+                } else {
+                    AFSI_HIP(hipMemcpyAsync(readback.host + slot * S_COUNT, w.state, S_COUNT * sizeof(double),
+                                            hipMemcpyDeviceToHost, s));
+                    AFSI_HIP(hipEventRecord(readback.done[slot], s));
+                    pending = it;
+                    pslot = slot;
+                    slot ^= 1;
+                }
             }
-            if (true_norm <= tol) {
-                info->status = 0;
-                return OK;
+            if (converged || restart || it == opt->max_iterations) {
+                // true residual of the iterate of iteration `at` (frozen since)
+                if (int r = true_residual()) return r;
+                if (int r = read_state()) return r;
+                true_norm = host[S_TRUE];
+                info->iterations = at;
+                info->residual_norm = true_norm;
+                if (!std::isfinite(true_norm)) {
+                    info->status = 3;
+                    return OK;
+                }
+                if (true_norm <= tol) {
+                    info->status = 0;
+                    return OK;
+                }
+                it = at;  // queued iterations after `at` were no-ops
+                pending = 0;
+                restart = true;
             }
-            restart = true;
         }
         if (restart) {  // r.z of the (true) residual, active = 1
             pcg_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(1, w.r, w.r, diagonal, n, w.partials);
