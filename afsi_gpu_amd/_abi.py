@@ -210,11 +210,12 @@ _SIGNATURES = {
     'afsi_mass_pcg_workspace_bytes': (c_size_t, [c_int64]),
     'afsi_mass_pcg_f64': (c_int, [POINTER(CSR), _P, _P, _P, _P, POINTER(PCGOptions), _P, c_size_t,
                                   POINTER(PCGInfo), _P]),
-    'afsi_p2_prepare_f64': (c_int, [_P, POINTER(P2Group), POINTER(Grid3), POINTER(P2Tables), _P, _P]),
-    'afsi_p2_spread_f64': (c_int, [_P, POINTER(P2Group), POINTER(P2Tables), POINTER(Grid3), c_double, _P, _P, _P,
-                                   _P, _P]),
-    'afsi_p2_gather_assemble_f64': (c_int, [_P, _P, _P, POINTER(P2Group), POINTER(P2Tables), POINTER(Grid3), _P,
-                                            _P, _P]),
+    **{f'afsi_p2_prepare_{t}': (c_int, [_P, POINTER(P2Group), POINTER(Grid3), POINTER(P2Tables), _P, _P])
+       for t in ('f64', 'f32')},
+    **{f'afsi_p2_spread_{t}': (c_int, [_P, POINTER(P2Group), POINTER(P2Tables), POINTER(Grid3), c_double, _P, _P,
+                                       _P, _P, _P]) for t in ('f64', 'f32')},
+    **{f'afsi_p2_gather_assemble_{t}': (c_int, [_P, _P, _P, POINTER(P2Group), POINTER(P2Tables), POINTER(Grid3),
+                                                _P, _P, _P]) for t in ('f64', 'f32')},
     'afsi_hash_accumulate_f64': (c_int, [_P, _P, c_int64, POINTER(HashTable), _P]),
     'afsi_p1_contract_f64': (c_int, [POINTER(FEGeometry), _P, _P, c_int, POINTER(Grid3), POINTER(HashTable), _P]),
     'afsi_csr_spmv_f64': (c_int, [POINTER(CSR), _P, c_int64, _P, c_int64, c_double, _P]),

@@ -37,19 +37,22 @@ constexpr int P2_BLOCK = 4;      // consecutive elements per spread workgroup
 constexpr int P2_SPREAD_PATCH = 2304;
 
 struct Lattice {
-    double origin[2][3];  // [0] faces: origin; [1] centres: origin + h/2
+    double origin[3];  // faces along their own axis; centres start at origin + h/2
     double h[3];
-    double limit[3];      // n - 2
+    double limit[3];   // n - 2
     int64_t n[3];
 };
 
 // ---------------------------------------------------------------------------
 // prepare: one thread per interaction point, all three axes.
-__global__ void __launch_bounds__(256) p2_prepare_kernel(const double* __restrict__ x,
+// Computed in the table type T, as the reference does (its origin and
+// spacing tensors share the positions' dtype).
+template <class T>
+__global__ void __launch_bounds__(256) p2_prepare_kernel(const T* __restrict__ x,
                                                          const int64_t* __restrict__ cells,
-                                                         const double* __restrict__ N, int64_t ne, int nq,
+                                                         const T* __restrict__ N, int64_t ne, int nq,
                                                          int64_t nodes, Lattice L, int64_t* __restrict__ base,
-                                                         double* __restrict__ phi, int64_t total, int64_t offset,
+                                                         T* __restrict__ phi, int64_t total, int64_t offset,
                                                          int* __restrict__ invalid) {
     const int64_t count = ne * nq;
     for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < count;
@@ -67,25 +70,27 @@ __global__ void __launch_bounds__(256) p2_prepare_kernel(const double* __restric
             raise_flag(invalid, F_INDEX);
             continue;
         }
-        double pos[3] = {0.0, 0.0, 0.0};
+        T pos[3] = {T(0), T(0), T(0)};
 #pragma unroll
         for (int a = 0; a < P2_NODES; ++a) {
-            const double w = N[q * P2_NODES + a];
+            const T w = N[q * P2_NODES + a];
             pos[0] += w * x[node[a] * 3];
             pos[1] += w * x[node[a] * 3 + 1];
             pos[2] += w * x[node[a] * 3 + 2];
         }
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const double physical = (pos[a] - L.origin[0][a]) / L.h[a];
-            const bool valid = isfinite(physical) && physical >= 2.0 && physical < L.limit[a];
+            const T h = T(L.h[a]), o = T(L.origin[a]);
+            const T physical = (pos[a] - o) / h;
+            const bool valid = isfinite(physical) && physical >= T(2) && physical < T(L.limit[a]);
             if (!valid) raise_flag(invalid, F_SUPPORT);
 #pragma unroll
             for (int l = 0; l < 2; ++l) {
-                const double scaled = valid ? (pos[a] - L.origin[l][a]) / L.h[a] : 0.0;
-                const double lower = floor(scaled - 1.0);
-                const double f = scaled - lower - 1.0;
-                double w[4];
+                const T start = l ? o + T(0.5) * h : o;
+                const T scaled = valid ? (pos[a] - start) / h : T(0);
+                const T lower = floor(scaled - T(1));
+                const T f = scaled - lower - T(1);
+                T w[4];
                 peskin_weights(f, w);
                 const int64_t index = (l * total + offset + p) * 3 + a;
                 base[index] = int64_t(lower);
@@ -107,9 +112,10 @@ struct Patch {
     int status;  // 0, F_SUPPORT (box outside the face arrays) or F_INDEX (node id)
 };
 
+template <class T>
 struct Stage {
-    double phi[2][P2_CHUNK * 12];  // [lattice][point * 12 + axis * 4 + k]
-    int base[2][P2_CHUNK * 3];     // [lattice][point * 3 + axis]
+    T phi[2][P2_CHUNK * 12];   // [lattice][point * 12 + axis * 4 + k]
+    int base[2][P2_CHUNK * 3]; // [lattice][point * 3 + axis]
 };
 
 __device__ inline int wave_min(int v) {
@@ -131,9 +137,10 @@ constexpr int BASE_LIMIT = 1 << 28;
 // range. Uniform across the workgroup on return.
 // With ``staged`` (every point of the element already in ``st``, bases
 // range-checked by stage_chunk into ``bad``) the box is taken from LDS.
+template <class T>
 __device__ void plan_patch(Patch& s, int (*red)[12], const int64_t* __restrict__ base, int64_t first, int nq,
                            int64_t total, const int64_t n[3], const int64_t* __restrict__ cell, int ids,
-                           int64_t nodes, const Stage* staged, bool ok) {
+                           int64_t nodes, const Stage<T>* staged, bool ok) {
     const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     int lo[6], hi[6];
 #pragma unroll
@@ -258,14 +265,15 @@ __device__ inline int patch_decode(const PatchRegs& s, int i, int& x, int& y, in
 
 // Coalesced copy of one chunk's tables into LDS; returns false when a base
 // of this thread's share is out of the representable range.
-__device__ inline bool stage_chunk(Stage& st, const int64_t* __restrict__ base, const double* __restrict__ phi,
+template <class T>
+__device__ inline bool stage_chunk(Stage<T>& st, const int64_t* __restrict__ base, const T* __restrict__ phi,
                                    int64_t first, int q0, int cnt, int64_t total) {
     const int tid = threadIdx.x;
     bool ok = true;
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         const int64_t row = l * total + first + q0;
-        const double* src = phi + row * 12;
+        const T* src = phi + row * 12;
         for (int i = tid; i < cnt * 12; i += P2_THREADS) st.phi[l][i] = src[i];
         const int64_t* bsrc = base + row * 3;
         for (int i = tid; i < cnt * 3; i += P2_THREADS) {
@@ -278,17 +286,19 @@ __device__ inline bool stage_chunk(Stage& st, const int64_t* __restrict__ base, 
 }
 
 // ---------------------------------------------------------------------------
+// Any T: products and the LDS patch in FP64, faces accumulated in T.
+template <class T>
 __global__ void __launch_bounds__(P2_THREADS) p2_spread_kernel(
-    const double* __restrict__ value, const int64_t* __restrict__ cells, const double* __restrict__ N,
-    const double* __restrict__ W, int64_t ne, int nq, const int64_t* __restrict__ base,
-    const double* __restrict__ phi, int64_t total, int64_t offset, double* __restrict__ u0,
-    double* __restrict__ u1, double* __restrict__ u2, int64_t nodes, Lattice L, double volume,
+    const T* __restrict__ value, const int64_t* __restrict__ cells, const T* __restrict__ N,
+    const T* __restrict__ W, int64_t ne, int nq, const int64_t* __restrict__ base,
+    const T* __restrict__ phi, int64_t total, int64_t offset, T* __restrict__ u0,
+    T* __restrict__ u1, T* __restrict__ u2, int64_t nodes, Lattice L, double volume,
     int* __restrict__ flags) {
     __shared__ Patch s;
     __shared__ int red[P2_WAVES + 1][12];
     __shared__ double nodal[P2_BLOCK][P2_NODES * 3];
     __shared__ double force[P2_CHUNK * 3];
-    __shared__ Stage st;
+    __shared__ Stage<T> st;
     __shared__ double patch[P2_SPREAD_PATCH];
 
     const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
@@ -312,11 +322,11 @@ __global__ void __launch_bounds__(P2_THREADS) p2_spread_kernel(
     const bool cached = s.off[3] <= P2_SPREAD_PATCH;
     if (tid < count * P2_NODES * 3) {
         const int k = tid / (P2_NODES * 3), r = tid % (P2_NODES * 3);
-        nodal[k][r] = value[cells[(e0 + k) * P2_NODES + r / 3] * 3 + r % 3];
+        nodal[k][r] = double(value[cells[(e0 + k) * P2_NODES + r / 3] * 3 + r % 3]);
     }
     if (cached)
         for (int i = tid; i < s.off[3]; i += P2_THREADS) patch[i] = 0.0;
-    double* const fields[3] = {u0, u1, u2};
+    T* const fields[3] = {u0, u1, u2};
     const int ti = lane >> 4, tj = (lane >> 2) & 3, tk = lane & 3;
     const PatchRegs R = patch_regs(s);
     const int uwave = __builtin_amdgcn_readfirstlane(wave);
@@ -331,16 +341,16 @@ __global__ void __launch_bounds__(P2_THREADS) p2_spread_kernel(
         if (tid < cnt * 3) {
             const int g = g0 + tid / 3, c = tid % 3;
             const int k = g / nq, q = g - k * nq;
-            const double* Nq = N + int64_t(q) * P2_NODES;
+            const T* Nq = N + int64_t(q) * P2_NODES;
             double f = 0.0;
 #pragma unroll
-            for (int a = 0; a < P2_NODES; ++a) f += Nq[a] * nodal[k][a * 3 + c];
-            force[tid] = (f * W[(e0 + k) * nq + q]) / volume;
+            for (int a = 0; a < P2_NODES; ++a) f += double(Nq[a]) * nodal[k][a * 3 + c];
+            force[tid] = (f * double(W[(e0 + k) * nq + q])) / volume;
         }
         __syncthreads();
         for (int q = uwave; q < cnt; q += P2_WAVES) {
-            const double* P0 = st.phi[0] + q * 12;
-            const double* P1 = st.phi[1] + q * 12;
+            const T* P0 = st.phi[0] + q * 12;
+            const T* P1 = st.phi[1] + q * 12;
             const double fx0 = P0[ti], fx1 = P1[ti];
             const double fy0 = P0[4 + tj], fy1 = P1[4 + tj];
             const double fz0 = P0[8 + tk], fz1 = P1[8 + tk];
@@ -359,7 +369,7 @@ __global__ void __launch_bounds__(P2_THREADS) p2_spread_kernel(
                                       R.ext[c][2] + (bz - R.start[c][2]);
                     atomicAdd(patch + point + tapoff[c], v);
                 } else {
-                    unsafeAtomicAdd(fields[c] + face_index(c, bx + ti, by + tj, bz + tk, L.n), v);
+                    unsafeAtomicAdd(fields[c] + face_index(c, bx + ti, by + tj, bz + tk, L.n), T(v));
                 }
             }
         }
@@ -371,22 +381,24 @@ __global__ void __launch_bounds__(P2_THREADS) p2_spread_kernel(
         if (v == 0.0) continue;
         int x, y, z;
         const int c = patch_decode(R, i, x, y, z);
-        unsafeAtomicAdd(fields[c] + face_index(c, x, y, z, L.n), v);
+        unsafeAtomicAdd(fields[c] + face_index(c, x, y, z, L.n), T(v));
     }
 }
 
 // ---------------------------------------------------------------------------
+// Any T: the patch, products and sums in FP64, rhs accumulated in T.
+template <class T>
 __global__ void __launch_bounds__(P2_THREADS) p2_gather_assemble_kernel(
-    const double* __restrict__ u0, const double* __restrict__ u1, const double* __restrict__ u2,
-    const int64_t* __restrict__ cells, const double* __restrict__ N, const double* __restrict__ W, int nq,
-    const int64_t* __restrict__ base, const double* __restrict__ phi, int64_t total, int64_t offset,
-    double* __restrict__ rhs, int64_t nodes, Lattice L, int* __restrict__ flags) {
+    const T* __restrict__ u0, const T* __restrict__ u1, const T* __restrict__ u2,
+    const int64_t* __restrict__ cells, const T* __restrict__ N, const T* __restrict__ W, int nq,
+    const int64_t* __restrict__ base, const T* __restrict__ phi, int64_t total, int64_t offset,
+    T* __restrict__ rhs, int64_t nodes, Lattice L, int* __restrict__ flags) {
     __shared__ Patch s;
     __shared__ int red[P2_WAVES + 1][12];
     __shared__ double part[4][P2_CHUNK * 3];  // [i tap][point * 3 + c]
     __shared__ double sums[30][8];
     __shared__ double nw[P2_CHUNK * P2_NODES];  // N_qa W_eq of the chunk's points
-    __shared__ Stage st;
+    __shared__ Stage<T> st;
     __shared__ double patch[P2_PATCH];
 
     const int tid = threadIdx.x;
@@ -404,13 +416,13 @@ __global__ void __launch_bounds__(P2_THREADS) p2_gather_assemble_kernel(
         return;
     }
     const bool cached = s.off[3] <= P2_PATCH;
-    const double* const fields[3] = {u0, u1, u2};
+    const T* const fields[3] = {u0, u1, u2};
     const PatchRegs R = patch_regs(s);
     if (cached)
         for (int i = tid; i < R.off[3]; i += P2_THREADS) {
             int x, y, z;
             const int c = patch_decode(R, i, x, y, z);
-            patch[i] = fields[c][face_index(c, x, y, z, L.n)];
+            patch[i] = double(fields[c][face_index(c, x, y, z, L.n)]);
         }
     // thread -> (point in chunk, tap slice i)
     const int pq = tid % P2_CHUNK, ti = tid / P2_CHUNK;
@@ -423,19 +435,19 @@ __global__ void __launch_bounds__(P2_THREADS) p2_gather_assemble_kernel(
         __syncthreads();
         if (!single) stage_chunk(st, base, phi, first, q0, cnt, total);
         for (int i = tid; i < cnt * P2_NODES; i += P2_THREADS)
-            nw[i] = N[int64_t(q0) * P2_NODES + i] * W[e * nq + q0 + i / P2_NODES];
+            nw[i] = double(N[int64_t(q0) * P2_NODES + i]) * double(W[e * nq + q0 + i / P2_NODES]);
         __syncthreads();
         if (pq < cnt) {
-            const double* P0 = st.phi[0] + pq * 12;
-            const double* P1 = st.phi[1] + pq * 12;
+            const T* P0 = st.phi[0] + pq * 12;
+            const T* P1 = st.phi[1] + pq * 12;
             const int* B0 = st.base[0] + pq * 3;
             const int* B1 = st.base[1] + pq * 3;
             const double fx0 = P0[ti], fx1 = P1[ti];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const double wx = c == 0 ? fx0 : fx1;
-                const double* Py = (c == 1 ? P0 : P1) + 4;
-                const double* Pz = (c == 2 ? P0 : P1) + 8;
+                const T* Py = (c == 1 ? P0 : P1) + 4;
+                const T* Pz = (c == 2 ? P0 : P1) + 8;
                 const int x = (c == 0 ? B0[0] : B1[0]) + ti;
                 const int y0 = c == 1 ? B0[1] : B1[1];
                 const int z0 = c == 2 ? B0[2] : B1[2];
@@ -447,18 +459,18 @@ __global__ void __launch_bounds__(P2_THREADS) p2_gather_assemble_kernel(
                                         (z0 - R.start[c][2]);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        const double wxy = wx * Py[j];
+                        const double wxy = wx * double(Py[j]);
                         const double* r = row + j * ez;
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) got += r[k] * (wxy * Pz[k]);
+                        for (int k = 0; k < 4; ++k) got += r[k] * (wxy * double(Pz[k]));
                     }
                 } else {
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        const double wxy = wx * Py[j];
+                        const double wxy = wx * double(Py[j]);
 #pragma unroll
                         for (int k = 0; k < 4; ++k)
-                            got += fields[c][face_index(c, x, y0 + j, z0 + k, L.n)] * (wxy * Pz[k]);
+                            got += double(fields[c][face_index(c, x, y0 + j, z0 + k, L.n)]) * (wxy * double(Pz[k]));
                     }
                 }
                 part[ti][pq * 3 + c] = got;
@@ -479,7 +491,7 @@ __global__ void __launch_bounds__(P2_THREADS) p2_gather_assemble_kernel(
         double v = 0.0;
 #pragma unroll
         for (int r = 0; r < 8; ++r) v += sums[tid][r];
-        unsafeAtomicAdd(rhs + cells[e * P2_NODES + tid / 3] * 3 + tid % 3, v);
+        unsafeAtomicAdd(rhs + cells[e * P2_NODES + tid / 3] * 3 + tid % 3, T(v));
     }
 }
 
@@ -491,8 +503,7 @@ int make_lattice(const afsi_grid3* g, Lattice& L) {
         AFSI_REQUIRE(std::isfinite(g->h[a]) && g->h[a] > 0, "invalid P2 MAC grid");
         L.n[a] = g->n[a];
         L.h[a] = g->h[a];
-        L.origin[0][a] = g->origin[a];
-        L.origin[1][a] = g->center_origin[a];
+        L.origin[a] = g->origin[a];
         L.limit[a] = double(g->n[a] - 2);
     }
     return OK;
@@ -514,54 +525,91 @@ int check_group(const afsi_p2_group* g, const afsi_p2_tables* t) {
 }  // namespace
 }  // namespace afsi
 
-using namespace afsi;
+namespace afsi {
+namespace {
 
-extern "C" {
-
-int afsi_p2_prepare_f64(const double* x, const afsi_p2_group* g, const afsi_grid3* grid, afsi_p2_tables* t,
-                        int* invalid, void* stream) {
+template <class T>
+int p2_prepare(const T* x, const afsi_p2_group* g, const afsi_grid3* grid, afsi_p2_tables* t, int* invalid,
+               void* stream) {
     if (int r = check_group(g, t)) return r;
     Lattice L;
     if (int r = make_lattice(grid, L)) return r;
     AFSI_REQUIRE(x, "null positions");
     const int64_t count = g->elements * g->quad;
     if (!count) return OK;
-    p2_prepare_kernel<<<grid_for(count, 256), 256, 0, hipStream_t(stream)>>>(
-        x, g->cells, g->values, g->elements, int(g->quad), g->node_count, L, t->base, t->phi, t->total, t->offset,
-        invalid);
+    p2_prepare_kernel<T><<<grid_for(count, 256), 256, 0, hipStream_t(stream)>>>(
+        x, g->cells, static_cast<const T*>(g->values), g->elements, int(g->quad), g->node_count, L, t->base,
+        static_cast<T*>(t->phi), t->total, t->offset, invalid);
     AFSI_LAUNCH_CHECK();
     return OK;
 }
 
-int afsi_p2_spread_f64(const double* value, const afsi_p2_group* g, const afsi_p2_tables* t,
-                       const afsi_grid3* grid, double volume, double* u0, double* u1, double* u2, int* flags,
-                       void* stream) {
+template <class T>
+int p2_spread(const T* value, const afsi_p2_group* g, const afsi_p2_tables* t, const afsi_grid3* grid,
+              double volume, T* u0, T* u1, T* u2, int* flags, void* stream) {
     if (int r = check_group(g, t)) return r;
     Lattice L;
     if (int r = make_lattice(grid, L)) return r;
     AFSI_REQUIRE(value && u0 && u1 && u2 && g->weights, "null P2 spread arrays");
     AFSI_REQUIRE(std::isfinite(volume) && volume > 0, "invalid P2 MAC grid");
     if (!g->elements) return OK;
-    p2_spread_kernel<<<unsigned(cdiv(g->elements, P2_BLOCK)), P2_THREADS, 0, hipStream_t(stream)>>>(
-        value, g->cells, g->values, g->weights, g->elements, int(g->quad), t->base, t->phi, t->total, t->offset, u0, u1, u2,
-        g->node_count, L, volume, flags);
+    p2_spread_kernel<T><<<unsigned(cdiv(g->elements, P2_BLOCK)), P2_THREADS, 0, hipStream_t(stream)>>>(
+        value, g->cells, static_cast<const T*>(g->values), static_cast<const T*>(g->weights), g->elements,
+        int(g->quad), t->base, static_cast<const T*>(t->phi), t->total, t->offset, u0, u1, u2, g->node_count, L,
+        volume, flags);
     AFSI_LAUNCH_CHECK();
     return OK;
 }
 
-int afsi_p2_gather_assemble_f64(const double* u0, const double* u1, const double* u2, const afsi_p2_group* g,
-                                const afsi_p2_tables* t, const afsi_grid3* grid, double* rhs, int* flags,
-                                void* stream) {
+template <class T>
+int p2_gather_assemble(const T* u0, const T* u1, const T* u2, const afsi_p2_group* g, const afsi_p2_tables* t,
+                       const afsi_grid3* grid, T* rhs, int* flags, void* stream) {
     if (int r = check_group(g, t)) return r;
     Lattice L;
     if (int r = make_lattice(grid, L)) return r;
     AFSI_REQUIRE(u0 && u1 && u2 && rhs && g->weights, "null P2 gather arrays");
     if (!g->elements) return OK;
-    p2_gather_assemble_kernel<<<unsigned(g->elements), P2_THREADS, 0, hipStream_t(stream)>>>(
-        u0, u1, u2, g->cells, g->values, g->weights, int(g->quad), t->base, t->phi, t->total, t->offset, rhs,
-        g->node_count, L, flags);
+    p2_gather_assemble_kernel<T><<<unsigned(g->elements), P2_THREADS, 0, hipStream_t(stream)>>>(
+        u0, u1, u2, g->cells, static_cast<const T*>(g->values), static_cast<const T*>(g->weights), int(g->quad),
+        t->base, static_cast<const T*>(t->phi), t->total, t->offset, rhs, g->node_count, L, flags);
     AFSI_LAUNCH_CHECK();
     return OK;
+}
+
+}  // namespace
+}  // namespace afsi
+
+using namespace afsi;
+
+extern "C" {
+
+int afsi_p2_prepare_f64(const double* x, const afsi_p2_group* g, const afsi_grid3* grid, afsi_p2_tables* t,
+                        int* invalid, void* stream) {
+    return p2_prepare<double>(x, g, grid, t, invalid, stream);
+}
+int afsi_p2_prepare_f32(const float* x, const afsi_p2_group* g, const afsi_grid3* grid, afsi_p2_tables* t,
+                        int* invalid, void* stream) {
+    return p2_prepare<float>(x, g, grid, t, invalid, stream);
+}
+int afsi_p2_spread_f64(const double* value, const afsi_p2_group* g, const afsi_p2_tables* t,
+                       const afsi_grid3* grid, double volume, double* u0, double* u1, double* u2, int* flags,
+                       void* stream) {
+    return p2_spread<double>(value, g, t, grid, volume, u0, u1, u2, flags, stream);
+}
+int afsi_p2_spread_f32(const float* value, const afsi_p2_group* g, const afsi_p2_tables* t,
+                       const afsi_grid3* grid, double volume, float* u0, float* u1, float* u2, int* flags,
+                       void* stream) {
+    return p2_spread<float>(value, g, t, grid, volume, u0, u1, u2, flags, stream);
+}
+int afsi_p2_gather_assemble_f64(const double* u0, const double* u1, const double* u2, const afsi_p2_group* g,
+                                const afsi_p2_tables* t, const afsi_grid3* grid, double* rhs, int* flags,
+                                void* stream) {
+    return p2_gather_assemble<double>(u0, u1, u2, g, t, grid, rhs, flags, stream);
+}
+int afsi_p2_gather_assemble_f32(const float* u0, const float* u1, const float* u2, const afsi_p2_group* g,
+                                const afsi_p2_tables* t, const afsi_grid3* grid, float* rhs, int* flags,
+                                void* stream) {
+    return p2_gather_assemble<float>(u0, u1, u2, g, t, grid, rhs, flags, stream);
 }
 
 }  // extern "C"

@@ -108,6 +108,14 @@ def _tables(base, phi, offset):
     return _abi.P2Tables(_abi.ptr(base), _abi.ptr(phi), base.shape[1], offset)
 
 
+def _suffix(dtype):
+    if dtype == torch.float64:
+        return 'f64'
+    if dtype == torch.float32:
+        return 'f32'
+    raise RuntimeError('P2 IB requires FP32/FP64')
+
+
 def _numel(shape):
     n = 1
     for s in shape:
@@ -135,8 +143,9 @@ def prepare(transfer, x, rule, *, buffers=None):
     ``buffers`` = (base, phi) to reuse; raises the reference's 'invalid P2
     stencil buffers' and 'support' ValueErrors."""
     require_cuda(x)
-    if x.dtype != torch.float64 or x.ndim != 2 or x.shape[1] != 3:
-        raise ValueError('P2 positions must be (N,3) float64')
+    suffix = _suffix(x.dtype)
+    if x.ndim != 2 or x.shape[1] != 3:
+        raise ValueError('P2 positions must be (N,3)')
     total = rule.point_count
     if buffers is None:
         base = torch.empty((2, total, 3), device=x.device, dtype=torch.int64)
@@ -155,7 +164,7 @@ def prepare(transfer, x, rule, *, buffers=None):
     grid = transfer.grid.abi()
     offset = 0
     for g in rule.groups:
-        _abi.call('afsi_p2_prepare_f64', _abi.ptr(x), _group(g, x.shape[0]), grid, _tables(base, phi, offset),
+        _abi.call(f'afsi_p2_prepare_{suffix}', _abi.ptr(x), _group(g, x.shape[0]), grid, _tables(base, phi, offset),
                   _word(status, 0), _abi.stream())
         offset += g.weights.numel()
     support, skipped = status.tolist()
@@ -173,7 +182,8 @@ def spread(transfer, coefficient, stencil):
     """Face force densities of the nodal force coefficient (mac/p2_cuda_transfer.py:49-56):
     fresh zeroed face arrays accumulated group by group."""
     require_cuda(coefficient)
-    if coefficient.ndim != 2 or coefficient.shape[1] != 3 or coefficient.dtype != torch.float64:
+    suffix = _suffix(coefficient.dtype)
+    if coefficient.ndim != 2 or coefficient.shape[1] != 3:
         raise RuntimeError('invalid P2 nodal field')
     _check_stencil(stencil, coefficient)
     fields = transfer.grid.zeros(device=coefficient.device, dtype=coefficient.dtype)
@@ -182,7 +192,7 @@ def spread(transfer, coefficient, stencil):
     grid = transfer.grid.abi()
     offset = 0
     for g in stencil.rule.groups:
-        _abi.call('afsi_p2_spread_f64', _abi.ptr(coefficient), _group(g, coefficient.shape[0]),
+        _abi.call(f'afsi_p2_spread_{suffix}', _abi.ptr(coefficient), _group(g, coefficient.shape[0]),
                   _tables(stencil.base, stencil.phi, offset), grid, float(transfer.grid.volume),
                   *(_abi.ptr(f) for f in fields), _word(status, 1), _abi.stream())
         offset += g.weights.numel()
@@ -192,6 +202,7 @@ def spread(transfer, coefficient, stencil):
 def assemble_velocity(transfer, velocity, stencil):
     """B^T W K u accumulated per element (mac/p2_cuda_transfer.py:59-67)."""
     rhs = torch.zeros_like(transfer.diagonal)
+    suffix = _suffix(rhs.dtype)
     velocity = tuple(v.contiguous() for v in velocity)
     if len(velocity) != 3 or any(v.dtype != rhs.dtype or v.device != rhs.device or
                                  v.numel() != _numel(transfer.grid.face_shape(c)) for c, v in enumerate(velocity)):
@@ -201,7 +212,7 @@ def assemble_velocity(transfer, velocity, stencil):
     grid = transfer.grid.abi()
     offset = 0
     for g in stencil.rule.groups:
-        _abi.call('afsi_p2_gather_assemble_f64', *(_abi.ptr(v) for v in velocity), _group(g, rhs.shape[0]),
+        _abi.call(f'afsi_p2_gather_assemble_{suffix}', *(_abi.ptr(v) for v in velocity), _group(g, rhs.shape[0]),
                   _tables(stencil.base, stencil.phi, offset), grid, _abi.ptr(rhs), _word(status, 1),
                   _abi.stream())
         offset += g.weights.numel()

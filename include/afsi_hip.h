@@ -536,8 +536,8 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
  * same reference rule N (quad, 10) and its own physical weights W. */
 typedef struct afsi_p2_group {
     const int64_t* cells;      /* (elements, 10) node ids */
-    const double* values;      /* (quad, 10): N_qa */
-    const double* weights;     /* (elements, quad): W_eq (unused by prepare) */
+    const void* values;        /* (quad, 10): N_qa, the call's element type */
+    const void* weights;       /* (elements, quad): W_eq (unused by prepare) */
     int64_t elements;
     int64_t quad;
     int64_t node_count;
@@ -548,7 +548,7 @@ typedef struct afsi_p2_group {
  * A group occupies rows offset .. offset + elements*quad. */
 typedef struct afsi_p2_tables {
     int64_t* base;
-    double* phi;
+    void* phi;                 /* the call's element type */
     int64_t total;
     int64_t offset;
 } afsi_p2_tables;
@@ -560,6 +560,10 @@ typedef struct afsi_p2_tables {
  * Closed boxes only (grid->periodic = 0). */
 int afsi_p2_prepare_f64(const double* x, const afsi_p2_group* g, const afsi_grid3* grid, afsi_p2_tables* t,
                         int* invalid, void* stream);
+/* The same in FP32 (positions, N and tables float; p2_transfer.cu
+ * dispatches both types). */
+int afsi_p2_prepare_f32(const float* x, const afsi_p2_group* g, const afsi_grid3* grid, afsi_p2_tables* t,
+                        int* invalid, void* stream);
 /* Replaces torch.ops.afsi_ib_cuda.p2_spread_ (p2_transfer.cpp:66-72,
  * 69; kernel p2_transfer.cu:97-188 with Spread=true): accumulates
  * f_c += (phi_x phi_y phi_z) (W_eq sum_a N_qa value[cells[e,a]]_c) / volume
@@ -570,12 +574,20 @@ int afsi_p2_prepare_f64(const double* x, const afsi_p2_group* g, const afsi_grid
 int afsi_p2_spread_f64(const double* value, const afsi_p2_group* g, const afsi_p2_tables* t,
                        const afsi_grid3* grid, double volume, double* u0, double* u1, double* u2, int* flags,
                        void* stream);
+/* FP32 fields and tables; products and the LDS patch are accumulated in FP64. */
+int afsi_p2_spread_f32(const float* value, const afsi_p2_group* g, const afsi_p2_tables* t,
+                       const afsi_grid3* grid, double volume, float* u0, float* u1, float* u2, int* flags,
+                       void* stream);
 /* Replaces torch.ops.afsi_ib_cuda.p2_gather_assemble_ (p2_transfer.cpp:61-65,
  * 68; kernel p2_transfer.cu:97-188 with Spread=false): accumulates
  * rhs[cells[e,a]]_c += sum_q (N_qa W_eq) sum_taps u_c w into rhs
  * (node_count, 3). Same flags as the spread. */
 int afsi_p2_gather_assemble_f64(const double* u0, const double* u1, const double* u2, const afsi_p2_group* g,
                                 const afsi_p2_tables* t, const afsi_grid3* grid, double* rhs, int* flags,
+                                void* stream);
+/* FP32 fields and tables; sums in FP64, rhs accumulated in FP32. */
+int afsi_p2_gather_assemble_f32(const float* u0, const float* u1, const float* u2, const afsi_p2_group* g,
+                                const afsi_p2_tables* t, const afsi_grid3* grid, float* rhs, int* flags,
                                 void* stream);
 
 /* ---- CSR-assembled quadrature operator (P1 meshes) -----------------------

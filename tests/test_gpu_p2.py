@@ -233,3 +233,39 @@ def test_p2_error_paths_and_buffer_ownership(gpu):
     bad = p2.Rule((p2.RuleGroup(geometry.cells, geometry.values, geometry.weights[:, :-1]),))
     with pytest.raises(ValueError, match='weights mismatch'):
         p2.check_rule(bad, g.node_count, geometry.weights)
+
+
+def test_p2_fp32_operators_match_fp64_oracle(gpu):
+    """FP32 entry points (positions, basis, weights, tables and fields in
+    FP32; p2_transfer.cu dispatches both types) against the FP64 oracle at
+    the reference's FP32 tolerances (tests/test_p2_cuda_execution.py:38-45):
+    tables, spread and gather-assemble, and the adjointness of the two."""
+    g, X = mesh()
+    grid, ogrid = grids()
+    x = (X + .013 * np.sin(3 * X)).astype(np.float32).astype(np.float64)
+    geometry = FEGeometry(t(g.cells, gpu, torch.int64), t(g.values, gpu, torch.float32),
+                          t(g.weights, gpu, torch.float32), g.node_count)
+    rule = p2.rule_from_geometry(geometry)
+    holder = Holder(grid, g.node_count, gpu)
+    holder.diagonal = holder.diagonal.float()
+    base, phi = p2.prepare(holder, t(x, gpu, torch.float32), rule)
+    assert phi.dtype == torch.float32
+    og = ofe.Geometry(g.cells, g.values.astype(np.float32).astype(np.float64),
+                      g.weights.astype(np.float32).astype(np.float64), g.node_count)
+    rb, rp, valid, _ = oracle_tables([og], x, ogrid)
+    np.testing.assert_array_equal(base.cpu().numpy(), rb)
+    np.testing.assert_allclose(phi.cpu().numpy(), rp, rtol=8e-5, atol=3e-5)
+    stencil = p2.SharedStencil(base, phi, rule)
+    C = (np.sin(1.3 * X) + .1).astype(np.float32)
+    fields = p2.spread(holder, t(C, gpu, torch.float32), stencil)
+    ref = oracle_spread([og], C.astype(np.float64), ogrid, x)
+    for c in range(3):
+        assert fields[c].dtype == torch.float32
+        assert rel(fields[c].double().cpu(), ref[c]) < 1e-5, c
+    velocity = tuple(v.float() for v in velocity_fields(grid, gpu))
+    rhs = p2.assemble_velocity(holder, velocity, stencil)
+    ref_rhs = oracle_assemble([og], [u.double().cpu().numpy() for u in velocity], ogrid, x)
+    assert rhs.dtype == torch.float32 and rel(rhs.double().cpu(), ref_rhs) < 1e-5
+    solid = float((t(C, gpu) * rhs.double()).sum())
+    fluid = float(grid.volume * sum((u.double() * f.double()).sum() for u, f in zip(velocity, fields)))
+    assert abs(solid - fluid) <= 1e-5 * max(abs(solid), 1.)
