@@ -683,6 +683,59 @@ __global__ void __launch_bounds__(256) prolong_rows_kernel(const T* __restrict__
     prolong_row<T, P>(coarse, fine, d, int(blockIdx.x) * 4 + wave);
 }
 
+// The same prolongation for a 2x2 group of fine rows: fine rows 2k+1 and
+// 2k+2 of an axis interpolate between the same coarse rows k and k+1 (only
+// their weights differ), so one wave reads the four coarse rows once and
+// updates up to four fine rows. Group (kx, ky), kx in [-1, C0), ky in
+// [-1, C1); each fine value is formed exactly as prolong_row forms it.
+template <class T, bool P>
+__global__ void __launch_bounds__(256) prolong_groups_kernel(const T* __restrict__ coarse,
+                                                             T* __restrict__ fine, Dim32 d) {
+    const int g = int(blockIdx.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int C0 = d.n0 / 2, C1 = d.n1 / 2, C2 = d.n2 / 2;
+    if (g >= (C0 + 1) * (C1 + 1)) return;
+    const int ax = g / (C1 + 1) - 1, ay = g - (ax + 1) * (C1 + 1) - 1;
+    int x0, x1, y0, y1;
+    if (P) {
+        x0 = wrap32(ax, C0); x1 = wrap32(ax + 1, C0);
+        y0 = wrap32(ay, C1); y1 = wrap32(ay + 1, C1);
+    } else {
+        x0 = ax > 0 ? ax : 0; x1 = ax + 1 < C0 - 1 ? ax + 1 : C0 - 1;
+        y0 = ay > 0 ? ay : 0; y1 = ay + 1 < C1 - 1 ? ay + 1 : C1 - 1;
+    }
+    const T* c00 = coarse + (x0 * C1 + y0) * C2;
+    const T* c01 = coarse + (x0 * C1 + y1) * C2;
+    const T* c10 = coarse + (x1 * C1 + y0) * C2;
+    const T* c11 = coarse + (x1 * C1 + y1) * C2;
+    const int fx[2] = {2 * ax + 1, 2 * ax + 2}, fy[2] = {2 * ay + 1, 2 * ay + 2};
+    for (int z = lane; z < d.n2; z += 64) {
+        const int az = z / 2 - 1 + (z & 1);
+        int z0, z1;
+        if (P) { z0 = az < 0 ? az + C2 : az; z1 = az + 1 >= C2 ? az + 1 - C2 : az + 1; }
+        else { z0 = az > 0 ? az : 0; z1 = az + 1 < C2 - 1 ? az + 1 : C2 - 1; }
+        const T wz = (z & 1) ? T(0.25) : T(0.75);
+        const T v00 = c00[z0] * (T(1) - wz) + c00[z1] * wz;
+        const T v01 = c01[z0] * (T(1) - wz) + c01[z1] * wz;
+        const T v10 = c10[z0] * (T(1) - wz) + c10[z1] * wz;
+        const T v11 = c11[z0] * (T(1) - wz) + c11[z1] * wz;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int x = fx[i];
+            if (x < 0 || x >= d.n0) continue;
+            const T wx = (x & 1) ? T(0.25) : T(0.75);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int y = fy[j];
+                if (y < 0 || y >= d.n1) continue;
+                const T wy = (y & 1) ? T(0.25) : T(0.75);
+                T* f = fine + (x * d.n1 + y) * d.n2;
+                f[z] = f[z] + ((v00 * (T(1) - wy) + v01 * wy) * (T(1) - wx) + (v10 * (T(1) - wy) + v11 * wy) * wx);
+            }
+        }
+    }
+}
+
 // per-workgroup [sum r, sum r^2, sum |r|, non-finite] of r = rhs - A p,
 // one wavefront per row, grid-stride over rows.
 template <class T, bool P>
@@ -1524,9 +1577,9 @@ struct FastRunner {
         T* cc = P(l + 1);
         if (int r = cycle(l + 1, cc)) return r;
         if (rows_ok(l)) {
-            const unsigned gf = unsigned(cdiv(int64_t(d.n0) * d.n1, 4));
-            if (pl.periodic) prolong_rows_kernel<T, true><<<gf, 256, 0, s>>>(cc, cur, d);
-            else prolong_rows_kernel<T, false><<<gf, 256, 0, s>>>(cc, cur, d);
+            const unsigned gf = unsigned(cdiv(int64_t(d.n0 / 2 + 1) * (d.n1 / 2 + 1), 4));
+            if (pl.periodic) prolong_groups_kernel<T, true><<<gf, 256, 0, s>>>(cc, cur, d);
+            else prolong_groups_kernel<T, false><<<gf, 256, 0, s>>>(cc, cur, d);
         } else {
             const unsigned gf = grid_for(cells(l), 256, 16384);
             if (pl.periodic) prolong_fast_kernel<T, true><<<gf, 256, 0, s>>>(cc, cur, d);
