@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(256) csr_spmv3_kernel(afsi_csr a, const double
 
 // ---------------------------------------------------------------------------
 // PCG. Device state (doubles): the scalars of the recurrence and its flags.
-enum : int { S_RZ = 0, S_CURV, S_ALPHA, S_RN, S_TOL, S_ACTIVE, S_BROKEN, S_WORKING, S_TRUE, S_RHS, S_COUNT };
+enum : int { S_RZ = 0, S_CURV, S_ALPHA, S_RN, S_TOL, S_ACTIVE, S_BROKEN, S_WORKING, S_TRUE, S_RHS, S_RZOLD, S_COUNT };
 
 __device__ inline double block_sum(double v, double* sh) {
     sh[threadIdx.x] = v;
@@ -182,16 +182,45 @@ __global__ void __launch_bounds__(RED_THREADS) pcg_spmv_dot_kernel(afsi_csr a, c
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// x += alpha d; r -= alpha Ad; partial sums of r^2 (partials[0, B)) and of
-// r . (r / diag) (partials[B, 2B)).
-__global__ void __launch_bounds__(RED_THREADS) pcg_update2_kernel(double* __restrict__ x, double* __restrict__ r,
-                                                                  const double* __restrict__ d,
-                                                                  const double* __restrict__ Ad,
-                                                                  const double* __restrict__ diag, int64_t n,
-                                                                  const double* __restrict__ st,
-                                                                  double* __restrict__ partials) {
+// Sum of `count` partials in the fixed order of pcg_scalar_kernel (thread k
+// adds entries k, k + RED_THREADS, ..., then the block tree): every block
+// that calls it gets the same bits.
+__device__ inline double sum_partials(const double* __restrict__ p, int count, double* sh) {
+    double own = 0.0;
+    for (int k = threadIdx.x; k < count; k += RED_THREADS) own += p[k];
+    return block_sum(own, sh);
+}
+
+// One PCG iteration's update with its scalar step folded in: every block
+// forms curvature = d . Ad from the product's SPMV_BLOCKS partials and
+// alpha exactly as the curvature step does; block 0 records the scalars
+// (S_BROKEN is re-derived identically by any block that reads the updated
+// value, and S_RZ is kept in S_RZOLD for the direction step). Then
+// x += alpha d, r -= alpha Ad with partial sums of r^2 (out[0, B)) and of
+// r . (r / diag) (out[B, 2B)).
+__global__ void __launch_bounds__(RED_THREADS) pcg_update_kernel(double* __restrict__ x, double* __restrict__ r,
+                                                                 const double* __restrict__ d,
+                                                                 const double* __restrict__ Ad,
+                                                                 const double* __restrict__ diag, int64_t n,
+                                                                 double* __restrict__ st,
+                                                                 const double* __restrict__ curv_partials,
+                                                                 double* __restrict__ out) {
     __shared__ double sh[RED_THREADS];
-    const double alpha = st[S_ALPHA];
+    const double curvature = sum_partials(curv_partials, SPMV_BLOCKS, sh);
+    const double rz = st[S_RZ];
+    const bool active = st[S_ACTIVE] != 0.0;
+    const bool valid = isfinite(curvature) && curvature > 0.0 && isfinite(rz) && rz > 0.0;
+    const bool broken = st[S_BROKEN] != 0.0 || (active && !valid);
+    const bool working = active && valid && !broken;
+    const double alpha = (working ? rz : 0.0) / (working ? curvature : 1.0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st[S_CURV] = curvature;
+        st[S_ALPHA] = alpha;
+        st[S_BROKEN] = broken ? 1.0 : 0.0;
+        st[S_WORKING] = working ? 1.0 : 0.0;
+        st[S_RZOLD] = rz;
+    }
+    __syncthreads();
     double s = 0.0, q = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * RED_THREADS + threadIdx.x; i < n; i += int64_t(RED_BLOCKS) * RED_THREADS) {
         x[i] = x[i] + alpha * d[i];
@@ -204,8 +233,38 @@ __global__ void __launch_bounds__(RED_THREADS) pcg_update2_kernel(double* __rest
     __syncthreads();
     const double u = block_sum(q, sh);
     if (threadIdx.x == 0) {
-        partials[blockIdx.x] = t;
-        partials[RED_BLOCKS + blockIdx.x] = u;
+        out[blockIdx.x] = t;
+        out[RED_BLOCKS + blockIdx.x] = u;
+    }
+}
+
+// The norm/active/beta step folded into the direction: every block forms
+// rn, r.z, active and beta = r.z / rz_old as the scalar step does (rz_old
+// from S_RZOLD, which this launch does not write); block 0 records S_RN,
+// S_BROKEN, S_ACTIVE, beta and the new rz. d = active ? z + beta d : 0.
+__global__ void __launch_bounds__(RED_THREADS) pcg_next_direction_kernel(double* __restrict__ d,
+                                                                         const double* __restrict__ r,
+                                                                         const double* __restrict__ diag, int64_t n,
+                                                                         double* __restrict__ st,
+                                                                         const double* __restrict__ sums) {
+    __shared__ double sh[RED_THREADS];
+    const double t = sum_partials(sums, RED_BLOCKS, sh);
+    __syncthreads();
+    const double t2 = sum_partials(sums + RED_BLOCKS, RED_BLOCKS, sh);
+    const double rn = sqrt(t);
+    const bool broken = st[S_BROKEN] != 0.0 || !isfinite(rn);
+    const bool active = st[S_WORKING] != 0.0 && rn > st[S_TOL];
+    const double beta = t2 / (active ? st[S_RZOLD] : 1.0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st[S_RN] = rn;
+        st[S_BROKEN] = broken ? 1.0 : 0.0;
+        st[S_ACTIVE] = active ? 1.0 : 0.0;
+        st[S_CURV] = beta;
+        st[S_RZ] = t2;
+    }
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const double z = r[i] / diag[i];
+        d[i] = active ? z + beta * d[i] : 0.0;
     }
 }
 
@@ -225,71 +284,24 @@ __global__ void __launch_bounds__(RED_THREADS) pcg_residual_kernel(double* __res
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// The scalar steps of the recurrence, one block summing the partials in
-// order. what: 0 curvature -> valid/broken/working/alpha; 1 recurrence norm
-// -> broken/active; 2 r.z -> beta (kept in S_CURV) and rz (restart: active =
-// 1, the direction restarts); 3 true residual norm; 4 initial r.z
-// (restart); 5 = 1 then 2 without restart, from the two sums of
-// pcg_update2_kernel (a restart that follows recomputes r.z with 2); 6 rhs
-// norm -> tolerance max(atol, rtol ||rhs||).
-__global__ void __launch_bounds__(RED_THREADS) pcg_scalar_kernel(int what, int restart,
-                                                                 const double* __restrict__ partials,
+// The scalar steps outside the iteration, one block summing the RED_BLOCKS
+// partials in order: what 2 restart r.z (rz, active = 1, beta = 0); 3 true
+// residual norm; 4 initial r.z (rz, active = 1, not broken); 6 rhs norm ->
+// tolerance max(atol, rtol ||rhs||).
+__global__ void __launch_bounds__(RED_THREADS) pcg_scalar_kernel(int what, const double* __restrict__ partials,
                                                                  double* __restrict__ st, double atol = 0.0,
                                                                  double rtol = 0.0) {
     __shared__ double sh[RED_THREADS];
-    // curvature (what 0) comes from the SPMV_BLOCKS partials of the product
-    double own = 0.0;
-    const int count = what == 0 ? SPMV_BLOCKS : RED_BLOCKS;
-    for (int k = threadIdx.x; k < count; k += RED_THREADS) own += partials[k];
-    const double t = block_sum(own, sh);
-    double t2 = 0.0;
-    if (what == 5) {
-        __syncthreads();
-        t2 = block_sum(partials[RED_BLOCKS + threadIdx.x], sh);
-    }
+    const double t = sum_partials(partials, RED_BLOCKS, sh);
     if (threadIdx.x != 0) return;
     if (what == 6) {
         const double rhs_norm = sqrt(t);
         st[S_RHS] = rhs_norm;
         st[S_TOL] = fmax(atol, rtol * rhs_norm);
-        return;
-    }
-    if (what == 5) {
-        const double rn = sqrt(t);
-        const bool broken = st[S_BROKEN] != 0.0 || !isfinite(rn);
-        const bool active = st[S_WORKING] != 0.0 && rn > st[S_TOL];
-        st[S_RN] = rn;
-        st[S_BROKEN] = broken ? 1.0 : 0.0;
-        st[S_ACTIVE] = active ? 1.0 : 0.0;
-        st[S_CURV] = t2 / (active ? st[S_RZ] : 1.0);  // beta
-        st[S_RZ] = t2;
-        return;
-    }
-    if (what == 0) {
-        const double curvature = t, rz = st[S_RZ];
-        const bool active = st[S_ACTIVE] != 0.0;
-        const bool valid = isfinite(curvature) && curvature > 0.0 && isfinite(rz) && rz > 0.0;
-        const bool broken = st[S_BROKEN] != 0.0 || (active && !valid);
-        const bool working = active && valid && !broken;
-        st[S_CURV] = curvature;
-        st[S_ALPHA] = (working ? rz : 0.0) / (working ? curvature : 1.0);
-        st[S_BROKEN] = broken ? 1.0 : 0.0;
-        st[S_WORKING] = working ? 1.0 : 0.0;
-    } else if (what == 1) {
-        const double rn = sqrt(t);
-        const bool broken = st[S_BROKEN] != 0.0 || !isfinite(rn);
-        st[S_RN] = rn;
-        st[S_BROKEN] = broken ? 1.0 : 0.0;
-        st[S_ACTIVE] = (st[S_WORKING] != 0.0 && rn > st[S_TOL]) ? 1.0 : 0.0;
     } else if (what == 2) {
-        const double next_rz = t;
-        if (restart) {
-            st[S_ACTIVE] = 1.0;
-            st[S_CURV] = 0.0;
-        } else {
-            st[S_CURV] = next_rz / (st[S_ACTIVE] != 0.0 ? st[S_RZ] : 1.0);  // beta
-        }
-        st[S_RZ] = next_rz;
+        st[S_ACTIVE] = 1.0;
+        st[S_CURV] = 0.0;
+        st[S_RZ] = t;
     } else if (what == 3) {
         st[S_TRUE] = sqrt(t);
     } else {
@@ -315,7 +327,9 @@ struct PcgSpace {
     double *r, *d, *Ad, *partials, *state;
 };
 
-constexpr int PARTIALS = SPMV_BLOCKS > 2 * RED_BLOCKS ? SPMV_BLOCKS : 2 * RED_BLOCKS;
+// partials: [0, SPMV_BLOCKS) the product's d . Ad (and the vector
+// reductions' RED_BLOCKS sums), then 2 * RED_BLOCKS for the update's sums.
+constexpr int PARTIALS = SPMV_BLOCKS + 2 * RED_BLOCKS;
 
 size_t pcg_bytes(int64_t rows) {
     const size_t v = (size_t(rows) * 3 * sizeof(double) + 255) & ~size_t(255);
@@ -430,8 +444,8 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
         AFSI_HIP(hipStreamSynchronize(s));
         return OK;
     };
-    auto scalar = [&](int what, int restart) -> int {
-        pcg_scalar_kernel<<<1, RED_THREADS, 0, s>>>(what, restart, w.partials, w.state);
+    auto scalar = [&](int what) -> int {
+        pcg_scalar_kernel<<<1, RED_THREADS, 0, s>>>(what, w.partials, w.state);
         AFSI_LAUNCH_CHECK();
         return OK;
     };
@@ -441,13 +455,13 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
         AFSI_LAUNCH_CHECK();
         pcg_residual_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(w.r, rhs, w.Ad, n, w.partials);
         AFSI_LAUNCH_CHECK();
-        return scalar(3, 0);
+        return scalar(3);
     };
     // rhs norm and tolerance (device), initial iterate, true residual: one
     // readback for both norms.
     pcg_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(2, rhs, rhs, diagonal, n, w.partials);
     AFSI_LAUNCH_CHECK();
-    pcg_scalar_kernel<<<1, RED_THREADS, 0, s>>>(6, 0, w.partials, w.state, opt->atol, opt->rtol);
+    pcg_scalar_kernel<<<1, RED_THREADS, 0, s>>>(6, w.partials, w.state, opt->atol, opt->rtol);
     AFSI_LAUNCH_CHECK();
     if (initial) {
         if (initial != x) AFSI_HIP(hipMemcpyAsync(x, initial, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -470,11 +484,12 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
     // d = z = r / diag, rz = r . z, active, not broken
     pcg_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(1, w.r, w.r, diagonal, n, w.partials);
     AFSI_LAUNCH_CHECK();
-    if (int r = scalar(4, 1)) return r;
+    if (int r = scalar(4)) return r;
     pcg_direction_kernel<<<ge, 256, 0, s>>>(w.d, w.r, diagonal, n, 1, w.state);
     AFSI_LAUNCH_CHECK();
-    // Per iteration: A d with d . Ad, alpha, the update with |r|^2 and r . z,
-    // the norm/active/beta scalars, the direction (five launches).
+    // Per iteration: A d with d . Ad; alpha folded into the update with |r|^2
+    // and r . z; the norm/active/beta step folded into the direction (three
+    // launches). A restart replaces that direction by z afterwards.
     //
     // Check iterations read the device state one check period late: the copy
     // goes to a pinned slot and the host looks at it when it reaches the next
@@ -494,10 +509,12 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
     for (int it = 1; it <= opt->max_iterations; ++it) {
         pcg_spmv_dot_kernel<<<SPMV_BLOCKS, RED_THREADS, 0, s>>>(*a, w.d, w.Ad, w.partials);
         AFSI_LAUNCH_CHECK();
-        if (int r = scalar(0, 0)) return r;
-        pcg_update2_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(x, w.r, w.d, w.Ad, diagonal, n, w.state, w.partials);
+        double* sums = w.partials + SPMV_BLOCKS;
+        pcg_update_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(x, w.r, w.d, w.Ad, diagonal, n, w.state, w.partials,
+                                                             sums);
         AFSI_LAUNCH_CHECK();
-        if (int r = scalar(5, 0)) return r;
+        pcg_next_direction_kernel<<<ge, RED_THREADS, 0, s>>>(w.d, w.r, diagonal, n, w.state, sums);
+        AFSI_LAUNCH_CHECK();
         bool restart = it % opt->recompute_every == 0;
         const bool check = restart || it % opt->check_every == 0 || it == opt->max_iterations;
         if (check) {
@@ -547,13 +564,13 @@ int afsi_mass_pcg_f64(const afsi_csr* a, const double* diagonal, const double* r
                 restart = true;
             }
         }
-        if (restart) {  // r.z of the (true) residual, active = 1
+        if (restart) {  // r.z of the (true) residual, active = 1, d = z
             pcg_dot_kernel<<<RED_BLOCKS, RED_THREADS, 0, s>>>(1, w.r, w.r, diagonal, n, w.partials);
             AFSI_LAUNCH_CHECK();
-            if (int r = scalar(2, 1)) return r;
+            if (int r = scalar(2)) return r;
+            pcg_direction_kernel<<<ge, 256, 0, s>>>(w.d, w.r, diagonal, n, 1, w.state);
+            AFSI_LAUNCH_CHECK();
         }
-        pcg_direction_kernel<<<ge, 256, 0, s>>>(w.d, w.r, diagonal, n, restart ? 1 : 0, w.state);
-        AFSI_LAUNCH_CHECK();
     }
     info->status = 2;
     info->iterations = opt->max_iterations;
