@@ -213,3 +213,33 @@ def test_fe_ib_step_with_full_flow_step_matches_oracle(gpu):
         err_u = max(rel(u[c].cpu(), u_ref[c]) for c in range(3))
         print('full FE step', k, err_U, err_u, info['pressure']['courant'])
         assert err_U < 1e-9 and err_u < 1e-9
+
+
+@pytest.mark.parametrize('check,recompute', [(1, 40), (3, 2), (4, 5), (4, 12), (5, 3), (7, 7)])
+def test_mass_pcg_schedules_match_oracle(gpu, check, recompute):
+    """The PCG reads each check one check period late and restarts and its
+    last iteration synchronously; across check and restart schedules
+    (restarts before, at and between checks) the iteration count, the status
+    and the result equal the oracle's recurrence (fluid/solvers.py:39-122)."""
+    g, X = mesh('p2')
+    grid, _ = grids()
+    tr = FETransfer(grid, on_device(g, gpu))
+    M = ofe.mass_matrix(g)
+    rhs = np.sin(5 * X) + .3 * np.cos(2 * X[:, ::-1])
+    for rtol, limit in ((1e-12, 500), (1e-14, 9)):
+        opts = SolverOptions(rtol=rtol, atol=1e-16, recompute_every=recompute, check_every=check,
+                             max_iterations=limit)
+        oopts = ofe.SolverOptions(rtol=rtol, atol=1e-16, recompute_every=recompute, check_every=check,
+                                  max_iterations=limit)
+        solver = MassSolver(tr.mass, tr.diagonal, opts)
+        try:
+            expected, einfo = ofe.pcg(lambda v: M @ v, rhs, ofe.mass_diagonal(g), None, oopts)
+        except ofe.PCGError as err:
+            with pytest.raises(RuntimeError, match='converge'):
+                solver.solve(torch.tensor(rhs, device=gpu))
+            print('schedule', check, recompute, limit, 'both fail:', str(err)[:40], flush=True)
+            continue
+        result, info = solver.solve(torch.tensor(rhs, device=gpu))
+        print('schedule', check, recompute, limit, info.iterations, einfo.iterations, flush=True)
+        assert info.iterations == einfo.iterations
+        assert rel(result.cpu(), expected) < 1e-10
