@@ -85,7 +85,7 @@ class FEIBStep:
 
     def __init__(self, grid, geometry, *, dt, rho=1., mu=1., device='cuda', options=None, transfer_options=None,
                  warm_start=False, max_displacement=.25, tile=DEFAULT_TILE, pressure_backend='graph',
-                 precision='fp64', predictor='force'):
+                 precision='fp64', predictor='force', fused=False):
         from .mac.fe import FETransfer
         if not max_displacement > 0 or max_displacement != max_displacement:
             raise ValueError('max_displacement must be finite and positive')
@@ -99,7 +99,8 @@ class FEIBStep:
         self.flow = MACFlow(grid, dt=dt, rho=rho, mu=mu, device=device, options=self.options,
                             pressure_backend=pressure_backend, precision=precision,
                             predictor=predictor == 'full')
-        self.transfer = FETransfer(grid, geometry, options=transfer_options, warm_start=warm_start, tile=tile)
+        self.transfer = FETransfer(grid, geometry, options=transfer_options, warm_start=warm_start, tile=tile,
+                                   fused=fused)
         self.rhs = torch.empty(grid.shape, device=self.device, dtype=torch.float64)
         self._h = torch.tensor(grid.spacing, device=self.device, dtype=torch.float64)
 
