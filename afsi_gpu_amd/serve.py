@@ -99,6 +99,9 @@ class Stats:
     admitted: int = 0
     waits_for_memory: int = 0
     wall: float = 0.
+    admit_seconds: list = field(default_factory=list)  # instance creation (allocation, setup, upload)
+    execute_seconds: list = field(default_factory=list)  # one batched GPU iteration
+    waits: list = field(default_factory=list)  # arrival to the start of the request's iteration
 
     def summary(self):
         lat = np.asarray(self.latencies) * 1e3
@@ -110,7 +113,12 @@ class Stats:
                     mean_batch=float(np.mean(self.batch_sizes)) if self.batch_sizes else 0.,
                     max_batch=int(max(self.batch_sizes)) if self.batch_sizes else 0,
                     instances=self.admitted, peak_resident_gib=self.peak_bytes / 2 ** 30,
-                    memory_waits=self.waits_for_memory, wall_s=self.wall)
+                    memory_waits=self.waits_for_memory, wall_s=self.wall,
+                    admit_ms_mean=1e3 * float(np.mean(self.admit_seconds)) if self.admit_seconds else 0.,
+                    admit_ms_max=1e3 * float(max(self.admit_seconds)) if self.admit_seconds else 0.,
+                    execute_ms_p50=1e3 * float(np.percentile(self.execute_seconds, 50)) if self.execute_seconds else 0.,
+                    execute_ms_p99=1e3 * float(np.percentile(self.execute_seconds, 99)) if self.execute_seconds else 0.,
+                    wait_ms_p99=1e3 * float(np.percentile(self.waits, 99)) if self.waits else 0.)
 
 
 class Server:
@@ -210,7 +218,9 @@ class Server:
                         stats.waits_for_memory += 1
                         busy.add(iid)
                         continue
+                    a0 = clock()
                     self.resident[iid] = self._create(specs[iid], data.pop(iid))
+                    stats.admit_seconds.append(clock() - a0)
                     self.resident_bytes[iid] = need
                     used += need
                     stats.admitted += 1
@@ -221,13 +231,16 @@ class Server:
             if not batch:
                 time.sleep(0.0005)
                 continue
+            start = clock() - t0
             self._execute(batch)
             done = clock() - t0
+            stats.execute_seconds.append(done - start)
             stats.iterations += 1
             stats.batch_sizes.append(len(batch))
             for j in chosen:
                 arr, iid = pending[j]
                 stats.latencies.append(done - arr)
+                stats.waits.append(start - arr)
                 remaining[iid] -= 1
                 if remaining[iid] == 0 and iid not in keep:
                     del self.resident[iid]
