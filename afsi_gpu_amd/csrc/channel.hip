@@ -338,6 +338,155 @@ int ch_prolong(const T* coarse, T* fine, const int64_t n[2], int periodic, hipSt
 
 }  // namespace
 
+// ---- whole V-cycles in one workgroup ----------------------------------------
+// For grids up to CHB_MAX_CELLS cells the launches of a V-cycle (a few
+// microseconds each) cost more than their work. One 1024-thread workgroup
+// then runs `cycles` complete V-cycles: every phase of the multi-launch
+// driver below becomes a block-strided loop over the same cells with the
+// same per-cell expressions (ch_apply, ch_diag, ch_coarse_value; the coarse
+// product with one wave per row and the same lane order and xor butterfly),
+// separated by __syncthreads(), which orders the global-memory writes of one
+// phase before the reads of the next within the workgroup. Results are the
+// multi-launch driver's bit for bit, except the periodic mean removal, whose
+// sum is taken in the workgroup's own fixed order.
+constexpr int CHB_THREADS = 1024;
+constexpr int64_t CHB_MAX_CELLS = 256 * 64;  // measured crossover: 2.05 vs 3.02 ms at 256x64, 6.4 vs 3.6 ms at 512x128
+
+template <class T>
+__device__ void chb_remove_mean(T* x, int n, double* red) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += CHB_THREADS) v += double(x[i]);
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = CHB_THREADS / 2; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const T mean = T(red[0] / double(n));
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += CHB_THREADS) x[i] = x[i] - mean;
+    __syncthreads();
+}
+
+template <class T, bool P>
+__device__ void chb_smooth(const afsi_mg2d_plan& pl, int l, double* red) {
+    const Dim2 d{int(pl.n[l][0]), int(pl.n[l][1])};
+    const Coef2<T> k = Coef2<T>{T(pl.h2[l][0]), T(pl.h2[l][1]), T(pl.inv_h2[l][0]), T(pl.inv_h2[l][1])};
+    T* p = static_cast<T*>(pl.p[l]);
+    const T* rhs = static_cast<const T*>(pl.rhs[l]);
+    const int n = d.nx * d.ny;
+    if (pl.smoother == 1) {
+        for (int it = 0; it < pl.nu; ++it)
+            for (int color = 0; color < 2; ++color) {
+                if (d.ny % 2 == 0) {
+                    const int hy = d.ny / 2, half = d.nx * hy;
+                    for (int e = threadIdx.x; e < half; e += CHB_THREADS) {
+                        const int x = e / hy, y = 2 * (e - x * hy) + ((x + color) & 1);
+                        const int i = x * d.ny + y;
+                        T c;
+                        const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
+                        p[i] = c + r / ch_diag<T, P>(x, y, d, k);
+                    }
+                } else {
+                    for (int i = threadIdx.x; i < n; i += CHB_THREADS) {
+                        const int x = i / d.ny, y = i - x * d.ny;
+                        if (((x + y) & 1) != color) continue;
+                        T c;
+                        const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
+                        p[i] = c + r / ch_diag<T, P>(x, y, d, k);
+                    }
+                }
+                __syncthreads();
+            }
+    } else {
+        T* a = p;
+        T* b = static_cast<T*>(pl.tmp[l]);
+        const T omega = T(pl.omega);
+        for (int it = 0; it < pl.nu; ++it) {
+            for (int i = threadIdx.x; i < n; i += CHB_THREADS) {
+                const int x = i / d.ny, y = i - x * d.ny;
+                T c;
+                const T r = rhs[i] - ch_apply<T, P>(a, x, y, d, k, c);
+                b[i] = c + (omega * r) / ch_diag<T, P>(x, y, d, k);
+            }
+            __syncthreads();
+            T* t = a; a = b; b = t;
+        }
+        if (a != p) {
+            for (int i = threadIdx.x; i < n; i += CHB_THREADS) p[i] = a[i];
+            __syncthreads();
+        }
+    }
+    if (P) chb_remove_mean(p, n, red);
+}
+
+template <class T, bool P>
+__global__ void __launch_bounds__(CHB_THREADS) ch_block_cycles_kernel(afsi_mg2d_plan pl, int cycles) {
+    __shared__ double red[CHB_THREADS];
+    const int last = pl.levels - 1;
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    for (int cyc = 0; cyc < cycles; ++cyc) {
+        for (int l = 0; l < last; ++l) {
+            chb_smooth<T, P>(pl, l, red);
+            const Dim2 d{int(pl.n[l][0]), int(pl.n[l][1])};
+            const Coef2<T> k = Coef2<T>{T(pl.h2[l][0]), T(pl.h2[l][1]), T(0), T(0)};
+            const T* p = static_cast<const T*>(pl.p[l]);
+            const T* rhs = static_cast<const T*>(pl.rhs[l]);
+            T* coarse = static_cast<T*>(pl.rhs[l + 1]);
+            const int cy = d.ny / 2, nc = (d.nx / 2) * cy;
+            for (int i = threadIdx.x; i < nc; i += CHB_THREADS) {
+                const int X = i / cy, Y = i - X * cy;
+                const int x = 2 * X, y = 2 * Y, j = x * d.ny + y;
+                T c;
+                const T r00 = rhs[j] - ch_apply<T, P>(p, x, y, d, k, c);
+                const T r01 = rhs[j + 1] - ch_apply<T, P>(p, x, y + 1, d, k, c);
+                const T r10 = rhs[j + d.ny] - ch_apply<T, P>(p, x + 1, y, d, k, c);
+                const T r11 = rhs[j + d.ny + 1] - ch_apply<T, P>(p, x + 1, y + 1, d, k, c);
+                coarse[i] = (((r00 + r01) + r10) + r11) * T(0.25);
+            }
+            __syncthreads();
+            if (P) chb_remove_mean(coarse, nc, red);
+            T* pc = static_cast<T*>(pl.p[l + 1]);
+            for (int i = threadIdx.x; i < nc; i += CHB_THREADS) pc[i] = T(0);
+            __syncthreads();
+        }
+        {   // coarsest: p = inverse * rhs, one wave per row
+            const int m = int(pl.coarse_cells);
+            const T* A = static_cast<const T*>(pl.coarse_inverse);
+            const T* xv = static_cast<const T*>(pl.rhs[last]);
+            T* y = static_cast<T*>(pl.p[last]);
+            for (int r = wave; r < m; r += CHB_THREADS / WAVE) {
+                T acc = T(0);
+                for (int c = lane; c < m; c += WAVE) acc += A[int64_t(r) * m + c] * xv[c];
+#pragma unroll
+                for (int dd = 32; dd >= 1; dd >>= 1) acc += __shfl_xor(acc, dd, WAVE);
+                if (lane == 0) y[r] = acc;
+            }
+            __syncthreads();
+            if (P) chb_remove_mean(y, m, red);
+        }
+        for (int l = last - 1; l >= 0; --l) {
+            const Dim2 d{int(pl.n[l][0]), int(pl.n[l][1])};
+            const T* coarse = static_cast<const T*>(pl.p[l + 1]);
+            T* fine = static_cast<T*>(pl.p[l]);
+            const int n = d.nx * d.ny, NX = d.nx / 2, NY = d.ny / 2;
+            for (int i = threadIdx.x; i < n; i += CHB_THREADS) {
+                const int x = i / d.ny, y = i - x * d.ny;
+                const int ax = x / 2 - 1 + (x & 1), ay = y / 2 - 1 + (y & 1);
+                const T wx = (x & 1) ? T(0.25) : T(0.75), wy = (y & 1) ? T(0.25) : T(0.75);
+                const T c00 = ch_coarse_value<T, P>(coarse, ax, ay, NX, NY);
+                const T c01 = ch_coarse_value<T, P>(coarse, ax, ay + 1, NX, NY);
+                const T c10 = ch_coarse_value<T, P>(coarse, ax + 1, ay, NX, NY);
+                const T c11 = ch_coarse_value<T, P>(coarse, ax + 1, ay + 1, NX, NY);
+                const T v = (c00 * (T(1) - wy) + c01 * wy) * (T(1) - wx) + (c10 * (T(1) - wy) + c11 * wy) * wx;
+                fine[i] = fine[i] + v;
+            }
+            __syncthreads();
+            chb_smooth<T, P>(pl, l, red);
+        }
+    }
+}
+
 // ---- native V-cycle driver ---------------------------------------------------
 
 template <class T>
@@ -449,6 +598,13 @@ template <class T>
 static int channel_cycles(const afsi_mg2d_plan* pl, int32_t cycles, void* stream) {
     if (int r = channel_check<T>(pl)) return r;
     AFSI_REQUIRE(cycles >= 0, "negative cycle count");
+    if (pl->n[0][0] * pl->n[0][1] <= CHB_MAX_CELLS) {
+        if (cycles == 0) return OK;
+        if (pl->periodic) ch_block_cycles_kernel<T, true><<<1, CHB_THREADS, 0, hipStream_t(stream)>>>(*pl, cycles);
+        else ch_block_cycles_kernel<T, false><<<1, CHB_THREADS, 0, hipStream_t(stream)>>>(*pl, cycles);
+        AFSI_LAUNCH_CHECK();
+        return OK;
+    }
     const ChannelRunner<T> run{*pl, hipStream_t(stream)};
     for (int c = 0; c < cycles; ++c)
         if (int r = run.cycle(0)) return r;
