@@ -85,9 +85,10 @@ class PCGError(RuntimeError):
     pass
 
 
-def pcg(action, rhs, diagonal, initial=None, options=SolverOptions()):
+def pcg(action, rhs, diagonal, initial=None, options=SolverOptions(), history=None):
     """Jacobi PCG with the reference's recurrence, check/restart schedule and
-    true-residual acceptance (fluid/solvers.py:39-122, no fixed DOFs)."""
+    true-residual acceptance (fluid/solvers.py:39-122, no fixed DOFs).
+    ``history`` (a list) receives the recurrence norm of every iteration."""
     norm = lambda v: float(np.linalg.norm(v))
     rhs_norm = norm(rhs)
     tol = max(options.atol, options.rtol * rhs_norm)
@@ -110,6 +111,8 @@ def pcg(action, rhs, diagonal, initial=None, options=SolverOptions()):
         correction = correction + alpha * direction
         residual = residual - alpha * Ad
         recurrence_norm = norm(residual)
+        if history is not None:
+            history.append(recurrence_norm)
         broken = broken or not isfinite(recurrence_norm)
         active = working and recurrence_norm > tol
         restart = iteration % options.recompute_every == 0
