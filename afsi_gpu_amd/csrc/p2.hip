@@ -35,6 +35,7 @@ constexpr int P2_CHUNK = 64;     // points staged per pass
 constexpr int P2_PATCH = 1536;   // LDS patch capacity (values, three components)
 constexpr int P2_BLOCK = 4;      // consecutive elements per spread workgroup
 constexpr int P2_SPREAD_PATCH = 2304;
+static_assert(P2_BLOCK * P2_NODES * 3 <= P2_THREADS, "one pass loads the block's nodal values");
 
 struct Lattice {
     double origin[3];  // faces along their own axis; centres start at origin + h/2
