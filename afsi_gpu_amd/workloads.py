@@ -268,6 +268,20 @@ def p2_values(points):
     return np.concatenate([L * (2 * L - 1), 4 * L[:, i] * L[:, j]], 1)
 
 
+def lv_p1(rule=4, **shape):
+    """The idealised LV wall of ``lv_p2`` as straight P1 tetrahedra (vertices
+    only) with the ``rule``^3-point collapsed Gauss rule and the barycentric
+    basis. Returns (nodes (V,3), cells (E,4) int64, values N (Q,4), weights W (E,Q))."""
+    nodes, cells, _, weights = lv_p2(rule=rule, **shape)
+    tets = cells[:, :4]
+    used = np.unique(tets)
+    remap = np.full(len(nodes), -1, dtype=np.int64)
+    remap[used] = np.arange(len(used))
+    pts, _ = collapsed_gauss_tet(rule)
+    values = np.concatenate([1 - pts.sum(1, keepdims=True), pts], 1)
+    return nodes[used], remap[tets], values, weights
+
+
 def lv_p2(nt=3, ntheta=48, nphi=70, rule=4, inner=(.36, .24), outer=(.42, .30), centre=(.5, .5, .5),
           theta=(.15, .6 * np.pi)):
     """Idealised left-ventricle wall of straight P2 tetrahedra: the truncated

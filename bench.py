@@ -72,7 +72,8 @@ def parse():
     ap.add_argument('--fe-level', type=int, default=7,
                     help='sphere subdivision level of the FE interaction-point sample (0: skip)')
     ap.add_argument('--no-p2', action='store_true', help='skip the P2 fused-transfer sample')
-    ap.add_argument('--only', choices=('p2', 'fe', 'batched', 'serving'),
+    ap.add_argument('--no-assembled', action='store_true', help='skip the CSR-assembled P1 sample')
+    ap.add_argument('--only', choices=('p2', 'fe', 'batched', 'serving', 'assembled'),
                     help='run one secondary sample alone and print its JSON (for profiling)')
     return ap.parse_args()
 
@@ -323,6 +324,59 @@ def p2_sample(torch, n=64, reps=10, warmup=2):
                 gather_roofline=roof(spread_bytes, fused_ms['gather_kernel']))
 
 
+def assembled_sample(torch, n=128, reps=5, warmup=1):
+    """CSR-assembled P1 operator (AssembledTransfer) on the idealised LV wall
+    as P1 tetrahedra with the 64-point rule (60,480 elements, 3,870,720
+    interaction points) in an n^3 closed unit box: the CSR build (prepare:
+    points, compact tables, per-cell contraction into the hash table, sort,
+    B and B^T), spread and interpolate with the mass solves, against the
+    tiled FE transfer on the same inputs."""
+    from afsi_gpu_amd import workloads
+    from afsi_gpu_amd.mac import MACGrid
+    from afsi_gpu_amd.mac.assembled import AssembledTransfer
+    from afsi_gpu_amd.mac.fe import FEGeometry, FETransfer
+    device = torch.device('cuda:0')
+    nodes, cells, values, weights = workloads.lv_p1()
+    t = lambda a, dt=torch.float64: torch.tensor(a, device=device, dtype=dt)
+    geometry = FEGeometry(t(cells, torch.int64), t(values), t(weights), len(nodes))
+    grid = MACGrid((n,) * 3, (1., 1., 1.))
+    gen = workloads.rng(workloads.GEOMETRY_SEED + 8)
+    x = t(nodes)
+    F = t(gen.normal(size=nodes.shape) * (weights.sum() / len(nodes)))
+    coords = [grid.coordinates(c, device=device) for c in range(3)]
+    velocity = tuple(torch.sin(2 * np.pi * coords[c][..., (c + 1) % 3]) for c in range(3))
+
+    def timed(fn):
+        for _ in range(warmup):
+            out = fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / reps, out
+
+    out = {}
+    for name, make in (('assembled', lambda: AssembledTransfer(grid, geometry, max_entries=128_000_000)),
+                       ('tiled', lambda: FETransfer(grid, geometry))):
+        tr = make()
+        ms = {}
+        ms['prepare'], st = timed(lambda: tr.prepare(x))
+        ms['spread_with_mass'], _ = timed(lambda: tr.spread(F, st))
+        ms['interp_with_mass'], _ = timed(lambda: tr.interpolate(velocity, st))
+        ms['mass_solve'], _ = timed(lambda: tr.solve_mass(F, None))
+        out[name] = dict(ms=ms)
+        if name == 'assembled':
+            out[name]['nnz'] = st.assembly['total_nnz']
+            out[name]['hash_attempts'] = [c['hash_attempts'] for c in st.assembly['components']]
+            out[name]['csr_bytes'] = st.storage_bytes
+        del st, tr
+        torch.cuda.empty_cache()
+    P = cells.shape[0] * values.shape[0]
+    return dict(workload=f'idealised LV, P1 tetrahedra, 64-point rule: {cells.shape[0]} elements, {len(nodes)} '
+                         f'nodes, {P} interaction points in {n}^3 closed unit box', points=P, **out)
+
+
 def batched_sample(torch, instances, steps=3, warmup=1):
     """Config 4: `instances` independent instances, grids uniform over
     {32^3, 48^3, 64^3, 96^3, 128x32, 256x64, 512x128}, points log-uniform in
@@ -435,6 +489,7 @@ def main():
         return
     if args.only:
         run = dict(p2=lambda: p2_sample(torch), fe=lambda: fe_sample(torch, args.n, args.fe_level),
+                   assembled=lambda: assembled_sample(torch),
                    batched=lambda: batched_sample(torch, args.batch_instances),
                    serving=lambda: serving_sample(torch, args.serve_seconds))[args.only]
         print(json.dumps({args.only: run()}))
@@ -500,6 +555,8 @@ def main():
         result['fe'] = fe_sample(torch, n, args.fe_level)
     if not args.no_p2:
         result['p2'] = p2_sample(torch)
+    if not args.no_assembled:
+        result['assembled'] = assembled_sample(torch)
     if not args.no_cpu_baseline:
         result['cpu_baseline'] = cpu_baseline(x_np, F_np, n, int(round(plain['mg_cycles_mean'])))
     print(json.dumps(result))
