@@ -4,10 +4,13 @@ oracle/fe.py, and the conservation checks of the reference's own test
 velocity, support and invalid-field errors, PCG restart/breakdown paths.
 
 Bars: B, W B, B^T W and the mass action within 1e-13 relative; the PCG
-takes the oracle's iteration count and agrees within 1e-10; transferred
+stops at an iteration the oracle reaches on the mass matrix moved by a few
+units in the last place (the oracle's own count wherever that is unique)
+and agrees within 1e-10; transferred
 fields within 1e-10 (the PCG tolerance is 1e-12 relative)."""
 import numpy as np
 import pytest
+import scipy.sparse as sp
 import torch
 
 from oracle import fe as ofe
@@ -39,15 +42,27 @@ def grids():
             ib3d.Grid3((16, 20, 24), (4., 6., 8.), (-2., -3., -4.)))
 
 
-def same_stop(got, expected, history, tol, period):
-    """The GPU and the oracle stop at the same iteration, or one check
-    period apart where the oracle's recurrence norm at the earlier stop is
-    within 3% of the tolerance: there the outcome is fixed by rounding, as
-    the sums of the GPU and of NumPy differ in their last bits."""
-    if got == expected:
-        return True
-    first = min(got, expected)
-    return abs(got - expected) <= period and abs(history[first - 1] / tol - 1) < .03
+def reachable_stops(M, rhs, diagonal, initial, options, count=16, seed=0):
+    """Iteration counts at which the oracle PCG stops on M and on ``count``
+    symmetric copies of M whose entries are moved by up to two units in the
+    last place. Where the recurrence norm crosses the tolerance at a
+    rounding-decided iteration (long unrestarted runs lose orthogonality:
+    on the P2 box mesh with no restart the stop moves by one iteration under
+    such perturbations at every rtol from 1e-8 to 1e-12) the set holds more
+    than one count; with restarts every few iterations it holds one, and the
+    GPU must take exactly that count."""
+    M = sp.csr_matrix(M)
+    rng = np.random.default_rng(seed)
+    ulps = lambda a: rng.integers(-2, 3, size=a.shape) * np.spacing(np.abs(a))
+    stops = set()
+    for k in range(count + 1):
+        P = M
+        if k:
+            upper = sp.triu(M, 1).tocsr()
+            upper.data = upper.data + ulps(upper.data)
+            P = (upper + upper.T + sp.diags(M.diagonal() + ulps(M.diagonal()))).tocsr()
+        stops.add(ofe.pcg(lambda v: P @ v, rhs, diagonal, initial, options)[1].iterations)
+    return stops
 
 
 def rel(a, b):
@@ -81,12 +96,11 @@ def test_mass_pcg_matches_oracle_and_error_paths(gpu):
     solver = MassSolver(tr.mass, tr.diagonal, opts)
     rhs, initial = np.sin(5 * X), .1 * np.cos(X)
     M = ofe.mass_matrix(g)
-    hist = []
-    expected, einfo = ofe.pcg(lambda v: M @ v, rhs, ofe.mass_diagonal(g), initial, oopts, hist)
+    expected, einfo = ofe.pcg(lambda v: M @ v, rhs, ofe.mass_diagonal(g), initial, oopts)
     rt, it = torch.tensor(rhs, device=gpu), torch.tensor(initial, device=gpu)
     result, info = solver.solve(rt, it)
     print('PCG iterations', info.iterations, 'oracle', einfo.iterations)
-    assert same_stop(info.iterations, einfo.iterations, hist, einfo.tolerance, 4)
+    assert info.iterations in reachable_stops(M, rhs, ofe.mass_diagonal(g), initial, oopts)
     assert rel(result.cpu(), expected) < 1e-10
     assert np.linalg.norm(rhs - M @ result.cpu().numpy()) <= info.tolerance * (1 + 1e-9)
     assert abs(info.tolerance / einfo.tolerance - 1) < 1e-12
@@ -232,7 +246,8 @@ def test_mass_pcg_schedules_match_oracle(gpu, check, recompute):
     """The PCG reads each check one check period late and restarts and its
     last iteration synchronously; across check and restart schedules
     (restarts before, at and between checks) the iteration count, the status
-    and the result equal the oracle's recurrence (fluid/solvers.py:39-122)."""
+    and the result follow the oracle's recurrence (fluid/solvers.py:39-122);
+    the count is one the oracle reaches under last-place perturbations."""
     g, X = mesh('p2')
     grid, _ = grids()
     tr = FETransfer(grid, on_device(g, gpu))
@@ -244,9 +259,8 @@ def test_mass_pcg_schedules_match_oracle(gpu, check, recompute):
         oopts = ofe.SolverOptions(rtol=rtol, atol=1e-16, recompute_every=recompute, check_every=check,
                                   max_iterations=limit)
         solver = MassSolver(tr.mass, tr.diagonal, opts)
-        hist = []
         try:
-            expected, einfo = ofe.pcg(lambda v: M @ v, rhs, ofe.mass_diagonal(g), None, oopts, hist)
+            expected, einfo = ofe.pcg(lambda v: M @ v, rhs, ofe.mass_diagonal(g), None, oopts)
         except ofe.PCGError as err:
             with pytest.raises(RuntimeError, match='converge'):
                 solver.solve(torch.tensor(rhs, device=gpu))
@@ -254,5 +268,5 @@ def test_mass_pcg_schedules_match_oracle(gpu, check, recompute):
             continue
         result, info = solver.solve(torch.tensor(rhs, device=gpu))
         print('schedule', check, recompute, limit, info.iterations, einfo.iterations, flush=True)
-        assert same_stop(info.iterations, einfo.iterations, hist, einfo.tolerance, check)
+        assert info.iterations in reachable_stops(M, rhs, ofe.mass_diagonal(g), None, oopts)
         assert rel(result.cpu(), expected) < 1e-10
