@@ -263,10 +263,10 @@ __device__ inline T prolong32(const T* __restrict__ coarse, int x, int y, int z,
                               int C2) {
     const int ax = x / 2 - 1 + x % 2, ay = y / 2 - 1 + y % 2, az = z / 2 - 1 + z % 2;
     int x0, x1, y0, y1, z0, z1;
-    if (P) {
-        x0 = wrap32(ax, C0); x1 = wrap32(ax + 1, C0);
-        y0 = wrap32(ay, C1); y1 = wrap32(ay + 1, C1);
-        z0 = wrap32(az, C2); z1 = wrap32(az + 1, C2);
+    if (P) {  // -1 <= ax and ax + 1 <= C0: one conditional wrap each
+        x0 = ax < 0 ? ax + C0 : ax; x1 = ax + 1 >= C0 ? ax + 1 - C0 : ax + 1;
+        y0 = ay < 0 ? ay + C1 : ay; y1 = ay + 1 >= C1 ? ay + 1 - C1 : ay + 1;
+        z0 = az < 0 ? az + C2 : az; z1 = az + 1 >= C2 ? az + 1 - C2 : az + 1;
     } else {
         x0 = ax > 0 ? ax : 0; x1 = ax + 1 < C0 - 1 ? ax + 1 : C0 - 1;
         y0 = ay > 0 ? ay : 0; y1 = ay + 1 < C1 - 1 ? ay + 1 : C1 - 1;
@@ -748,9 +748,9 @@ __device__ __forceinline__ void prolong_row(const T* __restrict__ coarse, T* __r
     const int C0 = d.n0 / 2, C1 = d.n1 / 2, C2 = d.n2 / 2;
     const int ax = x / 2 - 1 + (x & 1), ay = y / 2 - 1 + (y & 1);
     int x0, x1, y0, y1;
-    if (P) {
-        x0 = wrap32(ax, C0); x1 = wrap32(ax + 1, C0);
-        y0 = wrap32(ay, C1); y1 = wrap32(ay + 1, C1);
+    if (P) {  // -1 <= ax and ax + 1 <= C0
+        x0 = ax < 0 ? ax + C0 : ax; x1 = ax + 1 >= C0 ? ax + 1 - C0 : ax + 1;
+        y0 = ay < 0 ? ay + C1 : ay; y1 = ay + 1 >= C1 ? ay + 1 - C1 : ay + 1;
     } else {
         x0 = ax > 0 ? ax : 0; x1 = ax + 1 < C0 - 1 ? ax + 1 : C0 - 1;
         y0 = ay > 0 ? ay : 0; y1 = ay + 1 < C1 - 1 ? ay + 1 : C1 - 1;
@@ -797,9 +797,9 @@ __global__ void __launch_bounds__(256) prolong_groups_kernel(const T* __restrict
     if (g >= (C0 + 1) * (C1 + 1)) return;
     const int ax = g / (C1 + 1) - 1, ay = g - (ax + 1) * (C1 + 1) - 1;
     int x0, x1, y0, y1;
-    if (P) {
-        x0 = wrap32(ax, C0); x1 = wrap32(ax + 1, C0);
-        y0 = wrap32(ay, C1); y1 = wrap32(ay + 1, C1);
+    if (P) {  // -1 <= ax and ax + 1 <= C0
+        x0 = ax < 0 ? ax + C0 : ax; x1 = ax + 1 >= C0 ? ax + 1 - C0 : ax + 1;
+        y0 = ay < 0 ? ay + C1 : ay; y1 = ay + 1 >= C1 ? ay + 1 - C1 : ay + 1;
     } else {
         x0 = ax > 0 ? ax : 0; x1 = ax + 1 < C0 - 1 ? ax + 1 : C0 - 1;
         y0 = ay > 0 ? ay : 0; y1 = ay + 1 < C1 - 1 ? ay + 1 : C1 - 1;
@@ -1146,6 +1146,21 @@ __device__ __forceinline__ void coarse_cycle_body(const CoarseArgs<T>& a) {
     __shared__ uint32_t lists[COARSE_LIST];
     __shared__ int list_off[AFSI_MG_MAX_LEVELS];
     const int L = a.levels;
+    // The first level's rhs and initial iterate (at most COARSE_MAX cells)
+    // are loaded into registers first, so the loads' latency overlaps the
+    // cell-list build; they reach LDS after it.
+    constexpr int LOADS = int(COARSE_MAX / COARSE_THREADS);
+    static_assert(LOADS * COARSE_THREADS == COARSE_MAX, "whole loads per thread");
+    T rv[LOADS], pv[LOADS];
+    {
+        const int n = a.d[0].n0 * a.d[0].n1 * a.d[0].n2;
+#pragma unroll
+        for (int q = 0; q < LOADS; ++q) {
+            const int i = int(threadIdx.x) + q * COARSE_THREADS;
+            rv[q] = i < n ? a.rhs_in[i] : T(0);
+            pv[q] = i < n && a.p_in ? a.p_in[i] : T(0);
+        }
+    }
     if (threadIdx.x == 0) {
         int off = 0;
         for (int l = 0; l < L; ++l) {
@@ -1160,13 +1175,16 @@ __device__ __forceinline__ void coarse_cycle_body(const CoarseArgs<T>& a) {
     __syncthreads();
     for (int l = 0; l < L; ++l) build_list(lists + list_off[l], a.d[l]);
     {
-        const Dim32& d = a.d[0];
-        const int n = d.n0 * d.n1 * d.n2;
+        const int n = a.d[0].n0 * a.d[0].n1 * a.d[0].n2;
         T* p = smem + a.off_p[0];
         T* r = smem + a.off_r[0];
-        for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
-            r[i] = a.rhs_in[i];
-            p[i] = a.p_in ? a.p_in[i] : T(0);
+#pragma unroll
+        for (int q = 0; q < LOADS; ++q) {
+            const int i = int(threadIdx.x) + q * COARSE_THREADS;
+            if (i < n) {
+                r[i] = rv[q];
+                p[i] = pv[q];
+            }
         }
         __syncthreads();
     }
