@@ -36,12 +36,6 @@ int launch_reduce_final(const double* partials, int64_t nparts, double* result, 
 constexpr int COARSE_THREADS = 1024;
 constexpr int64_t COARSE_MAX = 4096;  // cells of the finest level solved in LDS
 constexpr int COARSE_LIST = 4800;     // packed cell descriptors of all LDS levels
-// Largest level whose prolongation runs inside its first post-smoothing
-// sweep. Measured on MI355X, 128^3 periodic, per V-cycle: fusing at 64^3
-// and 32^3 saves ~3 us (FP32 141.4 -> 138.1 us); fusing at 128^3 costs
-// 16 us in FP64 (the tile box plus the coarse box hold one workgroup per
-// CU) and gains nothing in FP32.
-constexpr int64_t FUSED_PROLONG_MAX = 64 * 64 * 64;
 
 struct Dim32 {
     int n0, n1, n2;
@@ -378,34 +372,12 @@ struct RowRec {
     int off, gx, gy, in;
 };
 
-// The prolongation correction of fine cell (x, y, z), formed exactly as
-// prolong_row forms it, from the four coarse rows it interpolates between.
-template <class T, bool P>
-__device__ inline T prolong_rows_at(const T* c00, const T* c01, const T* c10, const T* c11, T wx, T wy,
-                                    int z, int C2) {
-    const int az = z / 2 - 1 + (z & 1);
-    int z0, z1;
-    if (P) { z0 = az < 0 ? az + C2 : az; z1 = az + 1 >= C2 ? az + 1 - C2 : az + 1; }
-    else { z0 = az > 0 ? az : 0; z1 = az + 1 < C2 - 1 ? az + 1 : C2 - 1; }
-    const T wz = (z & 1) ? T(0.25) : T(0.75);
-    const T v00 = c00[z0] * (T(1) - wz) + c00[z1] * wz;
-    const T v01 = c01[z0] * (T(1) - wz) + c01[z1] * wz;
-    const T v10 = c10[z0] * (T(1) - wz) + c10[z1] * wz;
-    const T v11 = c11[z0] * (T(1) - wz) + c11[z1] * wz;
-    return (v00 * (T(1) - wy) + v01 * wy) * (T(1) - wx) + (v10 * (T(1) - wy) + v11 * wy) * wx;
-}
-
-// Tiles first, first + slots, ... (count of them) of one level. PR: the
-// input is the pre-prolongation iterate and `coarse` the coarse-level
-// correction; every staged fine value is first replaced by
-// p + trilinear(coarse) (prolong_row's arithmetic), so the sweep acts on
-// the prolongated iterate without it ever being written (one tile per
-// workgroup, exact tiling, N2 > 0).
-template <class T, bool P, int TX, int TY, int N2, bool PR = false>
+// Tiles first, first + slots, ... (count of them) of one level.
+template <class T, bool P, int TX, int TY, int N2>
 __device__ __forceinline__ void rb_rows_tiles(const T* __restrict__ pin, const T* __restrict__ rhs,
                                               T* __restrict__ pout, const Dim32 d, const W3<T> k,
                                               int zero_in, int ntx, int nty, int first, int count,
-                                              int slots, const T* __restrict__ coarse = nullptr) {
+                                              int slots) {
     constexpr int NW = 4;
     constexpr int BYr = TY + 4, ROWS = (TX + 4) * BYr;
     constexpr int RYr = TY + 2, RING = (TX + 2) * RYr;
@@ -486,77 +458,10 @@ __device__ __forceinline__ void rb_rows_tiles(const T* __restrict__ pin, const T
             }
         }
     };
-    // PR: coarse rows cx0 .. cx0+CXR-1 by cy0 .. cy0+CYR-1 (all coarse z)
-    // of the first tile, staged after the box in LDS (rows outside a
-    // Neumann level are never read and stay unset)
-    static_assert(!PR || N2 > 0, "fused prolongation needs compile-time extents");
-    constexpr int CXR = TX / 2 + 4, CYR = TY / 2 + 4, H2 = N2 > 0 ? N2 / 2 : 1;
-    constexpr int CV = PR ? (CXR * CYR * H2 + 255) / 256 : 1;
-    T* cbox = box + ROWS * n2;
-    const int C0 = d.n0 / 2, C1 = d.n1 / 2;
-    const int fx0 = (first / nty) * TX - 2, fy0 = (first % nty) * TY - 2;  // even
-    const int cx0 = fx0 / 2 - 1, cy0 = fy0 / 2 - 1;
-    T cv[CV];
-    if (PR) {
-#pragma unroll
-        for (int q = 0; q < CV; ++q) {
-            const int e = threadIdx.x + q * 256;
-            cv[q] = T(0);
-            if (e < CXR * CYR * H2) {
-                const int row = e / H2, cz = e - row * H2;
-                int gx = cx0 + row / CYR, gy = cy0 + row % CYR;
-                bool in = true;
-                if (P) {
-                    gx = gx < 0 ? gx + C0 : (gx >= C0 ? gx - C0 : gx);
-                    gy = gy < 0 ? gy + C1 : (gy >= C1 ? gy - C1 : gy);
-                } else {
-                    in = gx >= 0 && gx < C0 && gy >= 0 && gy < C1;
-                }
-                if (in) cv[q] = coarse[(int64_t(gx) * C1 + gy) * H2 + cz];
-            }
-        }
-    }
     build(0, first);
     if (!P && threadIdx.x < 27) tab[threadIdx.x] = inv_diag_entry<T>(threadIdx.x, k);
     __syncthreads();
     fetch(0, first);
-    if (PR) {
-#pragma unroll
-        for (int q = 0; q < CV; ++q) {
-            const int e = threadIdx.x + q * 256;
-            if (e < CXR * CYR * H2) cbox[e] = cv[q];
-        }
-    }
-    // PR: box row r (staged, synchronised) += its prolongation correction
-    auto correct_box = [&]() {
-#pragma unroll 1
-        for (int q = 0; q < LR; ++q) {
-            const int r = wave + q * NW;
-            if (r >= ROWS) continue;
-            const RowRec rc = load_rec(0, r);
-            if (!P && !rc.in) continue;
-            const int xf = fx0 + r / BYr, yf = fy0 + r % BYr;  // unwrapped fine row
-            const int ax = (xf >> 1) - 1 + (xf & 1), ay = (yf >> 1) - 1 + (yf & 1);
-            int lx0, lx1, ly0, ly1;
-            if (P) {
-                lx0 = ax - cx0; lx1 = lx0 + 1;
-                ly0 = ay - cy0; ly1 = ly0 + 1;
-            } else {
-                lx0 = (ax > 0 ? ax : 0) - cx0; lx1 = (ax + 1 < C0 - 1 ? ax + 1 : C0 - 1) - cx0;
-                ly0 = (ay > 0 ? ay : 0) - cy0; ly1 = (ay + 1 < C1 - 1 ? ay + 1 : C1 - 1) - cy0;
-            }
-            const T wx = (xf & 1) ? T(0.25) : T(0.75);
-            const T wy = (yf & 1) ? T(0.25) : T(0.75);
-            const T* c00 = cbox + (lx0 * CYR + ly0) * H2;
-            const T* c01 = cbox + (lx0 * CYR + ly1) * H2;
-            const T* c10 = cbox + (lx1 * CYR + ly0) * H2;
-            const T* c11 = cbox + (lx1 * CYR + ly1) * H2;
-            T* row = box + r * n2;
-            if (lane < n2) row[lane] = row[lane] + prolong_rows_at<T, P>(c00, c01, c10, c11, wx, wy, lane, H2);
-            if (lane + 64 < n2)
-                row[lane + 64] = row[lane + 64] + prolong_rows_at<T, P>(c00, c01, c10, c11, wx, wy, lane + 64, H2);
-        }
-    };
     const T inv_p = T(1) / (T(2) * (k.w[0] + k.w[1] + k.w[2]));
     auto update = [&](const T* row, int r, int gx, int gy, int z, T b) -> T {
         T s = T(0);
@@ -594,10 +499,6 @@ __device__ __forceinline__ void rb_rows_tiles(const T* __restrict__ pin, const T
         const bool more = j + 1 < count;
         if (more) build(cur ^ 1, tile + slots);
         __syncthreads();
-        if (PR) {  // count == 1
-            correct_box();
-            __syncthreads();
-        }
         if (more) fetch(cur ^ 1, tile + slots);
 #pragma unroll
         for (int q = 0; q < RR; ++q) {
@@ -628,12 +529,11 @@ __device__ __forceinline__ void rb_rows_tiles(const T* __restrict__ pin, const T
     }
 }
 
-template <class T, bool P, int TX, int TY, int N2, bool PR = false>
+template <class T, bool P, int TX, int TY, int N2>
 __global__ void __launch_bounds__(256) rb_rows_kernel(const T* __restrict__ pin,
                                                       const T* __restrict__ rhs,
                                                       T* __restrict__ pout, Dim32 d, W3<T> k,
-                                                      int zero_in, int ntx, int nty,
-                                                      const T* __restrict__ coarse = nullptr) {
+                                                      int zero_in, int ntx, int nty) {
     // tile sequence of this workgroup
     const int ntiles = ntx * nty;
     const bool by_xcd = (gridDim.x % 8 == 0) && (ntiles % 8 == 0);
@@ -643,7 +543,7 @@ __global__ void __launch_bounds__(256) rb_rows_kernel(const T* __restrict__ pin,
     const int first = (by_xcd ? int(blockIdx.x % 8) * per : 0) + slot;
     const int count = slot < per ? (per - slot + slots - 1) / slots : 0;
     if (count == 0) return;  // uniform over the workgroup
-    rb_rows_tiles<T, P, TX, TY, N2, PR>(pin, rhs, pout, d, k, zero_in, ntx, nty, first, count, slots, coarse);
+    rb_rows_tiles<T, P, TX, TY, N2>(pin, rhs, pout, d, k, zero_in, ntx, nty, first, count, slots);
 }
 
 // Pointers to a row and its four x/y neighbour rows (null where a Neumann
@@ -809,28 +709,52 @@ __global__ void __launch_bounds__(256) prolong_groups_kernel(const T* __restrict
     const T* c10 = coarse + (x1 * C1 + y0) * C2;
     const T* c11 = coarse + (x1 * C1 + y1) * C2;
     const int fx[2] = {2 * ax + 1, 2 * ax + 2}, fy[2] = {2 * ay + 1, 2 * ay + 2};
-    for (int z = lane; z < d.n2; z += 64) {
-        const int az = z / 2 - 1 + (z & 1);
-        int z0, z1;
-        if (P) { z0 = az < 0 ? az + C2 : az; z1 = az + 1 >= C2 ? az + 1 - C2 : az + 1; }
-        else { z0 = az > 0 ? az : 0; z1 = az + 1 < C2 - 1 ? az + 1 : C2 - 1; }
-        const T wz = (z & 1) ? T(0.25) : T(0.75);
-        const T v00 = c00[z0] * (T(1) - wz) + c00[z1] * wz;
-        const T v01 = c01[z0] * (T(1) - wz) + c01[z1] * wz;
-        const T v10 = c10[z0] * (T(1) - wz) + c10[z1] * wz;
-        const T v11 = c11[z0] * (T(1) - wz) + c11[z1] * wz;
+    // fine rows of the group (wave-uniform; null where outside the level)
+    T* rows[2][2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int x = fx[i];
-            if (x < 0 || x >= d.n0) continue;
-            const T wx = (x & 1) ? T(0.25) : T(0.75);
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int y = fy[j];
-                if (y < 0 || y >= d.n1) continue;
-                const T wy = (y & 1) ? T(0.25) : T(0.75);
-                T* f = fine + (x * d.n1 + y) * d.n2;
-                f[z] = f[z] + ((v00 * (T(1) - wy) + v01 * wy) * (T(1) - wx) + (v10 * (T(1) - wy) + v11 * wy) * wx);
+        for (int j = 0; j < 2; ++j)
+            rows[i][j] = fx[i] >= 0 && fx[i] < d.n0 && fy[j] >= 0 && fy[j] < d.n1
+                             ? fine + (fx[i] * d.n1 + fy[j]) * d.n2 : nullptr;
+    // Two z per lane per pass (z and z + 64): every load of a pass is issued
+    // before its stores, so a 128-wide row takes one memory round trip.
+    for (int zs = 0; zs < d.n2; zs += 128) {
+        T v[2][4], f[2][2][2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int z = zs + lane + 64 * h;
+            if (z >= d.n2) continue;
+            const int az = z / 2 - 1 + (z & 1);
+            int z0, z1;
+            if (P) { z0 = az < 0 ? az + C2 : az; z1 = az + 1 >= C2 ? az + 1 - C2 : az + 1; }
+            else { z0 = az > 0 ? az : 0; z1 = az + 1 < C2 - 1 ? az + 1 : C2 - 1; }
+            const T wz = (z & 1) ? T(0.25) : T(0.75);
+            v[h][0] = c00[z0] * (T(1) - wz) + c00[z1] * wz;
+            v[h][1] = c01[z0] * (T(1) - wz) + c01[z1] * wz;
+            v[h][2] = c10[z0] * (T(1) - wz) + c10[z1] * wz;
+            v[h][3] = c11[z0] * (T(1) - wz) + c11[z1] * wz;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    if (rows[i][j]) f[h][i][j] = rows[i][j][z];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int z = zs + lane + 64 * h;
+            if (z >= d.n2) continue;
+            const T v00 = v[h][0], v01 = v[h][1], v10 = v[h][2], v11 = v[h][3];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const T wx = (fx[i] & 1) ? T(0.25) : T(0.75);
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    if (!rows[i][j]) continue;
+                    const T wy = (fy[j] & 1) ? T(0.25) : T(0.75);
+                    rows[i][j][z] = f[h][i][j] +
+                                    ((v00 * (T(1) - wy) + v01 * wy) * (T(1) - wx) + (v10 * (T(1) - wy) + v11 * wy) * wx);
+                }
             }
         }
     }
@@ -1511,12 +1435,10 @@ __global__ void batch_begin_kernel(int n, const double* __restrict__ results, do
 // Dynamic LDS a kernel may request: the device's per-workgroup LDS minus the
 // kernel's static LDS. The first call per (kernel, device) raises the
 // kernel's dynamic-LDS attribute to that amount; `need` is checked against it.
-// Dynamic LDS a kernel may use on the current device (raising the kernel's
-// limit to it on first use).
-static int lds_available(const void* fn, size_t& avail_out) {
+static int dynamic_lds(const void* fn, size_t need) {
     struct Entry { const void* fn; int dev; size_t avail; };
     static std::mutex lock;
-    static Entry cache[256];
+    static Entry cache[64];
     static int used = 0;
     int dev = 0;
     AFSI_HIP(hipGetDevice(&dev));
@@ -1537,15 +1459,8 @@ static int lds_available(const void* fn, size_t& avail_out) {
         avail = limit > attr.sharedSizeBytes ? limit - attr.sharedSizeBytes : 0;
         AFSI_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(avail)));
         std::lock_guard<std::mutex> g(lock);
-        if (used < 256) cache[used++] = Entry{fn, dev, avail};
+        if (used < 64) cache[used++] = Entry{fn, dev, avail};
     }
-    avail_out = avail;
-    return OK;
-}
-
-static int dynamic_lds(const void* fn, size_t need) {
-    size_t avail = 0;
-    if (int r = lds_available(fn, avail)) return r;
     if (need > avail) {
         set_error("kernel needs %zu bytes of dynamic LDS; the device allows %zu", need, avail);
         return E_LIMIT;
@@ -1603,67 +1518,13 @@ struct FastRunner {
         return rb_rows_n<TXr, TYr, 0>(l, in, out, zero_in);
     }
 
-    // First post-smoothing sweep fused with the prolongation: reads the
-    // pre-prolongation iterate `in` and the coarse correction, writes the
-    // swept prolongated iterate to `out`; one tile per workgroup.
-    // `done` false (nothing launched) where the tile and coarse boxes do not
-    // fit in LDS together
-    template <int TXr, int TYr, int N2>
-    int rb_rows_pr_n(int l, const T* in, const T* coarse, T* out, bool& done) const {
-        const Dim32 d = D(l);
-        const int ntx = d.n0 / TXr, nty = d.n1 / TYr;
-        const size_t bytes =
-            (size_t((TXr + 4) * (TYr + 4)) * N2 + size_t(TXr / 2 + 4) * (TYr / 2 + 4) * (N2 / 2)) * sizeof(T);
-        const void* fn = pl.periodic ? reinterpret_cast<const void*>(rb_rows_kernel<T, true, TXr, TYr, N2, true>)
-                                     : reinterpret_cast<const void*>(rb_rows_kernel<T, false, TXr, TYr, N2, true>);
-        size_t avail = 0;
-        if (int r = lds_available(fn, avail)) return r;
-        done = bytes <= avail;
-        if (!done) return OK;
-        const unsigned g = unsigned(ntx * nty);
-        if (pl.periodic)
-            rb_rows_kernel<T, true, TXr, TYr, N2, true><<<g, 256, bytes, s>>>(in, R(l), out, d, K(l), 0, ntx, nty, coarse);
-        else
-            rb_rows_kernel<T, false, TXr, TYr, N2, true><<<g, 256, bytes, s>>>(in, R(l), out, d, K(l), 0, ntx, nty, coarse);
-        AFSI_LAUNCH_CHECK();
-        return OK;
-    }
-
-    template <int TXr, int TYr>
-    int rb_rows_pr(int l, const T* in, const T* coarse, T* out, bool& done) const {
-        const Dim32 d = D(l);
-        done = d.n0 % TXr == 0 && d.n1 % TYr == 0 && (d.n2 == 128 || d.n2 == 64 || d.n2 == 32);
-        if (!done) return OK;
-        if (d.n2 == 128) return rb_rows_pr_n<TXr, TYr, 128>(l, in, coarse, out, done);
-        if (d.n2 == 64) return rb_rows_pr_n<TXr, TYr, 64>(l, in, coarse, out, done);
-        return rb_rows_pr_n<TXr, TYr, 32>(l, in, coarse, out, done);
-    }
-
-    // tile code of a red-black row sweep at level l (plan choice or default)
-    int sweep_tile_of(int l) const {
-        const Dim32 d = D(l);
-        const int tile = pl.sweep_tile[l];
-        if (tile == 2 || tile == 4 || tile == 8 || tile == 84) return tile;
-        return cdiv(d.n0, 4) * cdiv(d.n1, 4) >= 1024 ? 4 : 2;
-    }
-
-    // prolongation fused into the first post-smoothing sweep where the
-    // level sweeps by exact row tiles; `done` false: caller does both
-    int sweep_prolong(int l, const T* in, const T* coarse, T* out, bool& done) const {
-        done = false;
-        if (pl.smoother != 1 || !rows_ok(l)) return OK;
-        const int tile = sweep_tile_of(l);
-        if (tile == 8) return rb_rows_pr<8, 8>(l, in, coarse, out, done);
-        if (tile == 84) return rb_rows_pr<8, 4>(l, in, coarse, out, done);
-        if (tile == 4) return rb_rows_pr<4, 4>(l, in, coarse, out, done);
-        return rb_rows_pr<2, 2>(l, in, coarse, out, done);
-    }
-
     int sweep(int l, const T* in, T* out, bool zero_in) const {
         const Dim32 d = D(l);
         const W3<T> k = K(l);
         if (pl.smoother == 1 && rows_ok(l)) {
-            const int tile = sweep_tile_of(l);
+            int tile = pl.sweep_tile[l];
+            if (tile != 2 && tile != 4 && tile != 8 && tile != 84)
+                tile = cdiv(d.n0, 4) * cdiv(d.n1, 4) >= 1024 ? 4 : 2;
             if (tile == 8) return rb_rows<8, 8>(l, in, out, zero_in);
             if (tile == 84) return rb_rows<8, 4>(l, in, out, zero_in);
             if (tile == 4) return rb_rows<4, 4>(l, in, out, zero_in);
@@ -1757,20 +1618,6 @@ struct FastRunner {
         AFSI_LAUNCH_CHECK();
         T* cc = P(l + 1);
         if (int r = cycle(l + 1, cc)) return r;
-        if (!((pl.unfused_prolong >> l) & 1) && cells(l) <= FUSED_PROLONG_MAX) {
-            T* out = (cur == P(l)) ? W(l) : P(l);
-            bool done = false;
-            if (int r = sweep_prolong(l, cur, cc, out, done)) return r;
-            if (done) {
-                cur = out;
-                for (int it = 1; it < pl.nu; ++it) {
-                    T* o2 = (cur == P(l)) ? W(l) : P(l);
-                    if (int r = sweep(l, cur, o2, false)) return r;
-                    cur = o2;
-                }
-                return OK;
-            }
-        }
         if (rows_ok(l)) {
             const unsigned gf = unsigned(cdiv(int64_t(d.n0 / 2 + 1) * (d.n1 / 2 + 1), 4));
             if (pl.periodic) prolong_groups_kernel<T, true><<<gf, 256, 0, s>>>(cc, cur, d);

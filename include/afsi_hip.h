@@ -253,11 +253,7 @@ typedef struct afsi_mg3d_plan {
     double* reduce_result;
     int64_t partials_capacity; /* doubles in reduce_partials */
     int32_t zero_initial;      /* fused driver: treat p[0] as zero on the first cycle */
-    /* fused driver, red-black: bit l clear = the prolongation into level l
-     * (if that level sweeps by exact row tiles and has at most 64^3 cells)
-     * runs inside its first post-smoothing sweep, with no launch of its own
-     * and the same values; bit l set = a separate prolongation launch */
-    int32_t unfused_prolong;
+    int32_t reserved;
     /* fused red-black sweeps, per level (0 = automatic): tile edge TX = TY
      * (2 or 4) and workgroups per launch (persistent: each workgroup sweeps
      * every G-th tile, overlapping the next tile's loads with compute). */

@@ -222,30 +222,3 @@ def test_autotuned_sweeps_keep_results(gpu):
     cache = autotune.export_cache()
     autotune.import_cache(cache)
     assert autotune.export_cache() == cache
-
-
-@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
-@pytest.mark.parametrize('periodic', [False, True])
-@pytest.mark.parametrize('shape,tile', [((64, 64, 64), 0), ((128, 64, 64), 0), ((24, 40, 32), 0),
-                                        ((64, 32, 128), 8), ((64, 64, 64), 84), ((32, 32, 64), 4)])
-def test_fused_prolongation_bit_exact(gpu, dtype, periodic, shape, tile):
-    """The prolongation folded into the first post-smoothing sweep (the
-    default) gives the separate prolongation launch's solve bit for bit,
-    for every sweep tile (0: the automatic choice)."""
-    grid = MACGrid(shape, tuple(n / 64 * f for n, f in zip(shape, (1., 1.1, .9))), periodic=periodic)
-    opts = MGOptions(smooth=2, smoother='rbgs')
-    gen = torch.Generator(device='cpu').manual_seed(11)
-    rhs = torch.randint(-8, 9, shape, generator=gen).to(torch.float64)
-    rhs.view(-1)[0] -= rhs.sum()   # exactly zero net flux in either precision
-    b = rhs.to(device=gpu, dtype=dtype)
-    rtol = 1e-4 if dtype == torch.float32 else None
-    runs = []
-    for unfused in (-1, 0, 1):   # all levels separate, all fused, only level 0 separate
-        mg = GeometricMultigrid(grid, device=gpu, dtype=dtype, options=opts, backend='workspace')
-        mg.plan.unfused_prolong = unfused
-        for l in range(len(mg.shapes)):
-            mg.plan.sweep_tile[l] = tile
-        runs.append(mg.solve(b, rtol=rtol))
-    (p0, i0) = runs[0]
-    for p1, i1 in runs[1:]:
-        assert i0['cycles'] == i1['cycles'] and torch.equal(p0, p1)
