@@ -72,3 +72,45 @@ def test_mixed_projection_divergence_and_velocity(gpu, periodic):
     assert np.linalg.norm(div) <= 1e-10 * np.linalg.norm(div0) * 1.01
     for a, b in zip(vel, v_ref):
         assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-9
+
+
+def seven_point(x, w, periodic):
+    """A x = sum_a w_a (2 x - x(+e_a) - x(-e_a)), Neumann neighbours omitted."""
+    a = torch.zeros_like(x)
+    for ax in range(3):
+        for step in (1, -1):
+            term = w[ax] * (x - torch.roll(x, -step, ax))
+            if not periodic:
+                edge = [slice(None)] * 3
+                edge[ax] = -1 if step == 1 else 0
+                term[tuple(edge)] = 0
+            a += term
+    return a
+
+
+@pytest.mark.parametrize('periodic', [False, True])
+@pytest.mark.parametrize('shape', [(8, 12, 16), (6, 4, 128), (6, 10, 130)])
+def test_refine_step_is_update_then_residual(gpu, periodic, shape):
+    """afsi_refine_step_3d forms x_out = x + e bit for bit and r = b - A x_out
+    (FP32 result within one FP32 rounding of torch's FP64 residual), with the
+    block reduction [sum r, sum r^2, sum |r|, non-finite] within 1e-12."""
+    from afsi_gpu_amd import _abi
+    g = torch.Generator(device='cpu').manual_seed(5)
+    x = torch.randn(shape, generator=g, dtype=torch.float64).to(gpu)
+    e = torch.randn(shape, generator=g, dtype=torch.float64).float().to(gpu)
+    b = torch.randn(shape, generator=g, dtype=torch.float64).to(gpu)
+    inv_h2 = (3.0, 1.7, 0.6)
+    xo = torch.empty_like(x)
+    r32 = torch.empty(shape, device=gpu, dtype=torch.float32)
+    partials = torch.zeros(4 * ((shape[0] * shape[1] + 3) // 4) + 64, device=gpu, dtype=torch.float64)
+    result = torch.zeros(4, device=gpu, dtype=torch.float64)
+    _abi.call('afsi_refine_step_3d', _abi.ptr(x), _abi.ptr(e), _abi.ptr(b), _abi.ptr(xo), _abi.ptr(r32),
+              _abi.i64x3(shape), _abi.f64x3(inv_h2), int(periodic), _abi.ptr(partials), partials.numel(),
+              _abi.ptr(result), _abi.stream())
+    xs = x + e.double()
+    assert torch.equal(xo, xs)
+    r = b - seven_point(xs, inv_h2, periodic)
+    ulp = torch.finfo(torch.float32).eps * r.abs().float() + torch.finfo(torch.float32).tiny
+    assert bool(((r32 - r.float()).abs() <= ulp).all())
+    want = torch.stack([r.sum(), (r * r).sum(), r.abs().sum(), torch.zeros((), device=gpu, dtype=torch.float64)])
+    assert torch.allclose(result, want, rtol=1e-12, atol=1e-12 * float(r.abs().sum()))
