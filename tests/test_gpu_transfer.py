@@ -180,6 +180,33 @@ def test_tiled_spread_and_interp(gpu, periodic, tile):
     assert max_err(U.cpu(), U_ref) < 10 * TOL
 
 
+
+@pytest.mark.parametrize('periodic', [False, True])
+def test_tiled_transfer_empty_and_single_point(gpu, periodic):
+    """No points: the spread fields are zero and the interpolation is (0, 3).
+    One point: spread and interpolation equal the oracle's."""
+    g, og = grids(periodic)
+    tr = TiledTransfer(g, gpu, (8, 8, 8))
+    none = torch.zeros((0, 3), device=gpu, dtype=torch.float64)
+    bins = bin_points(none, g, (8, 8, 8))
+    f = tr.spread(none, none.clone(), bins)
+    assert all(fc.shape == tuple(g.face_shape(c)) and int(fc.count_nonzero()) == 0 for c, fc in enumerate(f))
+    gen = np.random.default_rng(8)
+    u = tuple(gen.normal(size=og.face_shape(c)) for c in range(3))
+    ut = tuple(torch.tensor(a, device=gpu) for a in u)
+    assert tr.interpolate(none, ut, bins).shape == (0, 3)
+    x, F = points_in(og, 1, 9)
+    base, phi = ib3d.prepare_compact(x, og)
+    xt, Ft = torch.tensor(x, device=gpu), torch.tensor(F, device=gpu)
+    bins = bin_points(xt, g, (8, 8, 8))
+    f = tr.spread(xt, Ft, bins)
+    U = tr.interpolate(xt, ut, bins)
+    assert int(tr.flags.item()) == 0
+    f_ref = ib3d.spread_grid(F, base, phi, og)
+    for c in range(3):
+        assert rel_err(f[c].cpu(), f_ref[c]) < TOL
+    assert rel_err(U.cpu(), ib3d.gather_grid(u, base, phi, og)) < TOL
+
 def test_tiled_transfer_config2(gpu):
     x, F = workloads.config2()
     og = ib3d.Grid3((64, 64, 64), (1., 1., 1.), periodic=True)
