@@ -97,7 +97,7 @@ def lognormal_forces(gen, count):
     return magnitude * _sphere(gen, count)
 
 
-def config2(points=200_000, seed=GEOMETRY_SEED):
+def truncated_shell_64(points=200_000, seed=GEOMETRY_SEED):
     """64^3 unit cube: truncated shell (0.3,0.2,0.2), thickness 0.02,
     truncation x - 0.5 <= 0.6*0.3 (build's choice), N(0,1) forces."""
     gen = rng(seed)
@@ -105,7 +105,7 @@ def config2(points=200_000, seed=GEOMETRY_SEED):
     return x, normal_forces(gen, points)
 
 
-def config3(points=1_500_000, seed=GEOMETRY_SEED + 1):
+def clustered_shells_128(points=1_500_000, seed=GEOMETRY_SEED + 1):
     """128^3 unit cube: 90% on shells (0.35,0.25,0.25) and (0.3,0.2,0.2)
     split by area, 10% clustered within 0.05 of the outer +x tip;
     forces LogNormal(0,1) magnitude, uniform direction."""
@@ -121,7 +121,7 @@ def config3(points=1_500_000, seed=GEOMETRY_SEED + 1):
 
 
 def curves2d(gen, lengths=(4., 1.), shape=(256, 64), per_curve=2048):
-    """Config-1 curves: x_i = Lx (i+1/2)/N, y = 0.5 + k 0.1 sin(2 pi x / Lx),
+    """Channel curves: x_i = Lx (i+1/2)/N, y = 0.5 + k 0.1 sin(2 pi x / Lx),
     k = +-1, plus U(-0.05h, 0.05h) jitter on both coordinates."""
     h = lengths[0] / shape[0]
     i = np.arange(per_curve)

@@ -1,6 +1,6 @@
 """Benchmark: MAC/IB time steps at 128^3 with 1.5M points on one MI355X.
 
-Workload (config 3): periodic unit cube 128^3, 1.5M points on two nested
+Workload: periodic unit cube 128^3, 1.5M points on two nested
 ellipsoidal shells with 10% clustered in a 0.05 patch, lognormal-magnitude
 forces. One step = bin -> spread -> u* = u + dt f -> divergence -> pressure
 multigrid (red-black GS V(2,2)) to rtol 1e-10 -> gradient correction ->
@@ -12,10 +12,10 @@ Inputs are resident in HBM before the timed region; K steps per path are
 timed between device synchronisations, with Python's cyclic garbage
 collector paused inside timed regions. `value` is the mixed path's steps/s.
 
-`batched` (config 4): 64 uneven instances stepped together (3D: one
+`batched`: 64 uneven instances stepped together (3D: one
 batched mixed pressure solve, one launch per multigrid phase) against the
 same instances stepped one at a time on the plain path. `serving`
-(config 5, bounded sample): a Poisson request stream served with
+(bounded sample): a Poisson request stream served with
 continuous batching, p50/p99 latency.
 
 `roofline` is for the dominant kernel of the mixed step, the level-0
@@ -500,7 +500,7 @@ def main():
 
     device = torch.device('cuda:0')
     n, dt = args.n, 1e-3
-    x_np, F_np = workloads.config3(args.points)
+    x_np, F_np = workloads.clustered_shells_128(args.points)
     P, N = len(x_np), n ** 3
     grid = MACGrid((n, n, n), (1., 1., 1.), periodic=True)
     opts = MGOptions(smooth=2, smoother='rbgs')
@@ -527,7 +527,7 @@ def main():
         metric=METRIC, value=mixed['steps_per_s'], unit='steps/s', n_gpus=1, steps=args.steps,
         warmup=args.warmup, ms_per_step=mixed['ms_per_step'], higher_is_better=True,
         scaling='weak', vs_baseline=None, dtype='f32+f64', data='synthetic',
-        config=dict(workload=f'config 3: {n}^3 periodic unit cube, {P} points (two ellipsoidal '
+        config=dict(workload=f'{n}^3 periodic unit cube, {P} points (two ellipsoidal '
                              'shells + 10% in a 0.05 patch), lognormal forces; IB step with '
                              'red-black GS V(2,2) to rtol 1e-10; mixed = FP32 V-cycles in FP64 '
                              'refinement', grid=n, points=P, precision='mixed'),

@@ -5,7 +5,7 @@ spread and gather within 1e-12 relative; elementwise multigrid kernels
 bit-identical in FP64; one V-cycle within 1e-12 relative (the reference
 channel cycle has no reductions; the periodic one removes means); solve
 histories r_k/r_0 within 1e-10 of the oracle's; projected velocity within
-1e-10 relative; the config-1 step's point velocity within 1e-10 relative.
+1e-10 relative; the channel IB step's point velocity within 1e-10 relative.
 """
 import numpy as np
 import pytest
@@ -210,7 +210,7 @@ def test_projection_matches_oracle(gpu, periodic):
         flow.project((torch.tensor(u, device=gpu), torch.tensor(v, device=gpu)))
 
 
-def test_config1_step_matches_oracle(gpu):
+def test_channel_step_matches_oracle(gpu):
     """256 x 64 on [0,4] x [0,1], periodic in x, walls in y; two curves of
     2,048 points; N(0,1) forces; red-black GS V(2,2) to 1e-10."""
     gen = np.random.default_rng(11)

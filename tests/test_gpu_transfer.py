@@ -134,8 +134,8 @@ def test_binning_bit_exact(gpu, periodic, tile):
     assert np.array_equal(b.cell_ptr.cpu().numpy().astype(np.int64), ptr)
 
 
-def test_binning_config2_bit_exact(gpu):
-    x, _ = workloads.config2()
+def test_binning_truncated_shell_bit_exact(gpu):
+    x, _ = workloads.truncated_shell_64()
     og = ib3d.Grid3((64, 64, 64), (1., 1., 1.), periodic=True)
     g = MACGrid((64, 64, 64), (1., 1., 1.), periodic=True)
     keys, perm, ptr = obin.bin_points(x, og, (8, 8, 8))
@@ -207,8 +207,8 @@ def test_tiled_transfer_empty_and_single_point(gpu, periodic):
         assert rel_err(f[c].cpu(), f_ref[c]) < TOL
     assert rel_err(U.cpu(), ib3d.gather_grid(u, base, phi, og)) < TOL
 
-def test_tiled_transfer_config2(gpu):
-    x, F = workloads.config2()
+def test_tiled_transfer_truncated_shell(gpu):
+    x, F = workloads.truncated_shell_64()
     og = ib3d.Grid3((64, 64, 64), (1., 1., 1.), periodic=True)
     g = MACGrid((64, 64, 64), (1., 1., 1.), periodic=True)
     base, phi = ib3d.prepare_compact(x, og)
@@ -220,7 +220,7 @@ def test_tiled_transfer_config2(gpu):
     f = tr.spread(xt, Ft, bins)
     U = tr.interpolate(xt, f, bins)
     errs = [rel_err(f[c].cpu(), f_ref[c]) for c in range(3)] + [rel_err(U.cpu(), U_ref)]
-    print('config2 spread/interp l2-relative errors:', errs)
+    print('truncated-shell spread/interp l2-relative errors:', errs)
     assert max(errs) < TOL
 
 
@@ -238,7 +238,7 @@ def test_sqrt_and_division_round_like_numpy(gpu):
 
 def test_transfer_tile_autotuner(gpu):
     from afsi_gpu_amd import autotune
-    x, F = workloads.config2(50000)
+    x, F = workloads.truncated_shell_64(50000)
     g = MACGrid((64, 64, 64), (1., 1., 1.), periodic=True)
     edge, times = autotune.tune_transfer_tile(g, torch.tensor(x, device=gpu), torch.tensor(F, device=gpu))
     assert edge in (4, 8) and set(times) == {4, 8}
