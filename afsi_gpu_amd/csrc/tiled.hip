@@ -27,6 +27,10 @@ template <class T>
 int launch_fill(T* x, int64_t count, T value, hipStream_t s);
 
 constexpr int SPREAD_THREADS = 256;
+// Interpolation workgroups: the staged face box is the same for any thread
+// count, so 512 threads double the wavefronts per CU that the LDS allows
+// (VGPRs then set the occupancy).
+constexpr int INTERP_THREADS = 512;
 
 // Points per work item: the spread's items are short so that items of
 // dense tiles spread over more workgroups (its time is set by LDS atomic
@@ -359,8 +363,9 @@ __global__ void __launch_bounds__(SPREAD_THREADS) spread_tiled_kernel(
 // components is staged in LDS once per work item, then each point gathers
 // its 3 x 64 values from LDS (consecutive threads take consecutive binned
 // points; results go to the caller's point order).
-// One interpolation work item: stage the box, gather the points' values.
-template <int T>
+// One interpolation work item: stage the box, gather the points' values
+// (NT threads per workgroup).
+template <int T, int NT>
 __device__ __forceinline__ void interp_item(const WorkItem& it, const double* __restrict__ x,
                                             const int32_t* __restrict__ perm, int base,
                                             const afsi_grid3& g, const TileSpec& ts,
@@ -370,13 +375,13 @@ __device__ __forceinline__ void interp_item(const WorkItem& it, const double* __
     for (int c = 0; c < 3; ++c) {
         const int vol = box_dim<T>(ts, c, 0) * box_dim<T>(ts, c, 1) * box_dim<T>(ts, c, 2);
         double* box = lds + box_off<T>(ts, c);
-        for (int e = threadIdx.x; e < vol; e += SPREAD_THREADS) {
+        for (int e = threadIdx.x; e < vol; e += NT) {
             const int f = box_face<T>(e, c, it.t0, ts, g);
             box[e] = f >= 0 ? us[c][f] : 0.0;
         }
     }
     __syncthreads();
-    for (int j = it.pbeg + int(threadIdx.x); j < it.pend; j += SPREAD_THREADS) {
+    for (int j = it.pbeg + int(threadIdx.x); j < it.pend; j += NT) {
         const int idx = perm[j] - base;
         const double xp[3] = {x[3 * idx], x[3 * idx + 1], x[3 * idx + 2]};
         PointWeights pw;
@@ -408,7 +413,7 @@ __device__ __forceinline__ void interp_item(const WorkItem& it, const double* __
 // its 3 x 64 values from LDS (consecutive threads take consecutive binned
 // points; results go to the caller's point order).
 template <int T>
-__global__ void __launch_bounds__(SPREAD_THREADS) interp_lds_kernel(
+__global__ void __launch_bounds__(INTERP_THREADS) interp_lds_kernel(
     const double* __restrict__ x, const int32_t* __restrict__ perm,
     const int32_t* __restrict__ cell_ptr, const int32_t* __restrict__ chunk_start, afsi_grid3 g,
     TileSpec ts, const double* __restrict__ u0, const double* __restrict__ u1,
@@ -417,7 +422,8 @@ __global__ void __launch_bounds__(SPREAD_THREADS) interp_lds_kernel(
     const int total = chunk_start[ts.ntiles];
     const double* const us[3] = {u0, u1, u2};
     for (int w = blockIdx.x; w < total; w += gridDim.x)
-        interp_item<T>(work_item(w, chunk_start, cell_ptr, ts, INTERP_CHUNK), x, perm, 0, g, ts, us, out, flags, lds);
+        interp_item<T, INTERP_THREADS>(work_item(w, chunk_start, cell_ptr, ts, INTERP_CHUNK), x, perm, 0, g, ts, us, out,
+                                       flags, lds);
 }
 
 // Unbinned interpolation: one thread per point, gathers from HBM/L2.
@@ -603,7 +609,7 @@ __global__ void __launch_bounds__(SPREAD_THREADS) interp_batch_kernel(
         const WorkItem it = batch_item(w, desc, n, tiles, cell_ptr, chunk_start, INTERP_CHUNK, i);
         const XferDesc& d = desc[i];
         const double* const us[3] = {d.face[0], d.face[1], d.face[2]};
-        interp_item<T>(it, d.x, perm, d.point_begin, d.g, d.ts, us, d.out, flags, lds);
+        interp_item<T, SPREAD_THREADS>(it, d.x, perm, d.point_begin, d.g, d.ts, us, d.out, flags, lds);
     }
 }
 
@@ -769,15 +775,15 @@ int afsi_interp_tiled_f64(const double* x, int64_t points, const afsi_grid3* gri
     const size_t shmem = size_t(ts.lds) * sizeof(double);
     switch (cubic_edge(tile)) {
     case 8:
-        interp_lds_kernel<8><<<blocks, SPREAD_THREADS, shmem, s>>>(x, perm, cell_ptr, chunk_start, g, ts,
+        interp_lds_kernel<8><<<blocks, INTERP_THREADS, shmem, s>>>(x, perm, cell_ptr, chunk_start, g, ts,
                                                                   u, v, w, out, flags);
         break;
     case 4:
-        interp_lds_kernel<4><<<blocks, SPREAD_THREADS, shmem, s>>>(x, perm, cell_ptr, chunk_start, g, ts,
+        interp_lds_kernel<4><<<blocks, INTERP_THREADS, shmem, s>>>(x, perm, cell_ptr, chunk_start, g, ts,
                                                                   u, v, w, out, flags);
         break;
     default:
-        interp_lds_kernel<0><<<blocks, SPREAD_THREADS, shmem, s>>>(x, perm, cell_ptr, chunk_start, g, ts,
+        interp_lds_kernel<0><<<blocks, INTERP_THREADS, shmem, s>>>(x, perm, cell_ptr, chunk_start, g, ts,
                                                                   u, v, w, out, flags);
     }
     AFSI_LAUNCH_CHECK();
