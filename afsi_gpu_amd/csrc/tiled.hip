@@ -258,8 +258,9 @@ __device__ inline bool box_base(const PointWeights& pw, const int shift[3], int 
 __device__ inline void frame_shift(const PointWeights& pw, const afsi_grid3& g, int shift[3]) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const int cell = pw.bf[a] + 1;
-        shift[a] = g.periodic ? wrap32(cell, int(g.n[a])) - cell : 0;
+        const int cell = pw.bf[a] + 1, n = int(g.n[a]);
+        // a cell already inside the domain needs no integer remainder
+        shift[a] = !g.periodic || (cell >= 0 && cell < n) ? 0 : wrap32(cell, n) - cell;
     }
 }
 
