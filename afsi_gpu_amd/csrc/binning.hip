@@ -1,7 +1,7 @@
 // Point binning: integer cell keys, stable LSD radix sort, CSR offsets.
 //
 // The cell of a point on axis a is floor((x_a-origin_a)/h_a - 1) + 1: the
-// reference's face-lattice stencil base (mac/compact_transfer.py:161-163)
+// reference's face-lattice stencil base (mac/compact_transfer.py:21-23)
 // plus one, evaluated with the same IEEE operations (sub, div, sub, floor),
 // so keys are bit-identical to the oracle's. Keys are tile-major: tiles of
 // tx*ty*tz cells, tiles and cells inside a tile both x-major (z fastest), so

@@ -17,7 +17,7 @@
 //
 // Weights: per axis f = s - base - 1 and the four Peskin weights from the
 // shared radical sqrt(1+4f-4f^2); base = floor(s-1) exactly as the
-// reference (mac/compact_transfer.py:161-166) with s=(x-o)/h.
+// reference (mac/compact_transfer.py:21-26) with s=(x-o)/h.
 #include "binshare.hpp"
 #include <vector>
 
@@ -119,7 +119,7 @@ __device__ __forceinline__ double separable_sum(const double* __restrict__ u, in
     return sum;
 }
 
-// Bases and weights of one point (mac/compact_transfer.py:161-166 bases).
+// Bases and weights of one point (mac/compact_transfer.py:21-26 bases).
 __device__ inline void point_weights(const double xp[3], const afsi_grid3& g, PointWeights& pw) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {

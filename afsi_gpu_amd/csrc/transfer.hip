@@ -35,7 +35,7 @@ __global__ void prepare_compact_kernel(const double* __restrict__ x, int64_t poi
     const double o = (a == c) ? g.origin[a] : g.center_origin[a];
     const double scaled = (xa - o) / g.h[a];
     if (c == 0 && !g.periodic) {
-        // mac/compact_transfer.py:234-236 on the box-origin scale.
+        // mac/compact_transfer.py:94-96,115-117 on the box-origin scale.
         const double s0 = (xa - g.origin[a]) / g.h[a];
         if (!(s0 >= 2.0) || !(s0 < double(g.n[a]) - 2.0)) raise_flag(flags, F_SUPPORT);
     }

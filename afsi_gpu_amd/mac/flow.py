@@ -113,7 +113,7 @@ class MACFlow:
 
     @torch.no_grad()
     def project(self, tentative, initial=None):
-        """mac/flow.py:283-302 (torch branch): finite and closed-wall checks,
+        """mac/flow.py:102-121 (torch branch): finite and closed-wall checks,
         pressure solve, gradient correction into fresh velocity tensors."""
         self.grid.check_velocity(tentative)
         for c, u in enumerate(tentative):

@@ -1,6 +1,6 @@
 """P2 fused cell-block transfer on the GPU: ``prepare``, ``spread`` and
 ``assemble_velocity`` of mac/p2_cuda_transfer.py:27-67, with the shared
-face/centre stencil tables of mac/shared_stencil.py:8-20.
+face/centre stencil tables of mac/shared_stencil.py:7-19.
 
 The interaction points of 10-node tetrahedra are never materialised:
 ``prepare`` evaluates them in registers and writes the shared tables (base
@@ -59,7 +59,7 @@ class SharedStencil:
         return self.base.numel() * self.base.element_size() + self.phi.numel() * self.phi.element_size()
 
     def expanded(self):
-        """Per-component compact tables (mac/shared_stencil.py:13-20):
+        """Per-component compact tables (mac/shared_stencil.py:13-19):
         component c takes lattice 0 along axis c and lattice 1 elsewhere."""
         base = torch.stack([torch.stack([self.base[int(a != c), :, a] for a in range(3)], -1) for c in range(3)])
         phi = torch.stack([torch.stack([self.phi[int(a != c), :, a, :] for a in range(3)], 1) for c in range(3)])

@@ -1,7 +1,7 @@
 """Point-level 3D IB transfer on MAC faces: stencils, binning, tiled kernels.
 
 * ``CompactStencil`` / ``from_points``: the reference's compact separable
-  stencil (mac/compact_transfer.py:151-166,284-290), built on the GPU with
+  stencil (mac/compact_transfer.py:11-26,144-150), built on the GPU with
   the reference's two-branch Peskin formula; consumed by
   ``cuda_ib.compact_gather`` / ``compact_spread``.
 * ``bin_points``: build-own stable CSR of points by tile-major cell key.
@@ -34,9 +34,9 @@ def _check_points(points):
 
 
 def from_points(points, grid, *, check=True):
-    """Compact stencil of FP64 points (mac/compact_transfer.py:284-290).
+    """Compact stencil of FP64 points (mac/compact_transfer.py:144-150).
 
-    With ``check`` the support predicate of mac/compact_transfer.py:234-236
+    With ``check`` the support predicate of mac/compact_transfer.py:94-96,115-117
     is enforced (one host synchronisation), raising the reference's
     'support' ValueError."""
     require_cuda(points)

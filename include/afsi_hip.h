@@ -36,17 +36,17 @@ const char* afsi_last_error(void);
 int afsi_abi_version(void);
 
 /* ---- Stencil preparation -------------------------------------------------
- * Replaces mac/compact_transfer.py:161-166 (_prepare) as used by
+ * Replaces mac/compact_transfer.py:21-26 (_prepare) as used by
  * CompactFETransfer.from_points (:284-290): base (3,P,3) int64 and phi
  * (3,P,3,4) for the three staggered lattices, phi with the reference's two-
  * branch formula (ib.py:57-67). Sets flag 1 for points failing the support
- * predicate of mac/compact_transfer.py:234-236 (closed box only). */
+ * predicate of mac/compact_transfer.py:94-96,115-117 (closed box only). */
 int afsi_prepare_compact_f64(const double* x, int64_t points, const afsi_grid3* grid,
                              int64_t* base, double* phi, int* flags, void* stream);
 
 /* ---- Compact separable transfer (one component per call) ----------------
  * Replace torch.ops.afsi_ib_cuda.compact_gather / compact_spread
- * (mac/csrc/transfer.cpp:49-52,215-228; mac/cuda_ib.py:79-90).
+ * (mac/csrc/transfer.cpp:34-53; mac/cuda_ib.py:79-90).
  * field/out are the flat C-order face array of shape (nx,ny,nz) of this
  * component; base (P,3) int64, phi (P,3,4). The gather writes out[p*stride].
  * The spread reads force[p*stride], zero-fills `out` and accumulates
@@ -66,7 +66,7 @@ int afsi_compact_spread_f32(const float* force, int64_t force_stride, const int6
 
 /* ---- Generic indexed transfer --------------------------------------------
  * Replace torch.ops.afsi_ib_cuda.indexed_gather / indexed_spread
- * (mac/csrc/transfer.cpp:49-52,200-214; mac/cuda_ib.py:69-76): field (N,C),
+ * (mac/csrc/transfer.cpp:19-33,48-53; mac/cuda_ib.py:69-76): field (N,C),
  * ids (P,K) int64, signed weights (P,K). Spread zero-fills out (size,C) and
  * accumulates (f*w)/volume. Out-of-range ids set flag 2 and are skipped. */
 int afsi_indexed_gather_f64(const double* field, int64_t nodes, int64_t comps,
@@ -84,7 +84,7 @@ int afsi_indexed_spread_f32(const float* field, int64_t points, int64_t comps,
 
 /* ---- Point binning (build-own; no reference counterpart) ------------------
  * Integer cell of each point = face-lattice base + 1 on every axis
- * (the base of mac/compact_transfer.py:161-163), wrapped on a periodic grid.
+ * (the base of mac/compact_transfer.py:21-23), wrapped on a periodic grid.
  * Keys use a tile-major cell numbering (tile tx*ty*tz, both levels x-major).
  * Output: keys_sorted (P) uint32, perm (P) int32 = stable argsort of the
  * keys, cell_ptr (key_space+1) int32 CSR offsets. Stable LSD radix sort. */
@@ -192,7 +192,7 @@ int afsi_mg_prolong_add_3d_f64(const double* coarse, double* fine, const int64_t
 int afsi_mg_prolong_add_3d_f32(const float* coarse, float* fine, const int64_t n[3],
                                int periodic, void* stream);
 /* y = A x for a dense row-major (n x n) FP64 matrix: the coarse solve
- * coarse_inverse @ rhs of mac/multigrid.py:114 and mac/mg_workspace.py:259. */
+ * coarse_inverse @ rhs of mac/multigrid.py:114 and mac/mg_workspace.py:86-90. */
 int afsi_dense_matvec_f64(const double* A, const double* x, double* y, int64_t n, void* stream);
 int afsi_dense_matvec_f32(const float* A, const float* x, float* y, int64_t n, void* stream);
 
@@ -303,7 +303,7 @@ int afsi_refine_step_3d(const double* x, const float* e, const double* b, double
 int afsi_refine_update(double* x, const float* e, int64_t count, void* stream);
 /* Unfused driver with the reference workspace's launch structure (one
  * launch per colour sweep, separate reductions and mean subtractions,
- * mac/mg_workspace.py:246-305); kept as a cross-check of the fused one.
+ * mac/mg_workspace.py:70-133); kept as a cross-check of the fused one.
  * tmp[0] receives the residual in the _residual_ variant. */
 int afsi_mg3d_cycles_unfused_f64(const afsi_mg3d_plan* plan, int32_t cycles, void* stream);
 int afsi_mg3d_cycles_unfused_f32(const afsi_mg3d_plan* plan, int32_t cycles, void* stream);

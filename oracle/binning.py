@@ -3,7 +3,7 @@
 Test infrastructure only (see ``oracle/__init__.py``).
 
 The reference has no point binning. Its integer work on this path is the
-stencil base ``floor((x-o_c)/h-1)`` (mac/compact_transfer.py:161-163;
+stencil base ``floor((x-o_c)/h-1)`` (mac/compact_transfer.py:21-23;
 mac/transfer.py:86-87) and the Cartesian face ids (mac/transfer.py:96-99).
 The cell of a point is the face-lattice base plus one along each axis, i.e.
 ``base[a][:, a] + 1`` of the compact stencil, so every key is a pure integer

@@ -310,7 +310,7 @@ def p2_shared_tables(points, grid):
 
 
 def shared_expanded(base, phi):
-    """Per-component compact tables of the shared ones (mac/shared_stencil.py:13-20):
+    """Per-component compact tables of the shared ones (mac/shared_stencil.py:13-19):
     component c takes lattice 0 along axis c and lattice 1 elsewhere."""
     cb = np.stack([np.stack([base[int(a != c), :, a] for a in range(3)], -1) for c in range(3)])
     cp = np.stack([np.stack([phi[int(a != c), :, a, :] for a in range(3)], 1) for c in range(3)])

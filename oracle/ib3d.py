@@ -6,17 +6,17 @@ Reference (``src/afsi_torch``):
   ib.py:57-67                    peskin4 (two branches, clamped radicands)
   ib.py:78-105,114-144           nodal-lattice prepare/interpolate/spread
   mac/grid.py:27-40              spacing, face_shape, face_origin
-  mac/compact_transfer.py:161-166  _prepare: base=floor(scaled-1), phi
-  mac/compact_transfer.py:303-341  expanded 64-link ids/weights, chunked
+  mac/compact_transfer.py:21-26  _prepare: base=floor(scaled-1), phi
+  mac/compact_transfer.py:163-201  expanded 64-link ids/weights, chunked
                                    index_add_ spread and gather (4096 points)
-  mac/compact_transfer.py:234-236  support predicate 2 <= (x-o)/h < n-2
+  mac/compact_transfer.py:94-96,115-117  support predicate 2 <= (x-o)/h < n-2
 The periodic lattice has no reference counterpart: node indices wrap mod n and
 every face array has n entries per axis.
 """
 from dataclasses import dataclass
 import numpy as np
 
-CHUNK = 4096  # mac/compact_transfer.py:320
+CHUNK = 4096  # mac/compact_transfer.py:180-184
 
 
 @dataclass(frozen=True)
@@ -70,7 +70,7 @@ def peskin4(distance):
 
 
 def support_ok(points, grid):
-    """mac/compact_transfer.py:234-236 (and mac/transfer.py:71-78): every
+    """mac/compact_transfer.py:94-96,115-117 (and mac/transfer.py:71-78): every
     point finite with 2 <= (x-origin)/h < n-2 on each axis. Always true for
     finite points on the periodic lattice."""
     points = np.asarray(points, dtype=np.float64)
@@ -84,7 +84,7 @@ def support_ok(points, grid):
 
 
 def prepare_compact(points, grid):
-    """mac/compact_transfer.py:161-166: base (3,P,3) int64, phi (3,P,3,4).
+    """mac/compact_transfer.py:21-26: base (3,P,3) int64, phi (3,P,3,4).
 
     Component c uses origin ``face_origin(c)``; base=floor((x-o_c)/h-1)."""
     points = np.asarray(points, dtype=np.float64)
@@ -98,7 +98,7 @@ def prepare_compact(points, grid):
 
 
 def expanded_component(base, phi, grid, c):
-    """mac/compact_transfer.py:303-310 (+ periodic wrap): ids and weights
+    """mac/compact_transfer.py:163-170 (+ periodic wrap): ids and weights
     (P,64), z fastest, weight=(phi_x*phi_y)*phi_z."""
     nodes = base[c][..., None] + np.arange(4)  # (P,3,4)
     shape = grid.face_shape(c)
@@ -113,7 +113,7 @@ def expanded_component(base, phi, grid, c):
 
 
 def spread_grid(force, base, phi, grid):
-    """mac/compact_transfer.py:312-325: f_c[id] += (w*F_c)/V, chunked by
+    """mac/compact_transfer.py:172-185: f_c[id] += (w*F_c)/V, chunked by
     4096 points; returns three fresh face arrays."""
     force = np.asarray(force, dtype=np.float64)
     fields = [np.zeros(grid.face_shape(c)) for c in range(3)]
@@ -132,7 +132,7 @@ def spread_grid(force, base, phi, grid):
 
 
 def gather_grid(velocity, base, phi, grid):
-    """mac/compact_transfer.py:327-341: U_c[p]=sum_64 u_c[id]*w."""
+    """mac/compact_transfer.py:187-201: U_c[p]=sum_64 u_c[id]*w."""
     count = base.shape[1]
     out = np.empty((count, 3))
     for start in range(0, count, CHUNK):

@@ -6,10 +6,10 @@ Follows the ordering of the reference's coupled step (mac/coupling.py:74-85:
 prepare -> spread -> flow step -> interpolate -> x + dt*U) with the point
 forces given directly (no FE mass solve) and the predictor reduced to the
 force term u* = u + (dt/rho) f (convection/viscosity are not on this path):
-  f = spread(F)                      mac/compact_transfer.py:312-325
+  f = spread(F)                      mac/compact_transfer.py:172-185
   p = MG(-(rho/dt) D u*, p_prev)     mac/flow.py:101-120
   u = u* - (dt/rho) G p
-  U = gather(u)                      mac/compact_transfer.py:327-341
+  U = gather(u)                      mac/compact_transfer.py:187-201
   x_new = x + dt U                   mac/coupling.py:84-85
 """
 import numpy as np
