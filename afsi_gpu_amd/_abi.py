@@ -121,16 +121,23 @@ _SIGNATURES = {
     'afsi_prepare_compact_f64': (c_int, [_P, c_int64, POINTER(Grid3), _P, _P, _P, _P]),
     'afsi_compact_gather_f64': (c_int, [_P, _P, _P, c_int64, c_int64, c_int64, c_int64, _P, c_int64, _P, _P]),
     'afsi_compact_gather_f32': (c_int, [_P, _P, _P, c_int64, c_int64, c_int64, c_int64, _P, c_int64, _P, _P]),
-    'afsi_compact_spread_f64': (c_int, [_P, c_int64, _P, _P, c_int64, c_int64, c_int64, c_int64, c_double, _P, _P, _P]),
-    'afsi_compact_spread_f32': (c_int, [_P, c_int64, _P, _P, c_int64, c_int64, c_int64, c_int64, c_double, _P, _P, _P]),
+    'afsi_compact_spread_workspace_bytes': (c_size_t, [c_int64, c_int64, c_int64, c_int64]),
+    'afsi_compact_spread_f64': (c_int, [_P, c_int64, _P, _P, c_int64, c_int64, c_int64, c_int64, c_double, _P, _P,
+                                        c_size_t, _P, _P]),
+    'afsi_compact_spread_f32': (c_int, [_P, c_int64, _P, _P, c_int64, c_int64, c_int64, c_int64, c_double, _P, _P,
+                                        c_size_t, _P, _P]),
     'afsi_indexed_gather_f64': (c_int, [_P, c_int64, c_int64, _P, _P, c_int64, c_int64, _P, _P, _P]),
     'afsi_indexed_gather_f32': (c_int, [_P, c_int64, c_int64, _P, _P, c_int64, c_int64, _P, _P, _P]),
-    'afsi_indexed_spread_f64': (c_int, [_P, c_int64, c_int64, _P, _P, c_int64, c_int64, c_double, _P, _P, _P]),
-    'afsi_indexed_spread_f32': (c_int, [_P, c_int64, c_int64, _P, _P, c_int64, c_int64, c_double, _P, _P, _P]),
+    'afsi_indexed_spread_workspace_bytes': (c_size_t, [c_int64, c_int64, c_int64]),
+    'afsi_indexed_spread_f64': (c_int, [_P, c_int64, c_int64, _P, _P, c_int64, c_int64, c_double, _P, _P, c_size_t,
+                                        _P, _P]),
+    'afsi_indexed_spread_f32': (c_int, [_P, c_int64, c_int64, _P, _P, c_int64, c_int64, c_double, _P, _P, c_size_t,
+                                        _P, _P]),
     'afsi_bin_key_space': (c_int64, [POINTER(Grid3), POINTER(c_int32)]),
     'afsi_bin_workspace_bytes': (c_size_t, [c_int64]),
     'afsi_bin_points': (c_int, [_P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, c_size_t, _P, _P]),
     'afsi_spread_workspace_bytes': (c_size_t, [POINTER(Grid3), POINTER(c_int32)]),
+    'afsi_tile_box_bounds': (c_int, [POINTER(Grid3), POINTER(c_int32), _I64, _I64]),
     'afsi_spread_tiled_f64': (c_int, [_P, _P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, _P, _P, c_size_t, _P, _P]),
     'afsi_interp_tiled_f64': (c_int, [_P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, _P, _P, _P, c_size_t, _P, _P]),
     'afsi_mg_residual_3d_f64': (c_int, [_P, _P, _P, POINTER(c_int64), POINTER(c_double), c_int, _P]),
@@ -157,6 +164,10 @@ _SIGNATURES = {
     'afsi_divergence_3d_f64': (c_int, [_P, _P, _P, _P, POINTER(c_int64), POINTER(c_double), c_double, c_int, _P]),
     'afsi_grad_correct_3d_f64': (c_int, [_P, _P, _P, _P, POINTER(c_int64), POINTER(c_double), c_double, c_int, _P]),
     'afsi_axpy_f64': (c_int, [_P, _P, c_int64, c_double, _P]),
+    'afsi_project_rhs_3d_f64': (c_int, [_P, _P, _P, _P, _P, _P, c_double, _P, POINTER(c_int64), POINTER(c_double),
+                                        c_double, c_int, _P]),
+    'afsi_project_correct_3d_f64': (c_int, [_P, _P, _P, _P, _P, _P, c_double, _P, POINTER(c_int64),
+                                            POINTER(c_double), c_double, c_int, _P]),
     'afsi_mg3d_partials_doubles': (c_int64, [POINTER(c_int64)]),
     'afsi_mg3d_supported': (c_int, [POINTER(MGPlan3)]),
     'afsi_mg3d_sweep_f64': (c_int, [POINTER(MGPlan3), c_int32, _P]),

@@ -267,13 +267,16 @@ size_t bin_workspace_bytes(int64_t points) {
 }
 
 // LSD radix sort of (keys, vals) in k0/v0 (ping-pong with k1/v1; the last
-// pass writes keys_sorted/perm), then cell_ptr over [0, space].
+// pass writes keys_sorted/perm), then cell_ptr over [0, space]. Keys lie in
+// [0, space), or in [0, space] when `sentinel` (a key equal to space marks
+// an entry that sorts last and lies outside every CSR range).
 static int sort_and_index(int64_t points, int64_t space, uint32_t* k0, uint32_t* v0, uint32_t* k1,
                           uint32_t* v1, uint32_t* hist, uint32_t* total, uint32_t* keys_sorted,
-                          int32_t* perm, int32_t* cell_ptr, hipStream_t s) {
+                          int32_t* perm, int32_t* cell_ptr, hipStream_t s, bool sentinel = false) {
     const int64_t nb = sort_blocks(points);
+    const int64_t bound = sentinel ? space + 1 : space;  // keys < bound
     int bits = 0;
-    while ((int64_t(1) << bits) < space) ++bits;
+    while ((int64_t(1) << bits) < bound) ++bits;
     int passes = (bits + 7) / 8;
     if (passes < 1) passes = 1;
     uint32_t* kin = k0;
@@ -313,6 +316,27 @@ static SortSpace sort_space(void* workspace, int64_t points) {
     w.hist = reinterpret_cast<uint32_t*>(ws + 4 * kb);
     w.total = reinterpret_cast<uint32_t*>(ws + 4 * kb + align256(size_t(sort_blocks(points)) * RADIX * 4));
     return w;
+}
+
+// Keys/values already in the first two slices of the sort workspace
+// (sort_key_slots): stable sort, then ptr = lower bounds over [0, space].
+int sort_pairs(int64_t n, int64_t space, void* workspace, uint32_t* keys_sorted, int32_t* perm,
+               int32_t* ptr, hipStream_t s) {
+    AFSI_REQUIRE(n < (int64_t(1) << 31) && space < (int64_t(1) << 31), "sort exceeds 2^31 keys");
+    if (n == 0) {
+        launch_cell_ptr(keys_sorted, 0, space, ptr, s);
+        AFSI_LAUNCH_CHECK();
+        return OK;
+    }
+    const SortSpace w = sort_space(workspace, n);
+    return sort_and_index(n, space, w.k0, w.v0, w.k1, w.v1, w.hist, w.total, keys_sorted, perm, ptr, s,
+                          true);
+}
+
+void sort_key_slots(void* workspace, int64_t n, uint32_t** keys, uint32_t** vals) {
+    const SortSpace w = sort_space(workspace, n);
+    *keys = w.k0;
+    *vals = w.v0;
 }
 
 int64_t bin_key_space(const afsi_grid3& g, const int32_t tile[3]) { return key_space(g, tile); }

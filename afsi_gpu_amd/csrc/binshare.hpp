@@ -26,6 +26,15 @@ struct BinDesc {
 int64_t bin_key_space(const afsi_grid3& g, const int32_t tile[3]);
 TileGeom bin_tile_geom(const afsi_grid3& g, const int32_t tile[3]);
 size_t bin_workspace_bytes(int64_t points);
+// Generic stable sort of n (key < space, value) pairs through the radix
+// passes of the binning: sort_key_slots names where the caller writes the
+// unsorted pairs inside a workspace of bin_workspace_bytes(n); sort_pairs
+// sorts them into keys_sorted / perm and writes ptr[k] = lower_bound(k) for
+// k in [0, space]. Keys may equal space (entries that sort last and belong
+// to no CSR range).
+void sort_key_slots(void* workspace, int64_t n, uint32_t** keys, uint32_t** vals);
+int sort_pairs(int64_t n, int64_t space, void* workspace, uint32_t* keys_sorted, int32_t* perm,
+               int32_t* ptr, hipStream_t s);
 // keys/vals of all points of a batch into the sort workspace, then the
 // stable radix sort and cell_ptr over [0, key_space].
 int bin_batch(const BinDesc* desc_device, int count, int64_t total_points, int64_t key_space,

@@ -213,13 +213,25 @@ __global__ void sub_mean_kernel(T* __restrict__ x, int64_t count, const double* 
 
 // ---- projection pieces --------------------------------------------------------
 
-template <bool P>
+// Face value of the divergence's input: u, or u + a f (the IB predictor
+// folded into the projection, formed as the axpy kernel forms it).
+template <bool F>
+__device__ __forceinline__ double face_value(const double* __restrict__ u, const double* __restrict__ f,
+                                             double a, int64_t i) {
+    return F ? u[i] + a * f[i] : u[i];
+}
+
+template <bool P, bool F = false>
 __global__ void __launch_bounds__(256) divergence_kernel(const double* __restrict__ u,
                                                          const double* __restrict__ v,
                                                          const double* __restrict__ w,
                                                          double* __restrict__ out, Dims d,
                                                          double hx, double hy, double hz,
-                                                         double scale) {
+                                                         double scale,
+                                                         const double* __restrict__ fu = nullptr,
+                                                         const double* __restrict__ fv = nullptr,
+                                                         const double* __restrict__ fw = nullptr,
+                                                         double a = 0.0) {
     const int64_t total = d.n0 * d.n1 * d.n2;
     const int64_t e = P ? 0 : 1;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
@@ -231,18 +243,20 @@ __global__ void __launch_bounds__(256) divergence_kernel(const double* __restric
         const int64_t iv1 = P ? (x * d.n1 + (y + 1 < d.n1 ? y + 1 : 0)) * d.n2 + z : iv + d.n2;
         const int64_t iw = (x * d.n1 + y) * (d.n2 + e) + z;
         const int64_t iw1 = P ? (x * d.n1 + y) * d.n2 + (z + 1 < d.n2 ? z + 1 : 0) : iw + 1;
-        double s = (u[iu1] - u[iu]) / hx;
-        s = s + (v[iv1] - v[iv]) / hy;
-        s = s + (w[iw1] - w[iw]) / hz;
+        double s = (face_value<F>(u, fu, a, iu1) - face_value<F>(u, fu, a, iu)) / hx;
+        s = s + (face_value<F>(v, fv, a, iv1) - face_value<F>(v, fv, a, iv)) / hy;
+        s = s + (face_value<F>(w, fw, a, iw1) - face_value<F>(w, fw, a, iw)) / hz;
         out[i] = scale * s;
     }
 }
 
 // u_C -= coef * (p[i]-p[i-1])/h on interior faces (all faces if periodic).
-template <int C, bool P>
+template <int C, bool P, bool F = false>
 __global__ void __launch_bounds__(256) grad_correct_kernel(double* __restrict__ u,
                                                            const double* __restrict__ p, Dims d,
-                                                           double h, double coef) {
+                                                           double h, double coef,
+                                                           const double* __restrict__ f = nullptr,
+                                                           double a = 0.0) {
     const int64_t f0 = d.n0 + (C == 0 && !P), f1 = d.n1 + (C == 1 && !P), f2 = d.n2 + (C == 2 && !P);
     const int64_t total = f0 * f1 * f2;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
@@ -250,7 +264,11 @@ __global__ void __launch_bounds__(256) grad_correct_kernel(double* __restrict__ 
         const int64_t z = i % f2, y = (i / f2) % f1, x = i / (f1 * f2);
         const int64_t q = C == 0 ? x : (C == 1 ? y : z);
         const int64_t nq = C == 0 ? d.n0 : (C == 1 ? d.n1 : d.n2);
-        if (!P && (q == 0 || q == nq)) continue;
+        const double star = face_value<F>(u, f, a, i);
+        if (!P && (q == 0 || q == nq)) {
+            if (F) u[i] = star;
+            continue;
+        }
         const int64_t qm = q > 0 ? q - 1 : nq - 1;
         const int64_t qh = P ? q : q;  // face q lies between cells q-1 and q
         auto cell = [&](int64_t a, int64_t b, int64_t c) { return (a * d.n1 + b) * d.n2 + c; };
@@ -259,7 +277,7 @@ __global__ void __launch_bounds__(256) grad_correct_kernel(double* __restrict__ 
         else if (C == 1) { hi = cell(x, qh, z); lo = cell(x, qm, z); }
         else { hi = cell(x, y, qh); lo = cell(x, y, qm); }
         const double g = (p[hi] - p[lo]) / h;
-        u[i] = u[i] - coef * g;
+        u[i] = star - coef * g;
     }
 }
 
@@ -495,6 +513,45 @@ int afsi_grad_correct_3d_f64(double* u, double* v, double* w, const double* p, c
         grad_correct_kernel<0, false><<<g, 256, 0, s>>>(u, p, d, h[0], coef);
         grad_correct_kernel<1, false><<<g, 256, 0, s>>>(v, p, d, h[1], coef);
         grad_correct_kernel<2, false><<<g, 256, 0, s>>>(w, p, d, h[2], coef);
+    }
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_project_rhs_3d_f64(const double* u, const double* v, const double* w, const double* fu,
+                            const double* fv, const double* fw, double a, double* out,
+                            const int64_t n[3], const double h[3], double scale, int periodic,
+                            void* stream) {
+    AFSI_REQUIRE(dims_ok(n) && h && u && v && w && fu && fv && fw && out,
+                 "invalid projection arguments");
+    const Dims d{n[0], n[1], n[2]};
+    const unsigned g = grid_for(d.n0 * d.n1 * d.n2, 256, 16384);
+    const hipStream_t s = hipStream_t(stream);
+    if (periodic)
+        divergence_kernel<true, true><<<g, 256, 0, s>>>(u, v, w, out, d, h[0], h[1], h[2], scale, fu, fv, fw, a);
+    else
+        divergence_kernel<false, true><<<g, 256, 0, s>>>(u, v, w, out, d, h[0], h[1], h[2], scale, fu, fv, fw, a);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_project_correct_3d_f64(double* u, double* v, double* w, const double* fu,
+                                const double* fv, const double* fw, double a, const double* p,
+                                const int64_t n[3], const double h[3], double coef, int periodic,
+                                void* stream) {
+    AFSI_REQUIRE(dims_ok(n) && h && u && v && w && fu && fv && fw && p, "invalid projection arguments");
+    const Dims d{n[0], n[1], n[2]};
+    const hipStream_t s = hipStream_t(stream);
+    const int64_t cells = d.n0 * d.n1 * d.n2;
+    const unsigned g = grid_for(cells + d.n1 * d.n2 + d.n0 * d.n2 + d.n0 * d.n1, 256, 16384);
+    if (periodic) {
+        grad_correct_kernel<0, true, true><<<g, 256, 0, s>>>(u, p, d, h[0], coef, fu, a);
+        grad_correct_kernel<1, true, true><<<g, 256, 0, s>>>(v, p, d, h[1], coef, fv, a);
+        grad_correct_kernel<2, true, true><<<g, 256, 0, s>>>(w, p, d, h[2], coef, fw, a);
+    } else {
+        grad_correct_kernel<0, false, true><<<g, 256, 0, s>>>(u, p, d, h[0], coef, fu, a);
+        grad_correct_kernel<1, false, true><<<g, 256, 0, s>>>(v, p, d, h[1], coef, fv, a);
+        grad_correct_kernel<2, false, true><<<g, 256, 0, s>>>(w, p, d, h[2], coef, fw, a);
     }
     AFSI_LAUNCH_CHECK();
     return OK;

@@ -76,11 +76,11 @@ static TileSpec make_spec(const afsi_grid3& g, const int32_t tile[3]) {
 // Box extents as compile-time constants for cubic tiles of edge T (T = 0:
 // read them from the TileSpec at run time).
 template <int T>
-__device__ inline int box_dim(const TileSpec& ts, int c, int a) {
+__host__ __device__ inline int box_dim(const TileSpec& ts, int c, int a) {
     return T > 0 ? T + 4 - (a == c ? 1 : 0) : ts.dim[c][a];
 }
 template <int T>
-__device__ inline int box_off(const TileSpec& ts, int c) {
+__host__ __device__ inline int box_off(const TileSpec& ts, int c) {
     return T > 0 ? c * (T + 3) * (T + 4) * (T + 4) : ts.off[c];
 }
 template <int T>
@@ -88,7 +88,7 @@ __device__ inline int tile_edge(const TileSpec& ts, int a) {
     return T > 0 ? T : ts.t[a];
 }
 
-__device__ inline int wrap32(int i, int n) {
+__host__ __device__ inline int wrap32(int i, int n) {
     const int r = i % n;
     return r < 0 ? r + n : r;
 }
@@ -222,10 +222,14 @@ __device__ inline WorkItem work_item(int w, const int32_t* __restrict__ chunk_st
 }
 
 // Box entry e of component c -> global face offset, or -1 outside a closed
-// box. Box axis a starts at t0[a] - 1 (own axis) or t0[a] - 2.
+// box. Box axis a starts at t0[a] - 1 (own axis) or t0[a] - 2, so its
+// coordinates lie in [t0 - 2, t0 + tile + 1]. A periodic coordinate wraps
+// once when the tile is shorter than the axis; a tile at least as long as
+// a periodic axis (e.g. a 4-cell axis under the 8-cell default tile) needs
+// the full modulo, taken only on that rare branch.
 template <int T>
-__device__ inline int box_face(int e, int c, const int t0[3], const TileSpec& ts,
-                               const afsi_grid3& g) {
+__host__ __device__ inline int box_face(int e, int c, const int t0[3], const TileSpec& ts,
+                                        const afsi_grid3& g) {
     const int d1 = box_dim<T>(ts, c, 1), d2 = box_dim<T>(ts, c, 2);
     const int l[3] = {e / (d1 * d2), (e / d2) % d1, e % d2};
     int idx = 0;
@@ -233,8 +237,13 @@ __device__ inline int box_face(int e, int c, const int t0[3], const TileSpec& ts
     for (int a = 0; a < 3; ++a) {
         const int nf = int(g.n[a]) + ((!g.periodic && a == c) ? 1 : 0);
         int q = t0[a] - (a == c ? 1 : 2) + l[a];
-        if (g.periodic) q = q < 0 ? q + nf : (q >= nf ? q - nf : q);
-        else if (q < 0 || q >= nf) return -1;
+        if (g.periodic) {
+            if (q < 0) q += nf;
+            if (q >= nf) q -= nf;
+            if (q < 0 || q >= nf) q = wrap32(q, nf);
+        } else if (q < 0 || q >= nf) {
+            return -1;
+        }
         idx = idx * nf + q;
     }
     return idx;
@@ -697,6 +706,29 @@ static int xfer_ready(const afsi_tb_batch* bt, XferLayout& L) {
 using namespace afsi;
 
 extern "C" {
+
+int afsi_tile_box_bounds(const afsi_grid3* grid, const int32_t tile[3], int64_t lo[3], int64_t hi[3]) {
+    AFSI_REQUIRE(grid && tile && lo && hi, "invalid box bounds arguments");
+    AFSI_REQUIRE(grid->n[0] > 0 && grid->n[1] > 0 && grid->n[2] > 0, "invalid grid");
+    if (int r = check_tiled(*grid, tile)) return r;
+    const TileSpec ts = make_spec(*grid, tile);
+    for (int c = 0; c < 3; ++c) {
+        lo[c] = INT64_MAX;
+        hi[c] = -1;
+        const int vol = box_dim<0>(ts, c, 0) * box_dim<0>(ts, c, 1) * box_dim<0>(ts, c, 2);
+        for (int64_t t = 0; t < ts.ntiles; ++t) {
+            const int t0[3] = {int(t / (ts.nt[2] * ts.nt[1])) * ts.t[0], int((t / ts.nt[2]) % ts.nt[1]) * ts.t[1],
+                               int(t % ts.nt[2]) * ts.t[2]};
+            for (int e = 0; e < vol; ++e) {
+                const int f = box_face<0>(e, c, t0, ts, *grid);
+                if (f < 0) continue;
+                lo[c] = f < lo[c] ? f : lo[c];
+                hi[c] = f > hi[c] ? f : hi[c];
+            }
+        }
+    }
+    return OK;
+}
 
 size_t afsi_spread_workspace_bytes(const afsi_grid3* grid, const int32_t tile[3]) {
     if (!grid || !tile || tile[0] <= 0 || tile[1] <= 0 || tile[2] <= 0) return 0;

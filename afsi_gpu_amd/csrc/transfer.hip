@@ -1,11 +1,26 @@
-// Stencil preparation, compact separable and generic indexed IB transfer.
+// Stencil preparation and the reference-compatible compact / indexed IB
+// transfers (op layouts of mac/csrc/transfer.cpp:34-53: compact stencils
+// base (P,3) int64 + phi (P,3,4) per component, face arrays flat C-order,
+// z fastest; indexed stencils ids (P,K) int64 + weights (P,K), fields (N,C)).
 //
-// Layouts follow the reference op boundary (mac/csrc/transfer.cpp:49-52):
-// compact stencils base (P,3) int64 + phi (P,3,4) per component; face arrays
-// flat C-order (z fastest). One 64-lane wavefront per point: lane k owns link
-// (k/16, (k/4)%4, k%4), so a point's 64 links map onto exactly one wave64
-// with no idle lanes and the gather reduction is a single wave reduction.
-#include "common.hpp"
+// Design (gfx950):
+// * Gathers: one thread per point (compact) or per (point, component)
+//   (indexed). A compact point's 64 taps are summed separably from its
+//   12 one-dimensional weights (x rows of y rows of 4 contiguous z values,
+//   fused multiply-adds), so a thread issues 16 row reads of 4 doubles
+//   instead of 64 scattered ones.
+// * Spreads are gathers too, so no floating-point atomics are used and the
+//   result does not depend on scheduling: the points (compact) or links
+//   (indexed) are stably sorted by their target with the binning's radix
+//   passes, and every output entry sums its contributors itself in a fixed
+//   order (compact: the 4x4 columns of stencil bases that reach a face, in
+//   x-then-y order, each column's points in sorted order; indexed: a node's
+//   links in link order).
+// Out-of-range stencils (the reference's device assert,
+// mac/csrc/transfer.cu:19,41) raise F_INDEX: their in-range taps are kept
+// by the gathers, and the spreads drop the point (compact) or link
+// (indexed); callers turn the flag into an error.
+#include "binshare.hpp"
 #include <cstring>
 #include <string>
 
@@ -47,74 +62,11 @@ __global__ void prepare_compact_kernel(const double* __restrict__ x, int64_t poi
     for (int k = 0; k < 4; ++k) ph[k] = peskin_phi(scaled - double(b + k));
 }
 
-// ---------------------------------------------------------------------------
-// compact gather / spread, one wave per point, lane = link
-
-template <class T>
-__device__ inline T wave_sum(T v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
-    return v;
-}
-
-template <class T, bool Spread>
-__global__ void __launch_bounds__(256) compact_kernel(const T* __restrict__ f, int64_t fstride,
-                                                      const int64_t* __restrict__ base,
-                                                      const T* __restrict__ phi, T* out,
-                                                      int64_t ostride, int64_t points,
-                                                      int64_t nx, int64_t ny, int64_t nz,
-                                                      T volume, int* flags) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int64_t p = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / WAVE;
-    if (p >= points) return;
-    const int i = lane >> 4, j = (lane >> 2) & 3, l = lane & 3;
-    const int64_t xi = base[3 * p] + i, yj = base[3 * p + 1] + j, zl = base[3 * p + 2] + l;
-    const bool ok = xi >= 0 && xi < nx && yj >= 0 && yj < ny && zl >= 0 && zl < nz;
-    const T wgt = (phi[12 * p + i] * phi[12 * p + 4 + j]) * phi[12 * p + 8 + l];
-    const int64_t id = (xi * ny + yj) * nz + zl;
-    if (!ok) raise_flag(flags, F_INDEX);
-    if constexpr (Spread) {
-        if (ok) unsafeAtomicAdd(out + id, (wgt * f[p * fstride]) / volume);
-    } else {
-        T s = ok ? f[id] * wgt : T(0);
-        s = wave_sum(s);
-        if (lane == 0) out[p * ostride] = s;
-    }
-}
-
 template <class T>
 __global__ void fill_kernel(T* x, int64_t count, T value) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
          i += int64_t(gridDim.x) * blockDim.x)
         x[i] = value;
-}
-
-// ---------------------------------------------------------------------------
-// indexed gather / spread, one wave per (point, component), lanes over K
-
-template <class T, bool Spread>
-__global__ void __launch_bounds__(256) indexed_kernel(const T* __restrict__ f,
-                                                      const int64_t* __restrict__ ids,
-                                                      const T* __restrict__ w, T* out,
-                                                      int64_t points, int64_t neighbors,
-                                                      int64_t comps, int64_t nodes, T volume,
-                                                      int* flags) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int64_t task = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / WAVE;
-    if (task >= points * comps) return;
-    const int64_t p = task / comps, c = task % comps;
-    T s = T(0);
-    for (int64_t k = lane; k < neighbors; k += WAVE) {
-        const int64_t j = ids[p * neighbors + k];
-        if (j < 0 || j >= nodes) { raise_flag(flags, F_INDEX); continue; }
-        const T wt = w[p * neighbors + k];
-        if constexpr (Spread) unsafeAtomicAdd(out + j * comps + c, (f[p * comps + c] * wt) / volume);
-        else s += f[j * comps + c] * wt;
-    }
-    if constexpr (!Spread) {
-        s = wave_sum(s);
-        if (lane == 0) out[p * comps + c] = s;
-    }
 }
 
 template <class T>
@@ -127,6 +79,187 @@ int launch_fill(T* x, int64_t count, T value, hipStream_t s) {
 template int launch_fill<double>(double*, int64_t, double, hipStream_t);
 template int launch_fill<float>(float*, int64_t, float, hipStream_t);
 
+// ---------------------------------------------------------------------------
+// compact gather: one thread per point
+
+template <class T>
+__global__ void __launch_bounds__(256) compact_gather_kernel(const T* __restrict__ field,
+                                                             const int64_t* __restrict__ base,
+                                                             const T* __restrict__ phi, T* __restrict__ out,
+                                                             int64_t ostride, int64_t points, int64_t nx,
+                                                             int64_t ny, int64_t nz, int* flags) {
+    for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < points;
+         p += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t bx = base[3 * p], by = base[3 * p + 1], bz = base[3 * p + 2];
+        T w[3][4];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[a][k] = phi[12 * p + 4 * a + k];
+        const bool inside = bx >= 0 && bx + 3 < nx && by >= 0 && by + 3 < ny && bz >= 0 && bz + 3 < nz;
+        T sum = T(0);
+        if (inside) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                T sy = T(0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const T* row = field + ((bx + i) * ny + (by + j)) * nz + bz;
+                    T sz = row[0] * w[2][0];
+#pragma unroll
+                    for (int k = 1; k < 4; ++k) sz = __builtin_fma(row[k], w[2][k], sz);
+                    sy = __builtin_fma(w[1][j], sz, sy);
+                }
+                sum = __builtin_fma(w[0][i], sy, sum);
+            }
+        } else {
+            raise_flag(flags, F_INDEX);
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j)
+                    for (int k = 0; k < 4; ++k) {
+                        const int64_t xi = bx + i, yj = by + j, zk = bz + k;
+                        if (xi < 0 || xi >= nx || yj < 0 || yj >= ny || zk < 0 || zk >= nz) continue;
+                        sum += ((w[0][i] * w[1][j]) * w[2][k]) * field[(xi * ny + yj) * nz + zk];
+                    }
+        }
+        out[p * ostride] = sum;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// compact spread: sort points by stencil base, then one thread per face
+
+// Base b (per axis in [0, n-4]) -> key over the nb = n-3 valid positions
+// per axis; a stencil reaching outside the face array gets the sentinel
+// key `space` (sorted last, never visited) and raises F_INDEX.
+__global__ void compact_keys_kernel(const int64_t* __restrict__ base, int64_t points, int64_t nx,
+                                    int64_t ny, int64_t nz, uint32_t* __restrict__ keys,
+                                    uint32_t* __restrict__ vals, int* flags) {
+    const int64_t nbx = nx - 3, nby = ny - 3, nbz = nz - 3;
+    const int64_t space = nbx > 0 && nby > 0 && nbz > 0 ? nbx * nby * nbz : 0;
+    for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < points;
+         p += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t bx = base[3 * p], by = base[3 * p + 1], bz = base[3 * p + 2];
+        const bool ok = bx >= 0 && bx < nbx && by >= 0 && by < nby && bz >= 0 && bz < nbz;
+        if (!ok) raise_flag(flags, F_INDEX);
+        keys[p] = uint32_t(ok ? (bx * nby + by) * nbz + bz : space);
+        vals[p] = uint32_t(p);
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) compact_face_sum_kernel(
+    const T* __restrict__ force, int64_t fstride, const T* __restrict__ phi,
+    const uint32_t* __restrict__ keys, const int32_t* __restrict__ perm, const int32_t* __restrict__ ptr,
+    int64_t nx, int64_t ny, int64_t nz, double volume, T* __restrict__ out) {
+    const int64_t nby = ny - 3, nbz = nz - 3, nbx = nx - 3;
+    const T vol = T(volume);
+    const int64_t faces = nx * ny * nz;
+    for (int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; f < faces;
+         f += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t fz = f % nz, fy = (f / nz) % ny, fx = f / (ny * nz);
+        const int64_t zlo = fz - 3 > 0 ? fz - 3 : 0, zhi = fz < nbz - 1 ? fz : nbz - 1;
+        T acc = T(0);
+        if (zlo <= zhi) {
+            for (int64_t bx = fx - 3 > 0 ? fx - 3 : 0; bx <= fx && bx < nbx; ++bx) {
+                for (int64_t by = fy - 3 > 0 ? fy - 3 : 0; by <= fy && by < nby; ++by) {
+                    const int64_t col = (bx * nby + by) * nbz;
+                    const int32_t j1 = ptr[col + zhi + 1];
+                    for (int32_t j = ptr[col + zlo]; j < j1; ++j) {
+                        const int64_t p = perm[j];
+                        const int64_t bz = int64_t(keys[j]) - col;
+                        const T wgt = (phi[12 * p + (fx - bx)] * phi[12 * p + 4 + (fy - by)]) *
+                                      phi[12 * p + 8 + (fz - bz)];
+                        acc += (wgt * force[p * fstride]) / vol;
+                    }
+                }
+            }
+        }
+        out[f] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// indexed gather: one thread per (point, component), links in order
+
+template <class T>
+__global__ void __launch_bounds__(256) indexed_gather_kernel(const T* __restrict__ field,
+                                                             const int64_t* __restrict__ ids,
+                                                             const T* __restrict__ w, T* __restrict__ out,
+                                                             int64_t points, int64_t neighbors,
+                                                             int64_t comps, int64_t nodes, int* flags) {
+    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < points * comps;
+         t += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t p = t / comps, c = t % comps;
+        const int64_t* row = ids + p * neighbors;
+        const T* wr = w + p * neighbors;
+        T acc = T(0);
+        for (int64_t k = 0; k < neighbors; ++k) {
+            const int64_t j = row[k];
+            if (j < 0 || j >= nodes) {
+                raise_flag(flags, F_INDEX);
+                continue;
+            }
+            acc += field[j * comps + c] * wr[k];
+        }
+        out[t] = acc;
+    }
+}
+
+// indexed spread: sort links by target node, one thread per (node, comp)
+__global__ void link_keys_kernel(const int64_t* __restrict__ ids, int64_t links, int64_t size,
+                                 uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int* flags) {
+    for (int64_t l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; l < links;
+         l += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t j = ids[l];
+        const bool ok = j >= 0 && j < size;
+        if (!ok) raise_flag(flags, F_INDEX);
+        keys[l] = uint32_t(ok ? j : size);
+        vals[l] = uint32_t(l);
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) node_sum_kernel(const T* __restrict__ field, const T* __restrict__ w,
+                                                       const int32_t* __restrict__ perm,
+                                                       const int32_t* __restrict__ ptr, int64_t size,
+                                                       int64_t neighbors, int64_t comps, double volume,
+                                                       T* __restrict__ out) {
+    const T vol = T(volume);
+    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < size * comps;
+         t += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t node = t / comps, c = t % comps;
+        T acc = T(0);
+        for (int32_t j = ptr[node]; j < ptr[node + 1]; ++j) {
+            const int64_t l = perm[j];
+            acc += (field[(l / neighbors) * comps + c] * w[l]) / vol;
+        }
+        out[t] = acc;
+    }
+}
+
+// Workspace of a sort-based spread over n items and `space` targets: the
+// radix sort's scratch, then sorted keys, permutation and CSR offsets.
+struct SpreadSpace {
+    size_t sort_bytes, keys_off, perm_off, ptr_off, total;
+};
+
+static size_t alignb(size_t b) { return (b + 255) & ~size_t(255); }
+
+static SpreadSpace spread_space(int64_t n, int64_t space) {
+    SpreadSpace w{};
+    w.sort_bytes = alignb(bin_workspace_bytes(n > 0 ? n : 1));
+    w.keys_off = w.sort_bytes;
+    w.perm_off = w.keys_off + alignb(size_t(n > 0 ? n : 1) * 4);
+    w.ptr_off = w.perm_off + alignb(size_t(n > 0 ? n : 1) * 4);
+    w.total = w.ptr_off + alignb(size_t(space + 1) * 4);
+    return w;
+}
+
+static int64_t compact_space(int64_t nx, int64_t ny, int64_t nz) {
+    return nx > 3 && ny > 3 && nz > 3 ? (nx - 3) * (ny - 3) * (nz - 3) : 0;
+}
+
 template <class T>
 static int compact_gather(const T* field, const int64_t* base, const T* phi, int64_t points,
                           int64_t nx, int64_t ny, int64_t nz, T* out, int64_t stride,
@@ -135,9 +268,8 @@ static int compact_gather(const T* field, const int64_t* base, const T* phi, int
                  "invalid compact IB field/base");
     if (points == 0) return OK;
     AFSI_REQUIRE(field && base && phi && out, "null pointer");
-    const hipStream_t s = hipStream_t(stream);
-    compact_kernel<T, false><<<grid_for(points * WAVE, 256), 256, 0, s>>>(
-        field, 0, base, phi, out, stride, points, nx, ny, nz, T(1), flags);
+    compact_gather_kernel<T><<<grid_for(points, 256, 16384), 256, 0, hipStream_t(stream)>>>(
+        field, base, phi, out, stride, points, nx, ny, nz, flags);
     AFSI_LAUNCH_CHECK();
     return OK;
 }
@@ -145,17 +277,30 @@ static int compact_gather(const T* field, const int64_t* base, const T* phi, int
 template <class T>
 static int compact_spread(const T* force, int64_t fstride, const int64_t* base, const T* phi,
                           int64_t points, int64_t nx, int64_t ny, int64_t nz, double volume,
-                          T* out, int* flags, void* stream) {
+                          T* out, void* workspace, size_t workspace_bytes, int* flags, void* stream) {
     AFSI_REQUIRE(points >= 0 && nx > 0 && ny > 0 && nz > 0 && fstride > 0 &&
                      std::isfinite(volume) && volume > 0,
                  "invalid compact grid/volume");
     AFSI_REQUIRE(out, "null pointer");
+    AFSI_REQUIRE(points < (int64_t(1) << 31) && nx * ny * nz < (int64_t(1) << 31),
+                 "compact spread exceeds 2^31 points or faces");
     const hipStream_t s = hipStream_t(stream);
-    if (int r = launch_fill<T>(out, nx * ny * nz, T(0), s)) return r;
-    if (points == 0) return OK;
-    AFSI_REQUIRE(force && base && phi, "null pointer");
-    compact_kernel<T, true><<<grid_for(points * WAVE, 256), 256, 0, s>>>(
-        force, fstride, base, phi, out, 0, points, nx, ny, nz, T(volume), flags);
+    const int64_t space = compact_space(nx, ny, nz);
+    if (points == 0) return launch_fill<T>(out, nx * ny * nz, T(0), s);
+    AFSI_REQUIRE(force && base && phi && workspace, "null pointer");
+    const SpreadSpace L = spread_space(points, space);
+    AFSI_REQUIRE(workspace_bytes >= L.total, "compact spread workspace too small");
+    char* ws = static_cast<char*>(workspace);
+    uint32_t *k0, *v0;
+    sort_key_slots(ws, points, &k0, &v0);
+    compact_keys_kernel<<<grid_for(points, 256, 8192), 256, 0, s>>>(base, points, nx, ny, nz, k0, v0, flags);
+    AFSI_LAUNCH_CHECK();
+    uint32_t* keys = reinterpret_cast<uint32_t*>(ws + L.keys_off);
+    int32_t* perm = reinterpret_cast<int32_t*>(ws + L.perm_off);
+    int32_t* ptr = reinterpret_cast<int32_t*>(ws + L.ptr_off);
+    if (int r = sort_pairs(points, space, ws, keys, perm, ptr, s)) return r;
+    compact_face_sum_kernel<T><<<grid_for(nx * ny * nz, 256, 16384), 256, 0, s>>>(
+        force, fstride, phi, keys, perm, ptr, nx, ny, nz, volume, out);
     AFSI_LAUNCH_CHECK();
     return OK;
 }
@@ -168,9 +313,8 @@ static int indexed_gather(const T* field, int64_t nodes, int64_t comps, const in
     AFSI_REQUIRE(comps > 0 && neighbors > 0 && points >= 0, "IB indexed stencil shape mismatch");
     if (points == 0) return OK;
     AFSI_REQUIRE(field && ids && w && out, "null pointer");
-    const hipStream_t s = hipStream_t(stream);
-    indexed_kernel<T, false><<<grid_for(points * comps * WAVE, 256), 256, 0, s>>>(
-        field, ids, w, out, points, neighbors, comps, nodes, T(1), flags);
+    indexed_gather_kernel<T><<<grid_for(points * comps, 256, 16384), 256, 0, hipStream_t(stream)>>>(
+        field, ids, w, out, points, neighbors, comps, nodes, flags);
     AFSI_LAUNCH_CHECK();
     return OK;
 }
@@ -178,16 +322,28 @@ static int indexed_gather(const T* field, int64_t nodes, int64_t comps, const in
 template <class T>
 static int indexed_spread(const T* field, int64_t points, int64_t comps, const int64_t* ids,
                           const T* w, int64_t neighbors, int64_t size, double volume, T* out,
-                          int* flags, void* stream) {
+                          void* workspace, size_t workspace_bytes, int* flags, void* stream) {
     AFSI_REQUIRE(size > 0 && std::isfinite(volume) && volume > 0, "invalid IB output size/volume");
     AFSI_REQUIRE(comps > 0 && neighbors > 0 && points >= 0, "IB indexed stencil shape mismatch");
     AFSI_REQUIRE(out, "null pointer");
+    const int64_t links = points * neighbors;
+    AFSI_REQUIRE(links < (int64_t(1) << 31) && size < (int64_t(1) << 31), "indexed spread exceeds 2^31 links or nodes");
     const hipStream_t s = hipStream_t(stream);
-    if (int r = launch_fill<T>(out, size * comps, T(0), s)) return r;
-    if (points == 0) return OK;
-    AFSI_REQUIRE(field && ids && w, "null pointer");
-    indexed_kernel<T, true><<<grid_for(points * comps * WAVE, 256), 256, 0, s>>>(
-        field, ids, w, out, points, neighbors, comps, size, T(volume), flags);
+    if (points == 0) return launch_fill<T>(out, size * comps, T(0), s);
+    AFSI_REQUIRE(field && ids && w && workspace, "null pointer");
+    const SpreadSpace L = spread_space(links, size);
+    AFSI_REQUIRE(workspace_bytes >= L.total, "indexed spread workspace too small");
+    char* ws = static_cast<char*>(workspace);
+    uint32_t *k0, *v0;
+    sort_key_slots(ws, links, &k0, &v0);
+    link_keys_kernel<<<grid_for(links, 256, 8192), 256, 0, s>>>(ids, links, size, k0, v0, flags);
+    AFSI_LAUNCH_CHECK();
+    uint32_t* keys = reinterpret_cast<uint32_t*>(ws + L.keys_off);
+    int32_t* perm = reinterpret_cast<int32_t*>(ws + L.perm_off);
+    int32_t* ptr = reinterpret_cast<int32_t*>(ws + L.ptr_off);
+    if (int r = sort_pairs(links, size, ws, keys, perm, ptr, s)) return r;
+    node_sum_kernel<T><<<grid_for(size * comps, 256, 16384), 256, 0, s>>>(field, w, perm, ptr, size, neighbors,
+                                                                          comps, volume, out);
     AFSI_LAUNCH_CHECK();
     return OK;
 }
@@ -199,7 +355,7 @@ using namespace afsi;
 extern "C" {
 
 const char* afsi_last_error(void) { return afsi::g_error; }
-int afsi_abi_version(void) { return 1; }
+int afsi_abi_version(void) { return 2; }
 
 int afsi_prepare_compact_f64(const double* x, int64_t points, const afsi_grid3* grid,
                              int64_t* base, double* phi, int* flags, void* stream) {
@@ -217,51 +373,61 @@ int afsi_prepare_compact_f64(const double* x, int64_t points, const afsi_grid3* 
 int afsi_compact_gather_f64(const double* field, const int64_t* base, const double* phi,
                             int64_t points, int64_t nx, int64_t ny, int64_t nz, double* out,
                             int64_t out_stride, int* flags, void* stream) {
-    return compact_gather<double>(field, base, phi, points, nx, ny, nz, out, out_stride, flags,
-                                  stream);
+    return compact_gather<double>(field, base, phi, points, nx, ny, nz, out, out_stride, flags, stream);
 }
 int afsi_compact_gather_f32(const float* field, const int64_t* base, const float* phi,
                             int64_t points, int64_t nx, int64_t ny, int64_t nz, float* out,
                             int64_t out_stride, int* flags, void* stream) {
-    return compact_gather<float>(field, base, phi, points, nx, ny, nz, out, out_stride, flags,
-                                 stream);
+    return compact_gather<float>(field, base, phi, points, nx, ny, nz, out, out_stride, flags, stream);
+}
+
+size_t afsi_compact_spread_workspace_bytes(int64_t points, int64_t nx, int64_t ny, int64_t nz) {
+    if (points < 0 || nx <= 0 || ny <= 0 || nz <= 0) return 0;
+    return spread_space(points, compact_space(nx, ny, nz)).total;
 }
 int afsi_compact_spread_f64(const double* force, int64_t force_stride, const int64_t* base,
                             const double* phi, int64_t points, int64_t nx, int64_t ny,
-                            int64_t nz, double volume, double* out, int* flags, void* stream) {
-    return compact_spread<double>(force, force_stride, base, phi, points, nx, ny, nz, volume,
-                                  out, flags, stream);
+                            int64_t nz, double volume, double* out, void* workspace,
+                            size_t workspace_bytes, int* flags, void* stream) {
+    return compact_spread<double>(force, force_stride, base, phi, points, nx, ny, nz, volume, out,
+                                  workspace, workspace_bytes, flags, stream);
 }
 int afsi_compact_spread_f32(const float* force, int64_t force_stride, const int64_t* base,
                             const float* phi, int64_t points, int64_t nx, int64_t ny, int64_t nz,
-                            double volume, float* out, int* flags, void* stream) {
-    return compact_spread<float>(force, force_stride, base, phi, points, nx, ny, nz, volume,
-                                 out, flags, stream);
+                            double volume, float* out, void* workspace, size_t workspace_bytes,
+                            int* flags, void* stream) {
+    return compact_spread<float>(force, force_stride, base, phi, points, nx, ny, nz, volume, out,
+                                 workspace, workspace_bytes, flags, stream);
 }
 
 int afsi_indexed_gather_f64(const double* field, int64_t nodes, int64_t comps,
                             const int64_t* ids, const double* weights, int64_t points,
                             int64_t neighbors, double* out, int* flags, void* stream) {
-    return indexed_gather<double>(field, nodes, comps, ids, weights, points, neighbors, out,
-                                  flags, stream);
+    return indexed_gather<double>(field, nodes, comps, ids, weights, points, neighbors, out, flags, stream);
 }
 int afsi_indexed_gather_f32(const float* field, int64_t nodes, int64_t comps, const int64_t* ids,
                             const float* weights, int64_t points, int64_t neighbors, float* out,
                             int* flags, void* stream) {
-    return indexed_gather<float>(field, nodes, comps, ids, weights, points, neighbors, out,
-                                 flags, stream);
+    return indexed_gather<float>(field, nodes, comps, ids, weights, points, neighbors, out, flags, stream);
+}
+
+size_t afsi_indexed_spread_workspace_bytes(int64_t points, int64_t neighbors, int64_t size) {
+    if (points < 0 || neighbors <= 0 || size <= 0) return 0;
+    return spread_space(points * neighbors, size).total;
 }
 int afsi_indexed_spread_f64(const double* field, int64_t points, int64_t comps,
                             const int64_t* ids, const double* weights, int64_t neighbors,
-                            int64_t size, double volume, double* out, int* flags, void* stream) {
-    return indexed_spread<double>(field, points, comps, ids, weights, neighbors, size, volume,
-                                  out, flags, stream);
+                            int64_t size, double volume, double* out, void* workspace,
+                            size_t workspace_bytes, int* flags, void* stream) {
+    return indexed_spread<double>(field, points, comps, ids, weights, neighbors, size, volume, out,
+                                  workspace, workspace_bytes, flags, stream);
 }
 int afsi_indexed_spread_f32(const float* field, int64_t points, int64_t comps, const int64_t* ids,
                             const float* weights, int64_t neighbors, int64_t size, double volume,
-                            float* out, int* flags, void* stream) {
+                            float* out, void* workspace, size_t workspace_bytes, int* flags,
+                            void* stream) {
     return indexed_spread<float>(field, points, comps, ids, weights, neighbors, size, volume, out,
-                                 flags, stream);
+                                 workspace, workspace_bytes, flags, stream);
 }
 
 int afsi_fill_f64(double* x, int64_t count, double value, void* stream) {

@@ -6,8 +6,13 @@ argument meaning, ownership (fresh outputs; inputs made contiguous) and
 error behaviour (mac/cuda_ib.py:38-90; mac/csrc/transfer.cpp:10-57), and
 run hand-written gfx950 kernels from ``libafsi_hip.so`` on the current
 stream. Out-of-range indices do not trap the device as the reference's
-CUDA_KERNEL_ASSERT does; they set a flag and are skipped, and
-``check=True`` turns a raised flag into ``RuntimeError``.
+CUDA_KERNEL_ASSERT does (mac/csrc/transfer.cu:19,41); they set a device
+flag, and every call reads it back and raises ``RuntimeError('IB index out
+of range')`` as the failed assert surfaces in the reference. ``check=False``
+skips that one host synchronisation for callers that validate stencils
+themselves. The spreads are deterministic and use no atomics: points or
+links are sorted by target and every output entry sums its contributors in
+a fixed order.
 """
 import threading
 import torch
@@ -73,7 +78,7 @@ def _raise_flags(flags, check):
         raise RuntimeError('IB index out of range')
 
 
-def indexed_gather(field, indices, weights, *, check=False):
+def indexed_gather(field, indices, weights, *, check=True):
     """U[p,c] = sum_k field[indices[p,k],c]*weights[p,k] (mac/cuda_ib.py:69-71)."""
     require_cuda(field)
     field, indices, weights = field.contiguous(), indices.contiguous(), weights.contiguous()
@@ -95,7 +100,7 @@ def indexed_gather(field, indices, weights, *, check=False):
     return out
 
 
-def indexed_spread(field, indices, weights, size, volume=1., *, check=False):
+def indexed_spread(field, indices, weights, size, volume=1., *, check=True):
     """Fresh zeroed (size,C) with out[ids[p,k],c] += (f[p,c]*w[p,k])/volume
     (mac/cuda_ib.py:74-76)."""
     require_cuda(field)
@@ -114,9 +119,11 @@ def indexed_spread(field, indices, weights, size, volume=1., *, check=False):
         raise RuntimeError('IB spread point count mismatch')
     out = torch.empty((size, field.size(1)), device=field.device, dtype=field.dtype)
     flags = _flags(field.device)
+    nbytes = int(_abi.load().afsi_indexed_spread_workspace_bytes(field.size(0), indices.size(1), size))
+    workspace = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=field.device)
     _abi.call(f'afsi_indexed_spread_{_suffix(field.dtype)}', _abi.ptr(field), field.size(0),
               field.size(1), _abi.ptr(indices), _abi.ptr(weights), indices.size(1), size, volume,
-              _abi.ptr(out), _abi.ptr(flags), _abi.stream())
+              _abi.ptr(out), _abi.ptr(workspace), workspace.numel(), _abi.ptr(flags), _abi.stream())
     _raise_flags(flags, check)
     return out
 
@@ -132,7 +139,7 @@ def _compact_args(stencil, c, dtype, device):
     return base, phi
 
 
-def compact_gather(grid, velocity, stencil, *, check=False):
+def compact_gather(grid, velocity, stencil, *, check=True):
     """(P,3) point velocities from three face arrays (mac/cuda_ib.py:79-83)."""
     require_cuda(velocity[0])
     dtype, device = velocity[0].dtype, velocity[0].device
@@ -153,7 +160,7 @@ def compact_gather(grid, velocity, stencil, *, check=False):
     return out
 
 
-def compact_spread(grid, force, stencil, *, check=False):
+def compact_spread(grid, force, stencil, *, check=True):
     """Three fresh face arrays f_c += (w*F_c)/V (mac/cuda_ib.py:86-90)."""
     require_cuda(force)
     force = force.contiguous()
@@ -164,14 +171,17 @@ def compact_spread(grid, force, stencil, *, check=False):
         raise RuntimeError('compact IB field length mismatch')
     flags = _flags(device)
     result = []
+    nbytes = max(int(_abi.load().afsi_compact_spread_workspace_bytes(count, *grid.face_shape(c)))
+                 for c in range(3))
+    workspace = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
     for c in range(3):
         shape = grid.face_shape(c)
         out = torch.empty(shape, device=device, dtype=dtype)
         base, phi = _compact_args(stencil, c, dtype, device)
         _abi.call(f'afsi_compact_spread_{_suffix(dtype)}',
                   _abi.c_void_p(force.data_ptr() + c * force.element_size()), 3, _abi.ptr(base),
-                  _abi.ptr(phi), count, *shape, float(grid.volume), _abi.ptr(out), _abi.ptr(flags),
-                  _abi.stream())
+                  _abi.ptr(phi), count, *shape, float(grid.volume), _abi.ptr(out), _abi.ptr(workspace),
+                  workspace.numel(), _abi.ptr(flags), _abi.stream())
         result.append(out)
     _raise_flags(flags, check)
     return tuple(result)

@@ -75,6 +75,23 @@ def divergence(velocity, grid, scale=1., out=None):
     return out
 
 
+def project_rhs(velocity, force, a, grid, scale, out):
+    """scale * D(u + a f) with u left unchanged (the IB predictor folded into
+    the projection's right-hand side)."""
+    _abi.call('afsi_project_rhs_3d_f64', *(_abi.ptr(u) for u in velocity), *(_abi.ptr(f) for f in force),
+              float(a), _abi.ptr(out), _abi.i64x3(grid.shape), _abi.f64x3(grid.spacing), float(scale),
+              int(grid.periodic), _abi.stream())
+    return out
+
+
+def project_correct(velocity, force, a, pressure, grid, coef):
+    """In place u_c = (u_c + a f_c) - coef * G_c p."""
+    _abi.call('afsi_project_correct_3d_f64', *(_abi.ptr(u) for u in velocity), *(_abi.ptr(f) for f in force),
+              float(a), _abi.ptr(pressure), _abi.i64x3(grid.shape), _abi.f64x3(grid.spacing), float(coef),
+              int(grid.periodic), _abi.stream())
+    return velocity
+
+
 def correct(velocity, pressure, grid, coef):
     """In place u_c -= coef * G_c p (mac/flow.py:66-67)."""
     _abi.call('afsi_grad_correct_3d_f64', *(_abi.ptr(u) for u in velocity), _abi.ptr(pressure),

@@ -213,9 +213,13 @@ class GeometricMultigrid:
         return self.results[0, 1].sqrt()
 
     @torch.no_grad()
-    def solve(self, rhs, initial=None, *, rtol=None, history=None):
+    def solve(self, rhs, initial=None, *, rtol=None, history=None, precheck=None):
         """mac/multigrid.py:125-172 (workspace branch :75-98). ``history``, if
-        a list, receives the level-0 residual norm after every check."""
+        a list, receives the level-0 residual norm after every check;
+        ``precheck``, if given, is called right after the first host
+        synchronisation (before any cycle and before the RHS checks), so a
+        caller can read device flags it copied to pinned memory earlier on
+        the stream without a synchronisation of its own."""
         rtol = self.options.rtol if rtol is None else rtol
         if not isfinite(rtol) or rtol < 0:
             raise ValueError('invalid multigrid solve tolerance')
@@ -240,6 +244,8 @@ class GeometricMultigrid:
         count = w_rhs.numel()
         (s1, _, a1, nf1), (_, _, _, nf2), (_, q3, _, _), (_, q0, _, nf0) = \
             self.results[[1, 2, 3, 0]].tolist()
+        if precheck is not None:
+            precheck()
         if nf1:
             raise ValueError('invalid pressure RHS')
         mean, absolute_mean = s1 / count, a1 / count
@@ -390,9 +396,10 @@ class MixedMultigrid:
                   _abi.stream())
 
     @torch.no_grad()
-    def solve(self, rhs, initial=None, *, rtol=None, history=None):
-        """Same contract as ``GeometricMultigrid.solve``; ``info['cycles']``
-        counts outer iterations (one FP32 V-cycle each)."""
+    def solve(self, rhs, initial=None, *, rtol=None, history=None, precheck=None):
+        """Same contract as ``GeometricMultigrid.solve`` (``precheck``
+        included); ``info['cycles']`` counts outer iterations (one FP32
+        V-cycle each)."""
         rtol = self.options.rtol if rtol is None else rtol
         if not isfinite(rtol) or rtol < 0:
             raise ValueError('invalid multigrid solve tolerance')
@@ -415,6 +422,8 @@ class MixedMultigrid:
         self._residual()
         count = self.b.numel()
         res = self.results.tolist()
+        if precheck is not None:
+            precheck()
         (s1, _, a1, nf1), (_, _, _, nf2), (_, q3, _, _), (_, q0, _, nf0) = res[1], res[2], res[3], res[0]
         if nf1:
             raise ValueError('invalid pressure RHS')

@@ -65,12 +65,23 @@ class PointBins:
     flags: torch.Tensor
 
     def check(self):
-        bits = int(self.flags.item())
-        if bits & 4:
-            raise ValueError('nonfinite IB point positions')
-        if bits & 1:
-            raise ValueError('MAC IB support reaches a wall or is nonfinite; enlarge/refine the fluid box')
-        return bits
+        return raise_flags(int(self.flags.item()))
+
+
+F_SUPPORT, F_INDEX, F_NONFINITE = 1, 2, 4
+
+
+def raise_flags(bits):
+    """The reference's errors for the kernels' flag bits: a point whose
+    support reaches a closed wall or is nonfinite raises its ValueError
+    (mac/compact_transfer.py:94-96,115-117); a stencil index outside the
+    face arrays raises RuntimeError (the device assert of
+    mac/csrc/transfer.cu:19,41)."""
+    if bits & (F_SUPPORT | F_NONFINITE):
+        raise ValueError('MAC IB support reaches a wall or is nonfinite; enlarge/refine the fluid box')
+    if bits & F_INDEX:
+        raise RuntimeError('IB index out of range')
+    return bits
 
 
 class Binner:
@@ -114,13 +125,16 @@ def bin_points(points, grid, tile=DEFAULT_TILE):
 class TiledTransfer:
     """Spread / interpolate over binned points for a fixed grid and tile."""
 
-    def __init__(self, grid, device, tile=DEFAULT_TILE):
+    def __init__(self, grid, device, tile=DEFAULT_TILE, flags=None):
+        """``flags``: an int32 device word to OR the kernels' error bits
+        into (e.g. the binner's, which it clears at every binning); by
+        default a word of its own."""
         self.grid, self.tile = grid, tuple(int(t) for t in tile)
         self._g = grid.abi()
         self._tile = _abi.i32x3(self.tile)
         nbytes = int(_abi.load().afsi_spread_workspace_bytes(self._g, self._tile))
         self.workspace = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
-        self.flags = torch.zeros((), dtype=torch.int32, device=device)
+        self.flags = torch.zeros((), dtype=torch.int32, device=device) if flags is None else flags
 
     def spread(self, points, force, bins, out=None):
         """Three face arrays f_c = sum_p w_c(p) F_c(p) / V."""

@@ -6,15 +6,19 @@ spread the lagged point forces, project u* = u + (dt/rho) f with the
 previous pressure as the initial guess (mac/flow.py:101-120), interpolate
 the projected velocity and move the points x += dt U. Point forces are
 given directly (the FE mass solve and predictor physics are off-path).
-All work is on the current stream; the only host synchronisations are the
-multigrid convergence checks.
+All work is on the current stream. The only host synchronisations are the
+multigrid convergence checks; the binning and spread error flags are copied
+to pinned host memory on the stream and read at the solver's first
+synchronisation, before any velocity is changed, so a point whose support
+reaches a closed wall or is nonfinite raises the reference's ValueError
+(mac/compact_transfer.py:94-96,115-117) with the step's state untouched.
 """
 import torch
 
 from . import _abi
-from .mac.flow import MACFlow, correct, divergence
+from .mac.flow import MACFlow, correct, divergence, project_correct, project_rhs
 from .mac.multigrid import MGOptions
-from .mac.transfer import DEFAULT_TILE, Binner, TiledTransfer
+from .mac.transfer import DEFAULT_TILE, Binner, TiledTransfer, raise_flags
 
 
 class IBStep:
@@ -28,31 +32,35 @@ class IBStep:
         self.flow = MACFlow(grid, dt=dt, rho=rho, device=device, options=self.options,
                             pressure_backend=pressure_backend, precision=precision, predictor=False)
         self.binner = Binner(grid, self.count, self.device, tile)
-        self.transfer = TiledTransfer(grid, self.device, tile)
+        # one error word for binning and both transfers (cleared by the binner)
+        self.transfer = TiledTransfer(grid, self.device, tile, flags=self.binner.flags)
+        self.flags_host = torch.zeros((), dtype=torch.int32).pin_memory()
         self.force_density = grid.zeros(device=self.device)
         self.rhs = torch.empty(grid.shape, device=self.device, dtype=torch.float64)
         self.point_velocity = torch.empty((self.count, 3), device=self.device, dtype=torch.float64)
 
+    def _check_flags(self):
+        raise_flags(int(self.flags_host))
+
     def step(self, x, force, velocity, pressure=None, *, advance=True, timers=None):
         """Returns (velocity, pressure, point_velocity, info); ``velocity``
-        tensors are updated in place to u* and then to the projected field;
-        with ``advance`` the points ``x`` move by dt*U in place. ``timers``
-        maps phase names ('bin', 'spread', 'project', 'interp') to
-        (start, end) events recorded on the current stream around them."""
+        tensors are updated in place to the projected field; with
+        ``advance`` the points ``x`` move by dt*U in place. ``timers`` maps
+        phase names ('bin', 'spread', 'project', 'interp') to (start, end)
+        events recorded on the current stream around them."""
         mark = _Marks(timers)
         mark('bin', 0)
         bins = self.binner(x)
         mark('bin', 1)
         mark('spread', 0)
         f = self.transfer.spread(x, force, bins, out=self.force_density)
+        self.flags_host.copy_(self.binner.flags, non_blocking=True)
         mark('spread', 1)
         mark('project', 0)
         a = self.dt / self.rho
-        for uc, fc in zip(velocity, f):
-            _abi.call('afsi_axpy_f64', _abi.ptr(uc), _abi.ptr(fc), uc.numel(), a, _abi.stream())
-        rhs = divergence(velocity, self.grid, -self.rho / self.dt, out=self.rhs)
-        p, info = self.flow.pressure_solver.solve(rhs, pressure)
-        correct(velocity, p, self.grid, a)
+        rhs = project_rhs(velocity, f, a, self.grid, -self.rho / self.dt, out=self.rhs)
+        p, info = self.flow.pressure_solver.solve(rhs, pressure, precheck=self._check_flags)
+        project_correct(velocity, f, a, p, self.grid, a)
         mark('project', 1)
         mark('interp', 0)
         U = self.transfer.interpolate(x, velocity, bins, out=self.point_velocity)

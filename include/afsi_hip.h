@@ -37,7 +37,7 @@ int afsi_abi_version(void);
 
 /* ---- Stencil preparation -------------------------------------------------
  * Replaces mac/compact_transfer.py:21-26 (_prepare) as used by
- * CompactFETransfer.from_points (:284-290): base (3,P,3) int64 and phi
+ * CompactFETransfer.from_points (:144-150): base (3,P,3) int64 and phi
  * (3,P,3,4) for the three staggered lattices, phi with the reference's two-
  * branch formula (ib.py:57-67). Sets flag 1 for points failing the support
  * predicate of mac/compact_transfer.py:94-96,115-117 (closed box only). */
@@ -48,39 +48,57 @@ int afsi_prepare_compact_f64(const double* x, int64_t points, const afsi_grid3* 
  * Replace torch.ops.afsi_ib_cuda.compact_gather / compact_spread
  * (mac/csrc/transfer.cpp:34-53; mac/cuda_ib.py:79-90).
  * field/out are the flat C-order face array of shape (nx,ny,nz) of this
- * component; base (P,3) int64, phi (P,3,4). The gather writes out[p*stride].
- * The spread reads force[p*stride], zero-fills `out` and accumulates
- * (w*f)/volume with hardware FP atomics (out = fresh zeroed result). */
+ * component; base (P,3) int64, phi (P,3,4). The gather writes out[p*stride]
+ * (one thread per point, separable sums). The spread reads
+ * force[p*stride] and writes every entry of `out` (a fresh result) with
+ * sum (w*f)/volume over the points whose stencil reaches it: the points are
+ * stably sorted by stencil base and each face sums its 4x4 base columns in
+ * a fixed order, so the result is deterministic and uses no atomics. The
+ * spread needs `workspace` of afsi_compact_spread_workspace_bytes(). A
+ * stencil reaching outside the face array sets flag 2 (the reference's
+ * device assert): the gather keeps its in-range taps, the spread drops the
+ * point. */
 int afsi_compact_gather_f64(const double* field, const int64_t* base, const double* phi,
                             int64_t points, int64_t nx, int64_t ny, int64_t nz,
                             double* out, int64_t out_stride, int* flags, void* stream);
 int afsi_compact_gather_f32(const float* field, const int64_t* base, const float* phi,
                             int64_t points, int64_t nx, int64_t ny, int64_t nz,
                             float* out, int64_t out_stride, int* flags, void* stream);
+size_t afsi_compact_spread_workspace_bytes(int64_t points, int64_t nx, int64_t ny, int64_t nz);
 int afsi_compact_spread_f64(const double* force, int64_t force_stride, const int64_t* base,
                             const double* phi, int64_t points, int64_t nx, int64_t ny,
-                            int64_t nz, double volume, double* out, int* flags, void* stream);
+                            int64_t nz, double volume, double* out, void* workspace,
+                            size_t workspace_bytes, int* flags, void* stream);
 int afsi_compact_spread_f32(const float* force, int64_t force_stride, const int64_t* base,
                             const float* phi, int64_t points, int64_t nx, int64_t ny,
-                            int64_t nz, double volume, float* out, int* flags, void* stream);
+                            int64_t nz, double volume, float* out, void* workspace,
+                            size_t workspace_bytes, int* flags, void* stream);
 
 /* ---- Generic indexed transfer --------------------------------------------
  * Replace torch.ops.afsi_ib_cuda.indexed_gather / indexed_spread
  * (mac/csrc/transfer.cpp:19-33,48-53; mac/cuda_ib.py:69-76): field (N,C),
- * ids (P,K) int64, signed weights (P,K). Spread zero-fills out (size,C) and
- * accumulates (f*w)/volume. Out-of-range ids set flag 2 and are skipped. */
+ * ids (P,K) int64, signed weights (P,K). The gather sums a point's links in
+ * order (one thread per point and component). The spread writes out
+ * (size,C) = sum (f*w)/volume over the links into each node: links are
+ * stably sorted by node and each node sums its links in link order
+ * (deterministic, no atomics); it needs `workspace` of
+ * afsi_indexed_spread_workspace_bytes(). Out-of-range ids set flag 2 and
+ * are skipped. */
 int afsi_indexed_gather_f64(const double* field, int64_t nodes, int64_t comps,
                             const int64_t* ids, const double* weights, int64_t points,
                             int64_t neighbors, double* out, int* flags, void* stream);
 int afsi_indexed_gather_f32(const float* field, int64_t nodes, int64_t comps,
                             const int64_t* ids, const float* weights, int64_t points,
                             int64_t neighbors, float* out, int* flags, void* stream);
+size_t afsi_indexed_spread_workspace_bytes(int64_t points, int64_t neighbors, int64_t size);
 int afsi_indexed_spread_f64(const double* field, int64_t points, int64_t comps,
                             const int64_t* ids, const double* weights, int64_t neighbors,
-                            int64_t size, double volume, double* out, int* flags, void* stream);
+                            int64_t size, double volume, double* out, void* workspace,
+                            size_t workspace_bytes, int* flags, void* stream);
 int afsi_indexed_spread_f32(const float* field, int64_t points, int64_t comps,
                             const int64_t* ids, const float* weights, int64_t neighbors,
-                            int64_t size, double volume, float* out, int* flags, void* stream);
+                            int64_t size, double volume, float* out, void* workspace,
+                            size_t workspace_bytes, int* flags, void* stream);
 
 /* ---- Point binning (build-own; no reference counterpart) ------------------
  * Integer cell of each point = face-lattice base + 1 on every axis
@@ -107,6 +125,11 @@ int afsi_bin_points(const double* x, int64_t points, const afsi_grid3* grid,
  * point gathers from HBM and tile/cell_ptr/workspace are unused. Cubic
  * tiles of edge 8 or 4 use compile-time box extents. */
 size_t afsi_spread_workspace_bytes(const afsi_grid3* grid, const int32_t tile[3]);
+/* Host only (no device work): the smallest and largest face offset, per
+ * component, that any tile's LDS box entry maps to in the tiled kernels;
+ * lets a caller check that every staged read and flushed write stays inside
+ * the face arrays for its grid and tile (build-own; no reference counterpart). */
+int afsi_tile_box_bounds(const afsi_grid3* grid, const int32_t tile[3], int64_t lo[3], int64_t hi[3]);
 int afsi_spread_tiled_f64(const double* x, const double* force, int64_t points,
                           const afsi_grid3* grid, const int32_t tile[3],
                           const int32_t* perm, const int32_t* cell_ptr,
@@ -223,10 +246,24 @@ int afsi_grad_correct_3d_f64(double* u, double* v, double* w, const double* p,
                              void* stream);
 /* y += a * x (the predictor u* = u + (dt/rho) f of the IB step). */
 int afsi_axpy_f64(double* y, const double* x, int64_t count, double a, void* stream);
+/* The IB step's projection with the predictor u* = u + a f folded in
+ * (mac/coupling.py:80-83 then mac/flow.py:117-119), leaving u untouched
+ * until the pressure is known:
+ *   afsi_project_rhs_3d_f64:     out = scale * D(u + a f) (u, f read only);
+ *   afsi_project_correct_3d_f64: u_c = (u_c + a f_c) - coef * G_c p.
+ * Each face value u + a f is formed exactly as afsi_axpy_f64 forms it. */
+int afsi_project_rhs_3d_f64(const double* u, const double* v, const double* w, const double* fu,
+                            const double* fv, const double* fw, double a, double* out,
+                            const int64_t n[3], const double h[3], double scale, int periodic,
+                            void* stream);
+int afsi_project_correct_3d_f64(double* u, double* v, double* w, const double* fu,
+                                const double* fv, const double* fw, double a, const double* p,
+                                const int64_t n[3], const double h[3], double coef, int periodic,
+                                void* stream);
 
 /* ---- Native V-cycle driver -------------------------------------------------
  * Replaces MGWorkspace.cycle/_block/residual_norm (mac/mg_workspace.py:
- * 246-305): issues every launch of `cycles` V-cycles on one stream from C++
+ * 70-133): issues every launch of `cycles` V-cycles on one stream from C++
  * (capturable as one hipGraph). Per level: nu sweeps (Jacobi ping-pong with
  * tmp, or in-place red-black GS), mean removal, fused residual-restrict,
  * coarse mean removal, zero coarse iterate, recurse, prolong-add, nu sweeps,

@@ -2,6 +2,8 @@
 declares (no GPU calls: the CPU suite only loads the library)."""
 import ctypes
 import re
+
+import numpy as np
 from pathlib import Path
 
 import pytest
@@ -77,3 +79,25 @@ def test_struct_layouts_match_the_header(tmp_path):
         assert got[cname, 'size'] == ctypes.sizeof(py), cname
         for field, _ in py._fields_:
             assert got[cname, field] == getattr(py, field).offset, (cname, field)
+
+
+@pytest.mark.parametrize('shape,periodic,tile', [((4, 16, 16), True, (8, 8, 8)), ((4, 4, 4), True, (8, 8, 8)),
+                                                 ((5, 7, 9), True, (4, 4, 4)), ((12, 20, 8), False, (8, 8, 8)),
+                                                 ((128, 128, 128), True, (8, 8, 8)), ((6, 6, 6), True, (3, 5, 8))])
+def test_tile_boxes_stay_inside_the_face_arrays(shape, periodic, tile):
+    """Every face offset a tile's LDS box entry maps to (the addresses the
+    tiled interpolation stages and the spread flushes) lies inside the face
+    arrays, including periodic axes shorter than the tile (host-side walk of
+    the device mapping, no GPU)."""
+    from afsi_gpu_amd import _abi
+    from afsi_gpu_amd.mac import MACGrid
+    if not _abi.LIBRARY.exists():
+        pytest.fail(f'{_abi.LIBRARY} not built (run __graft_entry__.build())')
+    g = MACGrid(shape, (1., 1., 1.), periodic=periodic)
+    lo, hi = (ctypes.c_int64 * 3)(), (ctypes.c_int64 * 3)()
+    _abi.call('afsi_tile_box_bounds', g.abi(), _abi.i32x3(tile), lo, hi)
+    for c in range(3):
+        count = int(np.prod(g.face_shape(c)))
+        assert 0 <= lo[c] and hi[c] < count, (c, lo[c], hi[c], count)
+        # with the halo every face is reachable from some tile box
+        assert lo[c] == 0 and hi[c] == count - 1

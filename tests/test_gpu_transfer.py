@@ -244,3 +244,87 @@ def test_transfer_tile_autotuner(gpu):
     assert edge in (4, 8) and set(times) == {4, 8}
     print('transfer tile times (ms):', times)
 
+
+
+def test_compact_spread_deterministic_and_dense(gpu):
+    """The sort-based compact spread gives the same bits on every call, with
+    a dense cluster of points sharing one stencil base."""
+    g, og = grids(False)
+    x, F = points_in(og, 6000, 21)
+    x[:2000] = x[0] + 1e-4 * np.random.default_rng(22).normal(size=(2000, 3))
+    base, phi = ib3d.prepare_compact(x, og)
+    f_ref = ib3d.spread_grid(F, base, phi, og)
+    st = from_points(torch.tensor(x, device=gpu), g)
+    Ft = torch.tensor(F, device=gpu)
+    first = cuda_ib.compact_spread(g, Ft, st)
+    second = cuda_ib.compact_spread(g, Ft, st)
+    for c in range(3):
+        assert torch.equal(first[c], second[c])
+        assert rel_err(first[c].cpu(), f_ref[c]) < TOL
+
+
+def test_compact_out_of_range_raises(gpu):
+    """A stencil reaching outside the face array is the reference's device
+    assert: the drop-in ops raise RuntimeError by default."""
+    g, og = grids(False)
+    x, F = points_in(og, 10, 23)
+    st = from_points(torch.tensor(x, device=gpu), g)
+    st.base[0, 3, 0] = -2
+    with pytest.raises(RuntimeError, match='out of range'):
+        cuda_ib.compact_spread(g, torch.tensor(F, device=gpu), st)
+    u = tuple(torch.zeros(og.face_shape(c), device=gpu, dtype=torch.float64) for c in range(3))
+    with pytest.raises(RuntimeError, match='out of range'):
+        cuda_ib.compact_gather(g, u, st)
+
+
+@pytest.mark.parametrize('shape,tile', [((4, 16, 16), (8, 8, 8)), ((16, 6, 4), (8, 8, 8))])
+def test_tiled_transfer_periodic_axis_shorter_than_tile(gpu, shape, tile):
+    """Periodic axes shorter than the tile: every staged box entry wraps into
+    the face arrays (the host-side walk in test_abi checks the offsets);
+    spread and interpolation equal the oracle's."""
+    g = MACGrid(shape, (1., 1., 1.), periodic=True)
+    og = ib3d.Grid3(shape, (1., 1., 1.), periodic=True)
+    gen = np.random.default_rng(24)
+    x = gen.uniform(0, 1, size=(3000, 3))
+    F = gen.normal(size=(3000, 3))
+    base, phi = ib3d.prepare_compact(x, og)
+    f_ref = ib3d.spread_grid(F, base, phi, og)
+    u = tuple(gen.normal(size=og.face_shape(c)) for c in range(3))
+    U_ref = ib3d.gather_grid(u, base, phi, og)
+    xt, Ft = torch.tensor(x, device=gpu), torch.tensor(F, device=gpu)
+    bins = bin_points(xt, g, tile)
+    tr = TiledTransfer(g, gpu, tile)
+    f = tr.spread(xt, Ft, bins)
+    U = tr.interpolate(xt, tuple(torch.tensor(a, device=gpu) for a in u), bins)
+    assert int(tr.flags.item()) == 0
+    for c in range(3):
+        assert rel_err(f[c].cpu(), f_ref[c]) < TOL
+    assert rel_err(U.cpu(), U_ref) < TOL
+
+
+def test_ib_step_raises_on_wall_support_before_changing_state(gpu):
+    """Closed box: a point whose stencil reaches a wall raises the
+    reference's ValueError (mac/compact_transfer.py:94-96,115-117); the
+    velocity and the points are left as they were."""
+    from afsi_gpu_amd.mac import MGOptions
+    from afsi_gpu_amd.step import IBStep
+    g = MACGrid((16, 16, 16), (1., 1., 1.))
+    gen = np.random.default_rng(25)
+    x = gen.uniform(.3, .7, size=(500, 3))
+    x[7] = (.02, .5, .5)
+    xt = torch.tensor(x, device=gpu)
+    F = torch.tensor(gen.normal(size=(500, 3)), device=gpu)
+    stepper = IBStep(g, 500, dt=1e-3, device=gpu, options=MGOptions(smooth=2, smoother='rbgs'))
+    u = g.zeros(device=gpu)
+    for c in range(3):
+        u[c].copy_(torch.randn(u[c].shape, device=gpu, dtype=torch.float64))
+        u[c][tuple(slice(0, 1) if a == c else slice(None) for a in range(3))] = 0
+        u[c][tuple(slice(-1, None) if a == c else slice(None) for a in range(3))] = 0
+    before = [c.clone() for c in u]
+    with pytest.raises(ValueError, match='support reaches a wall'):
+        stepper.step(xt, F, u)
+    assert all(torch.equal(a, b) for a, b in zip(u, before))
+    assert torch.equal(xt.cpu(), torch.tensor(x))
+    # the same stepper runs once the point moves inside the support band
+    xt[7] = torch.tensor((.5, .5, .5), device=gpu, dtype=torch.float64)
+    stepper.step(xt, F, u)
