@@ -137,6 +137,7 @@ _SIGNATURES = {
     'afsi_bin_workspace_bytes': (c_size_t, [c_int64]),
     'afsi_bin_points': (c_int, [_P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, c_size_t, _P, _P]),
     'afsi_spread_workspace_bytes': (c_size_t, [POINTER(Grid3), POINTER(c_int32)]),
+    'afsi_spread_gather_workspace_bytes': (c_size_t, [POINTER(Grid3), POINTER(c_int32), c_int64]),
     'afsi_tile_box_bounds': (c_int, [POINTER(Grid3), POINTER(c_int32), _I64, _I64]),
     'afsi_spread_tiled_f64': (c_int, [_P, _P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, _P, _P, c_size_t, _P, _P]),
     'afsi_interp_tiled_f64': (c_int, [_P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, _P, _P, _P, c_size_t, _P, _P]),
@@ -228,7 +229,8 @@ _SIGNATURES = {
     **{f'afsi_p2_gather_assemble_{t}': (c_int, [_P, _P, _P, POINTER(P2Group), POINTER(P2Tables), POINTER(Grid3),
                                                 _P, _P, _P]) for t in ('f64', 'f32')},
     'afsi_hash_accumulate_f64': (c_int, [_P, _P, c_int64, POINTER(HashTable), _P]),
-    'afsi_p1_contract_f64': (c_int, [POINTER(FEGeometry), _P, _P, c_int, POINTER(Grid3), POINTER(HashTable), _P]),
+    'afsi_p1_contract_f64': (c_int, [POINTER(FEGeometry), _P, _P, c_int64, c_int64, c_int, POINTER(Grid3),
+                                     POINTER(HashTable), _P]),
     'afsi_csr_spmv_f64': (c_int, [POINTER(CSR), _P, c_int64, _P, c_int64, c_double, _P]),
 }
 

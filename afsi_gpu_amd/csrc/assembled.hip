@@ -80,16 +80,17 @@ __device__ inline int wmax(int v) {
 }
 
 // One wave per cell e of component c. base (3,P,3) / phi (3,P,3,4) are the
-// compact per-component tables of the cell-major interaction points.
+// compact per-component tables of cell-major interaction points; the cells'
+// points start at point `offset` of each lattice.
 __global__ void __launch_bounds__(WAVE) contract_kernel(afsi_fe_geometry g, const int64_t* __restrict__ base,
-                                                        const double* __restrict__ phi, int64_t P, int c,
-                                                        int64_t n0, int64_t n1, int64_t n2, Table t) {
+                                                        const double* __restrict__ phi, int64_t P, int64_t offset,
+                                                        int c, int64_t n0, int64_t n1, int64_t n2, Table t) {
     __shared__ double acc[AS_BOX];
     const int lane = threadIdx.x;
     const int64_t e = blockIdx.x;
     const int Q = g.quad, A = g.nodes_per_cell;
-    const int64_t* cb = base + (int64_t(c) * P + e * Q) * 3;
-    const double* cp = phi + (int64_t(c) * P + e * Q) * 12;
+    const int64_t* cb = base + (int64_t(c) * P + offset + e * Q) * 3;
+    const double* cp = phi + (int64_t(c) * P + offset + e * Q) * 12;
     int lo[3], hi[3];
     bool ok = true;
 #pragma unroll
@@ -202,10 +203,13 @@ int afsi_hash_accumulate_f64(const int64_t* keys, const double* values, int64_t 
     return OK;
 }
 
-int afsi_p1_contract_f64(const afsi_fe_geometry* g, const int64_t* base, const double* phi, int component,
-                         const afsi_grid3* grid, const afsi_hash_table* t, void* stream) {
+int afsi_p1_contract_f64(const afsi_fe_geometry* g, const int64_t* base, const double* phi, int64_t lattice_points,
+                         int64_t offset, int component, const afsi_grid3* grid, const afsi_hash_table* t,
+                         void* stream) {
     if (int r = check_table(t)) return r;
     AFSI_REQUIRE(g && g->cells && g->values && g->weights && base && phi && grid, "null contraction argument");
+    AFSI_REQUIRE(offset >= 0 && lattice_points >= 0 && g->elements * g->quad <= lattice_points - offset,
+                 "the cells' interaction points exceed the stencil tables");
     AFSI_REQUIRE(g->nodes_per_cell >= 1 && g->nodes_per_cell <= AS_NODES, "P1 contraction takes cells of at most %d nodes",
                  AS_NODES);
     AFSI_REQUIRE(g->quad >= 1 && g->elements >= 0 && g->node_count > 0, "invalid FE geometry sizes");
@@ -219,7 +223,7 @@ int afsi_p1_contract_f64(const afsi_fe_geometry* g, const int64_t* base, const d
     if (!g->elements) return OK;
     AFSI_REQUIRE(g->elements < (int64_t(1) << 31), "too many cells");
     contract_kernel<<<unsigned(g->elements), WAVE, 0, hipStream_t(stream)>>>(
-        *g, base, phi, g->elements * g->quad, component, n[0], n[1], n[2],
+        *g, base, phi, lattice_points, offset, component, n[0], n[1], n[2],
         Table{t->keys, t->values, t->capacity, t->flag});
     AFSI_LAUNCH_CHECK();
     return OK;

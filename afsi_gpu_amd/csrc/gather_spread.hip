@@ -1,0 +1,836 @@
+// Peskin-4 spread as a per-face-tile gather (owner computes).
+//
+// Every face of the grid belongs to exactly one tile (the tile of its cell
+// index; on a closed box the closing plane n_c of component c belongs to the
+// last tile along c). One work item produces all owned faces of one tile
+// from the points binned in the tile and in a two-cell halo around it
+// (the only cells whose Peskin stencils reach the tile's faces), and writes
+// each owned face once with a plain store. There are no global atomics, the
+// face arrays are written exactly once (no zero fill, no read-back), and a
+// face's value does not depend on how other tiles are scheduled.
+//
+// Inside a work item the points are staged in LDS in batches (their three
+// weight windows per axis, computed once per batch), cut into runs of at
+// most RUN points of one cell, and accumulated per (run, component, x-slot)
+// task in registers over the run's points: a task holds the 5x5 (or 4x5)
+// y-z window of its x-slot. Only slots whose x position this tile owns
+// become tasks. A task folds its window into the tile's LDS box (owned y-z
+// entries only) with ds_add_f64, so a point costs one LDS update per window
+// entry of its run instead of one per stencil link.
+//
+// Tiles whose halo holds more points than one work item takes are split
+// into chunks of the halo point list; each chunk writes its box to a scratch
+// slot and a second kernel adds the slots of the tile in chunk order.
+//
+// Weights and bases follow the reference's compact stencil
+// (mac/compact_transfer.py:21-26): s = (x - o)/h, base = floor(s - 1), the
+// four Peskin weights of f = s - base - 1 from the shared radical.
+#include "binshare.hpp"
+#include <vector>
+
+namespace afsi {
+
+template <class T>
+int launch_fill(T* x, int64_t count, T value, hipStream_t s);
+
+constexpr int GS_THREADS = 256;
+constexpr int GS_BATCH = 128;   // points staged per round (threads of the first two waves)
+constexpr int GS_RUN = 16;      // longest same-cell run one task accumulates
+constexpr int GS_REC = 30;      // doubles per staged point
+constexpr int GS_CHUNK = 2048;  // halo points per work item (larger halos are split)
+static_assert(GS_BATCH == 128 && GS_THREADS >= GS_BATCH, "runs are found by the first two waves");
+
+// Staged point record (doubles): face-lattice windows of 4 and centre-
+// lattice windows of 5 per axis (zero-padded at the end the point's centre
+// base does not reach), and F_c / V. Five-windows start at even offsets so
+// they load as two 16-byte reads plus one 8-byte read.
+enum : int {
+    R_WXF = 0, R_WYF = 4, R_WZF = 8, R_WYC = 12, R_FV0 = 17, R_WZC = 18, R_FV1 = 23, R_WXC = 24, R_FV2 = 29
+};
+
+struct GsGeom {
+    int32_t t[3];       // tile cells per axis
+    int32_t nt[3];      // tiles per axis
+    int32_t ntiles;
+    int32_t tvol;
+    int32_t bd[3][3];   // LDS box extent of component c along axis a
+    int32_t boff[3];    // LDS box offset (doubles) of component c
+    int32_t btotal;     // doubles per box
+    int32_t rows;       // halo rows bound (t0 + 4) * (t1 + 4)
+    int32_t maxp;       // key-contiguous pieces per halo row bound
+    int32_t chunk;      // halo points per work item
+};
+
+struct GsDesc {
+    const double* x;
+    const double* force;
+    double* face[3];
+    afsi_grid3 g;
+    GsGeom gg;
+    double volume;
+    int32_t point_begin;  // offset of the instance's ids in perm
+    int32_t key_offset;   // first key of the instance
+    int32_t tile_begin;   // first global tile of the instance
+    int32_t reserved;
+};
+
+static GsGeom make_gs_geom(const afsi_grid3& g, const int32_t tile[3]) {
+    GsGeom s{};
+    s.ntiles = 1;
+    s.tvol = 1;
+    for (int a = 0; a < 3; ++a) {
+        s.t[a] = tile[a];
+        s.nt[a] = int32_t(cdiv(g.n[a], tile[a]));
+        s.ntiles *= s.nt[a];
+        s.tvol *= tile[a];
+    }
+    int32_t off = 0;
+    for (int c = 0; c < 3; ++c) {
+        int32_t vol = 1;
+        for (int a = 0; a < 3; ++a) {
+            s.bd[c][a] = tile[a] + ((!g.periodic && a == c) ? 1 : 0);
+            vol *= s.bd[c][a];
+        }
+        s.boff[c] = off;
+        off += vol;
+    }
+    s.btotal = off;
+    s.rows = (tile[0] + 4) * (tile[1] + 4);
+    // a row of at most t2 + 4 consecutive cells meets at most cdiv(t2+4, t2)
+    // + 1 z-tiles, plus one cut where a periodic row wraps
+    const int32_t h2 = tile[2] + 4;
+    const int32_t p = int32_t(cdiv(h2, tile[2])) + 2;
+    s.maxp = p < h2 ? p : h2;
+    s.chunk = GS_CHUNK;
+    return s;
+}
+
+// LDS layout of the spread kernel (bytes), shared by host and device.
+struct GsLds {
+    int box, rec, pbeg, ppfx, hcell, srbeg, tcnt, tmask, tmp, total;
+};
+
+__host__ __device__ inline GsLds gs_lds(const GsGeom& s) {
+    GsLds L{};
+    int off = 0;
+    auto take = [&off](int bytes) {
+        const int at = off;
+        off += (bytes + 15) & ~15;
+        return at;
+    };
+    L.box = take(s.btotal * 8);
+    L.rec = take(GS_BATCH * GS_REC * 8);
+    L.pbeg = take(s.rows * s.maxp * 4);
+    L.ppfx = take((s.rows * s.maxp + 1) * 4);
+    L.hcell = take(GS_BATCH * 4);
+    L.srbeg = take((GS_BATCH + 1) * 4);
+    L.tcnt = take((3 * GS_BATCH + 1) * 4);
+    L.tmask = take(3 * GS_BATCH * 4);
+    L.tmp = take(32 * 4);
+    L.total = off;
+    return L;
+}
+
+// Geometry of one tile: first cell, owned cells, halo cells per axis, and
+// owned face extents per component.
+struct GsItem {
+    int t0[3], own[3], H[3];
+    int ownc[3][3];
+};
+
+__device__ inline void gs_item(int lt, const GsGeom& s, const afsi_grid3& g, GsItem& it) {
+    const int ti[3] = {lt / (s.nt[1] * s.nt[2]), (lt / s.nt[2]) % s.nt[1], lt % s.nt[2]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int n = int(g.n[a]);
+        it.t0[a] = ti[a] * s.t[a];
+        it.own[a] = min(s.t[a], n - it.t0[a]);
+        it.H[a] = g.periodic ? min(it.own[a] + 4, n) : it.own[a] + 4;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            it.ownc[c][a] = it.own[a] +
+                            ((!g.periodic && a == c && it.t0[a] + it.own[a] == int(g.n[a])) ? 1 : 0);
+}
+
+// Halo cell h (0 <= h < H[a]) of axis a -> grid cell, or -1 outside a closed box.
+__device__ inline int gs_cell(int h, int a, const GsItem& it, const afsi_grid3& g) {
+    const int n = int(g.n[a]);
+    int c = it.t0[a] - 2 + h;
+    if (g.periodic) {
+        if (c < 0) c += n;
+        if (c >= n) c -= n;
+        return c;
+    }
+    return (c >= 0 && c < n) ? c : -1;
+}
+
+// Tile-major key of a cell (the binning's numbering).
+__device__ inline int gs_key(int cx, int cy, int cz, const GsGeom& s) {
+    const int tid = ((cx / s.t[0]) * s.nt[1] + cy / s.t[1]) * s.nt[2] + cz / s.t[2];
+    const int loc = ((cx % s.t[0]) * s.t[1] + cy % s.t[1]) * s.t[2] + cz % s.t[2];
+    return tid * s.tvol + loc;
+}
+
+// Key-contiguous pieces [kb, ke] of halo row r; returns their count (<= maxp)
+// and sets *overflow if the row needs more.
+__device__ inline int gs_row_pieces(int r, const GsItem& it, const GsGeom& s, const afsi_grid3& g, int* kb,
+                                    int* ke, bool* overflow) {
+    const int hx = r / it.H[1], hy = r - hx * it.H[1];
+    const int cx = gs_cell(hx, 0, it, g), cy = gs_cell(hy, 1, it, g);
+    if (cx < 0 || cy < 0) return 0;
+    int np = 0;
+    for (int hz = 0; hz < it.H[2]; ++hz) {
+        const int cz = gs_cell(hz, 2, it, g);
+        if (cz < 0) continue;
+        const int key = gs_key(cx, cy, cz, s);
+        if (np > 0 && key == ke[np - 1] + 1) {
+            ke[np - 1] = key;
+        } else if (np < s.maxp) {
+            kb[np] = key;
+            ke[np] = key;
+            ++np;
+        } else {
+            *overflow = true;
+            break;
+        }
+    }
+    return np;
+}
+
+// In-place exclusive scan of a[0, m) by the whole workgroup; returns the total.
+__device__ inline int32_t gs_block_scan(int32_t* a, int m, int32_t* tmp) {
+    const int per = (m + GS_THREADS - 1) / GS_THREADS;
+    const int b = int(threadIdx.x) * per, e = min(m, b + per);
+    int32_t s = 0;
+    for (int i = b; i < e; ++i) s += a[i];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int32_t v = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int32_t up = __shfl_up(v, d, 64);
+        if (lane >= d) v += up;
+    }
+    if (lane == 63) tmp[wave] = v;
+    __syncthreads();
+    int32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < GS_THREADS / 64; ++w) {
+        before += w < wave ? tmp[w] : 0;
+        total += tmp[w];
+    }
+    int32_t run = before + v - s;
+    for (int i = b; i < e; ++i) {
+        const int32_t x = a[i];
+        a[i] = run;
+        run += x;
+    }
+    __syncthreads();
+    return total;
+}
+
+// Last index i in [0, n) with a[i] <= v (a non-decreasing, a[0] <= v).
+__device__ inline int gs_search(const int32_t* a, int n, int32_t v) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Tile-local position of window slot k (origin hl - 3 for a face-lattice
+// window, hl - 4 for a centre-lattice one), wrapped onto a periodic axis;
+// owned iff 0 <= L < own.
+__device__ inline int gs_pos(int hl, int origin, int k, int n, bool periodic) {
+    int L = hl - origin + k;
+    if (periodic) {
+        if (L < 0) L += n;
+        if (L >= n) L -= n;
+    }
+    return L;
+}
+
+// Stage one point: weights, windows and its halo cell.
+__device__ inline void gs_stage(const GsDesc& d, int id, const GsItem& it, double* rec, int32_t* hcell,
+                                int* flags) {
+    const afsi_grid3& g = d.g;
+    const double xp[3] = {d.x[3 * id], d.x[3 * id + 1], d.x[3 * id + 2]};
+    int hl[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int n = int(g.n[a]);
+        // face lattice: s = (x - o)/h, base = floor(s - 1) (the binning's
+        // cell - 1), f = s - base - 1; the centre lattice (s - 1/2) has the
+        // same base when f >= 1/2 (fraction f - 1/2) and one below otherwise
+        const double sf = (xp[a] - g.origin[a]) / g.h[a];
+        const double ff = floor(sf - 1.0);
+        const double f = sf - (ff + 1.0);
+        const bool upper = f >= 0.5;
+        const int bf = (ff > -1e9 && ff < 1e9) ? int(ff) : 0;
+        double wf[4], wc[4];
+        peskin_weights<double>(f, wf);
+        peskin_weights<double>(upper ? f - 0.5 : f + 0.5, wc);
+        int cell = bf + 1;
+        if (g.periodic) cell = cell < 0 || cell >= n ? ((cell % n) + n) % n : cell;
+        else cell = cell < 0 ? 0 : (cell >= n ? n - 1 : cell);
+        int h = cell - (it.t0[a] - 2);
+        if (g.periodic && (h < 0 || h >= it.H[a])) h = ((h % n) + n) % n;
+        if (h < 0 || h >= it.H[a]) {
+            raise_flag(flags, F_INDEX);
+            h = h < 0 ? 0 : it.H[a] - 1;
+        }
+        hl[a] = h;
+        const int delta = upper ? 0 : 1;  // 0: centre base = face base, 1: one below
+        const int fo = a == 0 ? R_WXF : (a == 1 ? R_WYF : R_WZF);
+        const int co = a == 0 ? R_WXC : (a == 1 ? R_WYC : R_WZC);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) rec[fo + k] = wf[k];
+        rec[co + 0] = delta ? wc[0] : 0.0;
+        rec[co + 1] = delta ? wc[1] : wc[0];
+        rec[co + 2] = delta ? wc[2] : wc[1];
+        rec[co + 3] = delta ? wc[3] : wc[2];
+        rec[co + 4] = delta ? 0.0 : wc[3];
+    }
+    rec[R_FV0] = d.force[3 * id] / d.volume;
+    rec[R_FV1] = d.force[3 * id + 1] / d.volume;
+    rec[R_FV2] = d.force[3 * id + 2] / d.volume;
+    *hcell = (hl[0] * it.H[1] + hl[1]) * it.H[2] + hl[2];
+}
+
+// One task: component C, x-slot k of the run [sb, se) of same-cell points.
+// Accumulates the run's y-z window in registers and adds its owned entries
+// to the LDS box.
+template <int C>
+__device__ inline void gs_task(const double* __restrict__ recs, int sb, int se, int hcell, int k,
+                               const GsItem& it, const GsGeom& s, const afsi_grid3& g, double* box) {
+    constexpr int NY = C == 1 ? 4 : 5, NZ = C == 2 ? 4 : 5;
+    constexpr int WX = C == 0 ? R_WXF : R_WXC;
+    constexpr int WY = C == 1 ? R_WYF : R_WYC;
+    constexpr int WZ = C == 2 ? R_WZF : R_WZC;
+    constexpr int FV = C == 0 ? R_FV0 : (C == 1 ? R_FV1 : R_FV2);
+    double acc[NY][NZ];
+#pragma unroll
+    for (int j = 0; j < NY; ++j)
+#pragma unroll
+        for (int l = 0; l < NZ; ++l) acc[j][l] = 0.0;
+    for (int p = sb; p < se; ++p) {
+        const double* r = recs + p * GS_REC;
+        const double t = r[FV] * r[WX + k];
+        double wz[NZ];
+#pragma unroll
+        for (int l = 0; l < NZ; ++l) wz[l] = r[WZ + l];
+#pragma unroll
+        for (int j = 0; j < NY; ++j) {
+            const double u = t * r[WY + j];
+#pragma unroll
+            for (int l = 0; l < NZ; ++l) acc[j][l] = __builtin_fma(u, wz[l], acc[j][l]);
+        }
+    }
+    const int hz = hcell % it.H[2];
+    const int hy = (hcell / it.H[2]) % it.H[1];
+    const int hx = hcell / (it.H[2] * it.H[1]);
+    const bool per = g.periodic != 0;
+    const int lx = gs_pos(hx, C == 0 ? 3 : 4, k, int(g.n[0]), per);
+    double* b = box + s.boff[C] + lx * s.bd[C][1] * s.bd[C][2];
+#pragma unroll
+    for (int j = 0; j < NY; ++j) {
+        const int ly = gs_pos(hy, C == 1 ? 3 : 4, j, int(g.n[1]), per);
+        if (ly < 0 || ly >= it.ownc[C][1]) continue;
+#pragma unroll
+        for (int l = 0; l < NZ; ++l) {
+            const int lz = gs_pos(hz, C == 2 ? 3 : 4, l, int(g.n[2]), per);
+            if (lz < 0 || lz >= it.ownc[C][2] || acc[j][l] == 0.0) continue;
+            atomicAdd(b + ly * s.bd[C][2] + lz, acc[j][l]);
+        }
+    }
+}
+
+// Bitmask of the owned x-slots of component c for a run in halo row hx.
+__device__ inline int gs_slots(int c, int hx, const GsItem& it, const afsi_grid3& g) {
+    const int width = c == 0 ? 4 : 5, origin = c == 0 ? 3 : 4;
+    int mask = 0;
+    for (int k = 0; k < width; ++k) {
+        const int L = gs_pos(hx, origin, k, int(g.n[0]), g.periodic != 0);
+        if (L >= 0 && L < it.ownc[c][0]) mask |= 1 << k;
+    }
+    return mask;
+}
+
+// Work item w's tile and chunk: tile = last gt with chunk_start[gt] <= w.
+struct GsWork {
+    int gt, chunk, nch;
+};
+
+__device__ inline GsWork gs_work(int w, const int32_t* __restrict__ chunk_start, int tiles) {
+    int lo = 0, hi = tiles;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (chunk_start[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    return GsWork{lo, w - chunk_start[lo], chunk_start[lo + 1] - chunk_start[lo]};
+}
+
+// Descriptor sources: one instance passed by value, or a device array of
+// instances found by their first global tile.
+struct GsOne {
+    GsDesc d;
+    __device__ const GsDesc& at(int gt, int& lt) const {
+        lt = gt;
+        return d;
+    }
+};
+
+struct GsMany {
+    const GsDesc* d;
+    int n;
+    __device__ const GsDesc& at(int gt, int& lt) const {
+        int lo = 0, hi = n;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (d[mid].tile_begin <= gt) lo = mid;
+            else hi = mid;
+        }
+        lt = gt - d[lo].tile_begin;
+        return d[lo];
+    }
+};
+
+// Build the halo piece table of a tile in LDS; returns the halo point count.
+__device__ inline int32_t gs_pieces(const GsDesc& d, const GsItem& it, const int32_t* __restrict__ cell_ptr,
+                                    int32_t* pbeg, int32_t* ppfx, int32_t* tmp, int* flags) {
+    const GsGeom& s = d.gg;
+    const int rows = it.H[0] * it.H[1];
+    const int m = rows * s.maxp;
+    for (int r = threadIdx.x; r < rows; r += GS_THREADS) {
+        int kb[8], ke[8];
+        bool overflow = false;
+        const int np = gs_row_pieces(r, it, s, d.g, kb, ke, &overflow);
+        if (overflow) raise_flag(flags, F_CAPACITY);
+        for (int i = 0; i < s.maxp; ++i) {
+            int32_t b = 0, c = 0;
+            if (i < np) {
+                b = cell_ptr[d.key_offset + kb[i]];
+                c = cell_ptr[d.key_offset + ke[i] + 1] - b;
+            }
+            pbeg[r * s.maxp + i] = b;
+            ppfx[r * s.maxp + i] = c;
+        }
+    }
+    __syncthreads();
+    const int32_t total = gs_block_scan(ppfx, m, tmp);
+    if (threadIdx.x == 0) ppfx[m] = total;
+    __syncthreads();
+    return total;
+}
+
+template <class Src>
+__global__ void __launch_bounds__(GS_THREADS) gs_count_kernel(Src src, int tiles,
+                                                              const int32_t* __restrict__ cell_ptr,
+                                                              int32_t* __restrict__ count, int* flags) {
+    const int lane = threadIdx.x & 63;
+    const int gt = int(blockIdx.x) * (GS_THREADS / 64) + int(threadIdx.x >> 6);
+    if (gt >= tiles) return;
+    int lt;
+    const GsDesc& d = src.at(gt, lt);
+    GsItem it;
+    gs_item(lt, d.gg, d.g, it);
+    int32_t sum = 0;
+    for (int r = lane; r < it.H[0] * it.H[1]; r += 64) {
+        int kb[8], ke[8];
+        bool overflow = false;
+        const int np = gs_row_pieces(r, it, d.gg, d.g, kb, ke, &overflow);
+        if (overflow) raise_flag(flags, F_CAPACITY);
+        for (int i = 0; i < np; ++i)
+            sum += cell_ptr[d.key_offset + ke[i] + 1] - cell_ptr[d.key_offset + kb[i]];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    if (lane == 0) count[gt] = sum;
+}
+
+// Single workgroup: chunk_start (work items per tile: max(1, ceil(count /
+// chunk))) and multi_start (scratch slots, only for tiles of >= 2 chunks).
+template <class Src>
+__global__ void __launch_bounds__(1024) gs_scan_kernel(Src src, int tiles, const int32_t* __restrict__ count,
+                                                       int32_t* __restrict__ chunk_start,
+                                                       int32_t* __restrict__ multi_start) {
+    __shared__ int32_t a[1024], b[1024];
+    int32_t ra = 0, rb = 0;
+    for (int base = 0; base < tiles; base += 1024) {
+        const int t = base + int(threadIdx.x);
+        int32_t va = 0, vb = 0;
+        if (t < tiles) {
+            int lt;
+            const GsDesc& d = src.at(t, lt);
+            const int32_t k = (count[t] + d.gg.chunk - 1) / d.gg.chunk;
+            va = k > 1 ? k : 1;
+            vb = k > 1 ? k : 0;
+        }
+        a[threadIdx.x] = va;
+        b[threadIdx.x] = vb;
+        __syncthreads();
+        for (int dd = 1; dd < 1024; dd <<= 1) {
+            const int32_t xa = threadIdx.x >= dd ? a[threadIdx.x - dd] : 0;
+            const int32_t xb = threadIdx.x >= dd ? b[threadIdx.x - dd] : 0;
+            __syncthreads();
+            a[threadIdx.x] += xa;
+            b[threadIdx.x] += xb;
+            __syncthreads();
+        }
+        if (t < tiles) {
+            chunk_start[t] = ra + a[threadIdx.x] - va;
+            multi_start[t] = rb + b[threadIdx.x] - vb;
+        }
+        ra += a[1023];
+        rb += b[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        chunk_start[tiles] = ra;
+        multi_start[tiles] = rb;
+    }
+}
+
+// Owned face e of component c -> (box index, global face index).
+__device__ inline void gs_face(int e, int c, const GsItem& it, const GsGeom& s, const afsi_grid3& g, int& bi,
+                               int& fi) {
+    const int oy = it.ownc[c][1], oz = it.ownc[c][2];
+    const int lx = e / (oy * oz), ly = (e / oz) % oy, lz = e % oz;
+    bi = s.boff[c] + (lx * s.bd[c][1] + ly) * s.bd[c][2] + lz;
+    const int nf1 = int(g.n[1]) + ((!g.periodic && c == 1) ? 1 : 0);
+    const int nf2 = int(g.n[2]) + ((!g.periodic && c == 2) ? 1 : 0);
+    fi = ((it.t0[0] + lx) * nf1 + it.t0[1] + ly) * nf2 + it.t0[2] + lz;
+}
+
+template <class Src>
+__global__ void __launch_bounds__(GS_THREADS) gs_spread_kernel(Src src, int tiles,
+                                                               const int32_t* __restrict__ perm,
+                                                               const int32_t* __restrict__ cell_ptr,
+                                                               const int32_t* __restrict__ chunk_start,
+                                                               const int32_t* __restrict__ multi_start,
+                                                               double* __restrict__ scratch, int scratch_slots,
+                                                               int* flags) {
+    extern __shared__ __align__(16) char lds[];
+    const int total_items = chunk_start[tiles];
+    // XCD-aware order: consecutive blocks run on different XCDs, so deal
+    // each XCD a contiguous run of work items (neighbouring tiles share
+    // halo points in that XCD's L2).
+    const int G = int(gridDim.x);
+    const int b = int(blockIdx.x);
+    const int first = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+    for (int w = first; w < total_items; w += G) {
+        const GsWork wk = gs_work(w, chunk_start, tiles);
+        int lt;
+        const GsDesc& d = src.at(wk.gt, lt);
+        const GsGeom& s = d.gg;
+        const GsLds L = gs_lds(s);
+        double* box = reinterpret_cast<double*>(lds + L.box);
+        double* recs = reinterpret_cast<double*>(lds + L.rec);
+        int32_t* pbeg = reinterpret_cast<int32_t*>(lds + L.pbeg);
+        int32_t* ppfx = reinterpret_cast<int32_t*>(lds + L.ppfx);
+        int32_t* hcell = reinterpret_cast<int32_t*>(lds + L.hcell);
+        int32_t* srbeg = reinterpret_cast<int32_t*>(lds + L.srbeg);
+        int32_t* tcnt = reinterpret_cast<int32_t*>(lds + L.tcnt);
+        int32_t* tmask = reinterpret_cast<int32_t*>(lds + L.tmask);
+        int32_t* tmp = reinterpret_cast<int32_t*>(lds + L.tmp);
+        GsItem it;
+        gs_item(lt, s, d.g, it);
+        for (int e = threadIdx.x; e < s.btotal; e += GS_THREADS) box[e] = 0.0;
+        const int32_t halo = gs_pieces(d, it, cell_ptr, pbeg, ppfx, tmp, flags);
+        const int m = it.H[0] * it.H[1] * s.maxp;
+        const int32_t per = (halo + wk.nch - 1) / wk.nch;
+        const int32_t cb = min(halo, wk.chunk * per), ce = min(halo, cb + per);
+        for (int32_t b0 = cb; b0 < ce; b0 += GS_BATCH) {
+            const int nb = min(GS_BATCH, int(ce - b0));
+            if (int(threadIdx.x) < nb) {
+                const int32_t f = b0 + int32_t(threadIdx.x);
+                const int pc = gs_search(ppfx, m + 1, f);
+                const int32_t pos = pbeg[pc] + (f - ppfx[pc]);
+                const int id = perm[pos] - d.point_begin;
+                gs_stage(d, id, it, recs + threadIdx.x * GS_REC, hcell + threadIdx.x, flags);
+            }
+            __syncthreads();
+            // runs of one cell, cut at every GS_RUN-th staged point (two waves
+            // hold the batch: wave counts through LDS, then run starts)
+            const int j = int(threadIdx.x);
+            const bool head = j < nb && (j == 0 || (j % GS_RUN) == 0 || hcell[j] != hcell[j - 1]);
+            const uint64_t bits = __ballot(head);
+            if (j < GS_BATCH && (j & 63) == 0) tmp[16 + (j >> 6)] = __popcll(bits);
+            __syncthreads();
+            const int nsr = tmp[16] + tmp[17];
+            if (head) srbeg[__popcll(bits & ((uint64_t(1) << (j & 63)) - 1)) + (j >= 64 ? tmp[16] : 0)] = j;
+            if (threadIdx.x == 0) srbeg[nsr] = nb;
+            __syncthreads();
+            // owned x-slots of each run and component -> task counts (component-major)
+            for (int r = threadIdx.x; r < nsr; r += GS_THREADS) {
+                const int hx = hcell[srbeg[r]] / (it.H[1] * it.H[2]);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int mask = gs_slots(c, hx, it, d.g);
+                    tmask[c * nsr + r] = mask;
+                    tcnt[c * nsr + r] = __popc(mask);
+                }
+            }
+            __syncthreads();
+            const int32_t ntask = gs_block_scan(tcnt, 3 * nsr, tmp);
+            if (threadIdx.x == 0) tcnt[3 * nsr] = ntask;
+            __syncthreads();
+            for (int q = threadIdx.x; q < ntask; q += GS_THREADS) {
+                const int idx = gs_search(tcnt, 3 * nsr + 1, q);
+                const int c = idx / nsr, r = idx - c * nsr;
+                int mask = tmask[idx];
+                for (int skip = q - tcnt[idx]; skip > 0; --skip) mask &= mask - 1;
+                const int k = __ffs(mask) - 1;
+                const int sb = srbeg[r], se = srbeg[r + 1];
+                if (c == 0) gs_task<0>(recs, sb, se, hcell[sb], k, it, s, d.g, box);
+                else if (c == 1) gs_task<1>(recs, sb, se, hcell[sb], k, it, s, d.g, box);
+                else gs_task<2>(recs, sb, se, hcell[sb], k, it, s, d.g, box);
+            }
+            __syncthreads();
+        }
+        if (wk.nch == 1) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int vol = it.ownc[c][0] * it.ownc[c][1] * it.ownc[c][2];
+                for (int e = threadIdx.x; e < vol; e += GS_THREADS) {
+                    int bi, fi;
+                    gs_face(e, c, it, s, d.g, bi, fi);
+                    d.face[c][fi] = box[bi];
+                }
+            }
+        } else {
+            const int slot = multi_start[wk.gt] + wk.chunk;
+            if (slot < scratch_slots) {
+                double* dst = scratch + size_t(slot) * s.btotal;
+                for (int e = threadIdx.x; e < s.btotal; e += GS_THREADS) dst[e] = box[e];
+            } else if (threadIdx.x == 0) {
+                raise_flag(flags, F_CAPACITY);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Tiles of >= 2 chunks: faces = sum of the chunk boxes in chunk order.
+template <class Src>
+__global__ void __launch_bounds__(GS_THREADS) gs_combine_kernel(Src src, int tiles,
+                                                                const int32_t* __restrict__ chunk_start,
+                                                                const int32_t* __restrict__ multi_start,
+                                                                const double* __restrict__ scratch,
+                                                                int scratch_slots) {
+    for (int gt = blockIdx.x; gt < tiles; gt += gridDim.x) {
+        const int nch = chunk_start[gt + 1] - chunk_start[gt];
+        if (nch < 2) continue;
+        const int slot = multi_start[gt];
+        if (slot + nch > scratch_slots) continue;
+        int lt;
+        const GsDesc& d = src.at(gt, lt);
+        GsItem it;
+        gs_item(lt, d.gg, d.g, it);
+        for (int c = 0; c < 3; ++c) {
+            const int vol = it.ownc[c][0] * it.ownc[c][1] * it.ownc[c][2];
+            for (int e = threadIdx.x; e < vol; e += GS_THREADS) {
+                int bi, fi;
+                gs_face(e, c, it, d.gg, d.g, bi, fi);
+                double v = 0.0;
+                for (int i = 0; i < nch; ++i) v += scratch[size_t(slot + i) * d.gg.btotal + bi];
+                d.face[c][fi] = v;
+            }
+        }
+    }
+}
+
+}  // namespace afsi
+
+// ---- host side ---------------------------------------------------------------
+
+namespace afsi {
+
+static size_t gs_align(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Tiles whose halo can hold one cell, per axis (a halo spans t + 4 cells).
+static int64_t gs_multiplicity(const afsi_grid3& g, const int32_t tile[3]) {
+    int64_t m = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int64_t k = cdiv(tile[a] + 4, tile[a]), nt = cdiv(g.n[a], tile[a]);
+        m *= k < nt ? k : nt;
+    }
+    return m;
+}
+
+// Scratch slots for tiles split into chunks: a tile of halo count h > K
+// takes ceil(h/K) <= 2h/K slots and the halo counts sum to at most
+// points * multiplicity.
+static int64_t gs_slot_bound(int64_t points, int64_t multiplicity) {
+    return 2 * cdiv(points * multiplicity, GS_CHUNK) + 1;
+}
+
+struct GsWsLayout {
+    size_t count, chunk_start, multi_start, scratch, total;
+};
+
+static GsWsLayout gs_ws_layout(int64_t tiles, int64_t slots, int64_t btotal) {
+    GsWsLayout L{};
+    size_t o = 0;
+    L.count = o;       o = gs_align(o + size_t(tiles) * 4);
+    L.chunk_start = o; o = gs_align(o + size_t(tiles + 1) * 4);
+    L.multi_start = o; o = gs_align(o + size_t(tiles + 1) * 4);
+    L.scratch = o;     o = gs_align(o + size_t(slots) * size_t(btotal) * 8);
+    L.total = o;
+    return L;
+}
+
+static int gs_check(const afsi_grid3& g, const int32_t tile[3]) {
+    AFSI_REQUIRE(tile[0] > 0 && tile[1] > 0 && tile[2] > 0, "tile sizes must be positive");
+    AFSI_REQUIRE(g.n[0] >= 1 && g.n[1] >= 1 && g.n[2] >= 1, "invalid grid");
+    AFSI_REQUIRE(!g.periodic || (g.n[0] >= 4 && g.n[1] >= 4 && g.n[2] >= 4),
+                 "a periodic grid needs at least 4 cells per axis");
+    AFSI_REQUIRE((g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1) < (int64_t(1) << 31),
+                 "grid exceeds 2^31 faces per component");
+    const GsGeom s = make_gs_geom(g, tile);
+    AFSI_REQUIRE(s.maxp <= 8, "tile too short along z");
+    AFSI_REQUIRE(gs_lds(s).total <= 160 * 1024, "tile too large for LDS");
+    return OK;
+}
+
+template <class Src>
+static int gs_run(const Src& src, int64_t tiles, int lds_bytes, const int32_t* perm, const int32_t* cell_ptr,
+                  void* workspace, const GsWsLayout& L, int64_t slots, int* flags, hipStream_t s) {
+    char* ws = static_cast<char*>(workspace);
+    int32_t* count = reinterpret_cast<int32_t*>(ws + L.count);
+    int32_t* chunk_start = reinterpret_cast<int32_t*>(ws + L.chunk_start);
+    int32_t* multi_start = reinterpret_cast<int32_t*>(ws + L.multi_start);
+    double* scratch = reinterpret_cast<double*>(ws + L.scratch);
+    const int nt = int(tiles);
+    gs_count_kernel<Src><<<unsigned(cdiv(nt, GS_THREADS / 64)), GS_THREADS, 0, s>>>(src, nt, cell_ptr, count,
+                                                                                     flags);
+    AFSI_LAUNCH_CHECK();
+    gs_scan_kernel<Src><<<1, 1024, 0, s>>>(src, nt, count, chunk_start, multi_start);
+    AFSI_LAUNCH_CHECK();
+    if (lds_bytes > 64 * 1024)
+        AFSI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gs_spread_kernel<Src>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes));
+    // one block per work item for all but the most split tilings (blocks
+    // past the item count exit at once); a multiple of 8 for the XCD order
+    int64_t blocks = tiles + tiles / 2 + 64;
+    blocks = (blocks + 7) / 8 * 8;
+    if (blocks > 65536) blocks = 65536;
+    gs_spread_kernel<Src><<<unsigned(blocks), GS_THREADS, size_t(lds_bytes), s>>>(
+        src, nt, perm, cell_ptr, chunk_start, multi_start, scratch, int(slots), flags);
+    AFSI_LAUNCH_CHECK();
+    gs_combine_kernel<Src><<<unsigned(tiles < 2048 ? tiles : 2048), GS_THREADS, 0, s>>>(
+        src, nt, chunk_start, multi_start, scratch, int(slots));
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+// Batched spread: descriptors of every instance (device array written by
+// gs_batch_fill) and one work list over all their tiles.
+size_t gs_desc_bytes() { return sizeof(GsDesc); }
+
+int gs_batch_sizes(const GsBatchItem* items, int n, const int32_t tile[3], int64_t* tiles, int64_t* slots,
+                   int64_t* btotal, int* lds_bytes) {
+    int64_t t = 0, pts = 0, m = 1, bt = 0;
+    int lds = 0;
+    for (int i = 0; i < n; ++i) {
+        if (int r = gs_check(items[i].g, tile)) return r;
+        const GsGeom s = make_gs_geom(items[i].g, tile);
+        t += s.ntiles;
+        pts += items[i].points;
+        const int64_t mi = gs_multiplicity(items[i].g, tile);
+        m = mi > m ? mi : m;
+        bt = s.btotal > bt ? s.btotal : bt;
+        const int l = gs_lds(s).total;
+        lds = l > lds ? l : lds;
+    }
+    *tiles = t;
+    *slots = gs_slot_bound(pts, m);
+    *btotal = bt;
+    *lds_bytes = lds;
+    return OK;
+}
+
+size_t gs_batch_workspace_bytes(int64_t tiles, int64_t slots, int64_t btotal) {
+    return gs_ws_layout(tiles, slots, btotal).total;
+}
+
+void gs_batch_fill(const GsBatchItem* items, int n, const int32_t tile[3], void* host_descs) {
+    GsDesc* out = static_cast<GsDesc*>(host_descs);
+    int64_t pbeg = 0, kbeg = 0, tbeg = 0;
+    for (int i = 0; i < n; ++i) {
+        GsDesc d{};
+        d.x = items[i].x;
+        d.force = items[i].force;
+        for (int c = 0; c < 3; ++c) d.face[c] = items[i].face[c];
+        d.g = items[i].g;
+        d.gg = make_gs_geom(items[i].g, tile);
+        d.volume = (d.g.h[0] * d.g.h[1]) * d.g.h[2];
+        d.point_begin = int32_t(pbeg);
+        d.key_offset = int32_t(kbeg);
+        d.tile_begin = int32_t(tbeg);
+        out[i] = d;
+        pbeg += items[i].points;
+        kbeg += int64_t(d.gg.ntiles) * d.gg.tvol;
+        tbeg += d.gg.ntiles;
+    }
+}
+
+int gs_batch_spread(const void* device_descs, int n, int64_t tiles, int64_t slots, int64_t btotal, int lds_bytes,
+                    const int32_t* perm, const int32_t* cell_ptr, void* workspace, int* flags, hipStream_t s) {
+    const GsMany src{static_cast<const GsDesc*>(device_descs), n};
+    return gs_run(src, tiles, lds_bytes, perm, cell_ptr, workspace, gs_ws_layout(tiles, slots, btotal), slots,
+                  flags, s);
+}
+
+}  // namespace afsi
+
+using namespace afsi;
+
+extern "C" {
+
+size_t afsi_spread_gather_workspace_bytes(const afsi_grid3* grid, const int32_t tile[3], int64_t points) {
+    if (!grid || !tile || points < 0 || gs_check(*grid, tile) != OK) return 0;
+    const GsGeom s = make_gs_geom(*grid, tile);
+    return gs_ws_layout(s.ntiles, gs_slot_bound(points, gs_multiplicity(*grid, tile)), s.btotal).total;
+}
+
+int afsi_spread_tiled_f64(const double* x, const double* force, int64_t points, const afsi_grid3* grid,
+                          const int32_t tile[3], const int32_t* perm, const int32_t* cell_ptr, double* u,
+                          double* v, double* w, void* workspace, size_t workspace_bytes, int* flags,
+                          void* stream) {
+    AFSI_REQUIRE(grid && tile && points >= 0 && points < (int64_t(1) << 31), "invalid spread arguments");
+    AFSI_REQUIRE(u && v && w, "null face array");
+    const afsi_grid3& g = *grid;
+    if (int r = gs_check(g, tile)) return r;
+    if (points == 0) {
+        double* outs[3] = {u, v, w};
+        for (int c = 0; c < 3; ++c) {
+            const int64_t nf = (g.n[0] + (!g.periodic && c == 0)) * (g.n[1] + (!g.periodic && c == 1)) *
+                               (g.n[2] + (!g.periodic && c == 2));
+            if (int r = launch_fill<double>(outs[c], nf, 0.0, hipStream_t(stream))) return r;
+        }
+        return OK;
+    }
+    AFSI_REQUIRE(x && force && perm && cell_ptr && workspace, "null pointer");
+    AFSI_REQUIRE(workspace_bytes >= afsi_spread_gather_workspace_bytes(grid, tile, points),
+                 "spread workspace too small");
+    GsOne src{};
+    src.d.x = x;
+    src.d.force = force;
+    src.d.face[0] = u;
+    src.d.face[1] = v;
+    src.d.face[2] = w;
+    src.d.g = g;
+    src.d.gg = make_gs_geom(g, tile);
+    src.d.volume = (g.h[0] * g.h[1]) * g.h[2];
+    const int64_t slots = gs_slot_bound(points, gs_multiplicity(g, tile));
+    return gs_run(src, src.d.gg.ntiles, gs_lds(src.d.gg).total, perm, cell_ptr, workspace,
+                  gs_ws_layout(src.d.gg.ntiles, slots, src.d.gg.btotal), slots, flags, hipStream_t(stream));
+}
+
+}  // extern "C"

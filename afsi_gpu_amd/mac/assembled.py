@@ -138,8 +138,8 @@ class AssembledTransfer(FETransfer):
         while True:
             workspace.reset(capacity, g.weights)
             attempts += 1
-            _abi.call('afsi_p1_contract_f64', self._g, _abi.ptr(base), _abi.ptr(phi), c, grid, workspace.abi(),
-                      _abi.stream())
+            _abi.call('afsi_p1_contract_f64', self._g, _abi.ptr(base), _abi.ptr(phi), base.shape[1], 0, c, grid,
+                      workspace.abi(), _abi.stream())
             flag = int(workspace.flag.item())
             if flag & 2:
                 raise ValueError('invalid P1 cell table or stencil box for the assembled operator')

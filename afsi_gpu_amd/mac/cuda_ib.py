@@ -1,9 +1,11 @@
 """Drop-in IB backend: the call signatures of afsi_torch ``mac/cuda_ib.py``.
 
 ``build``, ``require_cuda``, ``indexed_gather``, ``indexed_spread``,
-``compact_gather`` and ``compact_spread`` keep the reference's names,
+``compact_gather``, ``compact_spread``, ``contract`` and
+``hash_accumulate`` keep the reference's names,
 argument meaning, ownership (fresh outputs; inputs made contiguous) and
-error behaviour (mac/cuda_ib.py:38-90; mac/csrc/transfer.cpp:10-57), and
+error behaviour (mac/cuda_ib.py:38-117; mac/csrc/transfer.cpp:10-57;
+mac/csrc/ib.cpp:14-54), and
 run hand-written gfx950 kernels from ``libafsi_hip.so`` on the current
 stream. Out-of-range indices do not trap the device as the reference's
 CUDA_KERNEL_ASSERT does (mac/csrc/transfer.cu:19,41); they set a device
@@ -185,3 +187,91 @@ def compact_spread(grid, force, stencil, *, check=True):
         result.append(out)
     _raise_flags(flags, check)
     return tuple(result)
+
+
+def _hash_table(workspace):
+    """The reference's HashWorkspace (keys int64, values, flag int32;
+    mac/hash_transfer_assembly.py:26-50) as the C ABI's table."""
+    keys, values, flag = workspace.keys, workspace.values, workspace.flag
+    if keys is None or values is None or flag is None:
+        raise RuntimeError('IB hash workspace is not allocated')
+    n = keys.numel()
+    if keys.dtype != torch.int64 or values.dtype != torch.float64 or values.numel() != n:
+        raise RuntimeError('IB hash shape mismatch')
+    if n <= 0 or n & (n - 1):
+        raise RuntimeError('IB requires power-of-two hash capacity')
+    if flag.numel() != 1 or flag.dtype != torch.int32:
+        raise RuntimeError('IB flag must contain one int32')
+    for tensor in (keys, values, flag):
+        if not tensor.is_cuda or not tensor.is_contiguous():
+            raise RuntimeError('IB tensors must be contiguous and share a CUDA device')
+    return _abi.HashTable(_abi.ptr(keys), _abi.ptr(values), n, _abi.ptr(flag))
+
+
+def contract(stencil, group, component, offset, low, width, prefix, count, face_shape, workspace, cache=None):
+    """mac/cuda_ib.py:93-106: integrate one cell group's P1 contributions
+    B_c[node, face] += W_eq N_qa phi(face - x_eq) into the device hash table
+    under the key node * faces_c + face (mac/csrc/ib.cpp:58-65), without
+    materialising raw entries. ``low``/``width``/``prefix``/``count`` are
+    the reference's per-cell stencil boxes; they are checked for shape, and
+    each cell's wave derives its own box from the stencil. A probe overflow
+    sets bit 1 of ``workspace.flag`` (the reference's retry signal); an
+    invalid cell table or a box outside the face arrays sets bit 2."""
+    if cache is not None:
+        raise NotImplementedError('the cached contraction (cached_contract_) is not provided; pass cache=None')
+    if getattr(stencil, 'layout', None) == 'shared':
+        raise RuntimeError('IB contraction takes the compact (3,P,3) stencil layout')
+    values, weights, cells = group.values, group.weights, group.cells
+    require_cuda(weights)
+    base, phi = stencil.base, stencil.phi
+    if base.dim() != 3 or base.size(0) != 3 or base.size(2) != 3 or base.dtype != torch.int64:
+        raise RuntimeError('IB base shape mismatch')
+    if phi.dim() != 4 or phi.shape[:3] != base.shape or phi.size(3) != 4 or phi.dtype != torch.float64:
+        raise RuntimeError('IB phi shape mismatch')
+    if values.dim() != 2 or values.size(1) != cells.size(1) or values.size(1) > 4:
+        raise RuntimeError('P1 shape values must be Q x 4')
+    if cells.dim() != 2 or cells.size(1) != values.size(1):
+        raise RuntimeError('P1 cells must be E x 4')
+    E, Q = weights.shape
+    if cells.size(0) != E or values.size(0) != Q:
+        raise RuntimeError('IB quadrature shape mismatch')
+    if low.shape != (E, 3) or width.shape != low.shape or prefix.numel() != E + 1:
+        raise RuntimeError('IB bounds shape mismatch')
+    offset, count, component = int(offset), int(count), int(component)
+    if not (0 <= offset <= base.size(1)) or weights.numel() > base.size(1) - offset:
+        raise RuntimeError('IB site offset out of range')
+    if count < 0 or (count and not E) or component not in (0, 1, 2):
+        raise RuntimeError('invalid IB site count or component')
+    nf = tuple(int(n) for n in face_shape)
+    if len(nf) != 3 or min(nf) <= 0:
+        raise RuntimeError('invalid IB face shape')
+    cells64 = cells.to(torch.int64).contiguous()
+    vals = values.to(torch.float64).contiguous()
+    wts = weights.to(torch.float64).contiguous()
+    # node ids are checked against the largest count whose keys fit int64
+    # (no host read of the cell table)
+    node_bound = ((1 << 62) // (nf[0] * nf[1] * nf[2])) - 1
+    geometry = _abi.FEGeometry(_abi.ptr(cells64), _abi.ptr(vals), _abi.ptr(wts), E, cells.size(1), Q,
+                               node_bound, None, None)
+    grid = _abi.Grid3()
+    for a in range(3):
+        grid.n[a] = nf[a] - (1 if a == component else 0)
+        grid.h[a] = 1.
+    grid.periodic = 0
+    if E:
+        _abi.call('afsi_p1_contract_f64', geometry, _abi.ptr(base.contiguous()), _abi.ptr(phi.contiguous()),
+                  base.size(1), offset, component, grid, _hash_table(workspace), _abi.stream())
+
+
+def hash_accumulate(keys, values, workspace):
+    """mac/cuda_ib.py:109-112: values[slot(key)] += value for raw (key,
+    value) pairs, zeros skipped, SplitMix64 slots with linear probing
+    (mac/csrc/ib.cu:16-54,154-166); an overflow sets workspace.flag."""
+    require_cuda(values)
+    if keys.dim() != 1 or values.dim() != 1 or keys.numel() != values.numel():
+        raise RuntimeError('IB hash shape mismatch')
+    if keys.dtype != torch.int64 or values.dtype != torch.float64:
+        raise RuntimeError('IB hash keys must be int64 and values float64')
+    keys, values = keys.contiguous(), values.contiguous()
+    _abi.call('afsi_hash_accumulate_f64', _abi.ptr(keys), _abi.ptr(values), keys.numel(), _hash_table(workspace),
+              _abi.stream())

@@ -6,8 +6,9 @@
   ``cuda_ib.compact_gather`` / ``compact_spread``.
 * ``bin_points``: build-own stable CSR of points by tile-major cell key.
 * ``TiledTransfer``: spread / interpolate all three components straight
-  from positions over the CSR (weights in registers, per-tile LDS
-  accumulation), the fast path the IB step uses.
+  from positions over the CSR, the fast path the IB step uses. The spread
+  is a per-face-tile gather: every face is produced by the one workgroup
+  that owns its tile and written once (no atomics in HBM).
 """
 from dataclasses import dataclass
 import torch
@@ -133,17 +134,33 @@ class TiledTransfer:
         self._g = grid.abi()
         self._tile = _abi.i32x3(self.tile)
         nbytes = int(_abi.load().afsi_spread_workspace_bytes(self._g, self._tile))
-        self.workspace = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
+        if nbytes == 0:
+            raise ValueError('invalid grid or tile for the tiled transfer')
+        self.device = torch.device(device)
+        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        self.spread_workspace = None
         self.flags = torch.zeros((), dtype=torch.int32, device=device) if flags is None else flags
 
+    def _spread_workspace(self, count):
+        """Workspace of the gather spread for ``count`` points (halo counts,
+        work list, scratch slots of split tiles); grown on demand."""
+        nbytes = int(_abi.load().afsi_spread_gather_workspace_bytes(self._g, self._tile, count))
+        if nbytes == 0:
+            raise ValueError('invalid grid or tile for the tiled spread')
+        if self.spread_workspace is None or self.spread_workspace.numel() < nbytes:
+            self.spread_workspace = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        return self.spread_workspace
+
     def spread(self, points, force, bins, out=None):
-        """Three face arrays f_c = sum_p w_c(p) F_c(p) / V."""
+        """Three face arrays f_c = sum_p w_c(p) F_c(p) / V (every face
+        overwritten)."""
         if out is None:
             out = self.grid.zeros(device=points.device, dtype=torch.float64)
+        ws = self._spread_workspace(points.shape[0])
         _abi.call('afsi_spread_tiled_f64', _abi.ptr(points), _abi.ptr(force), points.shape[0],
                   self._g, self._tile, _abi.ptr(bins.perm), _abi.ptr(bins.cell_ptr),
-                  _abi.ptr(out[0]), _abi.ptr(out[1]), _abi.ptr(out[2]), _abi.ptr(self.workspace),
-                  self.workspace.numel(), _abi.ptr(self.flags), _abi.stream())
+                  _abi.ptr(out[0]), _abi.ptr(out[1]), _abi.ptr(out[2]), _abi.ptr(ws),
+                  ws.numel(), _abi.ptr(self.flags), _abi.stream())
         return out
 
     def interpolate(self, points, velocity, bins=None, out=None):

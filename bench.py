@@ -22,7 +22,8 @@ continuous batching, p50/p99 latency.
 red-black sweep (algorithmic bytes: read p and rhs, write p = 3 x 4 B per
 cell in FP32), timed with HIP events on the stream it is launched on;
 `traffic` is its measured HBM bytes per launch (rocprofv3 counters).
-`cpu_baseline` times the FP64 NumPy oracle on a bounded sample.
+`cpu_baseline` times the reference's CPU tensor path (restated in torch,
+FP64, all host threads) on full steps of the same workload.
 """
 import argparse
 import gc
@@ -63,6 +64,7 @@ def parse():
     ap.add_argument('--points', type=int, default=1_500_000)
     ap.add_argument('--n', type=int, default=128)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-dropin', action='store_true', help='skip the reference-API drop-in sample')
     ap.add_argument('--kernels-only', type=int, default=0, metavar='R',
                     help='only launch the roofline sweep and a copy kernel R times each')
     ap.add_argument('--serve-seconds', type=float, default=5.,
@@ -78,34 +80,66 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(x, F, n, cycles):
-    """FP64 oracle on this host: prepare+spread and gather on a 100k-point
-    sample and one RBGS V(2,2) cycle at n^3, scaled to the full step."""
-    from oracle import ib3d, mac3d
-    grid = ib3d.Grid3((n,) * 3, (1., 1., 1.), periodic=True)
-    sample = min(100_000, len(x))
+def host_cpu():
+    """CPU model, sockets, cores and threads of this host (lscpu)."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(['lscpu'], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    for line in out.splitlines():
+        key, _, value = line.partition(':')
+        if key.strip() in ('Model name', 'Socket(s)', 'Core(s) per socket', 'Thread(s) per core', 'CPU(s)'):
+            info[key.strip()] = value.strip()
+    return info
+
+
+def cpu_threads():
+    """Threads the baseline may use: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS where the host sets it (a shared host's CPU
+    share)."""
+    import os
+    cores = len(os.sched_getaffinity(0))
+    env = os.environ.get('OMP_NUM_THREADS', '')
+    return min(cores, int(env)) if env.isdigit() and int(env) > 0 else cores
+
+
+def cpu_baseline(x, F, n, dt, warmup=2, repeats=5):
+    """The reference's CPU tensor path (oracle/torch_cpu.py: chunked
+    index_add_ spread and gather, Jacobi V(4,4) multigrid to rtol 1e-10
+    checked every 2 cycles) on this host's cores: `warmup` untimed steps,
+    then the median of `repeats` timed full steps on all points, each from
+    the previous step's velocity and pressure."""
+    import torch
+    from oracle import torch_cpu
+    threads = cpu_threads()
+    torch.set_num_threads(threads)
+    shape, spacing = (n,) * 3, (1. / n,) * 3
+    mg = torch_cpu.JacobiMultigrid(shape, spacing, smooth=4)
+    xt, Ft = torch.tensor(x), torch.tensor(F)
+    u = tuple(torch.zeros(shape, dtype=torch.float64) for _ in range(3))
+    p, times, cycles = None, [], []
+    for k in range(warmup + repeats):
+        t0 = time.perf_counter()
+        _, u, p, _, c = torch_cpu.ib_step(xt, Ft, u, p, shape, spacing, mg, dt=dt)
+        if k >= warmup:
+            times.append(time.perf_counter() - t0)
+            cycles.append(c)
+    step_s = float(np.median(times))
+    base, phi = torch_cpu.prepare(xt, shape, spacing)
     t0 = time.perf_counter()
-    base, phi = ib3d.prepare_compact(x[:sample], grid)
-    f = ib3d.spread_grid(F[:sample], base, phi, grid)
+    f = torch_cpu.spread(Ft, base, phi, shape, spacing[0] ** 3)
     t1 = time.perf_counter()
-    ib3d.gather_grid(f, base, phi, grid)
+    torch_cpu.gather(f, base, phi, shape)
     t2 = time.perf_counter()
-    mg = mac3d.Multigrid((n,) * 3, grid.spacing, mac3d.MGOptions(smooth=2, smoother='rbgs'),
-                         periodic=True)
-    rhs = np.random.default_rng(0).normal(size=(n,) * 3)
-    rhs -= rhs.mean()
-    t3 = time.perf_counter()
-    mg.cycle(0, np.zeros_like(rhs), rhs)
-    t4 = time.perf_counter()
-    scale = len(x) / sample
-    spread_s, interp_s, cycle_s = (t1 - t0) * scale, (t2 - t1) * scale, t4 - t3
-    step_s = spread_s + interp_s + cycles * cycle_s
-    return dict(value=1 / step_s, unit='steps/s', cores=1, kind='port',
-                sample=f'oracle prepare+spread and gather on {sample} of {len(x)} points scaled '
-                       f'linearly, one RBGS V(2,2) cycle at {n}^3 times {cycles} cycles '
-                       f'(the GPU solve count); single-threaded NumPy FP64',
-                spread_mpts=len(x) / spread_s / 1e6, interp_mpts=len(x) / interp_s / 1e6,
-                vcycle_s=cycle_s)
+    return dict(value=1 / step_s, unit='steps/s', cores=threads, kind='port',
+                sample=f'{repeats} full steps (median, after {warmup} warm-up steps) of the reference CPU '
+                       f'tensor path restated in torch (oracle/torch_cpu.py) on all {len(x)} points at '
+                       f'{n}^3: prepare, chunked index_add_ spread, Jacobi V(4,4) multigrid to rtol 1e-10, '
+                       f'projection, gather; torch threads = {threads}',
+                ms_per_step=1e3 * step_s, mg_cycles=cycles, spread_mpts=len(x) / (t1 - t0) / 1e6,
+                interp_mpts=len(x) / (t2 - t1) / 1e6, host=host_cpu())
 
 
 class paused_gc:
@@ -148,14 +182,14 @@ def run_path(torch, stepper, x0, F, steps, warmup):
 
 
 def one_step(torch, stepper, x0, F):
-    """One step from rest; returns (U, velocity, ||D u*||, ||D u||)."""
+    """One step from rest; returns (U, velocity, ||D u*||, ||D u||) on the host."""
     from afsi_gpu_amd.mac.flow import divergence
     x = x0.clone()
     u = stepper.grid.zeros(device=x.device)
     u, p, U, info = stepper.step(x, F, u, None)
     tentative = float(torch.linalg.vector_norm(stepper.rhs)) * stepper.dt / stepper.rho
     after = float(torch.linalg.vector_norm(divergence(u, stepper.grid)))
-    return U.clone(), tuple(c.clone() for c in u), tentative, after
+    return U.cpu(), tuple(c.cpu() for c in u), tentative, after
 
 
 def time_sweep(torch, solver, repeats=50, batches=5):
@@ -454,6 +488,70 @@ def batched_sample(torch, instances, steps=3, warmup=1):
                      'one launch per multigrid phase, 2D = FP64 channel steps in both runs')
 
 
+def dropin_sample(torch, x0, F, n, reps=5, warmup=2):
+    """The reference-API drop-in path (``mac.cuda_ib.compact_spread`` /
+    ``compact_gather`` on a ``CompactStencil`` from ``mac.transfer.from_points``,
+    with the reference's error check: one flag read-back per call) on the same
+    points in the reference's closed n^3 box, timed with HIP events on the
+    current stream; the tiled path on the same closed box beside it."""
+    from afsi_gpu_amd.mac import MACGrid, cuda_ib
+    from afsi_gpu_amd.mac.transfer import TiledTransfer, bin_points, from_points
+    grid = MACGrid((n,) * 3, (1., 1., 1.))
+    P = x0.shape[0]
+
+    def timed(fn):
+        for _ in range(warmup):
+            out = fn()
+        start = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+        stop = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+        torch.cuda.synchronize()
+        for k in range(reps):
+            start[k].record()
+            out = fn()
+            stop[k].record()
+        torch.cuda.synchronize()
+        return float(np.median([a.elapsed_time(b) for a, b in zip(start, stop)])), out
+
+    ms = {}
+    ms['prepare'], st = timed(lambda: from_points(x0, grid))
+    ms['compact_spread'], f = timed(lambda: cuda_ib.compact_spread(grid, F, st))
+    ms['compact_gather'], _ = timed(lambda: cuda_ib.compact_gather(grid, f, st))
+    del st
+    torch.cuda.empty_cache()
+    tr = TiledTransfer(grid, x0.device)
+    ms['tiled_bin'], bins = timed(lambda: bin_points(x0, grid))
+    ms['tiled_spread'], f2 = timed(lambda: tr.spread(x0, F, bins))
+    ms['tiled_interp'], _ = timed(lambda: tr.interpolate(x0, f2, bins))
+    diff = max(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)) for a, b in zip(f2, f))
+    return dict(workload=f'{P} points of the clustered-shells workload in the closed {n}^3 unit box',
+                ms=ms, compact_spread_mpts=P / ms['compact_spread'] / 1e3,
+                compact_gather_mpts=P / ms['compact_gather'] / 1e3,
+                tiled_spread_mpts=P / ms['tiled_spread'] / 1e3, tiled_interp_mpts=P / ms['tiled_interp'] / 1e3,
+                tiled_vs_compact_spread_rel_l2=diff)
+
+
+def oracle_accuracy(torch, x_np, F_np, n, dt, states):
+    """Each path's one step from rest against the FP64 NumPy oracle's step on
+    the same inputs (oracle/step.ib_step, red-black GS V(2,2), rtol 1e-10):
+    l2-relative error of the interpolated point velocity and of the projected
+    face velocity, and the post-projection divergence relative to the
+    tentative field's."""
+    from oracle import ib3d, mac3d
+    from oracle.step import ib_step
+    grid = ib3d.Grid3((n,) * 3, (1., 1., 1.), periodic=True)
+    mg = mac3d.Multigrid(grid.shape, grid.spacing, mac3d.MGOptions(smooth=2, smoother='rbgs'), periodic=True)
+    u0 = tuple(np.zeros(grid.face_shape(c)) for c in range(3))
+    ref = ib_step(x_np, F_np, u0, None, grid, mg, dt=dt)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out = dict(oracle_cycles=int(ref['info']['cycles']))
+    for name, (U, u, div_star, div) in states.items():
+        out[name] = dict(interp_velocity_rel_l2_vs_oracle=rel(U.cpu().numpy(), ref['point_velocity']),
+                         face_velocity_rel_l2_vs_oracle=max(rel(u[c].cpu().numpy(), ref['velocity'][c])
+                                                            for c in range(3)),
+                         divergence_rel=div / div_star)
+    return out
+
+
 def kernels_only(torch, n, repeats):
     """For hardware-counter passes: `repeats` launches of the level-0 FP32
     red-black sweep at n^3 (the roofline kernel), then `repeats` launches of
@@ -512,12 +610,9 @@ def main():
         stepper = IBStep(grid, P, dt=dt, device=device, options=opts, precision=precision)
         steppers[precision] = stepper
         results[precision] = run_path(torch, stepper, x0, F, args.steps, args.warmup)
-    # accuracy of the mixed step against the plain FP64 step (same inputs)
-    U64, u64, div0, div64 = one_step(torch, steppers['fp64'], x0, F)
-    Umx, umx, _, divmx = one_step(torch, steppers['mixed'], x0, F)
-    u_err = float(torch.linalg.vector_norm(Umx - U64) / torch.linalg.vector_norm(U64))
-    face_err = max(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
-                   for a, b in zip(umx, u64))
+    # accuracy of both paths against the FP64 oracle (same inputs, one step from rest)
+    states = {name: one_step(torch, steppers[key], x0, F) for name, key in (('plain_fp64', 'fp64'),
+                                                                             ('mixed', 'mixed'))}
     inner = steppers['mixed'].flow.pressure_solver.inner
     sweep_s = time_sweep(torch, inner)
     sweep_bytes = 3 * 4 * N
@@ -544,9 +639,10 @@ def main():
                    spread_mpts=P / mixed['phases_ms']['spread'] / 1e3,
                    interp_mpts=P / mixed['phases_ms']['interp'] / 1e3),
         mixed_vs_plain=mixed['steps_per_s'] / plain['steps_per_s'],
-        accuracy=dict(interp_velocity_rel_l2_mixed_vs_fp64=u_err,
-                      face_velocity_rel_l2_mixed_vs_fp64=face_err,
-                      divergence_rel_fp64=div64 / div0, divergence_rel_mixed=divmx / div0))
+        accuracy=oracle_accuracy(torch, x_np, F_np, n, dt, states))
+    del states
+    if not args.no_dropin:
+        result['dropin'] = dropin_sample(torch, x0, F, n)
     if args.batch_instances:
         result['batched'] = batched_sample(torch, args.batch_instances)
     if args.serve_seconds > 0:
@@ -558,7 +654,7 @@ def main():
     if not args.no_assembled:
         result['assembled'] = assembled_sample(torch)
     if not args.no_cpu_baseline:
-        result['cpu_baseline'] = cpu_baseline(x_np, F_np, n, int(round(plain['mg_cycles_mean'])))
+        result['cpu_baseline'] = cpu_baseline(x_np, F_np, n, dt)
     print(json.dumps(result))
 
 

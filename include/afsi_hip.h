@@ -115,16 +115,22 @@ int afsi_bin_points(const double* x, int64_t points, const afsi_grid3* grid,
 
 /* ---- Tiled transfer over the binned CSR (build-own fast path) -------------
  * Same mathematics as compact_spread/compact_gather for all three
- * components at once, weights computed from positions in registers.
- * Spread: one work item per (tile, chunk of points); per-tile LDS
- * accumulation of all three components with ds_add_f64, then one global
- * atomic per touched face. u,v,w are overwritten (zero-filled first).
- * Interp: the same work items; each stages its tile's face box of all three
+ * components at once, weights computed from positions.
+ * Spread: a per-face-tile gather. The workgroup that owns a tile produces
+ * all its faces from the points binned in the tile and its two-cell halo
+ * (same-cell runs accumulated in registers, folded into an LDS tile box)
+ * and writes each face once; no global atomics. Tiles whose halo holds more
+ * points than one work item takes are split and their chunk boxes added in
+ * chunk order by a second launch. u,v,w are overwritten. The workspace
+ * (afsi_spread_gather_workspace_bytes) depends on the point count.
+ * Interp: work items of (tile, chunk of points); each stages its tile's face box of all three
  * components in LDS and gathers its points' 192 values from there; out (P,3)
  * in the caller's point order. With perm = NULL (unbinned) one thread per
  * point gathers from HBM and tile/cell_ptr/workspace are unused. Cubic
  * tiles of edge 8 or 4 use compile-time box extents. */
 size_t afsi_spread_workspace_bytes(const afsi_grid3* grid, const int32_t tile[3]);
+/* Workspace of afsi_spread_tiled_f64 for `points` points; 0 on invalid input. */
+size_t afsi_spread_gather_workspace_bytes(const afsi_grid3* grid, const int32_t tile[3], int64_t points);
 /* Host only (no device work): the smallest and largest face offset, per
  * component, that any tile's LDS box entry maps to in the tiled kernels;
  * lets a caller check that every staged read and flushed write stays inside
@@ -644,14 +650,18 @@ typedef struct afsi_hash_table {
  * 154-166): values[slot(keys[i])] += values[i], zeros skipped. */
 int afsi_hash_accumulate_f64(const int64_t* keys, const double* values, int64_t count, const afsi_hash_table* t,
                              void* stream);
-/* Replaces torch.ops.afsi_ib_cuda.ib_contract (mac/csrc/ib.cu:56-108,
- * 118-152) for one component: adds B_c[node, face] = sum over the cells'
- * interaction points of W_eq N_qa (phi_x phi_y phi_z)(face - x_eq) under the
- * key node * faces_c + face. base (3,P,3) int64 and phi (3,P,3,4) are the
- * compact tables of the cell-major interaction points (P = elements * quad,
- * afsi_prepare_compact_f64); cells of at most 4 nodes; closed boxes only. */
-int afsi_p1_contract_f64(const afsi_fe_geometry* g, const int64_t* base, const double* phi, int component,
-                         const afsi_grid3* grid, const afsi_hash_table* t, void* stream);
+/* Replaces torch.ops.afsi_ib_cuda.ib_contract / hash_contract_
+ * (mac/csrc/ib.cu:56-108, 118-152; mac/csrc/ib.cpp:58-65) for one
+ * component: adds B_c[node, face] = sum over the cells' interaction points
+ * of W_eq N_qa (phi_x phi_y phi_z)(face - x_eq) under the key
+ * node * faces_c + face. base (3,P,3) int64 and phi (3,P,3,4) are compact
+ * tables of P = lattice_points cell-major interaction points
+ * (afsi_prepare_compact_f64); the cells' points start at point `offset`
+ * (the reference's per-group offset). Cells of at most 4 nodes; closed
+ * boxes only. */
+int afsi_p1_contract_f64(const afsi_fe_geometry* g, const int64_t* base, const double* phi, int64_t lattice_points,
+                         int64_t offset, int component, const afsi_grid3* grid, const afsi_hash_table* t,
+                         void* stream);
 /* y[r * y_stride] = scale * sum_k A[r, cols[k]] x[cols[k] * x_stride]: the
  * CSR products of the assembled spread (B^T c / V) and gather (B u) of
  * mac/assembled_transfer.py:199-211 (torch.sparse.mm). */

@@ -122,3 +122,51 @@ def test_assembled_transfer_matches_fe_transfer_and_oracle(gpu):
         AssembledTransfer(grid, geometry, max_entries=100).prepare(xt)
     with pytest.raises(ValueError, match='support'):
         asm.prepare(xt + 10)
+
+
+def test_dropin_contract_and_hash_accumulate(gpu):
+    """The reference-API ops (mac/cuda_ib.py:93-112) called the way
+    mac/hash_transfer_assembly.py:80-130 calls them: per cell group with its
+    point offset and stencil boxes, into one hash table per component; the
+    table holds the oracle's dense B_c. The cells are split into two groups
+    so the second runs at a nonzero offset."""
+    from types import SimpleNamespace
+    from afsi_gpu_amd.mac import cuda_ib
+    from afsi_gpu_amd.mac.transfer import from_points
+    g, X, geometry, grid, ogrid = setup(gpu)
+    x = X + .013 * np.sin(3 * X)
+    pts = ofe.evaluate(g, x)
+    stencil = from_points(t(pts, gpu), grid)
+    cells, weights = t(g.cells, gpu, torch.int64), t(g.weights, gpu)
+    E, Q = weights.shape
+    half = E // 2
+    groups = [SimpleNamespace(values=t(g.values, gpu), weights=weights[:half].contiguous(), cells=cells[:half]),
+              SimpleNamespace(values=t(g.values, gpu), weights=weights[half:].contiguous(), cells=cells[half:])]
+    for c in range(3):
+        shape = grid.face_shape(c)
+        work = HashWorkspace()
+        work.reset(1 << 16, weights)
+        offset = 0
+        for group in groups:
+            e = group.weights.shape[0]
+            base = stencil.base[c, offset:offset + e * Q].view(e, Q, 3)
+            low = base.amin(1)
+            width = base.amax(1) - low + 4
+            sizes = width.prod(-1)
+            prefix = torch.cat((sizes.new_zeros(1), sizes.cumsum(0)))
+            cuda_ib.contract(stencil, group, c, offset, low, width, prefix, int(prefix[-1]), shape, work)
+            offset += group.weights.numel()
+        assert work.flag.item() == 0
+        keys, vals = sorted_entries(work)
+        nf = int(np.prod(shape))
+        B = oas.dense_operator(g, pts, ogrid, c)
+        got = np.zeros_like(B)
+        k = keys.cpu().numpy()
+        got[k // nf, k % nf] = vals.cpu().numpy()
+        assert rel(got, B) < 1e-13
+        # hash_accumulate of the same entries doubles every value
+        cuda_ib.hash_accumulate(keys, vals, work)
+        k2, v2 = sorted_entries(work)
+        assert torch.equal(k2, keys) and torch.equal(v2, 2 * vals)
+    with pytest.raises(NotImplementedError):
+        cuda_ib.contract(stencil, groups[0], 0, 0, low, width, prefix, 0, grid.face_shape(0), work, cache=object())

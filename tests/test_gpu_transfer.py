@@ -166,10 +166,13 @@ def test_tiled_spread_and_interp(gpu, periodic, tile):
     xt, Ft = torch.tensor(x, device=gpu), torch.tensor(F, device=gpu)
     bins = bin_points(xt, g, tile)
     tr = TiledTransfer(g, gpu, tile)
-    f = tr.spread(xt, Ft, bins)
+    # every face is owned by one tile and written once: stale values go
+    stale = tuple(torch.full(g.face_shape(c), float('nan'), device=gpu, dtype=torch.float64) for c in range(3))
+    f = tr.spread(xt, Ft, bins, out=stale)
     ut = tuple(torch.tensor(a, device=gpu) for a in u)
     U = tr.interpolate(xt, ut, bins)
     assert int(tr.flags.item()) == 0
+    assert all(bool(torch.isfinite(fc).all()) for fc in f)
     # unbinned path (one thread per point, HBM gathers): same weights and
     # summation order as the LDS-staged binned path, so identical bits
     assert torch.equal(tr.interpolate(xt, ut, None), U)
