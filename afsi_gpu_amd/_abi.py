@@ -80,6 +80,15 @@ class MGPlan2(Structure):
 
 
 
+
+class CHBItem(Structure):
+    _fields_ = [('x', c_void_p), ('points', c_int64), ('grid', Grid2), ('plan', MGPlan2)]
+
+
+class CHBBatch(Structure):
+    _fields_ = [('count', c_int32), ('reserved', c_int32), ('items', POINTER(CHBItem)),
+                ('workspace', c_void_p), ('workspace_bytes', c_size_t)]
+
 class FEGeometry(Structure):
     _fields_ = [('cells', c_void_p), ('values', c_void_p), ('weights', c_void_p), ('elements', c_int64),
                 ('nodes_per_cell', c_int32), ('quad', c_int32), ('node_count', c_int64),
@@ -229,6 +238,13 @@ _SIGNATURES = {
     **{f'afsi_p2_gather_assemble_{t}': (c_int, [_P, _P, _P, POINTER(P2Group), POINTER(P2Tables), POINTER(Grid3),
                                                 _P, _P, _P]) for t in ('f64', 'f32')},
     'afsi_hash_accumulate_f64': (c_int, [_P, _P, c_int64, POINTER(HashTable), _P]),
+    'afsi_chb_sizes': (c_int, [POINTER(CHBBatch), _I64, _I64, _I64, _I64, POINTER(c_size_t)]),
+    'afsi_chb_prepare': (c_int, [POINTER(CHBBatch), _P]),
+    'afsi_chb_stencil_f64': (c_int, [POINTER(CHBBatch), _P, _P, _P, _P, _P, _P]),
+    'afsi_chb_divergence_f64': (c_int, [POINTER(CHBBatch), _P, _P, c_double, _P]),
+    'afsi_chb_correct_f64': (c_int, [POINTER(CHBBatch), _P, _P, c_double, _P]),
+    'afsi_chb_cycles_f64': (c_int, [POINTER(CHBBatch), _P, c_int32, _P]),
+    'afsi_chb_stats_f64': (c_int, [POINTER(CHBBatch), c_int, _P, _P]),
     'afsi_p1_contract_f64': (c_int, [POINTER(FEGeometry), _P, _P, c_int64, c_int64, c_int, POINTER(Grid3),
                                      POINTER(HashTable), _P]),
     'afsi_csr_spmv_f64': (c_int, [POINTER(CSR), _P, c_int64, _P, c_int64, c_double, _P]),

@@ -14,6 +14,8 @@
 // (mac2d/_triton_mg.py:7-75), so results are bit-identical to the
 // restatement in FP64 (compiled with -ffp-contract=off).
 #include "common.hpp"
+#include <cstring>
+#include <vector>
 
 namespace afsi {
 
@@ -421,8 +423,7 @@ __device__ void chb_smooth(const afsi_mg2d_plan& pl, int l, double* red) {
 }
 
 template <class T, bool P>
-__global__ void __launch_bounds__(CHB_THREADS) ch_block_cycles_kernel(afsi_mg2d_plan pl, int cycles) {
-    __shared__ double red[CHB_THREADS];
+__device__ void chb_cycles(const afsi_mg2d_plan& pl, int cycles, double* red) {
     const int last = pl.levels - 1;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
     for (int cyc = 0; cyc < cycles; ++cyc) {
@@ -485,6 +486,12 @@ __global__ void __launch_bounds__(CHB_THREADS) ch_block_cycles_kernel(afsi_mg2d_
             chb_smooth<T, P>(pl, l, red);
         }
     }
+}
+
+template <class T, bool P>
+__global__ void __launch_bounds__(CHB_THREADS) ch_block_cycles_kernel(afsi_mg2d_plan pl, int cycles) {
+    __shared__ double red[CHB_THREADS];
+    chb_cycles<T, P>(pl, cycles, red);
 }
 
 // ---- native V-cycle driver ---------------------------------------------------
@@ -733,6 +740,372 @@ int afsi_mg2d_residual_f64(const afsi_mg2d_plan* plan, void* stream) {
 }
 int afsi_mg2d_residual_f32(const afsi_mg2d_plan* plan, void* stream) {
     return channel_residual<float>(plan, stream);
+}
+
+}  // extern "C"
+
+// ---- batched channel instances (build-own) ------------------------------------
+// Many channels of uneven size advanced together: one launch per phase over
+// all instances. Face arrays are concatenated (instance i's u faces start
+// at u_off, its v faces at v_off); each instance's pressure hierarchy is its
+// own plan. Stencils carry global face ids, so one indexed spread/gather
+// per component serves the whole batch.
+
+namespace afsi {
+namespace {
+
+struct ChbDesc {
+    const double* x;   // (points, 2)
+    int64_t points, point_begin;
+    Dim2 d;
+    double hx, hy, Lx, Ly;
+    int32_t periodic, reserved;
+    int64_t u_off, v_off, u_count, v_count, cell_count;
+};
+
+__device__ inline int chb_find(const ChbDesc* __restrict__ desc, int n, int64_t j, int which) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        const int64_t b = which == 0 ? desc[mid].point_begin : (which == 1 ? desc[mid].u_off : desc[mid].v_off);
+        if (b <= j) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The 16-link reflected stencil of every point (ch_prepare_kernel's
+// expressions) with the instance's face offset added to the ids.
+__global__ void __launch_bounds__(CH_THREADS) chb_stencil_kernel(const ChbDesc* __restrict__ desc, int n,
+                                                                 int64_t total, int64_t* __restrict__ ids_u,
+                                                                 double* __restrict__ w_u, int64_t* __restrict__ ids_v,
+                                                                 double* __restrict__ w_v, int* __restrict__ flags) {
+    for (int64_t t = int64_t(blockIdx.x) * CH_THREADS + threadIdx.x; t < 2 * total;
+         t += int64_t(gridDim.x) * CH_THREADS) {
+        const int c = int(t / total);
+        const int64_t g = t - c * total;
+        const ChbDesc& D = desc[chb_find(desc, n, g, 0)];
+        const int64_t q = g - D.point_begin;
+        const double px = D.x[2 * q], py = D.x[2 * q + 1];
+        const Dim2 d = D.d;
+        bool ok = isfinite(px) && isfinite(py) && py >= -D.hy && py <= D.Ly + D.hy;
+        if (!D.periodic) ok = ok && px >= 2 * D.hx && px <= D.Lx - 2 * D.hx;
+        int64_t* ids = (c == 0 ? ids_u : ids_v) + g * 16;
+        double* w = (c == 0 ? w_u : w_v) + g * 16;
+        const int64_t off = c == 0 ? D.u_off : D.v_off;
+        if (!ok) {
+            raise_flag(flags, F_SUPPORT);
+            for (int k = 0; k < 16; ++k) {
+                ids[k] = off;
+                w[k] = 0.0;
+            }
+            continue;
+        }
+        const double ox = D.hx * (c == 0 ? 0.0 : 0.5), oy = D.hy * (c == 1 ? 0.0 : 0.5);
+        const double sx = (px - ox) / D.hx, sy = (py - oy) / D.hy;
+        const int64_t bx = int64_t(floor(sx - 1)), by = int64_t(floor(sy - 1));
+        const int64_t fny = d.ny + (c == 1);
+        double fx[4], fy[4];
+        for (int a = 0; a < 4; ++a) {
+            fx[a] = peskin_phi(sx - double(bx + a));
+            fy[a] = peskin_phi(sy - double(by + a));
+        }
+        for (int i = 0; i < 4; ++i) {
+            int64_t nxi = bx + i;
+            if (D.periodic) nxi = ((nxi % d.nx) + d.nx) % d.nx;
+            for (int j = 0; j < 4; ++j) {
+                const int64_t row = by + j;
+                bool outside;
+                int64_t refl;
+                if (c == 0) {
+                    outside = row < 0 || row >= d.ny;
+                    refl = row < 0 ? -row - 1 : (row >= d.ny ? 2 * d.ny - row - 1 : row);
+                } else {
+                    outside = row < 0 || row > d.ny;
+                    refl = row < 0 ? -row : (row > d.ny ? 2 * d.ny - row : row);
+                }
+                const double wt = fx[i] * fy[j];
+                ids[4 * i + j] = off + nxi * fny + refl;
+                w[4 * i + j] = outside ? -wt : wt;
+            }
+        }
+    }
+}
+
+// rhs[0] of instance i = scale * D u (ch_divergence_kernel's expression).
+__global__ void __launch_bounds__(CH_THREADS) chb_divergence_kernel(const ChbDesc* __restrict__ desc,
+                                                                    const afsi_mg2d_plan* __restrict__ plans,
+                                                                    const int64_t* __restrict__ cell_begin, int n,
+                                                                    int64_t total, const double* __restrict__ u,
+                                                                    const double* __restrict__ v, double scale) {
+    for (int64_t j = int64_t(blockIdx.x) * CH_THREADS + threadIdx.x; j < total;
+         j += int64_t(gridDim.x) * CH_THREADS) {
+        int lo = 0, hi = n;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (cell_begin[mid] <= j) lo = mid;
+            else hi = mid;
+        }
+        const ChbDesc& D = desc[lo];
+        const Dim2 d = D.d;
+        const int i = int(j - cell_begin[lo]);
+        const int x = i / d.ny, y = i - x * d.ny;
+        const double* ui = u + D.u_off;
+        const double* vi = v + D.v_off;
+        const int xr = D.periodic ? (x + 1 < d.nx ? x + 1 : 0) : x + 1;
+        const double du = (ui[xr * d.ny + y] - ui[i]) / D.hx;
+        const double dv = (vi[x * (d.ny + 1) + y + 1] - vi[x * (d.ny + 1) + y]) / D.hy;
+        static_cast<double*>(plans[lo].rhs[0])[i] = scale * (du + dv);
+    }
+}
+
+// u -= coef * G p with p = the instance's p[0] (ch_grad_correct_kernel's
+// expressions); u faces of all instances first, then v faces.
+__global__ void __launch_bounds__(CH_THREADS) chb_correct_kernel(const ChbDesc* __restrict__ desc,
+                                                                 const afsi_mg2d_plan* __restrict__ plans, int n,
+                                                                 int64_t nu, int64_t nv, double* __restrict__ u,
+                                                                 double* __restrict__ v, double coef) {
+    for (int64_t t = int64_t(blockIdx.x) * CH_THREADS + threadIdx.x; t < nu + nv;
+         t += int64_t(gridDim.x) * CH_THREADS) {
+        if (t < nu) {
+            const int k = chb_find(desc, n, t, 1);
+            const ChbDesc& D = desc[k];
+            const Dim2 d = D.d;
+            const double* p = static_cast<const double*>(plans[k].p[0]);
+            const int i = int(t - D.u_off);
+            const int x = i / d.ny, y = i - x * d.ny;
+            double g;
+            if (D.periodic) g = (p[x * d.ny + y] - p[(x > 0 ? x - 1 : d.nx - 1) * d.ny + y]) / D.hx;
+            else if (x == 0) continue;
+            else if (x == d.nx) g = -2 * p[(d.nx - 1) * d.ny + y] / D.hx;
+            else g = (p[x * d.ny + y] - p[(x - 1) * d.ny + y]) / D.hx;
+            u[t] = u[t] - coef * g;
+        } else {
+            const int64_t tv = t - nu;
+            const int k = chb_find(desc, n, tv, 2);
+            const ChbDesc& D = desc[k];
+            const Dim2 d = D.d;
+            const double* p = static_cast<const double*>(plans[k].p[0]);
+            const int j = int(tv - D.v_off), x = j / (d.ny + 1), y = j - x * (d.ny + 1);
+            if (y == 0 || y == d.ny) continue;
+            const double g = (p[x * d.ny + y] - p[x * d.ny + y - 1]) / D.hy;
+            v[tv] = v[tv] - coef * g;
+        }
+    }
+}
+
+// One workgroup per active instance: `cycles` V-cycles (ch_block_cycles).
+__global__ void __launch_bounds__(CHB_THREADS) chb_cycles_kernel(const afsi_mg2d_plan* __restrict__ plans,
+                                                                 const int32_t* __restrict__ active, int cycles) {
+    __shared__ double red[CHB_THREADS];
+    const afsi_mg2d_plan& pl = plans[blockIdx.x];
+    if (active && !active[blockIdx.x]) return;
+    if (pl.periodic) chb_cycles<double, true>(pl, cycles, red);
+    else chb_cycles<double, false>(pl, cycles, red);
+}
+
+// One workgroup per instance: [sum, sum of squares, sum |.|, non-finite
+// count] of rhs[0] (which 0), p[0] (1) or rhs[0] - A p[0] (2), summed in a
+// fixed order; which 3/4: subtract the mean out[4i]/n from rhs[0]/p[0].
+__global__ void __launch_bounds__(CHB_THREADS) chb_stats_kernel(const afsi_mg2d_plan* __restrict__ plans, int which,
+                                                                double* __restrict__ out) {
+    __shared__ double red[4][CHB_THREADS];
+    const afsi_mg2d_plan& pl = plans[blockIdx.x];
+    const Dim2 d{int(pl.n[0][0]), int(pl.n[0][1])};
+    const int n = d.nx * d.ny;
+    double* rhs = static_cast<double*>(pl.rhs[0]);
+    double* p = static_cast<double*>(pl.p[0]);
+    if (which >= 3) {  // mean removal: periodic channels only (the reference channel is non-singular)
+        if (!pl.periodic) return;
+        double* x = which == 3 ? rhs : p;
+        const double mean = out[4 * blockIdx.x] / double(n);
+        for (int i = threadIdx.x; i < n; i += CHB_THREADS) x[i] = x[i] - mean;
+        return;
+    }
+    const Coef2<double> k{pl.h2[0][0], pl.h2[0][1], 0.0, 0.0};
+    double s = 0.0, q = 0.0, a = 0.0, bad = 0.0;
+    for (int i = threadIdx.x; i < n; i += CHB_THREADS) {
+        double v;
+        if (which == 0) v = rhs[i];
+        else if (which == 1) v = p[i];
+        else {
+            const int x = i / d.ny, y = i - x * d.ny;
+            double c;
+            v = pl.periodic ? rhs[i] - ch_apply<double, true>(p, x, y, d, k, c)
+                            : rhs[i] - ch_apply<double, false>(p, x, y, d, k, c);
+        }
+        if (isfinite(v)) {
+            s += v;
+            q += v * v;
+            a += fabs(v);
+        } else {
+            bad += 1.0;
+        }
+    }
+    red[0][threadIdx.x] = s;
+    red[1][threadIdx.x] = q;
+    red[2][threadIdx.x] = a;
+    red[3][threadIdx.x] = bad;
+    __syncthreads();
+    for (int w = CHB_THREADS / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w)
+            for (int m = 0; m < 4; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+struct ChbLayout {
+    int n = 0;
+    int64_t points = 0, nu = 0, nv = 0, cells = 0;
+    size_t desc = 0, plans = 0, cell_begin = 0, total = 0;
+};
+
+size_t chb_align(size_t b) { return (b + 255) & ~size_t(255); }
+
+int chb_layout(const afsi_chb_batch* b, ChbLayout& L, std::vector<ChbDesc>* descs,
+               std::vector<int64_t>* cell_begin) {
+    AFSI_REQUIRE(b && b->items && b->count >= 1, "invalid channel batch");
+    L.n = b->count;
+    for (int i = 0; i < L.n; ++i) {
+        const afsi_chb_item& it = b->items[i];
+        const afsi_grid2& g = it.grid;
+        AFSI_REQUIRE(dims2_ok(g.n) && g.n[0] >= 4 && g.n[1] >= 4, "channel batch item %d: invalid grid", i);
+        AFSI_REQUIRE(it.points >= 0 && (it.points == 0 || it.x), "channel batch item %d: invalid points", i);
+        const afsi_mg2d_plan& pl = it.plan;
+        if (int r = channel_check<double>(&pl)) return r;
+        AFSI_REQUIRE(pl.n[0][0] == g.n[0] && pl.n[0][1] == g.n[1] && pl.periodic == g.periodic,
+                     "channel batch item %d: plan does not match its grid", i);
+        ChbDesc d{};
+        d.x = it.x;
+        d.points = it.points;
+        d.point_begin = L.points;
+        d.d = Dim2{int(g.n[0]), int(g.n[1])};
+        d.hx = g.h[0];
+        d.hy = g.h[1];
+        d.Lx = g.lengths[0];
+        d.Ly = g.lengths[1];
+        d.periodic = g.periodic;
+        d.u_off = L.nu;
+        d.v_off = L.nv;
+        d.u_count = (g.periodic ? g.n[0] : g.n[0] + 1) * g.n[1];
+        d.v_count = g.n[0] * (g.n[1] + 1);
+        d.cell_count = g.n[0] * g.n[1];
+        if (descs) descs->push_back(d);
+        if (cell_begin) cell_begin->push_back(L.cells);
+        L.points += it.points;
+        L.nu += d.u_count;
+        L.nv += d.v_count;
+        L.cells += d.cell_count;
+    }
+    AFSI_REQUIRE(L.points < (int64_t(1) << 40) && L.nu < (int64_t(1) << 40), "channel batch too large");
+    size_t off = 0;
+    L.desc = off; off = chb_align(off + size_t(L.n) * sizeof(ChbDesc));
+    L.plans = off; off = chb_align(off + size_t(L.n) * sizeof(afsi_mg2d_plan));
+    L.cell_begin = off; off = chb_align(off + size_t(L.n) * sizeof(int64_t));
+    L.total = off;
+    return OK;
+}
+
+int chb_ready(const afsi_chb_batch* b, ChbLayout& L) {
+    if (int r = chb_layout(b, L, nullptr, nullptr)) return r;
+    AFSI_REQUIRE(b->workspace && b->workspace_bytes >= L.total, "channel batch workspace too small");
+    return OK;
+}
+
+template <class D>
+D* chb_at(const afsi_chb_batch* b, size_t off) {
+    return reinterpret_cast<D*>(static_cast<char*>(b->workspace) + off);
+}
+
+}  // namespace
+}  // namespace afsi
+
+extern "C" {
+
+int afsi_chb_sizes(const afsi_chb_batch* batch, int64_t* points, int64_t* u_faces, int64_t* v_faces,
+                   int64_t* cells, size_t* workspace_bytes) {
+    ChbLayout L;
+    if (int r = chb_layout(batch, L, nullptr, nullptr)) return r;
+    if (points) *points = L.points;
+    if (u_faces) *u_faces = L.nu;
+    if (v_faces) *v_faces = L.nv;
+    if (cells) *cells = L.cells;
+    if (workspace_bytes) *workspace_bytes = L.total;
+    return OK;
+}
+
+int afsi_chb_prepare(const afsi_chb_batch* batch, void* stream) {
+    ChbLayout L;
+    std::vector<ChbDesc> descs;
+    std::vector<int64_t> begins;
+    if (int r = chb_layout(batch, L, &descs, &begins)) return r;
+    AFSI_REQUIRE(batch->workspace && batch->workspace_bytes >= L.total, "channel batch workspace too small");
+    std::vector<char> host(L.total, 0);
+    std::memcpy(host.data() + L.desc, descs.data(), descs.size() * sizeof(ChbDesc));
+    for (int i = 0; i < L.n; ++i)
+        std::memcpy(host.data() + L.plans + size_t(i) * sizeof(afsi_mg2d_plan), &batch->items[i].plan,
+                    sizeof(afsi_mg2d_plan));
+    std::memcpy(host.data() + L.cell_begin, begins.data(), begins.size() * sizeof(int64_t));
+    const hipStream_t s = hipStream_t(stream);
+    AFSI_HIP(hipMemcpyAsync(batch->workspace, host.data(), host.size(), hipMemcpyHostToDevice, s));
+    AFSI_HIP(hipStreamSynchronize(s));
+    return OK;
+}
+
+int afsi_chb_stencil_f64(const afsi_chb_batch* batch, int64_t* ids_u, double* w_u, int64_t* ids_v, double* w_v,
+                         int* flags, void* stream) {
+    ChbLayout L;
+    if (int r = chb_ready(batch, L)) return r;
+    if (L.points == 0) return OK;
+    AFSI_REQUIRE(ids_u && w_u && ids_v && w_v && flags, "null channel stencil buffer");
+    chb_stencil_kernel<<<blocks_for(2 * L.points), CH_THREADS, 0, hipStream_t(stream)>>>(
+        chb_at<ChbDesc>(batch, L.desc), L.n, L.points, ids_u, w_u, ids_v, w_v, flags);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_chb_divergence_f64(const afsi_chb_batch* batch, const double* u, const double* v, double scale,
+                            void* stream) {
+    ChbLayout L;
+    if (int r = chb_ready(batch, L)) return r;
+    AFSI_REQUIRE(u && v, "null channel velocity");
+    chb_divergence_kernel<<<blocks_for(L.cells), CH_THREADS, 0, hipStream_t(stream)>>>(
+        chb_at<ChbDesc>(batch, L.desc), chb_at<afsi_mg2d_plan>(batch, L.plans), chb_at<int64_t>(batch, L.cell_begin),
+        L.n, L.cells, u, v, scale);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_chb_correct_f64(const afsi_chb_batch* batch, double* u, double* v, double coef, void* stream) {
+    ChbLayout L;
+    if (int r = chb_ready(batch, L)) return r;
+    AFSI_REQUIRE(u && v, "null channel velocity");
+    chb_correct_kernel<<<blocks_for(L.nu + L.nv), CH_THREADS, 0, hipStream_t(stream)>>>(
+        chb_at<ChbDesc>(batch, L.desc), chb_at<afsi_mg2d_plan>(batch, L.plans), L.n, L.nu, L.nv, u, v, coef);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_chb_cycles_f64(const afsi_chb_batch* batch, const int32_t* active, int32_t cycles, void* stream) {
+    ChbLayout L;
+    if (int r = chb_ready(batch, L)) return r;
+    AFSI_REQUIRE(cycles >= 0, "negative cycle count");
+    if (!cycles) return OK;
+    chb_cycles_kernel<<<unsigned(L.n), CHB_THREADS, 0, hipStream_t(stream)>>>(
+        chb_at<afsi_mg2d_plan>(batch, L.plans), active, cycles);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+int afsi_chb_stats_f64(const afsi_chb_batch* batch, int which, double* out, void* stream) {
+    ChbLayout L;
+    if (int r = chb_ready(batch, L)) return r;
+    AFSI_REQUIRE(which >= 0 && which <= 4 && out, "invalid channel batch statistics request");
+    chb_stats_kernel<<<unsigned(L.n), CHB_THREADS, 0, hipStream_t(stream)>>>(
+        chb_at<afsi_mg2d_plan>(batch, L.plans), which, out);
+    AFSI_LAUNCH_CHECK();
+    return OK;
 }
 
 }  // extern "C"

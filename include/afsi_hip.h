@@ -495,6 +495,44 @@ int afsi_mg2d_cycles_f32(const afsi_mg2d_plan* plan, int32_t cycles, void* strea
 int afsi_mg2d_residual_f64(const afsi_mg2d_plan* plan, void* stream);
 int afsi_mg2d_residual_f32(const afsi_mg2d_plan* plan, void* stream);
 
+/* ---- batched channel instances (build-own; no reference counterpart) ------
+ * Many channels of uneven size in one launch per phase. Face arrays of the
+ * batch are concatenated (u faces of instance 0, 1, ..., then separately v
+ * faces); the stencil kernel writes global face ids into (P_total,16) id and
+ * weight arrays (rows in instance order), so one afsi_indexed_spread_f64 /
+ * afsi_indexed_gather_f64 per component serves the batch. Each instance's
+ * FP64 pressure hierarchy is its own afsi_mg2d_plan: divergence writes its
+ * rhs[0], cycles run one 1024-thread workgroup per active instance
+ * (active[i] != 0; NULL: all), correct reads its p[0]. stats writes
+ * out[4i..4i+3] = [sum, sum of squares, sum |.|, non-finite count] of
+ * rhs[0] (which 0), p[0] (1) or rhs[0] - A p[0] (2) in a fixed order;
+ * which 3 / 4 subtracts out[4i]/cells from rhs[0] / p[0] of the periodic
+ * instances (the reference channel is non-singular). prepare uploads
+ * the descriptors into the workspace (synchronises the stream). */
+typedef struct afsi_chb_item {
+    const double* x;        /* (points,2) positions */
+    int64_t points;
+    afsi_grid2 grid;
+    afsi_mg2d_plan plan;    /* FP64 hierarchy of the instance's pressure solve */
+} afsi_chb_item;
+typedef struct afsi_chb_batch {
+    int32_t count;
+    int32_t reserved;
+    const afsi_chb_item* items;  /* host array */
+    void* workspace;             /* device */
+    size_t workspace_bytes;
+} afsi_chb_batch;
+int afsi_chb_sizes(const afsi_chb_batch* batch, int64_t* points, int64_t* u_faces, int64_t* v_faces,
+                   int64_t* cells, size_t* workspace_bytes);
+int afsi_chb_prepare(const afsi_chb_batch* batch, void* stream);
+int afsi_chb_stencil_f64(const afsi_chb_batch* batch, int64_t* ids_u, double* w_u, int64_t* ids_v, double* w_v,
+                         int* flags, void* stream);
+int afsi_chb_divergence_f64(const afsi_chb_batch* batch, const double* u, const double* v, double scale,
+                            void* stream);
+int afsi_chb_correct_f64(const afsi_chb_batch* batch, double* u, double* v, double coef, void* stream);
+int afsi_chb_cycles_f64(const afsi_chb_batch* batch, const int32_t* active, int32_t cycles, void* stream);
+int afsi_chb_stats_f64(const afsi_chb_batch* batch, int which, double* out, void* stream);
+
 /* ---- explicit MAC predictor of the closed box ------------------------------
  * u* = zero_normal(u + dt (-N(u) + (mu/rho) L u + f/rho)) with u =
  * zero_normal(u0); replaces MACFlow._predict (mac/flow.py:59-64) with the
