@@ -41,23 +41,4 @@ int bin_batch(const BinDesc* desc_device, int count, int64_t total_points, int64
               uint32_t* keys_sorted, int32_t* perm, int32_t* cell_ptr, void* sort_workspace,
               hipStream_t s);
 
-// Gather-form spread of many instances (gather_spread.hip): sizes of the
-// concatenated tile space, the scratch slots bound and the largest box and
-// LDS footprint; host descriptors (gs_desc_bytes() each) to upload once; the
-// launch sequence over one work list.
-struct GsBatchItem {
-    const double* x;
-    const double* force;
-    double* face[3];
-    afsi_grid3 g;
-    int64_t points;
-};
-size_t gs_desc_bytes();
-int gs_batch_sizes(const GsBatchItem* items, int n, const int32_t tile[3], int64_t* tiles, int64_t* slots,
-                   int64_t* btotal, int* lds_bytes);
-size_t gs_batch_workspace_bytes(int64_t tiles, int64_t slots, int64_t btotal);
-void gs_batch_fill(const GsBatchItem* items, int n, const int32_t tile[3], void* host_descs);
-int gs_batch_spread(const void* device_descs, int n, int64_t tiles, int64_t slots, int64_t btotal, int lds_bytes,
-                    const int32_t* perm, const int32_t* cell_ptr, void* workspace, int* flags, hipStream_t s);
-
 }  // namespace afsi

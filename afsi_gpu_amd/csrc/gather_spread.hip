@@ -22,6 +22,13 @@
 // into chunks of the halo point list; each chunk writes its box to a scratch
 // slot and a second kernel adds the slots of the tile in chunk order.
 //
+// Measured at 128^3 with 1.5M clustered points this runs 1.62 ms against
+// 0.28 ms for the point-binned LDS-atomic spread of tiled.hip: a work item
+// walks its halo in 128-point batches, each a chain of barrier-separated,
+// latency-bound phases, and every point is staged by about 3.4 tiles. It is
+// kept as the variant that writes every face exactly once with no atomics
+// in HBM.
+//
 // Weights and bases follow the reference's compact stencil
 // (mac/compact_transfer.py:21-26): s = (x - o)/h, base = floor(s - 1), the
 // four Peskin weights of f = s - base - 1 from the shared radical.
@@ -375,28 +382,14 @@ __device__ inline GsWork gs_work(int w, const int32_t* __restrict__ chunk_start,
     return GsWork{lo, w - chunk_start[lo], chunk_start[lo + 1] - chunk_start[lo]};
 }
 
-// Descriptor sources: one instance passed by value, or a device array of
-// instances found by their first global tile.
+// Descriptor source: one instance passed by value (the kernels take the
+// source as a template parameter, so a device array of instances can serve
+// a batch the same way).
 struct GsOne {
     GsDesc d;
     __device__ const GsDesc& at(int gt, int& lt) const {
         lt = gt;
         return d;
-    }
-};
-
-struct GsMany {
-    const GsDesc* d;
-    int n;
-    __device__ const GsDesc& at(int gt, int& lt) const {
-        int lo = 0, hi = n;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (d[mid].tile_begin <= gt) lo = mid;
-            else hi = mid;
-        }
-        lt = gt - d[lo].tile_begin;
-        return d[lo];
     }
 };
 
@@ -729,64 +722,6 @@ static int gs_run(const Src& src, int64_t tiles, int lds_bytes, const int32_t* p
     return OK;
 }
 
-// Batched spread: descriptors of every instance (device array written by
-// gs_batch_fill) and one work list over all their tiles.
-size_t gs_desc_bytes() { return sizeof(GsDesc); }
-
-int gs_batch_sizes(const GsBatchItem* items, int n, const int32_t tile[3], int64_t* tiles, int64_t* slots,
-                   int64_t* btotal, int* lds_bytes) {
-    int64_t t = 0, pts = 0, m = 1, bt = 0;
-    int lds = 0;
-    for (int i = 0; i < n; ++i) {
-        if (int r = gs_check(items[i].g, tile)) return r;
-        const GsGeom s = make_gs_geom(items[i].g, tile);
-        t += s.ntiles;
-        pts += items[i].points;
-        const int64_t mi = gs_multiplicity(items[i].g, tile);
-        m = mi > m ? mi : m;
-        bt = s.btotal > bt ? s.btotal : bt;
-        const int l = gs_lds(s).total;
-        lds = l > lds ? l : lds;
-    }
-    *tiles = t;
-    *slots = gs_slot_bound(pts, m);
-    *btotal = bt;
-    *lds_bytes = lds;
-    return OK;
-}
-
-size_t gs_batch_workspace_bytes(int64_t tiles, int64_t slots, int64_t btotal) {
-    return gs_ws_layout(tiles, slots, btotal).total;
-}
-
-void gs_batch_fill(const GsBatchItem* items, int n, const int32_t tile[3], void* host_descs) {
-    GsDesc* out = static_cast<GsDesc*>(host_descs);
-    int64_t pbeg = 0, kbeg = 0, tbeg = 0;
-    for (int i = 0; i < n; ++i) {
-        GsDesc d{};
-        d.x = items[i].x;
-        d.force = items[i].force;
-        for (int c = 0; c < 3; ++c) d.face[c] = items[i].face[c];
-        d.g = items[i].g;
-        d.gg = make_gs_geom(items[i].g, tile);
-        d.volume = (d.g.h[0] * d.g.h[1]) * d.g.h[2];
-        d.point_begin = int32_t(pbeg);
-        d.key_offset = int32_t(kbeg);
-        d.tile_begin = int32_t(tbeg);
-        out[i] = d;
-        pbeg += items[i].points;
-        kbeg += int64_t(d.gg.ntiles) * d.gg.tvol;
-        tbeg += d.gg.ntiles;
-    }
-}
-
-int gs_batch_spread(const void* device_descs, int n, int64_t tiles, int64_t slots, int64_t btotal, int lds_bytes,
-                    const int32_t* perm, const int32_t* cell_ptr, void* workspace, int* flags, hipStream_t s) {
-    const GsMany src{static_cast<const GsDesc*>(device_descs), n};
-    return gs_run(src, tiles, lds_bytes, perm, cell_ptr, workspace, gs_ws_layout(tiles, slots, btotal), slots,
-                  flags, s);
-}
-
 }  // namespace afsi
 
 using namespace afsi;
@@ -799,10 +734,10 @@ size_t afsi_spread_gather_workspace_bytes(const afsi_grid3* grid, const int32_t 
     return gs_ws_layout(s.ntiles, gs_slot_bound(points, gs_multiplicity(*grid, tile)), s.btotal).total;
 }
 
-int afsi_spread_tiled_f64(const double* x, const double* force, int64_t points, const afsi_grid3* grid,
-                          const int32_t tile[3], const int32_t* perm, const int32_t* cell_ptr, double* u,
-                          double* v, double* w, void* workspace, size_t workspace_bytes, int* flags,
-                          void* stream) {
+int afsi_spread_gather_f64(const double* x, const double* force, int64_t points, const afsi_grid3* grid,
+                           const int32_t tile[3], const int32_t* perm, const int32_t* cell_ptr, double* u,
+                           double* v, double* w, void* workspace, size_t workspace_bytes, int* flags,
+                           void* stream) {
     AFSI_REQUIRE(grid && tile && points >= 0 && points < (int64_t(1) << 31), "invalid spread arguments");
     AFSI_REQUIRE(u && v && w, "null face array");
     const afsi_grid3& g = *grid;

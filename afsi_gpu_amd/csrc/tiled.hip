@@ -1,5 +1,16 @@
-// Tiled Peskin-4 interpolation over the binned point CSR (the spread is the
-// per-face-tile gather of gather_spread.hip).
+// Tiled Peskin-4 spread and interpolation over the binned point CSR (the
+// per-face-tile gather spread, an alternative without atomics in HBM, is in
+// gather_spread.hip).
+//
+// Spread: the work list is (tile, chunk of <= SPREAD_CHUNK points); a
+// persistent grid walks it. A work item accumulates the contributions of its
+// points for all three components into an FP64 LDS box covering every face
+// the tile's points can touch (tile + 2 cells on each side; one less on the
+// component's own axis), then adds each nonzero LDS entry to HBM with one
+// FP64 global atomic. Dense tiles therefore collapse many point
+// contributions into one global update per face, and a point is read and
+// its weights evaluated exactly once. Time is set by the LDS atomics
+// (192 per point at about 2.4 lanes/clock/CU for scattered addresses).
 //
 // Interp: work items of <= INTERP_CHUNK points; the tile's face box is
 // staged in LDS and each binned point gathers its 64 links per component
@@ -17,14 +28,17 @@ namespace afsi {
 template <class T>
 int launch_fill(T* x, int64_t count, T value, hipStream_t s);
 
-constexpr int INTERP_BATCH_THREADS = 256;
+constexpr int SPREAD_THREADS = 256;
 // Interpolation workgroups: the staged face box is the same for any thread
 // count, so 512 threads double the wavefronts per CU that the LDS allows
 // (VGPRs then set the occupancy).
 constexpr int INTERP_THREADS = 512;
 
-// Points per interpolation work item (each item stages its tile's whole
-// face box, so items are long).
+// Points per work item: the spread's items are short so that items of
+// dense tiles spread over more workgroups (its time is set by LDS atomic
+// throughput, and small items balance it); interpolation stages the whole
+// face box per item, so its items are longer.
+constexpr int SPREAD_CHUNK = 512;
 constexpr int INTERP_CHUNK = 2048;
 
 struct TileSpec {
@@ -265,6 +279,102 @@ __device__ inline void frame_shift(const PointWeights& pw, const afsi_grid3& g, 
     }
 }
 
+// One spread work item: accumulate its points into the FP64 LDS box with
+// FP64 LDS atomics, then one global atomic per nonzero entry. perm holds
+// point ids offset by `base`. (FP32 LDS atomics are no alternative on
+// gfx950: ds_add_f32 sustains about 1/20 of ds_add_f64's lane rate.)
+template <int T>
+__device__ __forceinline__ void spread_item(const WorkItem& it, const double* __restrict__ x,
+                                            const double* __restrict__ force,
+                                            const int32_t* __restrict__ perm, int base,
+                                            const afsi_grid3& g, const TileSpec& ts, double volume,
+                                            double* const outs[3], int* flags, double* lds) {
+    const int nlds = T > 0 ? 3 * (T + 3) * (T + 4) * (T + 4) : ts.lds;
+    for (int e = threadIdx.x; e < nlds; e += SPREAD_THREADS) lds[e] = 0.0;
+    __syncthreads();
+    // Each thread takes a contiguous slice of the binned points. Lanes in a
+    // dense cell share a stencil base, so each lane walks its (j, k) links
+    // from a lane-dependent rotation: at any instant the lanes of one cell
+    // hit 16 different LDS addresses instead of one.
+    const int rj = (threadIdx.x >> 2) & 3, rk = threadIdx.x & 3;
+    const int n = it.pend - it.pbeg;
+    const int per = (n + SPREAD_THREADS - 1) / SPREAD_THREADS;
+    const int j0 = it.pbeg + int(threadIdx.x) * per;
+    const int j1 = min(it.pend, j0 + per);
+    for (int j = j0; j < j1; ++j) {
+        const int idx = perm[j] - base;
+        const double xp[3] = {x[3 * idx], x[3 * idx + 1], x[3 * idx + 2]};
+        PointWeights pw;
+        point_weights(xp, g, pw);
+        int shift[3];
+        frame_shift(pw, g, shift);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double fv = force[3 * idx + c] / volume;
+            int lb[3];
+            if (!box_base<T>(pw, shift, c, it.t0, ts, lb)) {
+                raise_flag(flags, F_INDEX);
+                continue;
+            }
+            const double* wx = (c == 0) ? pw.wf[0] : pw.wc[0];
+            const double* wy = (c == 1) ? pw.wf[1] : pw.wc[1];
+            const double* wz = (c == 2) ? pw.wf[2] : pw.wc[2];
+            const int d1 = box_dim<T>(ts, c, 1), d2 = box_dim<T>(ts, c, 2);
+            const int b0 = box_off<T>(ts, c) + (lb[0] * d1 + lb[1]) * d2 + lb[2];
+            double wyr[4], wzr[4];
+            int oy[4], oz[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int jq = (q + rj) & 3, kq = (q + rk) & 3;
+                wyr[q] = jq == 0 ? wy[0] : (jq == 1 ? wy[1] : (jq == 2 ? wy[2] : wy[3]));
+                wzr[q] = kq == 0 ? wz[0] : (kq == 1 ? wz[1] : (kq == 2 ? wz[2] : wz[3]));
+                oy[q] = jq * d2;
+                oz[q] = kq;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    const double wxy = wx[i] * wyr[jj];
+                    const int row = b0 + i * d1 * d2 + oy[jj];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        atomicAdd(lds + row + oz[k], (wxy * wzr[k]) * fv);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // Flush: one global atomic per nonzero LDS entry.
+    for (int e = threadIdx.x; e < nlds; e += SPREAD_THREADS) {
+        const double v = lds[e];
+        if (v == 0.0) continue;
+        const int c = e < box_off<T>(ts, 1) ? 0 : (e < box_off<T>(ts, 2) ? 1 : 2);
+        const int f = box_face<T>(e - box_off<T>(ts, c), c, it.t0, ts, g);
+        if (f < 0) {
+            raise_flag(flags, F_INDEX);
+            continue;
+        }
+        unsafeAtomicAdd(outs[c] + f, v);
+    }
+    __syncthreads();
+}
+
+template <int T>
+__global__ void __launch_bounds__(SPREAD_THREADS) spread_tiled_kernel(
+    const double* __restrict__ x, const double* __restrict__ force,
+    const int32_t* __restrict__ perm, const int32_t* __restrict__ cell_ptr,
+    const int32_t* __restrict__ chunk_start, afsi_grid3 g, TileSpec ts, double volume,
+    double* __restrict__ out0, double* __restrict__ out1, double* __restrict__ out2, int* flags) {
+    extern __shared__ double lds[];
+    const int total = chunk_start[ts.ntiles];
+    double* const outs[3] = {out0, out1, out2};
+    for (int w = blockIdx.x; w < total; w += gridDim.x)
+        spread_item<T>(work_item(w, chunk_start, cell_ptr, ts, SPREAD_CHUNK), x, force, perm, 0, g, ts, volume,
+                       outs, flags, lds);
+}
+
 // Interpolation over the same work list: the tile's face box of all three
 // components is staged in LDS once per work item, then each point gathers
 // its 3 x 64 values from LDS (consecutive threads take consecutive binned
@@ -402,6 +512,14 @@ struct XferDesc {
     int begin;        // first global tile
 };
 
+struct FillDesc {
+    double* p;
+    int n;
+    int begin;  // first workgroup
+};
+
+constexpr int FILL_CHUNK = 4096;
+
 template <class D>
 __device__ inline int find_begin(const D* __restrict__ desc, int n, int b) {
     int lo = 0, hi = n;
@@ -411,6 +529,13 @@ __device__ inline int find_begin(const D* __restrict__ desc, int n, int b) {
         else hi = mid;
     }
     return lo;
+}
+
+__global__ void __launch_bounds__(256) fill_batch_kernel(const FillDesc* __restrict__ desc, int n) {
+    const int b = blockIdx.x;
+    const FillDesc& d = desc[find_begin(desc, n, b)];
+    const int lo = (b - d.begin) * FILL_CHUNK, hi = min(d.n, lo + FILL_CHUNK);
+    for (int j = lo + int(threadIdx.x); j < hi; j += 256) d.p[j] = 0.0;
 }
 
 __device__ inline int tile_points(const XferDesc& d, const int32_t* __restrict__ cell_ptr, int t,
@@ -476,7 +601,21 @@ __device__ inline WorkItem batch_item(int w, const XferDesc* __restrict__ desc, 
 }
 
 template <int T>
-__global__ void __launch_bounds__(INTERP_BATCH_THREADS) interp_batch_kernel(
+__global__ void __launch_bounds__(SPREAD_THREADS) spread_batch_kernel(
+    const XferDesc* __restrict__ desc, int n, int tiles, const int32_t* __restrict__ perm,
+    const int32_t* __restrict__ cell_ptr, const int32_t* __restrict__ chunk_start, int* flags) {
+    extern __shared__ double lds[];
+    const int total = chunk_start[tiles];
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        int i;
+        const WorkItem it = batch_item(w, desc, n, tiles, cell_ptr, chunk_start, SPREAD_CHUNK, i);
+        const XferDesc& d = desc[i];
+        spread_item<T>(it, d.x, d.force, perm, d.point_begin, d.g, d.ts, d.volume, d.face, flags, lds);
+    }
+}
+
+template <int T>
+__global__ void __launch_bounds__(SPREAD_THREADS) interp_batch_kernel(
     const XferDesc* __restrict__ desc, int n, int tiles, const int32_t* __restrict__ perm,
     const int32_t* __restrict__ cell_ptr, const int32_t* __restrict__ chunk_start, int* flags) {
     extern __shared__ double lds[];
@@ -486,7 +625,7 @@ __global__ void __launch_bounds__(INTERP_BATCH_THREADS) interp_batch_kernel(
         const WorkItem it = batch_item(w, desc, n, tiles, cell_ptr, chunk_start, INTERP_CHUNK, i);
         const XferDesc& d = desc[i];
         const double* const us[3] = {d.face[0], d.face[1], d.face[2]};
-        interp_item<T, INTERP_BATCH_THREADS>(it, d.x, perm, d.point_begin, d.g, d.ts, us, d.out, flags, lds);
+        interp_item<T, SPREAD_THREADS>(it, d.x, perm, d.point_begin, d.g, d.ts, us, d.out, flags, lds);
     }
 }
 
@@ -514,22 +653,10 @@ static int check_tiled(const afsi_grid3& g, const int32_t tile[3]) {
 struct XferLayout {
     int n = 0;
     int64_t points = 0, keys = 0;
-    int tiles = 0;
-    size_t bin_off = 0, xfer_off = 0, gs_desc_off = 0, chunk_off = 0, gs_off = 0, sort_off = 0, total = 0;
+    int tiles = 0, fill_blocks = 0;
+    size_t bin_off = 0, xfer_off = 0, fill_off = 0, chunk_off = 0, sort_off = 0, total = 0;
     size_t lds = 0;
-    int64_t gs_tiles = 0, gs_slots = 0, gs_btotal = 0;
-    int gs_lds = 0;
 };
-
-static GsBatchItem gs_item_of(const afsi_tb_item& it) {
-    GsBatchItem b{};
-    b.x = it.x;
-    b.force = it.force;
-    for (int c = 0; c < 3; ++c) b.face[c] = it.face[c];
-    b.g = it.grid;
-    b.points = it.points;
-    return b;
-}
 
 static size_t alignb(size_t b) { return (b + 255) & ~size_t(255); }
 
@@ -548,20 +675,21 @@ static int xfer_layout(const afsi_tb_batch* bt, XferLayout& L) {
         L.points += it.points;
         L.keys += bin_key_space(it.grid, bt->tile);
         L.tiles += int(ts.ntiles);
+        for (int c = 0; c < 3; ++c) {
+            const int64_t nf = (it.grid.n[0] + (!it.grid.periodic && c == 0)) *
+                               (it.grid.n[1] + (!it.grid.periodic && c == 1)) *
+                               (it.grid.n[2] + (!it.grid.periodic && c == 2));
+            L.fill_blocks += int(cdiv(nf, FILL_CHUNK));
+        }
     }
     AFSI_REQUIRE(L.points < (int64_t(1) << 31) && L.keys < (int64_t(1) << 31), "batch too large");
     L.lds = size_t(ts0.lds) * sizeof(double);
     AFSI_REQUIRE(L.lds <= 64 * 1024, "tile too large for LDS");
-    std::vector<GsBatchItem> gi(L.n);
-    for (int i = 0; i < L.n; ++i) gi[i] = gs_item_of(bt->items[i]);
-    if (int r = gs_batch_sizes(gi.data(), L.n, bt->tile, &L.gs_tiles, &L.gs_slots, &L.gs_btotal, &L.gs_lds))
-        return r;
     size_t off = 0;
     L.bin_off = off; off = alignb(off + size_t(L.n) * sizeof(BinDesc));
     L.xfer_off = off; off = alignb(off + size_t(L.n) * sizeof(XferDesc));
-    L.gs_desc_off = off; off = alignb(off + size_t(L.n) * gs_desc_bytes());
+    L.fill_off = off; off = alignb(off + size_t(3 * L.n) * sizeof(FillDesc));
     L.chunk_off = off; off = alignb(off + size_t(L.tiles + 1) * sizeof(int32_t));
-    L.gs_off = off; off = alignb(off + gs_batch_workspace_bytes(L.gs_tiles, L.gs_slots, L.gs_btotal));
     L.sort_off = off; off = alignb(off + bin_workspace_bytes(L.points > 0 ? L.points : 1));
     L.total = off;
     return OK;
@@ -612,6 +740,48 @@ size_t afsi_spread_workspace_bytes(const afsi_grid3* grid, const int32_t tile[3]
     if (!grid || !tile || tile[0] <= 0 || tile[1] <= 0 || tile[2] <= 0) return 0;
     const TileSpec ts = make_spec(*grid, tile);
     return align256(size_t(ts.ntiles + 1) * sizeof(int32_t));
+}
+
+int afsi_spread_tiled_f64(const double* x, const double* force, int64_t points,
+                          const afsi_grid3* grid, const int32_t tile[3], const int32_t* perm,
+                          const int32_t* cell_ptr, double* u, double* v, double* w,
+                          void* workspace, size_t workspace_bytes, int* flags, void* stream) {
+    AFSI_REQUIRE(grid && tile && points >= 0 && points < (int64_t(1) << 31), "invalid spread arguments");
+    AFSI_REQUIRE(u && v && w, "null face array");
+    const afsi_grid3& g = *grid;
+    if (int r = check_tiled(g, tile)) return r;
+    const TileSpec ts = make_spec(g, tile);
+    AFSI_REQUIRE(size_t(ts.lds) * sizeof(double) <= 64 * 1024, "tile too large for LDS");
+    const hipStream_t s = hipStream_t(stream);
+    const int64_t nf[3][3] = {
+        {g.n[0] + !g.periodic, g.n[1], g.n[2]},
+        {g.n[0], g.n[1] + !g.periodic, g.n[2]},
+        {g.n[0], g.n[1], g.n[2] + !g.periodic}};
+    double* outs[3] = {u, v, w};
+    for (int c = 0; c < 3; ++c)
+        if (int r = launch_fill<double>(outs[c], nf[c][0] * nf[c][1] * nf[c][2], 0.0, s)) return r;
+    if (points == 0) return OK;
+    AFSI_REQUIRE(x && force && perm && cell_ptr && workspace, "null pointer");
+    AFSI_REQUIRE(workspace_bytes >= afsi_spread_workspace_bytes(grid, tile),
+                 "spread workspace too small");
+    int32_t* chunk_start = static_cast<int32_t*>(workspace);
+    chunk_scan_kernel<<<1, 1024, 0, s>>>(cell_ptr, ts.ntiles, ts.tvol, SPREAD_CHUNK, chunk_start);
+    AFSI_LAUNCH_CHECK();
+    const double volume = (g.h[0] * g.h[1]) * g.h[2];
+    const int64_t upper = ts.ntiles + points / SPREAD_CHUNK + 1;
+    const unsigned blocks = unsigned(upper < 4096 ? upper : 4096);
+    const size_t shmem = size_t(ts.lds) * sizeof(double);
+#define AFSI_SPREAD(E)                                                                          \
+    spread_tiled_kernel<E><<<blocks, SPREAD_THREADS, shmem, s>>>(x, force, perm, cell_ptr, chunk_start, \
+                                                                g, ts, volume, u, v, w, flags)
+    switch (cubic_edge(tile)) {
+    case 8: AFSI_SPREAD(8); break;
+    case 4: AFSI_SPREAD(4); break;
+    default: AFSI_SPREAD(0);
+    }
+#undef AFSI_SPREAD
+    AFSI_LAUNCH_CHECK();
+    return OK;
 }
 
 int afsi_interp_tiled_f64(const double* x, int64_t points, const afsi_grid3* grid,
@@ -676,10 +846,7 @@ int afsi_tb_prepare(const afsi_tb_batch* batch, void* stream) {
     std::vector<char> host(L.sort_off, 0);
     void* hw = host.data();
     int64_t pbeg = 0, kbeg = 0;
-    int tbeg = 0;
-    std::vector<GsBatchItem> gi(L.n);
-    for (int i = 0; i < L.n; ++i) gi[i] = gs_item_of(batch->items[i]);
-    gs_batch_fill(gi.data(), L.n, batch->tile, wsat<char>(hw, L.gs_desc_off));
+    int tbeg = 0, fbeg = 0;
     for (int i = 0; i < L.n; ++i) {
         const afsi_tb_item& it = batch->items[i];
         const TileSpec ts = make_spec(it.grid, batch->tile);
@@ -704,6 +871,13 @@ int afsi_tb_prepare(const afsi_tb_batch* batch, void* stream) {
         x.key_offset = int(kbeg);
         x.begin = tbeg;
         wsat<XferDesc>(hw, L.xfer_off)[i] = x;
+        for (int c = 0; c < 3; ++c) {
+            const int64_t nf = (it.grid.n[0] + (!it.grid.periodic && c == 0)) *
+                               (it.grid.n[1] + (!it.grid.periodic && c == 1)) *
+                               (it.grid.n[2] + (!it.grid.periodic && c == 2));
+            wsat<FillDesc>(hw, L.fill_off)[3 * i + c] = FillDesc{it.face[c], int(nf), fbeg};
+            fbeg += int(cdiv(nf, FILL_CHUNK));
+        }
         pbeg += it.points;
         kbeg += bin_key_space(it.grid, batch->tile);
         tbeg += int(ts.ntiles);
@@ -722,39 +896,44 @@ int afsi_tb_bin(const afsi_tb_batch* batch, void* stream) {
                      hipStream_t(stream));
 }
 
-int afsi_tb_spread(const afsi_tb_batch* batch, void* stream) {
+static int tb_transfer(const afsi_tb_batch* batch, bool spread, void* stream) {
     XferLayout L;
     if (int r = xfer_ready(batch, L)) return r;
-    return gs_batch_spread(wsat<char>(batch->workspace, L.gs_desc_off), L.n, L.gs_tiles, L.gs_slots, L.gs_btotal,
-                           L.gs_lds, batch->perm, batch->cell_ptr, wsat<char>(batch->workspace, L.gs_off),
-                           batch->flags, hipStream_t(stream));
-}
-
-int afsi_tb_interp(const afsi_tb_batch* batch, void* stream) {
-    XferLayout L;
-    if (int r = xfer_ready(batch, L)) return r;
-    if (L.points == 0) return OK;
     const hipStream_t s = hipStream_t(stream);
     const XferDesc* desc = wsat<XferDesc>(batch->workspace, L.xfer_off);
     int32_t* chunk_start = wsat<int32_t>(batch->workspace, L.chunk_off);
-    chunk_scan_batch_kernel<<<1, 1024, 0, s>>>(desc, L.n, L.tiles, batch->cell_ptr, INTERP_CHUNK, chunk_start);
-    AFSI_LAUNCH_CHECK();
-    const int64_t upper = L.tiles + L.points / INTERP_CHUNK + 1;
-    const unsigned blocks = unsigned(upper < 4096 ? upper : 4096);
-    switch (cubic_edge(batch->tile)) {
-    case 8:
-        interp_batch_kernel<8><<<blocks, INTERP_BATCH_THREADS, L.lds, s>>>(desc, L.n, L.tiles, batch->perm,
-                                                                          batch->cell_ptr, chunk_start, batch->flags);
-        break;
-    case 4:
-        interp_batch_kernel<4><<<blocks, INTERP_BATCH_THREADS, L.lds, s>>>(desc, L.n, L.tiles, batch->perm,
-                                                                          batch->cell_ptr, chunk_start, batch->flags);
-        break;
-    default:
-        interp_batch_kernel<0><<<blocks, INTERP_BATCH_THREADS, L.lds, s>>>(desc, L.n, L.tiles, batch->perm,
-                                                                          batch->cell_ptr, chunk_start, batch->flags);
+    if (spread) {
+        fill_batch_kernel<<<unsigned(L.fill_blocks), 256, 0, s>>>(wsat<FillDesc>(batch->workspace, L.fill_off),
+                                                                 3 * L.n);
+        AFSI_LAUNCH_CHECK();
     }
+    if (L.points == 0) return OK;
+    const int chunk = spread ? SPREAD_CHUNK : INTERP_CHUNK;
+    chunk_scan_batch_kernel<<<1, 1024, 0, s>>>(desc, L.n, L.tiles, batch->cell_ptr, chunk, chunk_start);
+    AFSI_LAUNCH_CHECK();
+    const int64_t upper = L.tiles + L.points / chunk + 1;
+    const unsigned blocks = unsigned(upper < 4096 ? upper : 4096);
+    const int edge = cubic_edge(batch->tile);
+#define AFSI_TB_ARGS(...) __VA_ARGS__
+#define AFSI_TB_LAUNCH(KERNEL, E)                                                                      \
+    KERNEL<AFSI_TB_ARGS E><<<blocks, SPREAD_THREADS, L.lds, s>>>(desc, L.n, L.tiles, batch->perm,       \
+                                                                 batch->cell_ptr, chunk_start, batch->flags)
+    if (spread) {
+        if (edge == 8) AFSI_TB_LAUNCH(spread_batch_kernel, (8));
+        else if (edge == 4) AFSI_TB_LAUNCH(spread_batch_kernel, (4));
+        else AFSI_TB_LAUNCH(spread_batch_kernel, (0));
+    } else {
+        if (edge == 8) AFSI_TB_LAUNCH(interp_batch_kernel, (8));
+        else if (edge == 4) AFSI_TB_LAUNCH(interp_batch_kernel, (4));
+        else AFSI_TB_LAUNCH(interp_batch_kernel, (0));
+    }
+#undef AFSI_TB_LAUNCH
+#undef AFSI_TB_ARGS
     AFSI_LAUNCH_CHECK();
     return OK;
 }
+
+int afsi_tb_spread(const afsi_tb_batch* batch, void* stream) { return tb_transfer(batch, true, stream); }
+int afsi_tb_interp(const afsi_tb_batch* batch, void* stream) { return tb_transfer(batch, false, stream); }
+
 }  // extern "C"

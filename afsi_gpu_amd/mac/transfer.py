@@ -6,9 +6,11 @@
   ``cuda_ib.compact_gather`` / ``compact_spread``.
 * ``bin_points``: build-own stable CSR of points by tile-major cell key.
 * ``TiledTransfer``: spread / interpolate all three components straight
-  from positions over the CSR, the fast path the IB step uses. The spread
-  is a per-face-tile gather: every face is produced by the one workgroup
-  that owns its tile and written once (no atomics in HBM).
+  from positions over the CSR (weights in registers, per-tile LDS
+  accumulation), the fast path the IB step uses. ``spread='gather'``
+  selects the per-face-tile gather spread instead (every face produced by
+  the one workgroup that owns its tile and written once, no atomics in HBM;
+  slower on clustered workloads).
 """
 from dataclasses import dataclass
 import torch
@@ -126,10 +128,14 @@ def bin_points(points, grid, tile=DEFAULT_TILE):
 class TiledTransfer:
     """Spread / interpolate over binned points for a fixed grid and tile."""
 
-    def __init__(self, grid, device, tile=DEFAULT_TILE, flags=None):
+    def __init__(self, grid, device, tile=DEFAULT_TILE, flags=None, spread='atomic'):
         """``flags``: an int32 device word to OR the kernels' error bits
         into (e.g. the binner's, which it clears at every binning); by
-        default a word of its own."""
+        default a word of its own. ``spread``: 'atomic' (point-binned LDS
+        accumulation, the fast path) or 'gather' (per-face-tile gather)."""
+        if spread not in ('atomic', 'gather'):
+            raise ValueError("spread must be 'atomic' or 'gather'")
+        self.method = spread
         self.grid, self.tile = grid, tuple(int(t) for t in tile)
         self._g = grid.abi()
         self._tile = _abi.i32x3(self.tile)
@@ -156,8 +162,11 @@ class TiledTransfer:
         overwritten)."""
         if out is None:
             out = self.grid.zeros(device=points.device, dtype=torch.float64)
-        ws = self._spread_workspace(points.shape[0])
-        _abi.call('afsi_spread_tiled_f64', _abi.ptr(points), _abi.ptr(force), points.shape[0],
+        if self.method == 'atomic':
+            name, ws = 'afsi_spread_tiled_f64', self.workspace
+        else:
+            name, ws = 'afsi_spread_gather_f64', self._spread_workspace(points.shape[0])
+        _abi.call(name, _abi.ptr(points), _abi.ptr(force), points.shape[0],
                   self._g, self._tile, _abi.ptr(bins.perm), _abi.ptr(bins.cell_ptr),
                   _abi.ptr(out[0]), _abi.ptr(out[1]), _abi.ptr(out[2]), _abi.ptr(ws),
                   ws.numel(), _abi.ptr(self.flags), _abi.stream())

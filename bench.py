@@ -66,7 +66,7 @@ def parse():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-dropin', action='store_true', help='skip the reference-API drop-in sample')
     ap.add_argument('--kernels-only', type=int, default=0, metavar='R',
-                    help='only launch the roofline sweep and a copy kernel R times each')
+                    help='only launch the roofline sweep, the transfers and a copy kernel R times each')
     ap.add_argument('--serve-seconds', type=float, default=5.,
                     help='arrival window of the served-latency sample (0: skip)')
     ap.add_argument('--batch-instances', type=int, default=64,
@@ -552,12 +552,16 @@ def oracle_accuracy(torch, x_np, F_np, n, dt, states):
     return out
 
 
-def kernels_only(torch, n, repeats):
+def kernels_only(torch, n, repeats, points):
     """For hardware-counter passes: `repeats` launches of the level-0 FP32
-    red-black sweep at n^3 (the roofline kernel), then `repeats` launches of
-    a plain copy kernel of known size (out = 1.0 * src over n^3 float64:
-    8 n^3 bytes read, 8 n^3 bytes written), which scales the counters."""
+    red-black sweep at n^3 (the roofline kernel), `repeats` binnings, spreads
+    and interpolations of the clustered-shells workload (`points` points),
+    then `repeats` launches of a plain copy kernel of known size (out = 1.0 *
+    src over n^3 float64: 8 n^3 bytes read, 8 n^3 bytes written), which
+    scales the counters."""
+    from afsi_gpu_amd import workloads
     from afsi_gpu_amd.mac import MACGrid, MGOptions, MixedMultigrid
+    from afsi_gpu_amd.mac.transfer import Binner, TiledTransfer
     grid = MACGrid((n, n, n), (1., 1., 1.), periodic=True)
     solver = MixedMultigrid(grid, device='cuda:0', options=MGOptions(smooth=2, smoother='rbgs'),
                             backend='workspace').inner
@@ -565,15 +569,26 @@ def kernels_only(torch, n, repeats):
     solver.p[0].copy_(torch.randn(solver.shapes[0], device='cuda:0', generator=gen))
     solver.rhs[0].copy_(torch.randn(solver.shapes[0], device='cuda:0', generator=gen))
     sweep_s = time_sweep(torch, solver, repeats=repeats, batches=1)
+    x_np, F_np = workloads.clustered_shells_128(points)
+    x = torch.tensor(x_np, device='cuda:0')
+    F = torch.tensor(F_np, device='cuda:0')
+    binner = Binner(grid, len(x_np), x.device)
+    transfer = TiledTransfer(grid, x.device, flags=binner.flags)
+    for _ in range(repeats):
+        bins = binner(x)
+        f = transfer.spread(x, F, bins)
+        transfer.interpolate(x, f, bins)
     src = torch.randn(n ** 3, device='cuda:0', dtype=torch.float64, generator=gen)
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
     for _ in range(repeats):
         torch.mul(src, 1.0, out=dst)
     torch.cuda.synchronize()
+    P, N = len(x_np), n ** 3
     print(json.dumps(dict(kernels_only=True, n=n, repeats=repeats, sweep_us=sweep_s * 1e6,
-                          sweep_algorithmic_bytes=12 * n ** 3, copy_bytes_read=8 * n ** 3,
-                          copy_bytes_written=8 * n ** 3)))
+                          sweep_algorithmic_bytes=12 * N, spread_algorithmic_bytes=48 * P + 24 * N,
+                          interp_algorithmic_bytes=48 * P + 24 * N, copy_bytes_read=8 * N,
+                          copy_bytes_written=8 * N)))
 
 
 def main():
@@ -583,7 +598,7 @@ def main():
         sys.exit(2)
     import torch
     if args.kernels_only:
-        kernels_only(torch, args.n, args.kernels_only)
+        kernels_only(torch, args.n, args.kernels_only, args.points)
         return
     if args.only:
         run = dict(p2=lambda: p2_sample(torch), fe=lambda: fe_sample(torch, args.n, args.fe_level),

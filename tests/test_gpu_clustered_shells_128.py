@@ -70,11 +70,12 @@ def test_binning_bit_exact(gpu, ref):
     assert np.array_equal(b.cell_ptr.cpu().numpy().astype(np.int64), ref['ptr'])
 
 
-def test_spread_and_interpolation(gpu, ref):
+@pytest.mark.parametrize('method', ['atomic', 'gather'])
+def test_spread_and_interpolation(gpu, ref, method):
     g = grid()
     x, F = torch.tensor(ref['x'], device=gpu), torch.tensor(ref['F'], device=gpu)
     b = Binner(g, len(ref['x']), gpu, (8, 8, 8))(x)
-    tr = TiledTransfer(g, gpu, (8, 8, 8))
+    tr = TiledTransfer(g, gpu, (8, 8, 8), spread=method)
     f = tr.spread(x, F, b)
     U = tr.interpolate(x, f, b)
     assert int(tr.flags.item()) == 0

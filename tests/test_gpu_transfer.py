@@ -150,9 +150,10 @@ def test_binning_empty(gpu):
     assert b.cell_ptr.count_nonzero().item() == 0
 
 
+@pytest.mark.parametrize('method', ['atomic', 'gather'])
 @pytest.mark.parametrize('periodic', [False, True])
 @pytest.mark.parametrize('tile', [(8, 8, 8), (4, 4, 4), (5, 3, 7)])
-def test_tiled_spread_and_interp(gpu, periodic, tile):
+def test_tiled_spread_and_interp(gpu, periodic, tile, method):
     g, og = grids(periodic)
     x, F = points_in(og, 20000, 5)
     x[:3000] = x[0] + 1e-3 * np.random.default_rng(6).normal(size=(3000, 3))  # dense cluster
@@ -165,8 +166,8 @@ def test_tiled_spread_and_interp(gpu, periodic, tile):
     U_ref = ib3d.gather_grid(u, base, phi, og)
     xt, Ft = torch.tensor(x, device=gpu), torch.tensor(F, device=gpu)
     bins = bin_points(xt, g, tile)
-    tr = TiledTransfer(g, gpu, tile)
-    # every face is owned by one tile and written once: stale values go
+    tr = TiledTransfer(g, gpu, tile, spread=method)
+    # every face is overwritten: stale values go
     stale = tuple(torch.full(g.face_shape(c), float('nan'), device=gpu, dtype=torch.float64) for c in range(3))
     f = tr.spread(xt, Ft, bins, out=stale)
     ut = tuple(torch.tensor(a, device=gpu) for a in u)
