@@ -149,6 +149,8 @@ _SIGNATURES = {
     'afsi_spread_gather_workspace_bytes': (c_size_t, [POINTER(Grid3), POINTER(c_int32), c_int64]),
     'afsi_tile_box_bounds': (c_int, [POINTER(Grid3), POINTER(c_int32), _I64, _I64]),
     'afsi_spread_tiled_f64': (c_int, [_P, _P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, _P, _P, c_size_t, _P, _P]),
+    'afsi_spread_tiled_noalloc_f64': (c_int, [_P, _P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, _P,
+                                              _P, _P]),
     'afsi_spread_gather_f64': (c_int, [_P, _P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, _P, _P, c_size_t, _P, _P]),
     'afsi_interp_tiled_f64': (c_int, [_P, c_int64, POINTER(Grid3), POINTER(c_int32), _P, _P, _P, _P, _P, _P, _P, c_size_t, _P, _P]),
     'afsi_mg_residual_3d_f64': (c_int, [_P, _P, _P, POINTER(c_int64), POINTER(c_double), c_int, _P]),

@@ -21,6 +21,7 @@
 // shared radical sqrt(1+4f-4f^2); base = floor(s-1) exactly as the
 // reference (mac/compact_transfer.py:21-26) with s=(x-o)/h.
 #include "binshare.hpp"
+#include <mutex>
 #include <vector>
 
 namespace afsi {
@@ -707,6 +708,74 @@ static int xfer_ready(const afsi_tb_batch* bt, XferLayout& L) {
     return OK;
 }
 
+// Fill, work-list scan and spread launch of afsi_spread_tiled_f64;
+// chunk_start holds ntiles + 1 ints of scratch (unused without points).
+static int spread_tiled(const double* x, const double* force, int64_t points, const afsi_grid3& g,
+                        const int32_t tile[3], const int32_t* perm, const int32_t* cell_ptr, double* u,
+                        double* v, double* w, int32_t* chunk_start, int* flags, hipStream_t s) {
+    AFSI_REQUIRE(points >= 0 && points < (int64_t(1) << 31), "invalid spread arguments");
+    AFSI_REQUIRE(u && v && w, "null face array");
+    if (int r = check_tiled(g, tile)) return r;
+    const TileSpec ts = make_spec(g, tile);
+    AFSI_REQUIRE(size_t(ts.lds) * sizeof(double) <= 64 * 1024, "tile too large for LDS");
+    const int64_t nf[3][3] = {
+        {g.n[0] + !g.periodic, g.n[1], g.n[2]},
+        {g.n[0], g.n[1] + !g.periodic, g.n[2]},
+        {g.n[0], g.n[1], g.n[2] + !g.periodic}};
+    double* outs[3] = {u, v, w};
+    for (int c = 0; c < 3; ++c)
+        if (int r = launch_fill<double>(outs[c], nf[c][0] * nf[c][1] * nf[c][2], 0.0, s)) return r;
+    if (points == 0) return OK;
+    AFSI_REQUIRE(x && force && perm && cell_ptr && chunk_start, "null pointer");
+    chunk_scan_kernel<<<1, 1024, 0, s>>>(cell_ptr, ts.ntiles, ts.tvol, SPREAD_CHUNK, chunk_start);
+    AFSI_LAUNCH_CHECK();
+    const double volume = (g.h[0] * g.h[1]) * g.h[2];
+    const int64_t upper = ts.ntiles + points / SPREAD_CHUNK + 1;
+    const unsigned blocks = unsigned(upper < 4096 ? upper : 4096);
+    const size_t shmem = size_t(ts.lds) * sizeof(double);
+#define AFSI_SPREAD(E)                                                                          \
+    spread_tiled_kernel<E><<<blocks, SPREAD_THREADS, shmem, s>>>(x, force, perm, cell_ptr, chunk_start, \
+                                                                g, ts, volume, u, v, w, flags)
+    switch (cubic_edge(tile)) {
+    case 8: AFSI_SPREAD(8); break;
+    case 4: AFSI_SPREAD(4); break;
+    default: AFSI_SPREAD(0);
+    }
+#undef AFSI_SPREAD
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+// ---- the spread without allocation ------------------------------------------
+// For callers that cannot allocate (code under graph capture, serving
+// loops): the spread's only scratch, the per-tile work-list prefix
+// chunk_start, lives in a fixed device array of NOALLOC_TILES + 1 ints
+// (4 MiB). Each GPU holds its own copy from module load until the process
+// ends; its address is looked up once per device and kept. Every call
+// writes all entries it reads before reading them, so nothing needs
+// clearing; calls that overlap on one GPU share the array and race.
+
+constexpr int64_t NOALLOC_TILES = (int64_t(1) << 20) - 1;
+static __device__ __align__(256) int32_t noalloc_chunk_start[NOALLOC_TILES + 1];
+
+static std::mutex noalloc_lock;
+static std::vector<int32_t*> noalloc_address;
+
+static int noalloc_scratch(int32_t** out) {
+    int dev = 0;
+    AFSI_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> hold(noalloc_lock);
+    if (size_t(dev) >= noalloc_address.size()) noalloc_address.resize(size_t(dev) + 1, nullptr);
+    if (!noalloc_address[dev]) {
+        void* p = nullptr;
+        AFSI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_chunk_start)));
+        AFSI_REQUIRE(p != nullptr, "no device address for the spread scratch array");
+        noalloc_address[dev] = static_cast<int32_t*>(p);
+    }
+    *out = noalloc_address[dev];
+    return OK;
+}
+
 }  // namespace afsi
 
 using namespace afsi;
@@ -746,42 +815,28 @@ int afsi_spread_tiled_f64(const double* x, const double* force, int64_t points,
                           const afsi_grid3* grid, const int32_t tile[3], const int32_t* perm,
                           const int32_t* cell_ptr, double* u, double* v, double* w,
                           void* workspace, size_t workspace_bytes, int* flags, void* stream) {
-    AFSI_REQUIRE(grid && tile && points >= 0 && points < (int64_t(1) << 31), "invalid spread arguments");
-    AFSI_REQUIRE(u && v && w, "null face array");
-    const afsi_grid3& g = *grid;
-    if (int r = check_tiled(g, tile)) return r;
-    const TileSpec ts = make_spec(g, tile);
-    AFSI_REQUIRE(size_t(ts.lds) * sizeof(double) <= 64 * 1024, "tile too large for LDS");
-    const hipStream_t s = hipStream_t(stream);
-    const int64_t nf[3][3] = {
-        {g.n[0] + !g.periodic, g.n[1], g.n[2]},
-        {g.n[0], g.n[1] + !g.periodic, g.n[2]},
-        {g.n[0], g.n[1], g.n[2] + !g.periodic}};
-    double* outs[3] = {u, v, w};
-    for (int c = 0; c < 3; ++c)
-        if (int r = launch_fill<double>(outs[c], nf[c][0] * nf[c][1] * nf[c][2], 0.0, s)) return r;
-    if (points == 0) return OK;
-    AFSI_REQUIRE(x && force && perm && cell_ptr && workspace, "null pointer");
-    AFSI_REQUIRE(workspace_bytes >= afsi_spread_workspace_bytes(grid, tile),
+    AFSI_REQUIRE(grid && tile, "invalid spread arguments");
+    AFSI_REQUIRE(points == 0 || workspace_bytes >= afsi_spread_workspace_bytes(grid, tile),
                  "spread workspace too small");
-    int32_t* chunk_start = static_cast<int32_t*>(workspace);
-    chunk_scan_kernel<<<1, 1024, 0, s>>>(cell_ptr, ts.ntiles, ts.tvol, SPREAD_CHUNK, chunk_start);
-    AFSI_LAUNCH_CHECK();
-    const double volume = (g.h[0] * g.h[1]) * g.h[2];
-    const int64_t upper = ts.ntiles + points / SPREAD_CHUNK + 1;
-    const unsigned blocks = unsigned(upper < 4096 ? upper : 4096);
-    const size_t shmem = size_t(ts.lds) * sizeof(double);
-#define AFSI_SPREAD(E)                                                                          \
-    spread_tiled_kernel<E><<<blocks, SPREAD_THREADS, shmem, s>>>(x, force, perm, cell_ptr, chunk_start, \
-                                                                g, ts, volume, u, v, w, flags)
-    switch (cubic_edge(tile)) {
-    case 8: AFSI_SPREAD(8); break;
-    case 4: AFSI_SPREAD(4); break;
-    default: AFSI_SPREAD(0);
-    }
-#undef AFSI_SPREAD
-    AFSI_LAUNCH_CHECK();
-    return OK;
+    return spread_tiled(x, force, points, *grid, tile, perm, cell_ptr, u, v, w,
+                        static_cast<int32_t*>(workspace), flags, hipStream_t(stream));
+}
+
+int afsi_spread_tiled_noalloc_f64(const double* x, const double* force, int64_t points,
+                                  const afsi_grid3* grid, const int32_t tile[3], const int32_t* perm,
+                                  const int32_t* cell_ptr, double* u, double* v, double* w, int* flags,
+                                  void* stream) {
+    AFSI_REQUIRE(grid && tile, "invalid spread arguments");
+    if (int r = check_tiled(*grid, tile)) return r;
+    const TileSpec ts = make_spec(*grid, tile);
+    AFSI_REQUIRE(ts.ntiles <= NOALLOC_TILES,
+                 "grid of %lld tiles exceeds the no-allocation spread's limit of %lld tiles",
+                 (long long)ts.ntiles, (long long)NOALLOC_TILES);
+    int32_t* chunk_start = nullptr;
+    if (points > 0)
+        if (int r = noalloc_scratch(&chunk_start)) return r;
+    return spread_tiled(x, force, points, *grid, tile, perm, cell_ptr, u, v, w, chunk_start, flags,
+                        hipStream_t(stream));
 }
 
 int afsi_interp_tiled_f64(const double* x, int64_t points, const afsi_grid3* grid,

@@ -135,6 +135,17 @@ int afsi_spread_tiled_f64(const double* x, const double* force, int64_t points,
                           const int32_t* perm, const int32_t* cell_ptr,
                           double* u, double* v, double* w, void* workspace,
                           size_t workspace_bytes, int* flags, void* stream);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops): afsi_spread_tiled_f64
+ * with no workspace argument. Its one scratch buffer, the per-tile work-list
+ * prefix, is a fixed device array of 2^20 ints (4 MiB) in the library, one
+ * per GPU, held until the process ends; no hipMalloc/hipFree is called.
+ * Grids of more than 2^20 - 1 tiles are rejected with an error. Calls that
+ * overlap on one GPU share the array and race. */
+int afsi_spread_tiled_noalloc_f64(const double* x, const double* force, int64_t points,
+                                  const afsi_grid3* grid, const int32_t tile[3], const int32_t* perm,
+                                  const int32_t* cell_ptr, double* u, double* v, double* w, int* flags,
+                                  void* stream);
 /* The same spread as a per-face-tile gather (build-own): the workgroup that
  * owns a tile produces all its faces from the points binned in the tile and
  * its two-cell halo (same-cell runs accumulated in registers per x-slot,
