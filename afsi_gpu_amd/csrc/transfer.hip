@@ -9,13 +9,16 @@
 //   12 one-dimensional weights (x rows of y rows of 4 contiguous z values,
 //   fused multiply-adds), so a thread issues 16 row reads of 4 doubles
 //   instead of 64 scattered ones.
-// * Spreads are gathers too, so no floating-point atomics are used and the
-//   result does not depend on scheduling: the points (compact) or links
-//   (indexed) are stably sorted by their target with the binning's radix
-//   passes, and every output entry sums its contributors itself in a fixed
-//   order (compact: the 4x4 columns of stencil bases that reach a face, in
-//   x-then-y order, each column's points in sorted order; indexed: a node's
-//   links in link order).
+// * Compact spread: the points are stably sorted by their stencil base into
+//   8x8x8 base tiles (the binning's radix passes); one work item per (tile,
+//   chunk of <= 512 points) accumulates its points' 64 links in an FP64 LDS
+//   box of the tile's 11x11x11 faces and adds each nonzero entry to HBM
+//   with one atomic, so a face gets one global update per touching work
+//   item instead of one per link (like the reference's CUDA op, the order of
+//   those updates is not fixed).
+// * Indexed spread: a gather, deterministic and free of atomics: the links
+//   are stably sorted by target and every output entry sums its links in
+//   link order.
 // Out-of-range stencils (the reference's device assert,
 // mac/csrc/transfer.cu:19,41) raise F_INDEX: their in-range taps are kept
 // by the gathers, and the spreads drop the point (compact) or link
@@ -129,53 +132,122 @@ __global__ void __launch_bounds__(256) compact_gather_kernel(const T* __restrict
 // ---------------------------------------------------------------------------
 // compact spread: sort points by stencil base, then one thread per face
 
-// Base b (per axis in [0, n-4]) -> key over the nb = n-3 valid positions
-// per axis; a stencil reaching outside the face array gets the sentinel
-// key `space` (sorted last, never visited) and raises F_INDEX.
+constexpr int CP_T = 8;                        // base tile edge
+constexpr int CP_TV = CP_T * CP_T * CP_T;       // bases per tile
+constexpr int CP_BOX = CP_T + 3;                // faces per box axis
+constexpr int CP_CHUNK = 512;                   // points per work item
+constexpr int CP_THREADS = 256;
+
+// Base b (per axis in [0, n-4]) -> tile-major key over the valid base
+// positions (tiles of 8^3 bases, bases inside a tile x-major). A stencil
+// reaching outside the face array gets the sentinel key `space` (sorted
+// last, never visited) and raises F_INDEX.
 __global__ void compact_keys_kernel(const int64_t* __restrict__ base, int64_t points, int64_t nx,
-                                    int64_t ny, int64_t nz, uint32_t* __restrict__ keys,
+                                    int64_t ny, int64_t nz, int64_t space, uint32_t* __restrict__ keys,
                                     uint32_t* __restrict__ vals, int* flags) {
     const int64_t nbx = nx - 3, nby = ny - 3, nbz = nz - 3;
-    const int64_t space = nbx > 0 && nby > 0 && nbz > 0 ? nbx * nby * nbz : 0;
+    const int64_t nty = cdiv(nby, CP_T), ntz = cdiv(nbz, CP_T);
     for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < points;
          p += int64_t(gridDim.x) * blockDim.x) {
         const int64_t bx = base[3 * p], by = base[3 * p + 1], bz = base[3 * p + 2];
         const bool ok = bx >= 0 && bx < nbx && by >= 0 && by < nby && bz >= 0 && bz < nbz;
         if (!ok) raise_flag(flags, F_INDEX);
-        keys[p] = uint32_t(ok ? (bx * nby + by) * nbz + bz : space);
+        const int64_t tile = ((bx / CP_T) * nty + by / CP_T) * ntz + bz / CP_T;
+        const int64_t local = ((bx % CP_T) * CP_T + by % CP_T) * CP_T + bz % CP_T;
+        keys[p] = uint32_t(ok ? tile * CP_TV + local : space);
         vals[p] = uint32_t(p);
     }
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) compact_face_sum_kernel(
-    const T* __restrict__ force, int64_t fstride, const T* __restrict__ phi,
-    const uint32_t* __restrict__ keys, const int32_t* __restrict__ perm, const int32_t* __restrict__ ptr,
-    int64_t nx, int64_t ny, int64_t nz, double volume, T* __restrict__ out) {
-    const int64_t nby = ny - 3, nbz = nz - 3, nbx = nx - 3;
-    const T vol = T(volume);
-    const int64_t faces = nx * ny * nz;
-    for (int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; f < faces;
-         f += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t fz = f % nz, fy = (f / nz) % ny, fx = f / (ny * nz);
-        const int64_t zlo = fz - 3 > 0 ? fz - 3 : 0, zhi = fz < nbz - 1 ? fz : nbz - 1;
-        T acc = T(0);
-        if (zlo <= zhi) {
-            for (int64_t bx = fx - 3 > 0 ? fx - 3 : 0; bx <= fx && bx < nbx; ++bx) {
-                for (int64_t by = fy - 3 > 0 ? fy - 3 : 0; by <= fy && by < nby; ++by) {
-                    const int64_t col = (bx * nby + by) * nbz;
-                    const int32_t j1 = ptr[col + zhi + 1];
-                    for (int32_t j = ptr[col + zlo]; j < j1; ++j) {
-                        const int64_t p = perm[j];
-                        const int64_t bz = int64_t(keys[j]) - col;
-                        const T wgt = (phi[12 * p + (fx - bx)] * phi[12 * p + 4 + (fy - by)]) *
-                                      phi[12 * p + 8 + (fz - bz)];
-                        acc += (wgt * force[p * fstride]) / vol;
-                    }
-                }
-            }
+// Work items per tile: exclusive scan of ceil(points / CP_CHUNK) over the
+// tiles, by one workgroup.
+__global__ void __launch_bounds__(1024) compact_chunks_kernel(const int32_t* __restrict__ ptr, int64_t tiles,
+                                                              int32_t* __restrict__ start) {
+    __shared__ int32_t s[1024];
+    int32_t running = 0;
+    for (int64_t b = 0; b < tiles; b += 1024) {
+        const int64_t t = b + threadIdx.x;
+        const int32_t v = t < tiles ? (ptr[(t + 1) * CP_TV] - ptr[t * CP_TV] + CP_CHUNK - 1) / CP_CHUNK : 0;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int d = 1; d < 1024; d <<= 1) {
+            const int32_t add = int(threadIdx.x) >= d ? s[threadIdx.x - d] : 0;
+            __syncthreads();
+            s[threadIdx.x] += add;
+            __syncthreads();
         }
-        out[f] = acc;
+        if (t < tiles) start[t] = running + s[threadIdx.x] - v;
+        running += s[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[tiles] = running;
+}
+
+// One work item: the points of a chunk of one base tile, accumulated into the
+// tile's LDS box with ds_add_f64 (a lane walks its (j,k) links from a
+// lane-dependent rotation, so the lanes of one dense base hit 16 addresses
+// at a time), then one atomic per nonzero box entry.
+template <class T>
+__global__ void __launch_bounds__(CP_THREADS) compact_tile_spread_kernel(
+    const T* __restrict__ force, int64_t fstride, const int64_t* __restrict__ base, const T* __restrict__ phi,
+    const int32_t* __restrict__ perm, const int32_t* __restrict__ ptr, const int32_t* __restrict__ start,
+    int64_t tiles, int64_t nx, int64_t ny, int64_t nz, double inv_volume, T* __restrict__ out) {
+    __shared__ double box[CP_BOX * CP_BOX * CP_BOX];
+    const int64_t nty = cdiv(ny - 3, CP_T), ntz = cdiv(nz - 3, CP_T);
+    const int total = start[tiles];
+    const int rj = (threadIdx.x >> 2) & 3, rk = threadIdx.x & 3;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        int lo = 0, hi = int(tiles);
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (start[mid] <= w) lo = mid;
+            else hi = mid;
+        }
+        const int64_t tile = lo;
+        const int chunk = w - start[lo];
+        const int beg = ptr[tile * CP_TV] + chunk * CP_CHUNK;
+        const int end = min(ptr[(tile + 1) * CP_TV], beg + CP_CHUNK);
+        const int64_t ox = (tile / (nty * ntz)) * CP_T, oy = ((tile / ntz) % nty) * CP_T, oz = (tile % ntz) * CP_T;
+        for (int e = threadIdx.x; e < CP_BOX * CP_BOX * CP_BOX; e += CP_THREADS) box[e] = 0.0;
+        __syncthreads();
+        for (int j = beg + int(threadIdx.x); j < end; j += CP_THREADS) {
+            const int64_t p = perm[j];
+            const int lx = int(base[3 * p] - ox), ly = int(base[3 * p + 1] - oy), lz = int(base[3 * p + 2] - oz);
+            double w3[3][4];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) w3[a][k] = double(phi[12 * p + 4 * a + k]);
+            const double fv = double(force[p * fstride]) * inv_volume;
+            double wyr[4], wzr[4];
+            int oyr[4], ozr[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int jq = (q + rj) & 3, kq = (q + rk) & 3;
+                wyr[q] = jq == 0 ? w3[1][0] : (jq == 1 ? w3[1][1] : (jq == 2 ? w3[1][2] : w3[1][3]));
+                wzr[q] = kq == 0 ? w3[2][0] : (kq == 1 ? w3[2][1] : (kq == 2 ? w3[2][2] : w3[2][3]));
+                oyr[q] = jq * CP_BOX;
+                ozr[q] = kq;
+            }
+            const int b0 = (lx * CP_BOX + ly) * CP_BOX + lz;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    const double wxy = w3[0][i] * wyr[jj];
+                    const int row = b0 + i * CP_BOX * CP_BOX + oyr[jj];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) atomicAdd(box + row + ozr[k], (wxy * wzr[k]) * fv);
+                }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < CP_BOX * CP_BOX * CP_BOX; e += CP_THREADS) {
+            const double v = box[e];
+            if (v == 0.0) continue;
+            const int64_t fx = ox + e / (CP_BOX * CP_BOX), fy = oy + (e / CP_BOX) % CP_BOX, fz = oz + e % CP_BOX;
+            if (fx < nx && fy < ny && fz < nz) unsafeAtomicAdd(out + (fx * ny + fy) * nz + fz, T(v));
+        }
+        __syncthreads();
     }
 }
 
@@ -241,7 +313,7 @@ __global__ void __launch_bounds__(256) node_sum_kernel(const T* __restrict__ fie
 // Workspace of a sort-based spread over n items and `space` targets: the
 // radix sort's scratch, then sorted keys, permutation and CSR offsets.
 struct SpreadSpace {
-    size_t sort_bytes, keys_off, perm_off, ptr_off, total;
+    size_t sort_bytes, keys_off, perm_off, ptr_off, total, start_off;
 };
 
 static size_t alignb(size_t b) { return (b + 255) & ~size_t(255); }
@@ -253,12 +325,16 @@ static SpreadSpace spread_space(int64_t n, int64_t space) {
     w.perm_off = w.keys_off + alignb(size_t(n > 0 ? n : 1) * 4);
     w.ptr_off = w.perm_off + alignb(size_t(n > 0 ? n : 1) * 4);
     w.total = w.ptr_off + alignb(size_t(space + 1) * 4);
+    w.start_off = w.total;
+    w.total += alignb(size_t(space / CP_TV + 1) * 4);
     return w;
 }
 
-static int64_t compact_space(int64_t nx, int64_t ny, int64_t nz) {
-    return nx > 3 && ny > 3 && nz > 3 ? (nx - 3) * (ny - 3) * (nz - 3) : 0;
+// Key space of the compact spread: whole 8^3 tiles over the valid bases.
+static int64_t compact_tiles(int64_t nx, int64_t ny, int64_t nz) {
+    return nx > 3 && ny > 3 && nz > 3 ? cdiv(nx - 3, CP_T) * cdiv(ny - 3, CP_T) * cdiv(nz - 3, CP_T) : 0;
 }
+static int64_t compact_space(int64_t nx, int64_t ny, int64_t nz) { return compact_tiles(nx, ny, nz) * CP_TV; }
 
 template <class T>
 static int compact_gather(const T* field, const int64_t* base, const T* phi, int64_t points,
@@ -285,22 +361,30 @@ static int compact_spread(const T* force, int64_t fstride, const int64_t* base, 
     AFSI_REQUIRE(points < (int64_t(1) << 31) && nx * ny * nz < (int64_t(1) << 31),
                  "compact spread exceeds 2^31 points or faces");
     const hipStream_t s = hipStream_t(stream);
-    const int64_t space = compact_space(nx, ny, nz);
-    if (points == 0) return launch_fill<T>(out, nx * ny * nz, T(0), s);
+    const int64_t space = compact_space(nx, ny, nz), tiles = compact_tiles(nx, ny, nz);
+    AFSI_REQUIRE(space < (int64_t(1) << 31), "compact spread exceeds 2^31 stencil bases");
+    if (int r = launch_fill<T>(out, nx * ny * nz, T(0), s)) return r;
+    if (points == 0) return OK;
     AFSI_REQUIRE(force && base && phi && workspace, "null pointer");
     const SpreadSpace L = spread_space(points, space);
     AFSI_REQUIRE(workspace_bytes >= L.total, "compact spread workspace too small");
     char* ws = static_cast<char*>(workspace);
     uint32_t *k0, *v0;
     sort_key_slots(ws, points, &k0, &v0);
-    compact_keys_kernel<<<grid_for(points, 256, 8192), 256, 0, s>>>(base, points, nx, ny, nz, k0, v0, flags);
+    compact_keys_kernel<<<grid_for(points, 256, 8192), 256, 0, s>>>(base, points, nx, ny, nz, space, k0, v0,
+                                                                     flags);
     AFSI_LAUNCH_CHECK();
     uint32_t* keys = reinterpret_cast<uint32_t*>(ws + L.keys_off);
     int32_t* perm = reinterpret_cast<int32_t*>(ws + L.perm_off);
     int32_t* ptr = reinterpret_cast<int32_t*>(ws + L.ptr_off);
+    int32_t* start = reinterpret_cast<int32_t*>(ws + L.start_off);
     if (int r = sort_pairs(points, space, ws, keys, perm, ptr, s)) return r;
-    compact_face_sum_kernel<T><<<grid_for(nx * ny * nz, 256, 16384), 256, 0, s>>>(
-        force, fstride, phi, keys, perm, ptr, nx, ny, nz, volume, out);
+    if (tiles == 0) return OK;
+    compact_chunks_kernel<<<1, 1024, 0, s>>>(ptr, tiles, start);
+    AFSI_LAUNCH_CHECK();
+    const int64_t upper = tiles + points / CP_CHUNK + 1;
+    compact_tile_spread_kernel<T><<<unsigned(upper < 8192 ? upper : 8192), CP_THREADS, 0, s>>>(
+        force, fstride, base, phi, perm, ptr, start, tiles, nx, ny, nz, 1.0 / volume, out);
     AFSI_LAUNCH_CHECK();
     return OK;
 }

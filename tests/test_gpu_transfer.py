@@ -251,8 +251,10 @@ def test_transfer_tile_autotuner(gpu):
 
 
 def test_compact_spread_deterministic_and_dense(gpu):
-    """The sort-based compact spread gives the same bits on every call, with
-    a dense cluster of points sharing one stencil base."""
+    """The base-tiled compact spread with a dense cluster of points sharing
+    one stencil base: within 1e-12 of the oracle, and calls agree to
+    rounding (the order of the LDS and HBM additions is not fixed, as in the
+    reference's atomic CUDA op)."""
     g, og = grids(False)
     x, F = points_in(og, 6000, 21)
     x[:2000] = x[0] + 1e-4 * np.random.default_rng(22).normal(size=(2000, 3))
@@ -263,7 +265,7 @@ def test_compact_spread_deterministic_and_dense(gpu):
     first = cuda_ib.compact_spread(g, Ft, st)
     second = cuda_ib.compact_spread(g, Ft, st)
     for c in range(3):
-        assert torch.equal(first[c], second[c])
+        assert rel_err(first[c].cpu(), second[c].cpu().numpy()) < 1e-14
         assert rel_err(first[c].cpu(), f_ref[c]) < TOL
 
 
