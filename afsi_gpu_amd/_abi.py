@@ -34,7 +34,7 @@ class MGPlan3(Structure):
                 ('tmp', c_void_p * MAX_LEVELS), ('coarse_inverse', c_void_p),
                 ('coarse_cells', c_int64), ('reduce_partials', c_void_p),
                 ('reduce_result', c_void_p), ('partials_capacity', c_int64),
-                ('zero_initial', c_int32), ('reserved', c_int32),
+                ('zero_initial', c_int32), ('fuse_cells', c_int32),
                 ('sweep_tile', c_int32 * MAX_LEVELS), ('sweep_blocks', c_int32 * MAX_LEVELS)]
 
 

@@ -760,6 +760,360 @@ __global__ void __launch_bounds__(256) prolong_groups_kernel(const T* __restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused small levels. On levels of up to 64^3 cells a launch costs a few
+// microseconds whatever its size (global-memory round trips and launch
+// gaps, not bytes), so the 2 nu + 2 launches a level takes per V-cycle
+// (nu sweeps, restriction, prolongation, nu sweeps) dominate its time.
+// Two kernels do a level's work in two launches, for nu = 2:
+// * vdown_kernel: the pre-smoothing sweeps, the residual and its
+//   restriction. A workgroup stages its TX x TY tile of full z rows in LDS
+//   with a halo of 2 nu + 1 rows of p and 2 nu rows of rhs, then runs the
+//   2 nu half-sweeps (red, black, red, black) on shrinking regions:
+//   half-sweep s updates the tile and a ring of 2 nu - s rows, whose
+//   neighbours were all updated by half-sweep s - 1. After the last one the
+//   tile and its 1-row ring hold exactly what nu global sweeps produce, so
+//   the residual of the tile and its 2x2x2 averages follow from LDS. The
+//   workgroup writes its tile of p and the coarse cells under the tile.
+// * vup_kernel: the prolongation of the coarse correction and the
+//   post-smoothing sweeps. The coarse rows under the tile and a 2 nu halo
+//   are z-interpolated once each into LDS (the v terms of
+//   prolong_groups_kernel), p + P(e) is formed on the tile and its 2 nu halo
+//   (p read from the pre-smoothed buffer), then 2 nu half-sweeps shrink to
+//   the tile, which is written to the output buffer.
+// Every tile recomputes the halo rows it needs. Each value is formed by the
+// same expressions in the same order as in rb_rows_tiles, restrict_row and
+// prolong_groups_kernel (sweep update, difference-form residual, trilinear
+// weights), so the fused levels reproduce the unfused ones.
+
+constexpr int FUSE_NU = 2;
+constexpr int FUSE_NT = 512;  // threads per fused workgroup
+
+// Box row (bx, by) of a box with halo H around the tile at (x0, y0): the
+// global row (wrapped on periodic axes); false where it lies outside a
+// closed domain. The halo never exceeds an axis extent (checked on the
+// host), so one wrap suffices.
+template <bool P, int H>
+__device__ __forceinline__ bool box_row(int x0, int y0, int bx, int by, const Dim32& d, int& gx, int& gy) {
+    gx = x0 - H + bx;
+    gy = y0 - H + by;
+    if (P) {
+        gx = gx < 0 ? gx + d.n0 : (gx >= d.n0 ? gx - d.n0 : gx);
+        gy = gy < 0 ? gy + d.n1 : (gy >= d.n1 ? gy - d.n1 : gy);
+        return true;
+    }
+    return gx >= 0 && gx < d.n0 && gy >= 0 && gy < d.n1;
+}
+
+// Red-black half-sweep `colour` (0 red: gx + gy + z even) over the tile and
+// a ring of W rows, in place in the p box (BY rows per box x, N2 per row);
+// the rhs box has a halo of H - 1. Same update as rb_rows_tiles. A thread
+// takes its cells in groups of FUSE_ILP: all LDS reads of a group are
+// issued before any of its writes (cells of one colour never read each
+// other), so their latencies overlap.
+constexpr int FUSE_ILP = 4;
+
+template <class T, bool P, int TX, int TY, int N2, int H, int W>
+__device__ __forceinline__ void fused_half_sweep(T* box, const T* rbx, int colour, int x0, int y0,
+                                                 const Dim32& d, const W3<T>& k, const T* tab, T inv_p) {
+    constexpr int BY = TY + 2 * H, RY = BY - 2, H2 = N2 / 2;
+    constexpr int RWY = TY + 2 * W, ITEMS = (TX + 2 * W) * RWY * H2;
+    constexpr int PER = (ITEMS + FUSE_NT - 1) / FUSE_NT;
+#pragma unroll
+    for (int g0 = 0; g0 < PER; g0 += FUSE_ILP) {
+        T val[FUSE_ILP];
+        int at[FUSE_ILP];
+#pragma unroll
+        for (int g = 0; g < FUSE_ILP; ++g) {
+            at[g] = -1;
+            const int i = int(threadIdx.x) + FUSE_NT * (g0 + g);
+            if (g0 + g >= PER || i >= ITEMS) continue;
+            const int rr = i / H2, zl = i % H2;
+            const int bx = H - W + rr / RWY, by = H - W + rr % RWY;
+            int gx, gy;
+            if (!box_row<P, H>(x0, y0, bx, by, d, gx, gy)) continue;
+            const int r = bx * BY + by;
+            const int z = 2 * zl + ((gx + gy + colour) & 1);
+            const T* row = box + r * N2;
+            const T b = rbx[((bx - 1) * RY + (by - 1)) * N2 + z];
+            T s = T(0);
+            if (P) {
+                s += k.w[2] * row[z + 1 < N2 ? z + 1 : 0];
+                s += k.w[2] * row[z > 0 ? z - 1 : N2 - 1];
+            } else {
+                if (z + 1 < N2) s += k.w[2] * row[z + 1];
+                if (z > 0) s += k.w[2] * row[z - 1];
+            }
+            if (P || gy + 1 < d.n1) s += k.w[1] * box[(r + 1) * N2 + z];
+            if (P || gy > 0) s += k.w[1] * box[(r - 1) * N2 + z];
+            if (P || gx + 1 < d.n0) s += k.w[0] * box[(r + BY) * N2 + z];
+            if (P || gx > 0) s += k.w[0] * box[(r - BY) * N2 + z];
+            const T inv = P ? inv_p : tab[face_class(gx, d.n0) * 9 + face_class(gy, d.n1) * 3 + face_class(z, N2)];
+            val[g] = (b + s) * inv;
+            at[g] = r * N2 + z;
+        }
+#pragma unroll
+        for (int g = 0; g < FUSE_ILP; ++g)
+            if (at[g] >= 0) box[at[g]] = val[g];
+    }
+}
+
+// The 2 nu half-sweeps of one smoothing stage; the first updates a ring of
+// W0 rows, each later one a ring one row narrower.
+template <class T, bool P, int TX, int TY, int N2, int H, int W0>
+__device__ __forceinline__ void fused_sweeps(T* box, const T* rbx, int x0, int y0, const Dim32& d,
+                                             const W3<T>& k, const T* tab, T inv_p) {
+    static_assert(2 * FUSE_NU == 4, "four half-sweeps");
+    fused_half_sweep<T, P, TX, TY, N2, H, W0>(box, rbx, 0, x0, y0, d, k, tab, inv_p);
+    __syncthreads();
+    fused_half_sweep<T, P, TX, TY, N2, H, W0 - 1>(box, rbx, 1, x0, y0, d, k, tab, inv_p);
+    __syncthreads();
+    fused_half_sweep<T, P, TX, TY, N2, H, W0 - 2>(box, rbx, 0, x0, y0, d, k, tab, inv_p);
+    __syncthreads();
+    fused_half_sweep<T, P, TX, TY, N2, H, W0 - 3>(box, rbx, 1, x0, y0, d, k, tab, inv_p);
+    __syncthreads();
+}
+
+// Global rows of the p box (halo H) and of the rhs box (halo H - 1) into
+// registers, 16 bytes per load; rows outside a closed domain read as 0.
+template <class T, bool P, int TX, int TY, int N2, int H, int NL, bool RHS>
+__device__ __forceinline__ void fused_fetch(const T* __restrict__ src, uint4 (&v)[NL], int x0, int y0,
+                                            const Dim32& d) {
+    constexpr int CPR = N2 * int(sizeof(T)) / 16;
+    constexpr int BY = TY + 2 * H;
+    constexpr int OX = RHS ? TX + 2 * H - 2 : TX + 2 * H, OY = RHS ? BY - 2 : BY;
+    constexpr int CHUNKS = OX * OY * CPR;
+    static_assert(NL * FUSE_NT >= CHUNKS, "register staging too small");
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        const int c = int(threadIdx.x) + FUSE_NT * i;
+        v[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (c < CHUNKS) {
+            const int row = c / CPR, part = c % CPR;
+            const int bx = row / OY + (RHS ? 1 : 0), by = row % OY + (RHS ? 1 : 0);
+            int gx, gy;
+            if (box_row<P, H>(x0, y0, bx, by, d, gx, gy))
+                v[i] = reinterpret_cast<const uint4*>(src + (int64_t(gx) * d.n1 + gy) * N2)[part];
+        }
+    }
+}
+
+template <class T, int NL, int CHUNKS>
+__device__ __forceinline__ void fused_store(T* lds, const uint4 (&v)[NL]) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        const int c = int(threadIdx.x) + FUSE_NT * i;
+        if (c < CHUNKS) reinterpret_cast<uint4*>(lds)[c] = v[i];
+    }
+}
+
+// Tile rows of the box to the output level (16-byte stores).
+template <class T, int TX, int TY, int N2, int H>
+__device__ __forceinline__ void fused_write_tile(const T* box, T* __restrict__ out, int x0, int y0,
+                                                 const Dim32& d) {
+    constexpr int CPR = N2 * int(sizeof(T)) / 16, BY = TY + 2 * H;
+    for (int c = threadIdx.x; c < TX * TY * CPR; c += FUSE_NT) {
+        const int row = c / CPR, part = c % CPR;
+        const int lx = row / TY, ly = row % TY;
+        reinterpret_cast<uint4*>(out + (int64_t(x0 + lx) * d.n1 + (y0 + ly)) * N2)[part] =
+            reinterpret_cast<const uint4*>(box + ((H + lx) * BY + (H + ly)) * N2)[part];
+    }
+}
+
+template <class T, bool P, int TX, int TY, int N2>
+struct FusedDown {
+    static constexpr int H = 2 * FUSE_NU + 1;
+    static constexpr int BX = TX + 2 * H, BY = TY + 2 * H, RX = BX - 2, RY = BY - 2;
+    static constexpr int CPR = N2 * int(sizeof(T)) / 16;
+    static constexpr int PCH = BX * BY * CPR, RCH = RX * RY * CPR;
+    static constexpr int NP = (PCH + FUSE_NT - 1) / FUSE_NT, NR = (RCH + FUSE_NT - 1) / FUSE_NT;
+    static constexpr size_t LDS = size_t(BX * BY + RX * RY) * N2 * sizeof(T);
+};
+
+template <class T, bool P, int TX, int TY, int N2>
+struct FusedUp {
+    static constexpr int H = 2 * FUSE_NU;
+    static constexpr int BX = TX + 2 * H, BY = TY + 2 * H, RX = BX - 2, RY = BY - 2;
+    static constexpr int CX = TX / 2 + H + 2, CY = TY / 2 + H + 2, C2 = N2 / 2;
+    static constexpr int CPR = N2 * int(sizeof(T)) / 16, CCPR = C2 * int(sizeof(T)) / 16;
+    static constexpr int PCH = BX * BY * CPR, RCH = RX * RY * CPR, CCH = CX * CY * CCPR;
+    static constexpr int NP = (PCH + FUSE_NT - 1) / FUSE_NT, NR = (RCH + FUSE_NT - 1) / FUSE_NT, NC = (CCH + FUSE_NT - 1) / FUSE_NT;
+    static constexpr size_t LDS = size_t(BX * BY + RX * RY) * N2 * sizeof(T) +
+                                  size_t(CX * CY) * (C2 + N2) * sizeof(T);
+};
+
+// nu = 2 pre-smoothing sweeps of pin (zero_in: of a zero iterate) into the
+// tile of pout, and coarse = avg8(rhs - A p) under the tile.
+template <class T, bool P, int TX, int TY, int N2>
+__global__ void __launch_bounds__(FUSE_NT) vdown_kernel(const T* __restrict__ pin, const T* __restrict__ rhs,
+                                                    T* __restrict__ pout, T* __restrict__ coarse, Dim32 d,
+                                                    W3<T> k, int zero_in, int ntx, int nty) {
+    using G = FusedDown<T, P, TX, TY, N2>;
+    constexpr int H = G::H, BY = G::BY, RY = G::RY, C2 = N2 / 2;
+    static_assert(TX % 2 == 0 && TY % 2 == 0 && N2 % 32 == 0 && N2 <= 128, "fused tile shape");
+    extern __shared__ unsigned char raw[];
+    T* box = reinterpret_cast<T*>(raw);
+    T* rbx = box + G::BX * BY * N2;
+    __shared__ T tab[27];
+    const int tile = int(xcd_tile(blockIdx.x, int64_t(ntx) * nty));
+    const int x0 = (tile / nty) * TX, y0 = (tile % nty) * TY;
+    {
+        uint4 vp[G::NP], vr[G::NR];
+        if (zero_in) {
+#pragma unroll
+            for (int i = 0; i < G::NP; ++i) vp[i] = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            fused_fetch<T, P, TX, TY, N2, H, G::NP, false>(pin, vp, x0, y0, d);
+        }
+        fused_fetch<T, P, TX, TY, N2, H, G::NR, true>(rhs, vr, x0, y0, d);
+        fused_store<T, G::NP, G::PCH>(box, vp);
+        fused_store<T, G::NR, G::RCH>(rbx, vr);
+    }
+    if (!P && threadIdx.x < 27) tab[threadIdx.x] = inv_diag_entry<T>(threadIdx.x, k);
+    __syncthreads();
+    const T inv_p = T(1) / (T(2) * (k.w[0] + k.w[1] + k.w[2]));
+    fused_sweeps<T, P, TX, TY, N2, H, H - 1>(box, rbx, x0, y0, d, k, tab, inv_p);
+    fused_write_tile<T, TX, TY, N2, H>(box, pout, x0, y0, d);
+    // residual on the tile (difference form of apply_row) and its 2x2x2
+    // averages, summed in restrict_row's order
+    const int C1 = d.n1 / 2;
+    for (int i = threadIdx.x; i < (TX / 2) * (TY / 2) * C2; i += FUSE_NT) {
+        const int cr = i / C2, cz = i % C2;
+        const int cxl = cr / (TY / 2), cyl = cr % (TY / 2);
+        T v = T(0);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int lx = 2 * cxl + a, ly = 2 * cyl + b;
+                const int gx = x0 + lx, gy = y0 + ly;
+                const int r = (H + lx) * BY + (H + ly);
+                const T* row = box + r * N2;
+                const T* rr = rbx + ((H - 1 + lx) * RY + (H - 1 + ly)) * N2;
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const int z = 2 * cz + c;
+                    const T cv = row[z];
+                    T ap = T(0);
+                    if (P) {
+                        ap += k.w[0] * (cv - box[(r + BY) * N2 + z]);
+                        ap += k.w[0] * (cv - box[(r - BY) * N2 + z]);
+                        ap += k.w[1] * (cv - box[(r + 1) * N2 + z]);
+                        ap += k.w[1] * (cv - box[(r - 1) * N2 + z]);
+                        ap += k.w[2] * (cv - row[z + 1 < N2 ? z + 1 : 0]);
+                        ap += k.w[2] * (cv - row[z > 0 ? z - 1 : N2 - 1]);
+                    } else {
+                        if (gx + 1 < d.n0) ap += k.w[0] * (cv - box[(r + BY) * N2 + z]);
+                        if (gx > 0) ap += k.w[0] * (cv - box[(r - BY) * N2 + z]);
+                        if (gy + 1 < d.n1) ap += k.w[1] * (cv - box[(r + 1) * N2 + z]);
+                        if (gy > 0) ap += k.w[1] * (cv - box[(r - 1) * N2 + z]);
+                        if (z + 1 < N2) ap += k.w[2] * (cv - row[z + 1]);
+                        if (z > 0) ap += k.w[2] * (cv - row[z - 1]);
+                    }
+                    v += rr[z] - ap;
+                }
+            }
+        coarse[(int64_t(x0 / 2 + cxl) * C1 + (y0 / 2 + cyl)) * C2 + cz] = v * T(0.125);
+    }
+}
+
+// pout = nu = 2 post-smoothing sweeps of pin + trilinear(coarse), tile by
+// tile.
+template <class T, bool P, int TX, int TY, int N2>
+__global__ void __launch_bounds__(FUSE_NT) vup_kernel(const T* __restrict__ pin, const T* __restrict__ coarse,
+                                                  const T* __restrict__ rhs, T* __restrict__ pout, Dim32 d,
+                                                  W3<T> k, int ntx, int nty) {
+    using G = FusedUp<T, P, TX, TY, N2>;
+    constexpr int H = G::H, BY = G::BY, CX = G::CX, CY = G::CY, C2 = G::C2;
+    static_assert(TX % 2 == 0 && TY % 2 == 0 && N2 % 32 == 0 && N2 <= 128, "fused tile shape");
+    extern __shared__ unsigned char raw[];
+    T* box = reinterpret_cast<T*>(raw);
+    T* rbx = box + G::BX * BY * N2;
+    T* craw = rbx + G::RX * G::RY * N2;  // CX*CY coarse rows of C2
+    T* vz = craw + CX * CY * C2;         // the same rows z-interpolated to N2
+    __shared__ T tab[27];
+    const int tile = int(xcd_tile(blockIdx.x, int64_t(ntx) * nty));
+    const int x0 = (tile / nty) * TX, y0 = (tile % nty) * TY;
+    const int C0 = d.n0 / 2, C1 = d.n1 / 2;
+    // coarse row ia of the staged block is the unwrapped coarse x index
+    // ax_lo + ia (wrapped, or clamped on a closed axis as prolong_groups
+    // clamps it)
+    const int ax_lo = x0 / 2 - FUSE_NU - 1, ay_lo = y0 / 2 - FUSE_NU - 1;
+    uint4 vp[G::NP], vr[G::NR];
+    fused_fetch<T, P, TX, TY, N2, H, G::NP, false>(pin, vp, x0, y0, d);
+    fused_fetch<T, P, TX, TY, N2, H, G::NR, true>(rhs, vr, x0, y0, d);
+    {
+        uint4 vc[G::NC];
+#pragma unroll
+        for (int i = 0; i < G::NC; ++i) {
+            const int c = int(threadIdx.x) + FUSE_NT * i;
+            vc[i] = make_uint4(0u, 0u, 0u, 0u);
+            if (c < G::CCH) {
+                const int row = c / G::CCPR, part = c % G::CCPR;
+                int cx = ax_lo + row / CY, cy = ay_lo + row % CY;
+                if (P) {
+                    cx = cx < 0 ? cx + C0 : (cx >= C0 ? cx - C0 : cx);
+                    cy = cy < 0 ? cy + C1 : (cy >= C1 ? cy - C1 : cy);
+                } else {
+                    cx = cx < 0 ? 0 : (cx > C0 - 1 ? C0 - 1 : cx);
+                    cy = cy < 0 ? 0 : (cy > C1 - 1 ? C1 - 1 : cy);
+                }
+                vc[i] = reinterpret_cast<const uint4*>(coarse + (int64_t(cx) * C1 + cy) * C2)[part];
+            }
+        }
+        fused_store<T, G::NC, G::CCH>(craw, vc);
+    }
+    if (!P && threadIdx.x < 27) tab[threadIdx.x] = inv_diag_entry<T>(threadIdx.x, k);
+    __syncthreads();
+    for (int i = threadIdx.x; i < CX * CY * N2; i += FUSE_NT) {
+        const int row = i / N2, z = i % N2;
+        const int az = z / 2 - 1 + (z & 1);
+        int z0, z1;
+        if (P) { z0 = az < 0 ? az + C2 : az; z1 = az + 1 >= C2 ? az + 1 - C2 : az + 1; }
+        else { z0 = az > 0 ? az : 0; z1 = az + 1 < C2 - 1 ? az + 1 : C2 - 1; }
+        const T wz = (z & 1) ? T(0.25) : T(0.75);
+        const T* c = craw + row * C2;
+        vz[i] = c[z0] * (T(1) - wz) + c[z1] * wz;
+    }
+    fused_store<T, G::NR, G::RCH>(rbx, vr);
+    __syncthreads();
+    // p + P(e) on the box (rows outside a closed domain stay unused)
+    constexpr int VEC = 16 / int(sizeof(T));
+#pragma unroll
+    for (int i = 0; i < G::NP; ++i) {
+        const int c = int(threadIdx.x) + FUSE_NT * i;
+        if (c >= G::PCH) continue;
+        const int row = c / G::CPR, part = c % G::CPR;
+        const int bx = row / BY, by = row % BY;
+        int gx, gy;
+        union { uint4 u; T t[VEC]; } val;
+        val.u = vp[i];
+        if (box_row<P, H>(x0, y0, bx, by, d, gx, gy)) {  // reads only: vz and registers
+            const int ux = x0 - H + bx, uy = y0 - H + by;  // unwrapped fine row
+            const int ia = ((ux - 1) >> 1) - ax_lo, ib = ((uy - 1) >> 1) - ay_lo;
+            const T* v00 = vz + (ia * CY + ib) * N2;
+            const T* v01 = vz + (ia * CY + ib + 1) * N2;
+            const T* v10 = vz + ((ia + 1) * CY + ib) * N2;
+            const T* v11 = vz + ((ia + 1) * CY + ib + 1) * N2;
+            const T wx = (ux & 1) ? T(0.25) : T(0.75);
+            const T wy = (uy & 1) ? T(0.25) : T(0.75);
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const int z = part * VEC + e;
+                val.t[e] = val.t[e] + ((v00[z] * (T(1) - wy) + v01[z] * wy) * (T(1) - wx) +
+                                       (v10[z] * (T(1) - wy) + v11[z] * wy) * wx);
+            }
+        }
+        vp[i] = val.u;
+    }
+    // all vz reads above are issued before the box writes below
+    fused_store<T, G::NP, G::PCH>(box, vp);
+    __syncthreads();
+    const T inv_p = T(1) / (T(2) * (k.w[0] + k.w[1] + k.w[2]));
+    fused_sweeps<T, P, TX, TY, N2, H, H - 1>(box, rbx, x0, y0, d, k, tab, inv_p);
+    fused_write_tile<T, TX, TY, N2, H>(box, pout, x0, y0, d);
+}
+
 // per-workgroup [sum r, sum r^2, sum |r|, non-finite] of r = rhs - A p,
 // one wavefront per row, grid-stride over rows.
 template <class T, bool P>
@@ -1600,8 +1954,82 @@ struct FastRunner {
         return OK;
     }
 
+    // Tile of the fused down/up launches at level l: 2 for 2x2, 4 for 4x4
+    // (FP32) or 4x2 (FP64) rows; 0 where the level runs unfused.
+    int fuse_code(int l) const {
+        if (pl.smoother != 1 || pl.nu != FUSE_NU || l >= coarse_first || pl.fuse_cells < 0) return 0;
+        const int64_t limit = pl.fuse_cells > 0 ? int64_t(pl.fuse_cells) : (int64_t(1) << 18);
+        const int64_t n0 = pl.n[l][0], n1 = pl.n[l][1], n2 = pl.n[l][2];
+        if (cells(l) > limit || (n2 != 32 && n2 != 64)) return 0;
+        const int code = n2 == 64 ? 4 : 2;
+        const int tx = code, ty = (code == 4 && sizeof(T) == 8) ? 2 : code;
+        if (n0 % tx || n1 % ty || n0 < 2 * FUSE_NU + 1 || n1 < 2 * FUSE_NU + 1) return 0;
+        const void* ptrs[4] = {pl.p[l], pl.tmp[l], pl.rhs[l], pl.p[l + 1]};
+        for (const void* q : ptrs)
+            if (reinterpret_cast<uintptr_t>(q) % 16) return 0;
+        return code;
+    }
+
+    template <int TX, int TY, int N2>
+    int vdown_n(int l, const T* in, T* out, bool zero_in) const {
+        const Dim32 d = D(l);
+        const int ntx = d.n0 / TX, nty = d.n1 / TY;
+        const size_t bytes = FusedDown<T, true, TX, TY, N2>::LDS;
+        const void* fn = pl.periodic ? reinterpret_cast<const void*>(vdown_kernel<T, true, TX, TY, N2>)
+                                     : reinterpret_cast<const void*>(vdown_kernel<T, false, TX, TY, N2>);
+        if (int r = dynamic_lds(fn, bytes)) return r;
+        const unsigned g = unsigned(ntx * nty);
+        if (pl.periodic)
+            vdown_kernel<T, true, TX, TY, N2><<<g, FUSE_NT, bytes, s>>>(in, R(l), out, R(l + 1), d, K(l), zero_in, ntx, nty);
+        else
+            vdown_kernel<T, false, TX, TY, N2><<<g, FUSE_NT, bytes, s>>>(in, R(l), out, R(l + 1), d, K(l), zero_in, ntx, nty);
+        AFSI_LAUNCH_CHECK();
+        return OK;
+    }
+
+    template <int TX, int TY, int N2>
+    int vup_n(int l, const T* in, const T* cc, T* out) const {
+        const Dim32 d = D(l);
+        const int ntx = d.n0 / TX, nty = d.n1 / TY;
+        const size_t bytes = FusedUp<T, true, TX, TY, N2>::LDS;
+        const void* fn = pl.periodic ? reinterpret_cast<const void*>(vup_kernel<T, true, TX, TY, N2>)
+                                     : reinterpret_cast<const void*>(vup_kernel<T, false, TX, TY, N2>);
+        if (int r = dynamic_lds(fn, bytes)) return r;
+        const unsigned g = unsigned(ntx * nty);
+        if (pl.periodic)
+            vup_kernel<T, true, TX, TY, N2><<<g, FUSE_NT, bytes, s>>>(in, cc, R(l), out, d, K(l), ntx, nty);
+        else
+            vup_kernel<T, false, TX, TY, N2><<<g, FUSE_NT, bytes, s>>>(in, cc, R(l), out, d, K(l), ntx, nty);
+        AFSI_LAUNCH_CHECK();
+        return OK;
+    }
+
+    int vdown(int code, int l, const T* in, T* out, bool zero_in) const {
+        constexpr int TY4 = sizeof(T) == 8 ? 2 : 4;
+        if (code == 2)
+            return D(l).n2 == 64 ? vdown_n<2, 2, 64>(l, in, out, zero_in) : vdown_n<2, 2, 32>(l, in, out, zero_in);
+        return D(l).n2 == 64 ? vdown_n<4, TY4, 64>(l, in, out, zero_in) : vdown_n<4, TY4, 32>(l, in, out, zero_in);
+    }
+
+    int vup(int code, int l, const T* in, const T* cc, T* out) const {
+        constexpr int TY4 = sizeof(T) == 8 ? 2 : 4;
+        if (code == 2) return D(l).n2 == 64 ? vup_n<2, 2, 64>(l, in, cc, out) : vup_n<2, 2, 32>(l, in, cc, out);
+        return D(l).n2 == 64 ? vup_n<4, TY4, 64>(l, in, cc, out) : vup_n<4, TY4, 32>(l, in, cc, out);
+    }
+
     int cycle(int l, T*& cur, bool zero_initial = false) const {
         if (l == coarse_first) return coarse(l, (l == 0 && !zero_initial) ? cur : nullptr, cur);
+        if (const int code = fuse_code(l)) {
+            T* out = cur == P(l) ? W(l) : P(l);
+            if (int r = vdown(code, l, cur, out, l > 0 || zero_initial)) return r;
+            cur = out;
+            T* cc = P(l + 1);
+            if (int r = cycle(l + 1, cc)) return r;
+            out = cur == P(l) ? W(l) : P(l);
+            if (int r = vup(code, l, cur, cc, out)) return r;
+            cur = out;
+            return OK;
+        }
         // coarse iterates (and a zero initial level-0 iterate) start at zero
         if (int r = smooth(l, cur, l > 0 || zero_initial)) return r;
         const Dim32 d = D(l);

@@ -315,7 +315,11 @@ typedef struct afsi_mg3d_plan {
     double* reduce_result;
     int64_t partials_capacity; /* doubles in reduce_partials */
     int32_t zero_initial;      /* fused driver: treat p[0] as zero on the first cycle */
-    int32_t reserved;
+    /* red-black, nu = 2: levels with at most fuse_cells cells (0: 262144;
+     * negative: none) and z extent 32 or 64 run their pre-smoothing,
+     * residual and restriction as one launch and their prolongation and
+     * post-smoothing as another (tiles recompute their halo rows). */
+    int32_t fuse_cells;
     /* fused red-black sweeps, per level (0 = automatic): tile edge TX = TY
      * (2 or 4) and workgroups per launch (persistent: each workgroup sweeps
      * every G-th tile, overlapping the next tile's loads with compute). */
@@ -329,7 +333,9 @@ int afsi_mg3d_supported(const afsi_mg3d_plan* plan);
 int64_t afsi_mg3d_partials_doubles(const int64_t n[3]);
 /* Fused driver (the production path). Per fine level: nu single-launch
  * sweeps (red-black: LDS tile + 2-cell halo, red on tile+1 ring then
- * black; or weighted Jacobi), one residual-restriction, one prolongation.
+ * black; or weighted Jacobi), one residual-restriction, one prolongation;
+ * levels selected by fuse_cells instead run two launches (sweeps +
+ * residual + restriction; prolongation + sweeps) with identical results.
  * All levels with <= 4096 cells run as one 1024-thread workgroup out of
  * LDS (including the dense coarse solve). No level performs the
  * reference's intermediate mean subtractions (mac/multigrid.py:107-123):
