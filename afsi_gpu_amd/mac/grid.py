@@ -52,20 +52,30 @@ class MACGrid:
         return tuple(torch.zeros(self.face_shape(c), device=device, dtype=dtype) for c in range(3))
 
     def coordinates(self, component=None, *, device='cpu', dtype=torch.float64):
-        shape = self.shape if component is None else self.face_shape(component)
-        origin = (tuple(o + .5 * h for o, h in zip(self.origin, self.spacing))
-                  if component is None else self.face_origin(component))
-        axes = [o + h * torch.arange(n, device=device, dtype=dtype)
-                for n, h, o in zip(shape, self.spacing, origin)]
-        return torch.stack(torch.meshgrid(*axes, indexing='ij'), -1)
+        """Positions, shape (n0, n1, n2, 3): cell centres (component None)
+        or the faces of velocity component ``component``; entry [i, j, k, a]
+        is start_a + h_a * index_a."""
+        if component is None:
+            counts = self.shape
+            start = tuple(o + .5 * h for o, h in zip(self.origin, self.spacing))
+        else:
+            counts, start = self.face_shape(component), self.face_origin(component)
+        out = torch.empty((*counts, 3), device=device, dtype=dtype)
+        for a, (n, h, s) in enumerate(zip(counts, self.spacing, start)):
+            line = s + h * torch.arange(n, device=device, dtype=dtype)
+            out[..., a] = line.view([n if b == a else 1 for b in range(3)])
+        return out
 
     def check_velocity(self, velocity):
+        """Three components on their face shapes, on one device, in one
+        floating dtype (FP32 or FP64)."""
         if len(velocity) != 3:
             raise ValueError('three MAC velocity components required')
-        for c, u in enumerate(velocity):
-            if (u.shape != self.face_shape(c) or u.dtype not in (torch.float32, torch.float64)
-                    or u.device != velocity[0].device or u.dtype != velocity[0].dtype):
-                raise ValueError('MAC field shape/device/dtype mismatch')
+        lead = velocity[0]
+        same = all(u.device == lead.device and u.dtype == lead.dtype for u in velocity)
+        shaped = all(tuple(u.shape) == self.face_shape(c) for c, u in enumerate(velocity))
+        if not (same and shaped and lead.dtype in (torch.float32, torch.float64)):
+            raise ValueError('MAC field shape/device/dtype mismatch')
 
     def abi(self):
         """The afsi_grid3 struct; centre origins use face_origin's own FP64

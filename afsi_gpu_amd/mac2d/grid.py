@@ -38,21 +38,26 @@ class ChannelGrid:
         return tuple(n + (axis == c) for axis, n in enumerate(self.shape))
 
     def coordinates(self, c=None, *, device='cpu', dtype=torch.float64):
-        shape = self.shape if c is None else self.face_shape(c)
-        axes = [h * (torch.arange(n, device=device, dtype=dtype) + (0 if c == a else .5))
-                for a, (n, h) in enumerate(zip(shape, self.spacing))]
-        return torch.stack(torch.meshgrid(*axes, indexing='ij'), -1)
+        """Positions, shape (n0, n1, 2): cell centres (c None) or the faces
+        of component c; the channel starts at the origin."""
+        counts = self.shape if c is None else self.face_shape(c)
+        out = torch.empty((*counts, 2), device=device, dtype=dtype)
+        for a, (n, h) in enumerate(zip(counts, self.spacing)):
+            index = torch.arange(n, device=device, dtype=dtype) + (0 if c == a else .5)
+            out[..., a] = (h * index).view([n if b == a else 1 for b in range(2)])
+        return out
 
     def zeros(self, *, device='cpu', dtype=torch.float64):
         return tuple(torch.zeros(self.face_shape(c), device=device, dtype=dtype) for c in range(2))
 
     def check_velocity(self, velocity):
+        """Two components on their face shapes, one device and one dtype."""
         if len(velocity) != 2:
             raise ValueError('two velocity components required')
-        for c, u in enumerate(velocity):
-            if u.shape != self.face_shape(c) or u.dtype != velocity[0].dtype or \
-                    u.device != velocity[0].device:
-                raise ValueError('channel velocity shape/device/dtype mismatch')
+        lead = velocity[0]
+        if any(tuple(u.shape) != self.face_shape(c) or u.dtype != lead.dtype or u.device != lead.device
+               for c, u in enumerate(velocity)):
+            raise ValueError('channel velocity shape/device/dtype mismatch')
 
     def abi(self):
         g = _abi.Grid2()
