@@ -89,6 +89,17 @@ class CHBBatch(Structure):
     _fields_ = [('count', c_int32), ('reserved', c_int32), ('items', POINTER(CHBItem)),
                 ('workspace', c_void_p), ('workspace_bytes', c_size_t)]
 
+
+class IB3BItem(Structure):
+    _fields_ = [('velocity', c_void_p * 3), ('force', c_void_p * 3), ('rhs', c_void_p),
+                ('pressure', c_void_p * 2), ('x', c_void_p), ('point_velocity', c_void_p),
+                ('points', c_int64), ('grid', Grid3)]
+
+
+class IB3BBatch(Structure):
+    _fields_ = [('count', c_int32), ('reserved', c_int32), ('items', POINTER(IB3BItem)),
+                ('workspace', c_void_p), ('workspace_bytes', c_size_t)]
+
 class FEGeometry(Structure):
     _fields_ = [('cells', c_void_p), ('values', c_void_p), ('weights', c_void_p), ('elements', c_int64),
                 ('nodes_per_cell', c_int32), ('quad', c_int32), ('node_count', c_int64),
@@ -205,6 +216,12 @@ _SIGNATURES = {
     'afsi_mgb_partials_doubles': (c_int64, [POINTER(MGBBatch)]),
     'afsi_mgb_prepare': (c_int, [POINTER(MGBBatch), _P]),
     'afsi_mgb_begin': (c_int, [POINTER(MGBBatch), c_double, c_double, _P]),
+    'afsi_mgb_begin_at': (c_int, [POINTER(MGBBatch), c_int32, c_double, c_double, _P]),
+    'afsi_ib3b_workspace_bytes': (c_size_t, [POINTER(IB3BBatch)]),
+    'afsi_ib3b_prepare': (c_int, [POINTER(IB3BBatch), _P]),
+    'afsi_ib3b_rhs': (c_int, [POINTER(IB3BBatch), c_double, c_double, _P]),
+    'afsi_ib3b_correct': (c_int, [POINTER(IB3BBatch), c_int32, c_double, c_double, _P]),
+    'afsi_ib3b_advance': (c_int, [POINTER(IB3BBatch), c_double, _P]),
     'afsi_mgb_iterate': (c_int, [POINTER(MGBBatch), c_int32, c_int32, _P]),
     'afsi_mgb_finish': (c_int, [POINTER(MGBBatch), c_int32, _P]),
     'afsi_channel_prepare_f64': (c_int, [_P, c_int64, POINTER(Grid2), _P, _P, _P, _P, _P, _P]),

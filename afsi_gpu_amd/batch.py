@@ -16,12 +16,12 @@ one launch per phase (one key space, one work list of (instance, tile,
 chunk)). ``BatchedIBStep`` advances a list of 3D instances by one IB step
 each with both.
 """
+import ctypes
 from ctypes import byref, c_double, c_int64, c_size_t
 from math import isfinite
 import torch
 
 from . import _abi
-from .mac.flow import correct, divergence
 from .mac.multigrid import GeometricMultigrid, MGOptions
 from .mac.transfer import DEFAULT_TILE
 
@@ -152,12 +152,22 @@ class BatchedMixedMultigrid:
                 if initial[i].shape != r.shape or initial[i].dtype != r.dtype:
                     raise ValueError('invalid initial pressure')
                 self.x[i][0].copy_(initial[i])
-        self.cur = 0
-        _abi.call('afsi_mgb_begin', self.batch, c_double(self.options.rtol),
+        infos = self.solve_resident(0)
+        return [self.x[i][self.cur].clone() for i in range(n)], infos
+
+    @torch.no_grad()
+    def solve_resident(self, start):
+        """The same solve on the solver's own buffers, with no copies: the
+        right-hand sides already in ``b[i]`` and the initial guesses in
+        ``x[i][start]`` (``start = cur`` continues from the previous
+        solution). The solutions are left in ``x[i][cur]``; returns the
+        per-instance info dicts."""
+        n = len(self.grids)
+        self.cur = int(start)
+        _abi.call('afsi_mgb_begin_at', self.batch, self.cur, c_double(self.options.rtol),
                   c_double(self.options.atol), _abi.stream())
         res = self.results.tolist()
         tol = self.tol.tolist()
-        infos = [None] * n
         for i in range(n):
             count = self.b[i].numel()
             s1, a1, nf1 = res[i][4], res[i][6], res[i][7]
@@ -191,11 +201,8 @@ class BatchedMixedMultigrid:
                 raise RuntimeError(f'pressure multigrid did not converge (instance {i}): '
                                    f'{residual[i]:g} > {tol[i]:g}')
         _abi.call('afsi_mgb_finish', self.batch, int(self.cur), _abi.stream())
-        out = [self.x[i][self.cur].clone() for i in range(n)]
-        for i in range(n):
-            infos[i] = dict(cycles=cycles[i], residual_norm=residual[i], tolerance=tol[i],
-                            backend=self.backend)
-        return out, infos
+        return [dict(cycles=cycles[i], residual_norm=residual[i], tolerance=tol[i], backend=self.backend)
+                for i in range(n)]
 
 
 class BatchedTransfer:
@@ -275,11 +282,12 @@ def ctypes_items(array):
 
 
 class BatchedIBStep:
-    """One IB step for each of many 3D instances: one launch per phase for
-    binning, spread, the multigrid phases of the mixed-precision pressure
-    solve and interpolation (per-instance launches only for the predictor
-    axpy, divergence and gradient correction). ``grids[i]`` is a
-    ``MACGrid`` and ``counts[i]`` its number of points."""
+    """One IB step for each of many 3D instances, one launch per phase over
+    all of them: binning, spread, the projection's right-hand side (the IB
+    predictor folded in), every multigrid phase of the mixed-precision
+    pressure solve, the gradient correction, interpolation and the point
+    advance. ``grids[i]`` is a ``MACGrid`` and ``counts[i]`` its number of
+    points; all grids share the boundary type."""
 
     def __init__(self, grids, counts, *, dt, rho=1., device='cuda', options=None,
                  tile=DEFAULT_TILE, graphs=True):
@@ -290,37 +298,85 @@ class BatchedIBStep:
         self.solver = BatchedMixedMultigrid(self.grids, device=self.device, options=options,
                                             graphs=graphs)
         self.density = [g.zeros(device=self.device) for g in self.grids]
-        self.rhs = [torch.empty(g.shape, device=self.device, dtype=torch.float64) for g in self.grids]
         self.point_velocity = [torch.empty((c, 3), device=self.device, dtype=torch.float64)
                                for c in self.counts]
         self.spreader = BatchedTransfer(self.grids, self.counts, device=self.device, tile=tile)
         self.gatherer = BatchedTransfer(self.grids, self.counts, device=self.device, tile=tile)
+        n = len(self.grids)
+        self.items = (_abi.IB3BItem * n)()
+        self.ib = _abi.IB3BBatch()
+        self.ib.count = n
+        self.ib.items = ctypes.cast(self.items, ctypes.POINTER(_abi.IB3BItem))
+        self.ib_workspace = None
         self._key = None
 
     def _bind(self, xs, forces, velocities):
         key = tuple(t.data_ptr() for t in (*xs, *forces, *(u for v in velocities for u in v)))
-        if key != self._key:
-            self.spreader.bind(xs, forces, self.density, self.point_velocity)
-            self.gatherer.bind(xs, forces, velocities, self.point_velocity)
-            self.gatherer.share_bins(self.spreader)
-            self._key = key
+        if key == self._key:
+            return
+        self.spreader.bind(xs, forces, self.density, self.point_velocity)
+        self.gatherer.bind(xs, forces, velocities, self.point_velocity)
+        self.gatherer.share_bins(self.spreader)
+        for i, g in enumerate(self.grids):
+            it = self.items[i]
+            for c in range(3):
+                it.velocity[c] = velocities[i][c].data_ptr()
+                it.force[c] = self.density[i][c].data_ptr()
+            it.rhs = self.solver.b[i].data_ptr()
+            it.pressure[0] = self.solver.x[i][0].data_ptr()
+            it.pressure[1] = self.solver.x[i][1].data_ptr()
+            it.x = xs[i].data_ptr()
+            it.point_velocity = self.point_velocity[i].data_ptr()
+            it.points = self.counts[i]
+            it.grid = g.abi()
+        nbytes = int(_abi.load().afsi_ib3b_workspace_bytes(ctypes.byref(self.ib)))
+        if nbytes == 0:
+            _abi.call('afsi_ib3b_prepare', self.ib, _abi.stream())  # raises with the reason
+        if self.ib_workspace is None or self.ib_workspace.numel() < nbytes:
+            self.ib_workspace = torch.empty(nbytes, device=self.device, dtype=torch.uint8)
+        self.ib.workspace = self.ib_workspace.data_ptr()
+        self.ib.workspace_bytes = self.ib_workspace.numel()
+        _abi.call('afsi_ib3b_prepare', self.ib, _abi.stream())
+        self._key = key
+
+    def _initial(self, pressures):
+        """Index of the solver buffer pair holding the initial guesses: the
+        previous solution when ``pressures`` are this stepper's own views of
+        it, else copies of ``pressures`` (zeros for None) in x[i][0]."""
+        sol = self.solver
+        own = pressures is not None and all(
+            p is not None and p.data_ptr() == sol.x[i][sol.cur].data_ptr() for i, p in enumerate(pressures))
+        if own:
+            return sol.cur
+        for i in range(len(self.grids)):
+            p = None if pressures is None else pressures[i]
+            if p is None:
+                sol.x[i][0].zero_()
+            else:
+                if p.shape != sol.x[i][0].shape or p.dtype != torch.float64:
+                    raise ValueError('invalid initial pressure')
+                sol.x[i][0].copy_(p)
+        return 0
 
     @torch.no_grad()
     def step(self, xs, forces, velocities, pressures=None, *, advance=True):
+        """Returns (velocities, pressures, point_velocities, infos). The
+        velocity tensors are updated in place; the pressures are views of
+        the solver's buffers, valid until the next step (pass them back as
+        ``pressures`` to warm-start it without a copy)."""
+        if len(xs) != len(self.grids) or len(forces) != len(self.grids) or len(velocities) != len(self.grids):
+            raise ValueError('one point set, force set and velocity per instance')
         a = self.dt / self.rho
         self._bind(xs, forces, velocities)
+        start = self._initial(pressures)
         self.spreader.bin()
         self.spreader.spread()
-        for i, g in enumerate(self.grids):
-            for uc, fc in zip(velocities[i], self.density[i]):
-                _abi.call('afsi_axpy_f64', _abi.ptr(uc), _abi.ptr(fc), uc.numel(), a, _abi.stream())
-            divergence(velocities[i], g, -self.rho / self.dt, out=self.rhs[i])
-        ps, infos = self.solver.solve(self.rhs, pressures)
-        for i, g in enumerate(self.grids):
-            correct(velocities[i], ps[i], g, a)
+        _abi.call('afsi_ib3b_rhs', self.ib, a, -self.rho / self.dt, _abi.stream())
+        infos = self.solver.solve_resident(start)
+        cur = self.solver.cur
+        _abi.call('afsi_ib3b_correct', self.ib, cur, a, a, _abi.stream())
         self.gatherer.interpolate()
         if advance:
-            for i in range(len(self.grids)):
-                _abi.call('afsi_axpy_f64', _abi.ptr(xs[i]), _abi.ptr(self.point_velocity[i]),
-                          xs[i].numel(), self.dt, _abi.stream())
+            _abi.call('afsi_ib3b_advance', self.ib, self.dt, _abi.stream())
+        ps = [self.solver.x[i][cur] for i in range(len(self.grids))]
         return velocities, ps, self.point_velocity, infos

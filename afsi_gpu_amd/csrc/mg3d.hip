@@ -9,6 +9,7 @@
 // rebuilt on the fly in the order of mac/multigrid.py:38-41 instead of being
 // read from HBM (one stream fewer per sweep).
 #include "mgcommon.hpp"
+#include <vector>
 
 namespace afsi {
 
@@ -221,6 +222,27 @@ __device__ __forceinline__ double face_value(const double* __restrict__ u, const
     return F ? u[i] + a * f[i] : u[i];
 }
 
+// scale * D(u + a f) at cell i (F) or scale * D u (!F).
+template <bool P, bool F>
+__device__ __forceinline__ double divergence_at(const double* __restrict__ u, const double* __restrict__ v,
+                                                const double* __restrict__ w, const Dims& d, double hx,
+                                                double hy, double hz, double scale, const double* __restrict__ fu,
+                                                const double* __restrict__ fv, const double* __restrict__ fw,
+                                                double a, int64_t i) {
+    const int64_t e = P ? 0 : 1;
+    const int64_t z = i % d.n2, y = (i / d.n2) % d.n1, x = i / (d.n1 * d.n2);
+    const int64_t iu = (x * d.n1 + y) * d.n2 + z;
+    const int64_t iu1 = P ? ((x + 1 < d.n0 ? x + 1 : 0) * d.n1 + y) * d.n2 + z : iu + d.n1 * d.n2;
+    const int64_t iv = (x * (d.n1 + e) + y) * d.n2 + z;
+    const int64_t iv1 = P ? (x * d.n1 + (y + 1 < d.n1 ? y + 1 : 0)) * d.n2 + z : iv + d.n2;
+    const int64_t iw = (x * d.n1 + y) * (d.n2 + e) + z;
+    const int64_t iw1 = P ? (x * d.n1 + y) * d.n2 + (z + 1 < d.n2 ? z + 1 : 0) : iw + 1;
+    double s = (face_value<F>(u, fu, a, iu1) - face_value<F>(u, fu, a, iu)) / hx;
+    s = s + (face_value<F>(v, fv, a, iv1) - face_value<F>(v, fv, a, iv)) / hy;
+    s = s + (face_value<F>(w, fw, a, iw1) - face_value<F>(w, fw, a, iw)) / hz;
+    return scale * s;
+}
+
 template <bool P, bool F = false>
 __global__ void __launch_bounds__(256) divergence_kernel(const double* __restrict__ u,
                                                          const double* __restrict__ v,
@@ -233,52 +255,58 @@ __global__ void __launch_bounds__(256) divergence_kernel(const double* __restric
                                                          const double* __restrict__ fw = nullptr,
                                                          double a = 0.0) {
     const int64_t total = d.n0 * d.n1 * d.n2;
-    const int64_t e = P ? 0 : 1;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
-         i += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t z = i % d.n2, y = (i / d.n2) % d.n1, x = i / (d.n1 * d.n2);
-        const int64_t iu = (x * d.n1 + y) * d.n2 + z;
-        const int64_t iu1 = P ? ((x + 1 < d.n0 ? x + 1 : 0) * d.n1 + y) * d.n2 + z : iu + d.n1 * d.n2;
-        const int64_t iv = (x * (d.n1 + e) + y) * d.n2 + z;
-        const int64_t iv1 = P ? (x * d.n1 + (y + 1 < d.n1 ? y + 1 : 0)) * d.n2 + z : iv + d.n2;
-        const int64_t iw = (x * d.n1 + y) * (d.n2 + e) + z;
-        const int64_t iw1 = P ? (x * d.n1 + y) * d.n2 + (z + 1 < d.n2 ? z + 1 : 0) : iw + 1;
-        double s = (face_value<F>(u, fu, a, iu1) - face_value<F>(u, fu, a, iu)) / hx;
-        s = s + (face_value<F>(v, fv, a, iv1) - face_value<F>(v, fv, a, iv)) / hy;
-        s = s + (face_value<F>(w, fw, a, iw1) - face_value<F>(w, fw, a, iw)) / hz;
-        out[i] = scale * s;
-    }
+         i += int64_t(gridDim.x) * blockDim.x)
+        out[i] = divergence_at<P, F>(u, v, w, d, hx, hy, hz, scale, fu, fv, fw, a, i);
 }
 
-// u_C -= coef * (p[i]-p[i-1])/h on interior faces (all faces if periodic).
+// Faces of component C per axis.
+template <int C, bool P>
+__device__ __forceinline__ void face_extents(const Dims& d, int64_t& f0, int64_t& f1, int64_t& f2) {
+    f0 = d.n0 + (C == 0 && !P);
+    f1 = d.n1 + (C == 1 && !P);
+    f2 = d.n2 + (C == 2 && !P);
+}
+
+// Face i of component C: u = u* - coef * (p[q]-p[q-1])/h with u* = u + a f
+// (F) or u (!F), on interior faces (all faces if periodic); boundary faces
+// of a closed box keep u*.
+template <int C, bool P, bool F>
+__device__ __forceinline__ void grad_correct_at(double* __restrict__ u, const double* __restrict__ p,
+                                                const Dims& d, double h, double coef,
+                                                const double* __restrict__ f, double a, int64_t i) {
+    int64_t f0, f1, f2;
+    face_extents<C, P>(d, f0, f1, f2);
+    const int64_t z = i % f2, y = (i / f2) % f1, x = i / (f1 * f2);
+    const int64_t q = C == 0 ? x : (C == 1 ? y : z);
+    const int64_t nq = C == 0 ? d.n0 : (C == 1 ? d.n1 : d.n2);
+    const double star = face_value<F>(u, f, a, i);
+    if (!P && (q == 0 || q == nq)) {
+        if (F) u[i] = star;
+        return;
+    }
+    const int64_t qm = q > 0 ? q - 1 : nq - 1;  // face q lies between cells q-1 and q
+    auto cell = [&](int64_t aa, int64_t b, int64_t c) { return (aa * d.n1 + b) * d.n2 + c; };
+    int64_t hi, lo;
+    if (C == 0) { hi = cell(q, y, z); lo = cell(qm, y, z); }
+    else if (C == 1) { hi = cell(x, q, z); lo = cell(x, qm, z); }
+    else { hi = cell(x, y, q); lo = cell(x, y, qm); }
+    const double g = (p[hi] - p[lo]) / h;
+    u[i] = star - coef * g;
+}
+
 template <int C, bool P, bool F = false>
 __global__ void __launch_bounds__(256) grad_correct_kernel(double* __restrict__ u,
                                                            const double* __restrict__ p, Dims d,
                                                            double h, double coef,
                                                            const double* __restrict__ f = nullptr,
                                                            double a = 0.0) {
-    const int64_t f0 = d.n0 + (C == 0 && !P), f1 = d.n1 + (C == 1 && !P), f2 = d.n2 + (C == 2 && !P);
+    int64_t f0, f1, f2;
+    face_extents<C, P>(d, f0, f1, f2);
     const int64_t total = f0 * f1 * f2;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
-         i += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t z = i % f2, y = (i / f2) % f1, x = i / (f1 * f2);
-        const int64_t q = C == 0 ? x : (C == 1 ? y : z);
-        const int64_t nq = C == 0 ? d.n0 : (C == 1 ? d.n1 : d.n2);
-        const double star = face_value<F>(u, f, a, i);
-        if (!P && (q == 0 || q == nq)) {
-            if (F) u[i] = star;
-            continue;
-        }
-        const int64_t qm = q > 0 ? q - 1 : nq - 1;
-        const int64_t qh = P ? q : q;  // face q lies between cells q-1 and q
-        auto cell = [&](int64_t a, int64_t b, int64_t c) { return (a * d.n1 + b) * d.n2 + c; };
-        int64_t hi, lo;
-        if (C == 0) { hi = cell(qh, y, z); lo = cell(qm, y, z); }
-        else if (C == 1) { hi = cell(x, qh, z); lo = cell(x, qm, z); }
-        else { hi = cell(x, y, qh); lo = cell(x, y, qm); }
-        const double g = (p[hi] - p[lo]) / h;
-        u[i] = star - coef * g;
-    }
+         i += int64_t(gridDim.x) * blockDim.x)
+        grad_correct_at<C, P, F>(u, p, d, h, coef, f, a, i);
 }
 
 __global__ void axpy_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t count,
@@ -286,6 +314,70 @@ __global__ void axpy_kernel(double* __restrict__ y, const double* __restrict__ x
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
          i += int64_t(gridDim.x) * blockDim.x)
         y[i] = y[i] + a * x[i];
+}
+
+// ---- elementwise passes of many 3D IB instances, one launch each ----------
+// Workgroup b of a pass handles elements [(b - begin) * IBB_CHUNK, + CHUNK)
+// of the instance whose range holds b (binary search over the begins); each
+// element is formed exactly as in the single-instance kernels above.
+
+constexpr int IBB_CHUNK = 2048;
+
+struct IbbDesc {
+    double* u[3];
+    const double* f[3];
+    double* rhs;
+    const double* p[2];
+    double* x;
+    const double* U;
+    int64_t cells, faces[3], coords;
+    Dims d;
+    double h[3];
+    int begin[3];  // first workgroup of the rhs, correct and advance passes
+};
+
+__device__ inline int ibb_find(const IbbDesc* __restrict__ desc, int n, int pass, int b) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (desc[mid].begin[pass] <= b) lo = mid;
+        else hi = mid;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+template <bool P>
+__global__ void __launch_bounds__(256) ibb_rhs_kernel(const IbbDesc* __restrict__ desc, int n, double a,
+                                                      double scale) {
+    const IbbDesc& s = desc[ibb_find(desc, n, 0, blockIdx.x)];
+    const int64_t lo = int64_t(int(blockIdx.x) - s.begin[0]) * IBB_CHUNK;
+    const int64_t hi = lo + IBB_CHUNK < s.cells ? lo + IBB_CHUNK : s.cells;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += 256)
+        s.rhs[i] = divergence_at<P, true>(s.u[0], s.u[1], s.u[2], s.d, s.h[0], s.h[1], s.h[2], scale, s.f[0],
+                                          s.f[1], s.f[2], a, i);
+}
+
+template <bool P>
+__global__ void __launch_bounds__(256) ibb_correct_kernel(const IbbDesc* __restrict__ desc, int n, int which,
+                                                          double a, double coef) {
+    const IbbDesc& s = desc[ibb_find(desc, n, 1, blockIdx.x)];
+    const int64_t total = s.faces[0] + s.faces[1] + s.faces[2];
+    const int64_t lo = int64_t(int(blockIdx.x) - s.begin[1]) * IBB_CHUNK;
+    const int64_t hi = lo + IBB_CHUNK < total ? lo + IBB_CHUNK : total;
+    const double* p = s.p[which];
+    for (int64_t j = lo + threadIdx.x; j < hi; j += 256) {
+        if (j < s.faces[0]) grad_correct_at<0, P, true>(s.u[0], p, s.d, s.h[0], coef, s.f[0], a, j);
+        else if (j < s.faces[0] + s.faces[1])
+            grad_correct_at<1, P, true>(s.u[1], p, s.d, s.h[1], coef, s.f[1], a, j - s.faces[0]);
+        else grad_correct_at<2, P, true>(s.u[2], p, s.d, s.h[2], coef, s.f[2], a, j - s.faces[0] - s.faces[1]);
+    }
+}
+
+__global__ void __launch_bounds__(256) ibb_advance_kernel(const IbbDesc* __restrict__ desc, int n, double dt) {
+    const IbbDesc& s = desc[ibb_find(desc, n, 2, blockIdx.x)];
+    const int64_t lo = int64_t(int(blockIdx.x) - s.begin[2]) * IBB_CHUNK;
+    const int64_t hi = lo + IBB_CHUNK < s.coords ? lo + IBB_CHUNK : s.coords;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += 256) s.x[i] = s.x[i] + dt * s.U[i];
 }
 
 // ---- host wrappers ---------------------------------------------------------
@@ -657,6 +749,63 @@ static int plan_cycles(const afsi_mg3d_plan* pl, int32_t cycles, void* stream) {
     return OK;
 }
 
+// Descriptors of a batch of IB instances (host side) and the workgroups of
+// each pass.
+struct IbbLayout {
+    std::vector<IbbDesc> desc;
+    int blocks[3] = {0, 0, 0};
+    int periodic = 0;
+};
+
+static int ibb_layout(const afsi_ib3b_batch* bt, IbbLayout& L) {
+    AFSI_REQUIRE(bt && bt->items && bt->count >= 1, "invalid IB batch");
+    L.desc.resize(size_t(bt->count));
+    L.periodic = bt->items[0].grid.periodic;
+    for (int i = 0; i < bt->count; ++i) {
+        const afsi_ib3b_item& it = bt->items[i];
+        const afsi_grid3& g = it.grid;
+        AFSI_REQUIRE(g.periodic == L.periodic, "instances of one IB batch share the boundary type");
+        AFSI_REQUIRE(dims_ok(g.n) && it.points >= 0, "invalid grid or point count (instance %d)", i);
+        for (int c = 0; c < 3; ++c)
+            AFSI_REQUIRE(it.velocity[c] && it.force[c], "missing face array (instance %d)", i);
+        AFSI_REQUIRE(it.rhs && it.pressure[0] && it.pressure[1], "missing pressure buffers (instance %d)", i);
+        AFSI_REQUIRE(it.points == 0 || (it.x && it.point_velocity), "missing point arrays (instance %d)", i);
+        IbbDesc& s = L.desc[size_t(i)];
+        for (int c = 0; c < 3; ++c) {
+            s.u[c] = it.velocity[c];
+            s.f[c] = it.force[c];
+            s.h[c] = g.h[c];
+        }
+        s.rhs = it.rhs;
+        s.p[0] = it.pressure[0];
+        s.p[1] = it.pressure[1];
+        s.x = it.x;
+        s.U = it.point_velocity;
+        s.d = Dims{g.n[0], g.n[1], g.n[2]};
+        s.cells = g.n[0] * g.n[1] * g.n[2];
+        const int e = g.periodic ? 0 : 1;
+        s.faces[0] = (g.n[0] + e) * g.n[1] * g.n[2];
+        s.faces[1] = g.n[0] * (g.n[1] + e) * g.n[2];
+        s.faces[2] = g.n[0] * g.n[1] * (g.n[2] + e);
+        s.coords = 3 * it.points;
+        const int64_t work[3] = {s.cells, s.faces[0] + s.faces[1] + s.faces[2], s.coords};
+        for (int q = 0; q < 3; ++q) {
+            s.begin[q] = L.blocks[q];
+            const int64_t nb = cdiv(work[q], IBB_CHUNK);
+            AFSI_REQUIRE(int64_t(L.blocks[q]) + nb < (int64_t(1) << 31), "IB batch too large");
+            L.blocks[q] += int(nb);
+        }
+    }
+    return OK;
+}
+
+static int ibb_ready(const afsi_ib3b_batch* bt, IbbLayout& L) {
+    if (int r = ibb_layout(bt, L)) return r;
+    AFSI_REQUIRE(bt->workspace && bt->workspace_bytes >= L.desc.size() * sizeof(IbbDesc),
+                 "IB batch workspace too small");
+    return OK;
+}
+
 template <class T>
 static int plan_residual(const afsi_mg3d_plan* pl, void* stream) {
     if (int r = plan_check<T>(pl)) return r;
@@ -685,5 +834,55 @@ int afsi_mg3d_residual_unfused_f64(const afsi_mg3d_plan* plan, void* stream) {
 }
 int afsi_mg3d_residual_unfused_f32(const afsi_mg3d_plan* plan, void* stream) {
     return afsi::plan_residual<float>(plan, stream);
+}
+
+size_t afsi_ib3b_workspace_bytes(const afsi_ib3b_batch* batch) {
+    afsi::IbbLayout L;
+    return afsi::ibb_layout(batch, L) == afsi::OK ? L.desc.size() * sizeof(afsi::IbbDesc) : 0;
+}
+int afsi_ib3b_prepare(const afsi_ib3b_batch* batch, void* stream) {
+    using namespace afsi;
+    IbbLayout L;
+    if (int r = ibb_ready(batch, L)) return r;
+    const hipStream_t s = hipStream_t(stream);
+    AFSI_HIP(hipMemcpyAsync(batch->workspace, L.desc.data(), L.desc.size() * sizeof(IbbDesc),
+                            hipMemcpyHostToDevice, s));
+    AFSI_HIP(hipStreamSynchronize(s));
+    return OK;
+}
+int afsi_ib3b_rhs(const afsi_ib3b_batch* batch, double a, double scale, void* stream) {
+    using namespace afsi;
+    IbbLayout L;
+    if (int r = ibb_ready(batch, L)) return r;
+    const IbbDesc* d = static_cast<const IbbDesc*>(batch->workspace);
+    const hipStream_t s = hipStream_t(stream);
+    if (L.periodic) ibb_rhs_kernel<true><<<unsigned(L.blocks[0]), 256, 0, s>>>(d, batch->count, a, scale);
+    else ibb_rhs_kernel<false><<<unsigned(L.blocks[0]), 256, 0, s>>>(d, batch->count, a, scale);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+int afsi_ib3b_correct(const afsi_ib3b_batch* batch, int32_t which, double a, double coef, void* stream) {
+    using namespace afsi;
+    IbbLayout L;
+    if (int r = ibb_ready(batch, L)) return r;
+    AFSI_REQUIRE(which == 0 || which == 1, "invalid pressure index");
+    const IbbDesc* d = static_cast<const IbbDesc*>(batch->workspace);
+    const hipStream_t s = hipStream_t(stream);
+    if (L.periodic)
+        ibb_correct_kernel<true><<<unsigned(L.blocks[1]), 256, 0, s>>>(d, batch->count, which, a, coef);
+    else
+        ibb_correct_kernel<false><<<unsigned(L.blocks[1]), 256, 0, s>>>(d, batch->count, which, a, coef);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+int afsi_ib3b_advance(const afsi_ib3b_batch* batch, double dt, void* stream) {
+    using namespace afsi;
+    IbbLayout L;
+    if (int r = ibb_ready(batch, L)) return r;
+    if (L.blocks[2] == 0) return OK;
+    ibb_advance_kernel<<<unsigned(L.blocks[2]), 256, 0, hipStream_t(stream)>>>(
+        static_cast<const IbbDesc*>(batch->workspace), batch->count, dt);
+    AFSI_LAUNCH_CHECK();
+    return OK;
 }
 }  // extern "C"

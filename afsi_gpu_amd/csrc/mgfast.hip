@@ -2156,7 +2156,7 @@ struct BatchLayout {
     size_t restrict_off[AFSI_MG_MAX_LEVELS], prolong_off[AFSI_MG_MAX_LEVELS];
     int restrict_blocks[AFSI_MG_MAX_LEVELS], prolong_blocks[AFSI_MG_MAX_LEVELS];
     size_t coarse_off = 0, coarse_lds = 0;
-    size_t ref_off[3] = {0, 0, 0};     // residual of x[0]; step from x[0]; step from x[1]
+    size_t ref_off[4] = {0, 0, 0, 0};  // residual of x[0]; step from x[0]; step from x[1]; residual of x[1]
     int ref_blocks = 0;
     size_t vec_off[3] = {0, 0, 0};     // b, x[0], x[1]
     int vec_blocks[3] = {0, 0, 0};
@@ -2227,7 +2227,7 @@ static int batch_layout(const afsi_mgb_batch* bt, BatchLayout& L) {
         if (cells * sizeof(float) > L.coarse_lds) L.coarse_lds = cells * sizeof(float);
         L.ref_blocks += int(cdiv(pl.n[0][0] * pl.n[0][1], 4));
     }
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < 4; ++v) {
         L.ref_off[v] = off;
         off = align64(off + size_t(L.n) * sizeof(RefDesc));
     }
@@ -2321,6 +2321,7 @@ static int batch_prepare(const afsi_mgb_batch* bt, hipStream_t s) {
         at<RefDesc>(hw, L.ref_off[0])[i] = RefDesc{it.x[0], e, it.b, it.x[0], r32, nullptr, d0, k0, rbeg};
         at<RefDesc>(hw, L.ref_off[1])[i] = RefDesc{it.x[0], e, it.b, it.x[1], r32, act, d0, k0, rbeg};
         at<RefDesc>(hw, L.ref_off[2])[i] = RefDesc{it.x[1], e, it.b, it.x[0], r32, act, d0, k0, rbeg};
+        at<RefDesc>(hw, L.ref_off[3])[i] = RefDesc{it.x[1], e, it.b, it.x[1], r32, nullptr, d0, k0, rbeg};
         rbeg += int(cdiv(int64_t(d0.n0) * d0.n1, 4));
         const int cells = d0.n0 * d0.n1 * d0.n2;
         at<VecDesc>(hw, L.vec_off[0])[i] = VecDesc{it.b, cells, vbeg};
@@ -2373,11 +2374,12 @@ static int batch_refine(const afsi_mgb_batch* bt, const BatchLayout& L, int whic
                         hipStream_t s) {
     const RefDesc* desc = at<RefDesc>(bt->workspace, L.ref_off[which]);
     const unsigned g = unsigned(L.ref_blocks);
-    if (which == 0) refine_batch_kernel<P, false, true><<<g, 256, 0, s>>>(desc, L.n, bt->partials);
+    const bool residual_only = which == 0 || which == 3;
+    if (residual_only) refine_batch_kernel<P, false, true><<<g, 256, 0, s>>>(desc, L.n, bt->partials);
     else if (reduce) refine_batch_kernel<P, true, true><<<g, 256, 0, s>>>(desc, L.n, bt->partials);
     else refine_batch_kernel<P, true, false><<<g, 256, 0, s>>>(desc, L.n, bt->partials);
     AFSI_LAUNCH_CHECK();
-    if (which == 0 || reduce) {
+    if (residual_only || reduce) {
         batch_reduce_kernel<RefDesc><<<unsigned(L.n), 64, 0, s>>>(desc, L.n, L.ref_blocks, bt->partials,
                                                                   bt->results, 0, bt->tol, bt->active);
         AFSI_LAUNCH_CHECK();
@@ -2479,18 +2481,23 @@ int64_t afsi_mgb_partials_doubles(const afsi_mgb_batch* batch) {
 int afsi_mgb_prepare(const afsi_mgb_batch* batch, void* stream) {
     return afsi::batch_prepare(batch, hipStream_t(stream));
 }
-int afsi_mgb_begin(const afsi_mgb_batch* batch, double rtol, double atol, void* stream) {
+int afsi_mgb_begin_at(const afsi_mgb_batch* batch, int32_t start, double rtol, double atol, void* stream) {
     using namespace afsi;
     BatchLayout L;
     if (int r = batch_check_ws(batch, L)) return r;
+    AFSI_REQUIRE(start == 0 || start == 1, "invalid iterate index");
     const hipStream_t s = hipStream_t(stream);
-    if (int r = batch_vec(batch, L, 0, 4, true, s)) return r;    // b: stats, mean removal
-    if (int r = batch_vec(batch, L, 1, 12, true, s)) return r;   // x[0]: stats, mean removal
-    if (int r = batch_vec(batch, L, 0, 8, false, s)) return r;   // ||b|| after the removal
+    if (int r = batch_vec(batch, L, 0, 4, true, s)) return r;          // b: stats, mean removal
+    if (int r = batch_vec(batch, L, 1 + start, 12, true, s)) return r;  // x[start]: stats, mean removal
+    if (int r = batch_vec(batch, L, 0, 8, false, s)) return r;         // ||b|| after the removal
     batch_begin_kernel<<<unsigned(cdiv(L.n, 256)), 256, 0, s>>>(L.n, batch->results, batch->tol,
                                                                batch->active, rtol, atol);
     AFSI_LAUNCH_CHECK();
-    return L.periodic ? batch_refine<true>(batch, L, 0, true, s) : batch_refine<false>(batch, L, 0, true, s);
+    const int which = start == 0 ? 0 : 3;  // residual of x[start]
+    return L.periodic ? batch_refine<true>(batch, L, which, true, s) : batch_refine<false>(batch, L, which, true, s);
+}
+int afsi_mgb_begin(const afsi_mgb_batch* batch, double rtol, double atol, void* stream) {
+    return afsi_mgb_begin_at(batch, 0, rtol, atol, stream);
 }
 int afsi_mgb_iterate(const afsi_mgb_batch* batch, int32_t start, int32_t count, void* stream) {
     using namespace afsi;

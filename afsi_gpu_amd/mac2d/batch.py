@@ -106,6 +106,15 @@ class BatchedChannelStep:
     def pressure(self, i):
         return self.solvers[i].p[0]
 
+    def reset(self):
+        """Fluid at rest: zero face velocities and pressures (the next solve
+        starts from p = 0)."""
+        self.u.zero_()
+        self.v.zero_()
+        for sol in self.solvers:
+            sol.p[0].zero_()
+        self.cycles = [0] * len(self.grids)
+
     def _stats(self, which):
         _abi.call('afsi_chb_stats_f64', byref(self.batch), which, _abi.ptr(self.stats), _abi.stream())
 

@@ -419,12 +419,50 @@ int afsi_mgb_prepare(const afsi_mgb_batch* batch, void* stream);
  * stats of b_i before the mean removal, results[16i+12..] = stats of x_i[0]
  * before its mean removal, active_i = residual > tol_i. */
 int afsi_mgb_begin(const afsi_mgb_batch* batch, double rtol, double atol, void* stream);
+/* The same from x_i[start] (start 0 or 1): a solve that continues from the
+ * iterate the previous one ended on, with no copy. */
+int afsi_mgb_begin_at(const afsi_mgb_batch* batch, int32_t start, double rtol, double atol, void* stream);
 /* `count` outer iterations from x[start] (the iterate alternates between
  * x[0] and x[1]); the last one reduces the residual and updates active.
  * Inactive instances copy their iterate so the alternation stays aligned. */
 int afsi_mgb_iterate(const afsi_mgb_batch* batch, int32_t start, int32_t count, void* stream);
 /* x_i[cur] loses its mean (FP64). */
 int afsi_mgb_finish(const afsi_mgb_batch* batch, int32_t cur, void* stream);
+
+/* ---- elementwise passes of a batch of 3D IB instances ---------------------
+ * Build-own (no single reference counterpart): the per-instance projection
+ * pieces of mac/flow.py:102-121 with the IB predictor folded in
+ * (afsi_project_rhs_3d_f64 / afsi_project_correct_3d_f64, element for
+ * element the same expressions) and the point advance x += dt U, each one
+ * launch over all instances. All instances share the boundary type. */
+typedef struct afsi_ib3b_item {
+    double* velocity[3];          /* face arrays, updated in place */
+    const double* force[3];       /* spread face force densities */
+    double* rhs;                  /* pressure right-hand side (n cells) */
+    const double* pressure[2];    /* the two pressure buffers correct() can read */
+    double* x;                    /* (points, 3) positions */
+    const double* point_velocity; /* (points, 3) */
+    int64_t points;
+    afsi_grid3 grid;
+} afsi_ib3b_item;
+typedef struct afsi_ib3b_batch {
+    int32_t count;
+    int32_t reserved;
+    const afsi_ib3b_item* items;  /* host array of count items */
+    void* workspace;              /* device, afsi_ib3b_workspace_bytes */
+    size_t workspace_bytes;
+} afsi_ib3b_batch;
+size_t afsi_ib3b_workspace_bytes(const afsi_ib3b_batch* batch);
+/* Copies the descriptors to the workspace (synchronises the stream); call
+ * again whenever an item's pointers change. */
+int afsi_ib3b_prepare(const afsi_ib3b_batch* batch, void* stream);
+/* rhs_i = scale * D(u_i + a f_i), u_i unchanged. */
+int afsi_ib3b_rhs(const afsi_ib3b_batch* batch, double a, double scale, void* stream);
+/* u_i = (u_i + a f_i) - coef * G pressure_i[which] (closed boxes: boundary
+ * faces keep u + a f). */
+int afsi_ib3b_correct(const afsi_ib3b_batch* batch, int32_t which, double a, double coef, void* stream);
+/* x_i += dt * point_velocity_i. */
+int afsi_ib3b_advance(const afsi_ib3b_batch* batch, double dt, void* stream);
 
 /* ==== 2D staggered channel (mac2d) ========================================
  * Cells nx x ny, cell (x, y) at x*ny + y; faces u (nx+1, ny), v (nx, ny+1).
