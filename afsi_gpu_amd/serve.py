@@ -6,15 +6,27 @@ resident in HBM across the instance's requests and is released after its
 last one. Admission is against an HBM budget: an instance's first request
 waits until its estimated footprint fits.
 
-Continuous batching at step granularity: every GPU iteration takes all
-requests that have arrived and whose instance is resident and idle (at
-most one per instance, in arrival order) and runs them together; requests
-that arrive meanwhile join the next iteration. The 3D requests of an
-iteration share one batched mixed-precision pressure solve (one launch per
-multigrid phase over all of them); their transfers and the 2D channel
-steps are issued on the same stream. Latency = completion - arrival.
+Three host threads keep the device busy:
+* the arrival thread (the caller's) queues requests as they arrive and
+  admits new instances: allocation, setup and upload run on a stream of
+  their own, so they overlap the steps in flight; an instance becomes
+  eligible once its upload event has completed;
+* the executor takes every eligible request (at most one per instance, in
+  arrival order) and launches them as one iteration: the 3D steps share
+  one batched pressure solve (mixed precision: one launch per multigrid
+  phase over all of them; plain FP64: one solve per instance), the 2D
+  channel steps run on one stepper per channel shape. It does not wait for
+  the iteration to finish: it records an event and takes the next batch
+  (an instance's next step queues behind its previous one on the stream);
+* the completer polls the events of iterations in flight and stamps their
+  requests complete (latency = completion - arrival), reads each request's
+  divergence ratio ||D u|| / ||D u*|| and releases finished instances.
+The solves still read their convergence flags on the host, so the executor
+itself waits inside a solve; everything else overlaps.
 """
+from collections import deque
 from dataclasses import dataclass, field
+import threading
 import time
 
 import numpy as np
@@ -24,8 +36,10 @@ from . import _abi, workloads
 from .batch import BatchedMixedMultigrid, MixedState
 from .mac import MACGrid, MGOptions
 from .mac.flow import correct, divergence
+from .mac.multigrid import GeometricMultigrid
 from .mac.transfer import DEFAULT_TILE, Binner, TiledTransfer
 from .mac2d import ChannelGrid, ChannelIBStep
+from .mac2d.flow import divergence as divergence2d
 
 HBM_BUDGET = 288 * 2 ** 30
 
@@ -46,20 +60,37 @@ def footprint(shape, count):
 
 
 class Instance3D:
-    def __init__(self, shape, x, force, *, device, dt, options, tile=DEFAULT_TILE):
+    """A resident 3D instance. ``solver`` is its own ``MixedState`` (mixed)
+    or a plain FP64 ``GeometricMultigrid`` shared by the instances of its
+    shape (fp64); the tiled transfer is shared per shape as well."""
+
+    def __init__(self, shape, x, force, *, device, dt, options, precision='mixed', shared=None,
+                 tile=DEFAULT_TILE):
         self.grid = MACGrid(tuple(shape), (1., 1., 1.), periodic=True)
         self.dt = dt
+        self.precision = precision
         self.x = torch.tensor(x, device=device)
         self.force = torch.tensor(force, device=device)
         self.velocity = list(self.grid.zeros(device=device))
         self.pressure = None
-        self.state = MixedState(self.grid, device=device, options=options)
+        shared = shared if shared is not None else {}
+        if precision == 'mixed':
+            self.state = MixedState(self.grid, device=device, options=options)
+            self.plain = None
+        else:
+            self.state = None
+            self.plain = shared.get('plain') or GeometricMultigrid(self.grid, device=device, options=options,
+                                                                   backend='workspace')
+            shared['plain'] = self.plain
+        self.transfer = shared.get('transfer') or TiledTransfer(self.grid, device, tile)
+        shared['transfer'] = self.transfer
         self.binner = Binner(self.grid, len(x), device, tile)
-        self.transfer = TiledTransfer(self.grid, device, tile)
         self.density = self.grid.zeros(device=device)
         self.rhs = torch.empty(self.grid.shape, device=device, dtype=torch.float64)
+        self.div = torch.empty(self.grid.shape, device=device, dtype=torch.float64)
         self.U = torch.empty_like(self.x)
         self.bins = None
+        self.ready = None
 
     def before_solve(self):
         self.bins = self.binner(self.x)
@@ -69,25 +100,41 @@ class Instance3D:
         return divergence(self.velocity, self.grid, -1 / self.dt, out=self.rhs)
 
     def after_solve(self, p):
+        """Correction, interpolation and advance; returns the device scalar
+        ||D u|| / ||D u*|| of this step."""
         correct(self.velocity, p, self.grid, self.dt)
         self.pressure = p
         U = self.transfer.interpolate(self.x, self.velocity, self.bins, out=self.U)
         _abi.call('afsi_axpy_f64', _abi.ptr(self.x), _abi.ptr(U), self.x.numel(), self.dt, _abi.stream())
+        after = torch.linalg.vector_norm(divergence(self.velocity, self.grid, out=self.div))
+        return after / (torch.linalg.vector_norm(self.rhs) * self.dt)
 
 
 class Instance2D:
-    def __init__(self, shape, x, force, *, device, dt, options):
+    """A resident 2D channel; the stepper (fields, multigrid workspace) is
+    shared by the instances of its shape, the state passed per step."""
+
+    def __init__(self, shape, x, force, *, device, dt, options, shared=None):
         self.grid = ChannelGrid(tuple(shape), (4., 1.), periodic=True)
-        self.stepper = ChannelIBStep(self.grid, dt=dt, device=device, options=options,
-                                     pressure_backend='workspace')
+        shared = shared if shared is not None else {}
+        self.stepper = shared.get('channel') or ChannelIBStep(self.grid, dt=dt, device=device, options=options,
+                                                              pressure_backend='workspace')
+        shared['channel'] = self.stepper
+        self.dt = dt
         self.x = torch.tensor(x, device=device)
         self.force = torch.tensor(force, device=device)
         self.velocity = self.grid.zeros(device=device)
         self.pressure = None
+        self.div = torch.empty(self.grid.shape, device=device, dtype=torch.float64)
+        self.ready = None
 
     def step(self):
+        """One step; returns the device scalar ||D u|| / ||D u*||."""
         self.velocity, self.pressure, _, _ = self.stepper.step(self.x, self.force, self.velocity,
                                                                self.pressure)
+        before = torch.linalg.vector_norm(self.stepper.flow._rhs) * self.dt
+        after = torch.linalg.vector_norm(divergence2d(self.velocity, self.grid, out=self.div))
+        return after / before
 
 
 @dataclass
@@ -100,47 +147,69 @@ class Stats:
     waits_for_memory: int = 0
     wall: float = 0.
     admit_seconds: list = field(default_factory=list)  # instance creation (allocation, setup, upload)
-    execute_seconds: list = field(default_factory=list)  # one batched GPU iteration
-    waits: list = field(default_factory=list)  # arrival to the start of the request's iteration
+    launch_seconds: list = field(default_factory=list)  # host time to launch one iteration
+    waits: list = field(default_factory=list)  # arrival to the launch of the request's iteration
+    divergence: list = field(default_factory=list)  # ||D u|| / ||D u*|| per request
+    kinds: list = field(default_factory=list)  # '2d' / '3d' / '128' per request
 
     def summary(self):
         lat = np.asarray(self.latencies) * 1e3
-        return dict(requests=len(lat), p50_ms=float(np.percentile(lat, 50)) if len(lat) else None,
-                    p99_ms=float(np.percentile(lat, 99)) if len(lat) else None,
-                    mean_ms=float(lat.mean()) if len(lat) else None,
-                    throughput_rps=len(lat) / self.wall if self.wall else None,
-                    iterations=self.iterations,
-                    mean_batch=float(np.mean(self.batch_sizes)) if self.batch_sizes else 0.,
-                    max_batch=int(max(self.batch_sizes)) if self.batch_sizes else 0,
-                    instances=self.admitted, peak_resident_gib=self.peak_bytes / 2 ** 30,
-                    memory_waits=self.waits_for_memory, wall_s=self.wall,
-                    admit_ms_mean=1e3 * float(np.mean(self.admit_seconds)) if self.admit_seconds else 0.,
-                    admit_ms_max=1e3 * float(max(self.admit_seconds)) if self.admit_seconds else 0.,
-                    execute_ms_p50=1e3 * float(np.percentile(self.execute_seconds, 50)) if self.execute_seconds else 0.,
-                    execute_ms_p99=1e3 * float(np.percentile(self.execute_seconds, 99)) if self.execute_seconds else 0.,
-                    wait_ms_p99=1e3 * float(np.percentile(self.waits, 99)) if self.waits else 0.)
+        div = np.asarray(self.divergence)
+        out = dict(requests=len(lat), p50_ms=float(np.percentile(lat, 50)) if len(lat) else None,
+                   p99_ms=float(np.percentile(lat, 99)) if len(lat) else None,
+                   mean_ms=float(lat.mean()) if len(lat) else None,
+                   throughput_rps=len(lat) / self.wall if self.wall else None,
+                   iterations=self.iterations,
+                   mean_batch=float(np.mean(self.batch_sizes)) if self.batch_sizes else 0.,
+                   max_batch=int(max(self.batch_sizes)) if self.batch_sizes else 0,
+                   instances=self.admitted, peak_resident_gib=self.peak_bytes / 2 ** 30,
+                   memory_waits=self.waits_for_memory, wall_s=self.wall,
+                   admit_ms_mean=1e3 * float(np.mean(self.admit_seconds)) if self.admit_seconds else 0.,
+                   admit_ms_max=1e3 * float(max(self.admit_seconds)) if self.admit_seconds else 0.,
+                   launch_ms_p50=1e3 * float(np.percentile(self.launch_seconds, 50)) if self.launch_seconds else 0.,
+                   launch_ms_p99=1e3 * float(np.percentile(self.launch_seconds, 99)) if self.launch_seconds else 0.,
+                   wait_ms_p99=1e3 * float(np.percentile(self.waits, 99)) if self.waits else 0.,
+                   divergence_max=float(div.max()) if len(div) else None,
+                   divergence_p99=float(np.percentile(div, 99)) if len(div) else None)
+        if len(lat) and self.kinds:
+            kinds = np.asarray(self.kinds)
+            out['p99_ms_by_kind'] = {k: float(np.percentile(lat[kinds == k], 99)) for k in np.unique(kinds)}
+        return out
+
+
+def _kind(shape):
+    return '2d' if len(shape) == 2 else ('128' if shape[0] >= 128 else '3d')
 
 
 class Server:
     """Runs a request stream (arrival times, instance ids, instance specs)
-    against the GPU in real time."""
+    against the GPU in real time. ``precision``: 'mixed' (batched FP32
+    V-cycles in FP64 refinement for the 3D instances) or 'fp64' (plain FP64
+    multigrid per 3D instance)."""
 
     def __init__(self, *, device='cuda', dt=1e-3, budget=HBM_BUDGET, options=None,
-                 max_batch=64):
+                 max_batch=64, precision='mixed'):
+        if precision not in ('mixed', 'fp64'):
+            raise ValueError("precision must be 'mixed' or 'fp64'")
         self.device = torch.device(device)
         self.dt = dt
         self.budget = budget
         self.options = options or MGOptions(smooth=2, smoother='rbgs')
         self.max_batch = max_batch
+        self.precision = precision
         self.resident = {}
         self.resident_bytes = {}
+        self.shared = {}          # per shape: steppers, plain solvers, transfers
+        self.admit_stream = torch.cuda.Stream(device=self.device)
 
     def _create(self, spec, data):
-        shape = spec[0]
+        shape = tuple(spec[0])
         x, F = data
+        shared = self.shared.setdefault(shape, {})
         if len(shape) == 3:
-            return Instance3D(shape, x, F, device=self.device, dt=self.dt, options=self.options)
-        return Instance2D(shape, x, F, device=self.device, dt=self.dt, options=self.options)
+            return Instance3D(shape, x, F, device=self.device, dt=self.dt, options=self.options,
+                              precision=self.precision, shared=shared)
+        return Instance2D(shape, x, F, device=self.device, dt=self.dt, options=self.options, shared=shared)
 
     def snapshot(self, path):
         """Write every resident instance's state (points, forces, face
@@ -171,18 +240,25 @@ class Server:
         return sorted(instances)
 
     def _execute(self, batch):
-        three = [inst for inst in batch if isinstance(inst, Instance3D)]
-        rhs = [inst.before_solve() for inst in three]
-        for inst in batch:
+        """Launch one iteration on the current stream; returns the (n,)
+        device tensor of the requests' divergence ratios (no host wait
+        beyond the solves' own convergence checks)."""
+        ratios = [None] * len(batch)
+        three = [(j, inst) for j, inst in enumerate(batch) if isinstance(inst, Instance3D)]
+        rhs = [inst.before_solve() for _, inst in three]
+        for j, inst in enumerate(batch):
             if isinstance(inst, Instance2D):
-                inst.step()
+                ratios[j] = inst.step()
         if three:
-            solver = BatchedMixedMultigrid(states=[inst.state for inst in three],
-                                           options=self.options, device=self.device, graphs=False)
-            ps, _ = solver.solve(rhs, [inst.pressure for inst in three])
-            for inst, p in zip(three, ps):
-                inst.after_solve(p)
-        torch.cuda.synchronize(self.device)
+            if self.precision == 'mixed':
+                solver = BatchedMixedMultigrid(states=[inst.state for _, inst in three],
+                                               options=self.options, device=self.device, graphs=False)
+                ps, _ = solver.solve(rhs, [inst.pressure for _, inst in three])
+            else:
+                ps = [inst.plain.solve(r, inst.pressure)[0] for (_, inst), r in zip(three, rhs)]
+            for (j, inst), p in zip(three, ps):
+                ratios[j] = inst.after_solve(p)
+        return torch.stack(ratios)
 
     def run(self, times, inst_ids, specs, *, keep=(), clock=time.perf_counter):
         """Serve every request; returns Stats. Instances in ``keep`` stay
@@ -196,56 +272,139 @@ class Server:
         order = np.argsort(times, kind='stable')
         times, inst_ids = np.asarray(times)[order], np.asarray(inst_ids)[order]
         remaining = {int(i): int(c) for i, c in zip(*np.unique(inst_ids, return_counts=True))}
-        pending = []            # (arrival, instance id)
-        nxt = 0
-        used = 0
+        lock = threading.Lock()
+        eligible = deque()      # (arrival, instance id), instance resident
+        inflight = deque()      # (done event, [(arrival, iid, launch time)], ratios)
+        shared = dict(used=0, arrived=0, launched=0, error=None)
+        total = len(times)
         t0 = clock()
-        while nxt < len(times) or pending:
-            now = clock() - t0
-            while nxt < len(times) and times[nxt] <= now:
-                pending.append((float(times[nxt]), int(inst_ids[nxt])))
-                nxt += 1
-            if not pending:
-                time.sleep(max(0., min(times[nxt] - now, 0.002)))
-                continue
-            batch, chosen, busy = [], [], set()
-            for j, (arr, iid) in enumerate(pending):
-                if iid in busy or len(batch) >= self.max_batch:
-                    continue
-                if iid not in self.resident:
-                    need = footprint(specs[iid][0], specs[iid][1])
-                    if used + need > self.budget:
-                        stats.waits_for_memory += 1
-                        busy.add(iid)
+        main_stream = torch.cuda.current_stream(self.device)
+
+        def fail(exc):
+            with lock:
+                if shared['error'] is None:
+                    shared['error'] = exc
+
+        def executor():
+            try:
+                with torch.cuda.stream(main_stream):
+                    while True:
+                        with lock:
+                            if shared['error'] is not None:
+                                return
+                            if shared['launched'] == total:
+                                return
+                            batch, reqs, busy, keep_q = [], [], set(), deque()
+                            while eligible:
+                                arr, iid = eligible.popleft()
+                                inst = self.resident[iid]
+                                ok = iid not in busy and len(batch) < self.max_batch and (
+                                    inst.ready is None or inst.ready.query())
+                                if ok:
+                                    busy.add(iid)
+                                    batch.append(inst)
+                                    reqs.append((arr, iid))
+                                else:
+                                    keep_q.append((arr, iid))
+                            eligible.extendleft(reversed(keep_q))
+                        if not batch:
+                            time.sleep(0.0002)
+                            continue
+                        start = clock() - t0
+                        ratios = self._execute(batch)
+                        host = torch.empty(ratios.shape, dtype=ratios.dtype, pin_memory=True)
+                        host.copy_(ratios, non_blocking=True)
+                        done = torch.cuda.Event()
+                        done.record(main_stream)
+                        stats.launch_seconds.append(clock() - t0 - start)
+                        stats.iterations += 1
+                        stats.batch_sizes.append(len(batch))
+                        with lock:
+                            inflight.append((done, [(a, i, start) for a, i in reqs], host))
+                            shared['launched'] += len(batch)
+            except BaseException as exc:  # surfaced by run()
+                fail(exc)
+
+        def completer():
+            try:
+                finished = 0
+                while finished < total:
+                    with lock:
+                        if shared['error'] is not None:
+                            return
+                        head = inflight[0] if inflight else None
+                    if head is None or not head[0].query():
+                        time.sleep(0.0002)
                         continue
-                    a0 = clock()
-                    self.resident[iid] = self._create(specs[iid], data.pop(iid))
-                    stats.admit_seconds.append(clock() - a0)
-                    self.resident_bytes[iid] = need
-                    used += need
-                    stats.admitted += 1
-                    stats.peak_bytes = max(stats.peak_bytes, used)
-                busy.add(iid)
-                batch.append(self.resident[iid])
-                chosen.append(j)
-            if not batch:
-                time.sleep(0.0005)
-                continue
-            start = clock() - t0
-            self._execute(batch)
-            done = clock() - t0
-            stats.execute_seconds.append(done - start)
-            stats.iterations += 1
-            stats.batch_sizes.append(len(batch))
-            for j in chosen:
-                arr, iid = pending[j]
-                stats.latencies.append(done - arr)
-                stats.waits.append(start - arr)
-                remaining[iid] -= 1
-                if remaining[iid] == 0 and iid not in keep:
-                    del self.resident[iid]
-                    used -= self.resident_bytes.pop(iid)
-            for j in sorted(chosen, reverse=True):
-                pending.pop(j)
+                    now = clock() - t0
+                    done, reqs, host = head
+                    values = host.tolist()
+                    with lock:
+                        inflight.popleft()
+                        for (arr, iid, start), ratio in zip(reqs, values):
+                            stats.latencies.append(now - arr)
+                            stats.waits.append(start - arr)
+                            stats.divergence.append(ratio)
+                            stats.kinds.append(_kind(specs[iid][0]))
+                            remaining[iid] -= 1
+                            if remaining[iid] == 0 and iid not in keep:
+                                del self.resident[iid]
+                                shared['used'] -= self.resident_bytes.pop(iid)
+                        finished += len(reqs)
+            except BaseException as exc:
+                fail(exc)
+
+        threads = [threading.Thread(target=executor, daemon=True), threading.Thread(target=completer, daemon=True)]
+        for th in threads:
+            th.start()
+        waiting = deque()       # arrived, instance not yet admitted (budget)
+        nxt = 0
+        try:
+            while nxt < total or waiting:
+                with lock:
+                    if shared['error'] is not None:
+                        break
+                now = clock() - t0
+                while nxt < total and times[nxt] <= now:
+                    waiting.append((float(times[nxt]), int(inst_ids[nxt])))
+                    nxt += 1
+                admitted_any = False
+                held = deque()
+                while waiting:
+                    arr, iid = waiting.popleft()
+                    with lock:
+                        resident = iid in self.resident
+                        used = shared['used']
+                    if not resident:
+                        need = footprint(specs[iid][0], specs[iid][1])
+                        if used + need > self.budget:
+                            stats.waits_for_memory += 1
+                            held.append((arr, iid))
+                            continue
+                        a0 = clock()
+                        with torch.cuda.stream(self.admit_stream):
+                            inst = self._create(specs[iid], data.pop(iid))
+                            inst.ready = torch.cuda.Event()
+                            inst.ready.record(self.admit_stream)
+                        stats.admit_seconds.append(clock() - a0)
+                        with lock:
+                            self.resident[iid] = inst
+                            self.resident_bytes[iid] = need
+                            shared['used'] += need
+                            stats.peak_bytes = max(stats.peak_bytes, shared['used'])
+                        stats.admitted += 1
+                        admitted_any = True
+                    with lock:
+                        eligible.append((arr, iid))
+                waiting.extend(held)
+                if not admitted_any:
+                    wake = times[nxt] - (clock() - t0) if nxt < total else 0.0005
+                    time.sleep(max(0., min(wake, 0.0005)))
+        finally:
+            for th in threads:
+                th.join()
         stats.wall = clock() - t0
+        if shared['error'] is not None:
+            raise shared['error']
+        torch.cuda.synchronize(self.device)
         return stats

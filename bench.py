@@ -67,7 +67,7 @@ def parse():
     ap.add_argument('--no-dropin', action='store_true', help='skip the reference-API drop-in sample')
     ap.add_argument('--kernels-only', type=int, default=0, metavar='R',
                     help='only launch the roofline sweep, the transfers and a copy kernel R times each')
-    ap.add_argument('--serve-seconds', type=float, default=5.,
+    ap.add_argument('--serve-seconds', type=float, default=60.,
                     help='arrival window of the served-latency sample (0: skip)')
     ap.add_argument('--batch-instances', type=int, default=64,
                     help='instances of the batched sample (0: skip)')
@@ -216,29 +216,37 @@ def serving_sample(torch, seconds, rate=200.):
     """Config 5 request stream (Poisson arrivals at `rate`/s; 60% 2D
     128x32..512x128 with 2k-20k points, 30% 3D 32^3-64^3 with 20k-200k,
     10% 128^3 with 0.5M-1.5M; 10 requests per instance) over a window of
-    `seconds`, served with continuous batching; latency percentiles."""
+    `seconds`, served with continuous batching (admission, launch and
+    completion overlapped on separate host threads), once with the mixed
+    batched 3D solve and once with plain FP64 per-instance solves: latency
+    percentiles and every request's divergence ratio ||D u|| / ||D u*||."""
     from afsi_gpu_amd import workloads
     from afsi_gpu_amd.serve import Server
     times, inst, specs = workloads.request_stream(workloads.GEOMETRY_SEED + 5, rate=rate,
                                                   duration=seconds)
     warm_t, warm_i, warm_s = workloads.request_stream(workloads.GEOMETRY_SEED + 6, rate=rate,
                                                       duration=.2)
-    Server(device='cuda:0').run(warm_t, warm_i, warm_s)  # compile/cache warm-up
-    with paused_gc():
-        stats = Server(device='cuda:0').run(times, inst, specs)
-    out = stats.summary()
+    out = {}
+    for precision in ('mixed', 'fp64'):
+        Server(device='cuda:0', precision=precision).run(warm_t, warm_i, warm_s)  # compile/cache warm-up
+        with paused_gc():
+            stats = Server(device='cuda:0', precision=precision).run(times, inst, specs)
+        out[precision] = stats.summary()
+        print(f'serving {precision}: {out[precision]["requests"]} requests, p99 {out[precision]["p99_ms"]:.2f} ms',
+              file=sys.stderr, flush=True)
     out.update(rate_per_s=rate, window_s=seconds,
                sample=f'{len(times)} requests over {seconds:g} s of the 60 s stream, '
-                      f'{len(specs)} instances')
+                      f'{len(specs)} instances, each precision')
     return out
 
 
 def fe_sample(torch, n=128, level=7, steps=10, warmup=2):
     """FE interaction-point steps (FEIBStep, mixed-precision pressure): a P1
     sphere of radius 0.3 (icosahedron subdivided `level` times, three
-    quadrature points per triangle) in an n^3 periodic unit cube, random
-    nodal forces of unit surface density; consistent-mass Jacobi PCG on both transfers (rtol 1e-12,
-    warm started)."""
+    quadrature points per triangle) in an n^3 periodic unit cube, nodal
+    forces of unit surface density turning between two random fields
+    (F_k = F0 cos 0.3k + F1 sin 0.3k); consistent-mass Jacobi PCG on both
+    transfers (rtol 1e-12, warm started)."""
     from afsi_gpu_amd import workloads
     from afsi_gpu_amd.mac import MACGrid, MGOptions
     from afsi_gpu_amd.mac.fe import FEGeometry
@@ -251,12 +259,17 @@ def fe_sample(torch, n=128, level=7, steps=10, warmup=2):
     stepper = FEIBStep(grid, geometry, dt=1e-3, device=device, options=MGOptions(smooth=2, smoother='rbgs'),
                        precision='mixed', warm_start=True)
     x = t(nodes)
-    # O(1) surface force density: nodal forces scale with the nodal area
-    F = t(np.random.default_rng(7).normal(size=nodes.shape) * (weights.sum() / len(nodes)))
+    # O(1) surface force density: nodal forces scale with the nodal area. The
+    # force turns between two random fields from step to step, so each
+    # warm-started mass solve has a new right-hand side to work on.
+    gen = np.random.default_rng(7)
+    scale = weights.sum() / len(nodes)
+    F0, F1 = (t(gen.normal(size=nodes.shape) * scale) for _ in range(2))
+    forces = [F0 * np.cos(.3 * k) + F1 * np.sin(.3 * k) for k in range(warmup + steps)]
     u = grid.zeros(device=device)
     p = None
-    for _ in range(warmup):
-        x, u, p, U, info = stepper.step(x, F, u, p)
+    for k in range(warmup):
+        x, u, p, U, info = stepper.step(x, forces[k], u, p)
     timers = [{ph: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                for ph in PHASES} for _ in range(steps)]
     iters = []
@@ -264,7 +277,7 @@ def fe_sample(torch, n=128, level=7, steps=10, warmup=2):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(steps):
-            x, u, p, U, info = stepper.step(x, F, u, p, timers=timers[k])
+            x, u, p, U, info = stepper.step(x, forces[warmup + k], u, p, timers=timers[k])
             iters.append((info['force_mass'].iterations, info['velocity_mass'].iterations))
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
