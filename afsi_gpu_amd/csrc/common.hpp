@@ -32,6 +32,10 @@ void set_error(const char* fmt, ...);
 // Check the launch that was just issued.
 #define AFSI_LAUNCH_CHECK() AFSI_HIP(hipGetLastError())
 
+// Raise kernel fn's dynamic-LDS limit to what the device allows (once per
+// kernel and device); E_LIMIT when `need` bytes do not fit.
+int dynamic_lds(const void* fn, size_t need);
+
 constexpr int WAVE = 64;
 
 // Device error bits written by kernels into a caller-provided int word.

@@ -1789,7 +1789,7 @@ __global__ void batch_begin_kernel(int n, const double* __restrict__ results, do
 // Dynamic LDS a kernel may request: the device's per-workgroup LDS minus the
 // kernel's static LDS. The first call per (kernel, device) raises the
 // kernel's dynamic-LDS attribute to that amount; `need` is checked against it.
-static int dynamic_lds(const void* fn, size_t need) {
+int dynamic_lds(const void* fn, size_t need) {
     struct Entry { const void* fn; int dev; size_t avail; };
     static std::mutex lock;
     static Entry cache[64];
