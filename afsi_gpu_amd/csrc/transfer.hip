@@ -291,29 +291,113 @@ __global__ void link_keys_kernel(const int64_t* __restrict__ ids, int64_t links,
     }
 }
 
+constexpr int SEG_SPAN = 512;  // sorted links per wavefront in the segmented sums
+
+// Per-node sums over the links sorted by node, as a segmented reduction of
+// the flat sorted list (so a node with thousands of links is summed by many
+// lanes, and every lane has work whatever the distribution). Wavefront gw
+// owns links [gw * SEG_SPAN, +SEG_SPAN); it walks them 64 at a time with a
+// segmented inclusive scan (Hillis-Steele over lanes, heads where the node
+// changes) plus a carry between steps. A node whose links lie inside one
+// span is written by that wavefront; the first and last node of a span
+// that continue across its ends leave partial sums (head, tail), which
+// seg_combine_kernel adds in span order. Links with key = size (out of range)
+// are skipped. Every sum is formed in a fixed order, so the result is
+// deterministic.
+__device__ inline bool seg_single(const uint32_t* keys, int64_t links, int64_t gw, uint32_t& first,
+                                  uint32_t& last, bool& from_prev, bool& cont_next) {
+    const int64_t j0 = gw * SEG_SPAN, j1 = j0 + SEG_SPAN < links ? j0 + SEG_SPAN : links;
+    first = keys[j0];
+    last = keys[j1 - 1];
+    from_prev = j0 > 0 && keys[j0 - 1] == first;
+    cont_next = j1 < links && keys[j1] == last;
+    return first == last;
+}
+
 template <class T>
-__global__ void __launch_bounds__(256) node_sum_kernel(const T* __restrict__ field, const T* __restrict__ w,
-                                                       const int32_t* __restrict__ perm,
-                                                       const int32_t* __restrict__ ptr, int64_t size,
-                                                       int64_t neighbors, int64_t comps, double volume,
-                                                       T* __restrict__ out) {
+__global__ void __launch_bounds__(256) seg_sum_kernel(const T* __restrict__ field, const T* __restrict__ w,
+                                                      const uint32_t* __restrict__ keys,
+                                                      const int32_t* __restrict__ perm, int64_t links,
+                                                      int64_t size, int64_t neighbors, int64_t comps, int64_t c,
+                                                      double volume, T* __restrict__ out,
+                                                      double* __restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int64_t gw = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (gw * SEG_SPAN >= links) return;  // wave-uniform
     const T vol = T(volume);
-    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < size * comps;
-         t += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t node = t / comps, c = t % comps;
-        T acc = T(0);
-        for (int32_t j = ptr[node]; j < ptr[node + 1]; ++j) {
+    uint32_t first, last;
+    bool from_prev, cont_next;
+    seg_single(keys, links, gw, first, last, from_prev, cont_next);
+    const int64_t j0 = gw * SEG_SPAN, j1 = j0 + SEG_SPAN < links ? j0 + SEG_SPAN : links;
+    T carry = T(0);
+    for (int64_t b = j0; b < j1; b += 64) {
+        const int64_t j = b + lane;
+        const bool in = j < j1;
+        const uint32_t k = in ? keys[j] : 0xffffffffu;
+        T v = T(0);
+        if (in && k < uint32_t(size)) {
             const int64_t l = perm[j];
-            acc += (field[(l / neighbors) * comps + c] * w[l]) / vol;
+            v = (field[(l / neighbors) * comps + c] * w[l]) / vol;
         }
-        out[t] = acc;
+        // head: the node starts here (within the span; the carry covers the
+        // segment continuing from the previous step or span)
+        const bool head = in && (j == j0 || k != keys[j - 1]);
+        T sum = v;
+        int h = head ? 1 : 0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const T up = __shfl_up(sum, d, 64);
+            const int hup = __shfl_up(h, d, 64);
+            if (lane >= d) {
+                if (!h) sum = up + sum;
+                h = h | hup;
+            }
+        }
+        if (!h) sum = carry + sum;  // no head from the step start to here: continues the carry
+        const bool end = in && (j == j1 - 1 || k != keys[j + 1]);
+        if (end && k < uint32_t(size)) {
+            const bool is_first = k == first && from_prev;
+            const bool is_last = k == last && cont_next;
+            if (is_first) part[2 * gw] = double(sum);
+            if (is_last) part[2 * gw + 1] = double(sum);
+            if (!is_first && !is_last) out[int64_t(k) * comps + c] = sum;
+        }
+        // carry into the next step: lane 63's running sum unless its node ends there
+        const T s63 = __shfl(sum, 63, 64);
+        const bool e63 = __shfl(end ? 1 : 0, 63, 64) != 0;
+        carry = e63 ? T(0) : s63;
+    }
+}
+
+// Nodes whose links cross span ends: the span where the node starts adds its
+// tail and the heads of the following spans, in order.
+template <class T>
+__global__ void __launch_bounds__(256) seg_combine_kernel(const uint32_t* __restrict__ keys, int64_t links,
+                                                          int64_t size, int64_t comps, int64_t c,
+                                                          const double* __restrict__ part, T* __restrict__ out) {
+    const int64_t spans = cdiv(links, SEG_SPAN);
+    for (int64_t gw = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; gw < spans;
+         gw += int64_t(gridDim.x) * blockDim.x) {
+        uint32_t first, last;
+        bool from_prev, cont_next;
+        const bool single = seg_single(keys, links, gw, first, last, from_prev, cont_next);
+        if (!cont_next || (single && from_prev) || last >= uint32_t(size)) continue;
+        T total = T(part[2 * gw + 1]);
+        for (int64_t k = gw + 1; k < spans; ++k) {
+            uint32_t f2, l2;
+            bool p2, n2;
+            const bool s2 = seg_single(keys, links, k, f2, l2, p2, n2);
+            total = total + T(part[2 * k]);
+            if (!(s2 && n2)) break;
+        }
+        out[int64_t(last) * comps + c] = total;
     }
 }
 
 // Workspace of a sort-based spread over n items and `space` targets: the
 // radix sort's scratch, then sorted keys, permutation and CSR offsets.
 struct SpreadSpace {
-    size_t sort_bytes, keys_off, perm_off, ptr_off, total, start_off;
+    size_t sort_bytes, keys_off, perm_off, ptr_off, total, start_off, part_off;
 };
 
 static size_t alignb(size_t b) { return (b + 255) & ~size_t(255); }
@@ -327,6 +411,8 @@ static SpreadSpace spread_space(int64_t n, int64_t space) {
     w.total = w.ptr_off + alignb(size_t(space + 1) * 4);
     w.start_off = w.total;
     w.total += alignb(size_t(space / CP_TV + 1) * 4);
+    w.part_off = w.total;  // head and tail partial sums of each SEG_SPAN span
+    w.total += alignb(size_t(2 * cdiv(n > 0 ? n : 1, SEG_SPAN)) * sizeof(double));
     return w;
 }
 
@@ -426,9 +512,17 @@ static int indexed_spread(const T* field, int64_t points, int64_t comps, const i
     int32_t* perm = reinterpret_cast<int32_t*>(ws + L.perm_off);
     int32_t* ptr = reinterpret_cast<int32_t*>(ws + L.ptr_off);
     if (int r = sort_pairs(links, size, ws, keys, perm, ptr, s)) return r;
-    node_sum_kernel<T><<<grid_for(size * comps, 256, 16384), 256, 0, s>>>(field, w, perm, ptr, size, neighbors,
-                                                                          comps, volume, out);
-    AFSI_LAUNCH_CHECK();
+    // nodes without links stay 0; every other node is written exactly once
+    if (int r = launch_fill<T>(out, size * comps, T(0), s)) return r;
+    double* part = reinterpret_cast<double*>(ws + L.part_off);
+    const int64_t spans = cdiv(links, SEG_SPAN);
+    for (int64_t c = 0; c < comps; ++c) {
+        seg_sum_kernel<T><<<unsigned(cdiv(spans, 4)), 256, 0, s>>>(field, w, keys, perm, links, size, neighbors,
+                                                                  comps, c, volume, out, part);
+        AFSI_LAUNCH_CHECK();
+        seg_combine_kernel<T><<<grid_for(spans, 256, 4096), 256, 0, s>>>(keys, links, size, comps, c, part, out);
+        AFSI_LAUNCH_CHECK();
+    }
     return OK;
 }
 

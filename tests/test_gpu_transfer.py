@@ -103,6 +103,37 @@ def test_indexed_signed_duplicate_weights(gpu, dtype, neighbors, components):
     assert rel_err(den.cpu(), ib3d.indexed_spread(f, ids, w, 53, v)) < tol
 
 
+@pytest.mark.parametrize('components', [1, 3])
+def test_indexed_spread_skewed_nodes(gpu, components):
+    """Nodes that receive thousands of links (spanning several of the
+    segmented sum's 512-link spans), nodes with one link and nodes with
+    none, plus out-of-range ids (skipped, flagged): against a direct sum."""
+    gen = np.random.default_rng(41)
+    size, points, neighbors = 300, 9000, 16
+    hot = gen.integers(0, size, size=3)
+    ids = np.where(gen.random((points, neighbors)) < .4, hot[gen.integers(0, 3, size=(points, neighbors))],
+                   gen.integers(0, size, size=(points, neighbors)))
+    ids[gen.random(ids.shape) < .01] = size + 5        # out of range
+    ids[:, :][ids == 7] = 8                            # node 7 gets no links
+    w = gen.normal(size=ids.shape)
+    f = gen.normal(size=(points, components))
+    v = .031
+    valid = ids < size
+    want = np.zeros((size, components))
+    for c in range(components):
+        contrib = (f[:, c:c + 1] * w) / v
+        want[:, c] = np.bincount(ids[valid], weights=contrib[valid], minlength=size)
+    assert np.bincount(ids[valid], minlength=size).max() > 2 * 512
+    out = cuda_ib.indexed_spread(torch.tensor(f, device=gpu), torch.tensor(ids, device=gpu),
+                                 torch.tensor(w, device=gpu), size, v, check=False)
+    got = out.cpu().numpy()
+    assert got[7].tolist() == [0.0] * components
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    with pytest.raises(RuntimeError, match='IB index out of range'):
+        cuda_ib.indexed_spread(torch.tensor(f, device=gpu), torch.tensor(ids, device=gpu),
+                               torch.tensor(w, device=gpu), size, v, check=True)
+
+
 def test_indexed_empty_and_rejects_cpu(gpu):
     f = torch.ones((0, 1), device=gpu, dtype=torch.float64)
     ids = torch.zeros((0, 2), dtype=torch.int64, device=gpu)
