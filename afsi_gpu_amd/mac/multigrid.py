@@ -213,13 +213,16 @@ class GeometricMultigrid:
         return self.results[0, 1].sqrt()
 
     @torch.no_grad()
-    def solve(self, rhs, initial=None, *, rtol=None, history=None, precheck=None):
+    def solve(self, rhs, initial=None, *, rtol=None, history=None, precheck=None, view=False):
         """mac/multigrid.py:125-172 (workspace branch :75-98). ``history``, if
         a list, receives the level-0 residual norm after every check;
         ``precheck``, if given, is called right after the first host
         synchronisation (before any cycle and before the RHS checks), so a
         caller can read device flags it copied to pinned memory earlier on
-        the stream without a synchronisation of its own."""
+        the stream without a synchronisation of its own. ``rhs`` or
+        ``initial`` that already are the solver's own level-0 buffers are not
+        copied; ``view=True`` returns the solver's level-0 iterate itself
+        (valid until the next solve) instead of a copy."""
         rtol = self.options.rtol if rtol is None else rtol
         if not isfinite(rtol) or rtol < 0:
             raise ValueError('invalid multigrid solve tolerance')
@@ -229,10 +232,11 @@ class GeometricMultigrid:
                                     or initial.dtype != rhs.dtype):
             raise ValueError('invalid initial pressure')
         w_p, w_rhs = self.p[0], self.rhs[0]
-        w_rhs.copy_(rhs)
+        if rhs.data_ptr() != w_rhs.data_ptr():
+            w_rhs.copy_(rhs)
         if initial is None:
             w_p.zero_()
-        else:
+        elif initial.data_ptr() != w_p.data_ptr():
             w_p.copy_(initial)
         # _start_metrics (mac/multigrid.py:68-73) with one host synchronisation.
         self._reduce(w_rhs, 1)
@@ -273,8 +277,8 @@ class GeometricMultigrid:
         if residual > tolerance:
             raise RuntimeError(f'pressure multigrid did not converge: {residual:g} > {tolerance:g}')
         self._materialize()
-        return w_p.clone(), dict(cycles=cycle, residual_norm=residual, tolerance=tolerance,
-                                 backend=self.backend)
+        return (w_p if view else w_p.clone()), dict(cycles=cycle, residual_norm=residual, tolerance=tolerance,
+                                                    backend=self.backend)
 
 
 class MixedMultigrid:
@@ -396,10 +400,11 @@ class MixedMultigrid:
                   _abi.stream())
 
     @torch.no_grad()
-    def solve(self, rhs, initial=None, *, rtol=None, history=None, precheck=None):
-        """Same contract as ``GeometricMultigrid.solve`` (``precheck``
-        included); ``info['cycles']`` counts outer iterations (one FP32
-        V-cycle each)."""
+    def solve(self, rhs, initial=None, *, rtol=None, history=None, precheck=None, view=False):
+        """Same contract as ``GeometricMultigrid.solve`` (``precheck`` and
+        ``view`` included; ``b`` and either FP64 iterate count as the
+        solver's own buffers); ``info['cycles']`` counts outer iterations
+        (one FP32 V-cycle each)."""
         rtol = self.options.rtol if rtol is None else rtol
         if not isfinite(rtol) or rtol < 0:
             raise ValueError('invalid multigrid solve tolerance')
@@ -408,11 +413,13 @@ class MixedMultigrid:
         if initial is not None and (initial.shape != rhs.shape or initial.device != rhs.device
                                     or initial.dtype != rhs.dtype):
             raise ValueError('invalid initial pressure')
-        self.b.copy_(rhs)
-        self.cur = 0
+        if rhs.data_ptr() != self.b.data_ptr():
+            self.b.copy_(rhs)
+        own = [k for k in (0, 1) if initial is not None and initial.data_ptr() == self.xs[k].data_ptr()]
+        self.cur = own[0] if own else 0
         if initial is None:
             self.x.zero_()
-        else:
+        elif not own:
             self.x.copy_(initial)
         self._reduce(self.b, 1)
         self._reduce(self.x, 2)
@@ -460,5 +467,5 @@ class MixedMultigrid:
         self.last_cycles = cycle
         self._reduce(self.x, 4)
         self._sub_mean(self.x, 4)
-        return self.x.clone(), dict(cycles=cycle, residual_norm=residual, tolerance=tolerance,
-                                    backend=self.backend)
+        return (self.x if view else self.x.clone()), dict(cycles=cycle, residual_norm=residual,
+                                                          tolerance=tolerance, backend=self.backend)

@@ -23,8 +23,13 @@ from .mac.transfer import DEFAULT_TILE, Binner, TiledTransfer, raise_flags
 
 class IBStep:
     def __init__(self, grid, points, *, dt, rho=1., device='cuda', options=None,
-                 tile=DEFAULT_TILE, pressure_backend='graph', precision='fp64'):
+                 tile=DEFAULT_TILE, pressure_backend='graph', precision='fp64', resident_pressure=False):
+        """``resident_pressure``: the projection's right-hand side is formed
+        in the solver's own buffer and the returned pressure is the solver's
+        iterate (valid until the next step; passed back as ``pressure`` it
+        warm-starts the next solve without a copy)."""
         self.grid, self.dt, self.rho = grid, float(dt), float(rho)
+        self.resident = bool(resident_pressure)
         self.device = torch.device(device)
         self.count = int(points)
         self.options = options or MGOptions(smooth=2, smoother='rbgs')
@@ -58,8 +63,13 @@ class IBStep:
         mark('spread', 1)
         mark('project', 0)
         a = self.dt / self.rho
-        rhs = project_rhs(velocity, f, a, self.grid, -self.rho / self.dt, out=self.rhs)
-        p, info = self.flow.pressure_solver.solve(rhs, pressure, precheck=self._check_flags)
+        solver = self.flow.pressure_solver
+        out = self.rhs
+        if self.resident:
+            out = solver.b if hasattr(solver, 'b') else solver.rhs[0]
+        rhs = project_rhs(velocity, f, a, self.grid, -self.rho / self.dt, out=out)
+        self.last_rhs = rhs
+        p, info = solver.solve(rhs, pressure, precheck=self._check_flags, view=self.resident)
         project_correct(velocity, f, a, p, self.grid, a)
         mark('project', 1)
         mark('interp', 0)

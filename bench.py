@@ -187,7 +187,7 @@ def one_step(torch, stepper, x0, F):
     x = x0.clone()
     u = stepper.grid.zeros(device=x.device)
     u, p, U, info = stepper.step(x, F, u, None)
-    tentative = float(torch.linalg.vector_norm(stepper.rhs)) * stepper.dt / stepper.rho
+    tentative = float(torch.linalg.vector_norm(stepper.last_rhs)) * stepper.dt / stepper.rho
     after = float(torch.linalg.vector_norm(divergence(u, stepper.grid)))
     return U.cpu(), tuple(c.cpu() for c in u), tentative, after
 
@@ -645,7 +645,8 @@ def main():
     results = {}
     steppers = {}
     for precision in ('fp64', 'mixed'):
-        stepper = IBStep(grid, P, dt=dt, device=device, options=opts, precision=precision)
+        stepper = IBStep(grid, P, dt=dt, device=device, options=opts, precision=precision,
+                         resident_pressure=True)
         steppers[precision] = stepper
         results[precision] = run_path(torch, stepper, x0, F, args.steps, args.warmup)
     # accuracy of both paths against the FP64 oracle (same inputs, one step from rest)
