@@ -353,6 +353,13 @@ int ch_prolong(const T* coarse, T* fine, const int64_t n[2], int periodic, hipSt
 // sum is taken in the workgroup's own fixed order.
 constexpr int CHB_THREADS = 1024;
 constexpr int64_t CHB_MAX_CELLS = 256 * 64;  // measured crossover: 2.05 vs 3.02 ms at 256x64, 6.4 vs 3.6 ms at 512x128
+// A thread takes its cells of a phase in groups of CHB_ILP: every load of a
+// group is issued before any of its stores, so the global-memory latencies
+// of the group overlap (the compiler cannot move a load above a store to
+// another array it may alias). Within a phase no cell reads what another
+// cell of the phase writes (a colour reads only the other colour), so the
+// values are those of the one-cell-at-a-time loop.
+constexpr int CHB_ILP = 4;
 
 template <class T>
 __device__ void chb_remove_mean(T* x, int n, double* red) {
@@ -382,20 +389,44 @@ __device__ void chb_smooth(const afsi_mg2d_plan& pl, int l, double* red) {
             for (int color = 0; color < 2; ++color) {
                 if (d.ny % 2 == 0) {
                     const int hy = d.ny / 2, half = d.nx * hy;
-                    for (int e = threadIdx.x; e < half; e += CHB_THREADS) {
-                        const int x = e / hy, y = 2 * (e - x * hy) + ((x + color) & 1);
-                        const int i = x * d.ny + y;
-                        T c;
-                        const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
-                        p[i] = c + r / ch_diag<T, P>(x, y, d, k);
+                    for (int e0 = threadIdx.x; e0 < half; e0 += CHB_THREADS * CHB_ILP) {
+                        T val[CHB_ILP];
+                        int at[CHB_ILP];
+#pragma unroll
+                        for (int g = 0; g < CHB_ILP; ++g) {
+                            const int e = e0 + g * CHB_THREADS;
+                            at[g] = -1;
+                            if (e >= half) continue;
+                            const int x = e / hy, y = 2 * (e - x * hy) + ((x + color) & 1);
+                            const int i = x * d.ny + y;
+                            T c;
+                            const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
+                            val[g] = c + r / ch_diag<T, P>(x, y, d, k);
+                            at[g] = i;
+                        }
+#pragma unroll
+                        for (int g = 0; g < CHB_ILP; ++g)
+                            if (at[g] >= 0) p[at[g]] = val[g];
                     }
                 } else {
-                    for (int i = threadIdx.x; i < n; i += CHB_THREADS) {
-                        const int x = i / d.ny, y = i - x * d.ny;
-                        if (((x + y) & 1) != color) continue;
-                        T c;
-                        const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
-                        p[i] = c + r / ch_diag<T, P>(x, y, d, k);
+                    for (int i0 = threadIdx.x; i0 < n; i0 += CHB_THREADS * CHB_ILP) {
+                        T val[CHB_ILP];
+                        int at[CHB_ILP];
+#pragma unroll
+                        for (int g = 0; g < CHB_ILP; ++g) {
+                            const int i = i0 + g * CHB_THREADS;
+                            at[g] = -1;
+                            if (i >= n) continue;
+                            const int x = i / d.ny, y = i - x * d.ny;
+                            if (((x + y) & 1) != color) continue;
+                            T c;
+                            const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
+                            val[g] = c + r / ch_diag<T, P>(x, y, d, k);
+                            at[g] = i;
+                        }
+#pragma unroll
+                        for (int g = 0; g < CHB_ILP; ++g)
+                            if (at[g] >= 0) p[at[g]] = val[g];
                     }
                 }
                 __syncthreads();
@@ -405,11 +436,20 @@ __device__ void chb_smooth(const afsi_mg2d_plan& pl, int l, double* red) {
         T* b = static_cast<T*>(pl.tmp[l]);
         const T omega = T(pl.omega);
         for (int it = 0; it < pl.nu; ++it) {
-            for (int i = threadIdx.x; i < n; i += CHB_THREADS) {
-                const int x = i / d.ny, y = i - x * d.ny;
-                T c;
-                const T r = rhs[i] - ch_apply<T, P>(a, x, y, d, k, c);
-                b[i] = c + (omega * r) / ch_diag<T, P>(x, y, d, k);
+            for (int i0 = threadIdx.x; i0 < n; i0 += CHB_THREADS * CHB_ILP) {
+                T val[CHB_ILP];
+#pragma unroll
+                for (int g = 0; g < CHB_ILP; ++g) {
+                    const int i = i0 + g * CHB_THREADS;
+                    if (i >= n) continue;
+                    const int x = i / d.ny, y = i - x * d.ny;
+                    T c;
+                    const T r = rhs[i] - ch_apply<T, P>(a, x, y, d, k, c);
+                    val[g] = c + (omega * r) / ch_diag<T, P>(x, y, d, k);
+                }
+#pragma unroll
+                for (int g = 0; g < CHB_ILP; ++g)
+                    if (i0 + g * CHB_THREADS < n) b[i0 + g * CHB_THREADS] = val[g];
             }
             __syncthreads();
             T* t = a; a = b; b = t;
@@ -435,15 +475,24 @@ __device__ void chb_cycles(const afsi_mg2d_plan& pl, int cycles, double* red) {
             const T* rhs = static_cast<const T*>(pl.rhs[l]);
             T* coarse = static_cast<T*>(pl.rhs[l + 1]);
             const int cy = d.ny / 2, nc = (d.nx / 2) * cy;
-            for (int i = threadIdx.x; i < nc; i += CHB_THREADS) {
-                const int X = i / cy, Y = i - X * cy;
-                const int x = 2 * X, y = 2 * Y, j = x * d.ny + y;
-                T c;
-                const T r00 = rhs[j] - ch_apply<T, P>(p, x, y, d, k, c);
-                const T r01 = rhs[j + 1] - ch_apply<T, P>(p, x, y + 1, d, k, c);
-                const T r10 = rhs[j + d.ny] - ch_apply<T, P>(p, x + 1, y, d, k, c);
-                const T r11 = rhs[j + d.ny + 1] - ch_apply<T, P>(p, x + 1, y + 1, d, k, c);
-                coarse[i] = (((r00 + r01) + r10) + r11) * T(0.25);
+            for (int i0 = threadIdx.x; i0 < nc; i0 += CHB_THREADS * CHB_ILP) {
+                T val[CHB_ILP];
+#pragma unroll
+                for (int g = 0; g < CHB_ILP; ++g) {
+                    const int i = i0 + g * CHB_THREADS;
+                    if (i >= nc) continue;
+                    const int X = i / cy, Y = i - X * cy;
+                    const int x = 2 * X, y = 2 * Y, j = x * d.ny + y;
+                    T c;
+                    const T r00 = rhs[j] - ch_apply<T, P>(p, x, y, d, k, c);
+                    const T r01 = rhs[j + 1] - ch_apply<T, P>(p, x, y + 1, d, k, c);
+                    const T r10 = rhs[j + d.ny] - ch_apply<T, P>(p, x + 1, y, d, k, c);
+                    const T r11 = rhs[j + d.ny + 1] - ch_apply<T, P>(p, x + 1, y + 1, d, k, c);
+                    val[g] = (((r00 + r01) + r10) + r11) * T(0.25);
+                }
+#pragma unroll
+                for (int g = 0; g < CHB_ILP; ++g)
+                    if (i0 + g * CHB_THREADS < nc) coarse[i0 + g * CHB_THREADS] = val[g];
             }
             __syncthreads();
             if (P) chb_remove_mean(coarse, nc, red);
@@ -471,16 +520,25 @@ __device__ void chb_cycles(const afsi_mg2d_plan& pl, int cycles, double* red) {
             const T* coarse = static_cast<const T*>(pl.p[l + 1]);
             T* fine = static_cast<T*>(pl.p[l]);
             const int n = d.nx * d.ny, NX = d.nx / 2, NY = d.ny / 2;
-            for (int i = threadIdx.x; i < n; i += CHB_THREADS) {
-                const int x = i / d.ny, y = i - x * d.ny;
-                const int ax = x / 2 - 1 + (x & 1), ay = y / 2 - 1 + (y & 1);
-                const T wx = (x & 1) ? T(0.25) : T(0.75), wy = (y & 1) ? T(0.25) : T(0.75);
-                const T c00 = ch_coarse_value<T, P>(coarse, ax, ay, NX, NY);
-                const T c01 = ch_coarse_value<T, P>(coarse, ax, ay + 1, NX, NY);
-                const T c10 = ch_coarse_value<T, P>(coarse, ax + 1, ay, NX, NY);
-                const T c11 = ch_coarse_value<T, P>(coarse, ax + 1, ay + 1, NX, NY);
-                const T v = (c00 * (T(1) - wy) + c01 * wy) * (T(1) - wx) + (c10 * (T(1) - wy) + c11 * wy) * wx;
-                fine[i] = fine[i] + v;
+            for (int i0 = threadIdx.x; i0 < n; i0 += CHB_THREADS * CHB_ILP) {
+                T val[CHB_ILP];
+#pragma unroll
+                for (int g = 0; g < CHB_ILP; ++g) {
+                    const int i = i0 + g * CHB_THREADS;
+                    if (i >= n) continue;
+                    const int x = i / d.ny, y = i - x * d.ny;
+                    const int ax = x / 2 - 1 + (x & 1), ay = y / 2 - 1 + (y & 1);
+                    const T wx = (x & 1) ? T(0.25) : T(0.75), wy = (y & 1) ? T(0.25) : T(0.75);
+                    const T c00 = ch_coarse_value<T, P>(coarse, ax, ay, NX, NY);
+                    const T c01 = ch_coarse_value<T, P>(coarse, ax, ay + 1, NX, NY);
+                    const T c10 = ch_coarse_value<T, P>(coarse, ax + 1, ay, NX, NY);
+                    const T c11 = ch_coarse_value<T, P>(coarse, ax + 1, ay + 1, NX, NY);
+                    const T v = (c00 * (T(1) - wy) + c01 * wy) * (T(1) - wx) + (c10 * (T(1) - wy) + c11 * wy) * wx;
+                    val[g] = fine[i] + v;
+                }
+#pragma unroll
+                for (int g = 0; g < CHB_ILP; ++g)
+                    if (i0 + g * CHB_THREADS < n) fine[i0 + g * CHB_THREADS] = val[g];
             }
             __syncthreads();
             chb_smooth<T, P>(pl, l, red);
@@ -895,13 +953,16 @@ __global__ void __launch_bounds__(CH_THREADS) chb_correct_kernel(const ChbDesc* 
 }
 
 // One workgroup per active instance: `cycles` V-cycles (ch_block_cycles).
+// One instantiation per boundary type (each holds one inlined V-cycle, so
+// the register budget of 1024 threads is not shared by two); workgroups of
+// the other type return at once.
+template <bool P>
 __global__ void __launch_bounds__(CHB_THREADS) chb_cycles_kernel(const afsi_mg2d_plan* __restrict__ plans,
                                                                  const int32_t* __restrict__ active, int cycles) {
     __shared__ double red[CHB_THREADS];
     const afsi_mg2d_plan& pl = plans[blockIdx.x];
-    if (active && !active[blockIdx.x]) return;
-    if (pl.periodic) chb_cycles<double, true>(pl, cycles, red);
-    else chb_cycles<double, false>(pl, cycles, red);
+    if ((active && !active[blockIdx.x]) || (pl.periodic != 0) != P) return;
+    chb_cycles<double, P>(pl, cycles, red);
 }
 
 // One workgroup per instance: [sum, sum of squares, sum |.|, non-finite
@@ -1092,8 +1153,11 @@ int afsi_chb_cycles_f64(const afsi_chb_batch* batch, const int32_t* active, int3
     if (int r = chb_ready(batch, L)) return r;
     AFSI_REQUIRE(cycles >= 0, "negative cycle count");
     if (!cycles) return OK;
-    chb_cycles_kernel<<<unsigned(L.n), CHB_THREADS, 0, hipStream_t(stream)>>>(
-        chb_at<afsi_mg2d_plan>(batch, L.plans), active, cycles);
+    bool periodic = false, closed = false;
+    for (int i = 0; i < batch->count; ++i) (batch->items[i].plan.periodic ? periodic : closed) = true;
+    const afsi_mg2d_plan* plans = chb_at<afsi_mg2d_plan>(batch, L.plans);
+    if (periodic) chb_cycles_kernel<true><<<unsigned(L.n), CHB_THREADS, 0, hipStream_t(stream)>>>(plans, active, cycles);
+    if (closed) chb_cycles_kernel<false><<<unsigned(L.n), CHB_THREADS, 0, hipStream_t(stream)>>>(plans, active, cycles);
     AFSI_LAUNCH_CHECK();
     return OK;
 }
