@@ -31,22 +31,48 @@ template <class T>
 struct Coef2 {
     T hx2, hy2;    // h^2 (operator divisions)
     T ihx2, ihy2;  // 1/h^2 (diagonal)
+    T rx, ry;      // exact 1/h^2 when h^2 is a power of two, else 0
+    __host__ __device__ bool exact() const { return rx != T(0) && ry != T(0); }
 };
+
+// 1/h when h is a positive power of two between 2^-100 and 2^100 (normal
+// in FP32 and FP64 alike, and so is 1/h), else 0. For such h, a / h and
+// a * (1/h) are the correctly rounded values of the same real number, so
+// they are equal for every a: the operator's divisions by h^2 become
+// multiplications without changing a bit.
+__host__ __device__ inline double exact_recip(double h) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, h);
+    const int64_t e = int64_t((b >> 52) & 0x7FF);
+    if ((b & ((uint64_t(1) << 52) - 1)) != 0 || (b >> 63) != 0 || e < 1023 - 100 || e > 1023 + 100) return 0.0;
+    return __builtin_bit_cast(double, uint64_t(2046 - e) << 52);
+}
+
+template <class T>
+__host__ __device__ inline Coef2<T> coef2_of(double hx2, double hy2, double ihx2, double ihy2) {
+    return Coef2<T>{T(hx2), T(hy2), T(ihx2), T(ihy2), T(exact_recip(hx2)), T(exact_recip(hy2))};
+}
+
+// a / h2, or a * (1/h2) when M (h2 a power of two: the same value).
+template <bool M, class T>
+__device__ inline T ch_quot(T a, T h2, T r) {
+    if constexpr (M) return a * r;
+    else return a / h2;
+}
 
 // -A p at cell (x, y), mac2d/_triton_mg.py:7-21 order: -(right diff)
 // + (left diff) per axis, then the outlet term.
-template <class T, bool P>
+template <class T, bool P, bool M = false>
 __device__ inline T ch_apply(const T* __restrict__ p, int x, int y, Dim2 d, const Coef2<T>& k,
                              T& centre) {
     const int i = x * d.ny + y;
     const T c = p[i];
     centre = c;
     T a = T(0);
-    if (P || x + 1 < d.nx) a -= (p[x + 1 < d.nx ? i + d.ny : y] - c) / k.hx2;
-    if (P || x > 0) a += (c - p[x > 0 ? i - d.ny : (d.nx - 1) * d.ny + y]) / k.hx2;
-    if (y + 1 < d.ny) a -= (p[i + 1] - c) / k.hy2;
-    if (y > 0) a += (c - p[i - 1]) / k.hy2;
-    if (!P && x == d.nx - 1) a += T(2) * c / k.hx2;
+    if (P || x + 1 < d.nx) a -= ch_quot<M>(p[x + 1 < d.nx ? i + d.ny : y] - c, k.hx2, k.rx);
+    if (P || x > 0) a += ch_quot<M>(c - p[x > 0 ? i - d.ny : (d.nx - 1) * d.ny + y], k.hx2, k.rx);
+    if (y + 1 < d.ny) a -= ch_quot<M>(p[i + 1] - c, k.hy2, k.ry);
+    if (y > 0) a += ch_quot<M>(c - p[i - 1], k.hy2, k.ry);
+    if (!P && x == d.nx - 1) a += ch_quot<M>(T(2) * c, k.hx2, k.rx);
     return a;
 }
 
@@ -63,7 +89,7 @@ __device__ inline T ch_diag(int x, int y, Dim2 d, const Coef2<T>& k) {
 }
 
 // Mode 0: out = rhs - A p. Mode 1: out = p + (omega*(rhs - A p))/diag.
-template <class T, bool P, int Mode>
+template <class T, bool P, int Mode, bool M>
 __global__ void __launch_bounds__(CH_THREADS) ch_stencil_kernel(const T* __restrict__ p,
                                                                 const T* __restrict__ rhs,
                                                                 T* __restrict__ out, Dim2 d,
@@ -72,7 +98,7 @@ __global__ void __launch_bounds__(CH_THREADS) ch_stencil_kernel(const T* __restr
     for (int i = blockIdx.x * CH_THREADS + threadIdx.x; i < n; i += gridDim.x * CH_THREADS) {
         const int x = i / d.ny, y = i - x * d.ny;
         T c;
-        const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
+        const T r = rhs[i] - ch_apply<T, P, M>(p, x, y, d, k, c);
         out[i] = Mode == 0 ? r : c + (omega * r) / ch_diag<T, P>(x, y, d, k);
     }
 }
@@ -80,7 +106,7 @@ __global__ void __launch_bounds__(CH_THREADS) ch_stencil_kernel(const T* __restr
 // One colour of a red-black Gauss-Seidel sweep, in place: cells with
 // (x + y) % 2 == color get p += (rhs - A p)/diag (their neighbours have the
 // other colour, so no cell read here is written here).
-template <class T, bool P>
+template <class T, bool P, bool M>
 __global__ void __launch_bounds__(CH_THREADS) ch_rbgs_kernel(T* __restrict__ p,
                                                              const T* __restrict__ rhs, Dim2 d,
                                                              Coef2<T> k, int color) {
@@ -90,13 +116,13 @@ __global__ void __launch_bounds__(CH_THREADS) ch_rbgs_kernel(T* __restrict__ p,
         const int x = e / hy, y = 2 * (e - x * hy) + ((x + color) & 1);
         const int i = x * d.ny + y;
         T c;
-        const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
+        const T r = rhs[i] - ch_apply<T, P, M>(p, x, y, d, k, c);
         p[i] = c + r / ch_diag<T, P>(x, y, d, k);
     }
 }
 
 // Fallback colour pass for odd ny: one thread per cell.
-template <class T, bool P>
+template <class T, bool P, bool M>
 __global__ void __launch_bounds__(CH_THREADS) ch_rbgs_any_kernel(T* __restrict__ p,
                                                                  const T* __restrict__ rhs,
                                                                  Dim2 d, Coef2<T> k, int color) {
@@ -105,14 +131,14 @@ __global__ void __launch_bounds__(CH_THREADS) ch_rbgs_any_kernel(T* __restrict__
         const int x = i / d.ny, y = i - x * d.ny;
         if (((x + y) & 1) != color) continue;
         T c;
-        const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
+        const T r = rhs[i] - ch_apply<T, P, M>(p, x, y, d, k, c);
         p[i] = c + r / ch_diag<T, P>(x, y, d, k);
     }
 }
 
 // coarse = (((r00 + r01) + r10) + r11) * 0.25 with r = rhs - A p on the fine
 // level (mac2d/_triton_mg.py:40-51); d is the fine level.
-template <class T, bool P>
+template <class T, bool P, bool M>
 __global__ void __launch_bounds__(CH_THREADS) ch_residual_restrict_kernel(
     const T* __restrict__ p, const T* __restrict__ rhs, T* __restrict__ coarse, Dim2 d,
     Coef2<T> k) {
@@ -121,10 +147,10 @@ __global__ void __launch_bounds__(CH_THREADS) ch_residual_restrict_kernel(
         const int X = i / cy, Y = i - X * cy;
         const int x = 2 * X, y = 2 * Y, j = x * d.ny + y;
         T c;
-        const T r00 = rhs[j] - ch_apply<T, P>(p, x, y, d, k, c);
-        const T r01 = rhs[j + 1] - ch_apply<T, P>(p, x, y + 1, d, k, c);
-        const T r10 = rhs[j + d.ny] - ch_apply<T, P>(p, x + 1, y, d, k, c);
-        const T r11 = rhs[j + d.ny + 1] - ch_apply<T, P>(p, x + 1, y + 1, d, k, c);
+        const T r00 = rhs[j] - ch_apply<T, P, M>(p, x, y, d, k, c);
+        const T r01 = rhs[j + 1] - ch_apply<T, P, M>(p, x, y + 1, d, k, c);
+        const T r10 = rhs[j + d.ny] - ch_apply<T, P, M>(p, x + 1, y, d, k, c);
+        const T r11 = rhs[j + d.ny + 1] - ch_apply<T, P, M>(p, x + 1, y + 1, d, k, c);
         coarse[i] = (((r00 + r01) + r10) + r11) * T(0.25);
     }
 }
@@ -274,7 +300,7 @@ bool dims2_ok(const int64_t n[2]) {
 
 template <class T>
 Coef2<T> coef2(const double h2[2], const double inv_h2[2]) {
-    return Coef2<T>{T(h2[0]), T(h2[1]), T(inv_h2 ? inv_h2[0] : 0.0), T(inv_h2 ? inv_h2[1] : 0.0)};
+    return coef2_of<T>(h2[0], h2[1], inv_h2 ? inv_h2[0] : 0.0, inv_h2 ? inv_h2[1] : 0.0);
 }
 
 unsigned blocks_for(int64_t n) { return grid_for(n, CH_THREADS, 8192); }
@@ -287,8 +313,11 @@ int ch_stencil(const T* p, const T* rhs, T* out, const int64_t n[2], const doubl
     const Dim2 d{int(n[0]), int(n[1])};
     const Coef2<T> k = coef2<T>(h2, inv_h2);
     const unsigned g = blocks_for(n[0] * n[1]);
-    if (periodic) ch_stencil_kernel<T, true, Mode><<<g, CH_THREADS, 0, s>>>(p, rhs, out, d, k, T(omega));
-    else ch_stencil_kernel<T, false, Mode><<<g, CH_THREADS, 0, s>>>(p, rhs, out, d, k, T(omega));
+    const bool m = k.exact();
+    if (periodic && m) ch_stencil_kernel<T, true, Mode, true><<<g, CH_THREADS, 0, s>>>(p, rhs, out, d, k, T(omega));
+    else if (periodic) ch_stencil_kernel<T, true, Mode, false><<<g, CH_THREADS, 0, s>>>(p, rhs, out, d, k, T(omega));
+    else if (m) ch_stencil_kernel<T, false, Mode, true><<<g, CH_THREADS, 0, s>>>(p, rhs, out, d, k, T(omega));
+    else ch_stencil_kernel<T, false, Mode, false><<<g, CH_THREADS, 0, s>>>(p, rhs, out, d, k, T(omega));
     AFSI_LAUNCH_CHECK();
     return OK;
 }
@@ -299,14 +328,19 @@ int ch_rbgs(T* p, const T* rhs, const int64_t n[2], const double h2[2], const do
     AFSI_REQUIRE(dims2_ok(n) && h2 && inv_h2 && p && rhs, "invalid channel smoothing arguments");
     const Dim2 d{int(n[0]), int(n[1])};
     const Coef2<T> k = coef2<T>(h2, inv_h2);
+    const bool m = k.exact();
     if (d.ny % 2 == 0) {
         const unsigned g = blocks_for((n[0] * n[1] + 1) / 2);
-        if (periodic) ch_rbgs_kernel<T, true><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
-        else ch_rbgs_kernel<T, false><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
+        if (periodic && m) ch_rbgs_kernel<T, true, true><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
+        else if (periodic) ch_rbgs_kernel<T, true, false><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
+        else if (m) ch_rbgs_kernel<T, false, true><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
+        else ch_rbgs_kernel<T, false, false><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
     } else {
         const unsigned g = blocks_for(n[0] * n[1]);
-        if (periodic) ch_rbgs_any_kernel<T, true><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
-        else ch_rbgs_any_kernel<T, false><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
+        if (periodic && m) ch_rbgs_any_kernel<T, true, true><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
+        else if (periodic) ch_rbgs_any_kernel<T, true, false><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
+        else if (m) ch_rbgs_any_kernel<T, false, true><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
+        else ch_rbgs_any_kernel<T, false, false><<<g, CH_THREADS, 0, s>>>(p, rhs, d, k, color);
     }
     AFSI_LAUNCH_CHECK();
     return OK;
@@ -320,8 +354,11 @@ int ch_residual_restrict(const T* p, const T* rhs, T* coarse, const int64_t n[2]
     const Dim2 d{int(n[0]), int(n[1])};
     const Coef2<T> k = coef2<T>(h2, nullptr);
     const unsigned g = blocks_for(n[0] * n[1] / 4);
-    if (periodic) ch_residual_restrict_kernel<T, true><<<g, CH_THREADS, 0, s>>>(p, rhs, coarse, d, k);
-    else ch_residual_restrict_kernel<T, false><<<g, CH_THREADS, 0, s>>>(p, rhs, coarse, d, k);
+    const bool m = k.exact();
+    if (periodic && m) ch_residual_restrict_kernel<T, true, true><<<g, CH_THREADS, 0, s>>>(p, rhs, coarse, d, k);
+    else if (periodic) ch_residual_restrict_kernel<T, true, false><<<g, CH_THREADS, 0, s>>>(p, rhs, coarse, d, k);
+    else if (m) ch_residual_restrict_kernel<T, false, true><<<g, CH_THREADS, 0, s>>>(p, rhs, coarse, d, k);
+    else ch_residual_restrict_kernel<T, false, false><<<g, CH_THREADS, 0, s>>>(p, rhs, coarse, d, k);
     AFSI_LAUNCH_CHECK();
     return OK;
 }
@@ -377,10 +414,10 @@ __device__ void chb_remove_mean(T* x, int n, double* red) {
     __syncthreads();
 }
 
-template <class T, bool P>
-__device__ void chb_smooth(const afsi_mg2d_plan& pl, int l, double* red) {
+template <class T, bool P, bool M>
+__device__ __forceinline__ void chb_smooth(const afsi_mg2d_plan& pl, int l, double* red) {
     const Dim2 d{int(pl.n[l][0]), int(pl.n[l][1])};
-    const Coef2<T> k = Coef2<T>{T(pl.h2[l][0]), T(pl.h2[l][1]), T(pl.inv_h2[l][0]), T(pl.inv_h2[l][1])};
+    const Coef2<T> k = coef2_of<T>(pl.h2[l][0], pl.h2[l][1], pl.inv_h2[l][0], pl.inv_h2[l][1]);
     T* p = static_cast<T*>(pl.p[l]);
     const T* rhs = static_cast<const T*>(pl.rhs[l]);
     const int n = d.nx * d.ny;
@@ -400,7 +437,7 @@ __device__ void chb_smooth(const afsi_mg2d_plan& pl, int l, double* red) {
                             const int x = e / hy, y = 2 * (e - x * hy) + ((x + color) & 1);
                             const int i = x * d.ny + y;
                             T c;
-                            const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
+                            const T r = rhs[i] - ch_apply<T, P, M>(p, x, y, d, k, c);
                             val[g] = c + r / ch_diag<T, P>(x, y, d, k);
                             at[g] = i;
                         }
@@ -420,7 +457,7 @@ __device__ void chb_smooth(const afsi_mg2d_plan& pl, int l, double* red) {
                             const int x = i / d.ny, y = i - x * d.ny;
                             if (((x + y) & 1) != color) continue;
                             T c;
-                            const T r = rhs[i] - ch_apply<T, P>(p, x, y, d, k, c);
+                            const T r = rhs[i] - ch_apply<T, P, M>(p, x, y, d, k, c);
                             val[g] = c + r / ch_diag<T, P>(x, y, d, k);
                             at[g] = i;
                         }
@@ -444,7 +481,7 @@ __device__ void chb_smooth(const afsi_mg2d_plan& pl, int l, double* red) {
                     if (i >= n) continue;
                     const int x = i / d.ny, y = i - x * d.ny;
                     T c;
-                    const T r = rhs[i] - ch_apply<T, P>(a, x, y, d, k, c);
+                    const T r = rhs[i] - ch_apply<T, P, M>(a, x, y, d, k, c);
                     val[g] = c + (omega * r) / ch_diag<T, P>(x, y, d, k);
                 }
 #pragma unroll
@@ -462,38 +499,60 @@ __device__ void chb_smooth(const afsi_mg2d_plan& pl, int l, double* red) {
     if (P) chb_remove_mean(p, n, red);
 }
 
+// Residual restriction of level l into rhs[l + 1] (the multi-launch
+// ch_residual_restrict_kernel's expressions).
+template <class T, bool P, bool M>
+__device__ __forceinline__ void chb_restrict(const afsi_mg2d_plan& pl, int l) {
+    const Dim2 d{int(pl.n[l][0]), int(pl.n[l][1])};
+    const Coef2<T> k = coef2_of<T>(pl.h2[l][0], pl.h2[l][1], 0.0, 0.0);
+    const T* p = static_cast<const T*>(pl.p[l]);
+    const T* rhs = static_cast<const T*>(pl.rhs[l]);
+    T* coarse = static_cast<T*>(pl.rhs[l + 1]);
+    const int cy = d.ny / 2, nc = (d.nx / 2) * cy;
+    for (int i0 = threadIdx.x; i0 < nc; i0 += CHB_THREADS * CHB_ILP) {
+        T val[CHB_ILP];
+#pragma unroll
+        for (int g = 0; g < CHB_ILP; ++g) {
+            const int i = i0 + g * CHB_THREADS;
+            if (i >= nc) continue;
+            const int X = i / cy, Y = i - X * cy;
+            const int x = 2 * X, y = 2 * Y, j = x * d.ny + y;
+            T c;
+            const T r00 = rhs[j] - ch_apply<T, P, M>(p, x, y, d, k, c);
+            const T r01 = rhs[j + 1] - ch_apply<T, P, M>(p, x, y + 1, d, k, c);
+            const T r10 = rhs[j + d.ny] - ch_apply<T, P, M>(p, x + 1, y, d, k, c);
+            const T r11 = rhs[j + d.ny + 1] - ch_apply<T, P, M>(p, x + 1, y + 1, d, k, c);
+            val[g] = (((r00 + r01) + r10) + r11) * T(0.25);
+        }
+#pragma unroll
+        for (int g = 0; g < CHB_ILP; ++g)
+            if (i0 + g * CHB_THREADS < nc) coarse[i0 + g * CHB_THREADS] = val[g];
+    }
+}
+
+// Whether level l's spacings are powers of two (exact multiplications).
+__device__ inline bool chb_exact(const afsi_mg2d_plan& pl, int l) {
+    return exact_recip(pl.h2[l][0]) != 0.0 && exact_recip(pl.h2[l][1]) != 0.0;
+}
+
 template <class T, bool P>
-__device__ void chb_cycles(const afsi_mg2d_plan& pl, int cycles, double* red) {
+__device__ __forceinline__ void chb_smooth_level(const afsi_mg2d_plan& pl, int l, double* red) {
+    if (chb_exact(pl, l)) chb_smooth<T, P, true>(pl, l, red);
+    else chb_smooth<T, P, false>(pl, l, red);
+}
+
+template <class T, bool P>
+__device__ __forceinline__ void chb_cycles(const afsi_mg2d_plan& pl, int cycles, double* red) {
     const int last = pl.levels - 1;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
     for (int cyc = 0; cyc < cycles; ++cyc) {
         for (int l = 0; l < last; ++l) {
-            chb_smooth<T, P>(pl, l, red);
+            chb_smooth_level<T, P>(pl, l, red);
+            if (chb_exact(pl, l)) chb_restrict<T, P, true>(pl, l);
+            else chb_restrict<T, P, false>(pl, l);
             const Dim2 d{int(pl.n[l][0]), int(pl.n[l][1])};
-            const Coef2<T> k = Coef2<T>{T(pl.h2[l][0]), T(pl.h2[l][1]), T(0), T(0)};
-            const T* p = static_cast<const T*>(pl.p[l]);
-            const T* rhs = static_cast<const T*>(pl.rhs[l]);
             T* coarse = static_cast<T*>(pl.rhs[l + 1]);
             const int cy = d.ny / 2, nc = (d.nx / 2) * cy;
-            for (int i0 = threadIdx.x; i0 < nc; i0 += CHB_THREADS * CHB_ILP) {
-                T val[CHB_ILP];
-#pragma unroll
-                for (int g = 0; g < CHB_ILP; ++g) {
-                    const int i = i0 + g * CHB_THREADS;
-                    if (i >= nc) continue;
-                    const int X = i / cy, Y = i - X * cy;
-                    const int x = 2 * X, y = 2 * Y, j = x * d.ny + y;
-                    T c;
-                    const T r00 = rhs[j] - ch_apply<T, P>(p, x, y, d, k, c);
-                    const T r01 = rhs[j + 1] - ch_apply<T, P>(p, x, y + 1, d, k, c);
-                    const T r10 = rhs[j + d.ny] - ch_apply<T, P>(p, x + 1, y, d, k, c);
-                    const T r11 = rhs[j + d.ny + 1] - ch_apply<T, P>(p, x + 1, y + 1, d, k, c);
-                    val[g] = (((r00 + r01) + r10) + r11) * T(0.25);
-                }
-#pragma unroll
-                for (int g = 0; g < CHB_ILP; ++g)
-                    if (i0 + g * CHB_THREADS < nc) coarse[i0 + g * CHB_THREADS] = val[g];
-            }
             __syncthreads();
             if (P) chb_remove_mean(coarse, nc, red);
             T* pc = static_cast<T*>(pl.p[l + 1]);
@@ -541,7 +600,7 @@ __device__ void chb_cycles(const afsi_mg2d_plan& pl, int cycles, double* red) {
                     if (i0 + g * CHB_THREADS < n) fine[i0 + g * CHB_THREADS] = val[g];
             }
             __syncthreads();
-            chb_smooth<T, P>(pl, l, red);
+            chb_smooth_level<T, P>(pl, l, red);
         }
     }
 }
@@ -983,7 +1042,8 @@ __global__ void __launch_bounds__(CHB_THREADS) chb_stats_kernel(const afsi_mg2d_
         for (int i = threadIdx.x; i < n; i += CHB_THREADS) x[i] = x[i] - mean;
         return;
     }
-    const Coef2<double> k{pl.h2[0][0], pl.h2[0][1], 0.0, 0.0};
+    const Coef2<double> k = coef2_of<double>(pl.h2[0][0], pl.h2[0][1], 0.0, 0.0);
+    const bool m = k.exact();
     double s = 0.0, q = 0.0, a = 0.0, bad = 0.0;
     for (int i = threadIdx.x; i < n; i += CHB_THREADS) {
         double v;
@@ -992,8 +1052,10 @@ __global__ void __launch_bounds__(CHB_THREADS) chb_stats_kernel(const afsi_mg2d_
         else {
             const int x = i / d.ny, y = i - x * d.ny;
             double c;
-            v = pl.periodic ? rhs[i] - ch_apply<double, true>(p, x, y, d, k, c)
-                            : rhs[i] - ch_apply<double, false>(p, x, y, d, k, c);
+            if (pl.periodic) v = rhs[i] - (m ? ch_apply<double, true, true>(p, x, y, d, k, c)
+                                             : ch_apply<double, true, false>(p, x, y, d, k, c));
+            else v = rhs[i] - (m ? ch_apply<double, false, true>(p, x, y, d, k, c)
+                                 : ch_apply<double, false, false>(p, x, y, d, k, c));
         }
         if (isfinite(v)) {
             s += v;

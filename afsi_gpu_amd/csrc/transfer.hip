@@ -4,7 +4,9 @@
 // z fastest; indexed stencils ids (P,K) int64 + weights (P,K), fields (N,C)).
 //
 // Design (gfx950):
-// * Gathers: one thread per point (compact) or per (point, component)
+// * Gathers: one thread per point (compact); one lane per link and a
+//   butterfly over the point's lanes when the indexed stencil width is a
+//   power of two up to 64, else one thread per (point, component)
 //   (indexed). A compact point's 64 taps are summed separably from its
 //   12 one-dimensional weights (x rows of y rows of 4 contiguous z values,
 //   fused multiply-adds), so a thread issues 16 row reads of 4 doubles
@@ -278,6 +280,37 @@ __global__ void __launch_bounds__(256) indexed_gather_kernel(const T* __restrict
     }
 }
 
+// Indexed gather with one lane per link, for G = neighbors a power of two
+// dividing the wavefront: the ids and weights of consecutive links are read
+// by consecutive lanes (fully coalesced; the one-thread-per-point kernel above
+// reads them with a 8*neighbors-byte stride), each lane forms its link's
+// products, and the G lanes of a point add them with an xor butterfly. The
+// launch loop advances whole workgroups and links = points * G, so a point's
+// group is either entirely live or entirely idle and every lane takes part
+// in every shuffle.
+template <class T, int G>
+__global__ void __launch_bounds__(256) indexed_gather_group_kernel(const T* __restrict__ field,
+                                                                   const int64_t* __restrict__ ids,
+                                                                   const T* __restrict__ w, T* __restrict__ out,
+                                                                   int64_t links, int64_t comps, int64_t nodes,
+                                                                   int* flags) {
+    static_assert(G > 1 && G <= WAVE && (G & (G - 1)) == 0, "group must be a power of two within a wavefront");
+    for (int64_t l0 = int64_t(blockIdx.x) * 256; l0 < links; l0 += int64_t(gridDim.x) * 256) {
+        const int64_t l = l0 + threadIdx.x;
+        const bool live = l < links;
+        const int64_t j = live ? ids[l] : 0;
+        const T wl = live ? w[l] : T(0);
+        const bool ok = j >= 0 && j < nodes;
+        if (live && !ok) raise_flag(flags, F_INDEX);
+        for (int64_t c = 0; c < comps; ++c) {
+            T v = live && ok ? field[j * comps + c] * wl : T(0);
+#pragma unroll
+            for (int s = G / 2; s > 0; s >>= 1) v += __shfl_xor(v, s);
+            if (live && (threadIdx.x & (G - 1)) == 0) out[(l / G) * comps + c] = v;
+        }
+    }
+}
+
 // indexed spread: sort links by target node, one thread per (node, comp)
 __global__ void link_keys_kernel(const int64_t* __restrict__ ids, int64_t links, int64_t size,
                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int* flags) {
@@ -483,8 +516,20 @@ static int indexed_gather(const T* field, int64_t nodes, int64_t comps, const in
     AFSI_REQUIRE(comps > 0 && neighbors > 0 && points >= 0, "IB indexed stencil shape mismatch");
     if (points == 0) return OK;
     AFSI_REQUIRE(field && ids && w && out, "null pointer");
-    indexed_gather_kernel<T><<<grid_for(points * comps, 256, 16384), 256, 0, hipStream_t(stream)>>>(
-        field, ids, w, out, points, neighbors, comps, nodes, flags);
+    const hipStream_t s = hipStream_t(stream);
+    const int64_t links = points * neighbors;
+    const unsigned blocks = grid_for(links, 256, 16384);
+    switch (neighbors) {
+    case 2: indexed_gather_group_kernel<T, 2><<<blocks, 256, 0, s>>>(field, ids, w, out, links, comps, nodes, flags); break;
+    case 4: indexed_gather_group_kernel<T, 4><<<blocks, 256, 0, s>>>(field, ids, w, out, links, comps, nodes, flags); break;
+    case 8: indexed_gather_group_kernel<T, 8><<<blocks, 256, 0, s>>>(field, ids, w, out, links, comps, nodes, flags); break;
+    case 16: indexed_gather_group_kernel<T, 16><<<blocks, 256, 0, s>>>(field, ids, w, out, links, comps, nodes, flags); break;
+    case 32: indexed_gather_group_kernel<T, 32><<<blocks, 256, 0, s>>>(field, ids, w, out, links, comps, nodes, flags); break;
+    case 64: indexed_gather_group_kernel<T, 64><<<blocks, 256, 0, s>>>(field, ids, w, out, links, comps, nodes, flags); break;
+    default:
+        indexed_gather_kernel<T><<<grid_for(points * comps, 256, 16384), 256, 0, s>>>(
+            field, ids, w, out, points, neighbors, comps, nodes, flags);
+    }
     AFSI_LAUNCH_CHECK();
     return OK;
 }
