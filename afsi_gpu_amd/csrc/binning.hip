@@ -94,104 +94,178 @@ __global__ void keys_batch_kernel(const BinDesc* __restrict__ desc, int n, int64
     }
 }
 
+// Per-block digit counts, one LDS histogram per wavefront (atomics contend
+// only within a wavefront), summed per digit at the end.
 __global__ void __launch_bounds__(SORT_THREADS) radix_hist_kernel(const uint32_t* __restrict__ keys,
                                                                  int64_t n, int shift,
                                                                  uint32_t* __restrict__ hist,
                                                                  int64_t nblocks) {
-    __shared__ uint32_t h[RADIX];
-    h[threadIdx.x] = 0;
+    __shared__ uint32_t h[SORT_WAVES][RADIX];
+    const int wave = threadIdx.x / WAVE;
+#pragma unroll
+    for (int w = 0; w < SORT_WAVES; ++w) h[w][threadIdx.x] = 0;
     __syncthreads();
     const int64_t start = int64_t(blockIdx.x) * SORT_TILE;
+    uint32_t k[SORT_ROUNDS];
+#pragma unroll
     for (int r = 0; r < SORT_ROUNDS; ++r) {
         const int64_t i = start + int64_t(r) * SORT_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RADIX - 1)], 1u);
+        k[r] = i < n ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < SORT_ROUNDS; ++r) {
+        const int64_t i = start + int64_t(r) * SORT_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[wave][(k[r] >> shift) & (RADIX - 1)], 1u);
     }
     __syncthreads();
-    hist[int64_t(threadIdx.x) * nblocks + blockIdx.x] = h[threadIdx.x];
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < SORT_WAVES; ++w) c += h[w][threadIdx.x];
+    hist[int64_t(threadIdx.x) * nblocks + blockIdx.x] = c;
 }
 
 // Exclusive scan of hist[d][0..nblocks) for digit d = blockIdx.x; total[d].
+// Each step covers SORT_THREADS * SCAN_PER consecutive entries: a thread sums
+// its SCAN_PER entries, a wavefront scan of those sums (shuffles) and the
+// wave totals in LDS give every thread its offset, and the entries are
+// rewritten in place. Two barriers per step.
+constexpr int SCAN_PER = 8;
+
 __global__ void __launch_bounds__(SORT_THREADS) radix_scan_kernel(uint32_t* __restrict__ hist,
                                                                  int64_t nblocks,
                                                                  uint32_t* __restrict__ total) {
-    __shared__ uint32_t s[SORT_THREADS];
+    __shared__ uint32_t wave_tot[SORT_WAVES];
     uint32_t* row = hist + int64_t(blockIdx.x) * nblocks;
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
     uint32_t running = 0;
-    for (int64_t base = 0; base < nblocks; base += SORT_THREADS) {
-        const int64_t i = base + threadIdx.x;
-        const uint32_t v = i < nblocks ? row[i] : 0u;
-        s[threadIdx.x] = v;
-        __syncthreads();
-        // Hillis-Steele inclusive scan in LDS.
-        for (int d = 1; d < SORT_THREADS; d <<= 1) {
-            const uint32_t add = threadIdx.x >= d ? s[threadIdx.x - d] : 0u;
-            __syncthreads();
-            s[threadIdx.x] += add;
-            __syncthreads();
+    for (int64_t base = 0; base < nblocks; base += SORT_THREADS * SCAN_PER) {
+        const int64_t i0 = base + int64_t(threadIdx.x) * SCAN_PER;
+        uint32_t v[SCAN_PER];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < SCAN_PER; ++q) {
+            v[q] = i0 + q < nblocks ? row[i0 + q] : 0u;
+            sum += v[q];
         }
-        if (i < nblocks) row[i] = running + s[threadIdx.x] - v;
-        running += s[SORT_THREADS - 1];
+        uint32_t inc = sum;
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const uint32_t up = __shfl_up(inc, d, WAVE);
+            if (lane >= d) inc += up;
+        }
+        if (lane == WAVE - 1) wave_tot[wave] = inc;
         __syncthreads();
+        uint32_t run = running + inc - sum;
+#pragma unroll
+        for (int w = 0; w < SORT_WAVES; ++w) {
+            if (w < wave) run += wave_tot[w];
+            running += wave_tot[w];
+        }
+        __syncthreads();  // wave_tot is rewritten by the next step
+#pragma unroll
+        for (int q = 0; q < SCAN_PER; ++q) {
+            if (i0 + q < nblocks) row[i0 + q] = run;
+            run += v[q];
+        }
     }
     if (threadIdx.x == 0) total[blockIdx.x] = running;
 }
 
+// Exclusive scan over the SORT_THREADS threads of one value each (wavefront
+// shuffles, wave totals through LDS). Every thread must call it.
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* wave_tot) {
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t up = __shfl_up(inc, d, WAVE);
+        if (lane >= d) inc += up;
+    }
+    if (lane == WAVE - 1) wave_tot[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+#pragma unroll
+    for (int w = 0; w < SORT_WAVES; ++w)
+        if (w < wave) before += wave_tot[w];
+    __syncthreads();
+    return before + inc - v;
+}
+
+// One tile of SORT_TILE keys per block, stable. The tile is first ordered
+// by digit in LDS: ranks come from per-round wavefront ballots in input
+// order (round r holds keys start + r * SORT_THREADS + tid), offset by the
+// tile's own digit starts. It is then written out in that order, so
+// consecutive threads store consecutive positions of one digit's run
+// instead of one isolated word per key.
 __global__ void __launch_bounds__(SORT_THREADS) radix_scatter_kernel(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n, int shift,
     const uint32_t* __restrict__ hist, const uint32_t* __restrict__ total, int64_t nblocks,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
-    __shared__ uint32_t base[RADIX];            // global start of this block's digit d
-    __shared__ uint32_t cnt[SORT_WAVES][RADIX];  // per-wave counts, then prefixes
-    __shared__ uint32_t scan[RADIX];
+    __shared__ uint32_t gbase[RADIX];            // global position of local index 0 for digit d
+    __shared__ uint32_t run[RADIX];              // next local position of digit d
+    __shared__ uint32_t cnt[SORT_WAVES][RADIX];  // per-wave counts, then positions
+    __shared__ uint32_t wave_tot[SORT_WAVES];
+    __shared__ uint32_t sk[SORT_TILE], sv[SORT_TILE];
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1), wave = tid / WAVE;
-    // digit bases: exclusive scan of totals (identical in every block).
-    scan[tid] = total[tid];
-    __syncthreads();
-    for (int d = 1; d < RADIX; d <<= 1) {
-        const uint32_t add = tid >= d ? scan[tid - d] : 0u;
-        __syncthreads();
-        scan[tid] += add;
-        __syncthreads();
-    }
-    base[tid] = scan[tid] - total[tid] + hist[int64_t(tid) * nblocks + blockIdx.x];
-    __syncthreads();
     const int64_t start = int64_t(blockIdx.x) * SORT_TILE;
-    const uint64_t lt = (uint64_t(1) << lane) - 1;
+    const int m = int(n - start < SORT_TILE ? n - start : SORT_TILE);
+    uint32_t key[SORT_ROUNDS], val[SORT_ROUNDS];
+    run[tid] = 0;
+    __syncthreads();
+#pragma unroll
     for (int r = 0; r < SORT_ROUNDS; ++r) {
-        const int64_t i = start + int64_t(r) * SORT_THREADS + tid;
-        const bool valid = i < n;
-        const uint32_t key = valid ? keys[i] : 0u;
-        const uint32_t val = valid ? vals[i] : 0u;
-        const uint32_t digit = (key >> shift) & (RADIX - 1);
+        const int e = r * SORT_THREADS + tid;
+        key[r] = e < m ? keys[start + e] : 0u;
+        val[r] = e < m ? vals[start + e] : 0u;
+        if (e < m) atomicAdd(&run[(key[r] >> shift) & (RADIX - 1)], 1u);
+    }
+    __syncthreads();
+    const uint32_t local_count = run[tid];
+    const uint32_t local_start = block_excl_scan(local_count, wave_tot);
+    const uint32_t digit_start = block_excl_scan(total[tid], wave_tot);
+    gbase[tid] = digit_start + hist[int64_t(tid) * nblocks + blockIdx.x] - local_start;
+    run[tid] = local_start;
+    const uint64_t lt = (uint64_t(1) << lane) - 1;
+#pragma unroll
+    for (int r = 0; r < SORT_ROUNDS; ++r) {
+        const bool valid = r * SORT_THREADS + tid < m;
+        const uint32_t digit = (key[r] >> shift) & (RADIX - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            const uint64_t m = __ballot((digit >> b) & 1u);
-            peers &= ((digit >> b) & 1u) ? m : ~m;
+            const uint64_t q = __ballot((digit >> b) & 1u);
+            peers &= ((digit >> b) & 1u) ? q : ~q;
         }
+#pragma unroll
         for (int w = 0; w < SORT_WAVES; ++w) cnt[w][tid] = 0u;
         __syncthreads();
-        const uint32_t rank = __popcll(peers & lt);
         // The lowest peer lane publishes the wave's count for this digit.
         if (valid && (peers & lt) == 0) cnt[wave][digit] = uint32_t(__popcll(peers));
         __syncthreads();
-        {   // thread tid owns digit tid: prefix over waves, advance the base.
-            uint32_t run = base[tid];
+        {   // thread tid owns digit tid: prefix over waves, advance the run.
+            uint32_t at = run[tid];
+#pragma unroll
             for (int w = 0; w < SORT_WAVES; ++w) {
                 const uint32_t c = cnt[w][tid];
-                cnt[w][tid] = run;
-                run += c;
+                cnt[w][tid] = at;
+                at += c;
             }
-            base[tid] = run;
+            run[tid] = at;
         }
         __syncthreads();
         if (valid) {
-            const uint32_t pos = cnt[wave][digit] + rank;
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+            const uint32_t pos = cnt[wave][digit] + uint32_t(__popcll(peers & lt));
+            sk[pos] = key[r];
+            sv[pos] = val[r];
         }
-        __syncthreads();
+        __syncthreads();  // cnt is cleared by the next round
+    }
+    for (int e = tid; e < m; e += SORT_THREADS) {
+        const uint32_t k = sk[e];
+        const uint32_t pos = gbase[(k >> shift) & (RADIX - 1)] + uint32_t(e);
+        keys_out[pos] = k;
+        vals_out[pos] = sv[e];
     }
 }
 

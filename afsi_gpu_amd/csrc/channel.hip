@@ -541,12 +541,14 @@ __device__ __forceinline__ void chb_smooth_level(const afsi_mg2d_plan& pl, int l
     else chb_smooth<T, P, false>(pl, l, red);
 }
 
+// `cycles` V-cycles of levels l0..last (l0 > 0: the lower part of a cycle
+// whose levels above l0 run as batched launches).
 template <class T, bool P>
-__device__ __forceinline__ void chb_cycles(const afsi_mg2d_plan& pl, int cycles, double* red) {
+__device__ __forceinline__ void chb_cycles(const afsi_mg2d_plan& pl, int cycles, double* red, int l0 = 0) {
     const int last = pl.levels - 1;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
     for (int cyc = 0; cyc < cycles; ++cyc) {
-        for (int l = 0; l < last; ++l) {
+        for (int l = l0; l < last; ++l) {
             chb_smooth_level<T, P>(pl, l, red);
             if (chb_exact(pl, l)) chb_restrict<T, P, true>(pl, l);
             else chb_restrict<T, P, false>(pl, l);
@@ -574,7 +576,7 @@ __device__ __forceinline__ void chb_cycles(const afsi_mg2d_plan& pl, int cycles,
             __syncthreads();
             if (P) chb_remove_mean(y, m, red);
         }
-        for (int l = last - 1; l >= 0; --l) {
+        for (int l = last - 1; l >= l0; --l) {
             const Dim2 d{int(pl.n[l][0]), int(pl.n[l][1])};
             const T* coarse = static_cast<const T*>(pl.p[l + 1]);
             T* fine = static_cast<T*>(pl.p[l]);
@@ -1014,14 +1016,259 @@ __global__ void __launch_bounds__(CH_THREADS) chb_correct_kernel(const ChbDesc* 
 // One workgroup per active instance: `cycles` V-cycles (ch_block_cycles).
 // One instantiation per boundary type (each holds one inlined V-cycle, so
 // the register budget of 1024 threads is not shared by two); workgroups of
-// the other type return at once.
+// the other type return at once, and so do instances whose upper levels run
+// as batched launches (split[i] > 0).
 template <bool P>
 __global__ void __launch_bounds__(CHB_THREADS) chb_cycles_kernel(const afsi_mg2d_plan* __restrict__ plans,
+                                                                 const int32_t* __restrict__ split,
                                                                  const int32_t* __restrict__ active, int cycles) {
     __shared__ double red[CHB_THREADS];
     const afsi_mg2d_plan& pl = plans[blockIdx.x];
-    if ((active && !active[blockIdx.x]) || (pl.periodic != 0) != P) return;
+    if ((active && !active[blockIdx.x]) || (pl.periodic != 0) != P || split[blockIdx.x] > 0) return;
     chb_cycles<double, P>(pl, cycles, red);
+}
+
+// The lower part (levels split[i]..last) of one V-cycle of every active
+// instance with split[i] > 0.
+template <bool P>
+__global__ void __launch_bounds__(CHB_THREADS) chb_tail_kernel(const afsi_mg2d_plan* __restrict__ plans,
+                                                               const int32_t* __restrict__ split,
+                                                               const int32_t* __restrict__ active) {
+    __shared__ double red[CHB_THREADS];
+    const afsi_mg2d_plan& pl = plans[blockIdx.x];
+    const int l0 = split[blockIdx.x];
+    if ((active && !active[blockIdx.x]) || (pl.periodic != 0) != P || l0 <= 0) return;
+    chb_cycles<double, P>(pl, 1, red, l0);
+}
+
+// ---- upper levels of large instances as batched launches ---------------------
+// A level with more than MLB_MIN_CELLS cells (RBGS instances) would keep one
+// CU busy for hundreds of microseconds per cycle in the one-workgroup kernel:
+// its FP64 stencil work is issued by that one CU. Such levels run instead as
+// one launch per phase over every instance that has them, MLB_CHUNK cells of
+// one instance per workgroup, with the per-cell expressions of the
+// one-workgroup kernel (so smoothing, restriction and prolongation give the
+// same values). Only the periodic mean removal sums in another fixed order:
+// per-workgroup partial sums, then the instance's partials in workgroup
+// order. Level l's work list holds the instances whose first split[i]
+// levels include l; the levels below run in chb_tail_kernel.
+constexpr int MLB_THREADS = 256;
+constexpr int MLB_CHUNK = 2048;           // level-l cells per workgroup
+constexpr int64_t MLB_MIN_CELLS = 4096;
+
+struct MlbEntry {
+    int32_t inst, block_begin;
+};
+
+__device__ inline MlbEntry mlb_find(const MlbEntry* __restrict__ list, int n) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (list[mid].block_begin <= int(blockIdx.x)) lo = mid;
+        else hi = mid;
+    }
+    return list[lo];
+}
+
+template <bool P, bool M>
+__device__ __forceinline__ void mlb_smooth_chunk(const afsi_mg2d_plan& pl, int l, int chunk, int color) {
+    const Dim2 d{int(pl.n[l][0]), int(pl.n[l][1])};
+    const Coef2<double> k = coef2_of<double>(pl.h2[l][0], pl.h2[l][1], pl.inv_h2[l][0], pl.inv_h2[l][1]);
+    double* p = static_cast<double*>(pl.p[l]);
+    const double* rhs = static_cast<const double*>(pl.rhs[l]);
+    const int hy = d.ny / 2, half = d.nx * hy;
+    constexpr int G = MLB_CHUNK / 2 / MLB_THREADS;
+    const int e0 = chunk * (MLB_CHUNK / 2) + int(threadIdx.x);
+    double val[G];
+    int at[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int e = e0 + g * MLB_THREADS;
+        at[g] = -1;
+        if (e >= half) continue;
+        const int x = e / hy, y = 2 * (e - x * hy) + ((x + color) & 1);
+        const int i = x * d.ny + y;
+        double c;
+        const double r = rhs[i] - ch_apply<double, P, M>(p, x, y, d, k, c);
+        val[g] = c + r / ch_diag<double, P>(x, y, d, k);
+        at[g] = i;
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+        if (at[g] >= 0) p[at[g]] = val[g];
+}
+
+// One colour of red-black iteration `it` on level l (instances with
+// pl.nu <= it skip it).
+__global__ void __launch_bounds__(MLB_THREADS) mlb_smooth_kernel(const afsi_mg2d_plan* __restrict__ plans,
+                                                                 const MlbEntry* __restrict__ list, int n, int l,
+                                                                 int it, int color,
+                                                                 const int32_t* __restrict__ active) {
+    const MlbEntry e = mlb_find(list, n);
+    const afsi_mg2d_plan& pl = plans[e.inst];
+    if ((active && !active[e.inst]) || it >= pl.nu) return;
+    const int chunk = int(blockIdx.x) - e.block_begin;
+    const bool m = chb_exact(pl, l);
+    if (pl.periodic) {
+        if (m) mlb_smooth_chunk<true, true>(pl, l, chunk, color);
+        else mlb_smooth_chunk<true, false>(pl, l, chunk, color);
+    } else {
+        if (m) mlb_smooth_chunk<false, true>(pl, l, chunk, color);
+        else mlb_smooth_chunk<false, false>(pl, l, chunk, color);
+    }
+}
+
+// The range of a chunk of level l in the array being averaged: p[l]
+// (coarse == 0, MLB_CHUNK cells) or rhs[l + 1] (coarse == 1, a quarter).
+__device__ inline void mlb_mean_range(const afsi_mg2d_plan& pl, int l, int coarse, int chunk, double*& x,
+                                      int& begin, int& end, int& count) {
+    const int cells = int(pl.n[l][0] * pl.n[l][1]);
+    const int per = coarse ? MLB_CHUNK / 4 : MLB_CHUNK;
+    count = coarse ? cells / 4 : cells;
+    x = static_cast<double*>(coarse ? pl.rhs[l + 1] : pl.p[l]);
+    begin = chunk * per;
+    end = begin + per < count ? begin + per : count;
+}
+
+// Partial sum of the chunk (periodic instances), in a fixed order.
+__global__ void __launch_bounds__(MLB_THREADS) mlb_sum_kernel(const afsi_mg2d_plan* __restrict__ plans,
+                                                              const MlbEntry* __restrict__ list, int n, int l,
+                                                              int coarse, const int32_t* __restrict__ active,
+                                                              double* __restrict__ partials) {
+    __shared__ double red[MLB_THREADS];
+    const MlbEntry e = mlb_find(list, n);
+    const afsi_mg2d_plan& pl = plans[e.inst];
+    if ((active && !active[e.inst]) || !pl.periodic) return;
+    double* x;
+    int begin, end, count;
+    mlb_mean_range(pl, l, coarse, int(blockIdx.x) - e.block_begin, x, begin, end, count);
+    double v = 0.0;
+    for (int i = begin + int(threadIdx.x); i < end; i += MLB_THREADS) v += x[i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = MLB_THREADS / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// x -= mean, the mean from the instance's partials in workgroup order.
+__global__ void __launch_bounds__(MLB_THREADS) mlb_sub_kernel(const afsi_mg2d_plan* __restrict__ plans,
+                                                              const MlbEntry* __restrict__ list, int n, int l,
+                                                              int coarse, const int32_t* __restrict__ active,
+                                                              const double* __restrict__ partials) {
+    __shared__ double total;
+    const MlbEntry e = mlb_find(list, n);
+    const afsi_mg2d_plan& pl = plans[e.inst];
+    if ((active && !active[e.inst]) || !pl.periodic) return;
+    double* x;
+    int begin, end, count;
+    mlb_mean_range(pl, l, coarse, int(blockIdx.x) - e.block_begin, x, begin, end, count);
+    if (threadIdx.x == 0) {
+        const int blocks = int((pl.n[l][0] * pl.n[l][1] + MLB_CHUNK - 1) / MLB_CHUNK);
+        double t = 0.0;
+        for (int b = 0; b < blocks; ++b) t += partials[e.block_begin + b];
+        total = t;
+    }
+    __syncthreads();
+    const double mean = total / double(count);
+    for (int i = begin + int(threadIdx.x); i < end; i += MLB_THREADS) x[i] = x[i] - mean;
+}
+
+template <bool P, bool M>
+__device__ __forceinline__ void mlb_restrict_chunk(const afsi_mg2d_plan& pl, int l, int chunk) {
+    const Dim2 d{int(pl.n[l][0]), int(pl.n[l][1])};
+    const Coef2<double> k = coef2_of<double>(pl.h2[l][0], pl.h2[l][1], 0.0, 0.0);
+    const double* p = static_cast<const double*>(pl.p[l]);
+    const double* rhs = static_cast<const double*>(pl.rhs[l]);
+    double* coarse = static_cast<double*>(pl.rhs[l + 1]);
+    double* pc = static_cast<double*>(pl.p[l + 1]);
+    const int cy = d.ny / 2, nc = (d.nx / 2) * cy;
+    constexpr int G = MLB_CHUNK / 4 / MLB_THREADS;
+    const int i0 = chunk * (MLB_CHUNK / 4) + int(threadIdx.x);
+    double val[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int i = i0 + g * MLB_THREADS;
+        if (i >= nc) continue;
+        const int X = i / cy, Y = i - X * cy;
+        const int x = 2 * X, y = 2 * Y, j = x * d.ny + y;
+        double c;
+        const double r00 = rhs[j] - ch_apply<double, P, M>(p, x, y, d, k, c);
+        const double r01 = rhs[j + 1] - ch_apply<double, P, M>(p, x, y + 1, d, k, c);
+        const double r10 = rhs[j + d.ny] - ch_apply<double, P, M>(p, x + 1, y, d, k, c);
+        const double r11 = rhs[j + d.ny + 1] - ch_apply<double, P, M>(p, x + 1, y + 1, d, k, c);
+        val[g] = (((r00 + r01) + r10) + r11) * 0.25;
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int i = i0 + g * MLB_THREADS;
+        if (i < nc) {
+            coarse[i] = val[g];
+            pc[i] = 0.0;
+        }
+    }
+}
+
+// rhs[l + 1] = restricted residual of level l; p[l + 1] = 0.
+__global__ void __launch_bounds__(MLB_THREADS) mlb_restrict_kernel(const afsi_mg2d_plan* __restrict__ plans,
+                                                                   const MlbEntry* __restrict__ list, int n, int l,
+                                                                   const int32_t* __restrict__ active) {
+    const MlbEntry e = mlb_find(list, n);
+    const afsi_mg2d_plan& pl = plans[e.inst];
+    if (active && !active[e.inst]) return;
+    const int chunk = int(blockIdx.x) - e.block_begin;
+    const bool m = chb_exact(pl, l);
+    if (pl.periodic) {
+        if (m) mlb_restrict_chunk<true, true>(pl, l, chunk);
+        else mlb_restrict_chunk<true, false>(pl, l, chunk);
+    } else {
+        if (m) mlb_restrict_chunk<false, true>(pl, l, chunk);
+        else mlb_restrict_chunk<false, false>(pl, l, chunk);
+    }
+}
+
+template <bool P>
+__device__ __forceinline__ void mlb_prolong_chunk(const afsi_mg2d_plan& pl, int l, int chunk) {
+    const Dim2 d{int(pl.n[l][0]), int(pl.n[l][1])};
+    const double* coarse = static_cast<const double*>(pl.p[l + 1]);
+    double* fine = static_cast<double*>(pl.p[l]);
+    const int n = d.nx * d.ny, NX = d.nx / 2, NY = d.ny / 2;
+    constexpr int G = MLB_CHUNK / MLB_THREADS;
+    const int i0 = chunk * MLB_CHUNK + int(threadIdx.x);
+    double val[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int i = i0 + g * MLB_THREADS;
+        if (i >= n) continue;
+        const int x = i / d.ny, y = i - x * d.ny;
+        const int ax = x / 2 - 1 + (x & 1), ay = y / 2 - 1 + (y & 1);
+        const double wx = (x & 1) ? 0.25 : 0.75, wy = (y & 1) ? 0.25 : 0.75;
+        const double c00 = ch_coarse_value<double, P>(coarse, ax, ay, NX, NY);
+        const double c01 = ch_coarse_value<double, P>(coarse, ax, ay + 1, NX, NY);
+        const double c10 = ch_coarse_value<double, P>(coarse, ax + 1, ay, NX, NY);
+        const double c11 = ch_coarse_value<double, P>(coarse, ax + 1, ay + 1, NX, NY);
+        const double v = (c00 * (1.0 - wy) + c01 * wy) * (1.0 - wx) + (c10 * (1.0 - wy) + c11 * wy) * wx;
+        val[g] = fine[i] + v;
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int i = i0 + g * MLB_THREADS;
+        if (i < n) fine[i] = val[g];
+    }
+}
+
+// p[l] += prolongation of p[l + 1].
+__global__ void __launch_bounds__(MLB_THREADS) mlb_prolong_kernel(const afsi_mg2d_plan* __restrict__ plans,
+                                                                  const MlbEntry* __restrict__ list, int n, int l,
+                                                                  const int32_t* __restrict__ active) {
+    const MlbEntry e = mlb_find(list, n);
+    const afsi_mg2d_plan& pl = plans[e.inst];
+    if (active && !active[e.inst]) return;
+    const int chunk = int(blockIdx.x) - e.block_begin;
+    if (pl.periodic) mlb_prolong_chunk<true>(pl, l, chunk);
+    else mlb_prolong_chunk<false>(pl, l, chunk);
 }
 
 // One workgroup per instance: [sum, sum of squares, sum |.|, non-finite
@@ -1081,13 +1328,27 @@ __global__ void __launch_bounds__(CHB_THREADS) chb_stats_kernel(const afsi_mg2d_
 struct ChbLayout {
     int n = 0;
     int64_t points = 0, nu = 0, nv = 0, cells = 0;
-    size_t desc = 0, plans = 0, cell_begin = 0, total = 0;
+    size_t desc = 0, plans = 0, cell_begin = 0, split = 0, lists = 0, partials = 0, total = 0;
+    // batched upper levels: levels 0..maxsplit-1 have work lists
+    int maxsplit = 0;
+    int count[AFSI_MG_MAX_LEVELS] = {}, blocks[AFSI_MG_MAX_LEVELS] = {}, max_nu[AFSI_MG_MAX_LEVELS] = {};
+    bool mean[AFSI_MG_MAX_LEVELS] = {};
+    bool whole_periodic = false, whole_closed = false, tail_periodic = false, tail_closed = false;
 };
+
+// Number of upper levels of an instance that run as batched launches.
+int mlb_split(const afsi_mg2d_plan& pl) {
+    if (pl.smoother != 1) return 0;
+    int s = 0;
+    while (s < pl.levels - 1 && pl.n[s][0] * pl.n[s][1] > MLB_MIN_CELLS && pl.n[s][1] % 2 == 0) ++s;
+    return s;
+}
 
 size_t chb_align(size_t b) { return (b + 255) & ~size_t(255); }
 
 int chb_layout(const afsi_chb_batch* b, ChbLayout& L, std::vector<ChbDesc>* descs,
-               std::vector<int64_t>* cell_begin) {
+               std::vector<int64_t>* cell_begin, std::vector<int32_t>* splits = nullptr,
+               std::vector<MlbEntry>* lists = nullptr) {
     AFSI_REQUIRE(b && b->items && b->count >= 1, "invalid channel batch");
     L.n = b->count;
     for (int i = 0; i < L.n; ++i) {
@@ -1116,16 +1377,41 @@ int chb_layout(const afsi_chb_batch* b, ChbLayout& L, std::vector<ChbDesc>* desc
         d.cell_count = g.n[0] * g.n[1];
         if (descs) descs->push_back(d);
         if (cell_begin) cell_begin->push_back(L.cells);
+        const int sp = mlb_split(pl);
+        if (splits) splits->push_back(sp);
+        if (sp > L.maxsplit) L.maxsplit = sp;
+        if (sp == 0) (pl.periodic ? L.whole_periodic : L.whole_closed) = true;
+        else (pl.periodic ? L.tail_periodic : L.tail_closed) = true;
         L.points += it.points;
         L.nu += d.u_count;
         L.nv += d.v_count;
         L.cells += d.cell_count;
     }
     AFSI_REQUIRE(L.points < (int64_t(1) << 40) && L.nu < (int64_t(1) << 40), "channel batch too large");
+    if (lists) lists->assign(size_t(AFSI_MG_MAX_LEVELS) * L.n, MlbEntry{0, 0});
+    int max_blocks = 1;
+    for (int l = 0; l < L.maxsplit; ++l) {
+        int64_t blocks = 0;
+        for (int i = 0; i < L.n; ++i) {
+            const afsi_mg2d_plan& pl = b->items[i].plan;
+            if (mlb_split(pl) <= l) continue;
+            if (lists) (*lists)[size_t(l) * L.n + L.count[l]] = MlbEntry{i, int32_t(blocks)};
+            ++L.count[l];
+            blocks += cdiv(pl.n[l][0] * pl.n[l][1], MLB_CHUNK);
+            if (pl.nu > L.max_nu[l]) L.max_nu[l] = pl.nu;
+            if (pl.periodic) L.mean[l] = true;
+        }
+        AFSI_REQUIRE(blocks < (int64_t(1) << 30), "channel batch level too large");
+        L.blocks[l] = int(blocks);
+        if (L.blocks[l] > max_blocks) max_blocks = L.blocks[l];
+    }
     size_t off = 0;
     L.desc = off; off = chb_align(off + size_t(L.n) * sizeof(ChbDesc));
     L.plans = off; off = chb_align(off + size_t(L.n) * sizeof(afsi_mg2d_plan));
     L.cell_begin = off; off = chb_align(off + size_t(L.n) * sizeof(int64_t));
+    L.split = off; off = chb_align(off + size_t(L.n) * sizeof(int32_t));
+    L.lists = off; off = chb_align(off + size_t(AFSI_MG_MAX_LEVELS) * L.n * sizeof(MlbEntry));
+    L.partials = off; off = chb_align(off + size_t(max_blocks) * sizeof(double));
     L.total = off;
     return OK;
 }
@@ -1162,7 +1448,9 @@ int afsi_chb_prepare(const afsi_chb_batch* batch, void* stream) {
     ChbLayout L;
     std::vector<ChbDesc> descs;
     std::vector<int64_t> begins;
-    if (int r = chb_layout(batch, L, &descs, &begins)) return r;
+    std::vector<int32_t> splits;
+    std::vector<MlbEntry> lists;
+    if (int r = chb_layout(batch, L, &descs, &begins, &splits, &lists)) return r;
     AFSI_REQUIRE(batch->workspace && batch->workspace_bytes >= L.total, "channel batch workspace too small");
     std::vector<char> host(L.total, 0);
     std::memcpy(host.data() + L.desc, descs.data(), descs.size() * sizeof(ChbDesc));
@@ -1170,6 +1458,8 @@ int afsi_chb_prepare(const afsi_chb_batch* batch, void* stream) {
         std::memcpy(host.data() + L.plans + size_t(i) * sizeof(afsi_mg2d_plan), &batch->items[i].plan,
                     sizeof(afsi_mg2d_plan));
     std::memcpy(host.data() + L.cell_begin, begins.data(), begins.size() * sizeof(int64_t));
+    std::memcpy(host.data() + L.split, splits.data(), splits.size() * sizeof(int32_t));
+    std::memcpy(host.data() + L.lists, lists.data(), lists.size() * sizeof(MlbEntry));
     const hipStream_t s = hipStream_t(stream);
     AFSI_HIP(hipMemcpyAsync(batch->workspace, host.data(), host.size(), hipMemcpyHostToDevice, s));
     AFSI_HIP(hipStreamSynchronize(s));
@@ -1215,12 +1505,47 @@ int afsi_chb_cycles_f64(const afsi_chb_batch* batch, const int32_t* active, int3
     if (int r = chb_ready(batch, L)) return r;
     AFSI_REQUIRE(cycles >= 0, "negative cycle count");
     if (!cycles) return OK;
-    bool periodic = false, closed = false;
-    for (int i = 0; i < batch->count; ++i) (batch->items[i].plan.periodic ? periodic : closed) = true;
+    const hipStream_t s = hipStream_t(stream);
     const afsi_mg2d_plan* plans = chb_at<afsi_mg2d_plan>(batch, L.plans);
-    if (periodic) chb_cycles_kernel<true><<<unsigned(L.n), CHB_THREADS, 0, hipStream_t(stream)>>>(plans, active, cycles);
-    if (closed) chb_cycles_kernel<false><<<unsigned(L.n), CHB_THREADS, 0, hipStream_t(stream)>>>(plans, active, cycles);
+    const int32_t* split = chb_at<int32_t>(batch, L.split);
+    const unsigned n = unsigned(L.n);
+    // instances without batched levels: whole cycles, one workgroup each
+    if (L.whole_periodic) chb_cycles_kernel<true><<<n, CHB_THREADS, 0, s>>>(plans, split, active, cycles);
+    if (L.whole_closed) chb_cycles_kernel<false><<<n, CHB_THREADS, 0, s>>>(plans, split, active, cycles);
     AFSI_LAUNCH_CHECK();
+    if (L.maxsplit == 0) return OK;
+    // the others: upper levels as batched launches around a one-workgroup tail
+    double* partials = chb_at<double>(batch, L.partials);
+    const auto list = [&](int l) { return chb_at<MlbEntry>(batch, L.lists) + size_t(l) * L.n; };
+    const auto smooth = [&](int l) {
+        for (int it = 0; it < L.max_nu[l]; ++it)
+            for (int color = 0; color < 2; ++color)
+                mlb_smooth_kernel<<<unsigned(L.blocks[l]), MLB_THREADS, 0, s>>>(plans, list(l), L.count[l], l, it,
+                                                                              color, active);
+    };
+    const auto remove_mean = [&](int l, int coarse) {
+        if (!L.mean[l]) return;
+        mlb_sum_kernel<<<unsigned(L.blocks[l]), MLB_THREADS, 0, s>>>(plans, list(l), L.count[l], l, coarse, active,
+                                                                      partials);
+        mlb_sub_kernel<<<unsigned(L.blocks[l]), MLB_THREADS, 0, s>>>(plans, list(l), L.count[l], l, coarse, active,
+                                                                      partials);
+    };
+    for (int c = 0; c < cycles; ++c) {
+        for (int l = 0; l < L.maxsplit; ++l) {
+            smooth(l);
+            remove_mean(l, 0);
+            mlb_restrict_kernel<<<unsigned(L.blocks[l]), MLB_THREADS, 0, s>>>(plans, list(l), L.count[l], l, active);
+            remove_mean(l, 1);
+        }
+        if (L.tail_periodic) chb_tail_kernel<true><<<n, CHB_THREADS, 0, s>>>(plans, split, active);
+        if (L.tail_closed) chb_tail_kernel<false><<<n, CHB_THREADS, 0, s>>>(plans, split, active);
+        for (int l = L.maxsplit - 1; l >= 0; --l) {
+            mlb_prolong_kernel<<<unsigned(L.blocks[l]), MLB_THREADS, 0, s>>>(plans, list(l), L.count[l], l, active);
+            smooth(l);
+            remove_mean(l, 0);
+        }
+        AFSI_LAUNCH_CHECK();
+    }
     return OK;
 }
 

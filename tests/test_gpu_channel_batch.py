@@ -1,9 +1,12 @@
 """Batched channel instances (mac2d/batch.py) against the oracle: each
 instance of an uneven batch (128x32, 256x64 and 512x128 periodic-x channels
-and one reference channel with inlet/outlet) advanced two steps with one
-launch per phase, against oracle/step.ib_step_2d on the same inputs: the
-solve's cycle count equal, projected velocity and interpolated point
-velocity within 1e-10 relative, positions within 1e-14."""
+and two reference channels with inlet/outlet, 64x32 and 128x64) advanced two
+steps with one launch per phase, against oracle/step.ib_step_2d on the same
+inputs: the solve's cycle count equal, projected velocity and interpolated
+point velocity within 1e-10 relative, positions within 1e-14. The levels
+above 4096 cells of the 256x64, 512x128 and 128x64 instances run as batched
+launches around a one-workgroup tail; the other instances cycle in one
+workgroup each."""
 import numpy as np
 import pytest
 import torch
@@ -26,7 +29,7 @@ def rel(a, b):
 def test_channel_batch_matches_oracle(gpu):
     gen = np.random.default_rng(21)
     specs = [((128, 32), True, 3000), ((512, 128), True, 20000), ((256, 64), True, 9000),
-             ((64, 32), False, 1500), ((128, 32), True, 1)]
+             ((64, 32), False, 1500), ((128, 32), True, 1), ((128, 64), False, 4000)]
     grids, chans, xs, Fs = [], [], [], []
     for shape, periodic, count in specs:
         lengths = (4., 1.) if periodic else (2., 1.)
