@@ -565,8 +565,10 @@ int afsi_mg2d_residual_f32(const afsi_mg2d_plan* plan, void* stream);
  * weight arrays (rows in instance order), so one afsi_indexed_spread_f64 /
  * afsi_indexed_gather_f64 per component serves the batch. Each instance's
  * FP64 pressure hierarchy is its own afsi_mg2d_plan: divergence writes its
- * rhs[0], cycles run one 1024-thread workgroup per active instance
- * (active[i] != 0; NULL: all), correct reads its p[0]. stats writes
+ * rhs[0], cycles advance every active instance (active[i] != 0; NULL: all)
+ * by `cycles` V-cycles: levels of more than 4096 cells of a red-black
+ * instance as one launch per phase over all instances that have them, the
+ * rest in one 1024-thread workgroup per instance; correct reads its p[0]. stats writes
  * out[4i..4i+3] = [sum, sum of squares, sum |.|, non-finite count] of
  * rhs[0] (which 0), p[0] (1) or rhs[0] - A p[0] (2) in a fixed order;
  * which 3 / 4 subtracts out[4i]/cells from rhs[0] / p[0] of the periodic
