@@ -894,60 +894,88 @@ __device__ inline int chb_find(const ChbDesc* __restrict__ desc, int n, int64_t 
 }
 
 // The 16-link reflected stencil of every point (ch_prepare_kernel's
-// expressions) with the instance's face offset added to the ids.
+// expressions) with the instance's face offset added to the ids. A
+// workgroup takes CH_THREADS consecutive points of one component; each
+// thread forms its point's 16 ids and weights in registers, and the block's
+// contiguous 16 * CH_THREADS ids, then weights, pass through LDS (rows
+// padded to 17 entries against bank conflicts) so that consecutive lanes
+// store consecutive entries (a thread storing its own 128-byte row leaves
+// every store instruction 64 scattered words).
+constexpr int CHS_PAD = 17;
+
 __global__ void __launch_bounds__(CH_THREADS) chb_stencil_kernel(const ChbDesc* __restrict__ desc, int n,
                                                                  int64_t total, int64_t* __restrict__ ids_u,
                                                                  double* __restrict__ w_u, int64_t* __restrict__ ids_v,
                                                                  double* __restrict__ w_v, int* __restrict__ flags) {
-    for (int64_t t = int64_t(blockIdx.x) * CH_THREADS + threadIdx.x; t < 2 * total;
-         t += int64_t(gridDim.x) * CH_THREADS) {
-        const int c = int(t / total);
-        const int64_t g = t - c * total;
-        const ChbDesc& D = desc[chb_find(desc, n, g, 0)];
-        const int64_t q = g - D.point_begin;
-        const double px = D.x[2 * q], py = D.x[2 * q + 1];
-        const Dim2 d = D.d;
-        bool ok = isfinite(px) && isfinite(py) && py >= -D.hy && py <= D.Ly + D.hy;
-        if (!D.periodic) ok = ok && px >= 2 * D.hx && px <= D.Lx - 2 * D.hx;
-        int64_t* ids = (c == 0 ? ids_u : ids_v) + g * 16;
-        double* w = (c == 0 ? w_u : w_v) + g * 16;
-        const int64_t off = c == 0 ? D.u_off : D.v_off;
-        if (!ok) {
-            raise_flag(flags, F_SUPPORT);
-            for (int k = 0; k < 16; ++k) {
-                ids[k] = off;
-                w[k] = 0.0;
-            }
-            continue;
-        }
-        const double ox = D.hx * (c == 0 ? 0.0 : 0.5), oy = D.hy * (c == 1 ? 0.0 : 0.5);
-        const double sx = (px - ox) / D.hx, sy = (py - oy) / D.hy;
-        const int64_t bx = int64_t(floor(sx - 1)), by = int64_t(floor(sy - 1));
-        const int64_t fny = d.ny + (c == 1);
-        double fx[4], fy[4];
-        for (int a = 0; a < 4; ++a) {
-            fx[a] = peskin_phi(sx - double(bx + a));
-            fy[a] = peskin_phi(sy - double(by + a));
-        }
-        for (int i = 0; i < 4; ++i) {
-            int64_t nxi = bx + i;
-            if (D.periodic) nxi = ((nxi % d.nx) + d.nx) % d.nx;
-            for (int j = 0; j < 4; ++j) {
-                const int64_t row = by + j;
-                bool outside;
-                int64_t refl;
-                if (c == 0) {
-                    outside = row < 0 || row >= d.ny;
-                    refl = row < 0 ? -row - 1 : (row >= d.ny ? 2 * d.ny - row - 1 : row);
-                } else {
-                    outside = row < 0 || row > d.ny;
-                    refl = row < 0 ? -row : (row > d.ny ? 2 * d.ny - row : row);
+    __shared__ uint64_t stage[CH_THREADS * CHS_PAD];
+    const int64_t per_comp = cdiv(total, CH_THREADS);
+    for (int64_t blk = blockIdx.x; blk < 2 * per_comp; blk += gridDim.x) {
+        const int c = int(blk / per_comp);
+        const int64_t g0 = (blk - c * per_comp) * CH_THREADS;
+        const int rows = int(total - g0 < CH_THREADS ? total - g0 : CH_THREADS);
+        const int64_t g = g0 + threadIdx.x;
+        int64_t ids[16];
+        double w[16];
+        if (int(threadIdx.x) < rows) {
+            const ChbDesc& D = desc[chb_find(desc, n, g, 0)];
+            const int64_t q = g - D.point_begin;
+            const double px = D.x[2 * q], py = D.x[2 * q + 1];
+            const Dim2 d = D.d;
+            bool ok = isfinite(px) && isfinite(py) && py >= -D.hy && py <= D.Ly + D.hy;
+            if (!D.periodic) ok = ok && px >= 2 * D.hx && px <= D.Lx - 2 * D.hx;
+            const int64_t off = c == 0 ? D.u_off : D.v_off;
+            if (!ok) {
+                raise_flag(flags, F_SUPPORT);
+                for (int k = 0; k < 16; ++k) {
+                    ids[k] = off;
+                    w[k] = 0.0;
                 }
-                const double wt = fx[i] * fy[j];
-                ids[4 * i + j] = off + nxi * fny + refl;
-                w[4 * i + j] = outside ? -wt : wt;
+            } else {
+                const double ox = D.hx * (c == 0 ? 0.0 : 0.5), oy = D.hy * (c == 1 ? 0.0 : 0.5);
+                const double sx = (px - ox) / D.hx, sy = (py - oy) / D.hy;
+                const int64_t bx = int64_t(floor(sx - 1)), by = int64_t(floor(sy - 1));
+                const int64_t fny = d.ny + (c == 1);
+                double fx[4], fy[4];
+                for (int a = 0; a < 4; ++a) {
+                    fx[a] = peskin_phi(sx - double(bx + a));
+                    fy[a] = peskin_phi(sy - double(by + a));
+                }
+                for (int i = 0; i < 4; ++i) {
+                    int64_t nxi = bx + i;
+                    if (D.periodic) nxi = ((nxi % d.nx) + d.nx) % d.nx;
+                    for (int j = 0; j < 4; ++j) {
+                        const int64_t row = by + j;
+                        bool outside;
+                        int64_t refl;
+                        if (c == 0) {
+                            outside = row < 0 || row >= d.ny;
+                            refl = row < 0 ? -row - 1 : (row >= d.ny ? 2 * d.ny - row - 1 : row);
+                        } else {
+                            outside = row < 0 || row > d.ny;
+                            refl = row < 0 ? -row : (row > d.ny ? 2 * d.ny - row : row);
+                        }
+                        const double wt = fx[i] * fy[j];
+                        ids[4 * i + j] = off + nxi * fny + refl;
+                        w[4 * i + j] = outside ? -wt : wt;
+                    }
+                }
             }
         }
+        int64_t* out_ids = (c == 0 ? ids_u : ids_v) + g0 * 16;
+        double* out_w = (c == 0 ? w_u : w_v) + g0 * 16;
+        uint64_t* mine = stage + threadIdx.x * CHS_PAD;
+        if (int(threadIdx.x) < rows)
+            for (int k = 0; k < 16; ++k) mine[k] = uint64_t(ids[k]);
+        __syncthreads();
+        for (int e = threadIdx.x; e < rows * 16; e += CH_THREADS)
+            out_ids[e] = int64_t(stage[(e >> 4) * CHS_PAD + (e & 15)]);
+        __syncthreads();
+        if (int(threadIdx.x) < rows)
+            for (int k = 0; k < 16; ++k) mine[k] = __builtin_bit_cast(uint64_t, w[k]);
+        __syncthreads();
+        for (int e = threadIdx.x; e < rows * 16; e += CH_THREADS)
+            out_w[e] = __builtin_bit_cast(double, stage[(e >> 4) * CHS_PAD + (e & 15)]);
+        __syncthreads();
     }
 }
 
@@ -1274,9 +1302,27 @@ __global__ void __launch_bounds__(MLB_THREADS) mlb_prolong_kernel(const afsi_mg2
 // One workgroup per instance: [sum, sum of squares, sum |.|, non-finite
 // count] of rhs[0] (which 0), p[0] (1) or rhs[0] - A p[0] (2), summed in a
 // fixed order; which 3/4: subtract the mean out[4i]/n from rhs[0]/p[0].
-__global__ void __launch_bounds__(CHB_THREADS) chb_stats_kernel(const afsi_mg2d_plan* __restrict__ plans, int which,
+// Value of cell i of level 0 for the statistics: rhs (which 0), p (1) or
+// the residual rhs - A p (2).
+__device__ inline double chb_stat_value(const afsi_mg2d_plan& pl, const Coef2<double>& k, bool m, int which,
+                                        int i) {
+    const Dim2 d{int(pl.n[0][0]), int(pl.n[0][1])};
+    const double* rhs = static_cast<const double*>(pl.rhs[0]);
+    const double* p = static_cast<const double*>(pl.p[0]);
+    if (which == 0) return rhs[i];
+    if (which == 1) return p[i];
+    const int x = i / d.ny, y = i - x * d.ny;
+    double c;
+    if (pl.periodic)
+        return rhs[i] - (m ? ch_apply<double, true, true>(p, x, y, d, k, c) : ch_apply<double, true, false>(p, x, y, d, k, c));
+    return rhs[i] - (m ? ch_apply<double, false, true>(p, x, y, d, k, c) : ch_apply<double, false, false>(p, x, y, d, k, c));
+}
+
+__global__ void __launch_bounds__(CHB_THREADS) chb_stats_kernel(const afsi_mg2d_plan* __restrict__ plans,
+                                                                const int32_t* __restrict__ split, int which,
                                                                 double* __restrict__ out) {
     __shared__ double red[4][CHB_THREADS];
+    if (split[blockIdx.x] > 0) return;  // level 0 is covered by mlb_stats_kernel
     const afsi_mg2d_plan& pl = plans[blockIdx.x];
     const Dim2 d{int(pl.n[0][0]), int(pl.n[0][1])};
     const int n = d.nx * d.ny;
@@ -1293,17 +1339,7 @@ __global__ void __launch_bounds__(CHB_THREADS) chb_stats_kernel(const afsi_mg2d_
     const bool m = k.exact();
     double s = 0.0, q = 0.0, a = 0.0, bad = 0.0;
     for (int i = threadIdx.x; i < n; i += CHB_THREADS) {
-        double v;
-        if (which == 0) v = rhs[i];
-        else if (which == 1) v = p[i];
-        else {
-            const int x = i / d.ny, y = i - x * d.ny;
-            double c;
-            if (pl.periodic) v = rhs[i] - (m ? ch_apply<double, true, true>(p, x, y, d, k, c)
-                                             : ch_apply<double, true, false>(p, x, y, d, k, c));
-            else v = rhs[i] - (m ? ch_apply<double, false, true>(p, x, y, d, k, c)
-                                 : ch_apply<double, false, false>(p, x, y, d, k, c));
-        }
+        const double v = chb_stat_value(pl, k, m, which, i);
         if (isfinite(v)) {
             s += v;
             q += v * v;
@@ -1323,6 +1359,66 @@ __global__ void __launch_bounds__(CHB_THREADS) chb_stats_kernel(const afsi_mg2d_
         __syncthreads();
     }
     if (threadIdx.x < 4) out[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// The same statistics for instances whose level 0 runs as batched launches
+// (level-0 work list): partial [sum, squares, |.|, non-finite] per chunk in
+// a fixed order, then mlb_stats_final_kernel adds an instance's partials in
+// chunk order. which 3 / 4: subtract out[4i] / cells from the chunk.
+__global__ void __launch_bounds__(MLB_THREADS) mlb_stats_kernel(const afsi_mg2d_plan* __restrict__ plans,
+                                                                const MlbEntry* __restrict__ list, int n, int which,
+                                                                double* __restrict__ partials,
+                                                                double* __restrict__ out) {
+    __shared__ double red[4][MLB_THREADS];
+    const MlbEntry e = mlb_find(list, n);
+    const afsi_mg2d_plan& pl = plans[e.inst];
+    const int cells = int(pl.n[0][0] * pl.n[0][1]);
+    const int begin = (int(blockIdx.x) - e.block_begin) * MLB_CHUNK;
+    const int end = begin + MLB_CHUNK < cells ? begin + MLB_CHUNK : cells;
+    if (which >= 3) {
+        if (!pl.periodic) return;
+        double* x = static_cast<double*>(which == 3 ? pl.rhs[0] : pl.p[0]);
+        const double mean = out[4 * e.inst] / double(cells);
+        for (int i = begin + int(threadIdx.x); i < end; i += MLB_THREADS) x[i] = x[i] - mean;
+        return;
+    }
+    const Coef2<double> k = coef2_of<double>(pl.h2[0][0], pl.h2[0][1], 0.0, 0.0);
+    const bool m = k.exact();
+    double s = 0.0, q = 0.0, a = 0.0, bad = 0.0;
+    for (int i = begin + int(threadIdx.x); i < end; i += MLB_THREADS) {
+        const double v = chb_stat_value(pl, k, m, which, i);
+        if (isfinite(v)) {
+            s += v;
+            q += v * v;
+            a += fabs(v);
+        } else {
+            bad += 1.0;
+        }
+    }
+    red[0][threadIdx.x] = s;
+    red[1][threadIdx.x] = q;
+    red[2][threadIdx.x] = a;
+    red[3][threadIdx.x] = bad;
+    __syncthreads();
+    for (int w = MLB_THREADS / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w)
+            for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) partials[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void mlb_stats_final_kernel(const afsi_mg2d_plan* __restrict__ plans, const MlbEntry* __restrict__ list,
+                                       int n, const double* __restrict__ partials, double* __restrict__ out) {
+    const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= 4 * n) return;
+    const MlbEntry e = list[t / 4];
+    const int c = t % 4;
+    const afsi_mg2d_plan& pl = plans[e.inst];
+    const int blocks = int((pl.n[0][0] * pl.n[0][1] + MLB_CHUNK - 1) / MLB_CHUNK);
+    double v = 0.0;
+    for (int b = 0; b < blocks; ++b) v += partials[4 * (e.block_begin + b) + c];
+    out[4 * e.inst + c] = v;
 }
 
 struct ChbLayout {
@@ -1411,7 +1507,7 @@ int chb_layout(const afsi_chb_batch* b, ChbLayout& L, std::vector<ChbDesc>* desc
     L.cell_begin = off; off = chb_align(off + size_t(L.n) * sizeof(int64_t));
     L.split = off; off = chb_align(off + size_t(L.n) * sizeof(int32_t));
     L.lists = off; off = chb_align(off + size_t(AFSI_MG_MAX_LEVELS) * L.n * sizeof(MlbEntry));
-    L.partials = off; off = chb_align(off + size_t(max_blocks) * sizeof(double));
+    L.partials = off; off = chb_align(off + 4 * size_t(max_blocks) * sizeof(double));
     L.total = off;
     return OK;
 }
@@ -1472,7 +1568,9 @@ int afsi_chb_stencil_f64(const afsi_chb_batch* batch, int64_t* ids_u, double* w_
     if (int r = chb_ready(batch, L)) return r;
     if (L.points == 0) return OK;
     AFSI_REQUIRE(ids_u && w_u && ids_v && w_v && flags, "null channel stencil buffer");
-    chb_stencil_kernel<<<blocks_for(2 * L.points), CH_THREADS, 0, hipStream_t(stream)>>>(
+    const int64_t stencil_blocks = 2 * cdiv(L.points, CH_THREADS);
+    chb_stencil_kernel<<<unsigned(stencil_blocks < 8192 ? stencil_blocks : 8192), CH_THREADS, 0,
+                         hipStream_t(stream)>>>(
         chb_at<ChbDesc>(batch, L.desc), L.n, L.points, ids_u, w_u, ids_v, w_v, flags);
     AFSI_LAUNCH_CHECK();
     return OK;
@@ -1553,9 +1651,19 @@ int afsi_chb_stats_f64(const afsi_chb_batch* batch, int which, double* out, void
     ChbLayout L;
     if (int r = chb_ready(batch, L)) return r;
     AFSI_REQUIRE(which >= 0 && which <= 4 && out, "invalid channel batch statistics request");
-    chb_stats_kernel<<<unsigned(L.n), CHB_THREADS, 0, hipStream_t(stream)>>>(
-        chb_at<afsi_mg2d_plan>(batch, L.plans), which, out);
+    const hipStream_t s = hipStream_t(stream);
+    const afsi_mg2d_plan* plans = chb_at<afsi_mg2d_plan>(batch, L.plans);
+    chb_stats_kernel<<<unsigned(L.n), CHB_THREADS, 0, s>>>(plans, chb_at<int32_t>(batch, L.split), which, out);
     AFSI_LAUNCH_CHECK();
+    if (L.maxsplit > 0) {
+        const MlbEntry* list = chb_at<MlbEntry>(batch, L.lists);
+        double* partials = chb_at<double>(batch, L.partials);
+        mlb_stats_kernel<<<unsigned(L.blocks[0]), MLB_THREADS, 0, s>>>(plans, list, L.count[0], which, partials, out);
+        if (which < 3)
+            mlb_stats_final_kernel<<<unsigned(cdiv(4 * L.count[0], 256)), 256, 0, s>>>(plans, list, L.count[0],
+                                                                                       partials, out);
+        AFSI_LAUNCH_CHECK();
+    }
     return OK;
 }
 
