@@ -265,6 +265,8 @@ _SIGNATURES = {
     'afsi_chb_correct_f64': (c_int, [POINTER(CHBBatch), _P, _P, c_double, _P]),
     'afsi_chb_cycles_f64': (c_int, [POINTER(CHBBatch), _P, c_int32, _P]),
     'afsi_chb_stats_f64': (c_int, [POINTER(CHBBatch), c_int, _P, _P]),
+    'afsi_chb_spread_workspace_bytes': (c_size_t, [POINTER(CHBBatch)]),
+    'afsi_chb_spread_f64': (c_int, [POINTER(CHBBatch), _P, _P, _P, _P, c_size_t, _P, _P]),
     'afsi_p1_contract_f64': (c_int, [POINTER(FEGeometry), _P, _P, c_int64, c_int64, c_int, POINTER(Grid3),
                                      POINTER(HashTable), _P]),
     'afsi_csr_spmv_f64': (c_int, [POINTER(CSR), _P, c_int64, _P, c_int64, c_double, _P]),

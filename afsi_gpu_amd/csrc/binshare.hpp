@@ -35,6 +35,11 @@ size_t bin_workspace_bytes(int64_t points);
 void sort_key_slots(void* workspace, int64_t n, uint32_t** keys, uint32_t** vals);
 int sort_pairs(int64_t n, int64_t space, void* workspace, uint32_t* keys_sorted, int32_t* perm,
                int32_t* ptr, hipStream_t s);
+// chunk_start[t] = sum over tiles before t of ceil(count_t / chunk), with
+// count_t = cell_ptr[(t + 1) * tvol] - cell_ptr[t * tvol], and
+// chunk_start[ntiles] the total (one workgroup).
+int launch_chunk_scan(const int32_t* cell_ptr, int64_t ntiles, int64_t tvol, int chunk, int32_t* chunk_start,
+                      hipStream_t s);
 // keys/vals of all points of a batch into the sort workspace, then the
 // stable radix sort and cell_ptr over [0, key_space].
 int bin_batch(const BinDesc* desc_device, int count, int64_t total_points, int64_t key_space,

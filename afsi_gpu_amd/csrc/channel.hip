@@ -13,7 +13,7 @@
 // kernel evaluates the reference's expression in the reference's order
 // (mac2d/_triton_mg.py:7-75), so results are bit-identical to the
 // restatement in FP64 (compiled with -ffp-contract=off).
-#include "common.hpp"
+#include "binshare.hpp"
 #include <cstring>
 #include <vector>
 
@@ -903,6 +903,32 @@ __device__ inline int chb_find(const ChbDesc* __restrict__ desc, int n, int64_t 
 // every store instruction 64 scattered words).
 constexpr int CHS_PAD = 17;
 
+// A point's support check and its stencil base for component c (the
+// reference's reflected 16-link stencil, mac2d/transfer.py:83-101).
+__device__ inline bool chb_point_base(const ChbDesc& D, int c, double px, double py, double& sx, double& sy,
+                                      int64_t& bx, int64_t& by) {
+    bool ok = isfinite(px) && isfinite(py) && py >= -D.hy && py <= D.Ly + D.hy;
+    if (!D.periodic) ok = ok && px >= 2 * D.hx && px <= D.Lx - 2 * D.hx;
+    if (!ok) return false;
+    const double ox = D.hx * (c == 0 ? 0.0 : 0.5), oy = D.hy * (c == 1 ? 0.0 : 0.5);
+    sx = (px - ox) / D.hx;
+    sy = (py - oy) / D.hy;
+    bx = int64_t(floor(sx - 1));
+    by = int64_t(floor(sy - 1));
+    return true;
+}
+
+// Row `row` of component c's face lattice reflected at the walls; outside:
+// the link's weight is negated.
+__device__ inline int64_t chb_reflect(int c, int64_t ny, int64_t row, bool& outside) {
+    if (c == 0) {
+        outside = row < 0 || row >= ny;
+        return row < 0 ? -row - 1 : (row >= ny ? 2 * ny - row - 1 : row);
+    }
+    outside = row < 0 || row > ny;
+    return row < 0 ? -row : (row > ny ? 2 * ny - row : row);
+}
+
 __global__ void __launch_bounds__(CH_THREADS) chb_stencil_kernel(const ChbDesc* __restrict__ desc, int n,
                                                                  int64_t total, int64_t* __restrict__ ids_u,
                                                                  double* __restrict__ w_u, int64_t* __restrict__ ids_v,
@@ -921,8 +947,9 @@ __global__ void __launch_bounds__(CH_THREADS) chb_stencil_kernel(const ChbDesc* 
             const int64_t q = g - D.point_begin;
             const double px = D.x[2 * q], py = D.x[2 * q + 1];
             const Dim2 d = D.d;
-            bool ok = isfinite(px) && isfinite(py) && py >= -D.hy && py <= D.Ly + D.hy;
-            if (!D.periodic) ok = ok && px >= 2 * D.hx && px <= D.Lx - 2 * D.hx;
+            double sx, sy;
+            int64_t bx, by;
+            const bool ok = chb_point_base(D, c, px, py, sx, sy, bx, by);
             const int64_t off = c == 0 ? D.u_off : D.v_off;
             if (!ok) {
                 raise_flag(flags, F_SUPPORT);
@@ -931,9 +958,6 @@ __global__ void __launch_bounds__(CH_THREADS) chb_stencil_kernel(const ChbDesc* 
                     w[k] = 0.0;
                 }
             } else {
-                const double ox = D.hx * (c == 0 ? 0.0 : 0.5), oy = D.hy * (c == 1 ? 0.0 : 0.5);
-                const double sx = (px - ox) / D.hx, sy = (py - oy) / D.hy;
-                const int64_t bx = int64_t(floor(sx - 1)), by = int64_t(floor(sy - 1));
                 const int64_t fny = d.ny + (c == 1);
                 double fx[4], fy[4];
                 for (int a = 0; a < 4; ++a) {
@@ -944,16 +968,8 @@ __global__ void __launch_bounds__(CH_THREADS) chb_stencil_kernel(const ChbDesc* 
                     int64_t nxi = bx + i;
                     if (D.periodic) nxi = ((nxi % d.nx) + d.nx) % d.nx;
                     for (int j = 0; j < 4; ++j) {
-                        const int64_t row = by + j;
                         bool outside;
-                        int64_t refl;
-                        if (c == 0) {
-                            outside = row < 0 || row >= d.ny;
-                            refl = row < 0 ? -row - 1 : (row >= d.ny ? 2 * d.ny - row - 1 : row);
-                        } else {
-                            outside = row < 0 || row > d.ny;
-                            refl = row < 0 ? -row : (row > d.ny ? 2 * d.ny - row : row);
-                        }
+                        const int64_t refl = chb_reflect(c, d.ny, by + j, outside);
                         const double wt = fx[i] * fy[j];
                         ids[4 * i + j] = off + nxi * fny + refl;
                         w[4 * i + j] = outside ? -wt : wt;
@@ -975,6 +991,146 @@ __global__ void __launch_bounds__(CH_THREADS) chb_stencil_kernel(const ChbDesc* 
         __syncthreads();
         for (int e = threadIdx.x; e < rows * 16; e += CH_THREADS)
             out_w[e] = __builtin_bit_cast(double, stage[(e >> 4) * CHS_PAD + (e & 15)]);
+        __syncthreads();
+    }
+}
+
+// ---- tiled channel spread ---------------------------------------------------
+// f_c[face] = sum over links of (force_c * w) for the whole batch, the links
+// being the stencil kernel's (same ids and weights). The (point, component)
+// pairs are stably sorted by the tile of their stencil base: tiles of
+// CHS_T x CHS_T bases per instance and component, the x base wrapped on
+// periodic channels, the y base offset by 3 (bases lie in [-3, ny]). One
+// work item per (tile, chunk of <= CHS_CHUNK pairs) adds its pairs' 16 links
+// into an FP64 LDS box of the tile's (CHS_T + 3)^2 unreflected face
+// positions, then adds every nonzero entry to its reflected face in HBM with
+// one atomic. As with the reference's index_add_, the order of the additions
+// into a face is not fixed.
+constexpr int CHS_T = 16;
+constexpr int CHS_BOX = CHS_T + 3;
+constexpr int CHS_CHUNK = 512;
+constexpr int CHS_THREADS = 256;
+
+struct ChsTile {
+    int32_t begin, ntx, nty, reserved;
+};
+
+// x base of the key: wrapped on periodic channels (the box unwraps it).
+__device__ inline int64_t chs_key_x(const ChbDesc& D, int64_t bx) {
+    return D.periodic ? ((bx % D.d.nx) + D.d.nx) % D.d.nx : bx;
+}
+
+__global__ void chs_keys_kernel(const ChbDesc* __restrict__ desc, int n, int64_t total,
+                                const ChsTile* __restrict__ tiles, int64_t space, uint32_t* __restrict__ keys,
+                                uint32_t* __restrict__ vals) {
+    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < 2 * total;
+         t += int64_t(gridDim.x) * blockDim.x) {
+        const int c = int(t / total);
+        const int64_t g = t - c * total;
+        const int k = chb_find(desc, n, g, 0);
+        const ChbDesc& D = desc[k];
+        const int64_t q = g - D.point_begin;
+        double sx, sy;
+        int64_t bx, by;
+        uint32_t key = uint32_t(space);  // unsupported points: sorted last, in no tile
+        if (chb_point_base(D, c, D.x[2 * q], D.x[2 * q + 1], sx, sy, bx, by)) {
+            const ChsTile T = tiles[2 * k + c];
+            const int64_t kx = chs_key_x(D, bx), ky = by + 3;
+            if (kx >= 0 && kx / CHS_T < T.ntx && ky >= 0 && ky / CHS_T < T.nty)
+                key = uint32_t(T.begin + (kx / CHS_T) * T.nty + ky / CHS_T);
+        }
+        keys[t] = key;
+        vals[t] = uint32_t(t);
+    }
+}
+
+__global__ void __launch_bounds__(CHS_THREADS) chs_spread_kernel(
+    const ChbDesc* __restrict__ desc, int n, const ChsTile* __restrict__ tiles, const int32_t* __restrict__ perm,
+    const int32_t* __restrict__ tile_ptr, const int32_t* __restrict__ chunk_start, int64_t ntiles, int64_t total,
+    const double* __restrict__ force, double* __restrict__ fu, double* __restrict__ fv, int* __restrict__ flags) {
+    __shared__ double box[CHS_BOX * CHS_BOX];
+    const int items = chunk_start[ntiles];
+    for (int w = blockIdx.x; w < items; w += gridDim.x) {
+        int lo = 0, hi = int(ntiles);
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (chunk_start[mid] <= w) lo = mid;
+            else hi = mid;
+        }
+        const int t = lo;
+        const int pbeg = tile_ptr[t] + (w - chunk_start[t]) * CHS_CHUNK;
+        const int pend = min(tile_ptr[t + 1], pbeg + CHS_CHUNK);
+        int a = 0, b = 2 * n;  // (instance, component) of tile t: last begin <= t
+        while (b - a > 1) {
+            const int mid = (a + b) >> 1;
+            if (tiles[mid].begin <= t) a = mid;
+            else b = mid;
+        }
+        const int inst = a / 2, c = a % 2;
+        const ChbDesc& D = desc[inst];
+        const ChsTile T = tiles[a];
+        const int local = t - T.begin;
+        const int64_t col0 = int64_t(local / T.nty) * CHS_T, row0 = int64_t(local % T.nty) * CHS_T - 3;
+        for (int e = threadIdx.x; e < CHS_BOX * CHS_BOX; e += CHS_THREADS) box[e] = 0.0;
+        __syncthreads();
+        // lanes of one cell walk their 16 links from different rotations
+        const int ri = (threadIdx.x >> 2) & 3, rj = threadIdx.x & 3;
+        for (int j = pbeg + int(threadIdx.x); j < pend; j += CHS_THREADS) {
+            const int64_t g = int64_t(perm[j]) - c * total;
+            const int64_t q = g - D.point_begin;
+            double sx, sy;
+            int64_t bx, by;
+            if (!chb_point_base(D, c, D.x[2 * q], D.x[2 * q + 1], sx, sy, bx, by)) continue;
+            const int64_t lx = chs_key_x(D, bx) - col0, ly = by - row0;
+            if (lx < 0 || lx >= CHS_T || ly < 0 || ly >= CHS_T) {
+                raise_flag(flags, F_INDEX);
+                continue;
+            }
+            double fx[4], fy[4];
+            bool out[4];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                fx[s4] = peskin_phi(sx - double(bx + s4));
+                fy[s4] = peskin_phi(sy - double(by + s4));
+                chb_reflect(c, D.d.ny, by + s4, out[s4]);
+            }
+            double wxr[4], wyr[4];
+            bool outr[4];
+            int oxr[4], oyr[4];
+#pragma unroll
+            for (int q4 = 0; q4 < 4; ++q4) {
+                const int iq = (q4 + ri) & 3, jq = (q4 + rj) & 3;
+                wxr[q4] = iq == 0 ? fx[0] : (iq == 1 ? fx[1] : (iq == 2 ? fx[2] : fx[3]));
+                wyr[q4] = jq == 0 ? fy[0] : (jq == 1 ? fy[1] : (jq == 2 ? fy[2] : fy[3]));
+                outr[q4] = jq == 0 ? out[0] : (jq == 1 ? out[1] : (jq == 2 ? out[2] : out[3]));
+                oxr[q4] = int(lx + iq) * CHS_BOX;
+                oyr[q4] = int(ly + jq);
+            }
+            const double f = force[2 * g + c];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    const double wt = wxr[i] * wyr[jj];
+                    atomicAdd(box + oxr[i] + oyr[jj], f * (outr[jj] ? -wt : wt));
+                }
+        }
+        __syncthreads();
+        double* out = c == 0 ? fu + D.u_off : fv + D.v_off;
+        const int64_t fny = D.d.ny + c, count = c == 0 ? D.u_count : D.v_count;
+        for (int e = threadIdx.x; e < CHS_BOX * CHS_BOX; e += CHS_THREADS) {
+            const double v = box[e];
+            if (v == 0.0) continue;
+            int64_t col = col0 + e / CHS_BOX;
+            if (D.periodic) col = col % D.d.nx;
+            bool outside;
+            const int64_t id = col * fny + chb_reflect(c, D.d.ny, row0 + e % CHS_BOX, outside);
+            if (id < 0 || id >= count) {
+                raise_flag(flags, F_INDEX);
+                continue;
+            }
+            unsafeAtomicAdd(out + id, v);
+        }
         __syncthreads();
     }
 }
@@ -1424,7 +1580,8 @@ __global__ void mlb_stats_final_kernel(const afsi_mg2d_plan* __restrict__ plans,
 struct ChbLayout {
     int n = 0;
     int64_t points = 0, nu = 0, nv = 0, cells = 0;
-    size_t desc = 0, plans = 0, cell_begin = 0, split = 0, lists = 0, partials = 0, total = 0;
+    size_t desc = 0, plans = 0, cell_begin = 0, split = 0, lists = 0, partials = 0, tiles = 0, total = 0;
+    int64_t ntiles = 0;  // spread tiles over all instances and both components
     // batched upper levels: levels 0..maxsplit-1 have work lists
     int maxsplit = 0;
     int count[AFSI_MG_MAX_LEVELS] = {}, blocks[AFSI_MG_MAX_LEVELS] = {}, max_nu[AFSI_MG_MAX_LEVELS] = {};
@@ -1444,7 +1601,7 @@ size_t chb_align(size_t b) { return (b + 255) & ~size_t(255); }
 
 int chb_layout(const afsi_chb_batch* b, ChbLayout& L, std::vector<ChbDesc>* descs,
                std::vector<int64_t>* cell_begin, std::vector<int32_t>* splits = nullptr,
-               std::vector<MlbEntry>* lists = nullptr) {
+               std::vector<MlbEntry>* lists = nullptr, std::vector<ChsTile>* tiles = nullptr) {
     AFSI_REQUIRE(b && b->items && b->count >= 1, "invalid channel batch");
     L.n = b->count;
     for (int i = 0; i < L.n; ++i) {
@@ -1473,6 +1630,13 @@ int chb_layout(const afsi_chb_batch* b, ChbLayout& L, std::vector<ChbDesc>* desc
         d.cell_count = g.n[0] * g.n[1];
         if (descs) descs->push_back(d);
         if (cell_begin) cell_begin->push_back(L.cells);
+        for (int c = 0; c < 2; ++c) {
+            // key x: wrapped base (periodic) or base <= n (closed); key y = base + 3 in [0, ny + 4)
+            const int64_t kx = g.periodic ? g.n[0] : g.n[0] + 1, ky = g.n[1] + 4;
+            const ChsTile t{int32_t(L.ntiles), int32_t(cdiv(kx, CHS_T)), int32_t(cdiv(ky, CHS_T)), 0};
+            if (tiles) tiles->push_back(t);
+            L.ntiles += int64_t(t.ntx) * t.nty;
+        }
         const int sp = mlb_split(pl);
         if (splits) splits->push_back(sp);
         if (sp > L.maxsplit) L.maxsplit = sp;
@@ -1483,7 +1647,9 @@ int chb_layout(const afsi_chb_batch* b, ChbLayout& L, std::vector<ChbDesc>* desc
         L.nv += d.v_count;
         L.cells += d.cell_count;
     }
-    AFSI_REQUIRE(L.points < (int64_t(1) << 40) && L.nu < (int64_t(1) << 40), "channel batch too large");
+    AFSI_REQUIRE(L.points < (int64_t(1) << 29) && L.nu < (int64_t(1) << 31) && L.nv < (int64_t(1) << 31) &&
+                     L.ntiles < (int64_t(1) << 30),
+                 "channel batch too large");
     if (lists) lists->assign(size_t(AFSI_MG_MAX_LEVELS) * L.n, MlbEntry{0, 0});
     int max_blocks = 1;
     for (int l = 0; l < L.maxsplit; ++l) {
@@ -1508,6 +1674,7 @@ int chb_layout(const afsi_chb_batch* b, ChbLayout& L, std::vector<ChbDesc>* desc
     L.split = off; off = chb_align(off + size_t(L.n) * sizeof(int32_t));
     L.lists = off; off = chb_align(off + size_t(AFSI_MG_MAX_LEVELS) * L.n * sizeof(MlbEntry));
     L.partials = off; off = chb_align(off + 4 * size_t(max_blocks) * sizeof(double));
+    L.tiles = off; off = chb_align(off + 2 * size_t(L.n) * sizeof(ChsTile));
     L.total = off;
     return OK;
 }
@@ -1546,7 +1713,8 @@ int afsi_chb_prepare(const afsi_chb_batch* batch, void* stream) {
     std::vector<int64_t> begins;
     std::vector<int32_t> splits;
     std::vector<MlbEntry> lists;
-    if (int r = chb_layout(batch, L, &descs, &begins, &splits, &lists)) return r;
+    std::vector<ChsTile> tiles;
+    if (int r = chb_layout(batch, L, &descs, &begins, &splits, &lists, &tiles)) return r;
     AFSI_REQUIRE(batch->workspace && batch->workspace_bytes >= L.total, "channel batch workspace too small");
     std::vector<char> host(L.total, 0);
     std::memcpy(host.data() + L.desc, descs.data(), descs.size() * sizeof(ChbDesc));
@@ -1556,6 +1724,7 @@ int afsi_chb_prepare(const afsi_chb_batch* batch, void* stream) {
     std::memcpy(host.data() + L.cell_begin, begins.data(), begins.size() * sizeof(int64_t));
     std::memcpy(host.data() + L.split, splits.data(), splits.size() * sizeof(int32_t));
     std::memcpy(host.data() + L.lists, lists.data(), lists.size() * sizeof(MlbEntry));
+    std::memcpy(host.data() + L.tiles, tiles.data(), tiles.size() * sizeof(ChsTile));
     const hipStream_t s = hipStream_t(stream);
     AFSI_HIP(hipMemcpyAsync(batch->workspace, host.data(), host.size(), hipMemcpyHostToDevice, s));
     AFSI_HIP(hipStreamSynchronize(s));
@@ -1572,6 +1741,54 @@ int afsi_chb_stencil_f64(const afsi_chb_batch* batch, int64_t* ids_u, double* w_
     chb_stencil_kernel<<<unsigned(stencil_blocks < 8192 ? stencil_blocks : 8192), CH_THREADS, 0,
                          hipStream_t(stream)>>>(
         chb_at<ChbDesc>(batch, L.desc), L.n, L.points, ids_u, w_u, ids_v, w_v, flags);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
+// Spread workspace: the sort of 2 * points (key, value) pairs, the sorted
+// keys and values, then tile_ptr and chunk_start over ntiles + 1 entries.
+static size_t chs_align(size_t b) { return (b + 255) & ~size_t(255); }
+
+size_t afsi_chb_spread_workspace_bytes(const afsi_chb_batch* batch) {
+    ChbLayout L;
+    if (chb_layout(batch, L, nullptr, nullptr)) return 0;
+    const int64_t m = 2 * L.points > 0 ? 2 * L.points : 1;
+    return chs_align(bin_workspace_bytes(m)) + 2 * chs_align(size_t(m) * 4) + 2 * chs_align(size_t(L.ntiles + 1) * 4);
+}
+
+int afsi_chb_spread_f64(const afsi_chb_batch* batch, const double* force, double* fu, double* fv, void* workspace,
+                        size_t workspace_bytes, int* flags, void* stream) {
+    ChbLayout L;
+    if (int r = chb_ready(batch, L)) return r;
+    AFSI_REQUIRE(fu && fv && flags, "null channel spread buffer");
+    const hipStream_t s = hipStream_t(stream);
+    if (int r = afsi_fill_f64(fu, L.nu, 0.0, stream)) return r;
+    if (int r = afsi_fill_f64(fv, L.nv, 0.0, stream)) return r;
+    if (L.points == 0) return OK;
+    AFSI_REQUIRE(force && workspace, "null channel spread buffer");
+    const size_t need = afsi_chb_spread_workspace_bytes(batch);
+    AFSI_REQUIRE(need > 0 && workspace_bytes >= need, "channel spread workspace too small");
+    const int64_t m = 2 * L.points;
+    char* ws = static_cast<char*>(workspace);
+    size_t off = chs_align(bin_workspace_bytes(m));
+    uint32_t* keys = reinterpret_cast<uint32_t*>(ws + off);
+    off += chs_align(size_t(m) * 4);
+    int32_t* perm = reinterpret_cast<int32_t*>(ws + off);
+    off += chs_align(size_t(m) * 4);
+    int32_t* tile_ptr = reinterpret_cast<int32_t*>(ws + off);
+    off += chs_align(size_t(L.ntiles + 1) * 4);
+    int32_t* chunk_start = reinterpret_cast<int32_t*>(ws + off);
+    uint32_t *k0, *v0;
+    sort_key_slots(ws, m, &k0, &v0);
+    const ChbDesc* desc = chb_at<ChbDesc>(batch, L.desc);
+    const ChsTile* tiles = chb_at<ChsTile>(batch, L.tiles);
+    chs_keys_kernel<<<grid_for(m, 256, 8192), 256, 0, s>>>(desc, L.n, L.points, tiles, L.ntiles, k0, v0);
+    AFSI_LAUNCH_CHECK();
+    if (int r = sort_pairs(m, L.ntiles, ws, keys, perm, tile_ptr, s)) return r;
+    if (int r = launch_chunk_scan(tile_ptr, L.ntiles, 1, CHS_CHUNK, chunk_start, s)) return r;
+    const int64_t upper = L.ntiles + m / CHS_CHUNK + 1;
+    chs_spread_kernel<<<unsigned(upper < 8192 ? upper : 8192), CHS_THREADS, 0, s>>>(
+        desc, L.n, tiles, perm, tile_ptr, chunk_start, L.ntiles, L.points, force, fu, fv, flags);
     AFSI_LAUNCH_CHECK();
     return OK;
 }

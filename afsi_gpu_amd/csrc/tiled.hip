@@ -640,6 +640,13 @@ static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 namespace afsi {
 
+int launch_chunk_scan(const int32_t* cell_ptr, int64_t ntiles, int64_t tvol, int chunk, int32_t* chunk_start,
+                      hipStream_t s) {
+    chunk_scan_kernel<<<1, 1024, 0, s>>>(cell_ptr, ntiles, tvol, chunk, chunk_start);
+    AFSI_LAUNCH_CHECK();
+    return OK;
+}
+
 static int cubic_edge(const int32_t tile[3]) {
     if (tile[0] == tile[1] && tile[1] == tile[2] && (tile[0] == 8 || tile[0] == 4)) return tile[0];
     return 0;

@@ -7,8 +7,11 @@ instances:
 
 * one stencil launch for every point of every instance (global face ids
   into the concatenated u and v face arrays);
-* one deterministic indexed spread and one gather per component over the
-  whole batch (``cuda_ib.indexed_spread`` / ``indexed_gather``);
+* one spread for both components of every point (``afsi_chb_spread_f64``:
+  points binned by stencil-base tile, FP64 LDS accumulation, one atomic per
+  face and tile; ``spread='indexed'`` selects the deterministic
+  ``cuda_ib.indexed_spread`` per component instead) and one indexed gather
+  per component (``cuda_ib.indexed_gather``);
 * the predictor and the boundary() of mac2d/grid.py:74-79 as masked
   elementwise updates of the concatenated faces;
 * one divergence and one gradient-correction launch;
@@ -32,9 +35,12 @@ from .multigrid import ChannelMultigrid
 
 
 class BatchedChannelStep:
-    def __init__(self, grids, counts, *, dt, rho=1., device='cuda', options=None):
+    def __init__(self, grids, counts, *, dt, rho=1., device='cuda', options=None, spread='tiled'):
         if not grids or len(grids) != len(counts):
             raise ValueError('one point count per channel grid required')
+        if spread not in ('tiled', 'indexed'):
+            raise ValueError("spread must be 'tiled' or 'indexed'")
+        self.spread = spread
         self.device = torch.device(device)
         self.grids = list(grids)
         self.counts = [int(c) for c in counts]
@@ -70,8 +76,12 @@ class BatchedChannelStep:
         self.batch.workspace = self.workspace.data_ptr()
         self.batch.workspace_bytes = nbytes.value
         _abi.call('afsi_chb_prepare', byref(self.batch), _abi.stream())
+        nbytes = int(_abi.load().afsi_chb_spread_workspace_bytes(byref(self.batch)))
+        self.spread_workspace = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.device)
         self.u = torch.zeros(self.nu, **kw)
         self.v = torch.zeros(self.nv, **kw)
+        self.fu = torch.empty(self.nu, **kw)
+        self.fv = torch.empty(self.nv, **kw)
         self.ids = [torch.empty((total, 16), dtype=torch.int64, device=self.device) for _ in range(2)]
         self.w = [torch.empty((total, 16), **kw) for _ in range(2)]
         self.flags = torch.zeros((), dtype=torch.int32, device=self.device)
@@ -175,14 +185,22 @@ class BatchedChannelStep:
         if int(self.flags.item()) & 1:
             raise ValueError('valve IB points leave the channel support region')
         scaled = self.force_all * self.inv_volume
-        fu = cuda_ib.indexed_spread(scaled[:, :1], self.ids[0], self.w[0], self.nu, check=False)
-        fv = cuda_ib.indexed_spread(scaled[:, 1:], self.ids[1], self.w[1], self.nv, check=False)
+        if self.spread == 'tiled':
+            _abi.call('afsi_chb_spread_f64', byref(self.batch), _abi.ptr(scaled), _abi.ptr(self.fu),
+                      _abi.ptr(self.fv), _abi.ptr(self.spread_workspace), self.spread_workspace.numel(),
+                      _abi.ptr(self.flags), _abi.stream())
+            fu, fv = self.fu, self.fv
+        else:
+            fu = cuda_ib.indexed_spread(scaled[:, :1], self.ids[0], self.w[0], self.nu, check=False)
+            fv = cuda_ib.indexed_spread(scaled[:, 1:], self.ids[1], self.w[1], self.nv, check=False)
         coef = self.dt / self.rho
         self.u.add_(coef * fu.view(-1) * self.u_mask)
         self.v.add_(coef * fv.view(-1)).mul_(self.v_mask)
         _abi.call('afsi_chb_divergence_f64', byref(self.batch), _abi.ptr(self.u), _abi.ptr(self.v),
                   -self.rho / self.dt, _abi.stream())
         cycles, res, tol = self._solve()
+        if int(self.flags.item()) & 2:
+            raise RuntimeError('channel spread: a stencil left its tile box')
         _abi.call('afsi_chb_correct_f64', byref(self.batch), _abi.ptr(self.u), _abi.ptr(self.v), coef,
                   _abi.stream())
         U = torch.cat((cuda_ib.indexed_gather(self.u.view(-1, 1), self.ids[0], self.w[0], check=False),

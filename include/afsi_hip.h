@@ -597,6 +597,17 @@ int afsi_chb_divergence_f64(const afsi_chb_batch* batch, const double* u, const 
 int afsi_chb_correct_f64(const afsi_chb_batch* batch, double* u, double* v, double coef, void* stream);
 int afsi_chb_cycles_f64(const afsi_chb_batch* batch, const int32_t* active, int32_t cycles, void* stream);
 int afsi_chb_stats_f64(const afsi_chb_batch* batch, int which, double* out, void* stream);
+/* fu (u_faces) and fv (v_faces) = the batch's face densities: every link of
+ * the stencil kernel's (P_total,16) arrays added, force[p,c] * w, with
+ * force (P_total,2) already divided by the cell volume (the two
+ * afsi_indexed_spread_f64 calls it replaces). Pairs (point, component) are
+ * sorted by stencil-base tile, each tile's links are summed in an FP64 LDS
+ * box and added to HBM with one atomic per nonzero entry, so the summation
+ * order is not fixed. Unsupported points are skipped (the stencil kernel
+ * flags them). workspace: afsi_chb_spread_workspace_bytes(batch) bytes. */
+size_t afsi_chb_spread_workspace_bytes(const afsi_chb_batch* batch);
+int afsi_chb_spread_f64(const afsi_chb_batch* batch, const double* force, double* fu, double* fv, void* workspace,
+                        size_t workspace_bytes, int* flags, void* stream);
 
 /* ---- explicit MAC predictor of the closed box ------------------------------
  * u* = zero_normal(u + dt (-N(u) + (mu/rho) L u + f/rho)) with u =
