@@ -35,7 +35,8 @@ class MGPlan3(Structure):
                 ('coarse_cells', c_int64), ('reduce_partials', c_void_p),
                 ('reduce_result', c_void_p), ('partials_capacity', c_int64),
                 ('zero_initial', c_int32), ('fuse_cells', c_int32),
-                ('sweep_tile', c_int32 * MAX_LEVELS), ('sweep_blocks', c_int32 * MAX_LEVELS)]
+                ('sweep_tile', c_int32 * MAX_LEVELS), ('sweep_blocks', c_int32 * MAX_LEVELS),
+                ('relax', c_double)]
 
 
 class TBItem(Structure):

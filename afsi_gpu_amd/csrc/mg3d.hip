@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(256) stencil_kernel(const T* __restrict__ p,
 // an even n2 (true on every smoothed level of the hierarchy).
 template <class T, bool P>
 __global__ void __launch_bounds__(256) rbgs_kernel(T* __restrict__ p, const T* __restrict__ rhs,
-                                                   Dims d, Coef<T> k, int color) {
+                                                   Dims d, Coef<T> k, int color, T relax) {
     const int64_t half = d.n2 / 2;
     const int64_t total = d.n0 * d.n1 * half;
     for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
@@ -49,7 +49,8 @@ __global__ void __launch_bounds__(256) rbgs_kernel(T* __restrict__ p, const T* _
         const int64_t i = (x * d.n1 + y) * d.n2 + z;
         T c;
         const T a = apply_A<T, P>(p, i, x, y, z, d, k, c);
-        p[i] = c + (rhs[i] - a) / diag_of<T, P>(x, y, z, d, k);
+        const T dp = (rhs[i] - a) / diag_of<T, P>(x, y, z, d, k);
+        p[i] = relax == T(1) ? c + dp : c + relax * dp;  // over-relaxed when relax != 1
     }
 }
 
@@ -411,7 +412,7 @@ static int stencil(const T* p, const T* rhs, T* out, const int64_t n[3], const d
 
 template <class T>
 static int rbgs(T* p, const T* rhs, const int64_t n[3], const double h2[3], const double w[3],
-                int color, int periodic, void* stream) {
+                int color, int periodic, void* stream, double relax = 1.0) {
     AFSI_REQUIRE(dims_ok(n) && h2 && w && p && rhs, "invalid multigrid arguments");
     AFSI_REQUIRE(n[2] % 2 == 0, "red-black smoothing needs an even z extent");
     AFSI_REQUIRE(!periodic || (n[0] % 2 == 0 && n[1] % 2 == 0), "periodic red-black needs even extents");
@@ -419,8 +420,9 @@ static int rbgs(T* p, const T* rhs, const int64_t n[3], const double h2[3], cons
     const Coef<T> k = make_coef<T>(h2, w);
     const unsigned g = grid_for(d.n0 * d.n1 * (d.n2 / 2), 256, 16384);
     const hipStream_t s = hipStream_t(stream);
-    if (periodic) rbgs_kernel<T, true><<<g, 256, 0, s>>>(p, rhs, d, k, color & 1);
-    else rbgs_kernel<T, false><<<g, 256, 0, s>>>(p, rhs, d, k, color & 1);
+    const T rw = T(relax > 0.0 ? relax : 1.0);
+    if (periodic) rbgs_kernel<T, true><<<g, 256, 0, s>>>(p, rhs, d, k, color & 1, rw);
+    else rbgs_kernel<T, false><<<g, 256, 0, s>>>(p, rhs, d, k, color & 1, rw);
     AFSI_LAUNCH_CHECK();
     return OK;
 }
@@ -683,7 +685,7 @@ struct PlanRunner {
         if (pl.smoother == 1) {
             for (int k = 0; k < pl.nu; ++k)
                 for (int color = 0; color < 2; ++color)
-                    if (int r = rbgs<T>(P(l), R(l), n, pl.h2[l], pl.inv_h2[l], color, pl.periodic, s))
+                    if (int r = rbgs<T>(P(l), R(l), n, pl.h2[l], pl.inv_h2[l], color, pl.periodic, s, pl.relax))
                         return r;
         } else {
             T* a = P(l);

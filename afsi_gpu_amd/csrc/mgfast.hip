@@ -43,8 +43,16 @@ struct Dim32 {
 
 template <class T>
 struct W3 {
-    T w[3];  // 1/h^2
+    T w[3];          // 1/h^2
+    T relax = T(1);  // red-black over-relaxation weight (1: Gauss-Seidel)
 };
+
+// Red-black update of a cell from its Gauss-Seidel value gs: gs itself, or
+// old + relax * (gs - old) when over-relaxed (relax is uniform per launch).
+template <class T>
+__device__ inline T rb_relax(const W3<T>& k, T old, T gs) {
+    return k.relax == T(1) ? gs : old + k.relax * (gs - old);
+}
 
 __device__ inline int wrap32(int g, int n) {
     while (g < 0) g += n;
@@ -181,7 +189,7 @@ __global__ void __launch_bounds__(NT) rb_sweep_kernel(const T* __restrict__ pin,
         const int li = (lx * BY + ly) * BZ + lz;
         T sum, dg;
         neighbour_sums<T, P>(box, li, BY * BZ, BZ, gx, gy, gz, d, k, sum, dg);
-        box[li] = (rr[j] + sum) / dg;
+        box[li] = rb_relax(k, box[li], (rr[j] + sum) / dg);
     }
     __syncthreads();
 #pragma unroll
@@ -196,7 +204,7 @@ __global__ void __launch_bounds__(NT) rb_sweep_kernel(const T* __restrict__ pin,
         if ((gx + gy + gz) & 1) {
             T sum, dg;
             neighbour_sums<T, P>(box, li, BY * BZ, BZ, gx, gy, gz, d, k, sum, dg);
-            v = (rt[j] + sum) / dg;
+            v = rb_relax(k, v, (rt[j] + sum) / dg);
         }
         pout[gx * sxg + gy * d.n2 + gz] = v;
     }
@@ -477,7 +485,7 @@ __device__ __forceinline__ void rb_rows_tiles(const T* __restrict__ pin, const T
         if (P || gx + 1 < d.n0) s += k.w[0] * box[(r + BYr) * n2 + z];
         if (P || gx > 0) s += k.w[0] * box[(r - BYr) * n2 + z];
         const T inv = P ? inv_p : tab[face_class(gx, d.n0) * 9 + face_class(gy, d.n1) * 3 + face_class(z, n2)];
-        return (b + s) * inv;
+        return rb_relax(k, row[z], (b + s) * inv);
     };
     for (int j = 0; j < count; ++j) {
         const int cur = j & 1;
@@ -849,7 +857,7 @@ __device__ __forceinline__ void fused_half_sweep(T* box, const T* rbx, int colou
             if (P || gx + 1 < d.n0) s += k.w[0] * box[(r + BY) * N2 + z];
             if (P || gx > 0) s += k.w[0] * box[(r - BY) * N2 + z];
             const T inv = P ? inv_p : tab[face_class(gx, d.n0) * 9 + face_class(gy, d.n1) * 3 + face_class(z, N2)];
-            val[g] = (b + s) * inv;
+            val[g] = rb_relax(k, row[z], (b + s) * inv);
             at[g] = r * N2 + z;
         }
 #pragma unroll
@@ -1396,7 +1404,7 @@ __device__ inline void lds_smooth(T* p, T* w, const T* r, const uint32_t* list, 
                         if (z + 1 < d.n2) s += k.w[2] * p[i + 1];
                         if (z > 0) s += k.w[2] * p[i - 1];
                     }
-                    p[i] = (r[i] + s) * lds_inv<T, P>(tab, inv_p, x, y, z, d);
+                    p[i] = rb_relax(k, p[i], (r[i] + s) * lds_inv<T, P>(tab, inv_p, x, y, z, d));
                 }
                 __syncthreads();
             }
@@ -1836,6 +1844,7 @@ struct FastRunner {
     W3<T> K(int l) const {
         W3<T> k;
         for (int a = 0; a < 3; ++a) k.w[a] = T(pl.inv_h2[l][a]);
+        k.relax = T(pl.relax > 0.0 ? pl.relax : 1.0);
         return k;
     }
 
@@ -2267,6 +2276,7 @@ static int batch_prepare(const afsi_mgb_batch* bt, hipStream_t s) {
             const Dim32 d{int(pl.n[l][0]), int(pl.n[l][1]), int(pl.n[l][2])};
             W3<float> k;
             for (int a = 0; a < 3; ++a) k.w[a] = float(pl.inv_h2[l][a]);
+            k.relax = float(pl.relax > 0.0 ? pl.relax : 1.0);
             float* P = static_cast<float*>(pl.p[l]);
             float* W = static_cast<float*>(pl.tmp[l]);
             const float* R = static_cast<const float*>(pl.rhs[l]);
@@ -2300,6 +2310,7 @@ static int batch_prepare(const afsi_mgb_batch* bt, hipStream_t s) {
             const int l = first + j;
             a.d[j] = Dim32{int(pl.n[l][0]), int(pl.n[l][1]), int(pl.n[l][2])};
             for (int q = 0; q < 3; ++q) a.k[j].w[q] = float(pl.inv_h2[l][q]);
+            a.k[j].relax = float(pl.relax > 0.0 ? pl.relax : 1.0);
             const int32_t c = int32_t(pl.n[l][0] * pl.n[l][1] * pl.n[l][2]);
             a.off_p[j] = o; o += c;
             a.off_r[j] = o; o += c;

@@ -33,6 +33,7 @@ class MGOptions:
     smooth: int = 4
     check_every: int = 2
     smoother: str = 'jacobi'
+    relax: float = 1.0  # build-own: red-black over-relaxation weight (1: Gauss-Seidel)
 
     def __post_init__(self):
         if (any(not isfinite(x) or x < 0 for x in (self.rtol, self.atol)) or
@@ -41,6 +42,8 @@ class MGOptions:
             raise ValueError('invalid multigrid options')
         if self.smoother not in ('jacobi', 'rbgs'):
             raise ValueError('smoother must be jacobi or rbgs')
+        if not (isfinite(self.relax) and 0 < self.relax < 2):
+            raise ValueError('relax must lie in (0, 2)')
 
 
 def hierarchy(shape, spacing):
@@ -131,6 +134,7 @@ class GeometricMultigrid:
         pl.smoother = 1 if self.options.smoother == 'rbgs' else 0
         pl.nu = self.options.smooth
         pl.omega = OMEGA
+        pl.relax = self.options.relax
         for l, (s, h) in enumerate(zip(self.shapes, self.spacings)):
             for a in range(3):
                 pl.n[l][a] = s[a]

@@ -325,6 +325,10 @@ typedef struct afsi_mg3d_plan {
      * every G-th tile, overlapping the next tile's loads with compute). */
     int32_t sweep_tile[AFSI_MG_MAX_LEVELS];
     int32_t sweep_blocks[AFSI_MG_MAX_LEVELS];
+    /* red-black over-relaxation: each red-black update p <- gs becomes
+     * p <- p + relax * (gs - p) (build-own; 0 or 1: plain Gauss-Seidel,
+     * the reference's red-black composition). */
+    double relax;
 } afsi_mg3d_plan;
 /* 1 if the fused driver accepts the plan (every level with <= 4096 cells
  * that it solves in LDS must have extents <= 64), else 0. */

@@ -34,6 +34,7 @@ class MGOptions:
     smooth: int = 4
     check_every: int = 2
     smoother: str = 'jacobi'   # build-own: 'rbgs'
+    relax: float = 1.0         # build-own: red-black over-relaxation weight
 
 
 # ---------------------------------------------------------------------------
@@ -191,15 +192,18 @@ class Multigrid:
         return p + (2 / 3) * (rhs - self.apply(level, p)) / self.diagonals[level]
 
     def rbgs(self, level, p, rhs):
-        """One red-black Gauss-Seidel sweep: cells with (i+j+k) even first."""
+        """One red-black Gauss-Seidel sweep: cells with (i+j+k) even first;
+        over-relaxed (p + relax * r / d) when options.relax != 1."""
         shape = p.shape
         parity = np.add.outer(np.add.outer(np.arange(shape[0]), np.arange(shape[1])),
                               np.arange(shape[2])) % 2
+        relax = self.options.relax
         p = p.copy()
         for color in (0, 1):
             r = rhs - self.apply(level, p)
             mask = parity == color
-            p[mask] = p[mask] + r[mask] / self.diagonals[level][mask]
+            step = r[mask] / self.diagonals[level][mask]
+            p[mask] = p[mask] + (step if relax == 1 else relax * step)
         return p
 
     def smooth(self, level, p, rhs):
