@@ -35,6 +35,7 @@ import torch
 from . import _abi, workloads
 from .batch import BatchedMixedMultigrid, MixedState
 from .mac import MACGrid, MGOptions
+from .mac.multigrid import MIXED_RELAX
 from .mac.flow import correct, divergence
 from .mac.multigrid import GeometricMultigrid
 from .mac.transfer import DEFAULT_TILE, Binner, TiledTransfer
@@ -194,7 +195,8 @@ class Server:
         self.device = torch.device(device)
         self.dt = dt
         self.budget = budget
-        self.options = options or MGOptions(smooth=2, smoother='rbgs')
+        self.options = options or MGOptions(smooth=2, smoother='rbgs',
+                                            relax=MIXED_RELAX if precision == 'mixed' else 1.)
         self.max_batch = max_batch
         self.precision = precision
         self.resident = {}

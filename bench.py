@@ -54,9 +54,6 @@ def measured_traffic(n):
             return float(line.split(':')[1])
     return None
 PHASES = ('bin', 'spread', 'project', 'interp')
-# Red-black over-relaxation weight of the mixed path's FP32 V-cycles (the
-# V(2,2) cycle's residual reduction falls from about 0.09 to 0.018 per cycle).
-MIXED_RELAX = 1.3
 
 
 def parse():
@@ -438,6 +435,7 @@ def batched_sample(torch, instances, steps=3, warmup=1):
     from afsi_gpu_amd import workloads
     from afsi_gpu_amd.batch import BatchedIBStep
     from afsi_gpu_amd.mac import MACGrid, MGOptions
+    from afsi_gpu_amd.mac.multigrid import MIXED_RELAX
     from afsi_gpu_amd.mac2d import ChannelGrid, ChannelIBStep
     from afsi_gpu_amd.mac2d.batch import BatchedChannelStep
     from afsi_gpu_amd.step import IBStep
@@ -637,6 +635,7 @@ def main():
         return
     from afsi_gpu_amd import workloads
     from afsi_gpu_amd.mac import MACGrid, MGOptions
+    from afsi_gpu_amd.mac.multigrid import MIXED_RELAX
     from afsi_gpu_amd.step import IBStep
 
     device = torch.device('cuda:0')

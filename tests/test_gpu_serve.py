@@ -151,7 +151,7 @@ def test_served_steps_match_single_instance_steps(gpu, precision):
     srv = Server(device=gpu, precision=precision)
     stats = srv.run(times, inst, specs, keep=keep)
     assert stats.summary()['requests'] == len(times)
-    opts = MGOptions(smooth=2, smoother='rbgs')
+    opts = srv.options  # the server's (over-relaxed red-black for the mixed path)
     for iid in keep:
         shape, count, seed = specs[iid]
         x0, F = workloads.instance_points(workloads.rng(seed), shape, count)
