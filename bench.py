@@ -42,14 +42,15 @@ HBM_PEAK = 8.0e12
 # HBM bytes per launch of the roofline kernel, measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes and scaled by a copy kernel of known size.
 TRAFFIC_FILE = ROOT / 'profiles' / 'rb_sweep_f32_128cubed_hbm_traffic.md'
+SPREAD_TRAFFIC_FILE = ROOT / 'profiles' / 'spread_128cubed_1p5M_hbm_traffic.md'
 
 
-def measured_traffic(n):
-    """traffic_bytes_per_launch from the counter summary (measured at
-    128^3), or None for another grid or when the summary is absent."""
-    if n != 128 or not TRAFFIC_FILE.exists():
+def measured_traffic(n, path=TRAFFIC_FILE):
+    """traffic_bytes_per_launch from a counter summary (measured at 128^3
+    with 1.5M points), or None for another grid or when it is absent."""
+    if n != 128 or not path.exists():
         return None
-    for line in TRAFFIC_FILE.read_text().splitlines():
+    for line in path.read_text().splitlines():
         if line.startswith('- traffic_bytes_per_launch:'):
             return float(line.split(':')[1])
     return None
@@ -207,6 +208,28 @@ def time_sweep(torch, solver, repeats=50, batches=5):
         start[i].record(stream)
         for _ in range(repeats):
             _abi.call(name, solver.plan, 0, _abi.stream())
+        end[i].record(stream)
+    torch.cuda.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in zip(start, end)])) / 1e3 / repeats
+
+
+def time_spread(torch, stepper, x, F, repeats=20, batches=3):
+    """Average duration of one tiled spread call (afsi_spread_tiled_f64: the
+    zero fill of the three face arrays, the work-list scan and
+    spread_tiled_kernel) on the stepper's binned points: HIP events on the
+    launch stream around `repeats` back-to-back calls, the mean over
+    `batches` brackets."""
+    bins = stepper.binner(x)
+    out = stepper.grid.zeros(device=x.device, dtype=torch.float64)
+    stepper.transfer.spread(x, F, bins, out=out)
+    stream = torch.cuda.current_stream()
+    start = [torch.cuda.Event(enable_timing=True) for _ in range(batches)]
+    end = [torch.cuda.Event(enable_timing=True) for _ in range(batches)]
+    torch.cuda.synchronize()
+    for i in range(batches):
+        start[i].record(stream)
+        for _ in range(repeats):
+            stepper.transfer.spread(x, F, bins, out=out)
         end[i].record(stream)
     torch.cuda.synchronize()
     return float(np.mean([a.elapsed_time(b) for a, b in zip(start, end)])) / 1e3 / repeats
@@ -665,6 +688,9 @@ def main():
     sweep_s = time_sweep(torch, inner)
     sweep_bytes = 3 * 4 * N
     achieved = sweep_bytes / sweep_s
+    # the mixed step's largest kernel by time (one launch per step): the spread
+    spread_s = time_spread(torch, steppers['mixed'], x0, F)
+    spread_bytes = 48 * P + 24 * N
     mixed, plain = results['mixed'], results['fp64']
     result = dict(
         metric=METRIC, value=mixed['steps_per_s'], unit='steps/s', n_gpus=1, steps=args.steps,
@@ -675,10 +701,16 @@ def main():
                              'red-black GS V(2,2) to rtol 1e-10; mixed = FP32 V-cycles with '
                              f'red-black over-relaxation {MIXED_RELAX} in FP64 refinement',
                     grid=n, points=P, precision='mixed', relax=MIXED_RELAX),
-        roofline=dict(bound='hbm', achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
-                      frac=achieved / HBM_PEAK, traffic=measured_traffic(n),
-                      kernel='rb_rows_kernel<float,true,4,4> (level-0 sweep)',
-                      algorithmic_bytes=sweep_bytes, avg_us=sweep_s * 1e6),
+        roofline=dict(bound='hbm', achieved=spread_bytes / spread_s / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
+                      frac=spread_bytes / spread_s / HBM_PEAK, traffic=measured_traffic(n, SPREAD_TRAFFIC_FILE),
+                      kernel='afsi_spread_tiled_f64 (zero fill, work-list scan, spread_tiled_kernel<8>)',
+                      algorithmic_bytes=spread_bytes, avg_us=spread_s * 1e6,
+                      note='the largest kernel of the mixed step by time (one launch per step); '
+                           'algorithmic bytes 48 per point + 24 per cell'),
+        roofline_sweep=dict(bound='hbm', achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
+                            frac=achieved / HBM_PEAK, traffic=measured_traffic(n),
+                            kernel='rb_rows_kernel<float,true,4,4> (level-0 sweep, 24 launches per mixed step)',
+                            algorithmic_bytes=sweep_bytes, avg_us=sweep_s * 1e6),
         plain_fp64=dict(steps_per_s=plain['steps_per_s'], ms_per_step=plain['ms_per_step'],
                         phases_ms=plain['phases_ms'], mg_cycles_mean=plain['mg_cycles_mean'],
                         spread_mpts=P / plain['phases_ms']['spread'] / 1e3,
