@@ -738,8 +738,11 @@ static int spread_tiled(const double* x, const double* force, int64_t points, co
     chunk_scan_kernel<<<1, 1024, 0, s>>>(cell_ptr, ts.ntiles, ts.tvol, SPREAD_CHUNK, chunk_start);
     AFSI_LAUNCH_CHECK();
     const double volume = (g.h[0] * g.h[1]) * g.h[2];
+    // one workgroup per work item up to 16384 (measured at 128^3 with 1.5M
+    // clustered points, about 7,000 items: 252 us against 270 us when 4096
+    // workgroups stride over the items)
     const int64_t upper = ts.ntiles + points / SPREAD_CHUNK + 1;
-    const unsigned blocks = unsigned(upper < 4096 ? upper : 4096);
+    const unsigned blocks = unsigned(upper < 16384 ? upper : 16384);
     const size_t shmem = size_t(ts.lds) * sizeof(double);
 #define AFSI_SPREAD(E)                                                                          \
     spread_tiled_kernel<E><<<blocks, SPREAD_THREADS, shmem, s>>>(x, force, perm, cell_ptr, chunk_start, \

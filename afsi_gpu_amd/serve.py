@@ -276,31 +276,39 @@ class Server:
         remaining = {int(i): int(c) for i, c in zip(*np.unique(inst_ids, return_counts=True))}
         lock = threading.Lock()
         eligible = deque()      # (arrival, instance id), instance resident
-        inflight = deque()      # (done event, [(arrival, iid, launch time)], ratios)
+        inflight = []           # (done event, [(arrival, iid, launch time)], ratios)
         shared = dict(used=0, arrived=0, launched=0, error=None)
         total = len(times)
         t0 = clock()
-        main_stream = torch.cuda.current_stream(self.device)
+        # Channel and 3D requests run on their own streams from their own
+        # executor threads, so a channel step never queues behind a 128^3
+        # solve; both streams have the default priority.
+        streams = dict(two=torch.cuda.current_stream(self.device), three=torch.cuda.Stream(device=self.device))
+        totals = dict(two=sum(1 for i in inst_ids if len(specs[int(i)][0]) == 2))
+        totals['three'] = total - totals['two']
 
         def fail(exc):
             with lock:
                 if shared['error'] is None:
                     shared['error'] = exc
 
-        def executor():
+        def executor(kind):
+            launched = 0
+            stream = streams[kind]
             try:
-                with torch.cuda.stream(main_stream):
+                with torch.cuda.stream(stream):
                     while True:
                         with lock:
                             if shared['error'] is not None:
                                 return
-                            if shared['launched'] == total:
+                            if launched == totals[kind]:
                                 return
                             batch, reqs, busy, keep_q = [], [], set(), deque()
                             while eligible:
                                 arr, iid = eligible.popleft()
                                 inst = self.resident[iid]
-                                ok = iid not in busy and len(batch) < self.max_batch and (
+                                mine = isinstance(inst, Instance2D) == (kind == 'two')
+                                ok = mine and iid not in busy and len(batch) < self.max_batch and (
                                     inst.ready is None or inst.ready.query())
                                 if ok:
                                     busy.add(iid)
@@ -317,11 +325,12 @@ class Server:
                         host = torch.empty(ratios.shape, dtype=ratios.dtype, pin_memory=True)
                         host.copy_(ratios, non_blocking=True)
                         done = torch.cuda.Event()
-                        done.record(main_stream)
+                        done.record(stream)
                         stats.launch_seconds.append(clock() - t0 - start)
-                        stats.iterations += 1
                         stats.batch_sizes.append(len(batch))
+                        launched += len(batch)
                         with lock:
+                            stats.iterations += 1
                             inflight.append((done, [(a, i, start) for a, i in reqs], host))
                             shared['launched'] += len(batch)
             except BaseException as exc:  # surfaced by run()
@@ -334,15 +343,16 @@ class Server:
                     with lock:
                         if shared['error'] is not None:
                             return
-                        head = inflight[0] if inflight else None
-                    if head is None or not head[0].query():
+                        pending = list(inflight)
+                    head = next((entry for entry in pending if entry[0].query()), None)
+                    if head is None:
                         time.sleep(0.0002)
                         continue
                     now = clock() - t0
                     done, reqs, host = head
                     values = host.tolist()
                     with lock:
-                        inflight.popleft()
+                        inflight.remove(head)
                         for (arr, iid, start), ratio in zip(reqs, values):
                             stats.latencies.append(now - arr)
                             stats.waits.append(start - arr)
@@ -356,7 +366,9 @@ class Server:
             except BaseException as exc:
                 fail(exc)
 
-        threads = [threading.Thread(target=executor, daemon=True), threading.Thread(target=completer, daemon=True)]
+        threads = [threading.Thread(target=executor, args=(kind,), daemon=True) for kind in ('two', 'three')
+                   if totals[kind]]
+        threads.append(threading.Thread(target=completer, daemon=True))
         for th in threads:
             th.start()
         waiting = deque()       # arrived, instance not yet admitted (budget)
