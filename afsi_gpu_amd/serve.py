@@ -35,9 +35,8 @@ import torch
 from . import _abi, workloads
 from .batch import BatchedMixedMultigrid, MixedState
 from .mac import MACGrid, MGOptions
-from .mac.multigrid import MIXED_RELAX
 from .mac.flow import correct, divergence
-from .mac.multigrid import GeometricMultigrid
+from .mac.multigrid import MIXED_RELAX, GeometricMultigrid
 from .mac.transfer import DEFAULT_TILE, Binner, TiledTransfer
 from .mac2d import ChannelGrid, ChannelIBStep
 from .mac2d.flow import divergence as divergence2d
@@ -241,6 +240,28 @@ class Server:
             self.resident_bytes[key] = footprint(saved['shape'], len(saved['x']))
         return sorted(instances)
 
+    def _prepare_shapes(self, shapes):
+        """Per-shape state shared by the instances of a shape, built before
+        serving starts (like loading a model): the channel stepper and, in
+        FP64, the 3D pressure solver with their V-cycle blocks captured as
+        HIP graphs once per shape (capturing while other threads launch work
+        would not be safe), and the 3D tiled transfer."""
+        for shape in shapes:
+            shared = self.shared.setdefault(shape, {})
+            if len(shape) == 2:
+                if 'channel' not in shared:
+                    grid = ChannelGrid(shape, (4., 1.), periodic=True)
+                    shared['channel'] = ChannelIBStep(grid, dt=self.dt, device=self.device, options=self.options,
+                                                      pressure_backend='graph')
+            else:
+                grid = MACGrid(shape, (1., 1., 1.), periodic=True)
+                if self.precision == 'fp64' and 'plain' not in shared:
+                    shared['plain'] = GeometricMultigrid(grid, device=self.device, options=self.options,
+                                                         backend='graph')
+                if 'transfer' not in shared:
+                    shared['transfer'] = TiledTransfer(grid, self.device, DEFAULT_TILE)
+        torch.cuda.synchronize(self.device)
+
     def _execute(self, batch):
         """Launch one iteration on the current stream; returns the (n,)
         device tensor of the requests' divergence ratios (no host wait
@@ -271,6 +292,7 @@ class Server:
         wanted = set(int(i) for i in inst_ids)
         data = {iid: workloads.instance_points(workloads.rng(sp[2]), sp[0], sp[1])
                 for iid, sp in specs.items() if iid in wanted}
+        self._prepare_shapes(sorted({tuple(sp[0]) for iid, sp in specs.items() if iid in wanted}))
         order = np.argsort(times, kind='stable')
         times, inst_ids = np.asarray(times)[order], np.asarray(inst_ids)[order]
         remaining = {int(i): int(c) for i, c in zip(*np.unique(inst_ids, return_counts=True))}
