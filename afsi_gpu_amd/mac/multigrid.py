@@ -27,6 +27,11 @@ OMEGA = 2 / 3  # mac/multigrid.py:109
 # own): the V(2,2) cycle's residual reduction falls from about 0.09 to 0.018
 # per cycle, 6 cycles instead of 10 to rtol 1e-10 at 128^3.
 MIXED_RELAX = 1.3
+# FP32 V-cycles per FP64 refinement step of the mixed path: with the relaxed
+# cycle's 0.018 reduction, three cycles cut the error by ~6e-6, well above
+# FP32 rounding, so a 6-cycle solve needs 2 FP64 residual passes instead of 6
+# (128^3 with 1.5M points: 603 -> 634 steps/s).
+MIXED_INNER_CYCLES = 3
 
 
 @dataclass(frozen=True)
@@ -38,6 +43,7 @@ class MGOptions:
     check_every: int = 2
     smoother: str = 'jacobi'
     relax: float = 1.0  # build-own: red-black over-relaxation weight (1: Gauss-Seidel)
+    inner_cycles: int = 1  # build-own: FP32 V-cycles per FP64 refinement step (mixed precision)
 
     def __post_init__(self):
         if (any(not isfinite(x) or x < 0 for x in (self.rtol, self.atol)) or
@@ -48,6 +54,8 @@ class MGOptions:
             raise ValueError('smoother must be jacobi or rbgs')
         if not (isfinite(self.relax) and 0 < self.relax < 2):
             raise ValueError('relax must lie in (0, 2)')
+        if type(self.inner_cycles) is not int or self.inner_cycles < 1:
+            raise ValueError('inner_cycles must be a positive integer')
 
 
 def hierarchy(shape, spacing):
@@ -293,8 +301,10 @@ class MixedMultigrid:
     """Mixed-precision pressure solve (build-own): FP32 V-cycles inside FP64
     iterative refinement.
 
-    Each outer iteration runs one fused FP32 V-cycle on the FP32 copy of the
-    FP64 residual (zero initial guess), then one fused pass adds the FP32
+    Each outer iteration runs ``options.inner_cycles`` fused FP32 V-cycles on
+    the FP32 copy of the FP64 residual (zero initial guess; the error of the
+    correction falls by the V-cycle factor per cycle, far above FP32
+    rounding for one or two cycles), then one fused pass adds the FP32
     correction to the FP64 iterate (ping-pong buffers) and recomputes
     r = b - A x in FP64; the residual norm is reduced only where the solve
     checks convergence. The residual, the accumulation of x and the removal
@@ -344,8 +354,8 @@ class MixedMultigrid:
                   _abi.stream())
 
     def _iteration(self, start, reduce):
-        """FP32 V-cycle, then x_{1-start} = x_start + e and the new residual."""
-        self.inner._cycles(1)
+        """FP32 V-cycles, then x_{1-start} = x_start + e and the new residual."""
+        self.inner._cycles(self.options.inner_cycles)
         _abi.call('afsi_refine_step_3d', _abi.ptr(self.xs[start]), _abi.ptr(self.inner.p[0]),
                   _abi.ptr(self.b), _abi.ptr(self.xs[1 - start]), _abi.ptr(self.inner.rhs[0]),
                   self._n, self._w, int(self.periodic), _abi.ptr(self.partials),
@@ -411,8 +421,9 @@ class MixedMultigrid:
     def solve(self, rhs, initial=None, *, rtol=None, history=None, precheck=None, view=False):
         """Same contract as ``GeometricMultigrid.solve`` (``precheck`` and
         ``view`` included; ``b`` and either FP64 iterate count as the
-        solver's own buffers); ``info['cycles']`` counts outer iterations
-        (one FP32 V-cycle each)."""
+        solver's own buffers); ``info['cycles']`` counts FP32 V-cycles
+        (``options.inner_cycles`` per outer iteration; convergence is
+        checked after whole outer iterations)."""
         rtol = self.options.rtol if rtol is None else rtol
         if not isfinite(rtol) or rtol < 0:
             raise ValueError('invalid multigrid solve tolerance')
@@ -458,10 +469,12 @@ class MixedMultigrid:
         # (consecutive time steps need about as many), then the residual is
         # checked after every cycle: two host synchronisations in the steady
         # state instead of one per check_every cycles.
+        inner = self.options.inner_cycles
+        most = -(-self.options.max_cycles // inner)  # outer iterations
         first = self.last_cycles - 1 if self.last_cycles and history is None else 0
-        while residual > tolerance and cycle < self.options.max_cycles:
+        while residual > tolerance and cycle < most:
             step = first if cycle == 0 and first > 0 else (1 if first > 0 else self.options.check_every)
-            n = min(step, self.options.max_cycles - cycle)
+            n = min(step, most - cycle)
             self.advance(n)
             cycle += n
             _, q0, _, nf0 = self.results[0].tolist()
@@ -475,5 +488,5 @@ class MixedMultigrid:
         self.last_cycles = cycle
         self._reduce(self.x, 4)
         self._sub_mean(self.x, 4)
-        return (self.x if view else self.x.clone()), dict(cycles=cycle, residual_norm=residual,
+        return (self.x if view else self.x.clone()), dict(cycles=cycle * inner, residual_norm=residual,
                                                           tolerance=tolerance, backend=self.backend)

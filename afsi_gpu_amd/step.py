@@ -17,7 +17,7 @@ import torch
 
 from . import _abi
 from .mac.flow import MACFlow, correct, divergence, project_correct, project_rhs
-from .mac.multigrid import MIXED_RELAX, MGOptions
+from .mac.multigrid import MIXED_INNER_CYCLES, MIXED_RELAX, MGOptions
 from .mac.transfer import DEFAULT_TILE, Binner, TiledTransfer, raise_flags
 
 
@@ -32,8 +32,9 @@ class IBStep:
         self.resident = bool(resident_pressure)
         self.device = torch.device(device)
         self.count = int(points)
-        self.options = options or MGOptions(smooth=2, smoother='rbgs',
-                                            relax=MIXED_RELAX if precision == 'mixed' else 1.)
+        self.options = options or (MGOptions(smooth=2, smoother='rbgs', relax=MIXED_RELAX,
+                                             inner_cycles=MIXED_INNER_CYCLES) if precision == 'mixed'
+                                   else MGOptions(smooth=2, smoother='rbgs'))
         self.precision = precision
         self.flow = MACFlow(grid, dt=dt, rho=rho, device=device, options=self.options,
                             pressure_backend=pressure_backend, precision=precision, predictor=False)
@@ -113,8 +114,9 @@ class FEIBStep:
         self.grid, self.dt, self.rho = grid, float(dt), float(rho)
         self.device = torch.device(device)
         self.max_displacement = float(max_displacement)
-        self.options = options or MGOptions(smooth=2, smoother='rbgs',
-                                            relax=MIXED_RELAX if precision == 'mixed' else 1.)
+        self.options = options or (MGOptions(smooth=2, smoother='rbgs', relax=MIXED_RELAX,
+                                             inner_cycles=MIXED_INNER_CYCLES) if precision == 'mixed'
+                                   else MGOptions(smooth=2, smoother='rbgs'))
         self.predictor = predictor
         self.flow = MACFlow(grid, dt=dt, rho=rho, mu=mu, device=device, options=self.options,
                             pressure_backend=pressure_backend, precision=precision,

@@ -658,7 +658,7 @@ def main():
         return
     from afsi_gpu_amd import workloads
     from afsi_gpu_amd.mac import MACGrid, MGOptions
-    from afsi_gpu_amd.mac.multigrid import MIXED_RELAX
+    from afsi_gpu_amd.mac.multigrid import MIXED_INNER_CYCLES, MIXED_RELAX
     from afsi_gpu_amd.step import IBStep
 
     device = torch.device('cuda:0')
@@ -667,7 +667,7 @@ def main():
     P, N = len(x_np), n ** 3
     grid = MACGrid((n, n, n), (1., 1., 1.), periodic=True)
     opts = MGOptions(smooth=2, smoother='rbgs')
-    relaxed = MGOptions(smooth=2, smoother='rbgs', relax=MIXED_RELAX)
+    relaxed = MGOptions(smooth=2, smoother='rbgs', relax=MIXED_RELAX, inner_cycles=MIXED_INNER_CYCLES)
     x0 = torch.tensor(x_np, device=device)
     F = torch.tensor(F_np, device=device)
     results = {}
@@ -699,8 +699,9 @@ def main():
         config=dict(workload=f'{n}^3 periodic unit cube, {P} points (two ellipsoidal '
                              'shells + 10% in a 0.05 patch), lognormal forces; IB step with '
                              'red-black GS V(2,2) to rtol 1e-10; mixed = FP32 V-cycles with '
-                             f'red-black over-relaxation {MIXED_RELAX} in FP64 refinement',
-                    grid=n, points=P, precision='mixed', relax=MIXED_RELAX),
+                             f'red-black over-relaxation {MIXED_RELAX}, {MIXED_INNER_CYCLES} per FP64 '
+                             'refinement step',
+                    grid=n, points=P, precision='mixed', relax=MIXED_RELAX, inner_cycles=MIXED_INNER_CYCLES),
         roofline=dict(bound='hbm', achieved=spread_bytes / spread_s / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
                       frac=spread_bytes / spread_s / HBM_PEAK, traffic=measured_traffic(n, SPREAD_TRAFFIC_FILE),
                       kernel='afsi_spread_tiled_f64 (zero fill, work-list scan, spread_tiled_kernel<8>)',
