@@ -55,7 +55,7 @@ class MGBItem(Structure):
 
 
 class MGBBatch(Structure):
-    _fields_ = [('count', c_int32), ('reserved', c_int32), ('items', POINTER(MGBItem)),
+    _fields_ = [('count', c_int32), ('inner_cycles', c_int32), ('items', POINTER(MGBItem)),
                 ('workspace', c_void_p), ('workspace_bytes', c_size_t), ('active', c_void_p),
                 ('tol', c_void_p), ('results', c_void_p), ('partials', c_void_p)]
 

@@ -93,6 +93,7 @@ class BatchedMixedMultigrid:
     def _batch(self, workspace, nbytes, partials):
         bt = _abi.MGBBatch()
         bt.count = len(self.states)
+        bt.inner_cycles = self.options.inner_cycles
         bt.items = _abi.cast_items(self.items)
         bt.workspace = workspace.data_ptr() if workspace is not None else None
         bt.workspace_bytes = nbytes
@@ -183,15 +184,17 @@ class BatchedMixedMultigrid:
         cycles = [0] * n
         active = self.active.tolist()
         cycle = 0
-        while any(active) and cycle < self.options.max_cycles:
-            step = min(self.options.check_every, self.options.max_cycles - cycle)
+        inner = self.options.inner_cycles   # FP32 V-cycles per refinement iteration
+        most = -(-self.options.max_cycles // inner)
+        while any(active) and cycle < most:
+            step = min(self.options.check_every, most - cycle)
             self.advance(step)
             cycle += step
             now = self.active.tolist()
             res = self.results.tolist()
             for i in range(n):
                 if active[i]:
-                    cycles[i] = cycle
+                    cycles[i] = cycle * inner
                     residual[i] = res[i][1] ** .5
                     if res[i][3] or not isfinite(residual[i]):
                         raise RuntimeError(f'nonfinite multigrid residual (instance {i})')

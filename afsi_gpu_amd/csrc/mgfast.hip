@@ -2164,7 +2164,7 @@ struct BatchLayout {
     size_t sweep_lds[AFSI_MG_MAX_LEVELS];
     size_t restrict_off[AFSI_MG_MAX_LEVELS], prolong_off[AFSI_MG_MAX_LEVELS];
     int restrict_blocks[AFSI_MG_MAX_LEVELS], prolong_blocks[AFSI_MG_MAX_LEVELS];
-    size_t coarse_off = 0, coarse_lds = 0;
+    size_t coarse_off[2] = {0, 0}, coarse_lds = 0;  // first cycle of an iteration (zero start); later ones
     size_t ref_off[4] = {0, 0, 0, 0};  // residual of x[0]; step from x[0]; step from x[1]; residual of x[1]
     int ref_blocks = 0;
     size_t vec_off[3] = {0, 0, 0};     // b, x[0], x[1]
@@ -2225,8 +2225,10 @@ static int batch_layout(const afsi_mgb_batch* bt, BatchLayout& L) {
         L.prolong_off[l] = off;
         off = align64(off + m * sizeof(RowDesc<float>));
     }
-    L.coarse_off = off;
-    off = align64(off + size_t(L.n) * sizeof(CoarseArgs<float>));
+    for (int c = 0; c < 2; ++c) {
+        L.coarse_off[c] = off;
+        off = align64(off + size_t(L.n) * sizeof(CoarseArgs<float>));
+    }
     L.coarse_lds = 0;
     L.ref_blocks = 0;
     for (int i = 0; i < L.n; ++i) {
@@ -2322,7 +2324,12 @@ static int batch_prepare(const afsi_mgb_batch* bt, hipStream_t s) {
         a.coarse_inverse = static_cast<const float*>(pl.coarse_inverse);
         a.coarse_cells = int32_t(pl.coarse_cells);
         a.active = bt->active + i;
-        at<CoarseArgs<float>>(hw, L.coarse_off)[i] = a;
+        at<CoarseArgs<float>>(hw, L.coarse_off[0])[i] = a;
+        // a later cycle of one refinement iteration continues from the
+        // correction so far when the coarse kernel holds level 0 (it loads
+        // p_in before it writes p_out, so the two may alias)
+        if (first == 0) a.p_in = a.p_out;
+        at<CoarseArgs<float>>(hw, L.coarse_off[1])[i] = a;
         const Dim32 d0{int(pl.n[0][0]), int(pl.n[0][1]), int(pl.n[0][2])};
         W3<double> k0;
         for (int q = 0; q < 3; ++q) k0.w[q] = pl.inv_h2[0][q];
@@ -2346,14 +2353,17 @@ static int batch_prepare(const afsi_mgb_batch* bt, hipStream_t s) {
 }
 
 template <bool P>
-static int batch_vcycle(const afsi_mgb_batch* bt, const BatchLayout& L, hipStream_t s) {
+static int batch_vcycle(const afsi_mgb_batch* bt, const BatchLayout& L, hipStream_t s, bool first = true) {
     void* ws = bt->workspace;
     const void* sweep_fn = reinterpret_cast<const void*>(rb_rows_batch_kernel<float, P>);
     for (int l = 0; l < L.levels; ++l) {
         const int m = int(L.members[l].size());
         if (int r = dynamic_lds(sweep_fn, L.sweep_lds[l])) return r;
         for (int it = 0; it < L.nu; ++it) {
-            const int v = it == 0 ? 0 : (it % 2 ? 2 : 1);
+            // the first sweep of a cycle starts from zero, except on level 0
+            // of a later cycle of one refinement iteration (which continues
+            // from the correction so far)
+            const int v = it == 0 ? (l == 0 && !first ? 1 : 0) : (it % 2 ? 2 : 1);
             rb_rows_batch_kernel<float, P><<<unsigned(L.sweep_blocks[l]), 256, L.sweep_lds[l], s>>>(
                 at<SweepDesc<float>>(ws, L.sweep_off[l][v]), m);
         }
@@ -2364,7 +2374,7 @@ static int batch_vcycle(const afsi_mgb_batch* bt, const BatchLayout& L, hipStrea
     const void* coarse_fn = reinterpret_cast<const void*>(coarse_batch_kernel<float, P>);
     if (int r = dynamic_lds(coarse_fn, L.coarse_lds)) return r;
     coarse_batch_kernel<float, P><<<unsigned(L.n), COARSE_THREADS, L.coarse_lds, s>>>(
-        at<CoarseArgs<float>>(ws, L.coarse_off));
+        at<CoarseArgs<float>>(ws, L.coarse_off[first ? 0 : 1]));
     AFSI_LAUNCH_CHECK();
     for (int l = L.levels - 1; l >= 0; --l) {
         const int m = int(L.members[l].size());
@@ -2516,14 +2526,17 @@ int afsi_mgb_iterate(const afsi_mgb_batch* batch, int32_t start, int32_t count, 
     if (int r = batch_check_ws(batch, L)) return r;
     AFSI_REQUIRE((start == 0 || start == 1) && count >= 1, "invalid batch iteration range");
     const hipStream_t s = hipStream_t(stream);
+    const int inner = batch->inner_cycles > 0 ? batch->inner_cycles : 1;
     for (int it = 0; it < count; ++it) {
         const int from = (start + it) % 2;
         const bool last = it == count - 1;
+        for (int c = 0; c < inner; ++c) {
+            const int r = L.periodic ? batch_vcycle<true>(batch, L, s, c == 0) : batch_vcycle<false>(batch, L, s, c == 0);
+            if (r) return r;
+        }
         if (L.periodic) {
-            if (int r = batch_vcycle<true>(batch, L, s)) return r;
             if (int r = batch_refine<true>(batch, L, 1 + from, last, s)) return r;
         } else {
-            if (int r = batch_vcycle<false>(batch, L, s)) return r;
             if (int r = batch_refine<false>(batch, L, 1 + from, last, s)) return r;
         }
     }

@@ -458,7 +458,7 @@ def batched_sample(torch, instances, steps=3, warmup=1):
     from afsi_gpu_amd import workloads
     from afsi_gpu_amd.batch import BatchedIBStep
     from afsi_gpu_amd.mac import MACGrid, MGOptions
-    from afsi_gpu_amd.mac.multigrid import MIXED_RELAX
+    from afsi_gpu_amd.mac.multigrid import MIXED_INNER_CYCLES, MIXED_RELAX
     from afsi_gpu_amd.mac2d import ChannelGrid, ChannelIBStep
     from afsi_gpu_amd.mac2d.batch import BatchedChannelStep
     from afsi_gpu_amd.step import IBStep
@@ -492,7 +492,7 @@ def batched_sample(torch, instances, steps=3, warmup=1):
             torch.cuda.synchronize()
             return (time.perf_counter() - t0) / steps
 
-    relaxed = MGOptions(smooth=2, smoother='rbgs', relax=MIXED_RELAX)
+    relaxed = MGOptions(smooth=2, smoother='rbgs', relax=MIXED_RELAX, inner_cycles=MIXED_INNER_CYCLES)
     batch3 = BatchedIBStep(g3, [specs[i][1] for i in i3], dt=1e-3, device=dev, options=relaxed)
     batch2 = BatchedChannelStep(g2, [specs[i][1] for i in i2], dt=1e-3, device=dev, options=opts) if i2 else None
 
@@ -532,7 +532,8 @@ def batched_sample(torch, instances, steps=3, warmup=1):
                                       mpts_per_s=points / t_seq / 1e6, ms_per_step=t_seq * 1e3),
                 speedup=t_seq / t_batch,
                 note='one step = every instance advanced once; batched = 3D instances in mixed precision '
-                     f'(FP32 V-cycles with red-black over-relaxation {MIXED_RELAX}) and 2D channels in FP64, '
+                     f'(FP32 V-cycles with red-black over-relaxation {MIXED_RELAX}, {MIXED_INNER_CYCLES} per FP64 '
+                     'refinement step) and 2D channels in FP64, '
                      'each group with one launch per phase over all its instances; sequential = the same '
                      'instances stepped one at a time on the plain FP64 paths')
 

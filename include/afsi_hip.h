@@ -402,7 +402,7 @@ typedef struct afsi_mgb_item {
 } afsi_mgb_item;
 typedef struct afsi_mgb_batch {
     int32_t count;
-    int32_t reserved;
+    int32_t inner_cycles;         /* FP32 V-cycles per refinement iteration (0: 1) */
     const afsi_mgb_item* items;   /* host array of count items */
     void* workspace;              /* device, afsi_mgb_workspace_bytes */
     size_t workspace_bytes;

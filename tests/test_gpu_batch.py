@@ -33,9 +33,13 @@ def rel(a, b):
     return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
 
 
+@pytest.mark.parametrize('relax,inner', [(1., 1), (1.3, 3)])
 @pytest.mark.parametrize('periodic', [True, False])
-def test_batched_solve_matches_single_instance(gpu, periodic):
-    opts = MGOptions(smooth=2, smoother='rbgs')
+def test_batched_solve_matches_single_instance(gpu, periodic, relax, inner):
+    """Also with the mixed path's settings: over-relaxation 1.3 and three
+    FP32 V-cycles per refinement iteration (later cycles of an iteration
+    continue from the correction so far on level 0)."""
+    opts = MGOptions(smooth=2, smoother='rbgs', relax=relax, inner_cycles=inner)
     grids = [MACGrid(s, (1., 1.5, 2.), periodic=periodic) for s in SHAPES]
     rhs = [rhs_for(g.shape, i, gpu) for i, g in enumerate(grids)]
     batch = BatchedMixedMultigrid(grids, device=gpu, options=opts, graphs=False)
