@@ -254,14 +254,29 @@ class GeometricMultigrid:
             w_p.zero_()
         elif initial.data_ptr() != w_p.data_ptr():
             w_p.copy_(initial)
-        # _start_metrics (mac/multigrid.py:68-73) with one host synchronisation.
+        self.launch_start()
+        cycle, residual, tolerance = self.settle(rtol, history, precheck)
+        self.launch_finish()
+        return (w_p if view else w_p.clone()), dict(cycles=cycle, residual_norm=residual, tolerance=tolerance,
+                                                    backend=self.backend)
+
+    # -- the parts of ``solve`` (IBStep captures the launch-only ones) ------
+    def launch_start(self):
+        """_start_metrics (mac/multigrid.py:68-73): mean removal of the RHS and
+        of the iterate in the level-0 buffers, the RHS norm and the initial
+        residual (device work only)."""
+        w_p, w_rhs = self.p[0], self.rhs[0]
         self._reduce(w_rhs, 1)
         self._reduce(w_p, 2)
         self._sub_mean(w_rhs, 1)
         self._sub_mean(w_p, 2)
         self._reduce(w_rhs, 3)
         self._residual()
-        count = w_rhs.numel()
+
+    def settle(self, rtol, history=None, precheck=None):
+        """One host synchronisation for the start metrics, their checks, then
+        V-cycles until the tolerance; returns (cycles, residual, tolerance)."""
+        count = self.rhs[0].numel()
         (s1, _, a1, nf1), (_, _, _, nf2), (_, q3, _, _), (_, q0, _, nf0) = \
             self.results[[1, 2, 3, 0]].tolist()
         if precheck is not None:
@@ -292,9 +307,11 @@ class GeometricMultigrid:
                 raise RuntimeError('nonfinite multigrid residual')
         if residual > tolerance:
             raise RuntimeError(f'pressure multigrid did not converge: {residual:g} > {tolerance:g}')
+        return cycle, residual, tolerance
+
+    def launch_finish(self):
+        """Remove the constant offset the fused driver leaves in p[0]."""
         self._materialize()
-        return (w_p if view else w_p.clone()), dict(cycles=cycle, residual_norm=residual, tolerance=tolerance,
-                                                    backend=self.backend)
 
 
 class MixedMultigrid:
@@ -440,26 +457,56 @@ class MixedMultigrid:
             self.x.zero_()
         elif not own:
             self.x.copy_(initial)
+        lead = self.lead(history)
+        self.launch_start(self.cur, lead)
+        cycle, residual, tolerance = self.settle(self.cur, lead, rtol, history, precheck)
+        self.launch_finish()
+        inner = self.options.inner_cycles
+        return (self.x if view else self.x.clone()), dict(cycles=cycle * inner, residual_norm=residual,
+                                                          tolerance=tolerance, backend=self.backend)
+
+    # -- the parts of ``solve`` (IBStep captures the launch-only ones) ------
+    # The first block runs one outer iteration less than the previous solve
+    # needed (consecutive time steps need about as many), then the residual is
+    # checked after every iteration. The block's first iteration ("lead") is
+    # launched before the readback of the start metrics (it writes only the
+    # other FP64 buffer, so it is dropped if the initial iterate already meets
+    # the tolerance or the checks raise): two host synchronisations per solve
+    # in the steady state.
+    def _outer_limit(self):
+        return -(-self.options.max_cycles // self.options.inner_cycles)
+
+    def lead(self, history=None):
+        """1 if the next solve launches its first outer iteration with the
+        start metrics, else 0."""
+        predicted = self.last_cycles - 1 if self.last_cycles and history is None else 0
+        return 1 if predicted > 0 and self._outer_limit() > 0 else 0
+
+    def launch_start(self, start, lead, captured=False):
+        """Mean removal and metrics of b and of the iterate xs[start], its
+        residual (results slot 5) and, with ``lead``, the first outer
+        iteration (device work only; ``captured``: issue the iteration's
+        launches directly, for a graph being captured)."""
+        self.cur = start
         self._reduce(self.b, 1)
         self._reduce(self.x, 2)
         self._sub_mean(self.b, 1)
         self._sub_mean(self.x, 2)
         self._reduce(self.b, 3)
         self._residual(5)
-        # The first block runs one outer iteration less than the previous
-        # solve needed (consecutive time steps need about as many), then the
-        # residual is checked after every iteration. The block's first
-        # iteration is launched before the readback of the start metrics (it
-        # writes only the other FP64 buffer, so it is dropped if the initial
-        # iterate already meets the tolerance or the checks raise): two host
-        # synchronisations per solve in the steady state.
-        inner = self.options.inner_cycles
-        most = -(-self.options.max_cycles // inner)  # outer iterations
-        predicted = self.last_cycles - 1 if self.last_cycles and history is None else 0
-        lead = 1 if predicted > 0 and most > 0 else 0
-        start = self.cur
         if lead:
-            self.advance(1)
+            if captured:
+                self._block(start, 1)
+            else:
+                self.advance(1)
+        self.cur = start
+
+    def settle(self, start, lead, rtol, history=None, precheck=None):
+        """Read the start metrics (one host synchronisation), check them, then
+        outer iterations until the tolerance; leaves ``cur`` on the result and
+        returns (outer iterations, residual, tolerance)."""
+        predicted = self.last_cycles - 1 if self.last_cycles and history is None else 0
+        most = self._outer_limit()
         count = self.b.numel()
         res = self.results.tolist()
         self.cur = start
@@ -505,7 +552,9 @@ class MixedMultigrid:
         if residual > tolerance:
             raise RuntimeError(f'pressure multigrid did not converge: {residual:g} > {tolerance:g}')
         self.last_cycles = cycle
+        return cycle, residual, tolerance
+
+    def launch_finish(self):
+        """Mean removal of the result xs[cur] (device work only)."""
         self._reduce(self.x, 4)
         self._sub_mean(self.x, 4)
-        return (self.x if view else self.x.clone()), dict(cycles=cycle * inner, residual_norm=residual,
-                                                          tolerance=tolerance, backend=self.backend)

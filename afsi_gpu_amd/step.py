@@ -18,18 +18,31 @@ import torch
 from . import _abi
 from .mac.flow import MACFlow, correct, divergence, project_correct, project_rhs
 from .mac.multigrid import MIXED_INNER_CYCLES, MIXED_RELAX, MGOptions
-from .mac.transfer import DEFAULT_TILE, Binner, TiledTransfer, raise_flags
+from .mac.transfer import DEFAULT_TILE, Binner, PointBins, TiledTransfer, raise_flags
 
 
 class IBStep:
     def __init__(self, grid, points, *, dt, rho=1., device='cuda', options=None,
-                 tile=DEFAULT_TILE, pressure_backend='graph', precision='fp64', resident_pressure=False):
+                 tile=DEFAULT_TILE, pressure_backend='graph', precision='fp64', resident_pressure=False,
+                 graphs=False):
         """``resident_pressure``: the projection's right-hand side is formed
         in the solver's own buffer and the returned pressure is the solver's
         iterate (valid until the next step; passed back as ``pressure`` it
-        warm-starts the next solve without a copy)."""
+        warm-starts the next solve without a copy).
+
+        ``graphs`` (needs ``resident_pressure``): the step runs as two HIP
+        graphs around the solver's host convergence checks, each captured
+        once per set of buffers and solver state and replayed afterwards:
+        binning, spread, right-hand side and the solver's start (metrics,
+        initial residual and the first outer iteration) in one; the solver's
+        finish, correction, interpolation and point advance in the other.
+        The V-cycle blocks in between are the solver's own graphs. Steps
+        given ``timers`` run as plain launches."""
         self.grid, self.dt, self.rho = grid, float(dt), float(rho)
         self.resident = bool(resident_pressure)
+        if graphs and not self.resident:
+            raise ValueError('graph steps need resident_pressure')
+        self.graphs = {} if graphs else None
         self.device = torch.device(device)
         self.count = int(points)
         self.options = options or (MGOptions(smooth=2, smoother='rbgs', relax=MIXED_RELAX,
@@ -55,6 +68,8 @@ class IBStep:
         ``advance`` the points ``x`` move by dt*U in place. ``timers`` maps
         phase names ('bin', 'spread', 'project', 'interp') to (start, end)
         events recorded on the current stream around them."""
+        if self.graphs is not None and not timers:
+            return self._graph_step(x, force, velocity, pressure, advance)
         mark = _Marks(timers)
         mark('bin', 0)
         bins = self.binner(x)
@@ -80,6 +95,90 @@ class IBStep:
         if advance:
             _abi.call('afsi_axpy_f64', _abi.ptr(x), _abi.ptr(U), x.numel(), self.dt, _abi.stream())
         return velocity, p, U, dict(pressure=info, bins=bins)
+
+
+    # -- graph steps ------------------------------------------------------------
+    def _capture(self, fn):
+        stream = torch.cuda.Stream(device=self.device)
+        stream.wait_stream(torch.cuda.current_stream())
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            fn()
+        torch.cuda.current_stream().wait_stream(stream)
+        return graph
+
+    def _launch_start(self, x, force, velocity, zero, start, lead):
+        """Binning, spread, right-hand side into the solver's buffer and the
+        solver's start (device work only)."""
+        solver = self.flow.pressure_solver
+        bins = self.binner(x)
+        f = self.transfer.spread(x, force, bins, out=self.force_density)
+        mixed = hasattr(solver, 'xs')
+        out = solver.b if mixed else solver.rhs[0]
+        project_rhs(velocity, f, self.dt / self.rho, self.grid, -self.rho / self.dt, out=out)
+        if mixed:
+            if zero:
+                solver.xs[start].zero_()
+            solver.launch_start(start, lead, captured=True)
+        else:
+            if zero:
+                solver.p[0].zero_()
+            solver.launch_start()
+
+    def _launch_finish(self, x, velocity, pressure, advance):
+        """The solver's finish, correction, interpolation and advance."""
+        a = self.dt / self.rho
+        self.flow.pressure_solver.launch_finish()
+        project_correct(velocity, self.force_density, a, pressure, self.grid, a)
+        U = self.transfer.interpolate(x, velocity, self._bins, out=self.point_velocity)
+        if advance:
+            _abi.call('afsi_axpy_f64', _abi.ptr(x), _abi.ptr(U), x.numel(), self.dt, _abi.stream())
+
+    def _graph_step(self, x, force, velocity, pressure, advance):
+        if x.shape != (self.count, 3) or x.dtype != torch.float64 or not x.is_contiguous():
+            raise ValueError('positions must match the step (N,3) contiguous float64')
+        if force.shape != x.shape or force.dtype != torch.float64 or not force.is_contiguous():
+            raise ValueError('forces must match the positions')
+        if len(velocity) != 3 or any(u.dtype != torch.float64 or not u.is_contiguous() or u.device != x.device or
+                                     tuple(u.shape) != self.grid.face_shape(c) for c, u in enumerate(velocity)):
+            raise ValueError('velocity must be the three contiguous float64 face arrays')
+        solver = self.flow.pressure_solver
+        mixed = hasattr(solver, 'xs')
+        buffers = (x.data_ptr(), force.data_ptr(), *(u.data_ptr() for u in velocity))
+        own = solver.xs if mixed else (solver.p[0],)
+        index = [k for k, t in enumerate(own) if pressure is not None and pressure.data_ptr() == t.data_ptr()]
+        if pressure is not None and not index:
+            if pressure.shape != own[0].shape or pressure.dtype != torch.float64:
+                raise ValueError('invalid initial pressure')
+            own[0].copy_(pressure)
+        start = index[0] if index else 0
+        lead = solver.lead() if mixed else 0
+        key = ('start', buffers, start, lead, pressure is None)
+        graph = self.graphs.get(key)
+        if graph is None:
+            graph = self.graphs[key] = self._capture(
+                lambda: self._launch_start(x, force, velocity, pressure is None, start, lead))
+        self._bins = PointBins(self.binner.tile, self.binner.keys, self.binner.perm, self.binner.cell_ptr,
+                               self.binner.flags)
+        graph.replay()
+        self.last_rhs = solver.b if mixed else solver.rhs[0]
+        self.flags_host.copy_(self.binner.flags, non_blocking=True)
+        rtol = solver.options.rtol
+        if mixed:
+            cycles, residual, tolerance = solver.settle(start, lead, rtol, precheck=self._check_flags)
+            cycles *= solver.options.inner_cycles
+            end = solver.cur
+        else:
+            cycles, residual, tolerance = solver.settle(rtol, precheck=self._check_flags)
+            end = 0
+        p = own[end]
+        key = ('finish', buffers, end, bool(advance))
+        graph = self.graphs.get(key)
+        if graph is None:
+            graph = self.graphs[key] = self._capture(lambda: self._launch_finish(x, velocity, p, advance))
+        graph.replay()
+        info = dict(cycles=cycles, residual_norm=residual, tolerance=tolerance, backend=solver.backend)
+        return velocity, p, self.point_velocity, dict(pressure=info, bins=self._bins)
 
 
 class _Marks:

@@ -156,8 +156,11 @@ class paused_gc:
         gc.enable()
 
 
-def run_path(torch, stepper, x0, F, steps, warmup):
-    """Warm up, then time `steps` steps; returns timings and the final state."""
+def run_path(torch, stepper, x0, F, steps, warmup, phase_steps=5):
+    """Warm up, then time `steps` steps; returns timings and the final state.
+    The timed steps run as the stepper runs them (HIP graphs when it has
+    them); the phase split comes from `phase_steps` further steps with HIP
+    events around each phase, which run as plain launches."""
     x = x0.clone()
     grid = stepper.grid
     u = grid.zeros(device=x.device)
@@ -166,16 +169,20 @@ def run_path(torch, stepper, x0, F, steps, warmup):
     for _ in range(warmup):
         u, p, U, info = stepper.step(x, F, u, p)
         cycles.append(info['pressure']['cycles'])
-    timers = [{ph: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-               for ph in PHASES} for _ in range(steps)]
     with paused_gc():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(steps):
-            u, p, U, info = stepper.step(x, F, u, p, timers=timers[k])
+            u, p, U, info = stepper.step(x, F, u, p)
             cycles.append(info['pressure']['cycles'])
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
+    timers = [{ph: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for ph in PHASES} for _ in range(phase_steps)]
+    with paused_gc():
+        for k in range(phase_steps):
+            u, p, U, info = stepper.step(x, F, u, p, timers=timers[k])
+        torch.cuda.synchronize()
     phase_ms = {ph: float(np.mean([t[ph][0].elapsed_time(t[ph][1]) for t in timers]))
                 for ph in PHASES}
     return dict(steps_per_s=steps / elapsed, ms_per_step=1e3 * elapsed / steps,
@@ -679,7 +686,7 @@ def main():
     for name, precision, options in (('fp64', 'fp64', opts), ('mixed', 'mixed', relaxed),
                                      ('fp64_relaxed', 'fp64', relaxed)):
         stepper = IBStep(grid, P, dt=dt, device=device, options=options, precision=precision,
-                         resident_pressure=True)
+                         resident_pressure=True, graphs=True)
         steppers[name] = stepper
         results[name] = run_path(torch, stepper, x0, F, args.steps, args.warmup)
     # accuracy of every path against the FP64 oracle (same inputs, one step from rest)
@@ -702,7 +709,10 @@ def main():
                              'red-black GS V(2,2) to rtol 1e-10; mixed = FP32 V-cycles with '
                              f'red-black over-relaxation {MIXED_RELAX}, {MIXED_INNER_CYCLES} per FP64 '
                              'refinement step',
-                    grid=n, points=P, precision='mixed', relax=MIXED_RELAX, inner_cycles=MIXED_INNER_CYCLES),
+                    grid=n, points=P, precision='mixed', relax=MIXED_RELAX, inner_cycles=MIXED_INNER_CYCLES,
+                    execution='every path: each step as two HIP graphs around the host convergence checks '
+                              '(V-cycle blocks as graphs of their own); phases_ms from 5 further steps '
+                              'as plain launches with HIP events per phase'),
         roofline=dict(bound='hbm', achieved=spread_bytes / spread_s / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s',
                       frac=spread_bytes / spread_s / HBM_PEAK, traffic=measured_traffic(n, SPREAD_TRAFFIC_FILE),
                       kernel='afsi_spread_tiled_f64 (zero fill, work-list scan, spread_tiled_kernel<8>)',
