@@ -299,12 +299,16 @@ __device__ inline void frame_shift(const PointWeights& pw, const afsi_grid3& g, 
 constexpr double FIX_MAGIC = 0x1.8p52;
 constexpr long long FIX_MAGIC_BITS = 0x4338000000000000LL;
 
+// Points per thread of one spread work item (its positions and forces are
+// held in registers from the bound pass on).
+constexpr int SPREAD_PPT = SPREAD_CHUNK / SPREAD_THREADS;
+static_assert(SPREAD_PPT * SPREAD_THREADS == SPREAD_CHUNK, "whole points per thread");
+
 template <int T, bool FIXED>
-__device__ __forceinline__ void spread_points(const WorkItem& it, const double* __restrict__ x,
-                                              const double* __restrict__ force,
-                                              const int32_t* __restrict__ perm, int base,
-                                              const afsi_grid3& g, const TileSpec& ts, double volume,
-                                              double scale, int* flags, double* lds) {
+__device__ __forceinline__ void spread_points(const WorkItem& it, double (&px)[SPREAD_PPT][3],
+                                              double (&pf)[SPREAD_PPT][3], const afsi_grid3& g,
+                                              const TileSpec& ts, double volume, double scale, int* flags,
+                                              double* lds) {
     unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds);
     // Consecutive lanes take consecutive binned points, so a wavefront's
     // points mostly share one or two cells and stencil bases. Each lane
@@ -314,16 +318,27 @@ __device__ __forceinline__ void spread_points(const WorkItem& it, const double* 
     // a bank pair. (Measured at 128^3 with 1.5M clustered points: 255 us
     // against 276 us for contiguous per-thread slices with 16 rotations.)
     const int ri = (threadIdx.x >> 4) & 3, rj = (threadIdx.x >> 2) & 3, rk = threadIdx.x & 3;
-    for (int j = it.pbeg + int(threadIdx.x); j < it.pend; j += SPREAD_THREADS) {
-        const int idx = perm[j] - base;
-        const double xp[3] = {x[3 * idx], x[3 * idx + 1], x[3 * idx + 2]};
+    // one copy of the loop body: point q is always in slot 0 (the slots
+    // shift down after each point; register arrays take static indices only)
+#pragma unroll 1
+    for (int q = 0; q < SPREAD_PPT; ++q) {
+        if (it.pbeg + int(threadIdx.x) + q * SPREAD_THREADS >= it.pend) break;
+        const double f0[3] = {pf[0][0], pf[0][1], pf[0][2]};
         PointWeights pw;
-        point_weights(xp, g, pw);
+        point_weights(px[0], g, pw);
+#pragma unroll
+        for (int r = 0; r + 1 < SPREAD_PPT; ++r) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                px[r][c] = px[r + 1][c];
+                pf[r][c] = pf[r + 1][c];
+            }
+        }
         int shift[3];
         frame_shift(pw, g, shift);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double fv = FIXED ? (force[3 * idx + c] / volume) * scale : force[3 * idx + c] / volume;
+            const double fv = FIXED ? (f0[c] / volume) * scale : f0[c] / volume;
             int lb[3];
             if (!box_base<T>(pw, shift, c, it.t0, ts, lb)) {
                 raise_flag(flags, F_INDEX);
@@ -337,14 +352,14 @@ __device__ __forceinline__ void spread_points(const WorkItem& it, const double* 
             double wxr[4], wyr[4], wzr[4];
             int ox[4], oy[4], oz[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int iq = (q + ri) & 3, jq = (q + rj) & 3, kq = (q + rk) & 3;
-                wxr[q] = iq == 0 ? wx[0] : (iq == 1 ? wx[1] : (iq == 2 ? wx[2] : wx[3]));
-                wyr[q] = jq == 0 ? wy[0] : (jq == 1 ? wy[1] : (jq == 2 ? wy[2] : wy[3]));
-                wzr[q] = kq == 0 ? wz[0] : (kq == 1 ? wz[1] : (kq == 2 ? wz[2] : wz[3]));
-                ox[q] = iq * d1 * d2;
-                oy[q] = jq * d2;
-                oz[q] = kq;
+            for (int r = 0; r < 4; ++r) {
+                const int iq = (r + ri) & 3, jq = (r + rj) & 3, kq = (r + rk) & 3;
+                wxr[r] = iq == 0 ? wx[0] : (iq == 1 ? wx[1] : (iq == 2 ? wx[2] : wx[3]));
+                wyr[r] = jq == 0 ? wy[0] : (jq == 1 ? wy[1] : (jq == 2 ? wy[2] : wy[3]));
+                wzr[r] = kq == 0 ? wz[0] : (kq == 1 ? wz[1] : (kq == 2 ? wz[2] : wz[3]));
+                ox[r] = iq * d1 * d2;
+                oy[r] = jq * d2;
+                oz[r] = kq;
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -407,13 +422,19 @@ __device__ __forceinline__ void spread_item(const WorkItem& it, const double* __
     __shared__ double wave_max[SPREAD_THREADS / 64];
     const int nlds = T > 0 ? 3 * (T + 3) * (T + 4) * (T + 4) : ts.lds;
     for (int e = threadIdx.x; e < nlds; e += SPREAD_THREADS) lds[e] = 0.0;
-    // largest |force component| of the item (+inf if any is nonfinite)
+    // the thread's points (gathered through the bin permutation once) and
+    // the item's largest |force component| (+inf if any is nonfinite)
+    double px[SPREAD_PPT][3], pf[SPREAD_PPT][3];
     double m = 0.0;
-    for (int j = it.pbeg + int(threadIdx.x); j < it.pend; j += SPREAD_THREADS) {
-        const int idx = perm[j] - base;
+#pragma unroll
+    for (int q = 0; q < SPREAD_PPT; ++q) {
+        const int j = it.pbeg + int(threadIdx.x) + q * SPREAD_THREADS;
+        const int idx = j < it.pend ? perm[j] - base : -1;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double a = fabs(force[3 * idx + c]);
+            px[q][c] = idx >= 0 ? x[3 * idx + c] : 0.0;
+            pf[q][c] = idx >= 0 ? force[3 * idx + c] : 0.0;
+            const double a = fabs(pf[q][c]);
             m = a <= m ? m : (a == a ? a : __builtin_inf());
         }
     }
@@ -430,11 +451,11 @@ __device__ __forceinline__ void spread_item(const WorkItem& it, const double* __
     if (fixed) {
         // 2^e > bound, so every scaled contribution is below 2^50
         const int e = bound == 0.0 ? 0 : ilogb(bound) + 1;
-        spread_points<T, true>(it, x, force, perm, base, g, ts, volume, ldexp(1.0, 50 - e), flags, lds);
+        spread_points<T, true>(it, px, pf, g, ts, volume, ldexp(1.0, 50 - e), flags, lds);
         __syncthreads();
         spread_flush<T, true>(it, g, ts, ldexp(1.0, e - 50), outs, flags, lds);
     } else {
-        spread_points<T, false>(it, x, force, perm, base, g, ts, volume, 1.0, flags, lds);
+        spread_points<T, false>(it, px, pf, g, ts, volume, 1.0, flags, lds);
         __syncthreads();
         spread_flush<T, false>(it, g, ts, 1.0, outs, flags, lds);
     }
