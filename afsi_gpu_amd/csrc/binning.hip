@@ -33,26 +33,27 @@ static int64_t key_space(const afsi_grid3& g, const int32_t t[3]) {
 }
 
 
-__device__ inline int64_t cell_of(double xa, double o, double h, int64_t n, bool periodic,
-                                  int* flags) {
+// Cell index of one coordinate. The key space is below 2^31 (checked by
+// every entry point), so cells and keys take 32-bit integer arithmetic; the
+// 64-bit remainder runs only for a periodic point outside the domain.
+__device__ inline uint32_t cell_of(double xa, double o, double h, int64_t n, bool periodic,
+                                   int* flags) {
     const double fl = floor((xa - o) / h - 1.0);
     if (!isfinite(fl)) {
         raise_flag(flags, F_NONFINITE);
         return 0;
     }
-    int64_t c = int64_t(fl) + 1;
-    if (periodic) return wrap(c, n);
-    if (c < 0 || c >= n) {
-        raise_flag(flags, F_SUPPORT);
-        c = c < 0 ? 0 : n - 1;  // keep every later access in bounds
-    }
-    return c;
+    const int64_t c = int64_t(fl) + 1;
+    if (c >= 0 && c < n) return uint32_t(c);
+    if (periodic) return uint32_t(wrap(c, n));
+    raise_flag(flags, F_SUPPORT);
+    return c < 0 ? 0u : uint32_t(n - 1);  // keep every later access in bounds
 }
 
 // Tile-major key of point p of one instance.
 __device__ inline uint32_t point_key(const double* __restrict__ x, int64_t p, const afsi_grid3& g,
                                      const TileGeom& tg, int* flags) {
-    int64_t c[3];
+    uint32_t c[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const double xa = x[3 * p + a];
@@ -62,9 +63,10 @@ __device__ inline uint32_t point_key(const double* __restrict__ x, int64_t p, co
         }
         c[a] = cell_of(xa, g.origin[a], g.h[a], g.n[a], g.periodic != 0, flags);
     }
-    const int64_t tid = ((c[0] / tg.t[0]) * tg.nt[1] + c[1] / tg.t[1]) * tg.nt[2] + c[2] / tg.t[2];
-    const int64_t local = ((c[0] % tg.t[0]) * tg.t[1] + c[1] % tg.t[1]) * tg.t[2] + c[2] % tg.t[2];
-    return uint32_t(tid * (int64_t(tg.t[0]) * tg.t[1] * tg.t[2]) + local);
+    const uint32_t t0 = uint32_t(tg.t[0]), t1 = uint32_t(tg.t[1]), t2 = uint32_t(tg.t[2]);
+    const uint32_t tid = ((c[0] / t0) * uint32_t(tg.nt[1]) + c[1] / t1) * uint32_t(tg.nt[2]) + c[2] / t2;
+    const uint32_t local = ((c[0] % t0) * t1 + c[1] % t1) * t2 + c[2] % t2;
+    return tid * (t0 * t1 * t2) + local;
 }
 
 __global__ void keys_kernel(const double* __restrict__ x, int64_t points, afsi_grid3 g,
