@@ -365,3 +365,63 @@ def test_ib_step_raises_on_wall_support_before_changing_state(gpu):
     # the same stepper runs once the point moves inside the support band
     xt[7] = torch.tensor((.5, .5, .5), device=gpu, dtype=torch.float64)
     stepper.step(xt, F, u)
+
+
+@pytest.mark.parametrize('periodic', [True, False])
+@pytest.mark.parametrize('scale', [1., 1e-265, 1e-280, 1e265, 1e275, 'wide'])
+def test_tiled_spread_force_magnitudes(gpu, periodic, scale):
+    """The spread's LDS box is 64-bit fixed point with a power-of-two scale
+    per work item from its largest force; items whose bound max|F|/V/8 lies
+    outside [2^-900, 2^900] use FP64 LDS atomics. Forces near both ends of
+    the fixed range (1e-265, 1e265), beyond them (1e-280, 1e275) and spread
+    over 16 decades within one item ('wide') all meet the 1e-12 bar against
+    the oracle (the fixed-point error is relative to the item's largest
+    contribution, so the bar holds in the l2 and max norms of the output)."""
+    g, og = grids(periodic)
+    x, F = points_in(og, 20000, 31)
+    x[:3000] = x[0] + 1e-3 * np.random.default_rng(32).normal(size=(3000, 3))  # dense cluster
+    if scale == 'wide':
+        F = F * 10. ** np.random.default_rng(33).uniform(-8, 8, size=(len(F), 1))
+    else:
+        F = F * scale
+    base, phi = ib3d.prepare_compact(x, og)
+    f_ref = ib3d.spread_grid(F, base, phi, og)
+    xt, Ft = torch.tensor(x, device=gpu), torch.tensor(F, device=gpu)
+    bins = bin_points(xt, g)
+    tr = TiledTransfer(g, gpu)
+    f = tr.spread(xt, Ft, bins)
+    assert int(tr.flags.item()) == 0
+    for c in range(3):
+        assert bool(torch.isfinite(f[c]).all())
+        # normalised first: squared norms of 1e266 overflow, of 1e-278 underflow
+        top = np.abs(f_ref[c]).max()
+        assert 0 < top < np.inf
+        got, want = f[c].cpu().numpy() / top, f_ref[c] / top
+        assert rel_err(got, want) < TOL
+        assert max_err(got, want) < 10 * TOL
+
+
+@pytest.mark.parametrize('periodic', [True, False])
+def test_tiled_spread_nonfinite_and_zero_forces(gpu, periodic):
+    """A NaN force component reaches exactly the faces the oracle's FP64 sum
+    makes NaN (its work item falls back to FP64 LDS atomics), the other
+    faces stay within the bar; all-zero forces give all-zero face arrays."""
+    g, og = grids(periodic)
+    x, F = points_in(og, 20000, 41)
+    x[:3000] = x[0] + 1e-3 * np.random.default_rng(42).normal(size=(3000, 3))
+    F[1500, 1] = np.nan
+    base, phi = ib3d.prepare_compact(x, og)
+    f_ref = ib3d.spread_grid(F, base, phi, og)
+    xt, Ft = torch.tensor(x, device=gpu), torch.tensor(F, device=gpu)
+    bins = bin_points(xt, g)
+    tr = TiledTransfer(g, gpu)
+    f = tr.spread(xt, Ft, bins)
+    for c in range(3):
+        got = f[c].cpu().numpy()
+        nan_ref = np.isnan(f_ref[c])
+        assert np.array_equal(np.isnan(got), nan_ref), c
+        assert nan_ref.any() == (c == 1)
+        assert rel_err(got[~nan_ref], f_ref[c][~nan_ref]) < TOL
+    zero = tr.spread(xt, torch.zeros_like(Ft), bins)
+    assert all(bool((fc == 0).all()) for fc in zero)
+    assert int(tr.flags.item()) == 0
