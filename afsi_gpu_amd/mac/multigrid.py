@@ -254,31 +254,37 @@ class GeometricMultigrid:
             w_p.zero_()
         elif initial.data_ptr() != w_p.data_ptr():
             w_p.copy_(initial)
-        self.launch_start()
-        cycle, residual, tolerance = self.settle(rtol, history, precheck)
+        self.launch_start(zero=initial is None)
+        cycle, residual, tolerance = self.settle(rtol, history, precheck, zero=initial is None)
         self.launch_finish()
         return (w_p if view else w_p.clone()), dict(cycles=cycle, residual_norm=residual, tolerance=tolerance,
                                                     backend=self.backend)
 
     # -- the parts of ``solve`` (IBStep captures the launch-only ones) ------
-    def launch_start(self):
+    def launch_start(self, zero=False):
         """_start_metrics (mac/multigrid.py:68-73): mean removal of the RHS and
         of the iterate in the level-0 buffers, the RHS norm and the initial
-        residual (device work only)."""
+        residual (device work only). ``zero``: the iterate is zero, so its
+        mean and finiteness need no pass."""
         w_p, w_rhs = self.p[0], self.rhs[0]
         self._reduce(w_rhs, 1)
-        self._reduce(w_p, 2)
+        if not zero:
+            self._reduce(w_p, 2)
         self._sub_mean(w_rhs, 1)
-        self._sub_mean(w_p, 2)
+        if not zero:
+            self._sub_mean(w_p, 2)
         self._reduce(w_rhs, 3)
         self._residual()
 
-    def settle(self, rtol, history=None, precheck=None):
+    def settle(self, rtol, history=None, precheck=None, zero=False):
         """One host synchronisation for the start metrics, their checks, then
-        V-cycles until the tolerance; returns (cycles, residual, tolerance)."""
+        V-cycles until the tolerance; returns (cycles, residual, tolerance).
+        ``zero`` as for ``launch_start``."""
         count = self.rhs[0].numel()
         (s1, _, a1, nf1), (_, _, _, nf2), (_, q3, _, _), (_, q0, _, nf0) = \
             self.results[[1, 2, 3, 0]].tolist()
+        if zero:
+            nf2 = 0
         if precheck is not None:
             precheck()
         if nf1:
@@ -458,8 +464,8 @@ class MixedMultigrid:
         elif not own:
             self.x.copy_(initial)
         lead = self.lead(history)
-        self.launch_start(self.cur, lead)
-        cycle, residual, tolerance = self.settle(self.cur, lead, rtol, history, precheck)
+        self.launch_start(self.cur, lead, zero=initial is None)
+        cycle, residual, tolerance = self.settle(self.cur, lead, rtol, history, precheck, zero=initial is None)
         self.launch_finish()
         inner = self.options.inner_cycles
         return (self.x if view else self.x.clone()), dict(cycles=cycle * inner, residual_norm=residual,
@@ -482,16 +488,19 @@ class MixedMultigrid:
         predicted = self.last_cycles - 1 if self.last_cycles and history is None else 0
         return 1 if predicted > 0 and self._outer_limit() > 0 else 0
 
-    def launch_start(self, start, lead, captured=False):
+    def launch_start(self, start, lead, captured=False, zero=False):
         """Mean removal and metrics of b and of the iterate xs[start], its
         residual (results slot 5) and, with ``lead``, the first outer
         iteration (device work only; ``captured``: issue the iteration's
-        launches directly, for a graph being captured)."""
+        launches directly, for a graph being captured; ``zero``: the iterate
+        is zero, so its mean and finiteness need no pass)."""
         self.cur = start
         self._reduce(self.b, 1)
-        self._reduce(self.x, 2)
+        if not zero:
+            self._reduce(self.x, 2)
         self._sub_mean(self.b, 1)
-        self._sub_mean(self.x, 2)
+        if not zero:
+            self._sub_mean(self.x, 2)
         self._reduce(self.b, 3)
         self._residual(5)
         if lead:
@@ -501,14 +510,17 @@ class MixedMultigrid:
                 self.advance(1)
         self.cur = start
 
-    def settle(self, start, lead, rtol, history=None, precheck=None):
+    def settle(self, start, lead, rtol, history=None, precheck=None, zero=False):
         """Read the start metrics (one host synchronisation), check them, then
         outer iterations until the tolerance; leaves ``cur`` on the result and
-        returns (outer iterations, residual, tolerance)."""
+        returns (outer iterations, residual, tolerance). ``zero`` as for
+        ``launch_start``."""
         predicted = self.last_cycles - 1 if self.last_cycles and history is None else 0
         most = self._outer_limit()
         count = self.b.numel()
         res = self.results.tolist()
+        if zero:
+            res[2][3] = 0
         self.cur = start
         if precheck is not None:
             precheck()

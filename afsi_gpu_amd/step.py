@@ -119,11 +119,11 @@ class IBStep:
         if mixed:
             if zero:
                 solver.xs[start].zero_()
-            solver.launch_start(start, lead, captured=True)
+            solver.launch_start(start, lead, captured=True, zero=zero)
         else:
             if zero:
                 solver.p[0].zero_()
-            solver.launch_start()
+            solver.launch_start(zero=zero)
 
     def _launch_finish(self, x, velocity, pressure, advance):
         """The solver's finish, correction, interpolation and advance."""
@@ -164,12 +164,13 @@ class IBStep:
         self.last_rhs = solver.b if mixed else solver.rhs[0]
         self.flags_host.copy_(self.binner.flags, non_blocking=True)
         rtol = solver.options.rtol
+        zero = pressure is None
         if mixed:
-            cycles, residual, tolerance = solver.settle(start, lead, rtol, precheck=self._check_flags)
+            cycles, residual, tolerance = solver.settle(start, lead, rtol, precheck=self._check_flags, zero=zero)
             cycles *= solver.options.inner_cycles
             end = solver.cur
         else:
-            cycles, residual, tolerance = solver.settle(rtol, precheck=self._check_flags)
+            cycles, residual, tolerance = solver.settle(rtol, precheck=self._check_flags, zero=zero)
             end = 0
         p = own[end]
         key = ('finish', buffers, end, bool(advance))
