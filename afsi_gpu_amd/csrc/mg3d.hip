@@ -231,7 +231,9 @@ __device__ __forceinline__ double divergence_at(const double* __restrict__ u, co
                                                 const double* __restrict__ fv, const double* __restrict__ fw,
                                                 double a, int64_t i) {
     const int64_t e = P ? 0 : 1;
-    const int64_t z = i % d.n2, y = (i / d.n2) % d.n1, x = i / (d.n1 * d.n2);
+    // 32-bit index arithmetic (the entry points keep every array below 2^31)
+    const uint32_t i32 = uint32_t(i), m1 = uint32_t(d.n1), m2 = uint32_t(d.n2);
+    const int64_t z = i32 % m2, y = (i32 / m2) % m1, x = i32 / (m1 * m2);
     const int64_t iu = (x * d.n1 + y) * d.n2 + z;
     const int64_t iu1 = P ? ((x + 1 < d.n0 ? x + 1 : 0) * d.n1 + y) * d.n2 + z : iu + d.n1 * d.n2;
     const int64_t iv = (x * (d.n1 + e) + y) * d.n2 + z;
@@ -278,7 +280,9 @@ __device__ __forceinline__ void grad_correct_at(double* __restrict__ u, const do
                                                 const double* __restrict__ f, double a, int64_t i) {
     int64_t f0, f1, f2;
     face_extents<C, P>(d, f0, f1, f2);
-    const int64_t z = i % f2, y = (i / f2) % f1, x = i / (f1 * f2);
+    // 32-bit index arithmetic (the entry points keep every array below 2^31)
+    const uint32_t i32 = uint32_t(i), m1 = uint32_t(f1), m2 = uint32_t(f2);
+    const int64_t z = i32 % m2, y = (i32 / m2) % m1, x = i32 / (m1 * m2);
     const int64_t q = C == 0 ? x : (C == 1 ? y : z);
     const int64_t nq = C == 0 ? d.n0 : (C == 1 ? d.n1 : d.n2);
     const double star = face_value<F>(u, f, a, i);
@@ -383,7 +387,12 @@ __global__ void __launch_bounds__(256) ibb_advance_kernel(const IbbDesc* __restr
 
 // ---- host wrappers ---------------------------------------------------------
 
-static bool dims_ok(const int64_t n[3]) { return n && n[0] > 0 && n[1] > 0 && n[2] > 0; }
+// Positive extents whose face arrays (n + 1 per axis at most) stay below
+// 2^31 entries, so the elementwise passes index them with 32-bit arithmetic.
+static bool dims_ok(const int64_t n[3]) {
+    return n && n[0] > 0 && n[1] > 0 && n[2] > 0 && n[0] < (int64_t(1) << 31) && n[1] < (int64_t(1) << 31) &&
+           n[2] < (int64_t(1) << 31) && (n[0] + 1) * (n[1] + 1) * (n[2] + 1) < (int64_t(1) << 31);
+}
 
 template <class T>
 static Coef<T> make_coef(const double h2[3], const double* w) {
