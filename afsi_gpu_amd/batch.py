@@ -16,6 +16,7 @@ one launch per phase (one key space, one work list of (instance, tile,
 chunk)). ``BatchedIBStep`` advances a list of 3D instances by one IB step
 each with both.
 """
+from concurrent.futures import ThreadPoolExecutor
 import ctypes
 from ctypes import byref, c_double, c_int64, c_size_t
 from math import isfinite
@@ -383,3 +384,43 @@ class BatchedIBStep:
             _abi.call('afsi_ib3b_advance', self.ib, self.dt, _abi.stream())
         ps = [self.solver.x[i][cur] for i in range(len(self.grids))]
         return velocities, ps, self.point_velocity, infos
+
+
+class ConcurrentGroups:
+    """Runs independent groups of a batch (e.g. the 3D ``BatchedIBStep`` and
+    the 2D ``BatchedChannelStep`` of one mix) each on its own stream and host
+    thread, so that their launches, host convergence checks and kernels
+    overlap on the device.
+
+    ``run(*steps)`` calls each step (a callable) with its stream current.
+    Each step starts after the work already queued on the caller's current
+    stream, and that stream waits for all of them; the results come back in
+    order. A step must not capture graphs or synchronise the whole device
+    (the batched steps synchronise only their own stream), and the groups
+    must not share buffers. Streams keep the default priority."""
+
+    def __init__(self, device='cuda', count=2):
+        self.device = torch.device(device)
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(count)]
+        self.pool = ThreadPoolExecutor(max_workers=count)
+
+    def run(self, *steps):
+        if len(steps) > len(self.streams):
+            raise ValueError(f'at most {len(self.streams)} concurrent groups')
+        origin = torch.cuda.current_stream(self.device)
+
+        def call(step, stream):
+            torch.cuda.set_device(self.device)
+            stream.wait_stream(origin)
+            with torch.cuda.stream(stream):
+                return step()
+
+        futures = [self.pool.submit(call, step, stream) for step, stream in zip(steps, self.streams)]
+        results = [f.result() for f in futures]
+        for stream in self.streams[:len(steps)]:
+            origin.wait_stream(stream)
+        return results
+
+    def close(self):
+        self.pool.shutdown()
+

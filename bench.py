@@ -468,7 +468,7 @@ def batched_sample(torch, instances, steps=3, warmup=1):
     against the same instances stepped one at a time on the plain FP64
     paths."""
     from afsi_gpu_amd import workloads
-    from afsi_gpu_amd.batch import BatchedIBStep
+    from afsi_gpu_amd.batch import BatchedIBStep, ConcurrentGroups
     from afsi_gpu_amd.mac import MACGrid, MGOptions
     from afsi_gpu_amd.mac.multigrid import MIXED_INNER_CYCLES, MIXED_RELAX
     from afsi_gpu_amd.mac2d import ChannelGrid, ChannelIBStep
@@ -508,20 +508,25 @@ def batched_sample(torch, instances, steps=3, warmup=1):
     batch3 = BatchedIBStep(g3, [specs[i][1] for i in i3], dt=1e-3, device=dev, options=relaxed)
     batch2 = BatchedChannelStep(g2, [specs[i][1] for i in i2], dt=1e-3, device=dev, options=opts) if i2 else None
 
+    groups = ConcurrentGroups(dev)
+
     def batched(xs, fs, state):
         v3 = state.setdefault('v3', [g.zeros(device=dev) for g in g3])
         p3 = state.get('p3')
-        v3, p3, _, _ = batch3.step([xs[i] for i in i3], [fs[i] for i in i3], v3, p3)
+        step3 = lambda: batch3.step([xs[i] for i in i3], [fs[i] for i in i3], v3, p3)
+        if batch2 is None:
+            state['p3'] = step3()[1]
+            return
+        if 'b2' not in state:
+            # the 2D instances live in the batch's own buffers
+            batch2.reset()
+            for j, i in enumerate(i2):
+                batch2.x[j].copy_(xs[i])
+                batch2.force[j].copy_(fs[i])
+            state['b2'] = True
+        # the two groups step concurrently, each on its own stream and host thread
+        (v3, p3, _, _), _ = groups.run(step3, batch2.step)
         state['p3'] = p3
-        if batch2 is not None:
-            if 'b2' not in state:
-                # the 2D instances live in the batch's own buffers
-                batch2.reset()
-                for j, i in enumerate(i2):
-                    batch2.x[j].copy_(xs[i])
-                    batch2.force[j].copy_(fs[i])
-                state['b2'] = True
-            batch2.step()
 
     plain3 = [IBStep(g, specs[i][1], dt=1e-3, device=dev, options=opts, precision='fp64')
               for g, i in zip(g3, i3)]
@@ -536,6 +541,7 @@ def batched_sample(torch, instances, steps=3, warmup=1):
             state[('v2', j)], state[('p2', j)], _, _ = s2[j].step(xs[i], fs[i], v, state.get(('p2', j)))
 
     t_batch = run(batched)
+    groups.close()
     t_seq = run(sequential)
     return dict(instances=instances, instances_3d=len(i3), instances_2d=len(i2), points=points,
                 batched=dict(steps_per_s=1 / t_batch, instance_steps_per_s=instances / t_batch,
@@ -546,7 +552,8 @@ def batched_sample(torch, instances, steps=3, warmup=1):
                 note='one step = every instance advanced once; batched = 3D instances in mixed precision '
                      f'(FP32 V-cycles with red-black over-relaxation {MIXED_RELAX}, {MIXED_INNER_CYCLES} per FP64 '
                      'refinement step) and 2D channels in FP64, '
-                     'each group with one launch per phase over all its instances; sequential = the same '
+                     'each group with one launch per phase over all its instances, the two groups '
+                     'concurrently on their own streams and host threads; sequential = the same '
                      'instances stepped one at a time on the plain FP64 paths')
 
 

@@ -24,7 +24,7 @@ from oracle import ib3d, mac2d as omac2d, mac3d
 from oracle import step as ostep
 from oracle.mac3d import MGOptions as OOptions
 from afsi_gpu_amd import workloads
-from afsi_gpu_amd.batch import BatchedIBStep
+from afsi_gpu_amd.batch import BatchedIBStep, ConcurrentGroups
 from afsi_gpu_amd.mac import MACGrid, MGOptions
 from afsi_gpu_amd.mac2d import ChannelGrid, ChannelIBStep
 from afsi_gpu_amd.mac2d.batch import BatchedChannelStep
@@ -39,7 +39,10 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def test_batch_of_64_mixed_instances(gpu):
+@pytest.mark.parametrize('concurrent', [False, True])
+def test_batch_of_64_mixed_instances(gpu, concurrent):
+    """``concurrent``: the 3D and 2D groups step on their own streams and
+    host threads (``ConcurrentGroups``), to the same bars."""
     gen = workloads.rng(workloads.GEOMETRY_SEED + 4)
     specs = [workloads.instance_spec(gen, 'batch') for _ in range(64)]
     data = [workloads.instance_points(gen, shape, count) for shape, count in specs]
@@ -62,11 +65,18 @@ def test_batch_of_64_mixed_instances(gpu):
         batch2.force[j].copy_(torch.tensor(data[i][1]))
     got3, got2 = [], []
     p3 = None
+    groups = ConcurrentGroups(gpu) if concurrent else None
     for _ in range(2):
-        vel3, p3, U3, info3 = batch3.step(xs3, fs3, vel3, p3)
+        step3 = lambda: batch3.step(xs3, fs3, vel3, p3)
+        if groups is not None:
+            (vel3, p3, U3, info3), (U2, info2) = groups.run(step3, batch2.step)
+        else:
+            vel3, p3, U3, info3 = step3()
+            U2, info2 = batch2.step()
         got3.append(([U.cpu().numpy() for U in U3], [inf['cycles'] for inf in info3]))
-        U2, info2 = batch2.step()
         got2.append((U2.cpu().numpy(), list(info2['cycles'])))
+    if groups is not None:
+        groups.close()
 
     # single-instance paths
     worst3 = worst2 = 0.
