@@ -315,8 +315,9 @@ __device__ __forceinline__ void spread_points(const WorkItem& it, double (&px)[S
     // walks its point's 64 (i, j, k) links from its own rotation of all
     // three axes: at any instant the 64 lanes of one cell hit 64 different
     // LDS addresses (no same-address serialisation), and at most two share
-    // a bank pair. (Measured at 128^3 with 1.5M clustered points: 255 us
-    // against 276 us for contiguous per-thread slices with 16 rotations.)
+    // a bank pair. (Measured with an FP64 box at 128^3 with 1.5M clustered
+    // points: 255 us against 276 us for contiguous per-thread slices with 16
+    // rotations.)
     const int ri = (threadIdx.x >> 4) & 3, rj = (threadIdx.x >> 2) & 3, rk = threadIdx.x & 3;
     // one copy of the loop body: point q is always in slot 0 (the slots
     // shift down after each point; register arrays take static indices only)

@@ -117,7 +117,9 @@ int afsi_bin_points(const double* x, int64_t points, const afsi_grid3* grid,
  * Same mathematics as compact_spread/compact_gather for all three
  * components at once, weights computed from positions in registers.
  * Spread: one work item per (tile, chunk of points); per-tile LDS
- * accumulation of all three components with ds_add_f64, then one global
+ * accumulation of all three components in 64-bit fixed point (a power-of-two
+ * scale per work item from its largest force, integer LDS atomics; FP64 LDS
+ * atomics for items with nonfinite or extreme forces), then one FP64 global
  * atomic per touched face. u,v,w are overwritten (zero-filled first).
  * Interp: the same work items; each stages its tile's face box of all three
  * components in LDS and gathers its points' 192 values from there; out (P,3)
