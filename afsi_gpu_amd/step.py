@@ -135,6 +135,8 @@ class IBStep:
             _abi.call('afsi_axpy_f64', _abi.ptr(x), _abi.ptr(U), x.numel(), self.dt, _abi.stream())
 
     def _graph_step(self, x, force, velocity, pressure, advance):
+        if not (x.is_cuda and force.is_cuda and x.device == force.device):
+            raise ValueError('positions and forces must be on the step\'s GPU')
         if x.shape != (self.count, 3) or x.dtype != torch.float64 or not x.is_contiguous():
             raise ValueError('positions must match the step (N,3) contiguous float64')
         if force.shape != x.shape or force.dtype != torch.float64 or not force.is_contiguous():
