@@ -818,6 +818,19 @@ static int xfer_ready(const afsi_tb_batch* bt, XferLayout& L) {
 
 // Fill, work-list scan and spread launch of afsi_spread_tiled_f64;
 // chunk_start holds ntiles + 1 ints of scratch (unused without points).
+// Zero the three face arrays of one spread in one launch (grid-stride over
+// their concatenated ranges).
+__global__ void __launch_bounds__(256) zero3_kernel(double* __restrict__ a, int64_t na, double* __restrict__ b,
+                                                    int64_t nb, double* __restrict__ c, int64_t nc) {
+    const int64_t total = na + nb + nc;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        if (i < na) a[i] = 0.0;
+        else if (i < na + nb) b[i - na] = 0.0;
+        else c[i - na - nb] = 0.0;
+    }
+}
+
 static int spread_tiled(const double* x, const double* force, int64_t points, const afsi_grid3& g,
                         const int32_t tile[3], const int32_t* perm, const int32_t* cell_ptr, double* u,
                         double* v, double* w, int32_t* chunk_start, int* flags, hipStream_t s) {
@@ -830,9 +843,11 @@ static int spread_tiled(const double* x, const double* force, int64_t points, co
         {g.n[0] + !g.periodic, g.n[1], g.n[2]},
         {g.n[0], g.n[1] + !g.periodic, g.n[2]},
         {g.n[0], g.n[1], g.n[2] + !g.periodic}};
-    double* outs[3] = {u, v, w};
-    for (int c = 0; c < 3; ++c)
-        if (int r = launch_fill<double>(outs[c], nf[c][0] * nf[c][1] * nf[c][2], 0.0, s)) return r;
+    const int64_t counts[3] = {nf[0][0] * nf[0][1] * nf[0][2], nf[1][0] * nf[1][1] * nf[1][2],
+                               nf[2][0] * nf[2][1] * nf[2][2]};
+    zero3_kernel<<<grid_for(counts[0] + counts[1] + counts[2], 256, 16384), 256, 0, s>>>(
+        u, counts[0], v, counts[1], w, counts[2]);
+    AFSI_LAUNCH_CHECK();
     if (points == 0) return OK;
     AFSI_REQUIRE(x && force && perm && cell_ptr && chunk_start, "null pointer");
     chunk_scan_kernel<<<1, 1024, 0, s>>>(cell_ptr, ts.ntiles, ts.tvol, SPREAD_CHUNK, chunk_start);
