@@ -401,6 +401,8 @@ class ConcurrentGroups:
 
     def __init__(self, device='cuda', count=2):
         self.device = torch.device(device)
+        if self.device.index is None:  # the worker threads set an indexed device
+            self.device = torch.device('cuda', torch.cuda.current_device())
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(count)]
         self.pool = ThreadPoolExecutor(max_workers=count)
 
